@@ -1,0 +1,137 @@
+/* C ABI of the MI355X adapt() path (libomega_h_amd.so).
+ *
+ * Omega_h has no plugin ABI: callers link the library and call
+ * Omega_h::adapt(Mesh*, AdaptOpts const&) (src/Omega_h_adapt.hpp:82) and the
+ * pass entry points.  This header is the thin C layer those C++ entry
+ * points (include/Omega_h_amd.hpp) and foreign-language callers bind to.
+ * Every function returns 0 on success and a nonzero status otherwise, never
+ * aborts, and leaves a message for osh_last_error().  All work is ordered on
+ * the context's HIP stream; functions that hand data to the host return
+ * after it has arrived.
+ */
+#ifndef OMEGA_H_AMD_H
+#define OMEGA_H_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct osh_ctx osh_ctx;   /* one HIP stream on one device */
+typedef struct osh_mesh osh_mesh; /* a device-resident simplex mesh */
+
+/* Omega_h::AdaptOpts, src/Omega_h_adapt.hpp:50-77 (defaults
+ * src/Omega_h_adapt.cpp:52-85) */
+typedef struct {
+  double min_length_desired;
+  double max_length_desired;
+  double max_length_allowed;
+  double min_quality_allowed;
+  double min_quality_desired;
+  int nsliver_layers;
+  int verbosity;
+  int should_refine;
+  int should_coarsen;
+  int should_swap;
+  int should_coarsen_slivers;
+} osh_adapt_opts;
+
+#define OSH_MAX_PASSES 4096
+/* per-pass record of an adapt() run: kind 0 refine, 1 coarsen, 2 swap,
+ * 3 coarsen slivers */
+typedef struct {
+  int npasses;
+  int kind[OSH_MAX_PASSES];
+  int nelems_in[OSH_MAX_PASSES];
+  int nelems_out[OSH_MAX_PASSES];
+  double seconds[OSH_MAX_PASSES];
+} osh_adapt_stats;
+
+/* message of the last failed call on this thread */
+const char* osh_last_error(void);
+
+int osh_ctx_create(int device, osh_ctx** out);
+int osh_ctx_destroy(osh_ctx* ctx);
+int osh_ctx_synchronize(osh_ctx* ctx);
+
+/* AdaptOpts::AdaptOpts(Int dim), src/Omega_h_adapt.cpp:52-85 */
+int osh_default_opts(int dim, osh_adapt_opts* out);
+
+/* build_box(comm, SIMPLEX, x, y, z, nx, ny, nz), src/Omega_h_build.cpp:136-149:
+ * box, simplex split, Hilbert reordering and box classification; nz = 0
+ * builds a 2D box */
+int osh_build_box(osh_ctx* ctx, int dim, double x, double y, double z, int nx,
+    int ny, int nz, osh_mesh** out);
+/* build_box_internal (no reordering, no classification),
+ * src/Omega_h_build.cpp:107-134 */
+int osh_build_box_internal(osh_ctx* ctx, int dim, double x, double y, double z,
+    int nx, int ny, int nz, osh_mesh** out);
+/* build_from_elems2verts, src/Omega_h_build.cpp:84-97 (host vertex lists) */
+int osh_build_from_elems(osh_ctx* ctx, int dim, int nelems, const int* ev2v,
+    int nverts, osh_mesh** out);
+int osh_mesh_destroy(osh_mesh* mesh);
+int osh_mesh_copy(osh_mesh* mesh, osh_mesh** out);
+
+/* Mesh::dim / Mesh::nents, src/Omega_h_mesh.hpp */
+int osh_mesh_dim(osh_mesh* mesh, int* out);
+int osh_mesh_nents(osh_mesh* mesh, int ent_dim, int* out);
+/* Mesh::ask_verts_of(dim), host copy of nents*(dim+1) */
+int osh_mesh_get_verts(osh_mesh* mesh, int ent_dim, int* out);
+/* Mesh::ask_down(dim, dim-1): entities and alignment codes */
+int osh_mesh_get_down(osh_mesh* mesh, int ent_dim, int* out, signed char* codes);
+/* Mesh::ask_up(low, high): CSR offsets, entities, codes */
+int osh_mesh_up_size(osh_mesh* mesh, int low, int high, int* out);
+int osh_mesh_get_up(osh_mesh* mesh, int low, int high, int* a2ab, int* ab2b,
+    signed char* codes);
+/* Mesh::ask_star(dim) */
+int osh_mesh_star_size(osh_mesh* mesh, int ent_dim, int* out);
+int osh_mesh_get_star(osh_mesh* mesh, int ent_dim, int* a2ab, int* ab2b);
+/* tags "coordinates", "metric", "class_id"/"class_dim", "length", "quality" */
+int osh_mesh_get_coords(osh_mesh* mesh, double* out);
+int osh_mesh_set_metric(osh_mesh* mesh, int ncomps, const double* metric);
+int osh_mesh_metric_ncomps(osh_mesh* mesh, int* out);
+int osh_mesh_get_metric(osh_mesh* mesh, double* out);
+int osh_mesh_get_class(osh_mesh* mesh, int ent_dim, int* ids, signed char* dims);
+/* Mesh::ask_lengths / ask_qualities, src/Omega_h_mesh.cpp:374-388 */
+int osh_mesh_get_lengths(osh_mesh* mesh, double* out);
+int osh_mesh_get_qualities(osh_mesh* mesh, double* out);
+
+/* passes: refine_by_size src/Omega_h_refine.cpp:92, coarsen_by_size
+ * src/Omega_h_coarsen.cpp:198, swap_edges src/Omega_h_swap.cpp:45,
+ * coarsen_slivers src/Omega_h_coarsen.cpp:210; *did = 1 if the mesh changed */
+int osh_refine_by_size(osh_mesh* mesh, const osh_adapt_opts* opts, int* did);
+int osh_coarsen_by_size(osh_mesh* mesh, const osh_adapt_opts* opts, int* did);
+int osh_swap_edges(osh_mesh* mesh, const osh_adapt_opts* opts, int* did);
+int osh_coarsen_slivers(osh_mesh* mesh, const osh_adapt_opts* opts, int* did);
+/* adapt(Mesh*, AdaptOpts const&), src/Omega_h_adapt.cpp:272-288;
+ * *did = 0 when the mesh already met the goals; stats may be NULL */
+int osh_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats,
+    int* did);
+
+/* measure_edges_metric, src/Omega_h_shape.cpp:7-48, on device pointers:
+ * out[i] = metric length of edge edges[i] (or i when edges is NULL) */
+int osh_measure_edges_metric(osh_ctx* ctx, int dim, int metric_ncomps, int n,
+    const int* edges, const int* ev2v, const double* coords,
+    const double* metric, double* out);
+/* measure_qualities, src/Omega_h_quality.cpp:7-41, on device pointers */
+int osh_measure_qualities(osh_ctx* ctx, int dim, int metric_ncomps, int n,
+    const int* elems, const int* ev2v, const double* coords,
+    const double* metric, double* out);
+
+/* cavity kernels on host arrays, for tests and diagnostics:
+ * refine_qualities src/Omega_h_refine_qualities.cpp:88-108 */
+int osh_refine_qualities(osh_mesh* mesh, int n, const int* cands, double* out);
+/* swap3d::find_loop src/Omega_h_swap3d_loop.hpp:24-78 (loop_verts holds 7) */
+int osh_find_loop(osh_mesh* mesh, int edge, int* eev2v, int* loop_verts,
+    int* size);
+/* find_indset src/Omega_h_indset.cpp:27-35 */
+int osh_find_indset(osh_mesh* mesh, int ent_dim, const double* quality,
+    const signed char* cands, signed char* out);
+/* device evaluation of the scalar transcendentals: which = 0 log, 1 exp,
+ * 2 cbrt, 3 acos, 4 cos */
+int osh_eval_math(osh_ctx* ctx, int which, int n, const double* in, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

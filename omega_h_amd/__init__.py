@@ -1,0 +1,313 @@
+"""MI355X-native Omega_h adapt() path: Python binding of libomega_h_amd.so.
+
+The library is the product: HIP kernels for gfx950 behind the C ABI declared
+in include/omega_h_amd.h.  This module only binds that ABI; it raises when the
+built library is missing instead of computing anything itself.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libomega_h_amd.so")
+
+MAX_PASSES = 4096
+PASS_NAMES = ("refine", "coarsen", "swap", "coarsen_slivers")
+
+
+class AdaptOpts(C.Structure):
+    """Omega_h::AdaptOpts (src/Omega_h_adapt.hpp:50-77)."""
+    _fields_ = [
+        ("min_length_desired", C.c_double),
+        ("max_length_desired", C.c_double),
+        ("max_length_allowed", C.c_double),
+        ("min_quality_allowed", C.c_double),
+        ("min_quality_desired", C.c_double),
+        ("nsliver_layers", C.c_int),
+        ("verbosity", C.c_int),
+        ("should_refine", C.c_int),
+        ("should_coarsen", C.c_int),
+        ("should_swap", C.c_int),
+        ("should_coarsen_slivers", C.c_int),
+    ]
+
+
+class AdaptStats(C.Structure):
+    _fields_ = [
+        ("npasses", C.c_int),
+        ("kind", C.c_int * MAX_PASSES),
+        ("nelems_in", C.c_int * MAX_PASSES),
+        ("nelems_out", C.c_int * MAX_PASSES),
+        ("seconds", C.c_double * MAX_PASSES),
+    ]
+
+
+class OshError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def load(path=None):
+    """Load the HIP library (built by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise OshError(f"{path} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(path)
+    P = C.c_void_p
+    I = C.c_int
+    IP = C.POINTER(C.c_int)
+    DP = C.POINTER(C.c_double)
+    BP = C.POINTER(C.c_byte)
+    sig = {
+        "osh_ctx_create": [I, C.POINTER(P)],
+        "osh_ctx_destroy": [P],
+        "osh_ctx_synchronize": [P],
+        "osh_default_opts": [I, C.POINTER(AdaptOpts)],
+        "osh_build_box": [P, I, C.c_double, C.c_double, C.c_double, I, I, I, C.POINTER(P)],
+        "osh_build_box_internal": [P, I, C.c_double, C.c_double, C.c_double, I, I, I,
+                                   C.POINTER(P)],
+        "osh_build_from_elems": [P, I, I, IP, I, C.POINTER(P)],
+        "osh_mesh_destroy": [P],
+        "osh_mesh_copy": [P, C.POINTER(P)],
+        "osh_mesh_dim": [P, IP],
+        "osh_mesh_nents": [P, I, IP],
+        "osh_mesh_get_verts": [P, I, IP],
+        "osh_mesh_get_down": [P, I, IP, BP],
+        "osh_mesh_up_size": [P, I, I, IP],
+        "osh_mesh_get_up": [P, I, I, IP, IP, BP],
+        "osh_mesh_star_size": [P, I, IP],
+        "osh_mesh_get_star": [P, I, IP, IP],
+        "osh_mesh_get_coords": [P, DP],
+        "osh_mesh_set_metric": [P, I, DP],
+        "osh_mesh_metric_ncomps": [P, IP],
+        "osh_mesh_get_metric": [P, DP],
+        "osh_mesh_get_class": [P, I, IP, BP],
+        "osh_mesh_get_lengths": [P, DP],
+        "osh_mesh_get_qualities": [P, DP],
+        "osh_refine_by_size": [P, C.POINTER(AdaptOpts), IP],
+        "osh_coarsen_by_size": [P, C.POINTER(AdaptOpts), IP],
+        "osh_swap_edges": [P, C.POINTER(AdaptOpts), IP],
+        "osh_coarsen_slivers": [P, C.POINTER(AdaptOpts), IP],
+        "osh_adapt": [P, C.POINTER(AdaptOpts), C.POINTER(AdaptStats), IP],
+        "osh_measure_edges_metric": [P, I, I, I, P, P, P, P, P],
+        "osh_measure_qualities": [P, I, I, I, P, P, P, P, P],
+        "osh_refine_qualities": [P, I, IP, DP],
+        "osh_find_loop": [P, I, IP, IP, IP],
+        "osh_find_indset": [P, I, DP, BP, BP],
+        "osh_eval_math": [P, I, I, DP, DP],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.osh_last_error.restype = C.c_char_p
+    L.osh_last_error.argtypes = []
+    if path == LIB_PATH or _lib is None:
+        _lib = L
+    return L
+
+
+def check(status, lib=None):
+    if status != 0:
+        lib = lib or load()
+        raise OshError(lib.osh_last_error().decode())
+
+
+class Context:
+    """One HIP stream on one device."""
+
+    def __init__(self, device=0, lib=None):
+        self.lib = lib or load()
+        h = C.c_void_p()
+        check(self.lib.osh_ctx_create(device, C.byref(h)), self.lib)
+        self.h = h
+
+    def synchronize(self):
+        check(self.lib.osh_ctx_synchronize(self.h), self.lib)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.osh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def default_opts(self, dim):
+        o = AdaptOpts()
+        check(self.lib.osh_default_opts(dim, C.byref(o)), self.lib)
+        return o
+
+    def eval_math(self, which, xs):
+        x = np.ascontiguousarray(xs, dtype=np.float64)
+        out = np.zeros_like(x)
+        check(self.lib.osh_eval_math(self.h, which, len(x), _p(x, C.c_double),
+                                     _p(out, C.c_double)), self.lib)
+        return out
+
+
+class Mesh:
+    """A device-resident simplex mesh (the Omega_h::Mesh of the adapt path)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.h = handle
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.osh_mesh_destroy(self.h)
+            self.h = None
+
+    @classmethod
+    def box(cls, ctx, dim, n, internal=False, x=1.0, y=1.0, z=1.0):
+        h = C.c_void_p()
+        f = ctx.lib.osh_build_box_internal if internal else ctx.lib.osh_build_box
+        nz = n if dim == 3 else 0
+        check(f(ctx.h, dim, x, y, z if dim == 3 else 0.0, n, n, nz, C.byref(h)), ctx.lib)
+        return cls(ctx, h)
+
+    @classmethod
+    def from_elems(cls, ctx, dim, ev2v, nverts):
+        a = np.ascontiguousarray(ev2v, dtype=np.int32)
+        h = C.c_void_p()
+        check(ctx.lib.osh_build_from_elems(ctx.h, dim, len(a) // (dim + 1), _p(a, C.c_int),
+                                           nverts, C.byref(h)), ctx.lib)
+        return cls(ctx, h)
+
+    def copy(self):
+        h = C.c_void_p()
+        check(self.lib.osh_mesh_copy(self.h, C.byref(h)), self.lib)
+        return Mesh(self.ctx, h)
+
+    def _int(self, f, *args):
+        out = C.c_int()
+        check(f(self.h, *args, C.byref(out)), self.lib)
+        return out.value
+
+    @property
+    def dim(self):
+        return self._int(self.lib.osh_mesh_dim)
+
+    def nents(self, d):
+        return self._int(self.lib.osh_mesh_nents, d)
+
+    def verts(self, d):
+        out = np.zeros(self.nents(d) * (d + 1), dtype=np.int32)
+        check(self.lib.osh_mesh_get_verts(self.h, d, _p(out, C.c_int)), self.lib)
+        return out
+
+    def down(self, d):
+        n = self.nents(d) * (d + 1)
+        out = np.zeros(n, dtype=np.int32)
+        codes = np.zeros(n, dtype=np.int8)
+        check(self.lib.osh_mesh_get_down(self.h, d, _p(out, C.c_int), _p(codes, C.c_byte)),
+              self.lib)
+        return out, codes
+
+    def up(self, low, high):
+        n = self._int(self.lib.osh_mesh_up_size, low, high)
+        a2ab = np.zeros(self.nents(low) + 1, dtype=np.int32)
+        ab2b = np.zeros(n, dtype=np.int32)
+        codes = np.zeros(n, dtype=np.int8)
+        check(self.lib.osh_mesh_get_up(self.h, low, high, _p(a2ab, C.c_int), _p(ab2b, C.c_int),
+                                       _p(codes, C.c_byte)), self.lib)
+        return a2ab, ab2b, codes
+
+    def star(self, d):
+        n = self._int(self.lib.osh_mesh_star_size, d)
+        a2ab = np.zeros(self.nents(d) + 1, dtype=np.int32)
+        ab2b = np.zeros(n, dtype=np.int32)
+        check(self.lib.osh_mesh_get_star(self.h, d, _p(a2ab, C.c_int), _p(ab2b, C.c_int)),
+              self.lib)
+        return a2ab, ab2b
+
+    def coords(self):
+        out = np.zeros(self.nents(0) * self.dim, dtype=np.float64)
+        check(self.lib.osh_mesh_get_coords(self.h, _p(out, C.c_double)), self.lib)
+        return out
+
+    def set_metric(self, metric, ncomps):
+        a = np.ascontiguousarray(metric, dtype=np.float64)
+        check(self.lib.osh_mesh_set_metric(self.h, ncomps, _p(a, C.c_double)), self.lib)
+
+    def metric(self):
+        nc = self._int(self.lib.osh_mesh_metric_ncomps)
+        out = np.zeros(self.nents(0) * nc, dtype=np.float64)
+        check(self.lib.osh_mesh_get_metric(self.h, _p(out, C.c_double)), self.lib)
+        return out
+
+    def classification(self, d):
+        ids = np.zeros(self.nents(d), dtype=np.int32)
+        dims = np.zeros(self.nents(d), dtype=np.int8)
+        check(self.lib.osh_mesh_get_class(self.h, d, _p(ids, C.c_int), _p(dims, C.c_byte)),
+              self.lib)
+        return ids, dims
+
+    def lengths(self):
+        out = np.zeros(self.nents(1), dtype=np.float64)
+        check(self.lib.osh_mesh_get_lengths(self.h, _p(out, C.c_double)), self.lib)
+        return out
+
+    def qualities(self):
+        out = np.zeros(self.nents(self.dim), dtype=np.float64)
+        check(self.lib.osh_mesh_get_qualities(self.h, _p(out, C.c_double)), self.lib)
+        return out
+
+    def refine_qualities(self, cands):
+        c = np.ascontiguousarray(cands, dtype=np.int32)
+        out = np.zeros(len(c), dtype=np.float64)
+        check(self.lib.osh_refine_qualities(self.h, len(c), _p(c, C.c_int),
+                                            _p(out, C.c_double)), self.lib)
+        return out
+
+    def find_loop(self, edge):
+        eev = np.zeros(2, dtype=np.int32)
+        lv = np.zeros(7, dtype=np.int32)
+        n = C.c_int()
+        check(self.lib.osh_find_loop(self.h, edge, _p(eev, C.c_int), _p(lv, C.c_int),
+                                     C.byref(n)), self.lib)
+        return eev, lv[:n.value]
+
+    def indset(self, d, quality, cands):
+        q = np.ascontiguousarray(quality, dtype=np.float64)
+        c = np.ascontiguousarray(cands, dtype=np.int8)
+        out = np.zeros(self.nents(d), dtype=np.int8)
+        check(self.lib.osh_find_indset(self.h, d, _p(q, C.c_double), _p(c, C.c_byte),
+                                       _p(out, C.c_byte)), self.lib)
+        return out
+
+    def _pass(self, f, opts):
+        did = C.c_int()
+        check(f(self.h, C.byref(opts), C.byref(did)), self.lib)
+        return did.value
+
+    def refine_by_size(self, opts):
+        return self._pass(self.lib.osh_refine_by_size, opts)
+
+    def coarsen_by_size(self, opts):
+        return self._pass(self.lib.osh_coarsen_by_size, opts)
+
+    def swap_edges(self, opts):
+        return self._pass(self.lib.osh_swap_edges, opts)
+
+    def coarsen_slivers(self, opts):
+        return self._pass(self.lib.osh_coarsen_slivers, opts)
+
+    def adapt(self, opts):
+        st = AdaptStats()
+        did = C.c_int()
+        check(self.lib.osh_adapt(self.h, C.byref(opts), C.byref(st), C.byref(did)), self.lib)
+        passes = [(st.kind[i], st.nelems_in[i], st.nelems_out[i], st.seconds[i])
+                  for i in range(st.npasses)]
+        return did.value, passes

@@ -1,0 +1,461 @@
+// Adjacency derivation on the device.
+//
+//   transit                 src/Omega_h_adj.cpp:443-510
+//   invert_adj              src/Omega_h_adj.cpp:178-263 (count, scan, fill,
+//                           then each row ordered by high index so the result
+//                           does not depend on atomic order)
+//   reflect_down / find_matches src/Omega_h_adj.cpp:297-441
+//   verts_across_edges, edges_across_tris/tets, add_edges
+//                           src/Omega_h_adj.cpp:512-589, src/Omega_h_graph.cpp:14-40
+//   elements_across_sides   src/Omega_h_adj.cpp:591-657
+//   find_unique             src/Omega_h_adj.cpp:55-153
+#include "osh_mesh.hpp"
+
+namespace osh {
+
+int simplex_degree(int from_dim, int to_dim) {
+  static int const deg[4][4] = {{1, -1, -1, -1}, {2, 1, -1, -1}, {3, 3, 1, -1}, {4, 6, 4, 1}};
+  return deg[from_dim][to_dim];
+}
+
+void DMesh::clear_caches() {
+  for (int d = 0; d < 4; ++d) {
+    verts[d] = LOs();
+    for (int e = 0; e < 4; ++e) up[d][e] = Adj();
+  }
+  star[0] = star[1] = Adj();
+  dual = Adj();
+  t2e = LOs();
+  t2e_codes = Bytes();
+}
+
+namespace {
+
+__device__ __forceinline__ int rotate_index(int n, int index, int rotation) {
+  return (index + rotation) % n;
+}
+__device__ __forceinline__ int invert_rotation(int n, int rotation) {
+  return (n - rotation) % n;
+}
+__device__ __forceinline__ I8 invert_alignment(int n, I8 code) {
+  if (code_is_flipped(code)) return code;
+  return make_code(0, invert_rotation(n, code_rotation(code)), 0);
+}
+__device__ __forceinline__ int align_vert_index(int n, int index, I8 code) {
+  int r = rotate_index(n, index, code_rotation(code));
+  return code_is_flipped(code) ? (r == 0 ? 0 : (1 + ((n - 1) - 1 - (r - 1)))) : r;
+}
+__device__ __forceinline__ int align_edge_index(int n, int index, I8 code) {
+  int r = rotate_index(n, index, code_rotation(code));
+  return code_is_flipped(code) ? (n - 1 - r) : r;
+}
+
+// first entry of simplex_up_template, src/Omega_h_simplex.hpp:91-229
+__device__ __forceinline__ void up_template0(int elem_dim, int bdry_dim, int which,
+    int& up, int& which_down, int& flipped) {
+  if (elem_dim == 3 && bdry_dim == 0) {
+    up = (which == 3) ? 5 : which;
+    which_down = (which == 3) ? 1 : 0;
+    flipped = 0;
+  } else if (elem_dim == 3 && bdry_dim == 1) {
+    int const t[6][2] = {{0, 2}, {0, 1}, {0, 0}, {1, 2}, {2, 2}, {3, 2}};
+    up = t[which][0];
+    which_down = t[which][1];
+    flipped = 1;
+  } else {
+    up = which;
+    which_down = 0;
+    flipped = 0;
+  }
+}
+
+template <int HIGH, int LOW>
+__global__ void transit_kernel(LO nhigh, LO const* hm2m, I8 const* m2hm_codes,
+    LO const* ml2l, I8 const* ml_codes, LO* hl2l, I8* codes) {
+  LO h = thread_index();
+  if (h >= nhigh) return;
+  constexpr int MID = LOW + 1;
+  constexpr int nmph = (HIGH == 3) ? (MID == 2 ? 4 : 6) : 3;
+  constexpr int nlpm = MID + 1;
+  constexpr int nlph = (HIGH == 3) ? (LOW == 0 ? 4 : 6) : 3;
+  for (int hl = 0; hl < nlph; ++hl) {
+    int hm, hml, canon_flipped;
+    up_template0(HIGH, LOW, hl, hm, hml, canon_flipped);
+    LO m = hm2m[h * nmph + hm];
+    I8 hm2m_code = invert_alignment(nlpm, m2hm_codes[h * nmph + hm]);
+    int ml = (LOW == 0) ? align_vert_index(nlpm, hml, hm2m_code)
+                        : align_edge_index(nlpm, hml, hm2m_code);
+    hl2l[h * nlph + hl] = ml2l[m * nlpm + ml];
+    if (LOW == 1) {
+      int region_face_flipped = code_is_flipped(hm2m_code);
+      int face_edge_flipped = (code_rotation(ml_codes[m * nlpm + ml]) == 1);
+      codes[h * nlph + hl] =
+          make_code(0, region_face_flipped ^ face_edge_flipped ^ canon_flipped, 0);
+    }
+  }
+}
+
+__global__ void count_lows(LO nhl, LO const* hl2l, LO* counts) {
+  LO i = thread_index();
+  if (i < nhl) atomicAdd(&counts[hl2l[i]], 1);
+}
+
+__global__ void fill_uses(LO nhl, LO const* hl2l, LO const* a2ab, LO* pos, LO* ab2hl) {
+  LO hl = thread_index();
+  if (hl >= nhl) return;
+  LO l = hl2l[hl];
+  LO k = atomicAdd(&pos[l], 1);
+  ab2hl[a2ab[l] + k] = hl;
+}
+
+// order each row by high index and form the codes of separate_upward_*
+__global__ void finish_rows(LO nlow, LO const* a2ab, LO* ab2b, I8* codes, int deg,
+    I8 const* down_codes) {
+  LO l = thread_index();
+  if (l >= nlow) return;
+  LO b = a2ab[l], e = a2ab[l + 1];
+  for (LO j = b + 1; j < e; ++j) {
+    LO x = ab2b[j];
+    LO k = j - 1;
+    while (k >= b && ab2b[k] > x) {
+      ab2b[k + 1] = ab2b[k];
+      --k;
+    }
+    ab2b[k + 1] = x;
+  }
+  for (LO j = b; j < e; ++j) {
+    LO hl = ab2b[j];
+    int which_down = hl % deg;
+    I8 c = down_codes ? make_code(code_is_flipped(down_codes[hl]),
+                            code_rotation(down_codes[hl]), which_down)
+                      : make_code(0, 0, which_down);
+    codes[j] = c;
+    ab2b[j] = hl / deg;
+  }
+}
+
+Adj invert_adj(Ctx& c, LO nhigh, int deg, LO nlow, LOs const& hl2l, Bytes const* down_codes) {
+  LO nhl = nhigh * deg;
+  LOs counts = fill_lo(c, nlow, 0);
+  OSH_LAUNCH(count_lows, nhl, c, hl2l.data(), counts.data());
+  Adj a;
+  a.a2ab = offset_scan(c, counts);
+  a.ab2b = LOs(c, nhl);
+  a.codes = Bytes(c, nhl);
+  LOs pos = fill_lo(c, nlow, 0);
+  OSH_LAUNCH(fill_uses, nhl, c, hl2l.data(), a.a2ab.data(), pos.data(), a.ab2b.data());
+  OSH_LAUNCH(finish_rows, nlow, c, a.a2ab.data(), a.ab2b.data(), a.codes.data(), deg,
+      down_codes ? down_codes->data() : (I8 const*)nullptr);
+  return a;
+}
+
+// reflect_down: form the uses of each product and find the matching entity
+template <int HIGH>
+__global__ void match_uses(LO nuses, LO const* prod_verts, LO const* lv2v,
+    LO const* v2l_a2ab, LO const* v2l_ab2b, I8 const* v2l_codes, LO* out, I8* out_codes,
+    int* failures) {
+  LO u = thread_index();
+  if (u >= nuses) return;
+  constexpr int LOW = HIGH - 1;
+  constexpr int nvh = HIGH + 1;
+  constexpr int nlh = HIGH + 1;
+  constexpr int nvl = LOW + 1;
+  LO h = u / nlh;
+  int which = u % nlh;
+  LO av[3];
+  for (int k = 0; k < nvl; ++k) av[k] = prod_verts[h * nvh + down_template(HIGH, LOW, which, k)];
+  LO fv = av[0];
+  int found = 0;
+  for (LO vb = v2l_a2ab[fv]; vb < v2l_a2ab[fv + 1]; ++vb) {
+    LO b = v2l_ab2b[vb];
+    int wd = code_which_down(v2l_codes[vb]);
+    LO const* bv = lv2v + b * nvl;
+    if (nvl == 2) {
+      if (av[1] == bv[1 - wd]) {
+        out[u] = b;
+        out_codes[u] = make_code(0, wd, 0);
+        ++found;
+      }
+    } else {
+      if (av[1] == bv[(wd + 1) % 3] && av[2] == bv[(wd + 2) % 3]) {
+        out[u] = b;
+        out_codes[u] = make_code(0, (3 - wd) % 3, 0);
+        ++found;
+      } else if (av[1] == bv[(wd + 2) % 3] && av[2] == bv[(wd + 1) % 3]) {
+        out[u] = b;
+        out_codes[u] = make_code(1, (3 - wd) % 3, 0);
+        ++found;
+      }
+    }
+  }
+  if (found != 1) atomicAdd(failures, 1);
+}
+
+__global__ void verts_across_edges_kernel(LO nve, LO const* ve2e, I8 const* codes,
+    LO const* ev2v, LO* vv2v) {
+  LO ve = thread_index();
+  if (ve >= nve) return;
+  LO e = ve2e[ve];
+  int eev = code_which_down(codes[ve]);
+  vv2v[ve] = ev2v[e * 2 + (1 - eev)];
+}
+
+__global__ void edge_star_degrees(LO ne, LO const* e2f_a2ab, LO const* e2t_a2ab, LO* deg) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  LO d = 2 * (e2f_a2ab[e + 1] - e2f_a2ab[e]);
+  if (e2t_a2ab) d += e2t_a2ab[e + 1] - e2t_a2ab[e];
+  deg[e] = d;
+}
+
+__global__ void edge_star_fill(LO ne, LO const* e2f_a2ab, LO const* e2f_ab2b,
+    I8 const* e2f_codes, LO const* fe2e, LO const* e2t_a2ab, LO const* e2t_ab2b,
+    I8 const* e2t_codes, LO const* te2e, LO const* a2ab, LO* ab2b) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  LO k = a2ab[e];
+  for (LO ef = e2f_a2ab[e]; ef < e2f_a2ab[e + 1]; ++ef) {
+    LO f = e2f_ab2b[ef];
+    int ffe = code_which_down(e2f_codes[ef]);
+    ab2b[k++] = fe2e[f * 3 + ((ffe + 1) % 3)];
+    ab2b[k++] = fe2e[f * 3 + ((ffe + 2) % 3)];
+  }
+  if (e2t_a2ab) {
+    for (LO er = e2t_a2ab[e]; er < e2t_a2ab[e + 1]; ++er) {
+      LO r = e2t_ab2b[er];
+      int rre = code_which_down(e2t_codes[er]);
+      ab2b[k++] = te2e[r * 6 + opposite_template(3, 1, rre)];
+    }
+  }
+}
+
+__global__ void dual_degrees(LO ne, int nspe, LO const* es2s, LO const* s2e_a2ab, LO* deg) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  LO d = 0;
+  for (int k = 0; k < nspe; ++k) {
+    LO s = es2s[e * nspe + k];
+    if (s2e_a2ab[s + 1] - s2e_a2ab[s] >= 2) ++d;
+  }
+  deg[e] = d;
+}
+
+__global__ void dual_fill(LO ne, int nspe, LO const* es2s, LO const* s2e_a2ab,
+    LO const* s2e_ab2b, LO const* a2ab, LO* ab2b) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  LO k = a2ab[e];
+  for (int j = 0; j < nspe; ++j) {
+    LO s = es2s[e * nspe + j];
+    if (s2e_a2ab[s + 1] - s2e_a2ab[s] >= 2) {
+      LO se = s2e_a2ab[s];
+      if (s2e_ab2b[se] == e) ++se;
+      ab2b[k++] = s2e_ab2b[se];
+    }
+  }
+}
+
+}  // namespace
+
+LOs const& verts_of(Ctx& c, DMesh& m, int d) {
+  if (d == 1) return m.down[1];
+  if (!m.verts[d].exists()) {
+    LO n = m.nents[d];
+    LOs out(c, n * (d + 1));
+    if (d == 2) {
+      OSH_LAUNCH((transit_kernel<2, 0>), n, c, m.down[2].data(), m.down_codes[2].data(),
+          m.down[1].data(), (I8 const*)nullptr, out.data(), (I8*)nullptr);
+    } else {
+      Bytes const* codes;
+      LOs const& t2e = ask_down(c, m, 3, 1, &codes);
+      OSH_LAUNCH((transit_kernel<3, 0>), n, c, t2e.data(), codes->data(), m.down[1].data(),
+          (I8 const*)nullptr, out.data(), (I8*)nullptr);
+    }
+    m.verts[d] = out;
+  }
+  return m.verts[d];
+}
+
+LOs const& ask_down(Ctx& c, DMesh& m, int high, int low, Bytes const** codes) {
+  if (low == 0) {
+    if (codes) *codes = nullptr;
+    return verts_of(c, m, high);
+  }
+  if (low == high - 1) {
+    if (codes) *codes = &m.down_codes[high];
+    return m.down[high];
+  }
+  if (!m.t2e.exists()) {
+    LO n = m.nents[3];
+    m.t2e = LOs(c, n * 6);
+    m.t2e_codes = Bytes(c, n * 6);
+    OSH_LAUNCH((transit_kernel<3, 1>), n, c, m.down[3].data(), m.down_codes[3].data(),
+        m.down[2].data(), m.down_codes[2].data(), m.t2e.data(), m.t2e_codes.data());
+  }
+  if (codes) *codes = &m.t2e_codes;
+  return m.t2e;
+}
+
+Adj const& ask_up(Ctx& c, DMesh& m, int low, int high) {
+  if (!m.up[low][high].exists()) {
+    Bytes const* codes = nullptr;
+    LOs const& down = ask_down(c, m, high, low, &codes);
+    m.up[low][high] = invert_adj(c, m.nents[high], simplex_degree(high, low), m.nents[low],
+        down, codes);
+  }
+  return m.up[low][high];
+}
+
+void reflect_down(Ctx& c, DMesh& m, int high_dim, LO nprods, LOs const& prod_verts,
+    LOs& down, Bytes& codes) {
+  int low_dim = high_dim - 1;
+  LO nuses = nprods * (high_dim + 1);
+  down = LOs(c, nuses);
+  codes = Bytes(c, nuses);
+  if (nuses == 0) return;
+  LOs const& lv2v = verts_of(c, m, low_dim);
+  Adj const& v2l = ask_up(c, m, 0, low_dim);
+  LOs failures = fill_lo(c, 1, 0);
+  if (high_dim == 2) {
+    OSH_LAUNCH(match_uses<2>, nuses, c, prod_verts.data(), lv2v.data(), v2l.a2ab.data(),
+        v2l.ab2b.data(), v2l.codes.data(), down.data(), codes.data(), failures.data());
+  } else {
+    OSH_LAUNCH(match_uses<3>, nuses, c, prod_verts.data(), lv2v.data(), v2l.a2ab.data(),
+        v2l.ab2b.data(), v2l.codes.data(), down.data(), codes.data(), failures.data());
+  }
+  int nfail = get_value(c, failures.data());
+  OSH_CHECK(nfail == 0, "reflect_down: entity without a unique match");
+}
+
+Adj const& ask_star(Ctx& c, DMesh& m, int d) {
+  if (!m.star[d].exists()) {
+    Adj g;
+    if (d == 0) {
+      Adj const& v2e = ask_up(c, m, 0, 1);
+      g.a2ab = v2e.a2ab;
+      g.ab2b = LOs(c, v2e.ab2b.size());
+      OSH_LAUNCH(verts_across_edges_kernel, v2e.ab2b.size(), c, v2e.ab2b.data(),
+          v2e.codes.data(), m.down[1].data(), g.ab2b.data());
+    } else {
+      LO ne = m.nents[1];
+      Adj const& e2f = ask_up(c, m, 1, 2);
+      Adj const* e2t = nullptr;
+      LO const* te2e = nullptr;
+      if (m.dim == 3) {
+        e2t = &ask_up(c, m, 1, 3);
+        te2e = ask_down(c, m, 3, 1, nullptr).data();
+      }
+      LOs deg(c, ne);
+      OSH_LAUNCH(edge_star_degrees, ne, c, e2f.a2ab.data(),
+          e2t ? e2t->a2ab.data() : (LO const*)nullptr, deg.data());
+      g.a2ab = offset_scan(c, deg);
+      g.ab2b = LOs(c, last(c, g.a2ab));
+      OSH_LAUNCH(edge_star_fill, ne, c, e2f.a2ab.data(), e2f.ab2b.data(), e2f.codes.data(),
+          m.down[2].data(), e2t ? e2t->a2ab.data() : (LO const*)nullptr,
+          e2t ? e2t->ab2b.data() : (LO const*)nullptr,
+          e2t ? e2t->codes.data() : (I8 const*)nullptr, te2e, g.a2ab.data(), g.ab2b.data());
+    }
+    m.star[d] = g;
+  }
+  return m.star[d];
+}
+
+Adj const& ask_dual(Ctx& c, DMesh& m) {
+  if (!m.dual.exists()) {
+    int dim = m.dim;
+    LO ne = m.nents[dim];
+    Adj const& s2e = ask_up(c, m, dim - 1, dim);
+    LOs deg(c, ne);
+    OSH_LAUNCH(dual_degrees, ne, c, dim + 1, m.down[dim].data(), s2e.a2ab.data(), deg.data());
+    Adj g;
+    g.a2ab = offset_scan(c, deg);
+    g.ab2b = LOs(c, last(c, g.a2ab));
+    OSH_LAUNCH(dual_fill, ne, c, dim + 1, m.down[dim].data(), s2e.a2ab.data(),
+        s2e.ab2b.data(), g.a2ab.data(), g.ab2b.data());
+    m.dual = g;
+  }
+  return m.dual;
+}
+
+// ---------------------------------------------------------------- find_unique
+namespace {
+
+template <int HIGH, int LOW>
+__global__ void form_canonical_uses(LO nuses, LO const* hv2v, LO* uv2v, I64* canon) {
+  LO u = thread_index();
+  if (u >= nuses) return;
+  constexpr int nvh = HIGH + 1;
+  constexpr int nlh = (HIGH == 3) ? (LOW == 1 ? 6 : 4) : 3;
+  constexpr int deg = LOW + 1;
+  LO h = u / nlh;
+  int which = u % nlh;
+  LO ev[3];
+  for (int k = 0; k < deg; ++k) {
+    ev[k] = hv2v[h * nvh + down_template(HIGH, LOW, which, k)];
+    uv2v[u * deg + k] = ev[k];
+  }
+  int min_j = 0;
+  LO min_v = ev[0];
+  for (int j = 1; j < deg; ++j)
+    if (ev[j] < min_v) {
+      min_j = j;
+      min_v = ev[j];
+    }
+  int rotation = (deg - min_j) % deg;
+  LO tmp[3];
+  for (int j = 0; j < deg; ++j) tmp[(j + rotation) % deg] = ev[j];
+  if (deg == 3 && tmp[2] < tmp[1]) {
+    LO t = tmp[1];
+    tmp[1] = tmp[2];
+    tmp[2] = t;
+  }
+  for (int j = 0; j < deg; ++j) canon[I64(u) * deg + j] = tmp[j];
+}
+
+__global__ void mark_group_ends(LO n, int deg, I64 const* canon, LO const* sorted2u, I8* last) {
+  LO s = thread_index();
+  if (s >= n) return;
+  I8 r = 1;
+  if (s < n - 1) {
+    LO a = sorted2u[s], b = sorted2u[s + 1];
+    r = 0;
+    for (int j = 0; j < deg; ++j)
+      if (canon[I64(a) * deg + j] != canon[I64(b) * deg + j]) r = 1;
+  }
+  last[s] = r;
+}
+
+__global__ void gather_unique(LO nlow, int deg, LO const* ends, LO const* sorted2u,
+    LO const* uv2v, LO* out) {
+  LO i = thread_index();
+  if (i >= nlow) return;
+  LO u = sorted2u[ends[i]];
+  for (int j = 0; j < deg; ++j) out[i * deg + j] = uv2v[u * deg + j];
+}
+
+}  // namespace
+
+LOs find_unique(Ctx& c, LO nhigh, int high_dim, int low_dim, LOs const& hv2v) {
+  int nlh = simplex_degree(high_dim, low_dim);
+  int deg = low_dim + 1;
+  LO nuses = nhigh * nlh;
+  LOs uv2v(c, nuses * deg);
+  DArray<I64> canon(c, nuses * deg);
+  if (high_dim == 3 && low_dim == 1)
+    OSH_LAUNCH((form_canonical_uses<3, 1>), nuses, c, hv2v.data(), uv2v.data(), canon.data());
+  else if (high_dim == 3 && low_dim == 2)
+    OSH_LAUNCH((form_canonical_uses<3, 2>), nuses, c, hv2v.data(), uv2v.data(), canon.data());
+  else
+    OSH_LAUNCH((form_canonical_uses<2, 1>), nuses, c, hv2v.data(), uv2v.data(), canon.data());
+  LOs sorted2u = sort_by_keys(c, canon, deg);
+  Bytes ends(c, nuses);
+  OSH_LAUNCH(mark_group_ends, nuses, c, deg, canon.data(), sorted2u.data(), ends.data());
+  LOs e2sorted = collect_marked(c, ends);
+  LO nlow = e2sorted.size();
+  LOs out(c, nlow * deg);
+  OSH_LAUNCH(gather_unique, nlow, c, deg, e2sorted.data(), sorted2u.data(), uv2v.data(),
+      out.data());
+  return out;
+}
+
+}  // namespace osh

@@ -1,0 +1,457 @@
+// The C ABI (include/omega_h_amd.h): status codes, thread-local error
+// messages, handles over the device runtime.
+#include <memory>
+#include <string>
+
+#include "../../include/omega_h_amd.h"
+#include "osh_math.hpp"
+#include "osh_passes.hpp"
+
+struct osh_ctx {
+  osh::Ctx ctx;
+};
+
+struct osh_mesh {
+  osh_ctx* owner;
+  osh::DMesh m;
+};
+
+namespace {
+
+thread_local std::string last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (std::exception const& e) {
+    last_error = e.what();
+  } catch (...) {
+    last_error = "unknown error";
+  }
+  return 1;
+}
+
+void to_options(osh_adapt_opts const* o, osh::AdaptOptions& r) {
+  r.min_length_desired = o->min_length_desired;
+  r.max_length_desired = o->max_length_desired;
+  r.max_length_allowed = o->max_length_allowed;
+  r.min_quality_allowed = o->min_quality_allowed;
+  r.min_quality_desired = o->min_quality_desired;
+  r.nsliver_layers = o->nsliver_layers;
+  r.verbosity = o->verbosity;
+  r.should_refine = o->should_refine;
+  r.should_coarsen = o->should_coarsen;
+  r.should_swap = o->should_swap;
+  r.should_coarsen_slivers = o->should_coarsen_slivers;
+}
+
+template <class T>
+void download(osh::Ctx& c, osh::DArray<T> const& d, T* out) {
+  if (d.size())
+    OSH_HIP(hipMemcpyAsync(out, d.data(), sizeof(T) * size_t(d.size()), hipMemcpyDeviceToHost,
+        c.stream));
+  c.sync();
+}
+
+void check_mesh(osh_mesh* m) {
+  OSH_CHECK(m != nullptr, "null mesh handle");
+}
+
+void check_dim(osh_mesh* m, int d) {
+  check_mesh(m);
+  OSH_CHECK(d >= 0 && d <= m->m.dim, "entity dimension out of range");
+}
+
+__global__ void eval_math_kernel(osh::LO n, int which, double const* in, double* out) {
+  osh::LO i = osh::thread_index();
+  if (i >= n) return;
+  double x = in[i];
+  double r;
+  switch (which) {
+    case 0: r = osh::math::log_(x); break;
+    case 1: r = osh::math::exp_(x); break;
+    case 2: r = osh::math::cbrt_(x); break;
+    case 3: r = osh::math::acos_(x); break;
+    default: r = osh::math::cos_(x); break;
+  }
+  out[i] = r;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* osh_last_error(void) { return last_error.c_str(); }
+
+int osh_ctx_create(int device, osh_ctx** out) {
+  return guarded([&] {
+    OSH_CHECK(out != nullptr, "null output pointer");
+    OSH_HIP(hipSetDevice(device));
+    *out = new osh_ctx();
+  });
+}
+
+int osh_ctx_destroy(osh_ctx* ctx) {
+  return guarded([&] { delete ctx; });
+}
+
+int osh_ctx_synchronize(osh_ctx* ctx) {
+  return guarded([&] { ctx->ctx.sync(); });
+}
+
+int osh_default_opts(int dim, osh_adapt_opts* out) {
+  return guarded([&] {
+    OSH_CHECK(dim == 2 || dim == 3, "dimension must be 2 or 3");
+    osh::AdaptOptions o;
+    osh::default_options(dim, o);
+    out->min_length_desired = o.min_length_desired;
+    out->max_length_desired = o.max_length_desired;
+    out->max_length_allowed = o.max_length_allowed;
+    out->min_quality_allowed = o.min_quality_allowed;
+    out->min_quality_desired = o.min_quality_desired;
+    out->nsliver_layers = o.nsliver_layers;
+    out->verbosity = o.verbosity;
+    out->should_refine = o.should_refine;
+    out->should_coarsen = o.should_coarsen;
+    out->should_swap = o.should_swap;
+    out->should_coarsen_slivers = o.should_coarsen_slivers;
+  });
+}
+
+static int build(osh_ctx* ctx, int dim, double x, double y, double z, int nx, int ny, int nz,
+    bool full, osh_mesh** out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && out, "null argument");
+    OSH_CHECK(dim == 2 || dim == 3, "dimension must be 2 or 3");
+    OSH_CHECK(nx > 0 && ny > 0 && (dim == 2 || nz > 0), "cell counts must be positive");
+    std::int64_t ncells = std::int64_t(nx) * ny * (dim == 3 ? nz : 1);
+    OSH_CHECK(ncells * (dim == 3 ? 6 : 2) < (std::int64_t(1) << 30), "box too large");
+    auto m = std::make_unique<osh_mesh>();
+    m->owner = ctx;
+    osh::build_box(ctx->ctx, m->m, dim, x, y, dim == 3 ? z : 0.0, nx, ny, dim == 3 ? nz : 0,
+        full);
+    ctx->ctx.sync();
+    *out = m.release();
+  });
+}
+
+int osh_build_box(osh_ctx* ctx, int dim, double x, double y, double z, int nx, int ny, int nz,
+    osh_mesh** out) {
+  return build(ctx, dim, x, y, z, nx, ny, nz, true, out);
+}
+
+int osh_build_box_internal(osh_ctx* ctx, int dim, double x, double y, double z, int nx, int ny,
+    int nz, osh_mesh** out) {
+  return build(ctx, dim, x, y, z, nx, ny, nz, false, out);
+}
+
+int osh_build_from_elems(osh_ctx* ctx, int dim, int nelems, const int* ev2v, int nverts,
+    osh_mesh** out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && out && (ev2v || nelems == 0), "null argument");
+    OSH_CHECK(dim == 2 || dim == 3, "dimension must be 2 or 3");
+    auto& c = ctx->ctx;
+    std::vector<int> h(ev2v, ev2v + std::size_t(nelems) * (dim + 1));
+    for (int v : h) OSH_CHECK(v >= 0 && v < nverts, "vertex index out of range");
+    auto m = std::make_unique<osh_mesh>();
+    m->owner = ctx;
+    osh::build_from_elems(c, m->m, dim, nelems, osh::to_device(c, h), nverts);
+    c.sync();
+    *out = m.release();
+  });
+}
+
+int osh_mesh_destroy(osh_mesh* mesh) {
+  return guarded([&] { delete mesh; });
+}
+
+int osh_mesh_copy(osh_mesh* mesh, osh_mesh** out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    auto m = std::make_unique<osh_mesh>(*mesh);
+    *out = m.release();
+  });
+}
+
+int osh_mesh_dim(osh_mesh* mesh, int* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    *out = mesh->m.dim;
+  });
+}
+
+int osh_mesh_nents(osh_mesh* mesh, int d, int* out) {
+  return guarded([&] {
+    check_dim(mesh, d);
+    *out = mesh->m.nents[d];
+  });
+}
+
+int osh_mesh_get_verts(osh_mesh* mesh, int d, int* out) {
+  return guarded([&] {
+    check_dim(mesh, d);
+    auto& c = mesh->owner->ctx;
+    if (d == 0) {
+      for (int i = 0; i < mesh->m.nents[0]; ++i) out[i] = i;
+      return;
+    }
+    download(c, osh::verts_of(c, mesh->m, d), out);
+  });
+}
+
+int osh_mesh_get_down(osh_mesh* mesh, int d, int* out, signed char* codes) {
+  return guarded([&] {
+    check_dim(mesh, d);
+    OSH_CHECK(d >= 1, "vertices have no downward adjacency");
+    auto& c = mesh->owner->ctx;
+    download(c, mesh->m.down[d], out);
+    if (codes) {
+      if (d >= 2)
+        download(c, mesh->m.down_codes[d], reinterpret_cast<osh::I8*>(codes));
+      else
+        std::memset(codes, 0, std::size_t(mesh->m.nents[d]) * 2);
+    }
+  });
+}
+
+int osh_mesh_up_size(osh_mesh* mesh, int low, int high, int* out) {
+  return guarded([&] {
+    check_dim(mesh, high);
+    OSH_CHECK(low >= 0 && low < high, "need low < high");
+    *out = osh::ask_up(mesh->owner->ctx, mesh->m, low, high).ab2b.size();
+  });
+}
+
+int osh_mesh_get_up(osh_mesh* mesh, int low, int high, int* a2ab, int* ab2b,
+    signed char* codes) {
+  return guarded([&] {
+    check_dim(mesh, high);
+    OSH_CHECK(low >= 0 && low < high, "need low < high");
+    auto& c = mesh->owner->ctx;
+    auto const& up = osh::ask_up(c, mesh->m, low, high);
+    download(c, up.a2ab, a2ab);
+    download(c, up.ab2b, ab2b);
+    download(c, up.codes, reinterpret_cast<osh::I8*>(codes));
+  });
+}
+
+int osh_mesh_star_size(osh_mesh* mesh, int d, int* out) {
+  return guarded([&] {
+    check_dim(mesh, d);
+    OSH_CHECK(d <= 1, "stars of vertices and edges only");
+    *out = osh::ask_star(mesh->owner->ctx, mesh->m, d).ab2b.size();
+  });
+}
+
+int osh_mesh_get_star(osh_mesh* mesh, int d, int* a2ab, int* ab2b) {
+  return guarded([&] {
+    check_dim(mesh, d);
+    OSH_CHECK(d <= 1, "stars of vertices and edges only");
+    auto& c = mesh->owner->ctx;
+    auto const& g = osh::ask_star(c, mesh->m, d);
+    download(c, g.a2ab, a2ab);
+    download(c, g.ab2b, ab2b);
+  });
+}
+
+int osh_mesh_get_coords(osh_mesh* mesh, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    download(mesh->owner->ctx, mesh->m.coords, out);
+  });
+}
+
+int osh_mesh_set_metric(osh_mesh* mesh, int ncomps, const double* metric) {
+  return guarded([&] {
+    check_mesh(mesh);
+    int dim = mesh->m.dim;
+    OSH_CHECK(ncomps == 1 || ncomps == (dim == 3 ? 6 : 3), "metric must be isotropic or full");
+    auto& c = mesh->owner->ctx;
+    osh::LO n = mesh->m.nents[0] * ncomps;
+    osh::Reals d(c, n);
+    osh::upload(c, d, metric, n);
+    mesh->m.metric = d;
+    mesh->m.metric_ncomps = ncomps;
+    // setting the metric invalidates cached lengths and qualities
+    // (src/Omega_h_mesh.cpp:167-177)
+    mesh->m.length = osh::Reals();
+    mesh->m.quality = osh::Reals();
+    c.sync();
+  });
+}
+
+int osh_mesh_metric_ncomps(osh_mesh* mesh, int* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    *out = mesh->m.metric_ncomps;
+  });
+}
+
+int osh_mesh_get_metric(osh_mesh* mesh, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
+    download(mesh->owner->ctx, mesh->m.metric, out);
+  });
+}
+
+int osh_mesh_get_class(osh_mesh* mesh, int d, int* ids, signed char* dims) {
+  return guarded([&] {
+    check_dim(mesh, d);
+    OSH_CHECK(mesh->m.class_id[d].exists(), "mesh is not classified");
+    auto& c = mesh->owner->ctx;
+    download(c, mesh->m.class_id[d], ids);
+    download(c, mesh->m.class_dim[d], reinterpret_cast<osh::I8*>(dims));
+  });
+}
+
+int osh_mesh_get_lengths(osh_mesh* mesh, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
+    auto& c = mesh->owner->ctx;
+    download(c, osh::ask_lengths(c, mesh->m), out);
+  });
+}
+
+int osh_mesh_get_qualities(osh_mesh* mesh, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
+    auto& c = mesh->owner->ctx;
+    download(c, osh::ask_qualities(c, mesh->m), out);
+  });
+}
+
+static int pass(osh_mesh* mesh, const osh_adapt_opts* opts, int* did, int kind) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(opts != nullptr, "null options");
+    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
+    OSH_CHECK(mesh->m.class_id[0].exists(), "mesh is not classified");
+    osh::AdaptOptions o;
+    to_options(opts, o);
+    auto& c = mesh->owner->ctx;
+    bool r;
+    if (kind == 0)
+      r = osh::refine_by_size(c, mesh->m, o);
+    else if (kind == 1)
+      r = osh::coarsen_by_size(c, mesh->m, o);
+    else if (kind == 2)
+      r = osh::swap_edges(c, mesh->m, o);
+    else
+      r = osh::coarsen_slivers(c, mesh->m, o);
+    c.sync();
+    if (did) *did = r ? 1 : 0;
+  });
+}
+
+int osh_refine_by_size(osh_mesh* m, const osh_adapt_opts* o, int* did) { return pass(m, o, did, 0); }
+int osh_coarsen_by_size(osh_mesh* m, const osh_adapt_opts* o, int* did) { return pass(m, o, did, 1); }
+int osh_swap_edges(osh_mesh* m, const osh_adapt_opts* o, int* did) { return pass(m, o, did, 2); }
+int osh_coarsen_slivers(osh_mesh* m, const osh_adapt_opts* o, int* did) { return pass(m, o, did, 3); }
+
+int osh_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats, int* did) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(opts != nullptr, "null options");
+    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
+    OSH_CHECK(mesh->m.class_id[0].exists(), "mesh is not classified");
+    osh::AdaptOptions o;
+    to_options(opts, o);
+    std::vector<osh::PassRecord> log;
+    bool r = osh::adapt(mesh->owner->ctx, mesh->m, o, &log);
+    mesh->owner->ctx.sync();
+    if (stats) {
+      stats->npasses = int(std::min<std::size_t>(log.size(), OSH_MAX_PASSES));
+      for (int i = 0; i < stats->npasses; ++i) {
+        stats->kind[i] = log[i].kind;
+        stats->nelems_in[i] = log[i].nelems_in;
+        stats->nelems_out[i] = log[i].nelems_out;
+        stats->seconds[i] = log[i].seconds;
+      }
+    }
+    if (did) *did = r ? 1 : 0;
+  });
+}
+
+int osh_measure_edges_metric(osh_ctx* ctx, int dim, int metric_ncomps, int n, const int* edges,
+    const int* ev2v, const double* coords, const double* metric, double* out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && ev2v && coords && metric && out, "null argument");
+    OSH_CHECK(dim == 2 || dim == 3, "dimension must be 2 or 3");
+    OSH_CHECK(metric_ncomps == 1 || metric_ncomps == (dim == 3 ? 6 : 3), "bad metric size");
+    OSH_CHECK(n >= 0, "negative count");
+    osh::launch_edge_lengths(ctx->ctx, dim, metric_ncomps == 1 ? 1 : dim, n, edges, ev2v,
+        coords, metric, out);
+  });
+}
+
+int osh_measure_qualities(osh_ctx* ctx, int dim, int metric_ncomps, int n, const int* elems,
+    const int* ev2v, const double* coords, const double* metric, double* out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && ev2v && coords && metric && out, "null argument");
+    OSH_CHECK(dim == 2 || dim == 3, "dimension must be 2 or 3");
+    OSH_CHECK(metric_ncomps == 1 || metric_ncomps == (dim == 3 ? 6 : 3), "bad metric size");
+    OSH_CHECK(n >= 0, "negative count");
+    osh::launch_elem_qualities(ctx->ctx, dim, metric_ncomps == 1 ? 1 : dim, n, elems, ev2v,
+        coords, metric, out);
+  });
+}
+
+int osh_refine_qualities(osh_mesh* mesh, int n, const int* cands, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
+    auto& c = mesh->owner->ctx;
+    std::vector<int> h(cands, cands + n);
+    for (int e : h) OSH_CHECK(e >= 0 && e < mesh->m.nents[1], "edge out of range");
+    auto q = osh::refine_qualities(c, mesh->m, osh::to_device(c, h));
+    download(c, q, out);
+  });
+}
+
+int osh_find_loop(osh_mesh* mesh, int edge, int* eev2v, int* loop_verts, int* size) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(mesh->m.dim == 3, "edge loops exist in 3D");
+    OSH_CHECK(edge >= 0 && edge < mesh->m.nents[1], "edge out of range");
+    auto& c = mesh->owner->ctx;
+    osh::LOs e2, lv, sz;
+    osh::find_loops(c, mesh->m, osh::to_device(c, std::vector<int>{edge}), e2, lv, sz);
+    download(c, e2, eev2v);
+    download(c, lv, loop_verts);
+    download(c, sz, size);
+  });
+}
+
+int osh_find_indset(osh_mesh* mesh, int d, const double* quality, const signed char* cands,
+    signed char* out) {
+  return guarded([&] {
+    check_dim(mesh, d);
+    auto& c = mesh->owner->ctx;
+    osh::LO n = mesh->m.nents[d];
+    osh::Reals q(c, n);
+    osh::Bytes cb(c, n);
+    osh::upload(c, q, quality, n);
+    osh::upload(c, cb, reinterpret_cast<osh::I8 const*>(cands), n);
+    auto r = osh::find_indset(c, mesh->m, d, q, cb);
+    download(c, r, reinterpret_cast<osh::I8*>(out));
+  });
+}
+
+int osh_eval_math(osh_ctx* ctx, int which, int n, const double* in, double* out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && in && out, "null argument");
+    OSH_CHECK(which >= 0 && which <= 4, "unknown function");
+    auto& c = ctx->ctx;
+    osh::Reals din(c, n), dout(c, n);
+    osh::upload(c, din, in, n);
+    OSH_LAUNCH(eval_math_kernel, n, c, which, din.data(), dout.data());
+    download(c, dout, out);
+  });
+}
+
+}  // extern "C"

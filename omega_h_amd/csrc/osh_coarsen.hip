@@ -1,0 +1,435 @@
+// Edge-collapse (coarsening) pass on the device.
+//
+//   coarsen_by_size / coarsen_slivers / coarsen / coarsen_verts
+//                                    src/Omega_h_coarsen.cpp:29-220
+//   check_collapse_class / check_collapse_exposure
+//                                    src/Omega_h_check_collapse.cpp:10-117
+//   prevent_coarsen_overshoot        src/Omega_h_coarsen_overshoot.cpp:12-49
+//   coarsen_qualities, filter_coarsen_min_qual / _improve
+//                                    src/Omega_h_coarsen_qualities.cpp:16-122
+//   choose_rails / find_rails        src/Omega_h_collapse_rail.cpp:15-101
+//   mark_dead_ents / find_coarsen_domains / coarsen_topology
+//                                    src/Omega_h_coarsen_topology.cpp:15-122
+//   transfer_coarsen                 src/Omega_h_transfer.cpp:585-619
+#include "osh_math.hpp"
+#include "osh_passes.hpp"
+
+namespace osh {
+
+namespace {
+
+constexpr I8 kDont = 0;
+
+__device__ __forceinline__ bool collapses(I8 code, int col) { return (code & (1 << col)) != 0; }
+__device__ __forceinline__ I8 dont_collapse(I8 code, int col) { return I8(code & ~(1 << col)); }
+
+__global__ void lt_kernel(LO n, Real const* a, Real v, I8* out) {
+  LO i = thread_index();
+  if (i < n) out[i] = a[i] < v;
+}
+
+__global__ void edge_codes_from_verts(LO ne, LO const* ev2v, I8 const* vmarks, I8* codes) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  I8 code = kDont;
+  for (int eev = 0; eev < 2; ++eev)
+    if (vmarks[ev2v[e * 2 + eev]]) code = I8(code | (1 << eev));
+  codes[e] = code;
+}
+
+__global__ void check_class_kernel(LO ne, LO const* ev2v, I8 const* vdim, I8 const* edim,
+    I8* codes) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  I8 code = codes[e];
+  if (code == kDont) return;
+  int cd0 = vdim[ev2v[e * 2]], cd1 = vdim[ev2v[e * 2 + 1]];
+  int ecd = edim[e];
+  I8 orig = code;
+  if (cd0 == cd1) {
+    if (cd0 != ecd) code = kDont;
+  } else {
+    int cds[2] = {cd0, cd1};
+    for (int col = 0; col < 2; ++col)
+      if (collapses(orig, col) && cds[col] != ecd) code = dont_collapse(code, col);
+  }
+  codes[e] = code;
+}
+
+__global__ void check_exposure_kernel(LO ne, int cell_dim, LO const* e2c_a2ab,
+    LO const* e2c_ab2b, I8 const* e2c_codes, LO const* cs2s, I8 const* cdim, I8 const* sdim,
+    I8* codes) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  I8 code = codes[e];
+  if (code == kDont) return;
+  int nccs = cell_dim + 1;
+  for (LO ec = e2c_a2ab[e]; ec < e2c_a2ab[e + 1]; ++ec) {
+    LO c = e2c_ab2b[ec];
+    int cce = code_which_down(e2c_codes[ec]);
+    int rot = code_rotation(e2c_codes[ec]);
+    int c_dim = cdim[c];
+    for (int col = 0; col < 2; ++col) {
+      if (!collapses(code, col)) continue;
+      int onto = 1 - col;
+      int ccv_onto = down_template(cell_dim, 1, cce, rot ^ onto);
+      int ccs_opp = opposite_template(cell_dim, 0, ccv_onto);
+      if (sdim[cs2s[c * nccs + ccs_opp]] != c_dim) code = dont_collapse(code, col);
+    }
+  }
+  codes[e] = code;
+}
+
+template <int D, int M>
+__global__ void overshoot_kernel(LO ne, Real max_length, LO const* ev2v, LO const* v2e_a2ab,
+    LO const* v2e_ab2b, I8 const* v2e_codes, Real const* coords, Real const* metric, I8* codes) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  I8 code = codes[e];
+  if (code == kDont) return;
+  constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
+  for (int col = 0; col < 2; ++col) {
+    if (!collapses(code, col)) continue;
+    LO v_col = ev2v[e * 2 + col];
+    LO v_onto = ev2v[e * 2 + (1 - col)];
+    for (LO ve = v2e_a2ab[v_col]; ve < v2e_a2ab[v_col + 1]; ++ve) {
+      LO e2 = v2e_ab2b[ve];
+      if (e2 == e) continue;
+      int eev_in = code_which_down(v2e_codes[ve]);
+      int eev_out = 1 - eev_in;
+      LO nv[2];
+      nv[eev_in] = v_onto;
+      nv[eev_out] = ev2v[e2 * 2 + eev_out];
+      double p0[D], p1[D], m0[nc], m1[nc];
+      for (int k = 0; k < D; ++k) {
+        p0[k] = coords[nv[0] * D + k];
+        p1[k] = coords[nv[1] * D + k];
+      }
+      for (int k = 0; k < nc; ++k) {
+        m0[k] = metric[nv[0] * nc + k];
+        m1[k] = metric[nv[1] * nc + k];
+      }
+      if (edge_length<D, M>(p0, p1, m0, m1) >= max_length) {
+        code = dont_collapse(code, col);
+        break;
+      }
+    }
+  }
+  codes[e] = code;
+}
+
+template <int D, int M>
+__global__ void __launch_bounds__(kBlock) coarsen_qualities_kernel(LO ne, LO const* ev2v,
+    LO const* cv2v, LO const* v2c_a2ab, LO const* v2c_ab2b, I8 const* v2c_codes,
+    Real const* coords, Real const* metric, I8 const* codes, Real* quals) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  I8 code = codes[e];
+  quals[e * 2] = -1.0;
+  quals[e * 2 + 1] = -1.0;
+  if (code == kDont) return;
+  constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
+  for (int col = 0; col < 2; ++col) {
+    if (!collapses(code, col)) continue;
+    LO v_col = ev2v[e * 2 + col];
+    LO v_onto = ev2v[e * 2 + (1 - col)];
+    double minqual = 1.0;
+    for (LO vc = v2c_a2ab[v_col]; vc < v2c_a2ab[v_col + 1]; ++vc) {
+      LO c = v2c_ab2b[vc];
+      int ccv_col = code_which_down(v2c_codes[vc]);
+      LO cv[D + 1];
+      bool will_die = false;
+      for (int k = 0; k < D + 1; ++k) {
+        cv[k] = cv2v[c * (D + 1) + k];
+        if (k != ccv_col && cv[k] == v_onto) will_die = true;
+      }
+      if (will_die) continue;
+      cv[ccv_col] = v_onto;
+      double p[D + 1][D];
+      double m[D + 1][nc];
+      double const* pp[D + 1];
+      double const* ms[D + 1];
+      for (int k = 0; k < D + 1; ++k) {
+        for (int j = 0; j < D; ++j) p[k][j] = coords[cv[k] * D + j];
+        for (int j = 0; j < nc; ++j) m[k][j] = metric[cv[k] * nc + j];
+        pp[k] = p[k];
+        ms[k] = m[k];
+      }
+      minqual = math::min2(minqual, element_quality<D, M>(pp, ms));
+    }
+    quals[e * 2 + col] = minqual;
+  }
+}
+
+__global__ void vert_min_quality(LO nv, LO const* a2ab, LO const* ab2b, Real const* q, Real* out) {
+  LO v = thread_index();
+  if (v >= nv) return;
+  double r = 1.79769313486231570815e+308;
+  for (LO j = a2ab[v]; j < a2ab[v + 1]; ++j) r = math::min2(r, q[ab2b[j]]);
+  out[v] = r;
+}
+
+__global__ void filter_quals_kernel(LO ne, LO const* ev2v, Real const* quals, Real qmin,
+    Real const* vold, I8* codes) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  I8 code = codes[e];
+  if (code == kDont) return;
+  for (int col = 0; col < 2; ++col)
+    if (!(quals[e * 2 + col] >= qmin)) code = dont_collapse(code, col);
+  if (vold)
+    for (int col = 0; col < 2; ++col)
+      if (!(quals[e * 2 + col] > vold[ev2v[e * 2 + col]])) code = dont_collapse(code, col);
+  codes[e] = code;
+}
+
+__global__ void choose_rails_kernel(LO nv, LO const* v2e_a2ab, LO const* v2e_ab2b,
+    I8 const* v2e_codes, I8 const* codes, Real const* lengths, Real const* quals, I8* vcand,
+    Real* vqual, LO* vrail) {
+  LO v = thread_index();
+  if (v >= nv) return;
+  LO best = -1;
+  double best_len = -1, best_q = -1;
+  I8 is_cand = 0;
+  for (LO ve = v2e_a2ab[v]; ve < v2e_a2ab[v + 1]; ++ve) {
+    LO e = v2e_ab2b[ve];
+    I8 code = codes[e];
+    if (code == kDont) continue;
+    int eev = code_which_down(v2e_codes[ve]);
+    if (!collapses(code, eev)) continue;
+    double len = lengths[e];
+    double q = quals[e * 2 + eev];
+    if (best == -1 || len < best_len || (len == best_len && q > best_q)) {
+      is_cand = 1;
+      best = e;
+      best_len = len;
+      best_q = q;
+    }
+  }
+  vcand[v] = is_cand;
+  vqual[v] = best_q;
+  vrail[v] = best;
+}
+
+__global__ void find_rails_kernel(LO nkeys, LO const* keys2verts, LO const* vrail,
+    LO const* v2e_a2ab, LO const* v2e_ab2b, I8 const* v2e_codes, LO const* ev2v,
+    LO* rails2edges, LO* col_dirs, LO* keys2onto) {
+  LO k = thread_index();
+  if (k >= nkeys) return;
+  LO v = keys2verts[k];
+  for (LO ve = v2e_a2ab[v]; ve < v2e_a2ab[v + 1]; ++ve) {
+    if (v2e_ab2b[ve] == vrail[v]) {
+      LO e = v2e_ab2b[ve];
+      int eev = code_which_down(v2e_codes[ve]);
+      rails2edges[k] = e;
+      col_dirs[k] = eev;
+      keys2onto[k] = ev2v[e * 2 + (1 - eev)];
+      return;
+    }
+  }
+}
+
+__global__ void mark_rails(LO nkeys, LO const* rails2edges, I8* dead_edges) {
+  LO k = thread_index();
+  if (k < nkeys) dead_edges[rails2edges[k]] = 1;
+}
+
+__global__ void mark_dead_kernel(LO nkeys, int cell_dim, LO const* rails2edges,
+    LO const* col_dirs, LO const* e2c_a2ab, LO const* e2c_ab2b, I8 const* e2c_codes,
+    LO const* cs2s, I8* dead_cells, I8* dead_sides) {
+  LO k = thread_index();
+  if (k >= nkeys) return;
+  LO e = rails2edges[k];
+  int onto = 1 - col_dirs[k];
+  int nccs = cell_dim + 1;
+  for (LO ec = e2c_a2ab[e]; ec < e2c_a2ab[e + 1]; ++ec) {
+    LO c = e2c_ab2b[ec];
+    int cce = code_which_down(e2c_codes[ec]);
+    int rot = code_rotation(e2c_codes[ec]);
+    int ccv_onto = down_template(cell_dim, 1, cce, rot ^ onto);
+    int ccs_opp = opposite_template(cell_dim, 0, ccv_onto);
+    dead_cells[c] = 1;
+    dead_sides[cs2s[c * nccs + ccs_opp]] = 1;
+  }
+}
+
+__global__ void live_counts(LO nkeys, LO const* keys2verts, LO const* a2ab, LO const* ab2b,
+    I8 const* dead, LO* counts) {
+  LO k = thread_index();
+  if (k >= nkeys) return;
+  LO v = keys2verts[k];
+  LO n = 0;
+  for (LO j = a2ab[v]; j < a2ab[v + 1]; ++j)
+    if (!dead[ab2b[j]]) ++n;
+  counts[k] = n;
+}
+
+__global__ void coarsen_products(LO nkeys, int nvd, LO const* keys2verts, LO const* keys2onto,
+    LO const* a2ab, LO const* ab2b, I8 const* codes, I8 const* dead, LO const* dv2v,
+    LO const* o2n, LO const* keys2prods, LO* pv, LO* psrc) {
+  LO k = thread_index();
+  if (k >= nkeys) return;
+  LO v = keys2verts[k];
+  LO prod = keys2prods[k];
+  for (LO j = a2ab[v]; j < a2ab[v + 1]; ++j) {
+    LO cell = ab2b[j];
+    if (dead[cell]) continue;
+    int ccv_col = code_which_down(codes[j]);
+    for (int ccv = 0; ccv < nvd; ++ccv) {
+      LO ov = (ccv == ccv_col) ? keys2onto[k] : dv2v[cell * nvd + ccv];
+      pv[prod * nvd + ccv] = o2n[ov];
+    }
+    psrc[prod] = cell;
+    ++prod;
+  }
+}
+
+bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool overshoot_allowed,
+    bool improve) {
+  int dim = m.dim;
+  LO ne = m.nents[1];
+  LO nv = m.nents[0];
+  int nc = m.metric_ncomps;
+  int md = (nc == 1) ? 1 : dim;
+  LO const* ev2v = m.down[1].data();
+  OSH_LAUNCH(check_class_kernel, ne, c, ev2v, m.class_dim[0].data(), m.class_dim[1].data(),
+      codes.data());
+  if (max_i8(c, codes) <= kDont) return false;
+  for (int cell_dim = 2; cell_dim <= dim; ++cell_dim) {
+    Adj const& e2c = ask_up(c, m, 1, cell_dim);
+    OSH_LAUNCH(check_exposure_kernel, ne, c, cell_dim, e2c.a2ab.data(), e2c.ab2b.data(),
+        e2c.codes.data(), m.down[cell_dim].data(), m.class_dim[cell_dim].data(),
+        m.class_dim[cell_dim - 1].data(), codes.data());
+  }
+  Real max_length = overshoot_allowed ? o.max_length_allowed : o.max_length_desired;
+  Adj const& v2e = ask_up(c, m, 0, 1);
+#define OSH_OV(D, M)                                                                    \
+  OSH_LAUNCH((overshoot_kernel<D, M>), ne, c, max_length, ev2v, v2e.a2ab.data(),           \
+      v2e.ab2b.data(), v2e.codes.data(), m.coords.data(), m.metric.data(), codes.data())
+  if (dim == 3 && md == 3)
+    OSH_OV(3, 3);
+  else if (dim == 3)
+    OSH_OV(3, 1);
+  else if (md == 2)
+    OSH_OV(2, 2);
+  else
+    OSH_OV(2, 1);
+#undef OSH_OV
+  if (max_i8(c, codes) <= kDont) return false;
+  Reals quals(c, ne * 2);
+  LOs const& cv2v = verts_of(c, m, dim);
+  Adj const& v2c = ask_up(c, m, 0, dim);
+#define OSH_CQ(D, M)                                                                    \
+  OSH_LAUNCH((coarsen_qualities_kernel<D, M>), ne, c, ev2v, cv2v.data(), v2c.a2ab.data(),  \
+      v2c.ab2b.data(), v2c.codes.data(), m.coords.data(), m.metric.data(), codes.data(),    \
+      quals.data())
+  if (dim == 3 && md == 3)
+    OSH_CQ(3, 3);
+  else if (dim == 3)
+    OSH_CQ(3, 1);
+  else if (md == 2)
+    OSH_CQ(2, 2);
+  else
+    OSH_CQ(2, 1);
+#undef OSH_CQ
+  Reals vold;
+  if (improve) {
+    vold = Reals(c, nv);
+    OSH_LAUNCH(vert_min_quality, nv, c, v2c.a2ab.data(), v2c.ab2b.data(),
+        ask_qualities(c, m).data(), vold.data());
+  }
+  OSH_LAUNCH(filter_quals_kernel, ne, c, ev2v, quals.data(), o.min_quality_allowed,
+      improve ? vold.data() : (Real const*)nullptr, codes.data());
+  if (max_i8(c, codes) <= kDont) return false;
+  Reals const& lengths = ask_lengths(c, m);
+  Bytes vcand(c, nv);
+  Reals vqual(c, nv);
+  LOs vrail(c, nv);
+  OSH_LAUNCH(choose_rails_kernel, nv, c, v2e.a2ab.data(), v2e.ab2b.data(), v2e.codes.data(),
+      codes.data(), lengths.data(), quals.data(), vcand.data(), vqual.data(), vrail.data());
+  Bytes keys = find_indset(c, m, 0, vqual, vcand);
+  LOs keys2verts = collect_marked(c, keys);
+  LO nkeys = keys2verts.size();
+  LOs rails2edges(c, nkeys), col_dirs(c, nkeys), keys2onto(c, nkeys);
+  OSH_LAUNCH(find_rails_kernel, nkeys, c, keys2verts.data(), vrail.data(), v2e.a2ab.data(),
+      v2e.ab2b.data(), v2e.codes.data(), ev2v, rails2edges.data(), col_dirs.data(),
+      keys2onto.data());
+  Bytes dead[4];
+  dead[1] = fill_i8(c, ne, 0);
+  OSH_LAUNCH(mark_rails, nkeys, c, rails2edges.data(), dead[1].data());
+  for (int d = 2; d <= dim; ++d) dead[d] = fill_i8(c, m.nents[d], 0);
+  for (int cell_dim = dim; cell_dim > 1; --cell_dim) {
+    Adj const& e2c = ask_up(c, m, 1, cell_dim);
+    OSH_LAUNCH(mark_dead_kernel, nkeys, c, cell_dim, rails2edges.data(), col_dirs.data(),
+        e2c.a2ab.data(), e2c.ab2b.data(), e2c.codes.data(), m.down[cell_dim].data(),
+        dead[cell_dim].data(), dead[cell_dim - 1].data());
+  }
+  DMesh nw;
+  nw.dim = dim;
+  nw.metric_ncomps = nc;
+  LOs o2n, old_lows2new_lows;
+  for (int d = 0; d <= dim; ++d) {
+    LOs k2p, pv, psrc;
+    if (d == 0) {
+      k2p = fill_lo(c, nkeys + 1, 0);
+    } else {
+      Adj const& v2d = ask_up(c, m, 0, d);
+      LOs counts(c, nkeys);
+      OSH_LAUNCH(live_counts, nkeys, c, keys2verts.data(), v2d.a2ab.data(), v2d.ab2b.data(),
+          dead[d].data(), counts.data());
+      k2p = offset_scan(c, counts);
+      LO nprods = last(c, k2p);
+      pv = LOs(c, nprods * (d + 1));
+      psrc = LOs(c, nprods);
+      LOs const& dv2v = verts_of(c, m, d);
+      OSH_LAUNCH(coarsen_products, nkeys, c, d + 1, keys2verts.data(), keys2onto.data(),
+          v2d.a2ab.data(), v2d.ab2b.data(), v2d.codes.data(), dead[d].data(), dv2v.data(),
+          o2n.data(), k2p.data(), pv.data(), psrc.data());
+    }
+    ModOut mo;
+    modify_ents_adapt(c, m, nw, d, 0, keys2verts, k2p, pv, old_lows2new_lows, mo);
+    if (d == 0) {
+      o2n = mo.old2new;
+      nw.coords = transfer<Real>(c, mo, dim, m.coords, nullptr);
+      nw.metric = transfer<Real>(c, mo, nc, m.metric, nullptr);
+      nw.class_id[0] = transfer<LO>(c, mo, 1, m.class_id[0], nullptr);
+      nw.class_dim[0] = transfer<I8>(c, mo, 1, m.class_dim[0], nullptr);
+    } else {
+      transfer_class(c, m, nw, d, mo, psrc, d, nullptr);
+      transfer_measures(c, m, nw, d, mo);
+    }
+    old_lows2new_lows = mo.old2new;
+  }
+  m = nw;
+  return true;
+}
+
+bool coarsen_verts(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes const& vmarks,
+    bool overshoot_allowed, bool improve) {
+  LO ne = m.nents[1];
+  Bytes codes(c, ne);
+  OSH_LAUNCH(edge_codes_from_verts, ne, c, m.down[1].data(), vmarks.data(), codes.data());
+  return coarsen(c, m, o, codes, overshoot_allowed, improve);
+}
+
+}  // namespace
+
+bool coarsen_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
+  LO ne = m.nents[1];
+  Reals const& lengths = ask_lengths(c, m);
+  ask_qualities(c, m);
+  Bytes cand(c, ne);
+  OSH_LAUNCH(lt_kernel, ne, c, lengths.data(), o.min_length_desired, cand.data());
+  if (max_i8(c, cand) != 1) return false;
+  Bytes vm = mark_down(c, m, 1, 0, cand);
+  return coarsen_verts(c, m, o, vm, false, false);
+}
+
+bool coarsen_slivers(Ctx& c, DMesh& m, AdaptOptions const& o) {
+  ask_lengths(c, m);
+  Bytes marks = mark_sliver_layers(c, m, o.min_quality_desired, o.nsliver_layers);
+  Bytes vm = mark_down(c, m, m.dim, 0, marks);
+  return coarsen_verts(c, m, o, vm, true, true);
+}
+
+}  // namespace osh

@@ -1,0 +1,117 @@
+// Device-resident simplex mesh: the Omega_h::Mesh data model
+// (src/Omega_h_mesh.hpp:36-177) with downward adjacencies stored and every
+// other adjacency derived on demand and cached, as Mesh::ask_adj does
+// (src/Omega_h_mesh.cpp:307-357).
+#pragma once
+
+#include "osh_runtime.hpp"
+
+namespace osh {
+
+// CSR adjacency (up adjacency or graph); codes may be absent
+struct Adj {
+  LOs a2ab;
+  LOs ab2b;
+  Bytes codes;
+  bool exists() const { return ab2b.exists(); }
+};
+
+struct DMesh {
+  int dim = 0;
+  LO nents[4] = {-1, -1, -1, -1};
+  // down[d]: entity of dim d -> (d+1) entities of dim d-1; down[1] = edge verts
+  LOs down[4];
+  Bytes down_codes[4];
+  Reals coords;
+  Reals metric;
+  int metric_ncomps = 0;
+  LOs class_id[4];
+  Bytes class_dim[4];
+  Reals length;   // cached edge lengths (may be absent)
+  Reals quality;  // cached element qualities (may be absent)
+  // derived caches
+  LOs verts[4];
+  LOs t2e;
+  Bytes t2e_codes;
+  Adj up[4][4];
+  Adj star[2];
+  Adj dual;
+  void clear_caches();
+};
+
+int simplex_degree(int from_dim, int to_dim);
+
+LOs const& verts_of(Ctx& c, DMesh& m, int d);
+LOs const& ask_down(Ctx& c, DMesh& m, int high, int low, Bytes const** codes);
+Adj const& ask_up(Ctx& c, DMesh& m, int low, int high);
+Adj const& ask_star(Ctx& c, DMesh& m, int d);
+Adj const& ask_dual(Ctx& c, DMesh& m);
+// vertex lists of new entities of dimension high_dim >= 2 matched against the
+// already-built dimension high_dim-1 of m (src/Omega_h_adj.cpp:424-441)
+void reflect_down(Ctx& c, DMesh& m, int high_dim, LO nprods, LOs const& prod_verts,
+    LOs& down, Bytes& codes);
+// unique lower entities of element vertex lists (src/Omega_h_adj.cpp:133-153)
+LOs find_unique(Ctx& c, LO nhigh, int high_dim, int low_dim, LOs const& hv2v);
+
+// metric measures (osh_measure.hip)
+void launch_edge_lengths(Ctx& c, int dim, int md, LO n, LO const* edges, LO const* ev2v,
+    Real const* coords, Real const* metric, Real* out);
+void launch_elem_qualities(Ctx& c, int dim, int md, LO n, LO const* elems, LO const* ev2v,
+    Real const* coords, Real const* metric, Real* out);
+void measure_edges(Ctx& c, DMesh& m, LO n, LO const* edges, Real* out);
+void measure_elems(Ctx& c, DMesh& m, LO n, LO const* elems, Real* out);
+Reals const& ask_lengths(Ctx& c, DMesh& m);
+Reals const& ask_qualities(Ctx& c, DMesh& m);
+void mident_metrics(Ctx& c, DMesh& m, LO n, LO const* edges, Real* out);
+
+// build_box (osh_build.hip)
+void build_box(Ctx& c, DMesh& m, int dim, Real x, Real y, Real z, LO nx, LO ny, LO nz,
+    bool reorder_and_classify);
+void build_from_elems(Ctx& c, DMesh& m, int dim, LO nelems, LOs const& ev2v, LO nverts);
+
+// ---- device-side templates (src/Omega_h_simplex.hpp, src/Omega_h_align.hpp) ----
+__host__ __device__ __forceinline__ int down_template(int elem_dim, int bdry_dim,
+    int which_bdry, int which_vert) {
+  if (bdry_dim == 0) return which_bdry;
+  if (elem_dim == 3 && bdry_dim == 2) {
+    // faces {0,2,1},{0,1,3},{1,2,3},{2,0,3}
+    int const t[4][3] = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {2, 0, 3}};
+    return t[which_bdry][which_vert];
+  }
+  if (elem_dim == 3 && bdry_dim == 1) {
+    int const t[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+    return t[which_bdry][which_vert];
+  }
+  // triangle edges {0,1},{1,2},{2,0}
+  return (which_vert == 0) ? which_bdry : ((which_bdry + 1) % 3);
+}
+
+__host__ __device__ __forceinline__ int opposite_template(int elem_dim, int bdry_dim,
+    int which_bdry) {
+  if (elem_dim == 3) {
+    if (bdry_dim == 0) {
+      int const t[4] = {2, 3, 1, 0};
+      return t[which_bdry];
+    }
+    if (bdry_dim == 1) {
+      int const t[6] = {5, 3, 4, 1, 2, 0};
+      return t[which_bdry];
+    }
+    int const t[4] = {3, 2, 0, 1};
+    return t[which_bdry];
+  }
+  if (elem_dim == 2) {
+    if (bdry_dim == 0) return (which_bdry + 1) % 3;
+    return (which_bdry + 2) % 3;
+  }
+  return 1 - which_bdry;
+}
+
+__host__ __device__ __forceinline__ int code_is_flipped(I8 c) { return c & 1; }
+__host__ __device__ __forceinline__ int code_rotation(I8 c) { return (c >> 1) & 3; }
+__host__ __device__ __forceinline__ int code_which_down(I8 c) { return c >> 3; }
+__host__ __device__ __forceinline__ I8 make_code(int flipped, int rot, int which_down) {
+  return I8((which_down << 3) | (rot << 1) | (flipped ? 1 : 0));
+}
+
+}  // namespace osh

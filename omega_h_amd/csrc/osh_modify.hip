@@ -1,0 +1,408 @@
+// Topology rebuild, field transfer, independent sets and marking on the
+// device.
+//
+//   modify_ents_adapt / get_mods2reps / get_rep_counts / get_rep2md_order /
+//   assign_new_numbering / modify_conn   src/Omega_h_modify.cpp:20-70,141-517
+//   (on one rank the global numbers equal the new local numbers, so the
+//   modify_globals exchange reduces to the same scan)
+//   transfer_common / transfer_inherit_* src/Omega_h_transfer.cpp:149-180
+//   indset::find / QualityCompare         src/Omega_h_indset_inline.hpp:12-72,
+//                                         src/Omega_h_indset.cpp:5-35
+//   mark_down / mark_dual_layers / mark_sliver_layers src/Omega_h_mark.cpp:12-47,190-212
+#include "osh_passes.hpp"
+
+namespace osh {
+
+namespace {
+
+__global__ void mark_image_kernel(LO n, LO const* a2b, I8* marks) {
+  LO i = thread_index();
+  if (i < n) marks[a2b[i]] = 1;
+}
+
+__global__ void mark_up_kernel(LO nh, int deg, LO const* hl2l, I8 const* low_marks, I8* out) {
+  LO h = thread_index();
+  if (h >= nh) return;
+  I8 r = 0;
+  for (int j = 0; j < deg; ++j)
+    if (low_marks[hl2l[h * deg + j]]) r = 1;
+  out[h] = r;
+}
+
+__global__ void invert_marks_kernel(LO n, I8 const* in, I8* out) {
+  LO i = thread_index();
+  if (i < n) out[i] = in[i] ? 0 : 1;
+}
+
+__global__ void reps_up_kernel(LO nkeys, LO const* keys2kds, LO const* a2ab, LO const* ab2b,
+    LO* reps) {
+  LO k = thread_index();
+  if (k < nkeys) reps[k] = ab2b[a2ab[keys2kds[k]]];
+}
+
+__global__ void reps_down_kernel(LO nkeys, LO const* keys2kds, int deg, LO const* down, LO* reps) {
+  LO k = thread_index();
+  if (k < nkeys) reps[k] = down[keys2kds[k] * deg];
+}
+
+__global__ void set_same_counts(LO nsame, LO const* same2old, LO* counts) {
+  LO s = thread_index();
+  if (s < nsame) counts[same2old[s]] = 1;
+}
+
+__global__ void add_rep_counts(LO nkeys, LO const* reps, LO const* keys2prods, LO* counts) {
+  LO k = thread_index();
+  if (k < nkeys) atomicAdd(&counts[reps[k]], keys2prods[k + 1] - keys2prods[k]);
+}
+
+__global__ void scatter_index(LO n, LO const* a2b, LO* b2a) {
+  LO i = thread_index();
+  if (i < n) b2a[a2b[i]] = i;
+}
+
+__global__ void rep2md_order_kernel(LO nreps, LO const* a2ab, LO const* ab2b, I8 const* codes,
+    LO const* md2mod, LO const* keys2prods, LO* order) {
+  LO rep = thread_index();
+  if (rep >= nreps) return;
+  LO offset = 0;
+  for (LO j = a2ab[rep]; j < a2ab[rep + 1]; ++j) {
+    LO md = ab2b[j];
+    LO mod = md2mod[md];
+    if (mod >= 0 && code_which_down(codes[j]) == 0) {
+      order[md] = offset;
+      offset += keys2prods[mod + 1] - keys2prods[mod];
+    }
+  }
+}
+
+__global__ void same_numbers(LO nsame, LO const* same2old, LO const* offsets, LO* same2new,
+    LO* old2new) {
+  LO s = thread_index();
+  if (s >= nsame) return;
+  LO o = same2old[s];
+  same2new[s] = offsets[o];
+  old2new[o] = offsets[o];
+}
+
+__global__ void prod_numbers(LO nkeys, LO const* reps, LO const* keys2kds, LO const* offsets,
+    LO const* order, LO const* keys2prods, LO* prods2new) {
+  LO k = thread_index();
+  if (k >= nkeys) return;
+  LO offset = offsets[reps[k]];
+  if (order) offset += order[keys2kds[k]] + 1;
+  for (LO p = keys2prods[k]; p < keys2prods[k + 1]; ++p) prods2new[p] = offset++;
+}
+
+__global__ void same_conn(LO nsame, int deg, LO const* same2old, LO const* same2new,
+    LO const* old_down, I8 const* old_codes, LO const* old_lows2new_lows, LO* down, I8* codes) {
+  LO s = thread_index();
+  if (s >= nsame) return;
+  LO o = same2old[s], n = same2new[s];
+  for (int j = 0; j < deg; ++j) {
+    down[n * deg + j] = old_lows2new_lows[old_down[o * deg + j]];
+    if (codes) codes[n * deg + j] = old_codes[o * deg + j];
+  }
+}
+
+__global__ void prod_conn(LO nprods, int deg, LO const* prods2new, LO const* pdown,
+    I8 const* pcodes, LO* down, I8* codes) {
+  LO p = thread_index();
+  if (p >= nprods) return;
+  LO n = prods2new[p];
+  for (int j = 0; j < deg; ++j) {
+    down[n * deg + j] = pdown[p * deg + j];
+    if (codes) codes[n * deg + j] = pcodes[p * deg + j];
+  }
+}
+
+template <class T>
+__global__ void scatter_same_data(LO nsame, int nc, LO const* same2old, LO const* same2new,
+    T const* old_data, T* data) {
+  LO s = thread_index();
+  if (s >= nsame) return;
+  for (int k = 0; k < nc; ++k) data[same2new[s] * nc + k] = old_data[same2old[s] * nc + k];
+}
+
+template <class T>
+__global__ void scatter_prod_data(LO nprods, int nc, LO const* prods2new, T const* pdata, T* data) {
+  LO p = thread_index();
+  if (p >= nprods) return;
+  for (int k = 0; k < nc; ++k) data[prods2new[p] * nc + k] = pdata[p * nc + k];
+}
+
+__global__ void gather_class(LO nprods, LO const* src, int src_dim, I8 const* src_dims,
+    LO const* ids_a, I8 const* dims_a, LO const* ids_b, I8 const* dims_b, LO* pid, I8* pdim) {
+  LO p = thread_index();
+  if (p >= nprods) return;
+  int sd = src_dims ? src_dims[p] : src_dim;
+  bool second = (sd != src_dim);
+  LO s = src[p];
+  pid[p] = second ? ids_b[s] : ids_a[s];
+  pdim[p] = second ? dims_b[s] : dims_a[s];
+}
+
+__global__ void indset_init(LO n, I8 const* cands, I8* state) {
+  LO i = thread_index();
+  if (i < n) state[i] = cands[i] ? 2 : 0;
+}
+
+__global__ void indset_iteration(LO n, LO const* a2ab, LO const* ab2b, Real const* q,
+    I8 const* old_state, I8* new_state) {
+  LO v = thread_index();
+  if (v >= n) return;
+  I8 s = old_state[v];
+  new_state[v] = s;
+  if (s != 2) return;
+  LO b = a2ab[v], e = a2ab[v + 1];
+  for (LO j = b; j < e; ++j)
+    if (old_state[ab2b[j]] == 1) {
+      new_state[v] = 0;
+      return;
+    }
+  Real qv = q[v];
+  for (LO j = b; j < e; ++j) {
+    LO u = ab2b[j];
+    if (old_state[u] == 0) continue;
+    Real qu = q[u];
+    bool beats = (qu != qv) ? (qu < qv) : (u < v);
+    if (!beats) return;
+  }
+  new_state[v] = 1;
+}
+
+__global__ void mark_down_kernel(LO nl, LO const* a2ab, LO const* ab2b, I8 const* marks, I8* out) {
+  LO l = thread_index();
+  if (l >= nl) return;
+  I8 r = 0;
+  for (LO j = a2ab[l]; j < a2ab[l + 1]; ++j)
+    if (marks[ab2b[j]]) r = 1;
+  out[l] = r;
+}
+
+__global__ void below_kernel(LO n, Real const* q, Real ceil, I8* out) {
+  LO i = thread_index();
+  if (i < n) out[i] = q[i] < ceil;
+}
+
+__global__ void dual_max_kernel(LO n, LO const* a2ab, LO const* ab2b, I8 const* in, I8* out) {
+  LO e = thread_index();
+  if (e >= n) return;
+  I8 r = -128;
+  for (LO j = a2ab[e]; j < a2ab[e + 1]; ++j) {
+    I8 x = in[ab2b[j]];
+    r = (r < x) ? x : r;
+  }
+  out[e] = r;
+}
+
+__global__ void degrees_kernel(LO n, LO const* a2ab, LO const* rows, LO* out) {
+  LO i = thread_index();
+  if (i >= n) return;
+  LO r = rows ? rows[i] : i;
+  out[i] = a2ab[r + 1] - a2ab[r];
+}
+
+}  // namespace
+
+LOs degrees_of(Ctx& c, LOs const& a2ab, LOs const& rows) {
+  LO n = rows.exists() ? rows.size() : a2ab.size() - 1;
+  LOs out(c, n);
+  OSH_LAUNCH(degrees_kernel, n, c, a2ab.data(), rows.exists() ? rows.data() : (LO const*)nullptr,
+      out.data());
+  return out;
+}
+
+void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
+    LOs const& keys2kds, LOs const& keys2prods, LOs const& prod_verts,
+    LOs const& old_lows2new_lows, ModOut& out) {
+  LO nkeys = keys2kds.size();
+  LO nold = old.nents[ent_dim];
+  // same entities: those not adjacent to a key (collect_same)
+  if (ent_dim >= key_dim) {
+    Bytes key_marks = fill_i8(c, old.nents[key_dim], 0);
+    OSH_LAUNCH(mark_image_kernel, nkeys, c, keys2kds.data(), key_marks.data());
+    Bytes mods;
+    if (ent_dim == key_dim) {
+      mods = key_marks;
+    } else {
+      mods = Bytes(c, nold);
+      int deg = simplex_degree(ent_dim, key_dim);
+      LOs const& hl2l = ask_down(c, old, ent_dim, key_dim, nullptr);
+      OSH_LAUNCH(mark_up_kernel, nold, c, deg, hl2l.data(), key_marks.data(), mods.data());
+    }
+    Bytes same(c, nold);
+    OSH_LAUNCH(invert_marks_kernel, nold, c, mods.data(), same.data());
+    out.same2old = collect_marked(c, same);
+  } else {
+    out.same2old = iota(c, nold);
+  }
+  LO nsame = out.same2old.size();
+  // representatives
+  LOs reps;
+  if (ent_dim == key_dim) {
+    reps = keys2kds;
+  } else if (ent_dim > key_dim) {
+    Adj const& up = ask_up(c, old, key_dim, ent_dim);
+    reps = LOs(c, nkeys);
+    OSH_LAUNCH(reps_up_kernel, nkeys, c, keys2kds.data(), up.a2ab.data(), up.ab2b.data(), reps.data());
+  } else {
+    int deg = simplex_degree(key_dim, ent_dim);
+    LOs const& down = ask_down(c, old, key_dim, ent_dim, nullptr);
+    reps = LOs(c, nkeys);
+    OSH_LAUNCH(reps_down_kernel, nkeys, c, keys2kds.data(), deg, down.data(), reps.data());
+  }
+  LOs counts = fill_lo(c, nold, 0);
+  OSH_LAUNCH(set_same_counts, nsame, c, out.same2old.data(), counts.data());
+  OSH_LAUNCH(add_rep_counts, nkeys, c, reps.data(), keys2prods.data(), counts.data());
+  LOs offsets = offset_scan(c, counts);
+  out.nnew = last(c, offsets);
+  LOs order;
+  if (key_dim > ent_dim) {
+    LO nkd = old.nents[key_dim];
+    order = fill_lo(c, nkd, -1);
+    LOs md2mod = fill_lo(c, nkd, -1);
+    OSH_LAUNCH(scatter_index, nkeys, c, keys2kds.data(), md2mod.data());
+    Adj const& up = ask_up(c, old, ent_dim, key_dim);
+    OSH_LAUNCH(rep2md_order_kernel, nold, c, up.a2ab.data(), up.ab2b.data(), up.codes.data(),
+        md2mod.data(), keys2prods.data(), order.data());
+  }
+  out.same2new = LOs(c, nsame);
+  out.old2new = fill_lo(c, nold, -1);
+  OSH_LAUNCH(same_numbers, nsame, c, out.same2old.data(), offsets.data(), out.same2new.data(),
+      out.old2new.data());
+  LO nprods = last(c, keys2prods);
+  out.prods2new = LOs(c, nprods);
+  OSH_LAUNCH(prod_numbers, nkeys, c, reps.data(), keys2kds.data(), offsets.data(),
+      order.exists() ? order.data() : (LO const*)nullptr, keys2prods.data(),
+      out.prods2new.data());
+  if (ent_dim == 0) {
+    nw.nents[0] = out.nnew;
+    return;
+  }
+  int deg = ent_dim + 1;
+  LO nnew = out.nnew;
+  LOs down(c, nnew * deg);
+  Bytes codes;
+  bool with_codes = ent_dim >= 2;
+  if (with_codes) codes = Bytes(c, nnew * deg);
+  OSH_LAUNCH(same_conn, nsame, c, deg, out.same2old.data(), out.same2new.data(),
+      old.down[ent_dim].data(), with_codes ? old.down_codes[ent_dim].data() : (I8 const*)nullptr,
+      old_lows2new_lows.data(), down.data(), with_codes ? codes.data() : (I8*)nullptr);
+  if (with_codes) {
+    LOs pdown;
+    Bytes pcodes;
+    reflect_down(c, nw, ent_dim, nprods, prod_verts, pdown, pcodes);
+    OSH_LAUNCH(prod_conn, nprods, c, deg, out.prods2new.data(), pdown.data(), pcodes.data(),
+        down.data(), codes.data());
+  } else {
+    OSH_LAUNCH(prod_conn, nprods, c, deg, out.prods2new.data(), prod_verts.data(),
+        (I8 const*)nullptr, down.data(), (I8*)nullptr);
+  }
+  nw.down[ent_dim] = down;
+  nw.down_codes[ent_dim] = codes;
+  nw.nents[ent_dim] = nnew;
+}
+
+template <class T>
+DArray<T> transfer(Ctx& c, ModOut const& mo, int nc, DArray<T> const& old_data,
+    DArray<T> const* prod_data) {
+  DArray<T> d(c, mo.nnew * nc);
+  if (prod_data)
+    OSH_LAUNCH(scatter_prod_data<T>, mo.prods2new.size(), c, nc, mo.prods2new.data(),
+        prod_data->data(), d.data());
+  OSH_LAUNCH(scatter_same_data<T>, mo.same2old.size(), c, nc, mo.same2old.data(),
+      mo.same2new.data(), old_data.data(), d.data());
+  return d;
+}
+
+template DArray<LO> transfer(Ctx&, ModOut const&, int, DArray<LO> const&, DArray<LO> const*);
+template DArray<I8> transfer(Ctx&, ModOut const&, int, DArray<I8> const&, DArray<I8> const*);
+template DArray<Real> transfer(Ctx&, ModOut const&, int, DArray<Real> const&, DArray<Real> const*);
+
+void transfer_class(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo,
+    LOs const& prod_src, int src_dim, Bytes const* src_dims) {
+  LO nprods = mo.prods2new.size();
+  LOs pid(c, nprods);
+  Bytes pdim(c, nprods);
+  int other = (src_dims && src_dim + 1 <= old.dim) ? src_dim + 1 : src_dim;
+  OSH_LAUNCH(gather_class, nprods, c, prod_src.data(), src_dim,
+      src_dims ? src_dims->data() : (I8 const*)nullptr, old.class_id[src_dim].data(),
+      old.class_dim[src_dim].data(), old.class_id[other].data(), old.class_dim[other].data(),
+      pid.data(), pdim.data());
+  nw.class_id[d] = transfer<LO>(c, mo, 1, old.class_id[d], &pid);
+  nw.class_dim[d] = transfer<I8>(c, mo, 1, old.class_dim[d], &pdim);
+}
+
+void transfer_measures(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo) {
+  LO nprods = mo.prods2new.size();
+  if (d == 1) {
+    Reals pd(c, nprods);
+    measure_edges(c, nw, nprods, mo.prods2new.data(), pd.data());
+    nw.length = transfer<Real>(c, mo, 1, ask_lengths(c, old), &pd);
+  }
+  if (d == old.dim) {
+    Reals pd(c, nprods);
+    measure_elems(c, nw, nprods, mo.prods2new.data(), pd.data());
+    nw.quality = transfer<Real>(c, mo, 1, ask_qualities(c, old), &pd);
+  }
+}
+
+Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& cands) {
+  LO n = m.nents[d];
+  if (d == m.dim) return cands;
+  Adj const& g = ask_star(c, m, d);
+  Bytes a(c, n), b(c, n);
+  OSH_LAUNCH(indset_init, n, c, cands.data(), a.data());
+  for (LO round = 0; max_i8(c, a) == 2; ++round) {
+    OSH_CHECK(round <= n + 1, "independent set does not converge");
+    OSH_LAUNCH(indset_iteration, n, c, g.a2ab.data(), g.ab2b.data(), quality.data(), a.data(),
+        b.data());
+    std::swap(a, b);
+  }
+  return a;
+}
+
+Bytes mark_down(Ctx& c, DMesh& m, int high, int low, Bytes const& marks) {
+  if (high == low) return marks;
+  Adj const& l2h = ask_up(c, m, low, high);
+  Bytes out(c, m.nents[low]);
+  OSH_LAUNCH(mark_down_kernel, m.nents[low], c, l2h.a2ab.data(), l2h.ab2b.data(), marks.data(),
+      out.data());
+  return out;
+}
+
+Bytes mark_sliver_layers(Ctx& c, DMesh& m, Real qual_ceil, int nlayers) {
+  LO ne = m.nents[m.dim];
+  Reals const& q = ask_qualities(c, m);
+  Bytes marks(c, ne);
+  OSH_LAUNCH(below_kernel, ne, c, q.data(), qual_ceil, marks.data());
+  Adj const& dual = ask_dual(c, m);
+  for (int i = 0; i < nlayers; ++i) {
+    Bytes next(c, ne);
+    OSH_LAUNCH(dual_max_kernel, ne, c, dual.a2ab.data(), dual.ab2b.data(), marks.data(),
+        next.data());
+    marks = next;
+  }
+  return marks;
+}
+
+void default_options(int dim, AdaptOptions& o) {
+  o.min_length_desired = 1.0 / __builtin_sqrt(2.0);
+  o.max_length_desired = __builtin_sqrt(2.0);
+  o.max_length_allowed = o.max_length_desired * 2.0;
+  if (dim == 3) {
+    o.min_quality_allowed = 0.20;
+    o.min_quality_desired = 0.30;
+  } else {
+    o.min_quality_allowed = 0.30;
+    o.min_quality_desired = 0.40;
+  }
+  o.nsliver_layers = 4;
+  o.verbosity = 0;
+  o.should_refine = 1;
+  o.should_coarsen = 1;
+  o.should_swap = 1;
+  o.should_coarsen_slivers = 1;
+}
+
+}  // namespace osh

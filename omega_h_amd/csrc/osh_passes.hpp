@@ -1,0 +1,72 @@
+// Internal interfaces shared by the adapt() passes.
+#pragma once
+
+#include "osh_mesh.hpp"
+
+namespace osh {
+
+// mirrors Omega_h::AdaptOpts (src/Omega_h_adapt.hpp:50-77)
+struct AdaptOptions {
+  Real min_length_desired;
+  Real max_length_desired;
+  Real max_length_allowed;
+  Real min_quality_allowed;
+  Real min_quality_desired;
+  int nsliver_layers;
+  int verbosity;
+  int should_refine;
+  int should_coarsen;
+  int should_swap;
+  int should_coarsen_slivers;
+};
+
+void default_options(int dim, AdaptOptions& o);
+
+// result of modify_ents for one dimension (src/Omega_h_modify.cpp:468-517)
+struct ModOut {
+  LO nnew = 0;
+  LOs same2old;
+  LOs same2new;
+  LOs prods2new;
+  LOs old2new;
+};
+
+void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
+    LOs const& keys2kds, LOs const& keys2prods, LOs const& prod_verts,
+    LOs const& old_lows2new_lows, ModOut& out);
+
+// new tag data: same entities copy old values, products take prod values
+template <class T>
+DArray<T> transfer(Ctx& c, ModOut const& mo, int ncomps, DArray<T> const& old_data,
+    DArray<T> const* prod_data);
+// class_id/class_dim of products from one old entity each, of dimension
+// src_dims[p] (or src_dim when src_dims is null)
+void transfer_class(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo,
+    LOs const& prod_src, int src_dim, Bytes const* src_dims);
+// lengths (d == 1) and qualities (d == dim) of products on the new mesh
+void transfer_measures(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo);
+
+Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& cands);
+Bytes mark_down(Ctx& c, DMesh& m, int high, int low, Bytes const& marks);
+Bytes mark_sliver_layers(Ctx& c, DMesh& m, Real qual_ceil, int nlayers);
+LOs degrees_of(Ctx& c, LOs const& a2ab, LOs const& rows);
+
+bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o);
+bool coarsen_by_size(Ctx& c, DMesh& m, AdaptOptions const& o);
+bool coarsen_slivers(Ctx& c, DMesh& m, AdaptOptions const& o);
+bool swap_edges(Ctx& c, DMesh& m, AdaptOptions const& o);
+
+// known-answer hooks used by the tests
+Reals refine_qualities(Ctx& c, DMesh& m, LOs const& cands);
+void find_loops(Ctx& c, DMesh& m, LOs const& edges, LOs& eev2v, LOs& loop_verts, LOs& sizes);
+
+struct PassRecord {
+  int kind;  // 0 refine, 1 coarsen, 2 swap, 3 coarsen slivers
+  LO nelems_in;
+  LO nelems_out;
+  double seconds;
+};
+
+bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log);
+
+}  // namespace osh

@@ -1,0 +1,151 @@
+// Device runtime for the MI355X adapt() path: one HIP stream per context,
+// stream-ordered device arrays, launch helpers and error reporting.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace osh {
+
+using LO = std::int32_t;
+using I8 = std::int8_t;
+using I64 = std::int64_t;
+using Real = double;
+
+struct Error : std::runtime_error {
+  explicit Error(std::string const& m) : std::runtime_error(m) {}
+};
+
+#define OSH_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t osh_e_ = (call);                                              \
+    if (osh_e_ != hipSuccess)                                                \
+      throw ::osh::Error(std::string("HIP error ") +                         \
+                         hipGetErrorString(osh_e_) + " at " __FILE__ ":" +   \
+                         std::to_string(__LINE__) + " in " #call);            \
+  } while (0)
+
+#define OSH_CHECK(cond, msg)                                                 \
+  do {                                                                       \
+    if (!(cond)) throw ::osh::Error(std::string("check failed: ") + (msg));  \
+  } while (0)
+
+struct Ctx {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // pinned scratch for small readbacks
+  void* pinned = nullptr;
+  Ctx();
+  ~Ctx();
+  Ctx(Ctx const&) = delete;
+  Ctx& operator=(Ctx const&) = delete;
+  void sync();
+};
+
+// reference-counted device buffer allocated and freed in stream order;
+// a default-constructed array does not exist (like an empty Omega_h Read<T>)
+template <class T>
+class DArray {
+ public:
+  DArray() = default;
+  DArray(Ctx& c, LO n) : n_(n), exists_(true) {
+    OSH_CHECK(n >= 0, "negative array size");
+    T* p = nullptr;
+    if (n > 0) OSH_HIP(hipMallocAsync((void**)&p, sizeof(T) * size_t(n), c.stream));
+    hipStream_t s = c.stream;
+    ptr_ = std::shared_ptr<T>(p, [s](T* q) {
+      if (q) (void)hipFreeAsync(q, s);
+    });
+  }
+  T* data() const { return ptr_.get(); }
+  LO size() const { return n_; }
+  bool exists() const { return exists_; }
+
+ private:
+  std::shared_ptr<T> ptr_;
+  LO n_ = 0;
+  bool exists_ = false;
+};
+
+using LOs = DArray<LO>;
+using Bytes = DArray<I8>;
+using Reals = DArray<Real>;
+
+template <class T>
+inline void upload(Ctx& c, DArray<T>& d, T const* h, LO n) {
+  if (n) OSH_HIP(hipMemcpyAsync(d.data(), h, sizeof(T) * size_t(n), hipMemcpyHostToDevice, c.stream));
+}
+
+template <class T>
+inline DArray<T> to_device(Ctx& c, std::vector<T> const& h) {
+  DArray<T> d(c, LO(h.size()));
+  upload(c, d, h.data(), LO(h.size()));
+  return d;
+}
+
+template <class T>
+inline std::vector<T> to_host(Ctx& c, DArray<T> const& d) {
+  std::vector<T> h(size_t(d.size()));
+  if (d.size())
+    OSH_HIP(hipMemcpyAsync(h.data(), d.data(), sizeof(T) * size_t(d.size()), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  return h;
+}
+
+template <class T>
+inline T get_value(Ctx& c, T const* dptr) {
+  OSH_HIP(hipMemcpyAsync(c.pinned, dptr, sizeof(T), hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  T v;
+  std::memcpy(&v, c.pinned, sizeof(T));
+  return v;
+}
+
+template <class T>
+inline T last(Ctx& c, DArray<T> const& a) {
+  return get_value(c, a.data() + a.size() - 1);
+}
+
+constexpr int kBlock = 256;
+
+inline unsigned nblocks(LO n, int block = kBlock) {
+  return unsigned((std::int64_t(n) + block - 1) / block);
+}
+
+#define OSH_LAUNCH(kernel, n, c, ...)                                          \
+  do {                                                                         \
+    ::osh::LO osh_n_ = (n);                                                         \
+    if (osh_n_ > 0) {                                                          \
+      hipLaunchKernelGGL(kernel, dim3(::osh::nblocks(osh_n_)),                 \
+          dim3(::osh::kBlock), 0, (c).stream, osh_n_, __VA_ARGS__);            \
+      OSH_HIP(hipGetLastError());                                              \
+    }                                                                          \
+  } while (0)
+
+__device__ __forceinline__ LO thread_index() {
+  return LO(blockIdx.x) * LO(blockDim.x) + LO(threadIdx.x);
+}
+
+// ---- primitives (osh_prims.hip) ----
+LOs offset_scan(Ctx& c, LOs const& counts);           // n+1, exclusive
+LOs offset_scan_bytes(Ctx& c, Bytes const& counts);   // n+1, exclusive
+LOs collect_marked(Ctx& c, Bytes const& marks);       // indices of nonzero
+LOs iota(Ctx& c, LO n, LO start = 0, LO stride = 1);
+LOs fill_lo(Ctx& c, LO n, LO value);
+Bytes fill_i8(Ctx& c, LO n, I8 value);
+Reals fill_real(Ctx& c, LO n, Real value);
+I8 max_i8(Ctx& c, Bytes const& a);
+Real min_real(Ctx& c, Reals const& a);
+Real max_real(Ctx& c, Reals const& a);
+LO sum_lo(Ctx& c, LOs const& a);
+// stable sort of n tuples of `width` int64 keys; returns the permutation
+LOs sort_by_keys(Ctx& c, DArray<I64> const& keys, int width);
+
+}  // namespace osh

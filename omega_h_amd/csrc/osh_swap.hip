@@ -1,0 +1,455 @@
+// Edge-swap pass on the device (3D edge swap over edge loops of up to 7
+// tets, 2D edge flip).
+//
+//   swap_edges / swap_part1 / filter_swap_improve src/Omega_h_swap.cpp:14-55
+//   swap3d_ghosted / swap3d_element_based        src/Omega_h_swap3d.cpp:14-84
+//   swap3d_qualities                             src/Omega_h_swap3d_qualities.cpp:11-75
+//   find_loop                                    src/Omega_h_swap3d_loop.hpp:24-78
+//   choose                                       src/Omega_h_swap3d_choice.hpp:16-100
+//   swap3d_keys_to_prods / swap3d_topology       src/Omega_h_swap3d_topology.cpp:12-119
+//   swap2d_qualities / swap2d_topology           src/Omega_h_swap2d_qualities.cpp:16-72,
+//                                                src/Omega_h_swap2d_topology.cpp:11-79
+//   transfer_swap / transfer_copy                src/Omega_h_transfer.cpp:631-730
+#include "osh_math.hpp"
+#include "osh_passes.hpp"
+#include "osh_swap_tables.hpp"
+
+namespace osh {
+
+namespace {
+
+using swap::Tables;
+
+struct Loop {
+  int size;
+  LO eev2v[2];
+  LO verts[swap::kMaxLoop];
+};
+
+__device__ void find_loop(LO edge, LO const* a2ab, LO const* ab2b, I8 const* codes,
+    LO const* ev2v, LO const* tv2v, Loop& loop) {
+  LO begin = a2ab[edge], end = a2ab[edge + 1];
+  loop.size = end - begin;
+  if (loop.size > swap::kMaxLoop) return;
+  loop.eev2v[0] = ev2v[edge * 2];
+  loop.eev2v[1] = ev2v[edge * 2 + 1];
+  LO tmp[swap::kMaxLoop][2];
+  for (int le = 0; le < loop.size; ++le) {
+    LO tet = ab2b[begin + le];
+    int rre = code_which_down(codes[begin + le]);
+    int rot = code_rotation(codes[begin + le]);
+    int rre_opp = opposite_template(3, 1, rre);
+    for (int eev = 0; eev < 2; ++eev)
+      tmp[le][eev] = tv2v[tet * 4 + down_template(3, 1, rre_opp, rot ^ eev)];
+  }
+  for (int i = 0; i < loop.size - 1; ++i) {
+    int j;
+    for (j = i + 1; j < loop.size; ++j)
+      if (tmp[j][0] == tmp[i][1]) break;
+    if (j >= loop.size) j = i + 1;  // topological error: the reference aborts here
+    LO t0 = tmp[i + 1][0], t1 = tmp[i + 1][1];
+    tmp[i + 1][0] = tmp[j][0];
+    tmp[i + 1][1] = tmp[j][1];
+    tmp[j][0] = t0;
+    tmp[j][1] = t1;
+  }
+  for (int lv = 0; lv < loop.size; ++lv) loop.verts[lv] = tmp[lv][0];
+}
+
+template <int M>
+__device__ double tet_quality(LO const* v, Real const* coords, Real const* metric) {
+  constexpr int nc = (M == 1) ? 1 : 6;
+  double p[4][3];
+  double m[4][nc];
+  double const* pp[4];
+  double const* ms[4];
+  for (int k = 0; k < 4; ++k) {
+    for (int j = 0; j < 3; ++j) p[k][j] = coords[v[k] * 3 + j];
+    for (int j = 0; j < nc; ++j) m[k][j] = metric[v[k] * nc + j];
+    pp[k] = p[k];
+    ms[k] = m[k];
+  }
+  return element_quality<3, M>(pp, ms);
+}
+
+template <int S, int M>
+__device__ double length_between(LO a, LO b, Real const* coords, Real const* metric) {
+  constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
+  double p0[S], p1[S], m0[nc], m1[nc];
+  for (int k = 0; k < S; ++k) {
+    p0[k] = coords[a * S + k];
+    p1[k] = coords[b * S + k];
+  }
+  for (int k = 0; k < nc; ++k) {
+    m0[k] = metric[a * nc + k];
+    m1[k] = metric[b * nc + k];
+  }
+  return edge_length<S, M>(p0, p1, m0, m1);
+}
+
+template <int M>
+__global__ void __launch_bounds__(kBlock) swap3d_qualities_kernel(LO ncands, LO const* cands,
+    LO const* e2t_a2ab, LO const* e2t_ab2b, I8 const* e2t_codes, LO const* ev2v,
+    LO const* tv2v, Real const* coords, Real const* metric, Real const* elem_quals,
+    Real max_length, Tables const* tb, I8* configs, Real* quals, I8* keep) {
+  LO cand = thread_index();
+  if (cand >= ncands) return;
+  LO edge = cands[cand];
+  Loop loop;
+  find_loop(edge, e2t_a2ab, e2t_ab2b, e2t_codes, ev2v, tv2v, loop);
+  int choice_mesh = -1;
+  double choice_quality = -1.0;
+  if (loop.size <= swap::kMaxLoop) {
+    choice_quality = 0.0;
+    int n = loop.size;
+    int nmeshes = tb->mesh_counts[n];
+    int nmesh_tris = tb->mesh_sizes[n];
+    bool tri_cached[swap::kMaxUniqueTris];
+    double tri_qual[swap::kMaxUniqueTris];
+    bool edge_cached[swap::kMaxUniqueEdges];
+    double edge_len[swap::kMaxUniqueEdges];
+    for (int i = 0; i < swap::kMaxUniqueTris; ++i) tri_cached[i] = false;
+    for (int i = 0; i < swap::kMaxUniqueEdges; ++i) edge_cached[i] = false;
+    for (int mesh = 0; mesh < nmeshes; ++mesh) {
+      double mesh_minqual = 1.0;
+      for (int mt = 0; mt < nmesh_tris; ++mt) {
+        int ut = tb->meshes[n][mesh * nmesh_tris + mt];
+        if (!tri_cached[ut]) {
+          LO tv[4];
+          for (int k = 0; k < 3; ++k) tv[k] = loop.verts[tb->tris[n][ut][k]];
+          double tri_minqual = 1.0;
+          for (int tri_tet = 0; tri_tet < 2; ++tri_tet) {
+            tv[3] = loop.eev2v[1 - tri_tet];
+            tri_minqual = math::min2(tri_minqual, tet_quality<M>(tv, coords, metric));
+            LO t = tv[1];
+            tv[1] = tv[2];
+            tv[2] = t;
+          }
+          tri_cached[ut] = true;
+          tri_qual[ut] = tri_minqual;
+        }
+        mesh_minqual = math::min2(mesh_minqual, tri_qual[ut]);
+        if (mesh_minqual < 0.0) break;
+      }
+      if (mesh_minqual > choice_quality) {
+        bool overshoot = false;
+        for (int me = 0; me < tb->nedges[n]; ++me) {
+          int ue = tb->medges[n][mesh * tb->nedges[n] + me];
+          if (!edge_cached[ue]) {
+            edge_len[ue] = length_between<3, M>(loop.verts[tb->uedges[n][ue][0]],
+                loop.verts[tb->uedges[n][ue][1]], coords, metric);
+            edge_cached[ue] = true;
+          }
+          if (edge_len[ue] > max_length) {
+            overshoot = true;
+            break;
+          }
+        }
+        if (!overshoot) {
+          choice_mesh = mesh;
+          choice_quality = mesh_minqual;
+        }
+      }
+    }
+  }
+  configs[edge] = I8(choice_mesh);
+  double old = 1.79769313486231570815e+308;
+  for (LO j = e2t_a2ab[edge]; j < e2t_a2ab[edge + 1]; ++j)
+    old = math::min2(old, elem_quals[e2t_ab2b[j]]);
+  // a key needs a triangulation; with inverted elements the reference can
+  // keep an edge without one and then index its tables out of bounds
+  if (choice_mesh >= 0 && choice_quality > old) {
+    keep[edge] = 1;
+    quals[edge] = choice_quality;
+  }
+}
+
+__global__ void and_interior(LO ne, int dim, I8 const* edim, I8* marks) {
+  LO e = thread_index();
+  if (e < ne) marks[e] = I8(marks[e] && (edim[e] == dim));
+}
+
+__global__ void swap3d_counts(LO nkeys, LO const* keys2edges, LO const* e2t_a2ab,
+    Tables const* tb, LO* ne, LO* nf, LO* nt) {
+  LO k = thread_index();
+  if (k >= nkeys) return;
+  LO e = keys2edges[k];
+  int ls = e2t_a2ab[e + 1] - e2t_a2ab[e];
+  int npt = tb->mesh_sizes[ls], npe = tb->nedges[ls];
+  ne[k] = npe;
+  nf[k] = npt + 2 * npe;
+  nt[k] = 2 * npt;
+}
+
+__global__ void swap3d_products(LO nkeys, LO const* keys2edges, I8 const* configs,
+    LO const* e2t_a2ab, LO const* e2t_ab2b, I8 const* e2t_codes, LO const* ev2v, LO const* tv2v,
+    Tables const* tb, LO const* k2e, LO const* k2f, LO const* k2t, LO* pe, LO* pf, LO* pt) {
+  LO k = thread_index();
+  if (k >= nkeys) return;
+  LO edge = keys2edges[k];
+  int config = configs[edge];
+  Loop loop;
+  find_loop(edge, e2t_a2ab, e2t_ab2b, e2t_codes, ev2v, tv2v, loop);
+  int n = loop.size;
+  int npt = tb->mesh_sizes[n], npe = tb->nedges[n];
+  for (int pe_i = 0; pe_i < npe; ++pe_i) {
+    int ue = tb->medges[n][config * npe + pe_i];
+    LO a = loop.verts[tb->uedges[n][ue][0]];
+    LO b = loop.verts[tb->uedges[n][ue][1]];
+    LO pedge = k2e[k] + pe_i;
+    pe[pedge * 2] = a;
+    pe[pedge * 2 + 1] = b;
+    for (int eev = 0; eev < 2; ++eev) {
+      LO ptri = k2f[k] + 2 * pe_i + eev;
+      pf[ptri * 3 + 0] = a;
+      pf[ptri * 3 + 1] = b;
+      pf[ptri * 3 + 2] = loop.eev2v[eev];
+    }
+  }
+  for (int pt_i = 0; pt_i < npt; ++pt_i) {
+    int ut = tb->meshes[n][config * npt + pt_i];
+    LO tv[3];
+    for (int j = 0; j < 3; ++j) tv[j] = loop.verts[tb->tris[n][ut][j]];
+    LO ptri = k2f[k] + 2 * npe + pt_i;
+    for (int j = 0; j < 3; ++j) pf[ptri * 3 + j] = tv[j];
+    for (int eev = 0; eev < 2; ++eev) {
+      LO ptet = k2t[k] + 2 * pt_i + eev;
+      LO nt4[4] = {tv[0], tv[1], tv[2], loop.eev2v[eev]};
+      if (eev == 0) {
+        LO t = nt4[1];
+        nt4[1] = nt4[2];
+        nt4[2] = t;
+      }
+      for (int j = 0; j < 4; ++j) pt[ptet * 4 + j] = nt4[j];
+    }
+  }
+}
+
+__global__ void key_edge_sources(LO nkeys, LO const* keys2edges, LO const* k2p, LO* src) {
+  LO k = thread_index();
+  if (k >= nkeys) return;
+  for (LO p = k2p[k]; p < k2p[k + 1]; ++p) src[p] = keys2edges[k];
+}
+
+__global__ void loops_kernel(LO n, LO const* edges, LO const* a2ab, LO const* ab2b,
+    I8 const* codes, LO const* ev2v, LO const* tv2v, LO* eev2v, LO* verts, LO* sizes) {
+  LO i = thread_index();
+  if (i >= n) return;
+  Loop loop;
+  find_loop(edges[i], a2ab, ab2b, codes, ev2v, tv2v, loop);
+  sizes[i] = loop.size;
+  if (loop.size > swap::kMaxLoop) return;
+  eev2v[i * 2] = loop.eev2v[0];
+  eev2v[i * 2 + 1] = loop.eev2v[1];
+  for (int k = 0; k < loop.size; ++k) verts[i * swap::kMaxLoop + k] = loop.verts[k];
+}
+
+// 2D: the quadrilateral around an interior edge and its flipped diagonal
+__device__ void opposite_verts(LO e, LO const* e2t_a2ab, LO const* e2t_ab2b,
+    I8 const* e2t_codes, LO const* tv2v, LO* ov) {
+  for (int i = 0; i < 2; ++i) {
+    LO et = e2t_a2ab[e] + i;
+    int tte = code_which_down(e2t_codes[et]);
+    int rot = code_rotation(e2t_codes[et]);
+    LO t = e2t_ab2b[et];
+    ov[rot] = tv2v[t * 3 + opposite_template(2, 1, tte)];
+  }
+}
+
+template <int M>
+__global__ void swap2d_qualities_kernel(LO ncands, LO const* cands, LO const* e2t_a2ab,
+    LO const* e2t_ab2b, I8 const* e2t_codes, LO const* ev2v, LO const* tv2v,
+    Real const* coords, Real const* metric, Real const* elem_quals, Real max_length,
+    I8* keep, Real* quals) {
+  LO cand = thread_index();
+  if (cand >= ncands) return;
+  constexpr int nc = (M == 1) ? 1 : 3;
+  LO e = cands[cand];
+  LO ov[2];
+  opposite_verts(e, e2t_a2ab, e2t_ab2b, e2t_codes, tv2v, ov);
+  double q;
+  if (length_between<2, M>(ov[0], ov[1], coords, metric) > max_length) {
+    q = -1.0;
+  } else {
+    double minqual = 1.0;
+    for (int i = 0; i < 2; ++i) {
+      LO ntv[3] = {ev2v[e * 2 + (1 - i)], ov[i], ov[1 - i]};
+      double p[3][2];
+      double m[3][nc];
+      double const* pp[3];
+      double const* ms[3];
+      for (int k = 0; k < 3; ++k) {
+        for (int j = 0; j < 2; ++j) p[k][j] = coords[ntv[k] * 2 + j];
+        for (int j = 0; j < nc; ++j) m[k][j] = metric[ntv[k] * nc + j];
+        pp[k] = p[k];
+        ms[k] = m[k];
+      }
+      minqual = math::min2(minqual, element_quality<2, M>(pp, ms));
+    }
+    q = minqual;
+  }
+  double old = 1.79769313486231570815e+308;
+  for (LO j = e2t_a2ab[e]; j < e2t_a2ab[e + 1]; ++j) old = math::min2(old, elem_quals[e2t_ab2b[j]]);
+  if (q > old) {
+    keep[e] = 1;
+    quals[e] = q;
+  }
+}
+
+__global__ void swap2d_products(LO nkeys, LO const* keys2edges, LO const* e2t_a2ab,
+    LO const* e2t_ab2b, I8 const* e2t_codes, LO const* ev2v, LO const* tv2v, LO* pe, LO* pf) {
+  LO k = thread_index();
+  if (k >= nkeys) return;
+  LO e = keys2edges[k];
+  LO ov[2];
+  opposite_verts(e, e2t_a2ab, e2t_ab2b, e2t_codes, tv2v, ov);
+  pe[k * 2] = ov[0];
+  pe[k * 2 + 1] = ov[1];
+  for (int i = 0; i < 2; ++i) {
+    LO* ntv = pf + (k * 2 + i) * 3;
+    ntv[0] = ev2v[e * 2 + (1 - i)];
+    ntv[1] = ov[i];
+    ntv[2] = ov[1 - i];
+  }
+}
+
+DArray<unsigned char> upload_tables(Ctx& c) {
+  Tables t = swap::make_tables();
+  DArray<unsigned char> d(c, LO(sizeof(Tables)));
+  upload(c, d, reinterpret_cast<unsigned char const*>(&t), LO(sizeof(Tables)));
+  c.sync();
+  return d;
+}
+
+void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LOs const* pv) {
+  int dim = m.dim;
+  LO nv = m.nents[0];
+  DMesh nw;
+  nw.dim = dim;
+  nw.metric_ncomps = m.metric_ncomps;
+  nw.nents[0] = nv;
+  nw.coords = m.coords;
+  nw.metric = m.metric;
+  nw.class_id[0] = m.class_id[0];
+  nw.class_dim[0] = m.class_dim[0];
+  LOs old_lows2new_lows = iota(c, nv);
+  LO nkeys = keys2edges.size();
+  for (int d = 1; d <= dim; ++d) {
+    ModOut mo;
+    modify_ents_adapt(c, m, nw, d, 1, keys2edges, k2p[d], pv[d], old_lows2new_lows, mo);
+    LOs src(c, mo.prods2new.size());
+    OSH_LAUNCH(key_edge_sources, nkeys, c, keys2edges.data(), k2p[d].data(), src.data());
+    transfer_class(c, m, nw, d, mo, src, 1, nullptr);
+    transfer_measures(c, m, nw, d, mo);
+    old_lows2new_lows = mo.old2new;
+  }
+  m = nw;
+}
+
+Bytes swap_candidates(Ctx& c, DMesh& m, AdaptOptions const& o) {
+  int dim = m.dim;
+  Bytes elems = mark_sliver_layers(c, m, o.min_quality_desired, o.nsliver_layers);
+  Bytes edges = mark_down(c, m, dim, 1, elems);
+  OSH_LAUNCH(and_interior, m.nents[1], c, dim, m.class_dim[1].data(), edges.data());
+  return edges;
+}
+
+bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
+  ask_lengths(c, m);
+  Bytes cand = swap_candidates(c, m, o);
+  if (max_i8(c, cand) <= 0) return false;
+  LOs cands = collect_marked(c, cand);
+  LO ne = m.nents[1];
+  int md = (m.metric_ncomps == 1) ? 1 : 3;
+  auto tables = upload_tables(c);
+  Tables const* tb = reinterpret_cast<Tables const*>(tables.data());
+  Bytes configs = fill_i8(c, ne, -1);
+  Bytes keep = fill_i8(c, ne, 0);
+  Reals quals = fill_real(c, ne, -1.0);
+  Adj const& e2t = ask_up(c, m, 1, 3);
+  LOs const& tv2v = verts_of(c, m, 3);
+  Reals const& eq = ask_qualities(c, m);
+  if (md == 3)
+    OSH_LAUNCH(swap3d_qualities_kernel<3>, cands.size(), c, cands.data(), e2t.a2ab.data(),
+        e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), m.coords.data(),
+        m.metric.data(), eq.data(), o.max_length_allowed, tb, configs.data(), quals.data(),
+        keep.data());
+  else
+    OSH_LAUNCH(swap3d_qualities_kernel<1>, cands.size(), c, cands.data(), e2t.a2ab.data(),
+        e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), m.coords.data(),
+        m.metric.data(), eq.data(), o.max_length_allowed, tb, configs.data(), quals.data(),
+        keep.data());
+  if (max_i8(c, keep) <= 0) return false;
+  Bytes keys = find_indset(c, m, 1, quals, keep);
+  LOs keys2edges = collect_marked(c, keys);
+  LO nkeys = keys2edges.size();
+  LOs ce(c, nkeys), cf(c, nkeys), ct(c, nkeys);
+  OSH_LAUNCH(swap3d_counts, nkeys, c, keys2edges.data(), e2t.a2ab.data(), tb, ce.data(),
+      cf.data(), ct.data());
+  LOs k2p[4], pv[4];
+  k2p[1] = offset_scan(c, ce);
+  k2p[2] = offset_scan(c, cf);
+  k2p[3] = offset_scan(c, ct);
+  for (int d = 1; d <= 3; ++d) pv[d] = LOs(c, last(c, k2p[d]) * (d + 1));
+  OSH_LAUNCH(swap3d_products, nkeys, c, keys2edges.data(), configs.data(), e2t.a2ab.data(),
+      e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), tb, k2p[1].data(),
+      k2p[2].data(), k2p[3].data(), pv[1].data(), pv[2].data(), pv[3].data());
+  swap_rebuild(c, m, keys2edges, k2p, pv);
+  return true;
+}
+
+bool swap_edges_2d(Ctx& c, DMesh& m, AdaptOptions const& o) {
+  ask_lengths(c, m);
+  Bytes cand = swap_candidates(c, m, o);
+  if (max_i8(c, cand) <= 0) return false;
+  LOs cands = collect_marked(c, cand);
+  LO ne = m.nents[1];
+  int md = (m.metric_ncomps == 1) ? 1 : 2;
+  Bytes keep = fill_i8(c, ne, 0);
+  Reals quals = fill_real(c, ne, -1.0);
+  Adj const& e2t = ask_up(c, m, 1, 2);
+  LOs const& tv2v = verts_of(c, m, 2);
+  Reals const& eq = ask_qualities(c, m);
+  if (md == 2)
+    OSH_LAUNCH(swap2d_qualities_kernel<2>, cands.size(), c, cands.data(), e2t.a2ab.data(),
+        e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), m.coords.data(),
+        m.metric.data(), eq.data(), o.max_length_allowed, keep.data(), quals.data());
+  else
+    OSH_LAUNCH(swap2d_qualities_kernel<1>, cands.size(), c, cands.data(), e2t.a2ab.data(),
+        e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), m.coords.data(),
+        m.metric.data(), eq.data(), o.max_length_allowed, keep.data(), quals.data());
+  if (max_i8(c, keep) <= 0) return false;
+  Bytes keys = find_indset(c, m, 1, quals, keep);
+  LOs keys2edges = collect_marked(c, keys);
+  LO nkeys = keys2edges.size();
+  LOs k2p[4], pv[4];
+  k2p[1] = iota(c, nkeys + 1);
+  k2p[2] = iota(c, nkeys + 1, 0, 2);
+  pv[1] = LOs(c, nkeys * 2);
+  pv[2] = LOs(c, nkeys * 6);
+  OSH_LAUNCH(swap2d_products, nkeys, c, keys2edges.data(), e2t.a2ab.data(), e2t.ab2b.data(),
+      e2t.codes.data(), m.down[1].data(), tv2v.data(), pv[1].data(), pv[2].data());
+  swap_rebuild(c, m, keys2edges, k2p, pv);
+  return true;
+}
+
+}  // namespace
+
+bool swap_edges(Ctx& c, DMesh& m, AdaptOptions const& o) {
+  if (m.dim == 3) return swap_edges_3d(c, m, o);
+  return swap_edges_2d(c, m, o);
+}
+
+void find_loops(Ctx& c, DMesh& m, LOs const& edges, LOs& eev2v, LOs& loop_verts, LOs& sizes) {
+  LO n = edges.size();
+  eev2v = fill_lo(c, n * 2, -1);
+  loop_verts = fill_lo(c, n * swap::kMaxLoop, -1);
+  sizes = LOs(c, n);
+  Adj const& e2t = ask_up(c, m, 1, 3);
+  LOs const& tv2v = verts_of(c, m, 3);
+  OSH_LAUNCH(loops_kernel, n, c, edges.data(), e2t.a2ab.data(), e2t.ab2b.data(),
+      e2t.codes.data(), m.down[1].data(), tv2v.data(), eev2v.data(), loop_verts.data(),
+      sizes.data());
+}
+
+}  // namespace osh

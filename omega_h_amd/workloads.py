@@ -1,0 +1,69 @@
+"""Synthetic adapt() workloads: box meshes and their target metric fields.
+
+Each metric is evaluated at vertex coordinates in float64 and stored in
+Omega_h's symmetric layout (src/Omega_h_matrix.hpp:391-431):
+1 component (isotropic 1/h^2), 3 components (xx, yy, xy) in 2D or
+6 components (xx, yy, zz, xy, yz, xz) in 3D.
+"""
+import math
+
+import numpy as np
+
+
+def iso_ring_metric_2d(coords):
+    """2D ring: h = 0.02 + 0.08 |r - 0.25|, r the distance to (0.5, 0.5)."""
+    p = coords.reshape(-1, 2)
+    r = np.sqrt((p[:, 0] - 0.5) ** 2 + (p[:, 1] - 0.5) ** 2)
+    h = 0.02 + 0.08 * np.abs(r - 0.25)
+    return 1.0 / (h * h), 1
+
+
+def levelset_metric_3d(coords, h_min=0.002, h_max=0.05, width=0.02,
+                       amplitude=0.1):
+    """3D anisotropic metric around the level set phi = z - 0.5 - A sin(2 pi x).
+
+    The normal eigen-length is h_n = h_min + (h_max - h_min)(1 - exp(-|phi|/w)),
+    the two tangential eigen-lengths are h_max, and the eigenvectors follow
+    grad(phi).  With two equal tangential eigenvalues the tensor is
+    M = l_t I + (l_n - l_t) n n^T for l = 1/h^2.
+    """
+    p = coords.reshape(-1, 3)
+    x, z = p[:, 0], p[:, 2]
+    phi = z - 0.5 - amplitude * np.sin(2.0 * math.pi * x)
+    g = np.stack([-2.0 * math.pi * amplitude * np.cos(2.0 * math.pi * x),
+                  np.zeros_like(x), np.ones_like(x)], axis=1)
+    n = g / np.linalg.norm(g, axis=1, keepdims=True)
+    hn = h_min + (h_max - h_min) * (1.0 - np.exp(-np.abs(phi) / width))
+    ln = 1.0 / (hn * hn)
+    lt = 1.0 / (h_max * h_max)
+    d = (ln - lt)
+    m = np.empty((len(p), 6))
+    m[:, 0] = lt + d * n[:, 0] * n[:, 0]
+    m[:, 1] = lt + d * n[:, 1] * n[:, 1]
+    m[:, 2] = lt + d * n[:, 2] * n[:, 2]
+    m[:, 3] = d * n[:, 0] * n[:, 1]
+    m[:, 4] = d * n[:, 1] * n[:, 2]
+    m[:, 5] = d * n[:, 0] * n[:, 2]
+    return m.ravel(), 6
+
+
+def rotated_constant_metric(nverts, dim, rng):
+    """Constant anisotropic metric: random rotation, eigen-lengths log-uniform
+    in [0.005, 0.1]."""
+    if dim == 3:
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        w, x, y, z = q
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    else:
+        a = rng.uniform(0, 2 * math.pi)
+        R = np.array([[math.cos(a), -math.sin(a)], [math.sin(a), math.cos(a)]])
+    h = np.exp(rng.uniform(math.log(0.005), math.log(0.1), dim))
+    M = R @ np.diag(1.0 / (h * h)) @ R.T
+    if dim == 3:
+        v = np.array([M[0, 0], M[1, 1], M[2, 2], M[0, 1], M[1, 2], M[0, 2]])
+    else:
+        v = np.array([M[0, 0], M[1, 1], M[0, 1]])
+    return np.tile(v, nverts), len(v)

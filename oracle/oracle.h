@@ -1,0 +1,95 @@
+/* Oracle (test infrastructure only): C API of the CPU restatement of
+ * Omega_h's adapt() hot path, loaded by tests/ through ctypes.  The product
+ * (omega_h_amd) never includes, links or calls this. */
+#ifndef OMEGA_H_ORACLE_H
+#define OMEGA_H_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct om_mesh om_mesh;
+
+/* mirrors Omega_h::AdaptOpts (src/Omega_h_adapt.hpp:50-77) */
+typedef struct {
+  double min_length_desired;
+  double max_length_desired;
+  double max_length_allowed;
+  double min_quality_allowed;
+  double min_quality_desired;
+  int nsliver_layers;
+  int verbosity;
+  int should_refine;
+  int should_coarsen;
+  int should_swap;
+  int should_coarsen_slivers;
+} om_adapt_opts;
+
+#define OM_MAX_PASSES 4096
+typedef struct {
+  int npasses;
+  int kind[OM_MAX_PASSES]; /* 0 refine, 1 coarsen, 2 swap, 3 coarsen slivers */
+  int nelems_in[OM_MAX_PASSES];
+  int nelems_out[OM_MAX_PASSES];
+  double seconds[OM_MAX_PASSES];
+} om_adapt_stats;
+
+void om_default_opts(int dim, om_adapt_opts* o);
+om_mesh* om_build_box(int dim, double x, double y, double z, int nx, int ny,
+    int nz);
+om_mesh* om_build_box_internal(int dim, double x, double y, double z, int nx,
+    int ny, int nz);
+om_mesh* om_build_from_elems(int dim, int nelems, int const* ev2v, int nverts);
+void om_mesh_free(om_mesh* m);
+int om_mesh_dim(om_mesh* m);
+int om_mesh_nents(om_mesh* m, int d);
+void om_get_verts(om_mesh* m, int d, int* out);
+void om_get_down(om_mesh* m, int d, int* out, signed char* codes);
+void om_get_coords(om_mesh* m, double* out);
+void om_set_metric(om_mesh* m, int ncomps, double const* metric);
+int om_metric_ncomps(om_mesh* m);
+void om_get_metric(om_mesh* m, double* out);
+void om_get_class(om_mesh* m, int d, int* ids, signed char* dims);
+void om_get_lengths(om_mesh* m, double* out);
+void om_get_qualities(om_mesh* m, double* out);
+int om_up_size(om_mesh* m, int low, int high);
+void om_get_up(om_mesh* m, int low, int high, int* a2ab, int* ab2b,
+    signed char* codes);
+int om_star_size(om_mesh* m, int d);
+void om_get_star(om_mesh* m, int d, int* a2ab, int* ab2b);
+
+int om_refine_by_size(om_mesh* m, om_adapt_opts const* o);
+int om_coarsen_by_size(om_mesh* m, om_adapt_opts const* o);
+int om_swap_edges(om_mesh* m, om_adapt_opts const* o);
+int om_coarsen_slivers(om_mesh* m, om_adapt_opts const* o);
+int om_adapt(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* stats);
+
+/* known-answer hooks for the reference's unit tests */
+double om_log(double x);
+double om_exp(double x);
+double om_cbrt(double x);
+double om_acos(double x);
+double om_cos(double x);
+void om_kat_hilbert(uint64_t* X, int b, int n, uint64_t* Y);
+int om_kat_roots3(double const* coeffs, double eps, double* roots, int* mults);
+void om_kat_decompose_eigen(int dim, double const* m, double* q, double* l);
+void om_kat_log_spd(int dim, double const* m, double* out);
+void om_kat_exp_spd(int dim, double const* m, double* out);
+double om_kat_quality(int dim, int mdim, double const* pts, double const* ms);
+double om_kat_edge_length(int sdim, int mdim, double const* p0,
+    double const* p1, double const* m0, double const* m1);
+double om_kat_aniso_length(double la, double lb);
+void om_kat_average_metric(int mdim, int n, double const* ms, double* out);
+void om_kat_intersect(int dim, double const* m1, double const* m2, double* out);
+void om_kat_refine_qualities(om_mesh* m, int n, int const* cands, double* out);
+int om_kat_find_loop(om_mesh* m, int edge, int* eev2v, int* loop_verts);
+void om_kat_indset(om_mesh* m, int d, double const* q, signed char const* c,
+    signed char* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

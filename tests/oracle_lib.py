@@ -1,0 +1,320 @@
+"""ctypes binding of the oracle (oracle/liboracle.so).
+
+The oracle is the CPU restatement of Omega_h's adapt() path that the tests
+use as the checker.  Only tests/, the smoke check and bench.py's CPU-baseline
+leg load it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_lib = None
+
+
+class AdaptOpts(C.Structure):
+    _fields_ = [
+        ("min_length_desired", C.c_double),
+        ("max_length_desired", C.c_double),
+        ("max_length_allowed", C.c_double),
+        ("min_quality_allowed", C.c_double),
+        ("min_quality_desired", C.c_double),
+        ("nsliver_layers", C.c_int),
+        ("verbosity", C.c_int),
+        ("should_refine", C.c_int),
+        ("should_coarsen", C.c_int),
+        ("should_swap", C.c_int),
+        ("should_coarsen_slivers", C.c_int),
+    ]
+
+
+MAX_PASSES = 4096
+
+
+class AdaptStats(C.Structure):
+    _fields_ = [
+        ("npasses", C.c_int),
+        ("kind", C.c_int * MAX_PASSES),
+        ("nelems_in", C.c_int * MAX_PASSES),
+        ("nelems_out", C.c_int * MAX_PASSES),
+        ("seconds", C.c_double * MAX_PASSES),
+    ]
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
+        L = C.CDLL(ORACLE_SO)
+        P = C.c_void_p
+        L.om_build_box.restype = P
+        L.om_build_box.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
+                                   C.c_int, C.c_int, C.c_int]
+        L.om_build_box_internal.restype = P
+        L.om_build_box_internal.argtypes = L.om_build_box.argtypes
+        L.om_build_from_elems.restype = P
+        L.om_build_from_elems.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]
+        L.om_mesh_free.argtypes = [P]
+        L.om_mesh_dim.argtypes = [P]
+        L.om_mesh_nents.argtypes = [P, C.c_int]
+        L.om_get_verts.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
+        L.om_get_down.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_byte)]
+        L.om_get_coords.argtypes = [P, C.POINTER(C.c_double)]
+        L.om_set_metric.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
+        L.om_metric_ncomps.argtypes = [P]
+        L.om_get_metric.argtypes = [P, C.POINTER(C.c_double)]
+        L.om_get_class.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_byte)]
+        L.om_get_lengths.argtypes = [P, C.POINTER(C.c_double)]
+        L.om_get_qualities.argtypes = [P, C.POINTER(C.c_double)]
+        L.om_up_size.argtypes = [P, C.c_int, C.c_int]
+        L.om_get_up.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                C.POINTER(C.c_int), C.POINTER(C.c_byte)]
+        L.om_star_size.argtypes = [P, C.c_int]
+        L.om_get_star.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        for name in ("om_refine_by_size", "om_coarsen_by_size", "om_swap_edges",
+                     "om_coarsen_slivers"):
+            getattr(L, name).argtypes = [P, C.POINTER(AdaptOpts)]
+        L.om_adapt.argtypes = [P, C.POINTER(AdaptOpts), C.POINTER(AdaptStats)]
+        L.om_default_opts.argtypes = [C.c_int, C.POINTER(AdaptOpts)]
+        for name in ("om_log", "om_exp", "om_cbrt", "om_acos", "om_cos"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [C.c_double]
+        L.om_kat_hilbert.argtypes = [C.POINTER(C.c_uint64), C.c_int, C.c_int,
+                                     C.POINTER(C.c_uint64)]
+        L.om_kat_roots3.argtypes = [C.POINTER(C.c_double), C.c_double,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.om_kat_decompose_eigen.argtypes = [C.c_int, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.om_kat_log_spd.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.om_kat_exp_spd.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.om_kat_quality.restype = C.c_double
+        L.om_kat_quality.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
+        L.om_kat_edge_length.restype = C.c_double
+        L.om_kat_edge_length.argtypes = [C.c_int, C.c_int] + [C.POINTER(C.c_double)] * 4
+        L.om_kat_aniso_length.restype = C.c_double
+        L.om_kat_aniso_length.argtypes = [C.c_double, C.c_double]
+        L.om_kat_average_metric.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double)]
+        L.om_kat_intersect.argtypes = [C.c_int] + [C.POINTER(C.c_double)] * 3
+        L.om_kat_refine_qualities.argtypes = [P, C.c_int, C.POINTER(C.c_int),
+                                              C.POINTER(C.c_double)]
+        L.om_kat_find_loop.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.om_kat_indset.argtypes = [P, C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_byte), C.POINTER(C.c_byte)]
+        _lib = L
+    return _lib
+
+
+def default_opts(dim):
+    o = AdaptOpts()
+    lib().om_default_opts(dim, C.byref(o))
+    return o
+
+
+class Mesh:
+    """Owning handle on an oracle mesh."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().om_mesh_free(self.h)
+            self.h = None
+
+    @classmethod
+    def box(cls, dim, n, internal=False, x=1.0, y=1.0, z=1.0):
+        nz = n if dim == 3 else 0
+        z = z if dim == 3 else 0.0
+        f = lib().om_build_box_internal if internal else lib().om_build_box
+        return cls(f(dim, x, y, z, n, n, nz))
+
+    @classmethod
+    def from_elems(cls, dim, ev2v, nverts):
+        a = np.ascontiguousarray(ev2v, dtype=np.int32)
+        return cls(lib().om_build_from_elems(dim, len(a) // (dim + 1), _ptr(a, C.c_int), nverts))
+
+    @property
+    def dim(self):
+        return lib().om_mesh_dim(self.h)
+
+    def nents(self, d):
+        return lib().om_mesh_nents(self.h, d)
+
+    def verts(self, d):
+        out = np.zeros(self.nents(d) * (d + 1), dtype=np.int32)
+        lib().om_get_verts(self.h, d, _ptr(out, C.c_int))
+        return out
+
+    def down(self, d):
+        n = self.nents(d) * (d + 1)
+        out = np.zeros(n, dtype=np.int32)
+        codes = np.zeros(n, dtype=np.int8)
+        lib().om_get_down(self.h, d, _ptr(out, C.c_int), _ptr(codes, C.c_byte))
+        return out, codes
+
+    def coords(self):
+        out = np.zeros(self.nents(0) * self.dim, dtype=np.float64)
+        lib().om_get_coords(self.h, _ptr(out, C.c_double))
+        return out
+
+    def set_metric(self, metric, ncomps):
+        a = np.ascontiguousarray(metric, dtype=np.float64)
+        lib().om_set_metric(self.h, ncomps, _ptr(a, C.c_double))
+
+    def metric(self):
+        nc = lib().om_metric_ncomps(self.h)
+        out = np.zeros(self.nents(0) * nc, dtype=np.float64)
+        lib().om_get_metric(self.h, _ptr(out, C.c_double))
+        return out
+
+    def classification(self, d):
+        ids = np.zeros(self.nents(d), dtype=np.int32)
+        dims = np.zeros(self.nents(d), dtype=np.int8)
+        lib().om_get_class(self.h, d, _ptr(ids, C.c_int), _ptr(dims, C.c_byte))
+        return ids, dims
+
+    def lengths(self):
+        out = np.zeros(self.nents(1), dtype=np.float64)
+        lib().om_get_lengths(self.h, _ptr(out, C.c_double))
+        return out
+
+    def qualities(self):
+        out = np.zeros(self.nents(self.dim), dtype=np.float64)
+        lib().om_get_qualities(self.h, _ptr(out, C.c_double))
+        return out
+
+    def up(self, low, high):
+        n = lib().om_up_size(self.h, low, high)
+        a2ab = np.zeros(self.nents(low) + 1, dtype=np.int32)
+        ab2b = np.zeros(n, dtype=np.int32)
+        codes = np.zeros(n, dtype=np.int8)
+        lib().om_get_up(self.h, low, high, _ptr(a2ab, C.c_int), _ptr(ab2b, C.c_int),
+                        _ptr(codes, C.c_byte))
+        return a2ab, ab2b, codes
+
+    def star(self, d):
+        n = lib().om_star_size(self.h, d)
+        a2ab = np.zeros(self.nents(d) + 1, dtype=np.int32)
+        ab2b = np.zeros(n, dtype=np.int32)
+        lib().om_get_star(self.h, d, _ptr(a2ab, C.c_int), _ptr(ab2b, C.c_int))
+        return a2ab, ab2b
+
+    def refine_qualities(self, cands):
+        c = np.ascontiguousarray(cands, dtype=np.int32)
+        out = np.zeros(len(c), dtype=np.float64)
+        lib().om_kat_refine_qualities(self.h, len(c), _ptr(c, C.c_int), _ptr(out, C.c_double))
+        return out
+
+    def find_loop(self, edge):
+        eev = np.zeros(2, dtype=np.int32)
+        lv = np.zeros(7, dtype=np.int32)
+        n = lib().om_kat_find_loop(self.h, edge, _ptr(eev, C.c_int), _ptr(lv, C.c_int))
+        return eev, lv[:n]
+
+    def indset(self, d, quality, cands):
+        q = np.ascontiguousarray(quality, dtype=np.float64)
+        c = np.ascontiguousarray(cands, dtype=np.int8)
+        out = np.zeros(self.nents(d), dtype=np.int8)
+        lib().om_kat_indset(self.h, d, _ptr(q, C.c_double), _ptr(c, C.c_byte),
+                            _ptr(out, C.c_byte))
+        return out
+
+    def refine_by_size(self, opts):
+        return lib().om_refine_by_size(self.h, C.byref(opts))
+
+    def coarsen_by_size(self, opts):
+        return lib().om_coarsen_by_size(self.h, C.byref(opts))
+
+    def swap_edges(self, opts):
+        return lib().om_swap_edges(self.h, C.byref(opts))
+
+    def coarsen_slivers(self, opts):
+        return lib().om_coarsen_slivers(self.h, C.byref(opts))
+
+    def adapt(self, opts):
+        st = AdaptStats()
+        r = lib().om_adapt(self.h, C.byref(opts), C.byref(st))
+        passes = [(st.kind[i], st.nelems_in[i], st.nelems_out[i], st.seconds[i])
+                  for i in range(st.npasses)]
+        return r, passes
+
+
+def kat_quality(dim, mdim, pts, ms):
+    p = np.ascontiguousarray(pts, dtype=np.float64).ravel()
+    m = np.ascontiguousarray(ms, dtype=np.float64).ravel()
+    return lib().om_kat_quality(dim, mdim, _ptr(p, C.c_double), _ptr(m, C.c_double))
+
+
+def kat_decompose_eigen(m):
+    """m: dim x dim numpy array (row i, col j); returns (q as columns, l)."""
+    dim = m.shape[0]
+    a = np.ascontiguousarray(m.T.ravel(), dtype=np.float64)  # column-major
+    q = np.zeros(dim * dim)
+    l = np.zeros(dim)
+    lib().om_kat_decompose_eigen(dim, _ptr(a, C.c_double), _ptr(q, C.c_double),
+                                 _ptr(l, C.c_double))
+    return q.reshape(dim, dim).T, l
+
+
+def kat_spd(fn, m):
+    dim = m.shape[0]
+    a = np.ascontiguousarray(m.T.ravel(), dtype=np.float64)
+    out = np.zeros(dim * dim)
+    getattr(lib(), fn)(dim, _ptr(a, C.c_double), _ptr(out, C.c_double))
+    return out.reshape(dim, dim).T
+
+
+def kat_roots3(coeffs, eps=1e-6):
+    c = np.ascontiguousarray(coeffs, dtype=np.float64)
+    r = np.zeros(3)
+    mm = np.zeros(3, dtype=np.int32)
+    n = lib().om_kat_roots3(_ptr(c, C.c_double), eps, _ptr(r, C.c_double), _ptr(mm, C.c_int))
+    return n, r, mm
+
+
+def kat_average_metric(mdim, ms):
+    m = np.ascontiguousarray(ms, dtype=np.float64).ravel()
+    nc = mdim * (mdim + 1) // 2
+    out = np.zeros(nc)
+    lib().om_kat_average_metric(mdim, len(m) // nc, _ptr(m, C.c_double), _ptr(out, C.c_double))
+    return out
+
+
+def kat_intersect(dim, m1, m2):
+    a = np.ascontiguousarray(m1, dtype=np.float64)
+    b = np.ascontiguousarray(m2, dtype=np.float64)
+    out = np.zeros(dim * (dim + 1) // 2)
+    lib().om_kat_intersect(dim, _ptr(a, C.c_double), _ptr(b, C.c_double), _ptr(out, C.c_double))
+    return out
+
+
+def kat_hilbert(X, b, n):
+    x = np.ascontiguousarray(X, dtype=np.uint64)
+    y = np.zeros(n, dtype=np.uint64)
+    lib().om_kat_hilbert(_ptr(x, C.c_uint64), b, n, _ptr(y, C.c_uint64))
+    return x, y
+
+
+def symm2vector(m):
+    """Omega_h's symmetric storage order (src/Omega_h_matrix.hpp:391-431)."""
+    if m.shape[0] == 2:
+        return np.array([m[0, 0], m[1, 1], m[0, 1]])
+    return np.array([m[0, 0], m[1, 1], m[2, 2], m[0, 1], m[1, 2], m[0, 2]])
+
+
+def vector2symm(v):
+    if len(v) == 3:
+        return np.array([[v[0], v[2]], [v[2], v[1]]])
+    return np.array([[v[0], v[3], v[5]], [v[3], v[1], v[4]], [v[5], v[4], v[2]]])
