@@ -28,7 +28,10 @@ def assert_same(a, b, what):
     b = np.asarray(b)
     assert a.shape == b.shape, (what, a.shape, b.shape)
     if a.dtype.kind == "f":
-        bad = np.nonzero(a.view(np.int64) != b.view(np.int64))[0]
+        # bit-exact, except that any two NaNs match (their sign bit is the
+        # platform's default NaN, not a result)
+        diff = a.view(np.int64) != b.view(np.int64)
+        bad = np.nonzero(diff & ~(np.isnan(a) & np.isnan(b)))[0]
     else:
         bad = np.nonzero(a != b)[0]
     assert len(bad) == 0, (what, len(bad), bad[:5], a[bad[:5]], b[bad[:5]])
