@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""adapt() throughput on one MI355X.
+
+Workload: build_box unit cube 64^3 cells, 6 tets per cell (1,572,864 tets),
+anisotropic metric around the level set phi = z - 0.5 - 0.1 sin(2 pi x)
+(normal eigen-length 0.002 + 0.048 (1 - exp(-|phi|/0.02)), tangential 0.05),
+one full adapt() with default AdaptOpts, float64 throughout.
+
+A step is one adapt() of a fresh copy of that input mesh (the copies are made
+before the timed region; the mesh lives in HBM the whole time).  `value` is
+the adapt() throughput in the per-pass sense of the metric: the elements that
+entered every pass that rebuilt the mesh, divided by the summed wall time of
+those passes (each pass timed on the host clock with the stream drained at
+both ends), in millions per second.  `ms_per_step` is the wall time of a
+whole adapt() call, including the calls that changed nothing.
+
+The `roofline` object covers the element-quality kernel
+(measure_qualities, 3D, 6-component metric) on the input mesh, timed with HIP
+events on the library's own stream.  Algorithmic bytes per launch:
+24 B per tet (4 vertex indices + one FP64 quality) + 72 B per vertex
+(3 FP64 coordinates + 6 FP64 metric components, each read once).
+
+The `cpu_baseline` object times the CPU restatement of the same algorithm
+(single thread) on the first passes of the same workload, bounded to about
+`--cpu-seconds` seconds of pass time.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PASS_NAMES = ("refine_by_size", "coarsen_by_size", "swap_edges", "coarsen_slivers")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def quality_roofline(ctx, mesh, launches):
+    """HIP-event timing of measure_qualities over the whole mesh."""
+    import numpy as np
+    import torch
+    dim = mesh.dim
+    nelems = mesh.nents(dim)
+    nverts = mesh.nents(0)
+    ev2v = torch.from_numpy(np.ascontiguousarray(mesh.verts(dim), dtype=np.int32)).cuda()
+    coords = torch.from_numpy(mesh.coords()).cuda()
+    metric = torch.from_numpy(mesh.metric()).cuda()
+    ncomps = mesh.metric_ncomps
+    out = torch.empty(nelems, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+
+    def launch():
+        ctx.measure_qualities(dim, ncomps, nelems, ev2v.data_ptr(), coords.data_ptr(),
+                              metric.data_ptr(), out.data_ptr())
+
+    for _ in range(3):
+        launch()
+    ctx.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(launches):
+        launch()
+    t1.record(stream)
+    t1.synchronize()
+    ms = t0.elapsed_time(t1) / launches
+    want = mesh.qualities()
+    got = out.cpu().numpy()
+    if not np.array_equal(want.view(np.int64), got.view(np.int64)):
+        raise RuntimeError("measure_qualities on device pointers differs from Mesh.qualities")
+    nbytes = 24 * nelems + 72 * nverts
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"kernel": "elem_qualities_kernel<3,6>", "launch_ms": ms, "bytes": nbytes,
+            "achieved": achieved}
+
+
+def cpu_baseline(n, seconds):
+    """Oracle passes, in adapt() order, on the same workload until `seconds`
+    of pass time have been spent."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from omega_h_amd import workloads as W
+    m = O.Mesh.box(3, n)
+    met, nc = W.levelset_metric_3d(m.coords())
+    m.set_metric(met, nc)
+    opts = O.default_opts(3)
+    spent = 0.0
+    nin = 0
+    done = []
+    order = [0, 1] * 64
+    for kind in order:
+        n0 = m.nents(3)
+        t0 = time.perf_counter()
+        changed = getattr(m, PASS_NAMES[kind])(opts)
+        dt = time.perf_counter() - t0
+        if changed:
+            spent += dt
+            nin += n0
+            done.append(PASS_NAMES[kind])
+        if spent >= seconds:
+            break
+    return {"value": nin / spent / 1e6 if spent > 0 else None, "unit": "Mtets/s",
+            "cores": 1, "kind": "port",
+            "sample": f"first {len(done)} mesh-changing passes ({', '.join(done)}) of the "
+                      f"{n}^3 level-set adapt, {spent:.1f} s of pass time, one thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=64, help="cells per side of the box")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-launches", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(the adapt() path has no multi-GPU form)", file=sys.stderr)
+        sys.exit(2)
+
+    import omega_h_amd as A
+    from omega_h_amd import workloads as W
+
+    ctx = A.Context(0)
+    base = A.Mesh.box(ctx, 3, args.n)
+    met, nc = W.levelset_metric_3d(base.coords())
+    base.set_metric(met, nc)
+    opts = ctx.default_opts(3)
+    nelems0 = base.nents(3)
+    log(f"input: {args.n}^3 box, {nelems0} tets, {base.nents(0)} verts")
+
+    copies = [base.copy() for _ in range(args.warmup + args.steps)]
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        res, passes = copies[i].adapt(opts)
+        log(f"warmup {i}: {len(passes)} passes, {copies[i].nents(3)} tets, "
+            f"{time.perf_counter() - t0:.2f} s")
+
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    runs = []
+    for i in range(args.steps):
+        m = copies[args.warmup + i]
+        runs.append(m.adapt(opts)[1])
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+
+    nin = sum(p[1] for passes in runs for p in passes)
+    secs = sum(p[3] for passes in runs for p in passes)
+    value = nin / secs / 1e6
+    final = copies[-1].nents(3)
+    for kind in range(4):
+        ps = [p for p in runs[-1] if p[0] == kind]
+        if ps:
+            log(f"{PASS_NAMES[kind]}: {len(ps)} passes, {sum(p[1] for p in ps)} tets in, "
+                f"{sum(p[3] for p in ps):.3f} s")
+    log(f"step: {len(runs[-1])} passes, {nelems0} -> {final} tets, {wall / args.steps:.3f} s")
+
+    roof = quality_roofline(ctx, base, args.roofline_launches)
+    log(f"roofline: {roof}")
+    line = {
+        "metric": "Mtets adapted/s per pass (plain FP64 vs mixed) at 1.57M tets; "
+                  "p50/p99 serve latency",
+        "value": value,
+        "unit": "Mtets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"build_box unit cube {args.n}^3 x 6 tets ({nelems0} tets), "
+                               "level-set anisotropic metric, full adapt() FP64, default "
+                               "AdaptOpts",
+                   "passes_per_step": len(runs[-1]), "final_tets": final},
+        "roofline": {"bound": "hbm", "achieved": roof["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": roof["achieved"] / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": roof["kernel"],
+                     "launch_us": roof["launch_ms"] * 1e3, "bytes": roof["bytes"]},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        line["cpu_baseline"] = cpu_baseline(args.n, args.cpu_seconds)
+        log(f"cpu baseline: {time.perf_counter() - t0:.1f} s")
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

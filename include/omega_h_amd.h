@@ -52,6 +52,8 @@ const char* osh_last_error(void);
 int osh_ctx_create(int device, osh_ctx** out);
 int osh_ctx_destroy(osh_ctx* ctx);
 int osh_ctx_synchronize(osh_ctx* ctx);
+/* the hipStream_t the context orders its work on (for events and interop) */
+int osh_ctx_stream(osh_ctx* ctx, void** out);
 
 /* AdaptOpts::AdaptOpts(Int dim), src/Omega_h_adapt.cpp:52-85 */
 int osh_default_opts(int dim, osh_adapt_opts* out);

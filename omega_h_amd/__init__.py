@@ -72,6 +72,7 @@ def load(path=None):
         "osh_ctx_create": [I, C.POINTER(P)],
         "osh_ctx_destroy": [P],
         "osh_ctx_synchronize": [P],
+        "osh_ctx_stream": [P, C.POINTER(P)],
         "osh_default_opts": [I, C.POINTER(AdaptOpts)],
         "osh_build_box": [P, I, C.c_double, C.c_double, C.c_double, I, I, I, C.POINTER(P)],
         "osh_build_box_internal": [P, I, C.c_double, C.c_double, C.c_double, I, I, I,
@@ -131,6 +132,26 @@ class Context:
         h = C.c_void_p()
         check(self.lib.osh_ctx_create(device, C.byref(h)), self.lib)
         self.h = h
+
+    @property
+    def stream(self):
+        """The hipStream_t (as an int) that all work of this context runs on."""
+        s = C.c_void_p()
+        check(self.lib.osh_ctx_stream(self.h, C.byref(s)), self.lib)
+        return s.value or 0
+
+    def measure_qualities(self, dim, metric_ncomps, n, ev2v_ptr, coords_ptr, metric_ptr,
+                          out_ptr, elems_ptr=0):
+        """measure_qualities on device pointers (ints), on this context's stream."""
+        check(self.lib.osh_measure_qualities(self.h, dim, metric_ncomps, n, elems_ptr or None,
+                                             ev2v_ptr, coords_ptr, metric_ptr, out_ptr), self.lib)
+
+    def measure_edges_metric(self, dim, metric_ncomps, n, ev2v_ptr, coords_ptr, metric_ptr,
+                             out_ptr, edges_ptr=0):
+        """measure_edges_metric on device pointers (ints), on this context's stream."""
+        check(self.lib.osh_measure_edges_metric(self.h, dim, metric_ncomps, n, edges_ptr or None,
+                                                ev2v_ptr, coords_ptr, metric_ptr, out_ptr),
+              self.lib)
 
     def synchronize(self):
         check(self.lib.osh_ctx_synchronize(self.h), self.lib)
@@ -240,6 +261,10 @@ class Mesh:
     def set_metric(self, metric, ncomps):
         a = np.ascontiguousarray(metric, dtype=np.float64)
         check(self.lib.osh_mesh_set_metric(self.h, ncomps, _p(a, C.c_double)), self.lib)
+
+    @property
+    def metric_ncomps(self):
+        return self._int(self.lib.osh_mesh_metric_ncomps)
 
     def metric(self):
         nc = self._int(self.lib.osh_mesh_metric_ncomps)

@@ -101,6 +101,13 @@ int osh_ctx_synchronize(osh_ctx* ctx) {
   return guarded([&] { ctx->ctx.sync(); });
 }
 
+int osh_ctx_stream(osh_ctx* ctx, void** out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && out, "null argument");
+    *out = reinterpret_cast<void*>(ctx->ctx.stream);
+  });
+}
+
 int osh_default_opts(int dim, osh_adapt_opts* out) {
   return guarded([&] {
     OSH_CHECK(dim == 2 || dim == 3, "dimension must be 2 or 3");
