@@ -18,7 +18,9 @@ The `roofline` object covers the element-quality kernel
 (measure_qualities, 3D, 6-component metric) on the input mesh, timed with HIP
 events on the library's own stream.  Algorithmic bytes per launch:
 24 B per tet (4 vertex indices + one FP64 quality) + 72 B per vertex
-(3 FP64 coordinates + 6 FP64 metric components, each read once).
+(3 FP64 coordinates + 6 FP64 metric components, each read once).  The edge
+length kernel is timed the same way (16 B per edge + 72 B per vertex) and
+reported on stderr.
 
 The `cpu_baseline` object times the CPU restatement of the same algorithm
 (single thread) on the first passes of the same workload, bounded to about
@@ -41,44 +43,44 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def quality_roofline(ctx, mesh, launches):
-    """HIP-event timing of measure_qualities over the whole mesh."""
+def kernel_rooflines(ctx, mesh, launches):
+    """HIP-event timing, on the library's stream, of measure_qualities over
+    every tet and measure_edges_metric over every edge of the mesh."""
     import numpy as np
-    import torch
+    from omega_h_amd.hip import DeviceArray, Event
     dim = mesh.dim
-    nelems = mesh.nents(dim)
     nverts = mesh.nents(0)
-    ev2v = torch.from_numpy(np.ascontiguousarray(mesh.verts(dim), dtype=np.int32)).cuda()
-    coords = torch.from_numpy(mesh.coords()).cuda()
-    metric = torch.from_numpy(mesh.metric()).cuda()
     ncomps = mesh.metric_ncomps
-    out = torch.empty(nelems, dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
-    stream = torch.cuda.ExternalStream(ctx.stream)
+    coords = DeviceArray(mesh.coords())
+    metric = DeviceArray(mesh.metric())
+    stream = ctx.stream
+    out = {}
+    for name, d, fn, want, bytes_per_ent in (
+            ("elem_qualities_kernel<3,6>", dim, ctx.measure_qualities, mesh.qualities(), 24),
+            ("edge_lengths_kernel<3,6>", 1, ctx.measure_edges_metric, mesh.lengths(), 16)):
+        n = mesh.nents(d)
+        ev2v = DeviceArray(np.ascontiguousarray(mesh.verts(d), dtype=np.int32))
+        res = DeviceArray(dtype=np.float64, n=n)
 
-    def launch():
-        ctx.measure_qualities(dim, ncomps, nelems, ev2v.data_ptr(), coords.data_ptr(),
-                              metric.data_ptr(), out.data_ptr())
+        def launch():
+            fn(dim, ncomps, n, ev2v.ptr, coords.ptr, metric.ptr, res.ptr)
 
-    for _ in range(3):
-        launch()
-    ctx.synchronize()
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    t0.record(stream)
-    for _ in range(launches):
-        launch()
-    t1.record(stream)
-    t1.synchronize()
-    ms = t0.elapsed_time(t1) / launches
-    want = mesh.qualities()
-    got = out.cpu().numpy()
-    if not np.array_equal(want.view(np.int64), got.view(np.int64)):
-        raise RuntimeError("measure_qualities on device pointers differs from Mesh.qualities")
-    nbytes = 24 * nelems + 72 * nverts
-    achieved = nbytes / (ms * 1e-3) / 1e9
-    return {"kernel": "elem_qualities_kernel<3,6>", "launch_ms": ms, "bytes": nbytes,
-            "achieved": achieved}
+        for _ in range(3):
+            launch()
+        ctx.synchronize()
+        t0, t1 = Event(), Event()
+        t0.record(stream)
+        for _ in range(launches):
+            launch()
+        t1.record(stream)
+        t1.synchronize()
+        ms = t0.elapsed_ms(t1) / launches
+        if not np.array_equal(want.view(np.int64), res.numpy().view(np.int64)):
+            raise RuntimeError(f"{name} on device pointers differs from the mesh's own values")
+        nbytes = bytes_per_ent * n + 72 * nverts
+        out[name] = {"kernel": name, "launch_ms": ms, "bytes": nbytes,
+                     "achieved": nbytes / (ms * 1e-3) / 1e9}
+    return out
 
 
 def cpu_baseline(n, seconds):
@@ -165,8 +167,11 @@ def main():
                 f"{sum(p[3] for p in ps):.3f} s")
     log(f"step: {len(runs[-1])} passes, {nelems0} -> {final} tets, {wall / args.steps:.3f} s")
 
-    roof = quality_roofline(ctx, base, args.roofline_launches)
-    log(f"roofline: {roof}")
+    roofs = kernel_rooflines(ctx, base, args.roofline_launches)
+    for r in roofs.values():
+        log(f"{r['kernel']}: {r['launch_ms'] * 1e3:.1f} us per launch, {r['bytes']} B, "
+            f"{r['achieved']:.0f} GB/s")
+    roof = roofs["elem_qualities_kernel<3,6>"]
     line = {
         "metric": "Mtets adapted/s per pass (plain FP64 vs mixed) at 1.57M tets; "
                   "p50/p99 serve latency",
