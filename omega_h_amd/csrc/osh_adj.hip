@@ -95,57 +95,53 @@ __global__ void transit_kernel(LO nhigh, LO const* hm2m, I8 const* m2hm_codes,
   }
 }
 
-__global__ void count_lows(LO nhl, LO const* hl2l, LO* counts) {
+// invert_adj in three sweeps over the uses hl = high * deg + which:
+//   count_lows: per-low use counts by atomics; each use keeps its slot
+//   fill_uses:  ab2hl[a2ab[l] + slot] = hl (slots are in atomic order)
+//   rank_uses:  each use counts the uses of its row with a smaller hl and
+//               writes itself at that rank, which sorts every row by high
+//               index (sort_by_high_index, src/Omega_h_adj.cpp:178-200)
+//               and forms the codes of separate_upward_with_codes (:202-217)
+__global__ void count_lows(LO nhl, LO const* hl2l, LO* counts, LO* slot) {
   LO i = thread_index();
-  if (i < nhl) atomicAdd(&counts[hl2l[i]], 1);
+  if (i < nhl) slot[i] = atomicAdd(&counts[hl2l[i]], 1);
 }
 
-__global__ void fill_uses(LO nhl, LO const* hl2l, LO const* a2ab, LO* pos, LO* ab2hl) {
+__global__ void fill_uses(LO nhl, LO const* hl2l, LO const* a2ab, LO const* slot, LO* ab2hl) {
+  LO hl = thread_index();
+  if (hl >= nhl) return;
+  ab2hl[a2ab[hl2l[hl]] + slot[hl]] = hl;
+}
+
+__global__ void rank_uses(LO nhl, LO const* hl2l, LO const* a2ab, LO const* ab2hl, int deg,
+    I8 const* down_codes, LO* ab2b, I8* codes) {
   LO hl = thread_index();
   if (hl >= nhl) return;
   LO l = hl2l[hl];
-  LO k = atomicAdd(&pos[l], 1);
-  ab2hl[a2ab[l] + k] = hl;
-}
-
-// order each row by high index and form the codes of separate_upward_*
-__global__ void finish_rows(LO nlow, LO const* a2ab, LO* ab2b, I8* codes, int deg,
-    I8 const* down_codes) {
-  LO l = thread_index();
-  if (l >= nlow) return;
   LO b = a2ab[l], e = a2ab[l + 1];
-  for (LO j = b + 1; j < e; ++j) {
-    LO x = ab2b[j];
-    LO k = j - 1;
-    while (k >= b && ab2b[k] > x) {
-      ab2b[k + 1] = ab2b[k];
-      --k;
-    }
-    ab2b[k + 1] = x;
-  }
-  for (LO j = b; j < e; ++j) {
-    LO hl = ab2b[j];
-    int which_down = hl % deg;
-    I8 c = down_codes ? make_code(code_is_flipped(down_codes[hl]),
-                            code_rotation(down_codes[hl]), which_down)
-                      : make_code(0, 0, which_down);
-    codes[j] = c;
-    ab2b[j] = hl / deg;
-  }
+  LO rank = 0;
+  for (LO j = b; j < e; ++j) rank += (ab2hl[j] < hl);
+  int which_down = hl % deg;
+  I8 c = down_codes ? make_code(code_is_flipped(down_codes[hl]), code_rotation(down_codes[hl]),
+                          which_down)
+                    : make_code(0, 0, which_down);
+  ab2b[b + rank] = hl / deg;
+  codes[b + rank] = c;
 }
 
 Adj invert_adj(Ctx& c, LO nhigh, int deg, LO nlow, LOs const& hl2l, Bytes const* down_codes) {
   LO nhl = nhigh * deg;
   LOs counts = fill_lo(c, nlow, 0);
-  OSH_LAUNCH(count_lows, nhl, c, hl2l.data(), counts.data());
+  LOs slot(c, nhl);
+  OSH_LAUNCH(count_lows, nhl, c, hl2l.data(), counts.data(), slot.data());
   Adj a;
   a.a2ab = offset_scan(c, counts);
+  LOs ab2hl(c, nhl);
+  OSH_LAUNCH(fill_uses, nhl, c, hl2l.data(), a.a2ab.data(), slot.data(), ab2hl.data());
   a.ab2b = LOs(c, nhl);
   a.codes = Bytes(c, nhl);
-  LOs pos = fill_lo(c, nlow, 0);
-  OSH_LAUNCH(fill_uses, nhl, c, hl2l.data(), a.a2ab.data(), pos.data(), a.ab2b.data());
-  OSH_LAUNCH(finish_rows, nlow, c, a.a2ab.data(), a.ab2b.data(), a.codes.data(), deg,
-      down_codes ? down_codes->data() : (I8 const*)nullptr);
+  OSH_LAUNCH(rank_uses, nhl, c, hl2l.data(), a.a2ab.data(), ab2hl.data(), deg,
+      down_codes ? down_codes->data() : (I8 const*)nullptr, a.ab2b.data(), a.codes.data());
   return a;
 }
 
@@ -348,8 +344,9 @@ Adj const& ask_star(Ctx& c, DMesh& m, int d) {
       LOs deg(c, ne);
       OSH_LAUNCH(edge_star_degrees, ne, c, e2f.a2ab.data(),
           e2t ? e2t->a2ab.data() : (LO const*)nullptr, deg.data());
-      g.a2ab = offset_scan(c, deg);
-      g.ab2b = LOs(c, last(c, g.a2ab));
+      LO total;
+      g.a2ab = offset_scan(c, deg, &total);
+      g.ab2b = LOs(c, total);
       OSH_LAUNCH(edge_star_fill, ne, c, e2f.a2ab.data(), e2f.ab2b.data(), e2f.codes.data(),
           m.down[2].data(), e2t ? e2t->a2ab.data() : (LO const*)nullptr,
           e2t ? e2t->ab2b.data() : (LO const*)nullptr,
@@ -368,8 +365,9 @@ Adj const& ask_dual(Ctx& c, DMesh& m) {
     LOs deg(c, ne);
     OSH_LAUNCH(dual_degrees, ne, c, dim + 1, m.down[dim].data(), s2e.a2ab.data(), deg.data());
     Adj g;
-    g.a2ab = offset_scan(c, deg);
-    g.ab2b = LOs(c, last(c, g.a2ab));
+    LO total;
+    g.a2ab = offset_scan(c, deg, &total);
+    g.ab2b = LOs(c, total);
     OSH_LAUNCH(dual_fill, ne, c, dim + 1, m.down[dim].data(), s2e.a2ab.data(),
         s2e.ab2b.data(), g.a2ab.data(), g.ab2b.data());
     m.dual = g;

@@ -390,8 +390,8 @@ void build_box(Ctx& c, DMesh& m, int dim, Real x, Real y, Real z, LO nx, LO ny, 
     LO nxy = nx * ny, nh = nxy * nz;
     LOs deg(c, nh);
     OSH_LAUNCH(hex_degrees, nh, c, nx, nxy, nvx, nvxy, deg.data());
-    LOs h2ht = offset_scan(c, deg);
-    LO nt = last(c, h2ht);
+    LO nt;
+    LOs h2ht = offset_scan(c, deg, &nt);
     LOs tv2v(c, nt * 4);
     OSH_LAUNCH(tets_from_hexes_kernel, nh, c, nx, nxy, nvx, nvxy, h2ht.data(), tv2v.data());
     build_from_elems(c, m, 3, nt, tv2v, nv);

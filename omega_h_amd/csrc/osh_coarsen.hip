@@ -377,8 +377,8 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
       LOs counts(c, nkeys);
       OSH_LAUNCH(live_counts, nkeys, c, keys2verts.data(), v2d.a2ab.data(), v2d.ab2b.data(),
           dead[d].data(), counts.data());
-      k2p = offset_scan(c, counts);
-      LO nprods = last(c, k2p);
+      LO nprods;
+      k2p = offset_scan(c, counts, &nprods);
       pv = LOs(c, nprods * (d + 1));
       psrc = LOs(c, nprods);
       LOs const& dv2v = verts_of(c, m, d);

@@ -254,8 +254,7 @@ void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
   LOs counts = fill_lo(c, nold, 0);
   OSH_LAUNCH(set_same_counts, nsame, c, out.same2old.data(), counts.data());
   OSH_LAUNCH(add_rep_counts, nkeys, c, reps.data(), keys2prods.data(), counts.data());
-  LOs offsets = offset_scan(c, counts);
-  out.nnew = last(c, offsets);
+  LOs offsets = offset_scan(c, counts, &out.nnew);
   LOs order;
   if (key_dim > ent_dim) {
     LO nkd = old.nents[key_dim];

@@ -7,10 +7,66 @@
 
 namespace osh {
 
-Ctx::Ctx() {
+std::size_t Pool::round_up(std::size_t bytes) {
+  if (bytes <= 256) return 256;
+  int lg = 63 - __builtin_clzll(bytes - 1);  // 2^lg < bytes <= 2^(lg+1)
+  std::size_t step = std::size_t(1) << (lg - 1);
+  return (bytes + step - 1) / step * step;
+}
+
+void* Pool::get(std::size_t bytes) {
+  std::size_t size = round_up(bytes);
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = free_.find(size);
+    if (it != free_.end() && !it->second.empty()) {
+      void* p = it->second.back();
+      it->second.pop_back();
+      cached_ -= size;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, size) != hipSuccess) {
+    (void)hipGetLastError();
+    release_all();
+    OSH_HIP(hipMalloc(&p, size));
+  }
+  return p;
+}
+
+void Pool::put(void* p, std::size_t bytes) {
+  std::size_t size = round_up(bytes);
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    if (!closed_ && cached_ + size <= kCap) {
+      free_[size].push_back(p);
+      cached_ += size;
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
+void Pool::release_all() {
+  std::unordered_map<std::size_t, std::vector<void*>> blocks;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    blocks.swap(free_);
+    cached_ = 0;
+  }
+  for (auto& kv : blocks)
+    for (void* p : kv.second) (void)hipFree(p);
+}
+
+Ctx::Ctx() : pool(std::make_shared<Pool>()) {
   OSH_HIP(hipGetDevice(&device));
   OSH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   OSH_HIP(hipHostMalloc(&pinned, 64, hipHostMallocDefault));
+  OSH_HIP(hipMalloc((void**)&counter, sizeof(unsigned)));
+  unsigned zero = 0;
+  OSH_HIP(hipMemcpy(counter, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  OSH_HIP(hipMalloc(&partials, 1024 * sizeof(double)));
 }
 
 Ctx::~Ctx() {
@@ -18,6 +74,9 @@ Ctx::~Ctx() {
     (void)hipStreamSynchronize(stream);
     (void)hipStreamDestroy(stream);
   }
+  if (pool) pool->close();
+  if (counter) (void)hipFree(counter);
+  if (partials) (void)hipFree(partials);
   if (pinned) (void)hipHostFree(pinned);
 }
 
@@ -60,9 +119,17 @@ __device__ __forceinline__ LO block_exclusive_scan(LO v, LO* total) {
   return before + inc - v;
 }
 
-// out[1 + i] = inclusive prefix within the tile; bsum[block] = tile total
 template <class T>
-__global__ void __launch_bounds__(kBlock) scan_tiles(LO n, T const* in, LO* out, LO* bsum) {
+__global__ void fill_kernel(LO n, T* out, T value) {
+  LO i = thread_index();
+  if (i < n) out[i] = value;
+}
+
+// out[1 + i] = inclusive prefix within the tile; bsum[block] = tile total;
+// a single-tile scan also writes its total to *total when given
+template <class T>
+__global__ void __launch_bounds__(kBlock) scan_tiles(LO n, T const* in, LO* out, LO* bsum,
+    LO* total_out) {
   LO base = LO(blockIdx.x) * kScanTile + LO(threadIdx.x) * kScanItems;
   LO vals[kScanItems];
   LO acc = 0;
@@ -83,6 +150,7 @@ __global__ void __launch_bounds__(kBlock) scan_tiles(LO n, T const* in, LO* out,
   if (threadIdx.x == 0) {
     bsum[blockIdx.x] = total;
     if (blockIdx.x == 0) out[0] = 0;
+    if (total_out && gridDim.x == 1) *total_out = total;
   }
 }
 
@@ -94,22 +162,24 @@ __global__ void __launch_bounds__(kBlock) add_tile_offsets(LO n, LO* out, LO con
   }
 }
 
+// the grand total lands in *total_out (device-visible) when given: the
+// top level of the recursion is always a single tile
 template <class T>
-LOs offset_scan_t(Ctx& c, DArray<T> const& counts) {
+LOs offset_scan_t(Ctx& c, DArray<T> const& counts, LO* total_out) {
   LO n = counts.size();
   LOs out(c, n + 1);
   if (n == 0) {
-    LO zero = 0;
-    upload(c, out, &zero, 1);
+    OSH_LAUNCH(fill_kernel<LO>, 1, c, out.data(), 0);
+    if (total_out) OSH_LAUNCH(fill_kernel<LO>, 1, c, total_out, 0);
     return out;
   }
   LO nb = LO((std::int64_t(n) + kScanTile - 1) / kScanTile);
   LOs bsum(c, nb);
   hipLaunchKernelGGL(scan_tiles<T>, dim3(nb), dim3(kBlock), 0, c.stream, n,
-      counts.data(), out.data(), bsum.data());
+      counts.data(), out.data(), bsum.data(), total_out);
   OSH_HIP(hipGetLastError());
   if (nb > 1) {
-    LOs boff = offset_scan_t<LO>(c, bsum);
+    LOs boff = offset_scan_t<LO>(c, bsum, total_out);
     hipLaunchKernelGGL(add_tile_offsets, dim3(nb), dim3(kBlock), 0, c.stream,
         n, out.data(), boff.data());
     OSH_HIP(hipGetLastError());
@@ -132,12 +202,6 @@ __global__ void iota_kernel(LO n, LO* out, LO start, LO stride) {
   if (i < n) out[i] = start + i * stride;
 }
 
-template <class T>
-__global__ void fill_kernel(LO n, T* out, T value) {
-  LO i = thread_index();
-  if (i < n) out[i] = value;
-}
-
 enum ReduceOp { kMin, kMax, kSum };
 
 template <class T, int Op>
@@ -147,12 +211,16 @@ __device__ __forceinline__ T combine(T a, T b) {
   return a + b;
 }
 
+// one launch: every block reduces a grid-stride slice into partials, and
+// the block that finishes last (by the context's counter, which it resets)
+// reduces the partials and writes the result to host-visible memory
 template <class T, int Op>
-__global__ void __launch_bounds__(kBlock) reduce_blocks(LO n, T const* in, T* out, T init) {
+__global__ void __launch_bounds__(kBlock) reduce_all(LO n, T const* in, T init, T* partials,
+    unsigned* counter, T* result) {
   __shared__ T buf[kBlock];
+  __shared__ bool is_last;
   T acc = init;
-  for (LO i = LO(blockIdx.x) * kBlock + LO(threadIdx.x); i < n;
-       i += LO(gridDim.x) * kBlock)
+  for (LO i = LO(blockIdx.x) * kBlock + LO(threadIdx.x); i < n; i += LO(gridDim.x) * kBlock)
     acc = combine<T, Op>(acc, in[i]);
   buf[threadIdx.x] = acc;
   __syncthreads();
@@ -160,7 +228,26 @@ __global__ void __launch_bounds__(kBlock) reduce_blocks(LO n, T const* in, T* ou
     if (int(threadIdx.x) < s) buf[threadIdx.x] = combine<T, Op>(buf[threadIdx.x], buf[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = buf[0];
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&partials[blockIdx.x], buf[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned done = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    is_last = (done == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  acc = init;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
+    acc = combine<T, Op>(acc, __hip_atomic_load(&partials[b], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+  buf[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) buf[threadIdx.x] = combine<T, Op>(buf[threadIdx.x], buf[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *result = buf[0];
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 template <class T, int Op>
@@ -169,15 +256,10 @@ T reduce(Ctx& c, DArray<T> const& a, T init) {
   if (n == 0) return init;
   LO nb = LO(nblocks(n));
   if (nb > 1024) nb = 1024;
-  DArray<T> part(c, nb);
-  hipLaunchKernelGGL((reduce_blocks<T, Op>), dim3(nb), dim3(kBlock), 0, c.stream, n,
-      a.data(), part.data(), init);
+  hipLaunchKernelGGL((reduce_all<T, Op>), dim3(nb), dim3(kBlock), 0, c.stream, n, a.data(), init,
+      static_cast<T*>(c.partials), c.counter, static_cast<T*>(c.pinned));
   OSH_HIP(hipGetLastError());
-  DArray<T> fin(c, 1);
-  hipLaunchKernelGGL((reduce_blocks<T, Op>), dim3(1), dim3(kBlock), 0, c.stream, nb,
-      part.data(), fin.data(), init);
-  OSH_HIP(hipGetLastError());
-  return get_value(c, fin.data());
+  return pinned_value<T>(c);
 }
 
 __global__ void gather_key_column(LO n, I64 const* keys, LO const* perm, int width,
@@ -188,15 +270,23 @@ __global__ void gather_key_column(LO n, I64 const* keys, LO const* perm, int wid
 
 }  // namespace
 
-LOs offset_scan(Ctx& c, LOs const& counts) { return offset_scan_t<LO>(c, counts); }
-LOs offset_scan_bytes(Ctx& c, Bytes const& counts) { return offset_scan_t<I8>(c, counts); }
+LOs offset_scan(Ctx& c, LOs const& counts, LO* total) {
+  LOs out = offset_scan_t<LO>(c, counts, total ? static_cast<LO*>(c.pinned) : nullptr);
+  if (total) *total = pinned_value<LO>(c);
+  return out;
+}
+LOs offset_scan_bytes(Ctx& c, Bytes const& counts, LO* total) {
+  LOs out = offset_scan_t<I8>(c, counts, total ? static_cast<LO*>(c.pinned) : nullptr);
+  if (total) *total = pinned_value<LO>(c);
+  return out;
+}
 
 LOs collect_marked(Ctx& c, Bytes const& marks) {
   LO n = marks.size();
   Bytes flags(c, n);
   OSH_LAUNCH(nonzero_flags, n, c, marks.data(), flags.data());
-  LOs offsets = offset_scan_bytes(c, flags);
-  LO count = last(c, offsets);
+  LO count;
+  LOs offsets = offset_scan_bytes(c, flags, &count);
   LOs out(c, count);
   OSH_LAUNCH(scatter_marked, n, c, marks.data(), offsets.data(), out.data());
   return out;

@@ -234,8 +234,8 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
       OSH_LAUNCH(prod_counts, nkeys, c, keys2edges.data(),
           pairs ? pairs->a2ab.data() : (LO const*)nullptr,
           cuts ? cuts->a2ab.data() : (LO const*)nullptr, counts.data());
-      k2p = offset_scan(c, counts);
-      LO nprods = last(c, k2p);
+      LO nprods;
+      k2p = offset_scan(c, counts, &nprods);
       pv = LOs(c, nprods * (d + 1));
       psrc = LOs(c, nprods);
       psrc_dim = Bytes(c, nprods);

@@ -8,7 +8,9 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -37,11 +39,48 @@ struct Error : std::runtime_error {
     if (!(cond)) throw ::osh::Error(std::string("check failed: ") + (msg));  \
   } while (0)
 
+// Caching device allocator of one context.  Every array of a context is
+// used only on that context's stream, so a block returned by a dead array can
+// be handed to the next request at once: the work that used it is already
+// ordered before any work that the new owner enqueues.  Sizes are rounded up
+// to classes at most 25% apart; blocks stay cached (up to a cap) until the
+// context closes, so a steady adapt loop makes no runtime allocation calls.
+class Pool {
+ public:
+  Pool() = default;
+  Pool(Pool const&) = delete;
+  Pool& operator=(Pool const&) = delete;
+  ~Pool() { release_all(); }
+  void* get(std::size_t bytes);
+  void put(void* p, std::size_t bytes);
+  // return every cached block to the runtime (the caller has drained the
+  // stream); later puts free directly
+  void release_all();
+  void close() {
+    release_all();
+    std::lock_guard<std::mutex> g(mu_);
+    closed_ = true;
+  }
+  static std::size_t round_up(std::size_t bytes);
+
+ private:
+  std::mutex mu_;
+  std::unordered_map<std::size_t, std::vector<void*>> free_;
+  std::size_t cached_ = 0;
+  bool closed_ = false;
+  static constexpr std::size_t kCap = std::size_t(16) << 30;
+};
+
 struct Ctx {
   hipStream_t stream = nullptr;
   int device = 0;
-  // pinned scratch for small readbacks
+  // pinned host words that kernels write results into (totals, reductions)
   void* pinned = nullptr;
+  // device words for single-launch reductions (a completion counter that the
+  // last block resets) and their per-block partials
+  unsigned* counter = nullptr;
+  void* partials = nullptr;
+  std::shared_ptr<Pool> pool;
   Ctx();
   ~Ctx();
   Ctx(Ctx const&) = delete;
@@ -49,8 +88,9 @@ struct Ctx {
   void sync();
 };
 
-// reference-counted device buffer allocated and freed in stream order;
-// a default-constructed array does not exist (like an empty Omega_h Read<T>)
+// reference-counted device buffer from the context's pool, used in stream
+// order; a default-constructed array does not exist (like an empty Omega_h
+// Read<T>)
 template <class T>
 class DArray {
  public:
@@ -58,10 +98,11 @@ class DArray {
   DArray(Ctx& c, LO n) : n_(n), exists_(true) {
     OSH_CHECK(n >= 0, "negative array size");
     T* p = nullptr;
-    if (n > 0) OSH_HIP(hipMallocAsync((void**)&p, sizeof(T) * size_t(n), c.stream));
-    hipStream_t s = c.stream;
-    ptr_ = std::shared_ptr<T>(p, [s](T* q) {
-      if (q) (void)hipFreeAsync(q, s);
+    std::size_t bytes = sizeof(T) * std::size_t(n);
+    if (n > 0) p = static_cast<T*>(c.pool->get(bytes));
+    std::shared_ptr<Pool> pool = c.pool;
+    ptr_ = std::shared_ptr<T>(p, [pool, bytes](T* q) {
+      if (q) pool->put(q, bytes);
     });
   }
   T* data() const { return ptr_.get(); }
@@ -99,6 +140,15 @@ inline std::vector<T> to_host(Ctx& c, DArray<T> const& d) {
   return h;
 }
 
+// a value that a kernel wrote into c.pinned, after draining the stream
+template <class T>
+inline T pinned_value(Ctx& c, int slot = 0) {
+  c.sync();
+  T v;
+  std::memcpy(&v, static_cast<char*>(c.pinned) + 8 * slot, sizeof(T));
+  return v;
+}
+
 template <class T>
 inline T get_value(Ctx& c, T const* dptr) {
   OSH_HIP(hipMemcpyAsync(c.pinned, dptr, sizeof(T), hipMemcpyDeviceToHost, c.stream));
@@ -134,8 +184,9 @@ __device__ __forceinline__ LO thread_index() {
 }
 
 // ---- primitives (osh_prims.hip) ----
-LOs offset_scan(Ctx& c, LOs const& counts);           // n+1, exclusive
-LOs offset_scan_bytes(Ctx& c, Bytes const& counts);   // n+1, exclusive
+// n+1 exclusive offsets; with `total`, also the sum (read back after a sync)
+LOs offset_scan(Ctx& c, LOs const& counts, LO* total = nullptr);
+LOs offset_scan_bytes(Ctx& c, Bytes const& counts, LO* total = nullptr);
 LOs collect_marked(Ctx& c, Bytes const& marks);       // indices of nonzero
 LOs iota(Ctx& c, LO n, LO start = 0, LO stride = 1);
 LOs fill_lo(Ctx& c, LO n, LO value);

@@ -87,60 +87,113 @@ __device__ double length_between(LO a, LO b, Real const* coords, Real const* met
   return edge_length<S, M>(p0, p1, m0, m1);
 }
 
+// swap3d_qualities (src/Omega_h_swap3d_qualities.cpp:11-75) in four flat
+// sweeps instead of one thread per candidate edge:
+//   swap3d_loops:     the loop of each candidate and its counts of unique
+//                     triangles and unique interior edges
+//   swap3d_tri_quals: one thread per (candidate, unique triangle): the min
+//                     quality of the two tets on that triangle
+//   swap3d_edge_lens: one thread per (candidate, unique edge): its length
+//   swap3d_choose:    per candidate, choose() of src/Omega_h_swap3d_choice.hpp
+//                     over those values, in the reference's order
+// Every value is the one the reference computes (each unique triangle and
+// edge evaluated once), so the choice is the same.
+__global__ void swap3d_loops(LO ncands, LO const* cands, LO const* e2t_a2ab, LO const* e2t_ab2b,
+    I8 const* e2t_codes, LO const* ev2v, LO const* tv2v, Tables const* tb, LO* lsize,
+    LO* leev, LO* lverts, LO* ntris, LO* nedges) {
+  LO cand = thread_index();
+  if (cand >= ncands) return;
+  Loop loop;
+  find_loop(cands[cand], e2t_a2ab, e2t_ab2b, e2t_codes, ev2v, tv2v, loop);
+  lsize[cand] = loop.size;
+  if (loop.size > swap::kMaxLoop) {
+    ntris[cand] = 0;
+    nedges[cand] = 0;
+    return;
+  }
+  leev[cand * 2 + 0] = loop.eev2v[0];
+  leev[cand * 2 + 1] = loop.eev2v[1];
+  for (int k = 0; k < swap::kMaxLoop; ++k)
+    lverts[cand * swap::kMaxLoop + k] = (k < loop.size) ? loop.verts[k] : -1;
+  ntris[cand] = tb->ntris[loop.size];
+  nedges[cand] = tb->nuedges[loop.size];
+}
+
+// the candidate whose range [off[c], off[c + 1]) holds item i
+__device__ LO owner_of(LO i, LO n, LO const* off) {
+  LO lo = 0, hi = n;
+  while (hi - lo > 1) {
+    LO mid = lo + (hi - lo) / 2;
+    if (off[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 template <int M>
-__global__ void __launch_bounds__(kBlock) swap3d_qualities_kernel(LO ncands, LO const* cands,
-    LO const* e2t_a2ab, LO const* e2t_ab2b, I8 const* e2t_codes, LO const* ev2v,
-    LO const* tv2v, Real const* coords, Real const* metric, Real const* elem_quals,
-    Real max_length, Tables const* tb, I8* configs, Real* quals, I8* keep) {
+__global__ void swap3d_tri_quals(LO npairs, LO ncands, LO const* toff, LO const* lsize,
+    LO const* leev, LO const* lverts, Real const* coords, Real const* metric, Tables const* tb,
+    Real* tq) {
+  LO p = thread_index();
+  if (p >= npairs) return;
+  LO cand = owner_of(p, ncands, toff);
+  int ut = p - toff[cand];
+  int n = lsize[cand];
+  LO tv[4];
+  for (int k = 0; k < 3; ++k) tv[k] = lverts[cand * swap::kMaxLoop + tb->tris[n][ut][k]];
+  double tri_minqual = 1.0;
+  for (int tri_tet = 0; tri_tet < 2; ++tri_tet) {
+    tv[3] = leev[cand * 2 + 1 - tri_tet];
+    tri_minqual = math::min2(tri_minqual, tet_quality<M>(tv, coords, metric));
+    LO t = tv[1];
+    tv[1] = tv[2];
+    tv[2] = t;
+  }
+  tq[p] = tri_minqual;
+}
+
+template <int M>
+__global__ void swap3d_edge_lens(LO npairs, LO ncands, LO const* eoff, LO const* lsize,
+    LO const* lverts, Real const* coords, Real const* metric, Tables const* tb, Real* el) {
+  LO q = thread_index();
+  if (q >= npairs) return;
+  LO cand = owner_of(q, ncands, eoff);
+  int ue = q - eoff[cand];
+  int n = lsize[cand];
+  LO const* lv = lverts + cand * swap::kMaxLoop;
+  el[q] = length_between<3, M>(lv[tb->uedges[n][ue][0]], lv[tb->uedges[n][ue][1]], coords,
+      metric);
+}
+
+__global__ void swap3d_choose(LO ncands, LO const* cands, LO const* lsize, LO const* toff,
+    LO const* eoff, Real const* tq, Real const* el, LO const* e2t_a2ab, LO const* e2t_ab2b,
+    Real const* elem_quals, Real max_length, Tables const* tb, I8* configs, Real* quals,
+    I8* keep) {
   LO cand = thread_index();
   if (cand >= ncands) return;
   LO edge = cands[cand];
-  Loop loop;
-  find_loop(edge, e2t_a2ab, e2t_ab2b, e2t_codes, ev2v, tv2v, loop);
+  int n = lsize[cand];
   int choice_mesh = -1;
   double choice_quality = -1.0;
-  if (loop.size <= swap::kMaxLoop) {
+  if (n <= swap::kMaxLoop) {
     choice_quality = 0.0;
-    int n = loop.size;
     int nmeshes = tb->mesh_counts[n];
     int nmesh_tris = tb->mesh_sizes[n];
-    bool tri_cached[swap::kMaxUniqueTris];
-    double tri_qual[swap::kMaxUniqueTris];
-    bool edge_cached[swap::kMaxUniqueEdges];
-    double edge_len[swap::kMaxUniqueEdges];
-    for (int i = 0; i < swap::kMaxUniqueTris; ++i) tri_cached[i] = false;
-    for (int i = 0; i < swap::kMaxUniqueEdges; ++i) edge_cached[i] = false;
+    int nmesh_edges = tb->nedges[n];
+    Real const* ctq = tq + toff[cand];
+    Real const* cel = el + eoff[cand];
     for (int mesh = 0; mesh < nmeshes; ++mesh) {
       double mesh_minqual = 1.0;
       for (int mt = 0; mt < nmesh_tris; ++mt) {
-        int ut = tb->meshes[n][mesh * nmesh_tris + mt];
-        if (!tri_cached[ut]) {
-          LO tv[4];
-          for (int k = 0; k < 3; ++k) tv[k] = loop.verts[tb->tris[n][ut][k]];
-          double tri_minqual = 1.0;
-          for (int tri_tet = 0; tri_tet < 2; ++tri_tet) {
-            tv[3] = loop.eev2v[1 - tri_tet];
-            tri_minqual = math::min2(tri_minqual, tet_quality<M>(tv, coords, metric));
-            LO t = tv[1];
-            tv[1] = tv[2];
-            tv[2] = t;
-          }
-          tri_cached[ut] = true;
-          tri_qual[ut] = tri_minqual;
-        }
-        mesh_minqual = math::min2(mesh_minqual, tri_qual[ut]);
+        mesh_minqual = math::min2(mesh_minqual, ctq[tb->meshes[n][mesh * nmesh_tris + mt]]);
         if (mesh_minqual < 0.0) break;
       }
       if (mesh_minqual > choice_quality) {
         bool overshoot = false;
-        for (int me = 0; me < tb->nedges[n]; ++me) {
-          int ue = tb->medges[n][mesh * tb->nedges[n] + me];
-          if (!edge_cached[ue]) {
-            edge_len[ue] = length_between<3, M>(loop.verts[tb->uedges[n][ue][0]],
-                loop.verts[tb->uedges[n][ue][1]], coords, metric);
-            edge_cached[ue] = true;
-          }
-          if (edge_len[ue] > max_length) {
+        for (int me = 0; me < nmesh_edges; ++me) {
+          if (cel[tb->medges[n][mesh * nmesh_edges + me]] > max_length) {
             overshoot = true;
             break;
           }
@@ -369,16 +422,30 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   Adj const& e2t = ask_up(c, m, 1, 3);
   LOs const& tv2v = verts_of(c, m, 3);
   Reals const& eq = ask_qualities(c, m);
-  if (md == 3)
-    OSH_LAUNCH(swap3d_qualities_kernel<3>, cands.size(), c, cands.data(), e2t.a2ab.data(),
-        e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), m.coords.data(),
-        m.metric.data(), eq.data(), o.max_length_allowed, tb, configs.data(), quals.data(),
-        keep.data());
-  else
-    OSH_LAUNCH(swap3d_qualities_kernel<1>, cands.size(), c, cands.data(), e2t.a2ab.data(),
-        e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), m.coords.data(),
-        m.metric.data(), eq.data(), o.max_length_allowed, tb, configs.data(), quals.data(),
-        keep.data());
+  LO ncands = cands.size();
+  LOs lsize(c, ncands), leev(c, ncands * 2), lverts(c, ncands * swap::kMaxLoop);
+  LOs ntris(c, ncands), nedges(c, ncands);
+  OSH_LAUNCH(swap3d_loops, ncands, c, cands.data(), e2t.a2ab.data(), e2t.ab2b.data(),
+      e2t.codes.data(), m.down[1].data(), tv2v.data(), tb, lsize.data(), leev.data(),
+      lverts.data(), ntris.data(), nedges.data());
+  LO ntp, nep;
+  LOs toff = offset_scan(c, ntris, &ntp);
+  LOs eoff = offset_scan(c, nedges, &nep);
+  Reals tq(c, ntp), el(c, nep);
+  if (md == 3) {
+    OSH_LAUNCH(swap3d_tri_quals<3>, ntp, c, ncands, toff.data(), lsize.data(), leev.data(),
+        lverts.data(), m.coords.data(), m.metric.data(), tb, tq.data());
+    OSH_LAUNCH(swap3d_edge_lens<3>, nep, c, ncands, eoff.data(), lsize.data(), lverts.data(),
+        m.coords.data(), m.metric.data(), tb, el.data());
+  } else {
+    OSH_LAUNCH(swap3d_tri_quals<1>, ntp, c, ncands, toff.data(), lsize.data(), leev.data(),
+        lverts.data(), m.coords.data(), m.metric.data(), tb, tq.data());
+    OSH_LAUNCH(swap3d_edge_lens<1>, nep, c, ncands, eoff.data(), lsize.data(), lverts.data(),
+        m.coords.data(), m.metric.data(), tb, el.data());
+  }
+  OSH_LAUNCH(swap3d_choose, ncands, c, cands.data(), lsize.data(), toff.data(), eoff.data(),
+      tq.data(), el.data(), e2t.a2ab.data(), e2t.ab2b.data(), eq.data(), o.max_length_allowed,
+      tb, configs.data(), quals.data(), keep.data());
   if (max_i8(c, keep) <= 0) return false;
   Bytes keys = find_indset(c, m, 1, quals, keep);
   LOs keys2edges = collect_marked(c, keys);
@@ -387,10 +454,11 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   OSH_LAUNCH(swap3d_counts, nkeys, c, keys2edges.data(), e2t.a2ab.data(), tb, ce.data(),
       cf.data(), ct.data());
   LOs k2p[4], pv[4];
-  k2p[1] = offset_scan(c, ce);
-  k2p[2] = offset_scan(c, cf);
-  k2p[3] = offset_scan(c, ct);
-  for (int d = 1; d <= 3; ++d) pv[d] = LOs(c, last(c, k2p[d]) * (d + 1));
+  LO nprods[4];
+  k2p[1] = offset_scan(c, ce, &nprods[1]);
+  k2p[2] = offset_scan(c, cf, &nprods[2]);
+  k2p[3] = offset_scan(c, ct, &nprods[3]);
+  for (int d = 1; d <= 3; ++d) pv[d] = LOs(c, nprods[d] * (d + 1));
   OSH_LAUNCH(swap3d_products, nkeys, c, keys2edges.data(), configs.data(), e2t.a2ab.data(),
       e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), tb, k2p[1].data(),
       k2p[2].data(), k2p[3].data(), pv[1].data(), pv[2].data(), pv[3].data());

@@ -23,6 +23,8 @@ struct Tables {
   int mesh_sizes[8];   // triangles per triangulation
   int mesh_counts[8];  // triangulations per loop size
   int nedges[8];       // interior edges per triangulation
+  int ntris[8];        // unique triangles per loop size
+  int nuedges[8];      // unique interior edges per loop size
   signed char tris[8][kMaxUniqueTris][3];
   signed char meshes[8][42 * 5];
   signed char uedges[8][kMaxUniqueEdges][2];
@@ -69,6 +71,8 @@ inline Tables make_tables() {
   }
   for (auto const& s : srcs) {
     int n = s.n;
+    t.ntris[n] = int(std::strlen(s.tris) / 3);
+    t.nuedges[n] = int(std::strlen(s.edges) / 2);
     for (size_t i = 0; i < std::strlen(s.tris); ++i) t.tris[n][i / 3][i % 3] = static_cast<signed char>(s.tris[i] - '0');
     for (size_t i = 0; i < std::strlen(s.meshes); ++i) t.meshes[n][i] = static_cast<signed char>(b36(s.meshes[i]));
     for (size_t i = 0; i < std::strlen(s.edges); ++i) t.uedges[n][i / 2][i % 2] = static_cast<signed char>(s.edges[i] - '0');
