@@ -149,7 +149,7 @@ Adj invert_adj(Ctx& c, LO nhigh, int deg, LO nlow, LOs const& hl2l, Bytes const*
 template <int HIGH>
 __global__ void match_uses(LO nuses, LO const* prod_verts, LO const* lv2v,
     LO const* v2l_a2ab, LO const* v2l_ab2b, I8 const* v2l_codes, LO* out, I8* out_codes,
-    int* failures) {
+    int* error) {
   LO u = thread_index();
   if (u >= nuses) return;
   constexpr int LOW = HIGH - 1;
@@ -184,7 +184,15 @@ __global__ void match_uses(LO nuses, LO const* prod_verts, LO const* lv2v,
       }
     }
   }
-  if (found != 1) atomicAdd(failures, 1);
+  if (found != 1) {
+    // keep every index in range for the kernels that run before the host
+    // sees the error
+    if (found == 0) {
+      out[u] = 0;
+      out_codes[u] = 0;
+    }
+    *error = kReflectDownNoMatch;
+  }
 }
 
 __global__ void verts_across_edges_kernel(LO nve, LO const* ve2e, I8 const* codes,
@@ -311,16 +319,15 @@ void reflect_down(Ctx& c, DMesh& m, int high_dim, LO nprods, LOs const& prod_ver
   if (nuses == 0) return;
   LOs const& lv2v = verts_of(c, m, low_dim);
   Adj const& v2l = ask_up(c, m, 0, low_dim);
-  LOs failures = fill_lo(c, 1, 0);
+  // a use without a unique match sets the context's error word, which the
+  // next stream sync raises
   if (high_dim == 2) {
     OSH_LAUNCH(match_uses<2>, nuses, c, prod_verts.data(), lv2v.data(), v2l.a2ab.data(),
-        v2l.ab2b.data(), v2l.codes.data(), down.data(), codes.data(), failures.data());
+        v2l.ab2b.data(), v2l.codes.data(), down.data(), codes.data(), c.error_word());
   } else {
     OSH_LAUNCH(match_uses<3>, nuses, c, prod_verts.data(), lv2v.data(), v2l.a2ab.data(),
-        v2l.ab2b.data(), v2l.codes.data(), down.data(), codes.data(), failures.data());
+        v2l.ab2b.data(), v2l.codes.data(), down.data(), codes.data(), c.error_word());
   }
-  int nfail = get_value(c, failures.data());
-  OSH_CHECK(nfail == 0, "reflect_down: entity without a unique match");
 }
 
 Adj const& ask_star(Ctx& c, DMesh& m, int d) {

@@ -370,6 +370,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   LOs o2n, old_lows2new_lows;
   for (int d = 0; d <= dim; ++d) {
     LOs k2p, pv, psrc;
+    LO nprods = 0;
     if (d == 0) {
       k2p = fill_lo(c, nkeys + 1, 0);
     } else {
@@ -377,7 +378,6 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
       LOs counts(c, nkeys);
       OSH_LAUNCH(live_counts, nkeys, c, keys2verts.data(), v2d.a2ab.data(), v2d.ab2b.data(),
           dead[d].data(), counts.data());
-      LO nprods;
       k2p = offset_scan(c, counts, &nprods);
       pv = LOs(c, nprods * (d + 1));
       psrc = LOs(c, nprods);
@@ -387,7 +387,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
           o2n.data(), k2p.data(), pv.data(), psrc.data());
     }
     ModOut mo;
-    modify_ents_adapt(c, m, nw, d, 0, keys2verts, k2p, pv, old_lows2new_lows, mo);
+    modify_ents_adapt(c, m, nw, d, 0, keys2verts, k2p, nprods, pv, old_lows2new_lows, mo);
     if (d == 0) {
       o2n = mo.old2new;
       nw.coords = transfer<Real>(c, mo, dim, m.coords, nullptr);

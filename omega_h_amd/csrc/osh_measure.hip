@@ -11,41 +11,66 @@ namespace osh {
 
 namespace {
 
+// one vertex's coordinates and metric; a 6-component metric row is 48 bytes
+// at a 16-byte aligned offset and is read as three 16-byte loads
+template <int S, int NC>
+__device__ __forceinline__ void load_vertex(Real const* coords, Real const* metric, LO v,
+    double* p, double* m) {
+#pragma unroll
+  for (int j = 0; j < S; ++j) p[j] = coords[v * S + j];
+  if constexpr (NC == 6) {
+    double2 const* q = reinterpret_cast<double2 const*>(metric + std::size_t(v) * 6);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double2 t = q[j];
+      m[2 * j] = t.x;
+      m[2 * j + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) m[j] = metric[v * NC + j];
+  }
+}
+
 template <int S, int M>
 __global__ void __launch_bounds__(kBlock) edge_lengths_kernel(LO n, LO const* edges,
     LO const* ev2v, Real const* coords, Real const* metric, Real* out) {
-  LO i = thread_index();
+  LO i = xcd_thread_index();
   if (i >= n) return;
   LO e = edges ? edges[i] : i;
   constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
-  LO a = ev2v[2 * e], b = ev2v[2 * e + 1];
+  int2 ab = reinterpret_cast<int2 const*>(ev2v)[e];
   double p0[S], p1[S], m0[nc], m1[nc];
-  for (int k = 0; k < S; ++k) {
-    p0[k] = coords[a * S + k];
-    p1[k] = coords[b * S + k];
-  }
-  for (int k = 0; k < nc; ++k) {
-    m0[k] = metric[a * nc + k];
-    m1[k] = metric[b * nc + k];
-  }
+  load_vertex<S, nc>(coords, metric, ab.x, p0, m0);
+  load_vertex<S, nc>(coords, metric, ab.y, p1, m1);
   out[i] = edge_length<S, M>(p0, p1, m0, m1);
 }
 
 template <int D, int M>
 __global__ void __launch_bounds__(kBlock) elem_qualities_kernel(LO n, LO const* elems,
     LO const* ev2v, Real const* coords, Real const* metric, Real* out) {
-  LO i = thread_index();
+  LO i = xcd_thread_index();
   if (i >= n) return;
   LO e = elems ? elems[i] : i;
   constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
+  LO vs[D + 1];
+  if constexpr (D == 3) {
+    int4 t = reinterpret_cast<int4 const*>(ev2v)[e];
+    vs[0] = t.x;
+    vs[1] = t.y;
+    vs[2] = t.z;
+    vs[3] = t.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < D + 1; ++k) vs[k] = ev2v[e * (D + 1) + k];
+  }
   double p[D + 1][D];
   double m[D + 1][nc];
   double const* pp[D + 1];
   double const* mm[D + 1];
+#pragma unroll
   for (int k = 0; k < D + 1; ++k) {
-    LO v = ev2v[e * (D + 1) + k];
-    for (int j = 0; j < D; ++j) p[k][j] = coords[v * D + j];
-    for (int j = 0; j < nc; ++j) m[k][j] = metric[v * nc + j];
+    load_vertex<D, nc>(coords, metric, vs[k], p[k], m[k]);
     pp[k] = p[k];
     mm[k] = m[k];
   }
@@ -71,8 +96,20 @@ int metric_dim(DMesh const& m) { return (m.metric_ncomps == 1) ? 1 : m.dim; }
 
 }  // namespace
 
+// the kernels read a 6-component metric row with 16-byte loads, a tet's
+// vertices with one 16-byte load and an edge's with one 8-byte load
+static void check_alignment(int dim, int md, int nverts_per_ent, LO const* ev2v,
+    Real const* metric) {
+  auto off = [](void const* p, std::uintptr_t a) { return std::uintptr_t(p) % a; };
+  OSH_CHECK(!(dim == 3 && md == 3) || off(metric, 16) == 0,
+      "a 6-component metric array must be 16-byte aligned");
+  OSH_CHECK(off(ev2v, nverts_per_ent == 4 ? 16 : (nverts_per_ent == 2 ? 8 : 4)) == 0,
+      "the vertex list array is not aligned to its row size");
+}
+
 void launch_edge_lengths(Ctx& c, int dim, int md, LO n, LO const* edges, LO const* ev2v,
     Real const* coords, Real const* metric, Real* out) {
+  check_alignment(dim, md, 2, ev2v, metric);
   if (dim == 3 && md == 3)
     OSH_LAUNCH((edge_lengths_kernel<3, 3>), n, c, edges, ev2v, coords, metric, out);
   else if (dim == 3)
@@ -85,6 +122,7 @@ void launch_edge_lengths(Ctx& c, int dim, int md, LO n, LO const* edges, LO cons
 
 void launch_elem_qualities(Ctx& c, int dim, int md, LO n, LO const* elems, LO const* ev2v,
     Real const* coords, Real const* metric, Real* out) {
+  check_alignment(dim, md, dim + 1, ev2v, metric);
   if (dim == 3 && md == 3)
     OSH_LAUNCH((elem_qualities_kernel<3, 3>), n, c, elems, ev2v, coords, metric, out);
   else if (dim == 3)

@@ -213,7 +213,7 @@ LOs degrees_of(Ctx& c, LOs const& a2ab, LOs const& rows) {
 }
 
 void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
-    LOs const& keys2kds, LOs const& keys2prods, LOs const& prod_verts,
+    LOs const& keys2kds, LOs const& keys2prods, LO nprods, LOs const& prod_verts,
     LOs const& old_lows2new_lows, ModOut& out) {
   LO nkeys = keys2kds.size();
   LO nold = old.nents[ent_dim];
@@ -269,7 +269,6 @@ void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
   out.old2new = fill_lo(c, nold, -1);
   OSH_LAUNCH(same_numbers, nsame, c, out.same2old.data(), offsets.data(), out.same2new.data(),
       out.old2new.data());
-  LO nprods = last(c, keys2prods);
   out.prods2new = LOs(c, nprods);
   OSH_LAUNCH(prod_numbers, nkeys, c, reps.data(), keys2kds.data(), offsets.data(),
       order.exists() ? order.data() : (LO const*)nullptr, keys2prods.data(),
@@ -346,17 +345,30 @@ void transfer_measures(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo) {
   }
 }
 
-Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& cands) {
+Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& cands,
+    bool* any_cands) {
   LO n = m.nents[d];
-  if (d == m.dim) return cands;
+  if (d == m.dim) {  // src/Omega_h_indset.cpp:29
+    if (any_cands) *any_cands = max_i8(c, cands) > 0;
+    return cands;
+  }
   Adj const& g = ask_star(c, m, d);
   Bytes a(c, n), b(c, n);
   OSH_LAUNCH(indset_init, n, c, cands.data(), a.data());
-  for (LO round = 0; max_i8(c, a) == 2; ++round) {
+  // a round leaves decided nodes as they are, so rounds after convergence
+  // change nothing: check for undecided nodes every second round only
+  bool first = true;
+  for (LO round = 0;;) {
+    bool undecided = max_i8(c, a) == 2;
+    if (first && any_cands) *any_cands = undecided;
+    first = false;
+    if (!undecided) break;
     OSH_CHECK(round <= n + 1, "independent set does not converge");
-    OSH_LAUNCH(indset_iteration, n, c, g.a2ab.data(), g.ab2b.data(), quality.data(), a.data(),
-        b.data());
-    std::swap(a, b);
+    for (int k = 0; k < 2; ++k, ++round) {
+      OSH_LAUNCH(indset_iteration, n, c, g.a2ab.data(), g.ab2b.data(), quality.data(),
+          a.data(), b.data());
+      std::swap(a, b);
+    }
   }
   return a;
 }

@@ -32,7 +32,7 @@ struct ModOut {
 };
 
 void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
-    LOs const& keys2kds, LOs const& keys2prods, LOs const& prod_verts,
+    LOs const& keys2kds, LOs const& keys2prods, LO nprods, LOs const& prod_verts,
     LOs const& old_lows2new_lows, ModOut& out);
 
 // new tag data: same entities copy old values, products take prod values
@@ -46,7 +46,9 @@ void transfer_class(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo,
 // lengths (d == 1) and qualities (d == dim) of products on the new mesh
 void transfer_measures(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo);
 
-Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& cands);
+// *any_cands (when given) tells whether any entity was a candidate
+Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& cands,
+    bool* any_cands = nullptr);
 Bytes mark_down(Ctx& c, DMesh& m, int high, int low, Bytes const& marks);
 Bytes mark_sliver_layers(Ctx& c, DMesh& m, Real qual_ceil, int nlayers);
 LOs degrees_of(Ctx& c, LOs const& a2ab, LOs const& rows);

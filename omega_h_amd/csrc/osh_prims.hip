@@ -62,10 +62,8 @@ void Pool::release_all() {
 Ctx::Ctx() : pool(std::make_shared<Pool>()) {
   OSH_HIP(hipGetDevice(&device));
   OSH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  OSH_HIP(hipHostMalloc(&pinned, 64, hipHostMallocDefault));
-  OSH_HIP(hipMalloc((void**)&counter, sizeof(unsigned)));
-  unsigned zero = 0;
-  OSH_HIP(hipMemcpy(counter, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  OSH_HIP(hipHostMalloc(&pinned, 8 * kSlots + 64, hipHostMallocDefault));
+  std::memset(pinned, 0, 8 * kSlots + 64);
   OSH_HIP(hipMalloc(&partials, 1024 * sizeof(double)));
 }
 
@@ -75,12 +73,21 @@ Ctx::~Ctx() {
     (void)hipStreamDestroy(stream);
   }
   if (pool) pool->close();
-  if (counter) (void)hipFree(counter);
   if (partials) (void)hipFree(partials);
   if (pinned) (void)hipHostFree(pinned);
 }
 
-void Ctx::sync() { OSH_HIP(hipStreamSynchronize(stream)); }
+void Ctx::sync() {
+  OSH_HIP(hipStreamSynchronize(stream));
+  int* err = error_word();
+  if (*err != kNoError) {
+    int code = *err;
+    *err = kNoError;
+    if (code == kReflectDownNoMatch)
+      throw Error("reflect_down: entity without a unique match");
+    throw Error("device error " + std::to_string(code));
+  }
+}
 
 namespace {
 
@@ -211,14 +218,12 @@ __device__ __forceinline__ T combine(T a, T b) {
   return a + b;
 }
 
-// one launch: every block reduces a grid-stride slice into partials, and
-// the block that finishes last (by the context's counter, which it resets)
-// reduces the partials and writes the result to host-visible memory
+// grid-stride partial reductions into out[blockIdx.x]; a one-block launch
+// over the partials then writes the result into host-visible memory (the
+// kernel boundary orders the two, so no device-wide fences are needed)
 template <class T, int Op>
-__global__ void __launch_bounds__(kBlock) reduce_all(LO n, T const* in, T init, T* partials,
-    unsigned* counter, T* result) {
+__global__ void __launch_bounds__(kBlock) reduce_blocks(LO n, T const* in, T* out, T init) {
   __shared__ T buf[kBlock];
-  __shared__ bool is_last;
   T acc = init;
   for (LO i = LO(blockIdx.x) * kBlock + LO(threadIdx.x); i < n; i += LO(gridDim.x) * kBlock)
     acc = combine<T, Op>(acc, in[i]);
@@ -228,36 +233,27 @@ __global__ void __launch_bounds__(kBlock) reduce_all(LO n, T const* in, T init, 
     if (int(threadIdx.x) < s) buf[threadIdx.x] = combine<T, Op>(buf[threadIdx.x], buf[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&partials[blockIdx.x], buf[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned done = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (done == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  acc = init;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
-    acc = combine<T, Op>(acc, __hip_atomic_load(&partials[b], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
-  buf[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (int(threadIdx.x) < s) buf[threadIdx.x] = combine<T, Op>(buf[threadIdx.x], buf[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    *result = buf[0];
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (threadIdx.x == 0) out[blockIdx.x] = buf[0];
 }
 
 template <class T, int Op>
 T reduce(Ctx& c, DArray<T> const& a, T init) {
   LO n = a.size();
   if (n == 0) return init;
-  LO nb = LO(nblocks(n));
+  LO nb = LO(nblocks(n, kBlock * 4));
   if (nb > 1024) nb = 1024;
-  hipLaunchKernelGGL((reduce_all<T, Op>), dim3(nb), dim3(kBlock), 0, c.stream, n, a.data(), init,
-      static_cast<T*>(c.partials), c.counter, static_cast<T*>(c.pinned));
+  T* result = static_cast<T*>(c.pinned);
+  if (nb == 1) {
+    hipLaunchKernelGGL((reduce_blocks<T, Op>), dim3(1), dim3(kBlock), 0, c.stream, n, a.data(),
+        result, init);
+  } else {
+    T* part = static_cast<T*>(c.partials);
+    hipLaunchKernelGGL((reduce_blocks<T, Op>), dim3(nb), dim3(kBlock), 0, c.stream, n, a.data(),
+        part, init);
+    OSH_HIP(hipGetLastError());
+    hipLaunchKernelGGL((reduce_blocks<T, Op>), dim3(1), dim3(kBlock), 0, c.stream, nb, part,
+        result, init);
+  }
   OSH_HIP(hipGetLastError());
   return pinned_value<T>(c);
 }
@@ -274,6 +270,10 @@ LOs offset_scan(Ctx& c, LOs const& counts, LO* total) {
   LOs out = offset_scan_t<LO>(c, counts, total ? static_cast<LO*>(c.pinned) : nullptr);
   if (total) *total = pinned_value<LO>(c);
   return out;
+}
+LOs offset_scan_async(Ctx& c, LOs const& counts, int slot) {
+  OSH_CHECK(slot >= 0 && slot < Ctx::kSlots, "pinned slot out of range");
+  return offset_scan_t<LO>(c, counts, slot_ptr<LO>(c, slot));
 }
 LOs offset_scan_bytes(Ctx& c, Bytes const& counts, LO* total) {
   LOs out = offset_scan_t<I8>(c, counts, total ? static_cast<LO*>(c.pinned) : nullptr);

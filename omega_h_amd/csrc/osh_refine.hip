@@ -206,15 +206,16 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
   ask_qualities(c, m);
   Bytes cand(c, ne);
   OSH_LAUNCH(gt_kernel, ne, c, lengths.data(), o.max_length_desired, cand.data());
-  if (max_i8(c, cand) != 1) return false;
   LOs cands = collect_marked(c, cand);
+  if (cands.size() == 0) return false;
   Reals quals = refine_qualities(c, m, cands);
   Bytes initial = fill_i8(c, ne, 0);
   Reals edge_quals = fill_real(c, ne, 0.0);
   OSH_LAUNCH(seed_keys, cands.size(), c, cands.data(), quals.data(), o.min_quality_allowed,
       initial.data(), edge_quals.data());
-  if (max_i8(c, initial) != 1) return false;
-  Bytes keys = find_indset(c, m, 1, edge_quals, initial);
+  bool any;
+  Bytes keys = find_indset(c, m, 1, edge_quals, initial, &any);
+  if (!any) return false;
   LOs keys2edges = collect_marked(c, keys);
   LO nkeys = keys2edges.size();
   int nc = m.metric_ncomps;
@@ -225,6 +226,7 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
   for (int d = 0; d <= dim; ++d) {
     LOs k2p, pv, psrc;
     Bytes psrc_dim;
+    LO nprods = nkeys;
     if (d == 0) {
       k2p = iota(c, nkeys + 1);
     } else {
@@ -234,7 +236,6 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
       OSH_LAUNCH(prod_counts, nkeys, c, keys2edges.data(),
           pairs ? pairs->a2ab.data() : (LO const*)nullptr,
           cuts ? cuts->a2ab.data() : (LO const*)nullptr, counts.data());
-      LO nprods;
       k2p = offset_scan(c, counts, &nprods);
       pv = LOs(c, nprods * (d + 1));
       psrc = LOs(c, nprods);
@@ -259,7 +260,7 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
 #undef OSH_RP
     }
     ModOut mo;
-    modify_ents_adapt(c, m, nw, d, 1, keys2edges, k2p, pv, old_lows2new_lows, mo);
+    modify_ents_adapt(c, m, nw, d, 1, keys2edges, k2p, nprods, pv, old_lows2new_lows, mo);
     if (d == 0) {
       keys2mid = mo.prods2new;
       o2n = mo.old2new;

@@ -74,12 +74,16 @@ class Pool {
 struct Ctx {
   hipStream_t stream = nullptr;
   int device = 0;
-  // pinned host words that kernels write results into (totals, reductions)
+  // pinned host words that kernels write results into: 8-byte slots
+  // 0..kSlots-1 (totals, reductions) and one error word that kernels set
+  // when they find a broken invariant; sync() raises it
+  static constexpr int kSlots = 16;
   void* pinned = nullptr;
-  // device words for single-launch reductions (a completion counter that the
-  // last block resets) and their per-block partials
-  unsigned* counter = nullptr;
+  int* error_word() const { return reinterpret_cast<int*>(static_cast<char*>(pinned) + 8 * kSlots); }
+  // per-block partials of reductions
   void* partials = nullptr;
+  // device copy of the edge-swap tables, uploaded on first use
+  std::shared_ptr<void> swap_tables;
   std::shared_ptr<Pool> pool;
   Ctx();
   ~Ctx();
@@ -140,14 +144,26 @@ inline std::vector<T> to_host(Ctx& c, DArray<T> const& d) {
   return h;
 }
 
-// a value that a kernel wrote into c.pinned, after draining the stream
+// a value that a kernel wrote into a pinned slot; read_slot assumes the
+// stream has been drained, pinned_value drains it first
 template <class T>
-inline T pinned_value(Ctx& c, int slot = 0) {
-  c.sync();
+inline T read_slot(Ctx& c, int slot) {
   T v;
   std::memcpy(&v, static_cast<char*>(c.pinned) + 8 * slot, sizeof(T));
   return v;
 }
+template <class T>
+inline T pinned_value(Ctx& c, int slot = 0) {
+  c.sync();
+  return read_slot<T>(c, slot);
+}
+template <class T>
+inline T* slot_ptr(Ctx& c, int slot) {
+  return reinterpret_cast<T*>(static_cast<char*>(c.pinned) + 8 * slot);
+}
+
+// error codes kernels leave in Ctx::error_word()
+enum DeviceError { kNoError = 0, kReflectDownNoMatch = 1 };
 
 template <class T>
 inline T get_value(Ctx& c, T const* dptr) {
@@ -183,9 +199,25 @@ __device__ __forceinline__ LO thread_index() {
   return LO(blockIdx.x) * LO(blockDim.x) + LO(threadIdx.x);
 }
 
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
+// with its own L2.  For gather kernels over locality-ordered entities this
+// renumbers the workgroups so that each XCD walks one contiguous range and
+// neighbouring elements share that XCD's L2 (a bijection of [0, gridDim.x)).
+constexpr unsigned kNumXcds = 8;
+__device__ __forceinline__ LO xcd_thread_index() {
+  unsigned nb = gridDim.x, b = blockIdx.x;
+  unsigned per = nb / kNumXcds, rem = nb % kNumXcds;
+  unsigned x = b % kNumXcds, k = b / kNumXcds;
+  unsigned start = x * per + (x < rem ? x : rem);
+  return LO(start + k) * LO(blockDim.x) + LO(threadIdx.x);
+}
+
 // ---- primitives (osh_prims.hip) ----
 // n+1 exclusive offsets; with `total`, also the sum (read back after a sync)
 LOs offset_scan(Ctx& c, LOs const& counts, LO* total = nullptr);
+// without a sync: the total lands in pinned slot `slot` (read it with
+// read_slot after the next sync)
+LOs offset_scan_async(Ctx& c, LOs const& counts, int slot);
 LOs offset_scan_bytes(Ctx& c, Bytes const& counts, LO* total = nullptr);
 LOs collect_marked(Ctx& c, Bytes const& marks);       // indices of nonzero
 LOs iota(Ctx& c, LO n, LO start = 0, LO stride = 1);

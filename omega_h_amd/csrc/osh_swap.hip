@@ -366,15 +366,21 @@ __global__ void swap2d_products(LO nkeys, LO const* keys2edges, LO const* e2t_a2
   }
 }
 
-DArray<unsigned char> upload_tables(Ctx& c) {
-  Tables t = swap::make_tables();
-  DArray<unsigned char> d(c, LO(sizeof(Tables)));
-  upload(c, d, reinterpret_cast<unsigned char const*>(&t), LO(sizeof(Tables)));
-  c.sync();
-  return d;
+// the context keeps one device copy of the tables
+Tables const* device_tables(Ctx& c) {
+  if (!c.swap_tables) {
+    Tables t = swap::make_tables();
+    auto d = std::make_shared<DArray<unsigned char>>(c, LO(sizeof(Tables)));
+    upload(c, *d, reinterpret_cast<unsigned char const*>(&t), LO(sizeof(Tables)));
+    c.sync();
+    c.swap_tables = d;
+  }
+  return reinterpret_cast<Tables const*>(
+      static_cast<DArray<unsigned char>*>(c.swap_tables.get())->data());
 }
 
-void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LOs const* pv) {
+void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO const* nprods,
+    LOs const* pv) {
   int dim = m.dim;
   LO nv = m.nents[0];
   DMesh nw;
@@ -389,7 +395,8 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LOs c
   LO nkeys = keys2edges.size();
   for (int d = 1; d <= dim; ++d) {
     ModOut mo;
-    modify_ents_adapt(c, m, nw, d, 1, keys2edges, k2p[d], pv[d], old_lows2new_lows, mo);
+    modify_ents_adapt(c, m, nw, d, 1, keys2edges, k2p[d], nprods[d], pv[d], old_lows2new_lows,
+        mo);
     LOs src(c, mo.prods2new.size());
     OSH_LAUNCH(key_edge_sources, nkeys, c, keys2edges.data(), k2p[d].data(), src.data());
     transfer_class(c, m, nw, d, mo, src, 1, nullptr);
@@ -410,12 +417,11 @@ Bytes swap_candidates(Ctx& c, DMesh& m, AdaptOptions const& o) {
 bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   ask_lengths(c, m);
   Bytes cand = swap_candidates(c, m, o);
-  if (max_i8(c, cand) <= 0) return false;
   LOs cands = collect_marked(c, cand);
+  if (cands.size() == 0) return false;
   LO ne = m.nents[1];
   int md = (m.metric_ncomps == 1) ? 1 : 3;
-  auto tables = upload_tables(c);
-  Tables const* tb = reinterpret_cast<Tables const*>(tables.data());
+  Tables const* tb = device_tables(c);
   Bytes configs = fill_i8(c, ne, -1);
   Bytes keep = fill_i8(c, ne, 0);
   Reals quals = fill_real(c, ne, -1.0);
@@ -428,9 +434,10 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   OSH_LAUNCH(swap3d_loops, ncands, c, cands.data(), e2t.a2ab.data(), e2t.ab2b.data(),
       e2t.codes.data(), m.down[1].data(), tv2v.data(), tb, lsize.data(), leev.data(),
       lverts.data(), ntris.data(), nedges.data());
-  LO ntp, nep;
-  LOs toff = offset_scan(c, ntris, &ntp);
-  LOs eoff = offset_scan(c, nedges, &nep);
+  LOs toff = offset_scan_async(c, ntris, 1);
+  LOs eoff = offset_scan_async(c, nedges, 2);
+  c.sync();
+  LO ntp = read_slot<LO>(c, 1), nep = read_slot<LO>(c, 2);
   Reals tq(c, ntp), el(c, nep);
   if (md == 3) {
     OSH_LAUNCH(swap3d_tri_quals<3>, ntp, c, ncands, toff.data(), lsize.data(), leev.data(),
@@ -446,31 +453,36 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   OSH_LAUNCH(swap3d_choose, ncands, c, cands.data(), lsize.data(), toff.data(), eoff.data(),
       tq.data(), el.data(), e2t.a2ab.data(), e2t.ab2b.data(), eq.data(), o.max_length_allowed,
       tb, configs.data(), quals.data(), keep.data());
-  if (max_i8(c, keep) <= 0) return false;
-  Bytes keys = find_indset(c, m, 1, quals, keep);
+  bool any;
+  Bytes keys = find_indset(c, m, 1, quals, keep, &any);
+  if (!any) return false;
   LOs keys2edges = collect_marked(c, keys);
   LO nkeys = keys2edges.size();
   LOs ce(c, nkeys), cf(c, nkeys), ct(c, nkeys);
   OSH_LAUNCH(swap3d_counts, nkeys, c, keys2edges.data(), e2t.a2ab.data(), tb, ce.data(),
       cf.data(), ct.data());
   LOs k2p[4], pv[4];
-  LO nprods[4];
-  k2p[1] = offset_scan(c, ce, &nprods[1]);
-  k2p[2] = offset_scan(c, cf, &nprods[2]);
-  k2p[3] = offset_scan(c, ct, &nprods[3]);
-  for (int d = 1; d <= 3; ++d) pv[d] = LOs(c, nprods[d] * (d + 1));
+  LO nprods[4] = {0, 0, 0, 0};
+  k2p[1] = offset_scan_async(c, ce, 1);
+  k2p[2] = offset_scan_async(c, cf, 2);
+  k2p[3] = offset_scan_async(c, ct, 3);
+  c.sync();
+  for (int d = 1; d <= 3; ++d) {
+    nprods[d] = read_slot<LO>(c, d);
+    pv[d] = LOs(c, nprods[d] * (d + 1));
+  }
   OSH_LAUNCH(swap3d_products, nkeys, c, keys2edges.data(), configs.data(), e2t.a2ab.data(),
       e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), tb, k2p[1].data(),
       k2p[2].data(), k2p[3].data(), pv[1].data(), pv[2].data(), pv[3].data());
-  swap_rebuild(c, m, keys2edges, k2p, pv);
+  swap_rebuild(c, m, keys2edges, k2p, nprods, pv);
   return true;
 }
 
 bool swap_edges_2d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   ask_lengths(c, m);
   Bytes cand = swap_candidates(c, m, o);
-  if (max_i8(c, cand) <= 0) return false;
   LOs cands = collect_marked(c, cand);
+  if (cands.size() == 0) return false;
   LO ne = m.nents[1];
   int md = (m.metric_ncomps == 1) ? 1 : 2;
   Bytes keep = fill_i8(c, ne, 0);
@@ -486,18 +498,20 @@ bool swap_edges_2d(Ctx& c, DMesh& m, AdaptOptions const& o) {
     OSH_LAUNCH(swap2d_qualities_kernel<1>, cands.size(), c, cands.data(), e2t.a2ab.data(),
         e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), m.coords.data(),
         m.metric.data(), eq.data(), o.max_length_allowed, keep.data(), quals.data());
-  if (max_i8(c, keep) <= 0) return false;
-  Bytes keys = find_indset(c, m, 1, quals, keep);
+  bool any;
+  Bytes keys = find_indset(c, m, 1, quals, keep, &any);
+  if (!any) return false;
   LOs keys2edges = collect_marked(c, keys);
   LO nkeys = keys2edges.size();
   LOs k2p[4], pv[4];
   k2p[1] = iota(c, nkeys + 1);
   k2p[2] = iota(c, nkeys + 1, 0, 2);
+  LO nprods[3] = {0, nkeys, 2 * nkeys};
   pv[1] = LOs(c, nkeys * 2);
   pv[2] = LOs(c, nkeys * 6);
   OSH_LAUNCH(swap2d_products, nkeys, c, keys2edges.data(), e2t.a2ab.data(), e2t.ab2b.data(),
       e2t.codes.data(), m.down[1].data(), tv2v.data(), pv[1].data(), pv[2].data());
-  swap_rebuild(c, m, keys2edges, k2p, pv);
+  swap_rebuild(c, m, keys2edges, k2p, nprods, pv);
   return true;
 }
 
