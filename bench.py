@@ -83,6 +83,37 @@ def kernel_rooflines(ctx, mesh, launches):
     return out
 
 
+def roofline_only(ctx, mesh, launches):
+    import numpy as np
+    from omega_h_amd.hip import DeviceArray
+    for r in kernel_rooflines(ctx, mesh, launches).values():
+        log(f"{r['kernel']}: {r['launch_ms'] * 1e3:.1f} us per launch, {r['bytes']} B, "
+            f"{r['achieved']:.0f} GB/s")
+    n = COPY_BYTES // 8
+    a = DeviceArray(np.arange(n, dtype=np.float64))
+    b = DeviceArray(dtype=np.float64, n=n)
+    for _ in range(5):
+        b.copy_from(a)
+    log(f"copied {COPY_BYTES} B 5 times")
+
+
+def traffic_per_launch(kernel):
+    """HBM bytes per launch of `kernel` over the whole 64^3 input, from the
+    counter summary in profiles/ (FETCH_SIZE + WRITE_SIZE, each scaled by
+    what the counter reads for a device copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "measure_kernels_64cube_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
+COPY_BYTES = 64 << 20
+
+
 def cpu_baseline(n, seconds):
     """Oracle passes, in adapt() order, on the same workload until `seconds`
     of pass time have been spent."""
@@ -123,6 +154,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-launches", type=int, default=20)
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the measure-kernel sweeps and a 64 MiB device copy (the "
+                         "reference kernel for calibrating HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -139,6 +173,9 @@ def main():
     opts = ctx.default_opts(3)
     nelems0 = base.nents(3)
     log(f"input: {args.n}^3 box, {nelems0} tets, {base.nents(0)} verts")
+    if args.roofline_only:
+        roofline_only(ctx, base, args.roofline_launches)
+        return
 
     copies = [base.copy() for _ in range(args.warmup + args.steps)]
     for i in range(args.warmup):
@@ -192,7 +229,11 @@ def main():
                    "passes_per_step": len(runs[-1]), "final_tets": final},
         "roofline": {"bound": "hbm", "achieved": roof["achieved"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": roof["achieved"] / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": roof["kernel"],
+                     "traffic": traffic_per_launch(roof["kernel"]) if args.n == 64 else None,
+                     "kernel": roof["kernel"],
+                     # the counters show HBM traffic within 6% of the algorithmic
+                     # bytes; the kernel is bound by its FP64 instruction issue
+                     "binding": "fp64 valu",
                      "launch_us": roof["launch_ms"] * 1e3, "bytes": roof["bytes"]},
         "cpu_baseline": None,
     }

@@ -17,6 +17,7 @@ _rt = None
 
 H2D = 1
 D2H = 2
+D2D = 3
 
 
 def runtime():
@@ -64,6 +65,13 @@ class DeviceArray:
         self.ptr = p.value
         if host is not None and self.nbytes:
             _check(rt.hipMemcpy(self.ptr, host.ctypes.data, self.nbytes, H2D), "hipMemcpy")
+
+    def copy_from(self, other):
+        """Device-to-device copy of `other` (same byte size) into this array."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        if self.nbytes:
+            _check(runtime().hipMemcpy(self.ptr, other.ptr, self.nbytes, D2D), "hipMemcpy")
 
     def numpy(self):
         out = np.empty(self.n, dtype=self.dtype)
