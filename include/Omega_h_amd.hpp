@@ -1,0 +1,445 @@
+// C++ drop-in facade of the MI355X adapt() path (header only, over the C ABI
+// of omega_h_amd.h).
+//
+// It keeps the names and call shapes Omega_h callers use on this path:
+//   Omega_h::Library, Omega_h::Mesh           src/Omega_h_library.hpp,
+//                                             src/Omega_h_mesh.hpp:36-170
+//   Omega_h::Read<T> / Write<T> / HostWrite    src/Omega_h_array.hpp:21-204
+//   Omega_h::AdaptOpts, Omega_h::adapt         src/Omega_h_adapt.hpp:44-82
+//   Omega_h::build_box                         src/Omega_h_build.hpp:24
+//   refine_by_size, coarsen_by_size, coarsen_slivers, swap_edges
+// so that a call site such as src/aniso_test.cpp:15-27 compiles against it
+// by changing the include.  The mesh lives on the GPU behind an osh_mesh
+// handle; Read<T>/Write<T> here are reference-counted host arrays that cross
+// the boundary when a tag is set or read.  Errors throw Omega_h::exception,
+// the reference's behaviour under Omega_h_THROW (src/Omega_h_fail.hpp:33-83).
+#ifndef OMEGA_H_AMD_HPP
+#define OMEGA_H_AMD_HPP
+
+#include <cstdint>
+#include <initializer_list>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "omega_h_amd.h"
+
+enum Omega_h_Family { OMEGA_H_SIMPLEX = 0 };
+
+namespace Omega_h {
+
+using Int = std::int32_t;
+using LO = std::int32_t;
+using GO = std::int64_t;
+using I8 = std::int8_t;
+using Real = double;
+
+enum { VERT = 0, EDGE = 1, FACE = 2, REGION = 3 };
+
+enum Verbosity { SILENT = 0, EACH_ADAPT = 1, EACH_REBUILD = 2, EXTRA_STATS = 3 };
+
+struct exception : public std::runtime_error {
+  explicit exception(std::string const& msg) : std::runtime_error(msg) {}
+};
+
+inline void check_status(int status) {
+  if (status != 0) throw exception(osh_last_error());
+}
+
+template <class T>
+class Write {
+ public:
+  Write() = default;
+  explicit Write(LO size, std::string const& = "")
+      : data_(std::make_shared<std::vector<T>>(std::size_t(size))) {}
+  Write(LO size, T value, std::string const& = "")
+      : data_(std::make_shared<std::vector<T>>(std::size_t(size), value)) {}
+  Write(std::initializer_list<T> l, std::string const& = "")
+      : data_(std::make_shared<std::vector<T>>(l)) {}
+  explicit Write(std::vector<T> v) : data_(std::make_shared<std::vector<T>>(std::move(v))) {}
+  LO size() const { return data_ ? LO(data_->size()) : 0; }
+  bool exists() const { return bool(data_); }
+  T* data() const { return data_ ? data_->data() : nullptr; }
+  T& operator[](LO i) const { return (*data_)[std::size_t(i)]; }
+  void set(LO i, T v) const { (*data_)[std::size_t(i)] = v; }
+  T get(LO i) const { return (*data_)[std::size_t(i)]; }
+
+ private:
+  template <class U>
+  friend class Read;
+  std::shared_ptr<std::vector<T>> data_;
+};
+
+// the reference's host-side arrays are the same thing here
+template <class T>
+using HostWrite = Write<T>;
+
+template <class T>
+class Read {
+ public:
+  Read() = default;
+  Read(Write<T> w) : data_(w.data_) {}
+  Read(std::initializer_list<T> l, std::string const& = "")
+      : data_(std::make_shared<std::vector<T>>(l)) {}
+  Read(LO size, T value, std::string const& = "")
+      : data_(std::make_shared<std::vector<T>>(std::size_t(size), value)) {}
+  LO size() const { return data_ ? LO(data_->size()) : 0; }
+  bool exists() const { return bool(data_); }
+  T const* data() const { return data_ ? data_->data() : nullptr; }
+  T const& operator[](LO i) const { return (*data_)[std::size_t(i)]; }
+  T get(LO i) const { return (*data_)[std::size_t(i)]; }
+  T last() const { return data_->back(); }
+
+ private:
+  std::shared_ptr<std::vector<T>> data_;
+};
+
+template <class T>
+using HostRead = Read<T>;
+
+using LOs = Read<LO>;
+using GOs = Read<GO>;
+using Reals = Read<Real>;
+using Bytes = Read<I8>;
+
+// up/down adjacency (src/Omega_h_adj.hpp:11-49): down has ab2b and codes,
+// up adds the a2ab offsets
+struct Adj {
+  LOs a2ab;
+  LOs ab2b;
+  Bytes codes;
+};
+struct Graph {
+  LOs a2ab;
+  LOs ab2b;
+};
+
+class Library;
+
+// stands in for the communicator of build_box(lib.world(), ...): one rank
+struct Comm {
+  Library* library;
+};
+using CommPtr = std::shared_ptr<Comm>;
+
+// one HIP stream on one GPU (the device is the only argument that matters)
+class Library {
+ public:
+  explicit Library(int* argc = nullptr, char*** argv = nullptr, int device = 0) {
+    (void)argc;
+    (void)argv;
+    check_status(osh_ctx_create(device, &ctx_));
+  }
+  Library(Library const&) = delete;
+  Library& operator=(Library const&) = delete;
+  ~Library() { osh_ctx_destroy(ctx_); }
+  osh_ctx* ctx() const { return ctx_; }
+  CommPtr world() { return std::make_shared<Comm>(Comm{this}); }
+  CommPtr self() { return world(); }
+
+ private:
+  osh_ctx* ctx_ = nullptr;
+};
+
+class Mesh {
+ public:
+  Mesh() = default;
+  explicit Mesh(Library* library) : library_(library) {}
+  // wraps a handle this object now owns
+  Mesh(Library* library, osh_mesh* h)
+      : library_(library), h_(h, [](osh_mesh* m) { osh_mesh_destroy(m); }) {}
+  // copies share the device arrays; passes build new ones (like assigning
+  // an Omega_h::Mesh)
+  Mesh(Mesh const& o) : library_(o.library_) {
+    if (o.h_) {
+      osh_mesh* c = nullptr;
+      check_status(osh_mesh_copy(o.h_.get(), &c));
+      h_.reset(c, [](osh_mesh* m) { osh_mesh_destroy(m); });
+    }
+  }
+  Mesh& operator=(Mesh const& o) {
+    if (this != &o) {
+      Mesh t(o);
+      library_ = t.library_;
+      h_ = std::move(t.h_);
+    }
+    return *this;
+  }
+  Mesh(Mesh&&) = default;
+  Mesh& operator=(Mesh&&) = default;
+
+  Library* library() const { return library_; }
+  osh_mesh* handle() const {
+    if (!h_) throw exception("mesh has no entities yet");
+    return h_.get();
+  }
+  Omega_h_Family family() const { return OMEGA_H_SIMPLEX; }
+  Int dim() const {
+    int d;
+    check_status(osh_mesh_dim(handle(), &d));
+    return d;
+  }
+  LO nents(Int ent_dim) const {
+    int n;
+    check_status(osh_mesh_nents(handle(), ent_dim, &n));
+    return n;
+  }
+  LO nverts() const { return nents(VERT); }
+  LO nedges() const { return nents(EDGE); }
+  LO nfaces() const { return nents(FACE); }
+  LO nregions() const { return nents(REGION); }
+  LO nelems() const { return nents(dim()); }
+  GO nglobal_ents(Int ent_dim) const { return nents(ent_dim); }
+  LO nents_owned(Int ent_dim) const { return nents(ent_dim); }
+
+  Reals coords() const {
+    Write<Real> w(nverts() * dim());
+    check_status(osh_mesh_get_coords(handle(), w.data()));
+    return w;
+  }
+  // one rank: the global number of an entity is its index
+  Read<GO> globals(Int ent_dim) const {
+    Write<GO> w(nents(ent_dim));
+    for (LO i = 0; i < w.size(); ++i) w[i] = i;
+    return w;
+  }
+  LOs ask_verts_of(Int ent_dim) const {
+    Write<LO> w(nents(ent_dim) * (ent_dim + 1));
+    check_status(osh_mesh_get_verts(handle(), ent_dim, w.data()));
+    return w;
+  }
+  LOs ask_elem_verts() const { return ask_verts_of(dim()); }
+  Adj ask_down(Int from, Int to) const {
+    if (to == VERT) return Adj{LOs(), ask_verts_of(from), Bytes()};
+    if (to != from - 1) throw exception("ask_down: only one dimension down (or to vertices)");
+    LO n = nents(from) * (from + 1);
+    Write<LO> ab2b(n);
+    Write<I8> codes(n);
+    check_status(osh_mesh_get_down(handle(), from, ab2b.data(),
+        reinterpret_cast<signed char*>(codes.data())));
+    return Adj{LOs(), ab2b, codes};
+  }
+  Adj ask_up(Int from, Int to) const {
+    int n;
+    check_status(osh_mesh_up_size(handle(), from, to, &n));
+    Write<LO> a2ab(nents(from) + 1), ab2b(n);
+    Write<I8> codes(n);
+    check_status(osh_mesh_get_up(handle(), from, to, a2ab.data(), ab2b.data(),
+        reinterpret_cast<signed char*>(codes.data())));
+    return Adj{a2ab, ab2b, codes};
+  }
+  Graph ask_star(Int ent_dim) const {
+    int n;
+    check_status(osh_mesh_star_size(handle(), ent_dim, &n));
+    Write<LO> a2ab(nents(ent_dim) + 1), ab2b(n);
+    check_status(osh_mesh_get_star(handle(), ent_dim, a2ab.data(), ab2b.data()));
+    return Graph{a2ab, ab2b};
+  }
+  Reals ask_lengths() const {
+    Write<Real> w(nedges());
+    check_status(osh_mesh_get_lengths(handle(), w.data()));
+    return w;
+  }
+  Reals ask_qualities() const {
+    Write<Real> w(nelems());
+    check_status(osh_mesh_get_qualities(handle(), w.data()));
+    return w;
+  }
+  Real min_quality() const {
+    Reals q = ask_qualities();
+    Real r = 1.0;
+    for (LO i = 0; i < q.size(); ++i) r = (q[i] < r) ? q[i] : r;
+    return r;
+  }
+  Real max_length() const {
+    Reals l = ask_lengths();
+    Real r = 0.0;
+    for (LO i = 0; i < l.size(); ++i) r = (l[i] > r) ? l[i] : r;
+    return r;
+  }
+
+  // tags on this path: "coordinates", "metric", "class_id", "class_dim",
+  // "length" (edges), "quality" (elements); only "metric" is set by callers
+  bool has_tag(Int ent_dim, std::string const& name) const {
+    if (name == "coordinates") return ent_dim == VERT;
+    if (name == "metric") {
+      int nc;
+      check_status(osh_mesh_metric_ncomps(handle(), &nc));
+      return ent_dim == VERT && nc > 0;
+    }
+    if (name == "class_id" || name == "class_dim") return ent_dim <= dim();
+    if (name == "length") return ent_dim == EDGE;
+    if (name == "quality") return ent_dim == dim();
+    return false;
+  }
+  template <class T>
+  void add_tag(Int ent_dim, std::string const& name, Int ncomps, Read<T> array,
+      bool internal = false);
+  template <class T>
+  void set_tag(Int ent_dim, std::string const& name, Read<T> array, bool internal = false);
+  template <class T>
+  Read<T> get_array(Int ent_dim, std::string const& name) const;
+
+ private:
+  Library* library_ = nullptr;
+  std::shared_ptr<osh_mesh> h_;
+};
+
+namespace detail {
+inline void set_metric(Mesh& m, Int ent_dim, std::string const& name, Int ncomps,
+    Real const* data, LO size) {
+  if (name != "metric" || ent_dim != VERT)
+    throw exception("only the vertex \"metric\" tag is carried through adapt(), not \"" +
+                    name + "\"");
+  if (size != m.nverts() * ncomps) throw exception("metric array has the wrong size");
+  check_status(osh_mesh_set_metric(m.handle(), ncomps, data));
+}
+}  // namespace detail
+
+template <>
+inline void Mesh::add_tag<Real>(Int ent_dim, std::string const& name, Int ncomps,
+    Read<Real> array, bool) {
+  detail::set_metric(*this, ent_dim, name, ncomps, array.data(), array.size());
+}
+
+template <>
+inline void Mesh::set_tag<Real>(Int ent_dim, std::string const& name, Read<Real> array, bool) {
+  int nc = (nverts() > 0) ? array.size() / nverts() : 0;
+  detail::set_metric(*this, ent_dim, name, nc, array.data(), array.size());
+}
+
+template <>
+inline Read<Real> Mesh::get_array<Real>(Int ent_dim, std::string const& name) const {
+  if (name == "coordinates" && ent_dim == VERT) return coords();
+  if (name == "metric" && ent_dim == VERT) {
+    int nc;
+    check_status(osh_mesh_metric_ncomps(handle(), &nc));
+    Write<Real> w(nverts() * nc);
+    check_status(osh_mesh_get_metric(handle(), w.data()));
+    return w;
+  }
+  if (name == "length" && ent_dim == EDGE) return ask_lengths();
+  if (name == "quality" && ent_dim == dim()) return ask_qualities();
+  throw exception("no Real tag \"" + name + "\" on dimension " + std::to_string(ent_dim));
+}
+
+template <>
+inline Read<LO> Mesh::get_array<LO>(Int ent_dim, std::string const& name) const {
+  if (name != "class_id") throw exception("no LO tag \"" + name + "\"");
+  Write<LO> ids(nents(ent_dim));
+  Write<I8> dims(nents(ent_dim));
+  check_status(osh_mesh_get_class(handle(), ent_dim, ids.data(),
+      reinterpret_cast<signed char*>(dims.data())));
+  return ids;
+}
+
+template <>
+inline Read<I8> Mesh::get_array<I8>(Int ent_dim, std::string const& name) const {
+  if (name != "class_dim") throw exception("no I8 tag \"" + name + "\"");
+  Write<LO> ids(nents(ent_dim));
+  Write<I8> dims(nents(ent_dim));
+  check_status(osh_mesh_get_class(handle(), ent_dim, ids.data(),
+      reinterpret_cast<signed char*>(dims.data())));
+  return dims;
+}
+
+// src/Omega_h_build.hpp:24; nz == 0 builds a 2D box (as the reference does
+// when z == 0)
+inline Mesh build_box(CommPtr comm, Omega_h_Family family, Real x, Real y, Real z, LO nx,
+    LO ny, LO nz, bool symmetric = false) {
+  if (family != OMEGA_H_SIMPLEX) throw exception("only simplex meshes are supported");
+  if (symmetric) throw exception("symmetric boxes are not supported");
+  Library* lib = comm->library;
+  int dim = (nz > 0) ? 3 : 2;
+  osh_mesh* h = nullptr;
+  check_status(osh_build_box(lib->ctx(), dim, x, y, z, nx, ny, nz, &h));
+  return Mesh(lib, h);
+}
+
+// src/Omega_h_adapt.hpp:50-77, defaults src/Omega_h_adapt.cpp:52-85
+struct AdaptOpts {
+  AdaptOpts() = default;
+  explicit AdaptOpts(Int dim) {
+    osh_adapt_opts o;
+    check_status(osh_default_opts(dim, &o));
+    from_c(o);
+  }
+  explicit AdaptOpts(Mesh* mesh) : AdaptOpts(mesh->dim()) {}
+  Real min_length_desired = 0;
+  Real max_length_desired = 0;
+  Real max_length_allowed = 0;
+  Real min_quality_allowed = 0;
+  Real min_quality_desired = 0;
+  Int nsliver_layers = 0;
+  Verbosity verbosity = SILENT;
+  bool should_refine = true;
+  bool should_coarsen = true;
+  bool should_swap = true;
+  bool should_coarsen_slivers = true;
+
+  osh_adapt_opts to_c() const {
+    osh_adapt_opts o;
+    o.min_length_desired = min_length_desired;
+    o.max_length_desired = max_length_desired;
+    o.max_length_allowed = max_length_allowed;
+    o.min_quality_allowed = min_quality_allowed;
+    o.min_quality_desired = min_quality_desired;
+    o.nsliver_layers = nsliver_layers;
+    o.verbosity = int(verbosity);
+    o.should_refine = should_refine;
+    o.should_coarsen = should_coarsen;
+    o.should_swap = should_swap;
+    o.should_coarsen_slivers = should_coarsen_slivers;
+    return o;
+  }
+  void from_c(osh_adapt_opts const& o) {
+    min_length_desired = o.min_length_desired;
+    max_length_desired = o.max_length_desired;
+    max_length_allowed = o.max_length_allowed;
+    min_quality_allowed = o.min_quality_allowed;
+    min_quality_desired = o.min_quality_desired;
+    nsliver_layers = o.nsliver_layers;
+    verbosity = Verbosity(o.verbosity);
+    should_refine = o.should_refine != 0;
+    should_coarsen = o.should_coarsen != 0;
+    should_swap = o.should_swap != 0;
+    should_coarsen_slivers = o.should_coarsen_slivers != 0;
+  }
+};
+
+// src/Omega_h_adapt.cpp:272-288: false iff the mesh already met the goals
+inline bool adapt(Mesh* mesh, AdaptOpts const& opts) {
+  osh_adapt_opts o = opts.to_c();
+  int did = 0;
+  check_status(osh_adapt(mesh->handle(), &o, nullptr, &did));
+  return did != 0;
+}
+
+namespace detail {
+inline bool run_pass(int (*f)(osh_mesh*, const osh_adapt_opts*, int*), Mesh* mesh,
+    AdaptOpts const& opts) {
+  osh_adapt_opts o = opts.to_c();
+  int did = 0;
+  check_status(f(mesh->handle(), &o, &did));
+  return did != 0;
+}
+}  // namespace detail
+
+inline bool refine_by_size(Mesh* mesh, AdaptOpts const& opts) {
+  return detail::run_pass(osh_refine_by_size, mesh, opts);
+}
+inline bool coarsen_by_size(Mesh* mesh, AdaptOpts const& opts) {
+  return detail::run_pass(osh_coarsen_by_size, mesh, opts);
+}
+inline bool coarsen_slivers(Mesh* mesh, AdaptOpts const& opts) {
+  return detail::run_pass(osh_coarsen_slivers, mesh, opts);
+}
+inline bool swap_edges(Mesh* mesh, AdaptOpts const& opts) {
+  return detail::run_pass(osh_swap_edges, mesh, opts);
+}
+
+}  // namespace Omega_h
+
+#endif

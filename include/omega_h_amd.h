@@ -110,11 +110,16 @@ int osh_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats
     int* did);
 
 /* measure_edges_metric, src/Omega_h_shape.cpp:7-48, on device pointers:
- * out[i] = metric length of edge edges[i] (or i when edges is NULL) */
+ * out[i] = metric length of edge edges[i] (or i when edges is NULL).
+ * Asynchronous: ordered on the context's stream (osh_ctx_stream), so the
+ * caller synchronises before reading out.  ev2v must be 8-byte aligned and a
+ * 6-component metric 16-byte aligned (any hipMalloc pointer is). */
 int osh_measure_edges_metric(osh_ctx* ctx, int dim, int metric_ncomps, int n,
     const int* edges, const int* ev2v, const double* coords,
     const double* metric, double* out);
-/* measure_qualities, src/Omega_h_quality.cpp:7-41, on device pointers */
+/* measure_qualities, src/Omega_h_quality.cpp:7-41, on device pointers;
+ * asynchronous like osh_measure_edges_metric; a tet vertex list must be
+ * 16-byte aligned */
 int osh_measure_qualities(osh_ctx* ctx, int dim, int metric_ncomps, int n,
     const int* elems, const int* ev2v, const double* coords,
     const double* metric, double* out);

@@ -1,0 +1,65 @@
+// Drives adapt() through the C++ facade the way an Omega_h caller does
+// (src/aniso_test.cpp:15-27): build_box, add the vertex metric tag, adapt,
+// read the result back.  Used by tests/test_facade.py.
+//
+//   facade_adapt <dim> <n> <ncomps> <metric.f64> <out_prefix>
+//
+// writes <out_prefix>.counts (int32 nents per dimension, then the adapt()
+// return value), .verts (int32 element vertices), .coords and .quality
+// (float64).
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "Omega_h_amd.hpp"
+
+template <class T>
+static void write_file(std::string const& path, T const* data, std::size_t n) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<char const*>(data), std::streamsize(n * sizeof(T)));
+  if (!f) throw std::runtime_error("cannot write " + path);
+}
+
+int main(int argc, char** argv) {
+  if (argc != 6) {
+    std::cerr << "usage: facade_adapt <dim> <n> <ncomps> <metric.f64> <out_prefix>\n";
+    return 2;
+  }
+  int dim = std::atoi(argv[1]);
+  int n = std::atoi(argv[2]);
+  int ncomps = std::atoi(argv[3]);
+  std::string prefix = argv[5];
+  try {
+    Omega_h::Library lib(&argc, &argv);
+    Omega_h::Mesh mesh = Omega_h::build_box(lib.world(), OMEGA_H_SIMPLEX, 1.0, 1.0,
+        dim == 3 ? 1.0 : 0.0, n, n, dim == 3 ? n : 0);
+    std::vector<double> metric(std::size_t(mesh.nverts()) * ncomps);
+    std::ifstream f(argv[4], std::ios::binary);
+    f.read(reinterpret_cast<char*>(metric.data()), std::streamsize(metric.size() * 8));
+    if (!f) throw std::runtime_error("cannot read the metric file");
+    mesh.add_tag(Omega_h::VERT, "metric", ncomps,
+        Omega_h::Reals(Omega_h::Write<Omega_h::Real>(metric)));
+    Omega_h::AdaptOpts opts(&mesh);
+    opts.verbosity = Omega_h::SILENT;
+    bool did = Omega_h::adapt(&mesh, opts);
+    std::vector<int> counts;
+    for (int d = 0; d <= dim; ++d) counts.push_back(mesh.nents(d));
+    counts.push_back(did ? 1 : 0);
+    write_file(prefix + ".counts", counts.data(), counts.size());
+    auto ev2v = mesh.ask_elem_verts();
+    write_file(prefix + ".verts", ev2v.data(), std::size_t(ev2v.size()));
+    auto coords = mesh.coords();
+    write_file(prefix + ".coords", coords.data(), std::size_t(coords.size()));
+    auto q = mesh.get_array<Omega_h::Real>(dim, "quality");
+    write_file(prefix + ".quality", q.data(), std::size_t(q.size()));
+    std::printf("adapted: %d elements, min quality %.6f\n", mesh.nelems(),
+        mesh.min_quality());
+  } catch (Omega_h::exception const& e) {
+    std::cerr << "Omega_h::exception: " << e.what() << "\n";
+    return 1;
+  }
+  return 0;
+}

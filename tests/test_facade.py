@@ -1,0 +1,52 @@
+"""The C++ drop-in facade (include/Omega_h_amd.hpp).
+
+CPU: the header compiles on its own.  GPU: a C++ program written the way an
+Omega_h caller writes adapt() (tests/cpp/facade_adapt.cpp, built by
+__graft_entry__.build()) gives bit-for-bit the oracle's adapted mesh.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "omega_h_amd", "_build", "facade_adapt")
+
+
+@pytest.mark.parametrize("std", ["c++14", "c++17"])
+def test_facade_header_compiles(tmp_path, std):
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "Omega_h_amd.hpp"\n'
+                   "int main() { Omega_h::AdaptOpts o; (void)o; return 0; }\n")
+    r = subprocess.run(["g++", f"-std={std}", "-Wall", "-Werror", "-fsyntax-only",
+                        "-I" + os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,n", [(2, 16), (3, 6)])
+def test_facade_adapt_matches_oracle(tmp_path, dim, n):
+    from omega_h_amd import workloads as W
+    assert os.path.exists(BIN), "build the facade driver with __graft_entry__.build()"
+    o = O.Mesh.box(dim, n)
+    met, nc = W.iso_ring_metric_2d(o.coords()) if dim == 2 else W.levelset_metric_3d(o.coords())
+    mfile = tmp_path / "metric.f64"
+    np.ascontiguousarray(met, dtype=np.float64).tofile(mfile)
+    prefix = str(tmp_path / "out")
+    print(f"facade adapt dim={dim} n={n}", flush=True)
+    r = subprocess.run([BIN, str(dim), str(n), str(nc), str(mfile), prefix],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    o.set_metric(met, nc)
+    did, _ = o.adapt(O.default_opts(dim))
+    counts = np.fromfile(prefix + ".counts", dtype=np.int32)
+    assert list(counts[:-1]) == [o.nents(d) for d in range(dim + 1)]
+    assert counts[-1] == did
+    assert np.array_equal(np.fromfile(prefix + ".verts", dtype=np.int32), o.verts(dim))
+    for name, want in (("coords", o.coords()), ("quality", o.qualities())):
+        got = np.fromfile(prefix + "." + name, dtype=np.float64)
+        assert np.array_equal(got.view(np.int64), want.view(np.int64)), name

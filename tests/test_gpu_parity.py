@@ -184,3 +184,37 @@ def test_anisotropic_2d_adapt_identical(ctx):
     rg, pg = g.adapt(ctx.default_opts(2))
     assert [p[:3] for p in po] == [p[:3] for p in pg]
     compare_meshes(o, g)
+
+
+@pytest.mark.parametrize("dim,n", [(2, 12), (3, 8)])
+def test_measure_entry_points_on_device_pointers(ctx, dim, n):
+    from omega_h_amd.hip import DeviceArray
+    o, g = boxes(ctx, dim, n)
+    nc = g.metric_ncomps
+    coords, metric = DeviceArray(g.coords()), DeviceArray(g.metric())
+    for d, fn, want in ((dim, ctx.measure_qualities, o.qualities()),
+                        (1, ctx.measure_edges_metric, o.lengths())):
+        nents = g.nents(d)
+        ev2v = DeviceArray(g.verts(d))
+        out = DeviceArray(dtype=np.float64, n=nents)
+        fn(dim, nc, nents, ev2v.ptr, coords.ptr, metric.ptr, out.ptr)
+        ctx.synchronize()  # the entry points are asynchronous on the context's stream
+        assert_same(want, out.numpy(), f"measure dim {d}")
+        # a subset through the entity list
+        sub = np.arange(1, nents, 3, dtype=np.int32)
+        dsub = DeviceArray(sub)
+        out2 = DeviceArray(dtype=np.float64, n=len(sub))
+        fn(dim, nc, len(sub), ev2v.ptr, coords.ptr, metric.ptr, out2.ptr, dsub.ptr)
+        ctx.synchronize()
+        assert_same(want[sub], out2.numpy(), f"measure subset dim {d}")
+
+
+def test_measure_rejects_misaligned_metric(ctx):
+    from omega_h_amd.hip import DeviceArray
+    o, g = boxes(ctx, 3, 2)
+    coords, metric = DeviceArray(g.coords()), DeviceArray(g.metric())
+    ev2v = DeviceArray(g.verts(3))
+    out = DeviceArray(dtype=np.float64, n=g.nents(3))
+    # the check runs on the host before any launch
+    with pytest.raises(A.OshError, match="aligned"):
+        ctx.measure_qualities(3, 6, g.nents(3), ev2v.ptr, coords.ptr, metric.ptr + 8, out.ptr)
