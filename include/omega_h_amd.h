@@ -124,6 +124,18 @@ int osh_measure_qualities(osh_ctx* ctx, int dim, int metric_ncomps, int n,
     const int* elems, const int* ev2v, const double* coords,
     const double* metric, double* out);
 
+/* limit_metric_gradation(mesh, metric, max_rate, tol),
+ * src/Omega_h_metric.cpp:200-220, applied to the mesh's metric in place
+ * (each sweep intersects every vertex metric with its neighbours' scaled
+ * metrics, src/Omega_h_metric.cpp:156-180, until two sweeps are close within
+ * tol); *nsteps = sweeps taken */
+int osh_limit_metric_gradation(osh_mesh* mesh, double max_rate, double tol, int* nsteps);
+/* intersect_metrics, src/Omega_h_metric_intersect.hpp:215-269 (with the
+ * degenerate cases :18-213), for n pairs of symmetric metrics of dimension
+ * dim (1, 2 or 3) in host arrays */
+int osh_intersect_metrics(osh_ctx* ctx, int dim, int n, const double* m1, const double* m2,
+    double* out);
+
 /* cavity kernels on host arrays, for tests and diagnostics:
  * refine_qualities src/Omega_h_refine_qualities.cpp:88-108 */
 int osh_refine_qualities(osh_mesh* mesh, int n, const int* cands, double* out);

@@ -106,6 +106,8 @@ def load(path=None):
         "osh_find_loop": [P, I, IP, IP, IP],
         "osh_find_indset": [P, I, DP, BP, BP],
         "osh_eval_math": [P, I, I, DP, DP],
+        "osh_limit_metric_gradation": [P, C.c_double, C.c_double, IP],
+        "osh_intersect_metrics": [P, I, I, DP, DP, DP],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -168,6 +170,17 @@ class Context:
         o = AdaptOpts()
         check(self.lib.osh_default_opts(dim, C.byref(o)), self.lib)
         return o
+
+    def intersect_metrics(self, dim, m1, m2):
+        """Elementwise intersection of two arrays of symmetric metrics."""
+        a = np.ascontiguousarray(m1, dtype=np.float64)
+        b = np.ascontiguousarray(m2, dtype=np.float64)
+        nc = dim * (dim + 1) // 2
+        n = a.size // nc
+        out = np.zeros_like(a)
+        check(self.lib.osh_intersect_metrics(self.h, dim, n, _p(a, C.c_double), _p(b, C.c_double),
+                                             _p(out, C.c_double)), self.lib)
+        return out
 
     def eval_math(self, which, xs):
         x = np.ascontiguousarray(xs, dtype=np.float64)
@@ -311,6 +324,12 @@ class Mesh:
         check(self.lib.osh_find_indset(self.h, d, _p(q, C.c_double), _p(c, C.c_byte),
                                        _p(out, C.c_byte)), self.lib)
         return out
+
+    def limit_metric_gradation(self, max_rate, tol=1e-2):
+        """Gradation limiting of the mesh's metric; returns the sweep count."""
+        n = C.c_int()
+        check(self.lib.osh_limit_metric_gradation(self.h, max_rate, tol, C.byref(n)), self.lib)
+        return n.value
 
     def _pass(self, f, opts):
         did = C.c_int()

@@ -408,6 +408,32 @@ int osh_measure_qualities(osh_ctx* ctx, int dim, int metric_ncomps, int n, const
   });
 }
 
+int osh_limit_metric_gradation(osh_mesh* mesh, double max_rate, double tol, int* nsteps) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(tol >= 0.0, "tolerance must not be negative");
+    auto& c = mesh->owner->ctx;
+    int steps = osh::limit_metric_gradation(c, mesh->m, max_rate, tol);
+    c.sync();
+    if (nsteps) *nsteps = steps;
+  });
+}
+
+int osh_intersect_metrics(osh_ctx* ctx, int dim, int n, const double* m1, const double* m2,
+    double* out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && (n == 0 || (m1 && m2 && out)), "null argument");
+    OSH_CHECK(dim >= 1 && dim <= 3, "dimension must be 1, 2 or 3");
+    OSH_CHECK(n >= 0, "negative count");
+    auto& c = ctx->ctx;
+    osh::LO nc = (dim * (dim + 1)) / 2;
+    std::vector<double> ha(m1, m1 + std::size_t(n) * nc), hb(m2, m2 + std::size_t(n) * nc);
+    osh::Reals r;
+    osh::intersect_metric_arrays(c, dim, n, osh::to_device(c, ha), osh::to_device(c, hb), r);
+    download(c, r, out);
+  });
+}
+
 int osh_refine_qualities(osh_mesh* mesh, int n, const int* cands, double* out) {
   return guarded([&] {
     check_mesh(mesh);

@@ -53,6 +53,13 @@ void reflect_down(Ctx& c, DMesh& m, int high_dim, LO nprods, LOs const& prod_ver
 // unique lower entities of element vertex lists (src/Omega_h_adj.cpp:133-153)
 LOs find_unique(Ctx& c, LO nhigh, int high_dim, int low_dim, LOs const& hv2v);
 
+// metric preparation (osh_metric.hip): limit_metric_gradation on the mesh's
+// metric, returning the number of sweeps; elementwise intersection of n
+// metrics of dimension dim
+int limit_metric_gradation(Ctx& c, DMesh& m, Real max_rate, Real tol);
+void intersect_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals const& b,
+    Reals& out);
+
 // metric measures (osh_measure.hip)
 void launch_edge_lengths(Ctx& c, int dim, int md, LO n, LO const* edges, LO const* ev2v,
     Real const* coords, Real const* metric, Real* out);

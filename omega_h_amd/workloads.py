@@ -67,3 +67,21 @@ def rotated_constant_metric(nverts, dim, rng):
     else:
         v = np.array([M[0, 0], M[1, 1], M[0, 1]])
     return np.tile(v, nverts), len(v)
+
+
+def banded_aniso_metric(coords, dim, rng):
+    """Anisotropic metric with a fixed random rotation whose two leading
+    eigen-lengths grow away from the plane x = 0.5: h1 = 0.02 + 0.3|x - 0.5|,
+    h2 = 2.5 h1, h3 = 0.15.  Steep enough near the plane for gradation
+    limiting to change it over several sweeps."""
+    q, _ = np.linalg.qr(rng.normal(size=(dim, dim)))
+    p = coords.reshape(-1, dim)
+    h1 = 0.02 + 0.3 * np.abs(p[:, 0] - 0.5)
+    hs = np.stack([h1, 2.5 * h1, np.full_like(h1, 0.15)][:dim], axis=1)
+    lam = 1.0 / (hs * hs)
+    m = np.einsum("ij,nj,kj->nik", q, lam, q)
+    if dim == 3:
+        v = np.stack([m[:, 0, 0], m[:, 1, 1], m[:, 2, 2], m[:, 0, 1], m[:, 1, 2], m[:, 0, 2]], 1)
+    else:
+        v = np.stack([m[:, 0, 0], m[:, 1, 1], m[:, 0, 1]], 1)
+    return v.ravel(), v.shape[1]

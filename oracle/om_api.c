@@ -87,3 +87,27 @@ void om_get_star(om_mesh* m, int d, int* a2ab, int* ab2b) {
   memcpy(a2ab, g->a2ab, sizeof(int) * (size_t)(g->n + 1));
   memcpy(ab2b, g->ab2b, sizeof(int) * (size_t)g->a2ab[g->n]);
 }
+
+/* limit_metric_gradation(mesh, metric, max_rate, tol), src/Omega_h_metric.cpp:200-220,
+ * applied to the mesh's own metric; returns the number of sweeps */
+int om_limit_metric_gradation(om_mesh* m, double max_rate, double tol) {
+  int nc = m->metric_ncomps;
+  int mdim = (nc == 1) ? 1 : m->dim;
+  int nv = m->nents[0];
+  size_t n = (size_t)nv * nc;
+  om_graph const* g = om_ask_star(m, 0);
+  double* values = (double*)om_malloc(sizeof(double) * n);
+  double* values2 = (double*)om_malloc(sizeof(double) * n);
+  memcpy(values2, m->metric, sizeof(double) * n);
+  int steps = 0;
+  do {
+    memcpy(values, values2, sizeof(double) * n);
+    om_limit_gradation_once(m->dim, mdim, nv, g->a2ab, g->ab2b, m->coords, values, max_rate,
+        values2);
+    ++steps;
+  } while (!om_are_close((int)n, values, values2, tol, 1e-10) && steps < 10000);
+  om_set_metric(m, nc, values2);
+  free(values);
+  free(values2);
+  return steps;
+}

@@ -45,6 +45,9 @@ double om_element_quality(int dim, int mdim, double const* const* p,
     double const* const* ms);
 void om_average_metric(int mdim, int n, double const* const* ms, double* out);
 void om_intersect_metrics(int dim, double const* m1, double const* m2, double* out);
+void om_limit_gradation_once(int sdim, int mdim, int nverts, int const* a2ab, int const* ab2b,
+    double const* coords, double const* in, double max_rate, double* out);
+int om_are_close(int n, double const* a, double const* b, double tol, double floor);
 
 /* growable int arrays */
 typedef struct {

@@ -83,6 +83,8 @@ double om_kat_edge_length(int sdim, int mdim, double const* p0,
 double om_kat_aniso_length(double la, double lb);
 void om_kat_average_metric(int mdim, int n, double const* ms, double* out);
 void om_kat_intersect(int dim, double const* m1, double const* m2, double* out);
+/* limit_metric_gradation on the mesh's metric; returns the sweep count */
+int om_limit_metric_gradation(om_mesh* m, double max_rate, double tol);
 void om_kat_refine_qualities(om_mesh* m, int n, int const* cands, double* out);
 int om_kat_find_loop(om_mesh* m, int edge, int* eev2v, int* loop_verts);
 void om_kat_indset(om_mesh* m, int d, double const* q, signed char const* c,

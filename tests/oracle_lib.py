@@ -85,6 +85,7 @@ def lib():
                      "om_coarsen_slivers"):
             getattr(L, name).argtypes = [P, C.POINTER(AdaptOpts)]
         L.om_adapt.argtypes = [P, C.POINTER(AdaptOpts), C.POINTER(AdaptStats)]
+        L.om_limit_metric_gradation.argtypes = [P, C.c_double, C.c_double]
         L.om_default_opts.argtypes = [C.c_int, C.POINTER(AdaptOpts)]
         for name in ("om_log", "om_exp", "om_cbrt", "om_acos", "om_cos"):
             getattr(L, name).restype = C.c_double
@@ -243,6 +244,9 @@ class Mesh:
     def coarsen_slivers(self, opts):
         return lib().om_coarsen_slivers(self.h, C.byref(opts))
 
+    def limit_metric_gradation(self, max_rate, tol=1e-2):
+        return lib().om_limit_metric_gradation(self.h, max_rate, tol)
+
     def adapt(self, opts):
         st = AdaptStats()
         r = lib().om_adapt(self.h, C.byref(opts), C.byref(st))
@@ -309,12 +313,16 @@ def kat_hilbert(X, b, n):
 
 def symm2vector(m):
     """Omega_h's symmetric storage order (src/Omega_h_matrix.hpp:391-431)."""
+    if m.shape[0] == 1:
+        return np.array([m[0, 0]])
     if m.shape[0] == 2:
         return np.array([m[0, 0], m[1, 1], m[0, 1]])
     return np.array([m[0, 0], m[1, 1], m[2, 2], m[0, 1], m[1, 2], m[0, 2]])
 
 
 def vector2symm(v):
+    if len(v) == 1:
+        return np.array([[v[0]]])
     if len(v) == 3:
         return np.array([[v[0], v[2]], [v[2], v[1]]])
     return np.array([[v[0], v[3], v[5]], [v[3], v[1], v[4]], [v[5], v[4], v[2]]])

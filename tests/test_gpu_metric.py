@@ -1,0 +1,66 @@
+"""Metric intersection and gradation limiting on the GPU against the oracle,
+bit for bit (src/Omega_h_metric_intersect.hpp:18-269,
+src/Omega_h_metric.cpp:156-220)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+A = pytest.importorskip("omega_h_amd")
+from omega_h_amd import workloads as W  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return A.Context(0)
+
+
+def metric_pairs(dim, n, rng):
+    """Random SPD pairs; three in four have one or two null eigenvalues so
+    that every degenerate branch is taken."""
+    a, b = [], []
+    for k in range(n):
+        pair = []
+        for _ in range(2):
+            q, _ = np.linalg.qr(rng.normal(size=(dim, dim)))
+            lam = np.exp(rng.uniform(-3, 3, dim))
+            if k % 4 >= 1:
+                lam[rng.integers(dim)] = 0.0
+            if k % 4 >= 2 and dim == 3:
+                lam[rng.integers(dim)] = 0.0
+            pair.append(O.symm2vector(q @ np.diag(lam) @ q.T))
+        a.append(pair[0])
+        b.append(pair[1])
+    return np.concatenate(a), np.concatenate(b)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_intersect_metrics_identical(ctx, dim):
+    a, b = metric_pairs(dim, 2000, np.random.default_rng(dim))
+    got = ctx.intersect_metrics(dim, a, b)
+    nc = dim * (dim + 1) // 2
+    want = np.concatenate([O.kat_intersect(dim, a[i:i + nc], b[i:i + nc])
+                           for i in range(0, len(a), nc)])
+    bad = np.nonzero(got.view(np.int64) != want.view(np.int64))[0]
+    assert len(bad) == 0, (len(bad), bad[:5])
+
+
+@pytest.mark.parametrize("dim,n", [(2, 16), (3, 8)])
+def test_limit_metric_gradation_identical(ctx, dim, n):
+    o = O.Mesh.box(dim, n)
+    g = A.Mesh.box(ctx, dim, n)
+    met, nc = W.banded_aniso_metric(o.coords(), dim, np.random.default_rng(7))
+    o.set_metric(met, nc)
+    g.set_metric(met, nc)
+    so = o.limit_metric_gradation(0.3, 1e-2)
+    sg = g.limit_metric_gradation(0.3, 1e-2)
+    assert so == sg and so > 1
+    assert np.array_equal(o.metric().view(np.int64), g.metric().view(np.int64))
+    # the main metric then adapts identically
+    ro, po = o.adapt(O.default_opts(dim))
+    rg, pg = g.adapt(ctx.default_opts(dim))
+    assert [p[:3] for p in po] == [p[:3] for p in pg]
+    assert np.array_equal(o.verts(dim), g.verts(dim))
+    assert np.array_equal(o.coords().view(np.int64), g.coords().view(np.int64))

@@ -1,0 +1,167 @@
+"""The oracle's metric intersection (all of Barral's degenerate cases) and
+gradation limiting, pinned by closed forms and invariants.
+
+src/Omega_h_metric_intersect.hpp:18-269, src/Omega_h_metric.cpp:156-220.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+
+def inter(dim, a, b):
+    return O.vector2symm(O.kat_intersect(dim, O.symm2vector(np.asarray(a, float)),
+                                         O.symm2vector(np.asarray(b, float))))
+
+
+def rot3(rng):
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    return q
+
+
+@pytest.mark.parametrize("case,a,b,want", [
+    # 3D case 2.a: two rank-1 metrics along the same axis -> the larger one
+    ("2a", np.diag([4, 0, 0]), np.diag([9, 0, 0]), np.diag([9, 0, 0])),
+    # 3D case 2.b: rank-1 along x and y -> diag(a, b, 0)
+    ("2b", np.diag([4, 0, 0]), np.diag([0, 9, 0]), np.diag([4, 9, 0])),
+    # 3D case 3.a: rank 2 and rank 1 in its plane -> elementwise max in the plane
+    ("3a", np.diag([4, 5, 0]), np.diag([7, 0, 0]), np.diag([7, 5, 0])),
+    # 3D case 4.b: two rank-2 metrics with different null axes -> elementwise max
+    ("4b", np.diag([4, 5, 0]), np.diag([7, 0, 2]), np.diag([7, 5, 2])),
+    # 3D case 4.a: same null axis -> intersection within the plane
+    ("4a", np.diag([4, 5, 0]), np.diag([1, 8, 0]), np.diag([4, 8, 0])),
+    # all null -> null
+    ("null", np.zeros((3, 3)), np.zeros((3, 3)), np.zeros((3, 3))),
+    # 2D, one full metric: the normal procedure, elementwise max for diagonals
+    ("2d", np.diag([4, 1]), np.diag([0, 9]), np.diag([4, 9])),
+])
+def test_degenerate_cases_closed_forms(case, a, b, want):
+    dim = len(want)
+    got = inter(dim, a, b)
+    assert np.allclose(got, want, rtol=1e-13, atol=1e-13), (case, got)
+    # the intersection is symmetric in its arguments
+    assert np.allclose(inter(dim, b, a), want, rtol=1e-13, atol=1e-13), case
+
+
+def test_case5a_keeps_the_reference_rule():
+    # 2D, both rank 1 with orthogonal null directions: the reference returns
+    # max(l1, l2) along the first metric's range (src/Omega_h_metric_intersect.hpp:51-53)
+    assert np.allclose(inter(2, np.diag([4, 0]), np.diag([0, 9])), np.diag([9, 0]))
+
+
+def test_case5b_invariants():
+    # 2D, both rank 1 with non-orthogonal null directions: P = [v1, v2] and
+    # M = P^-T diag(v1.m2.v1, v2.m1.v2) P^-1 (src/Omega_h_metric_intersect.hpp:54-64)
+    for t in (0.3, 0.7, 1.1):
+        u = np.array([math.cos(t), math.sin(t)])
+        m1 = np.diag([4.0, 0.0])
+        m2 = 9.0 * np.outer(u, u)
+        got = inter(2, m1, m2)
+        v1 = np.array([0.0, 1.0])
+        v2 = np.array([-math.sin(t), math.cos(t)])
+        assert np.isclose(v1 @ got @ v1, v1 @ m2 @ v1, rtol=1e-12)
+        assert np.isclose(v2 @ got @ v2, v2 @ m1 @ v2, rtol=1e-12)
+        assert abs(v1 @ got @ v2) < 1e-12 * np.abs(got).max()
+
+
+def test_case2b_contains_both_metrics():
+    # rank-1 metrics along different axes: in the basis (u1 x u, u2 x u, u)
+    # both are diagonal, so the result contains both exactly
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        r = rot3(rng)
+        u1, u2 = r[:, 0], (r[:, 0] + r[:, 1]) / math.sqrt(2)
+        a = 3.0 * np.outer(u1, u1)
+        b = 5.0 * np.outer(u2, u2)
+        m = inter(3, a, b)
+        for x in (a, b):
+            w = np.linalg.eigvalsh(m - x)
+            assert w.min() > -1e-9 * np.abs(m).max()
+
+
+def test_case3_defining_properties():
+    # m1 of rank 2 (null axis w1), m2 of rank 1 (range u2).  The reference
+    # branches on the signed product u2 . w1 of the eigenvectors it computed
+    # (src/Omega_h_metric_intersect.hpp:134): below 1e-10 it intersects
+    # within m1's range plane (3.a), otherwise (3.b) it returns
+    # M = P^-T diag(v2.m1.v2, w2.m1.w2, w1.m2.w1) P^-1, P = [v2, w2, w1], with
+    # (v2, w2) an orthonormal basis of m2's null plane
+    rng = np.random.default_rng(4)
+    seen = set()
+    for _ in range(40):
+        r = rot3(rng)
+        a = r @ np.diag([2.0, 6.0, 0.0]) @ r.T
+        u = (r[:, 2] + r[:, 0]) / math.sqrt(2)
+        b = 4.0 * np.outer(u, u)
+        m = inter(3, a, b)
+        q1, l1 = O.kat_decompose_eigen(a)
+        q2, l2 = O.kat_decompose_eigen(b)
+        w1 = q1[:, int(np.argmin(l1))]
+        u2 = q2[:, int(np.argmax(l2))]
+        scale = np.abs(m).max()
+        if u2 @ w1 < 1e-10:
+            seen.add("3a")
+            # within the range plane of m1: M kills w1 and contains m1 there
+            assert np.abs(m @ w1).max() < 1e-10 * scale
+            w = np.linalg.eigvalsh(m - a)
+            assert w.min() > -1e-9 * scale
+        else:
+            seen.add("3b")
+            assert np.isclose(w1 @ m @ w1, w1 @ b @ w1, rtol=1e-12)
+            qq, _ = np.linalg.qr(np.column_stack([u2, rng.normal(size=3), rng.normal(size=3)]))
+            plane = qq[:, 1:]
+            for k in range(2):
+                assert abs(plane[:, k] @ m @ w1) < 1e-10 * scale
+            assert np.isclose(np.trace(plane.T @ m @ plane), np.trace(plane.T @ a @ plane),
+                              rtol=1e-12)
+    assert seen == {"3a", "3b"}
+
+
+def test_full_rank_intersection_contains_both():
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        r1, r2 = rot3(rng), rot3(rng)
+        a = r1 @ np.diag(np.exp(rng.uniform(-3, 3, 3))) @ r1.T
+        b = r2 @ np.diag(np.exp(rng.uniform(-3, 3, 3))) @ r2.T
+        m = inter(3, a, b)
+        for x in (a, b):
+            w = np.linalg.eigvalsh(m - x)
+            assert w.min() > -1e-9 * np.abs(m).max()
+
+
+def star_graph(m):
+    a2ab, ab2b = m.star(0)
+    return a2ab, ab2b
+
+
+@pytest.mark.parametrize("dim,n", [(2, 10), (3, 4)])
+def test_gradation_of_isotropic_metric_is_a_fixed_point(dim, n):
+    rate = 0.5
+    m = O.Mesh.box(dim, n)
+    c = m.coords().reshape(-1, dim)
+    # a sharp isotropic spike at one corner
+    h = np.where(np.linalg.norm(c, axis=1) < 0.3, 0.05, 0.3)
+    m.set_metric(1.0 / (h * h), 1)
+    before = m.metric().copy()
+    steps = m.limit_metric_gradation(rate, 1e-3)
+    assert steps > 1
+    after = m.metric()
+    # intersection only ever increases a metric
+    assert np.all(after >= before * (1 - 1e-12))
+    # converged: every edge satisfies the gradation bound, m_v >= m_u / (1 + rate d)^2,
+    # with d the neighbour's metric distance, up to the tolerance
+    a2ab, ab2b = star_graph(m)
+    for v in range(m.nents(0)):
+        for j in range(a2ab[v], a2ab[v + 1]):
+            u = ab2b[j]
+            d = math.sqrt(after[u] * float(np.sum((c[u] - c[v]) ** 2)))
+            assert after[v] >= after[u] / (1.0 + rate * d) ** 2 * (1 - 2e-3)
+
+
+def test_gradation_leaves_a_graded_metric_alone():
+    m = O.Mesh.box(3, 3)
+    m.set_metric(np.full(m.nents(0), 4.0), 1)
+    assert m.limit_metric_gradation(1.0, 1e-6) == 1
+    assert np.allclose(m.metric(), 4.0, rtol=1e-13)
