@@ -114,6 +114,126 @@ def traffic_per_launch(kernel):
 COPY_BYTES = 64 << 20
 
 
+def constant_metric_request(rng, dim, elems, h_lo, h_hi):
+    """A box of about `elems` elements under a constant anisotropic metric:
+    a uniformly random rotation and eigen-lengths log-uniform in [h_lo, h_hi]."""
+    import numpy as np
+    n = max(1, round((elems / 2) ** 0.5)) if dim == 2 else max(1, round((elems / 6) ** (1 / 3)))
+    g = rng.normal(size=(dim, dim))
+    q, r = np.linalg.qr(g)
+    q = q * np.sign(np.diag(r))
+    h = np.exp(rng.uniform(np.log(h_lo), np.log(h_hi), dim))
+    m = q @ np.diag(1.0 / h ** 2) @ q.T
+    if dim == 3:
+        v = [m[0, 0], m[1, 1], m[2, 2], m[0, 1], m[1, 2], m[0, 2]]
+    else:
+        v = [m[0, 0], m[1, 1], m[0, 1]]
+    return dim, n, v
+
+
+def percentile(xs, p):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs), p)) if xs else None
+
+
+def serve_main(args):
+    """Poisson stream of adapt requests (30% 2D, 70% 3D boxes, element counts
+    log-uniform in [1e3, --max-elems], constant metrics as in the batched
+    form) against the request server; p50/p99 latency and throughput."""
+    import numpy as np
+    import omega_h_amd as A
+    ctx = A.Context(0)
+    opts = {2: ctx.default_opts(2), 3: ctx.default_opts(3)}
+    rng = np.random.default_rng(args.seed)
+    out = []
+    for rate in args.rates:
+        server = A.Server(nslots=args.slots, hbm_budget_bytes=args.budget)
+        # one request of each dimension first, so that code paths are warm
+        for d in (2, 3):
+            server.wait(server.submit(*constant_metric_request(rng, d, 1e3, args.h_lo, args.h_hi),
+                                      opts[d]))
+        t = 0.0
+        arrivals = []
+        while True:
+            t += rng.exponential(1.0 / rate)
+            if t > args.duration:
+                break
+            dim = 2 if rng.random() < 0.3 else 3
+            elems = float(np.exp(rng.uniform(np.log(1e3), np.log(args.max_elems))))
+            arrivals.append((t, constant_metric_request(rng, dim, elems, args.h_lo, args.h_hi)))
+        t0 = time.perf_counter()
+        tickets = []
+        for at, (dim, n, v) in arrivals:
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(server.submit(dim, n, v, opts[dim]))
+        res = [server.wait(tk) for tk in tickets]
+        wall = time.perf_counter() - t0
+        server.close()
+        lat = [r.latency_seconds for r in res]
+        elems = sum(r.pass_elems for r in res)
+        rec = {"rate": rate, "requests": len(res), "p50_s": percentile(lat, 50),
+               "p99_s": percentile(lat, 99), "mean_queue_s": float(np.mean([r.queue_seconds for r in res])),
+               "Mtets_per_s": elems / wall / 1e6, "wall_s": wall,
+               "elems_out": int(sum(r.nelems_out for r in res))}
+        log(f"serve rate {rate}/s: {rec}")
+        out.append(rec)
+    line = {
+        "metric": "p50/p99 serve latency (adapt requests, continuous batching)",
+        "value": out[-1]["p99_s"] if out else None, "unit": "s (p99 at the highest rate)",
+        "n_gpus": 1, "higher_is_better": False, "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"Poisson adapt requests, 30% 2D / 70% 3D boxes, elements "
+                               f"log-uniform in [1e3, {args.max_elems:g}], constant metrics with "
+                               f"eigen-lengths log-uniform in [{args.h_lo}, {args.h_hi}], "
+                               f"{args.duration:g} s per rate",
+                   "slots": args.slots, "hbm_budget_bytes": args.budget},
+        "rates": out,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def batch_main(args):
+    """Many independent boxes of uneven size (cells per side log-uniform in
+    [8, 48] in 3D, constant rotated metrics) adapted concurrently by the
+    request server, against the same boxes one at a time."""
+    import numpy as np
+    import omega_h_amd as A
+    ctx = A.Context(0)
+    opts = ctx.default_opts(3)
+    rng = np.random.default_rng(args.seed)
+    boxes = []
+    for _ in range(args.boxes):
+        n = int(round(np.exp(rng.uniform(np.log(8), np.log(48)))))
+        d, _, v = constant_metric_request(rng, 3, 6 * n ** 3, args.h_lo, args.h_hi)
+        boxes.append((n, v))
+    out = {}
+    for slots in (1, args.slots):
+        server = A.Server(nslots=slots, hbm_budget_bytes=args.budget)
+        server.wait(server.submit(3, 4, boxes[0][1], opts))
+        t0 = time.perf_counter()
+        tickets = [server.submit(3, n, v, opts) for n, v in boxes]
+        res = [server.wait(t) for t in tickets]
+        wall = time.perf_counter() - t0
+        server.close()
+        elems = sum(r.pass_elems for r in res)
+        out[slots] = {"wall_s": wall, "Mtets_per_s": elems / wall / 1e6,
+                      "tets_in": int(sum(r.nelems_in for r in res)),
+                      "tets_out": int(sum(r.nelems_out for r in res))}
+        log(f"batch slots {slots}: {out[slots]}")
+    line = {
+        "metric": "Mtets adapted/s per pass, batched boxes",
+        "value": out[args.slots]["Mtets_per_s"], "unit": "Mtets/s", "n_gpus": 1,
+        "higher_is_better": True, "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.boxes} tet boxes, cells per side log-uniform in [8, 48], "
+                               f"constant metrics with eigen-lengths log-uniform in "
+                               f"[{args.h_lo}, {args.h_hi}]", "slots": args.slots},
+        "one_slot": out[1], "concurrent": out[args.slots],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(n, seconds):
     """Oracle passes, in adapt() order, on the same workload until `seconds`
     of pass time have been spent."""
@@ -154,6 +274,18 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-launches", type=int, default=20)
+    ap.add_argument("--mode", choices=["adapt", "serve", "batch"], default="adapt",
+                    help="adapt: the 64^3 level-set adapt (default); serve: Poisson request "
+                         "stream; batch: many boxes at once")
+    ap.add_argument("--rates", type=float, nargs="+", default=[5.0, 20.0, 50.0])
+    ap.add_argument("--duration", type=float, default=20.0)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--budget", type=float, default=200e9)
+    ap.add_argument("--max-elems", type=float, default=2e5)
+    ap.add_argument("--h-lo", type=float, default=0.02)
+    ap.add_argument("--h-hi", type=float, default=0.1)
+    ap.add_argument("--boxes", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the measure-kernel sweeps and a 64 MiB device copy (the "
                          "reference kernel for calibrating HBM byte counters)")
@@ -162,6 +294,13 @@ def main():
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(the adapt() path has no multi-GPU form)", file=sys.stderr)
         sys.exit(2)
+
+    if args.mode == "serve":
+        serve_main(args)
+        return
+    if args.mode == "batch":
+        batch_main(args)
+        return
 
     import omega_h_amd as A
     from omega_h_amd import workloads as W

@@ -136,6 +136,39 @@ int osh_limit_metric_gradation(osh_mesh* mesh, double max_rate, double tol, int*
 int osh_intersect_metrics(osh_ctx* ctx, int dim, int n, const double* m1, const double* m2,
     double* out);
 
+/* FNV-1a digest of the element vertex lists and coordinate bits (for
+ * comparing adapted meshes) */
+int osh_mesh_digest(osh_mesh* mesh, unsigned long long* out);
+
+/* Request server (no counterpart in the reference, whose adapt() serves one
+ * mesh per call): adapt() requests on unit boxes with a constant metric,
+ * run by nslots worker contexts (one HIP stream each) so concurrent
+ * requests overlap on the GPU; admitted in arrival order when a slot is free
+ * and the predicted footprint (1 KiB per element of the larger of the input
+ * and the metric's expected element count) fits the HBM budget beside the
+ * running requests.  osh_server_wait blocks until the ticket is done. */
+typedef struct osh_server osh_server;
+typedef struct {
+  int status;                 /* 0 ok, 1 failed (see error) */
+  int ticket;
+  int nelems_in, nelems_out;  /* elements of the box and of the result */
+  int npasses;                /* mesh-changing passes */
+  long long pass_elems;       /* elements entering those passes */
+  double pass_seconds;        /* their summed wall time */
+  double queue_seconds;       /* submit to start */
+  double run_seconds;         /* start to done */
+  double latency_seconds;     /* submit to done */
+  unsigned long long digest;  /* osh_mesh_digest of the result, if asked */
+  char error[200];
+} osh_serve_result;
+int osh_server_create(int device, int nslots, double hbm_budget_bytes, osh_server** out);
+int osh_server_destroy(osh_server* server);
+/* metric: metric_ncomps values of the constant metric (1, or 3 in 2D / 6 in
+ * 3D in Omega_h's symmetric order); nz = 0 for a 2D box */
+int osh_server_submit(osh_server* server, int dim, int nx, int ny, int nz, int metric_ncomps,
+    const double* metric, const osh_adapt_opts* opts, int want_digest, int* ticket);
+int osh_server_wait(osh_server* server, int ticket, osh_serve_result* out);
+
 /* cavity kernels on host arrays, for tests and diagnostics:
  * refine_qualities src/Omega_h_refine_qualities.cpp:88-108 */
 int osh_refine_qualities(osh_mesh* mesh, int n, const int* cands, double* out);

@@ -43,6 +43,24 @@ class AdaptStats(C.Structure):
     ]
 
 
+class ServeResult(C.Structure):
+    """osh_serve_result of include/omega_h_amd.h."""
+    _fields_ = [
+        ("status", C.c_int),
+        ("ticket", C.c_int),
+        ("nelems_in", C.c_int),
+        ("nelems_out", C.c_int),
+        ("npasses", C.c_int),
+        ("pass_elems", C.c_longlong),
+        ("pass_seconds", C.c_double),
+        ("queue_seconds", C.c_double),
+        ("run_seconds", C.c_double),
+        ("latency_seconds", C.c_double),
+        ("digest", C.c_ulonglong),
+        ("error", C.c_char * 200),
+    ]
+
+
 class OshError(RuntimeError):
     pass
 
@@ -108,6 +126,11 @@ def load(path=None):
         "osh_eval_math": [P, I, I, DP, DP],
         "osh_limit_metric_gradation": [P, C.c_double, C.c_double, IP],
         "osh_intersect_metrics": [P, I, I, DP, DP, DP],
+        "osh_mesh_digest": [P, C.POINTER(C.c_ulonglong)],
+        "osh_server_create": [I, I, C.c_double, C.POINTER(P)],
+        "osh_server_destroy": [P],
+        "osh_server_submit": [P, I, I, I, I, I, DP, C.POINTER(AdaptOpts), I, IP],
+        "osh_server_wait": [P, I, C.POINTER(ServeResult)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -325,6 +348,12 @@ class Mesh:
                                        _p(out, C.c_byte)), self.lib)
         return out
 
+    def digest(self):
+        """FNV-1a digest of the element vertex lists and coordinate bits."""
+        d = C.c_ulonglong()
+        check(self.lib.osh_mesh_digest(self.h, C.byref(d)), self.lib)
+        return d.value
+
     def limit_metric_gradation(self, max_rate, tol=1e-2):
         """Gradation limiting of the mesh's metric; returns the sweep count."""
         n = C.c_int()
@@ -355,3 +384,40 @@ class Mesh:
         passes = [(st.kind[i], st.nelems_in[i], st.nelems_out[i], st.seconds[i])
                   for i in range(st.npasses)]
         return did.value, passes
+
+
+class Server:
+    """Continuous-batching adapt() server on one GPU (osh_server_*): nslots
+    worker contexts, admission by predicted HBM footprint."""
+
+    def __init__(self, nslots=4, hbm_budget_bytes=200e9, device=0, lib=None):
+        self.lib = lib or load()
+        h = C.c_void_p()
+        check(self.lib.osh_server_create(device, nslots, float(hbm_budget_bytes), C.byref(h)),
+              self.lib)
+        self.h = h
+
+    def submit(self, dim, n, metric, opts, want_digest=False):
+        """Adapt an n-cells-per-side unit box under a constant metric
+        (1, 3 or 6 values); returns a ticket."""
+        m = np.ascontiguousarray(metric, dtype=np.float64)
+        t = C.c_int()
+        check(self.lib.osh_server_submit(self.h, dim, n, n, n if dim == 3 else 0, len(m),
+                                         _p(m, C.c_double), C.byref(opts), int(want_digest),
+                                         C.byref(t)), self.lib)
+        return t.value
+
+    def wait(self, ticket):
+        r = ServeResult()
+        check(self.lib.osh_server_wait(self.h, ticket, C.byref(r)), self.lib)
+        if r.status != 0:
+            raise OshError(r.error.decode())
+        return r
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.osh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -4,6 +4,7 @@
 #include <string>
 
 #include "../../include/omega_h_amd.h"
+#include "osh_guard.hpp"
 #include "osh_math.hpp"
 #include "osh_passes.hpp"
 
@@ -16,24 +17,10 @@ struct osh_mesh {
   osh::DMesh m;
 };
 
-namespace {
+namespace osh {
+thread_local std::string last_error_message;
 
-thread_local std::string last_error;
-
-template <class F>
-int guarded(F&& f) {
-  try {
-    f();
-    return 0;
-  } catch (std::exception const& e) {
-    last_error = e.what();
-  } catch (...) {
-    last_error = "unknown error";
-  }
-  return 1;
-}
-
-void to_options(osh_adapt_opts const* o, osh::AdaptOptions& r) {
+void to_options(osh_adapt_opts const* o, AdaptOptions& r) {
   r.min_length_desired = o->min_length_desired;
   r.max_length_desired = o->max_length_desired;
   r.max_length_allowed = o->max_length_allowed;
@@ -46,6 +33,30 @@ void to_options(osh_adapt_opts const* o, osh::AdaptOptions& r) {
   r.should_swap = o->should_swap;
   r.should_coarsen_slivers = o->should_coarsen_slivers;
 }
+
+// FNV-1a over the element vertex lists and the bits of the coordinates
+std::uint64_t mesh_digest(Ctx& c, DMesh& m) {
+  auto ev2v = to_host(c, verts_of(c, m, m.dim));
+  auto coords = to_host(c, m.coords);
+  std::uint64_t h = 1469598103934665603ull;
+  auto mix = [&](void const* p, std::size_t n) {
+    auto b = static_cast<unsigned char const*>(p);
+    for (std::size_t i = 0; i < n; ++i) {
+      h ^= b[i];
+      h *= 1099511628211ull;
+    }
+  };
+  mix(ev2v.data(), ev2v.size() * sizeof(LO));
+  mix(coords.data(), coords.size() * sizeof(Real));
+  return h;
+}
+}  // namespace osh
+
+namespace {
+
+using osh::guarded;
+using osh::to_options;
+
 
 template <class T>
 void download(osh::Ctx& c, osh::DArray<T> const& d, T* out) {
@@ -83,7 +94,15 @@ __global__ void eval_math_kernel(osh::LO n, int which, double const* in, double*
 
 extern "C" {
 
-const char* osh_last_error(void) { return last_error.c_str(); }
+const char* osh_last_error(void) { return osh::last_error_message.c_str(); }
+
+int osh_mesh_digest(osh_mesh* mesh, unsigned long long* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(out != nullptr, "null output pointer");
+    *out = osh::mesh_digest(mesh->owner->ctx, mesh->m);
+  });
+}
 
 int osh_ctx_create(int device, osh_ctx** out) {
   return guarded([&] {

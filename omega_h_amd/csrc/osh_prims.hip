@@ -39,7 +39,7 @@ void Pool::put(void* p, std::size_t bytes) {
   std::size_t size = round_up(bytes);
   {
     std::lock_guard<std::mutex> g(mu_);
-    if (!closed_ && cached_ + size <= kCap) {
+    if (!closed_ && cached_ + size <= cap_) {
       free_[size].push_back(p);
       cached_ += size;
       return;

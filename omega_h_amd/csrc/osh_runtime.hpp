@@ -62,13 +62,18 @@ class Pool {
     closed_ = true;
   }
   static std::size_t round_up(std::size_t bytes);
+  // most bytes kept cached; blocks returned beyond it go back to the runtime
+  void set_cap(std::size_t bytes) {
+    std::lock_guard<std::mutex> g(mu_);
+    cap_ = bytes;
+  }
 
  private:
   std::mutex mu_;
   std::unordered_map<std::size_t, std::vector<void*>> free_;
   std::size_t cached_ = 0;
+  std::size_t cap_ = std::size_t(16) << 30;
   bool closed_ = false;
-  static constexpr std::size_t kCap = std::size_t(16) << 30;
 };
 
 struct Ctx {

@@ -1,0 +1,83 @@
+"""The request server (continuous batching of adapt() requests, one worker
+context per slot): every result is the same alone and in the mix, and the
+same as the oracle's."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+A = pytest.importorskip("omega_h_amd")
+
+
+def fnv1a(*arrays):
+    h = 1469598103934665603
+    for a in arrays:
+        for b in np.ascontiguousarray(a).view(np.uint8):
+            h ^= int(b)
+            h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def requests():
+    rng = np.random.default_rng(11)
+    out = []
+    for k in range(12):
+        dim = 2 if k % 3 == 0 else 3
+        n = int(rng.integers(6, 20)) if dim == 2 else int(rng.integers(2, 5))
+        q, _ = np.linalg.qr(rng.normal(size=(dim, dim)))
+        h = np.exp(rng.uniform(np.log(0.08), np.log(0.3), dim))
+        m = q @ np.diag(1.0 / h ** 2) @ q.T
+        v = O.symm2vector(m) if k % 2 else np.array([1.0 / h[0] ** 2])
+        out.append((dim, n, v))
+    return out
+
+
+def test_submit_rejects_bad_requests_before_running_them():
+    try:
+        s = A.Server(nslots=1, hbm_budget_bytes=1e9)
+    except A.OshError:
+        pytest.skip("no GPU")
+    with pytest.raises(A.OshError):
+        s.submit(4, 3, [1.0], A.AdaptOpts())
+    with pytest.raises(A.OshError):
+        s.submit(3, 3, [1.0, 2.0], A.AdaptOpts())
+    s.close()
+
+
+@pytest.mark.gpu
+def test_results_same_alone_and_in_the_mix():
+    ctx = A.Context(0)
+    reqs = requests()
+    server = A.Server(nslots=4, hbm_budget_bytes=50e9)
+    print(f"serving {len(reqs)} requests on 4 slots", flush=True)
+    tickets = [server.submit(d, n, v, ctx.default_opts(d), want_digest=True) for d, n, v in reqs]
+    mixed = [server.wait(t) for t in tickets]
+    server.close()
+    for (d, n, v), r in zip(reqs, mixed):
+        g = A.Mesh.box(ctx, d, n)
+        g.set_metric(np.tile(v, g.nents(0)), len(v))
+        g.adapt(ctx.default_opts(d))
+        assert r.digest == g.digest(), (d, n)
+        assert r.nelems_out == g.nents(d)
+        assert r.latency_seconds >= r.run_seconds >= r.pass_seconds > 0
+    # and the oracle's result for the first few
+    for (d, n, v), r in list(zip(reqs, mixed))[:4]:
+        o = O.Mesh.box(d, n)
+        o.set_metric(np.tile(v, o.nents(0)), len(v))
+        o.adapt(O.default_opts(d))
+        assert r.digest == fnv1a(o.verts(d), o.coords()), (d, n)
+
+
+@pytest.mark.gpu
+def test_admission_under_a_tiny_budget_runs_requests_one_at_a_time():
+    ctx = A.Context(0)
+    reqs = requests()[:5]
+    server = A.Server(nslots=3, hbm_budget_bytes=1.0)
+    tickets = [server.submit(d, n, v, ctx.default_opts(d), want_digest=True) for d, n, v in reqs]
+    res = [server.wait(t) for t in tickets]
+    server.close()
+    # every request exceeds the budget, so each ran alone, in arrival order:
+    # a request's queue time covers the run times of those before it
+    for i in range(1, len(res)):
+        assert res[i].queue_seconds >= res[i - 1].run_seconds * 0.9
+    assert all(r.status == 0 for r in res)
