@@ -136,6 +136,28 @@ int osh_limit_metric_gradation(osh_mesh* mesh, double max_rate, double tol, int*
 int osh_intersect_metrics(osh_ctx* ctx, int dim, int n, const double* m1, const double* m2,
     double* out);
 
+/* vertex tags carried through adapt(): Mesh::add_tag(VERT, name, ncomps,
+ * array) with the transfer type TransferOpts gives it
+ * (src/Omega_h_adapt.hpp:30-42, src/Omega_h_transfer.cpp:391-428):
+ * kind 0 = OMEGA_H_LINEAR_INTERP (midpoint average), 1 = OMEGA_H_METRIC
+ * (log-Euclidean midpoint); surviving vertices keep their values.  The tag
+ * "target_metric" drives osh_approach_metric. */
+int osh_mesh_add_vertex_tag(osh_mesh* mesh, const char* name, int ncomps, int kind,
+    const double* data);
+int osh_mesh_vertex_tag_ncomps(osh_mesh* mesh, const char* name, int* out); /* -1: absent */
+int osh_mesh_get_vertex_tag(osh_mesh* mesh, const char* name, double* out);
+int osh_mesh_remove_vertex_tag(osh_mesh* mesh, const char* name);
+/* approach_metric(mesh, opts, min_step), src/Omega_h_approach.cpp:83-126:
+ * set the metric to the "target_metric" tag, or halve the log-Euclidean
+ * step toward it until the mesh's quality and lengths are acceptable;
+ * *did = 0 when there is no target.  The reference's idiom
+ * (src/aniso_test.cpp:15-27) is: while (approach_metric) adapt. */
+int osh_approach_metric(osh_mesh* mesh, const osh_adapt_opts* opts, double min_step, int* did);
+/* interpolate_between_metrics, src/Omega_h_metric.cpp:106-111, for n pairs of
+ * metrics of dimension dim in host arrays */
+int osh_interpolate_metrics(osh_ctx* ctx, int dim, int n, const double* a, const double* b,
+    double t, double* out);
+
 /* FNV-1a digest of the element vertex lists and coordinate bits (for
  * comparing adapted meshes) */
 int osh_mesh_digest(osh_mesh* mesh, unsigned long long* out);

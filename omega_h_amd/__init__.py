@@ -127,6 +127,12 @@ def load(path=None):
         "osh_limit_metric_gradation": [P, C.c_double, C.c_double, IP],
         "osh_intersect_metrics": [P, I, I, DP, DP, DP],
         "osh_mesh_digest": [P, C.POINTER(C.c_ulonglong)],
+        "osh_mesh_add_vertex_tag": [P, C.c_char_p, I, I, DP],
+        "osh_mesh_vertex_tag_ncomps": [P, C.c_char_p, IP],
+        "osh_mesh_get_vertex_tag": [P, C.c_char_p, DP],
+        "osh_mesh_remove_vertex_tag": [P, C.c_char_p],
+        "osh_approach_metric": [P, C.POINTER(AdaptOpts), C.c_double, IP],
+        "osh_interpolate_metrics": [P, I, I, DP, DP, C.c_double, DP],
         "osh_server_create": [I, I, C.c_double, C.POINTER(P)],
         "osh_server_destroy": [P],
         "osh_server_submit": [P, I, I, I, I, I, DP, C.POINTER(AdaptOpts), I, IP],
@@ -203,6 +209,17 @@ class Context:
         out = np.zeros_like(a)
         check(self.lib.osh_intersect_metrics(self.h, dim, n, _p(a, C.c_double), _p(b, C.c_double),
                                              _p(out, C.c_double)), self.lib)
+        return out
+
+    def interpolate_metrics(self, dim, a, b, t):
+        """interpolate_between_metrics of two arrays of symmetric metrics."""
+        x = np.ascontiguousarray(a, dtype=np.float64)
+        y = np.ascontiguousarray(b, dtype=np.float64)
+        n = x.size // (dim * (dim + 1) // 2)
+        out = np.zeros_like(x)
+        check(self.lib.osh_interpolate_metrics(self.h, dim, n, _p(x, C.c_double),
+                                               _p(y, C.c_double), t, _p(out, C.c_double)),
+              self.lib)
         return out
 
     def eval_math(self, which, xs):
@@ -347,6 +364,31 @@ class Mesh:
         check(self.lib.osh_find_indset(self.h, d, _p(q, C.c_double), _p(c, C.c_byte),
                                        _p(out, C.c_byte)), self.lib)
         return out
+
+    LINEAR, METRIC = 0, 1
+
+    def add_vertex_tag(self, name, ncomps, kind, data):
+        a = np.ascontiguousarray(data, dtype=np.float64)
+        check(self.lib.osh_mesh_add_vertex_tag(self.h, name.encode(), ncomps, kind,
+                                               _p(a, C.c_double)), self.lib)
+
+    def vertex_tag(self, name):
+        nc = self._int(self.lib.osh_mesh_vertex_tag_ncomps, name.encode())
+        if nc < 0:
+            return None
+        out = np.zeros(self.nents(0) * nc, dtype=np.float64)
+        check(self.lib.osh_mesh_get_vertex_tag(self.h, name.encode(), _p(out, C.c_double)),
+              self.lib)
+        return out
+
+    def remove_vertex_tag(self, name):
+        check(self.lib.osh_mesh_remove_vertex_tag(self.h, name.encode()), self.lib)
+
+    def approach_metric(self, opts, min_step=1e-4):
+        did = C.c_int()
+        check(self.lib.osh_approach_metric(self.h, C.byref(opts), min_step, C.byref(did)),
+              self.lib)
+        return did.value
 
     def digest(self):
         """FNV-1a digest of the element vertex lists and coordinate bits."""

@@ -438,6 +438,93 @@ int osh_limit_metric_gradation(osh_mesh* mesh, double max_rate, double tol, int*
   });
 }
 
+int osh_mesh_add_vertex_tag(osh_mesh* mesh, const char* name, int ncomps, int kind,
+    const double* data) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(name && data, "null argument");
+    OSH_CHECK(kind == 0 || kind == 1, "kind must be 0 (linear) or 1 (metric)");
+    OSH_CHECK(ncomps >= 1 && ncomps <= 64, "1 to 64 components");
+    if (kind == 1)
+      OSH_CHECK(ncomps == 1 || ncomps == (mesh->m.dim == 3 ? 6 : 3), "metric tag size");
+    auto& c = mesh->owner->ctx;
+    osh::LO n = mesh->m.nents[0] * ncomps;
+    osh::Reals d(c, n);
+    osh::upload(c, d, data, n);
+    c.sync();
+    for (auto& t : mesh->m.vtags)
+      if (t.name == name) {
+        t = {name, ncomps, kind, d};
+        return;
+      }
+    mesh->m.vtags.push_back({name, ncomps, kind, d});
+  });
+}
+
+int osh_mesh_vertex_tag_ncomps(osh_mesh* mesh, const char* name, int* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(name && out, "null argument");
+    *out = -1;
+    for (auto const& t : mesh->m.vtags)
+      if (t.name == name) *out = t.ncomps;
+  });
+}
+
+int osh_mesh_get_vertex_tag(osh_mesh* mesh, const char* name, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(name && out, "null argument");
+    for (auto const& t : mesh->m.vtags)
+      if (t.name == name) {
+        download(mesh->owner->ctx, t.data, out);
+        return;
+      }
+    throw osh::Error(std::string("no vertex tag ") + name);
+  });
+}
+
+int osh_mesh_remove_vertex_tag(osh_mesh* mesh, const char* name) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(name != nullptr, "null argument");
+    auto& v = mesh->m.vtags;
+    for (auto it = v.begin(); it != v.end(); ++it)
+      if (it->name == name) {
+        v.erase(it);
+        return;
+      }
+  });
+}
+
+int osh_approach_metric(osh_mesh* mesh, const osh_adapt_opts* opts, double min_step, int* did) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(opts != nullptr, "null options");
+    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
+    osh::AdaptOptions o;
+    to_options(opts, o);
+    bool r = osh::approach_metric(mesh->owner->ctx, mesh->m, o, min_step);
+    mesh->owner->ctx.sync();
+    if (did) *did = r ? 1 : 0;
+  });
+}
+
+int osh_interpolate_metrics(osh_ctx* ctx, int dim, int n, const double* a, const double* b,
+    double t, double* out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && (n == 0 || (a && b && out)), "null argument");
+    OSH_CHECK(dim >= 1 && dim <= 3, "dimension must be 1, 2 or 3");
+    OSH_CHECK(n >= 0, "negative count");
+    auto& c = ctx->ctx;
+    osh::LO nc = (dim * (dim + 1)) / 2;
+    std::vector<double> ha(a, a + std::size_t(n) * nc), hb(b, b + std::size_t(n) * nc);
+    osh::Reals r;
+    osh::interpolate_metric_arrays(c, dim, n, osh::to_device(c, ha), osh::to_device(c, hb), t, r);
+    download(c, r, out);
+  });
+}
+
 int osh_intersect_metrics(osh_ctx* ctx, int dim, int n, const double* m1, const double* m2,
     double* out) {
   return guarded([&] {

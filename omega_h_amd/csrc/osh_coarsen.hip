@@ -392,6 +392,9 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
       o2n = mo.old2new;
       nw.coords = transfer<Real>(c, mo, dim, m.coords, nullptr);
       nw.metric = transfer<Real>(c, mo, nc, m.metric, nullptr);
+      for (auto const& t : m.vtags)
+        nw.vtags.push_back(
+            {t.name, t.ncomps, t.kind, transfer<Real>(c, mo, t.ncomps, t.data, nullptr)});
       nw.class_id[0] = transfer<LO>(c, mo, 1, m.class_id[0], nullptr);
       nw.class_dim[0] = transfer<I8>(c, mo, 1, m.class_dim[0], nullptr);
     } else {

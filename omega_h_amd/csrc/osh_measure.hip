@@ -33,25 +33,19 @@ __device__ __forceinline__ void load_vertex(Real const* coords, Real const* metr
 }
 
 template <int S, int M>
-__global__ void __launch_bounds__(kBlock) edge_lengths_kernel(LO n, LO const* edges,
-    LO const* ev2v, Real const* coords, Real const* metric, Real* out) {
-  LO i = xcd_thread_index();
-  if (i >= n) return;
-  LO e = edges ? edges[i] : i;
+__device__ __forceinline__ Real length_of(LO e, LO const* ev2v, Real const* coords,
+    Real const* metric) {
   constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
   int2 ab = reinterpret_cast<int2 const*>(ev2v)[e];
   double p0[S], p1[S], m0[nc], m1[nc];
   load_vertex<S, nc>(coords, metric, ab.x, p0, m0);
   load_vertex<S, nc>(coords, metric, ab.y, p1, m1);
-  out[i] = edge_length<S, M>(p0, p1, m0, m1);
+  return edge_length<S, M>(p0, p1, m0, m1);
 }
 
 template <int D, int M>
-__global__ void __launch_bounds__(kBlock) elem_qualities_kernel(LO n, LO const* elems,
-    LO const* ev2v, Real const* coords, Real const* metric, Real* out) {
-  LO i = xcd_thread_index();
-  if (i >= n) return;
-  LO e = elems ? elems[i] : i;
+__device__ __forceinline__ Real quality_of(LO e, LO const* ev2v, Real const* coords,
+    Real const* metric) {
   constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
   LO vs[D + 1];
   if constexpr (D == 3) {
@@ -74,7 +68,43 @@ __global__ void __launch_bounds__(kBlock) elem_qualities_kernel(LO n, LO const* 
     pp[k] = p[k];
     mm[k] = m[k];
   }
-  out[i] = element_quality<D, M>(pp, mm);
+  return element_quality<D, M>(pp, mm);
+}
+
+template <int S, int M>
+__global__ void __launch_bounds__(kBlock) edge_lengths_kernel(LO n, LO const* edges,
+    LO const* ev2v, Real const* coords, Real const* metric, Real* out) {
+  LO i = xcd_thread_index();
+  if (i >= n) return;
+  out[i] = length_of<S, M>(edges ? edges[i] : i, ev2v, coords, metric);
+}
+
+template <int D, int M>
+__global__ void __launch_bounds__(kBlock) elem_qualities_kernel(LO n, LO const* elems,
+    LO const* ev2v, Real const* coords, Real const* metric, Real* out) {
+  LO i = xcd_thread_index();
+  if (i >= n) return;
+  out[i] = quality_of<D, M>(elems ? elems[i] : i, ev2v, coords, metric);
+}
+
+// the length (WHICH 1) or quality (WHICH 0) tag of a rebuilt mesh in one
+// launch: threads [0, nsame) copy the old value of a same entity, the rest
+// measure a product on the new mesh
+template <int WHICH, int D, int M>
+__global__ void __launch_bounds__(kBlock) measure_transfer_kernel(LO n, LO nsame,
+    LO const* same2old, LO const* same2new, Real const* old_vals, LO const* prods2new,
+    LO const* ev2v, Real const* coords, Real const* metric, Real* out) {
+  LO i = thread_index();
+  if (i >= n) return;
+  if (i < nsame) {
+    out[same2new[i]] = old_vals[same2old[i]];
+    return;
+  }
+  LO e = prods2new[i - nsame];
+  if constexpr (WHICH == 1)
+    out[e] = length_of<D, M>(e, ev2v, coords, metric);
+  else
+    out[e] = quality_of<D, M>(e, ev2v, coords, metric);
 }
 
 template <int M>
@@ -160,14 +190,46 @@ Reals const& ask_qualities(Ctx& c, DMesh& m) {
   return m.quality;
 }
 
-void mident_metrics(Ctx& c, DMesh& m, LO n, LO const* edges, Real* out) {
-  int md = metric_dim(m);
+Reals measure_transfer(Ctx& c, DMesh& nw, int d, LO nsame, LO const* same2old,
+    LO const* same2new, Real const* old_vals, LO nprods, LO const* prods2new) {
+  OSH_CHECK(d == 1 || d == nw.dim, "lengths of edges or qualities of elements only");
+  Reals out(c, nw.nents[d]);
+  int md = metric_dim(nw);
+  LO const* ev2v = (d == 1) ? nw.down[1].data() : verts_of(c, nw, d).data();
+  check_alignment(nw.dim, md, d + 1, ev2v, nw.metric.data());
+  LO n = nsame + nprods;
+#define OSH_MT(W, D, M)                                                                    \
+  OSH_LAUNCH((measure_transfer_kernel<W, D, M>), n, c, nsame, same2old, same2new, old_vals, \
+      prods2new, ev2v, nw.coords.data(), nw.metric.data(), out.data())
+  if (d == 1) {
+    if (nw.dim == 3 && md == 3) OSH_MT(1, 3, 3);
+    else if (nw.dim == 3) OSH_MT(1, 3, 1);
+    else if (md == 2) OSH_MT(1, 2, 2);
+    else OSH_MT(1, 2, 1);
+  } else {
+    if (nw.dim == 3 && md == 3) OSH_MT(0, 3, 3);
+    else if (nw.dim == 3) OSH_MT(0, 3, 1);
+    else if (md == 2) OSH_MT(0, 2, 2);
+    else OSH_MT(0, 2, 1);
+  }
+#undef OSH_MT
+  return out;
+}
+
+void mident_values(Ctx& c, DMesh& m, LO n, LO const* edges, Real const* data, int ncomps,
+    Real* out) {
+  int md = (ncomps == 1) ? 1 : (ncomps == 3 ? 2 : 3);
+  OSH_CHECK(ncomps == 1 || ncomps == 3 || ncomps == 6, "a metric has 1, 3 or 6 components");
   if (md == 3)
-    OSH_LAUNCH(mident_kernel<3>, n, c, edges, m.down[1].data(), m.metric.data(), out);
+    OSH_LAUNCH(mident_kernel<3>, n, c, edges, m.down[1].data(), data, out);
   else if (md == 2)
-    OSH_LAUNCH(mident_kernel<2>, n, c, edges, m.down[1].data(), m.metric.data(), out);
+    OSH_LAUNCH(mident_kernel<2>, n, c, edges, m.down[1].data(), data, out);
   else
-    OSH_LAUNCH(mident_kernel<1>, n, c, edges, m.down[1].data(), m.metric.data(), out);
+    OSH_LAUNCH(mident_kernel<1>, n, c, edges, m.down[1].data(), data, out);
+}
+
+void mident_metrics(Ctx& c, DMesh& m, LO n, LO const* edges, Real* out) {
+  mident_values(c, m, n, edges, m.metric.data(), m.metric_ncomps, out);
 }
 
 }  // namespace osh

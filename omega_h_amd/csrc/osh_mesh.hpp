@@ -4,6 +4,8 @@
 // (src/Omega_h_mesh.cpp:307-357).
 #pragma once
 
+#include <string>
+
 #include "osh_runtime.hpp"
 
 namespace osh {
@@ -27,6 +29,17 @@ struct DMesh {
   int metric_ncomps = 0;
   LOs class_id[4];
   Bytes class_dim[4];
+  // vertex tags carried through adapt(): kind 0 values are averaged at
+  // refinement midpoints (average_field, src/Omega_h_mesh.cpp:822-844), kind 1
+  // values get log-Euclidean midpoint metrics (get_mident_metrics,
+  // src/Omega_h_metric.cpp:56-104); surviving vertices keep theirs
+  struct VertexTag {
+    std::string name;
+    int ncomps;
+    int kind;
+    Reals data;
+  };
+  std::vector<VertexTag> vtags;
   Reals length;   // cached edge lengths (may be absent)
   Reals quality;  // cached element qualities (may be absent)
   // derived caches
@@ -59,6 +72,9 @@ LOs find_unique(Ctx& c, LO nhigh, int high_dim, int low_dim, LOs const& hv2v);
 int limit_metric_gradation(Ctx& c, DMesh& m, Real max_rate, Real tol);
 void intersect_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals const& b,
     Reals& out);
+// interpolate_between_metrics (src/Omega_h_metric.cpp:106-111) of n metrics
+void interpolate_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals const& b, Real t,
+    Reals& out);
 
 // metric measures (osh_measure.hip)
 void launch_edge_lengths(Ctx& c, int dim, int md, LO n, LO const* edges, LO const* ev2v,
@@ -67,9 +83,16 @@ void launch_elem_qualities(Ctx& c, int dim, int md, LO n, LO const* elems, LO co
     Real const* coords, Real const* metric, Real* out);
 void measure_edges(Ctx& c, DMesh& m, LO n, LO const* edges, Real* out);
 void measure_elems(Ctx& c, DMesh& m, LO n, LO const* elems, Real* out);
+// the length (d = 1) or quality (d = dim) tag of a rebuilt mesh: same
+// entities copy old values, products are measured on the new mesh
+Reals measure_transfer(Ctx& c, DMesh& nw, int d, LO nsame, LO const* same2old,
+    LO const* same2new, Real const* old_vals, LO nprods, LO const* prods2new);
 Reals const& ask_lengths(Ctx& c, DMesh& m);
 Reals const& ask_qualities(Ctx& c, DMesh& m);
 void mident_metrics(Ctx& c, DMesh& m, LO n, LO const* edges, Real* out);
+// the same for any per-vertex metric array with `ncomps` components
+void mident_values(Ctx& c, DMesh& m, LO n, LO const* edges, Real const* data, int ncomps,
+    Real* out);
 
 // build_box (osh_build.hip)
 void build_box(Ctx& c, DMesh& m, int dim, Real x, Real y, Real z, LO nx, LO ny, LO nz,

@@ -56,6 +56,21 @@ __global__ void intersect_kernel(LO n, Real const* a, Real const* b, Real* out) 
   set_symm<M>(r, out + std::size_t(i) * nc);
 }
 
+// interpolate_between_metrics, src/Omega_h_metric.cpp:106-111: linearize
+// (log), a (1 - t) + b t per stored component
+// (src/Omega_h_array_ops.cpp:567-574), delinearize (exp)
+template <int M>
+__global__ void interpolate_kernel(LO n, Real const* a, Real const* b, Real t, Real* out) {
+  LO i = thread_index();
+  if (i >= n) return;
+  constexpr int nc = (M * (M + 1)) / 2;
+  double va[6], vb[6], vc[6];
+  set_symm<M>(log_spd<M>(get_symm<M>(a + std::size_t(i) * nc)), va);
+  set_symm<M>(log_spd<M>(get_symm<M>(b + std::size_t(i) * nc)), vb);
+  for (int k = 0; k < nc; ++k) vc[k] = va[k] * (1.0 - t) + vb[k] * t;
+  set_symm<M>(exp_spd<M>(get_symm<M>(vc)), out + std::size_t(i) * nc);
+}
+
 Reals limit_gradation_once(Ctx& c, DMesh& m, Reals const& values, Real max_rate) {
   LO nv = m.nents[0];
   Adj const& star = ask_star(c, m, 0);
@@ -102,6 +117,66 @@ int limit_metric_gradation(Ctx& c, DMesh& m, Real max_rate, Real tol) {
   m.length = Reals();
   m.quality = Reals();
   return steps;
+}
+
+void interpolate_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals const& b, Real t,
+    Reals& out) {
+  out = Reals(c, a.size());
+  if (dim == 3)
+    OSH_LAUNCH(interpolate_kernel<3>, n, c, a.data(), b.data(), t, out.data());
+  else if (dim == 2)
+    OSH_LAUNCH(interpolate_kernel<2>, n, c, a.data(), b.data(), t, out.data());
+  else
+    OSH_LAUNCH(interpolate_kernel<1>, n, c, a.data(), b.data(), t, out.data());
+}
+
+namespace {
+
+void replace_metric(DMesh& m, Reals const& v) {
+  m.metric = v;
+  m.length = Reals();
+  m.quality = Reals();
+}
+
+bool acceptable(Ctx& c, DMesh& m, AdaptOptions const& o) {
+  if (min_real(c, ask_qualities(c, m)) < o.min_quality_allowed) return false;
+  return !(max_real(c, ask_lengths(c, m)) > o.max_length_allowed);
+}
+
+}  // namespace
+
+bool approach_metric(Ctx& c, DMesh& m, AdaptOptions const& o, Real min_step) {
+  auto it = m.vtags.begin();
+  for (; it != m.vtags.end(); ++it)
+    if (it->name == "target_metric") break;
+  if (it == m.vtags.end()) return false;
+  OSH_CHECK(it->ncomps == m.metric_ncomps, "target_metric and metric differ in size");
+  Real minq = min_real(c, ask_qualities(c, m));
+  OSH_CHECK(minq >= o.min_quality_allowed,
+      "approach_metric: minimum element quality " + std::to_string(minq) +
+          " < minimum allowed quality " + std::to_string(o.min_quality_allowed));
+  Real maxl = max_real(c, ask_lengths(c, m));
+  OSH_CHECK(!(maxl > o.max_length_allowed),
+      "approach_metric: maximum edge length " + std::to_string(maxl) +
+          " > maximum allowed length " + std::to_string(o.max_length_allowed));
+  Reals orig = m.metric;
+  Reals target = it->data;
+  replace_metric(m, target);
+  if (acceptable(c, m, o)) {
+    m.vtags.erase(it);
+    return true;
+  }
+  int mdim = (m.metric_ncomps == 1) ? 1 : m.dim;
+  Real factor = 1.0;
+  do {
+    factor /= 2.0;
+    OSH_CHECK(factor >= min_step, "approach_metric has stalled at step size " +
+                                      std::to_string(factor) + " < " + std::to_string(min_step));
+    Reals cur;
+    interpolate_metric_arrays(c, mdim, m.nents[0], orig, target, factor, cur);
+    replace_metric(m, cur);
+  } while (!acceptable(c, m, o));
+  return true;
 }
 
 void intersect_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals const& b,

@@ -115,30 +115,43 @@ __global__ void prod_conn(LO nprods, int deg, LO const* prods2new, LO const* pdo
   }
 }
 
+// one launch for a transferred tag: threads [0, nsame) copy same entities,
+// the rest place the products' values (src/Omega_h_transfer.cpp:631-653)
 template <class T>
-__global__ void scatter_same_data(LO nsame, int nc, LO const* same2old, LO const* same2new,
-    T const* old_data, T* data) {
-  LO s = thread_index();
-  if (s >= nsame) return;
-  for (int k = 0; k < nc; ++k) data[same2new[s] * nc + k] = old_data[same2old[s] * nc + k];
+__global__ void scatter_data(LO n, LO nsame, int nc, LO const* same2old, LO const* same2new,
+    T const* old_data, LO const* prods2new, T const* pdata, T* data) {
+  LO i = thread_index();
+  if (i >= n) return;
+  if (i < nsame) {
+    for (int k = 0; k < nc; ++k) data[same2new[i] * nc + k] = old_data[same2old[i] * nc + k];
+  } else {
+    LO p = i - nsame;
+    for (int k = 0; k < nc; ++k) data[prods2new[p] * nc + k] = pdata[p * nc + k];
+  }
 }
 
-template <class T>
-__global__ void scatter_prod_data(LO nprods, int nc, LO const* prods2new, T const* pdata, T* data) {
-  LO p = thread_index();
-  if (p >= nprods) return;
-  for (int k = 0; k < nc; ++k) data[prods2new[p] * nc + k] = pdata[p * nc + k];
-}
-
-__global__ void gather_class(LO nprods, LO const* src, int src_dim, I8 const* src_dims,
-    LO const* ids_a, I8 const* dims_a, LO const* ids_b, I8 const* dims_b, LO* pid, I8* pdim) {
-  LO p = thread_index();
-  if (p >= nprods) return;
+// class_id and class_dim of the new entities in one launch: same entities
+// copy theirs, products inherit from their source entity of dimension
+// src_dims[p] (or src_dim), src/Omega_h_transfer.cpp:20-34,212-267
+__global__ void transfer_class_kernel(LO n, LO nsame, LO const* same2old, LO const* same2new,
+    LO const* old_ids, I8 const* old_dims, LO const* prods2new, LO const* src, int src_dim,
+    I8 const* src_dims, LO const* ids_a, I8 const* dims_a, LO const* ids_b, I8 const* dims_b,
+    LO* ids, I8* dims) {
+  LO i = thread_index();
+  if (i >= n) return;
+  if (i < nsame) {
+    LO o = same2old[i], w = same2new[i];
+    ids[w] = old_ids[o];
+    dims[w] = old_dims[o];
+    return;
+  }
+  LO p = i - nsame;
   int sd = src_dims ? src_dims[p] : src_dim;
   bool second = (sd != src_dim);
   LO s = src[p];
-  pid[p] = second ? ids_b[s] : ids_a[s];
-  pdim[p] = second ? dims_b[s] : dims_a[s];
+  LO w = prods2new[p];
+  ids[w] = second ? ids_b[s] : ids_a[s];
+  dims[w] = second ? dims_b[s] : dims_a[s];
 }
 
 __global__ void indset_init(LO n, I8 const* cands, I8* state) {
@@ -305,11 +318,11 @@ template <class T>
 DArray<T> transfer(Ctx& c, ModOut const& mo, int nc, DArray<T> const& old_data,
     DArray<T> const* prod_data) {
   DArray<T> d(c, mo.nnew * nc);
-  if (prod_data)
-    OSH_LAUNCH(scatter_prod_data<T>, mo.prods2new.size(), c, nc, mo.prods2new.data(),
-        prod_data->data(), d.data());
-  OSH_LAUNCH(scatter_same_data<T>, mo.same2old.size(), c, nc, mo.same2old.data(),
-      mo.same2new.data(), old_data.data(), d.data());
+  LO nsame = mo.same2old.size();
+  LO nprods = prod_data ? mo.prods2new.size() : 0;
+  OSH_LAUNCH(scatter_data<T>, nsame + nprods, c, nsame, nc, mo.same2old.data(),
+      mo.same2new.data(), old_data.data(), mo.prods2new.data(),
+      prod_data ? prod_data->data() : (T const*)nullptr, d.data());
   return d;
 }
 
@@ -320,29 +333,29 @@ template DArray<Real> transfer(Ctx&, ModOut const&, int, DArray<Real> const&, DA
 void transfer_class(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo,
     LOs const& prod_src, int src_dim, Bytes const* src_dims) {
   LO nprods = mo.prods2new.size();
-  LOs pid(c, nprods);
-  Bytes pdim(c, nprods);
+  LO nsame = mo.same2old.size();
   int other = (src_dims && src_dim + 1 <= old.dim) ? src_dim + 1 : src_dim;
-  OSH_LAUNCH(gather_class, nprods, c, prod_src.data(), src_dim,
+  LOs ids(c, mo.nnew);
+  Bytes dims(c, mo.nnew);
+  OSH_LAUNCH(transfer_class_kernel, nsame + nprods, c, nsame, mo.same2old.data(),
+      mo.same2new.data(), old.class_id[d].data(), old.class_dim[d].data(),
+      mo.prods2new.data(), prod_src.data(), src_dim,
       src_dims ? src_dims->data() : (I8 const*)nullptr, old.class_id[src_dim].data(),
       old.class_dim[src_dim].data(), old.class_id[other].data(), old.class_dim[other].data(),
-      pid.data(), pdim.data());
-  nw.class_id[d] = transfer<LO>(c, mo, 1, old.class_id[d], &pid);
-  nw.class_dim[d] = transfer<I8>(c, mo, 1, old.class_dim[d], &pdim);
+      ids.data(), dims.data());
+  nw.class_id[d] = ids;
+  nw.class_dim[d] = dims;
 }
 
 void transfer_measures(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo) {
   LO nprods = mo.prods2new.size();
-  if (d == 1) {
-    Reals pd(c, nprods);
-    measure_edges(c, nw, nprods, mo.prods2new.data(), pd.data());
-    nw.length = transfer<Real>(c, mo, 1, ask_lengths(c, old), &pd);
-  }
-  if (d == old.dim) {
-    Reals pd(c, nprods);
-    measure_elems(c, nw, nprods, mo.prods2new.data(), pd.data());
-    nw.quality = transfer<Real>(c, mo, 1, ask_qualities(c, old), &pd);
-  }
+  LO nsame = mo.same2old.size();
+  if (d == 1)
+    nw.length = measure_transfer(c, nw, 1, nsame, mo.same2old.data(), mo.same2new.data(),
+        ask_lengths(c, old).data(), nprods, mo.prods2new.data());
+  if (d == old.dim)
+    nw.quality = measure_transfer(c, nw, d, nsame, mo.same2old.data(), mo.same2new.data(),
+        ask_qualities(c, old).data(), nprods, mo.prods2new.data());
 }
 
 Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& cands,

@@ -70,5 +70,8 @@ struct PassRecord {
 };
 
 bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log);
+// approach_metric (src/Omega_h_approach.cpp:83-126): move the metric toward
+// the "target_metric" vertex tag; false when there is no target
+bool approach_metric(Ctx& c, DMesh& m, AdaptOptions const& o, Real min_step);
 
 }  // namespace osh

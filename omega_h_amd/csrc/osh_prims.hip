@@ -65,6 +65,9 @@ Ctx::Ctx() : pool(std::make_shared<Pool>()) {
   OSH_HIP(hipHostMalloc(&pinned, 8 * kSlots + 64, hipHostMallocDefault));
   std::memset(pinned, 0, 8 * kSlots + 64);
   OSH_HIP(hipMalloc(&partials, 1024 * sizeof(double)));
+  OSH_HIP(hipMalloc((void**)&tile_counter, sizeof(unsigned)));
+  unsigned zero = 0;
+  OSH_HIP(hipMemcpy(tile_counter, &zero, sizeof(zero), hipMemcpyHostToDevice));
 }
 
 Ctx::~Ctx() {
@@ -72,6 +75,8 @@ Ctx::~Ctx() {
     (void)hipStreamSynchronize(stream);
     (void)hipStreamDestroy(stream);
   }
+  if (scan_status) (void)hipFree(scan_status);
+  if (tile_counter) (void)hipFree(tile_counter);
   if (pool) pool->close();
   if (partials) (void)hipFree(partials);
   if (pinned) (void)hipHostFree(pinned);
@@ -85,6 +90,8 @@ void Ctx::sync() {
     *err = kNoError;
     if (code == kReflectDownNoMatch)
       throw Error("reflect_down: entity without a unique match");
+    if (code == kScanTimeout) throw Error("scan: a tile waited too long for its predecessor");
+    if (code == kScanTicket) throw Error("scan: tile tickets out of step");
     throw Error("device error " + std::to_string(code));
   }
 }
@@ -132,46 +139,123 @@ __global__ void fill_kernel(LO n, T* out, T value) {
   if (i < n) out[i] = value;
 }
 
-// out[1 + i] = inclusive prefix within the tile; bsum[block] = tile total;
-// a single-tile scan also writes its total to *total when given
-template <class T>
-__global__ void __launch_bounds__(kBlock) scan_tiles(LO n, T const* in, LO* out, LO* bsum,
-    LO* total_out) {
-  LO base = LO(blockIdx.x) * kScanTile + LO(threadIdx.x) * kScanItems;
+// Single-pass exclusive scan with decoupled look-back.  Each workgroup takes
+// the next tile ticket from a counter (so every tile it waits on belongs to
+// a workgroup that is already running), scans its 2048 values, publishes
+// its aggregate, then walks back over the predecessors' published words
+// until it meets an inclusive prefix.  A status word is
+// [epoch:24 | flag:8 | value:32]; the epoch changes every scan, so words
+// left by earlier scans read as "not yet published" and no clearing pass is
+// needed.  The word carries the value itself, so relaxed agent-scope atomics
+// suffice.  A wait that outlasts kSpinLimit polls (it cannot happen on a
+// correct schedule) stops, sets the context's error word and lets every
+// workgroup finish; the next stream sync raises it.
+constexpr unsigned kFlagAggregate = 1u;
+constexpr unsigned kFlagInclusive = 2u;
+constexpr unsigned kSpinLimit = 1u << 20;
+
+__device__ __forceinline__ unsigned long long status_word(unsigned epoch, unsigned flag, LO v) {
+  return (static_cast<unsigned long long>(epoch) << 40) |
+         (static_cast<unsigned long long>(flag) << 32) | static_cast<unsigned>(v);
+}
+
+template <class T, bool NONZERO>
+__global__ void __launch_bounds__(kBlock) scan_lookback(LO n, T const* in, LO* out,
+    unsigned long long* status, unsigned* tile_counter, unsigned ticket_base, unsigned epoch,
+    LO* total_out, int* error) {
+  __shared__ unsigned s_tile;
+  __shared__ LO s_prefix;
+  if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u) - ticket_base;
+  __syncthreads();
+  unsigned tile = s_tile;
+  if (tile >= gridDim.x) {  // the ticket base is out of step: refuse to write
+    if (threadIdx.x == 0) *error = kScanTicket;
+    return;
+  }
+  LO base = LO(tile) * kScanTile + LO(threadIdx.x) * kScanItems;
   LO vals[kScanItems];
   LO acc = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     LO i = base + k;
-    LO v = (i < n) ? LO(in[i]) : 0;
+    LO v = 0;
+    if (i < n) v = NONZERO ? LO(in[i] != 0) : LO(in[i]);
     acc += v;
     vals[k] = acc;
   }
   LO total;
   LO excl = block_exclusive_scan(acc, &total);
+  if (threadIdx.x == 0) {
+    LO prefix = 0;
+    if (tile == 0) {
+      __hip_atomic_store(&status[0], status_word(epoch, kFlagInclusive, total), __ATOMIC_RELAXED,
+          __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(&status[tile], status_word(epoch, kFlagAggregate, total),
+          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      unsigned j = tile - 1;
+      unsigned spins = 0;
+      for (;;) {
+        unsigned long long w =
+            __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned flag = unsigned(w >> 32) & 0xffu;
+        if (unsigned(w >> 40) != epoch || flag == 0) {
+          if (++spins > kSpinLimit) {
+            *error = kScanTimeout;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        prefix += LO(unsigned(w));
+        if (flag == kFlagInclusive || j == 0) break;
+        --j;
+      }
+      __hip_atomic_store(&status[tile], status_word(epoch, kFlagInclusive, prefix + total),
+          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_prefix = prefix;
+    if (tile == 0) out[0] = 0;
+    if (total_out && tile == gridDim.x - 1) *total_out = prefix + total;
+  }
+  __syncthreads();
+  LO prefix = s_prefix + excl;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     LO i = base + k;
-    if (i < n) out[1 + i] = excl + vals[k];
-  }
-  if (threadIdx.x == 0) {
-    bsum[blockIdx.x] = total;
-    if (blockIdx.x == 0) out[0] = 0;
-    if (total_out && gridDim.x == 1) *total_out = total;
+    if (i < n) out[1 + i] = prefix + vals[k];
   }
 }
 
-__global__ void __launch_bounds__(kBlock) add_tile_offsets(LO n, LO* out, LO const* boff) {
-  LO i = LO(blockIdx.x) * kScanTile + LO(threadIdx.x);
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k, i += kBlock) {
-    if (i < n) out[1 + i] += boff[blockIdx.x];
-  }
+__global__ void zero_u64(LO n, unsigned long long* a) {
+  LO i = thread_index();
+  if (i < n) a[i] = 0ull;
 }
 
-// the grand total lands in *total_out (device-visible) when given: the
-// top level of the recursion is always a single tile
-template <class T>
+// the context's tile status words, grown (and cleared) when a scan needs
+// more tiles than they hold; a new epoch per scan, with a clearing pass when
+// the 24-bit epoch wraps
+unsigned long long* scan_status_words(Ctx& c, LO ntiles, unsigned* epoch) {
+  if (ntiles > c.scan_status_cap || c.scan_epoch >= (1u << 24)) {
+    LO cap = ntiles > c.scan_status_cap ? (ntiles > 2 * c.scan_status_cap ? ntiles
+                                                                          : 2 * c.scan_status_cap)
+                                        : c.scan_status_cap;
+    if (cap != c.scan_status_cap) {
+      if (c.scan_status) OSH_HIP(hipStreamSynchronize(c.stream));
+      if (c.scan_status) (void)hipFree(c.scan_status);
+      c.scan_status = nullptr;
+      OSH_HIP(hipMalloc(&c.scan_status, sizeof(unsigned long long) * std::size_t(cap)));
+      c.scan_status_cap = cap;
+    }
+    OSH_LAUNCH(zero_u64, cap, c, static_cast<unsigned long long*>(c.scan_status));
+    c.scan_epoch = 0;
+  }
+  *epoch = ++c.scan_epoch;
+  return static_cast<unsigned long long*>(c.scan_status);
+}
+
+// the grand total lands in *total_out (device-visible) when given
+template <class T, bool NONZERO>
 LOs offset_scan_t(Ctx& c, DArray<T> const& counts, LO* total_out) {
   LO n = counts.size();
   LOs out(c, n + 1);
@@ -181,23 +265,17 @@ LOs offset_scan_t(Ctx& c, DArray<T> const& counts, LO* total_out) {
     return out;
   }
   LO nb = LO((std::int64_t(n) + kScanTile - 1) / kScanTile);
-  LOs bsum(c, nb);
-  hipLaunchKernelGGL(scan_tiles<T>, dim3(nb), dim3(kBlock), 0, c.stream, n,
-      counts.data(), out.data(), bsum.data(), total_out);
+  unsigned epoch;
+  unsigned long long* status = scan_status_words(c, nb, &epoch);
+  unsigned ticket_base = c.tile_tickets;
+  c.tile_tickets += unsigned(nb);
+  hipLaunchKernelGGL((scan_lookback<T, NONZERO>), dim3(nb), dim3(kBlock), 0, c.stream, n,
+      counts.data(), out.data(), status, c.tile_counter, ticket_base, epoch, total_out,
+      c.error_word());
   OSH_HIP(hipGetLastError());
-  if (nb > 1) {
-    LOs boff = offset_scan_t<LO>(c, bsum, total_out);
-    hipLaunchKernelGGL(add_tile_offsets, dim3(nb), dim3(kBlock), 0, c.stream,
-        n, out.data(), boff.data());
-    OSH_HIP(hipGetLastError());
-  }
   return out;
 }
 
-__global__ void nonzero_flags(LO n, I8 const* marks, I8* flags) {
-  LO i = thread_index();
-  if (i < n) flags[i] = marks[i] != 0;
-}
 
 __global__ void scatter_marked(LO n, I8 const* marks, LO const* offsets, LO* out) {
   LO i = thread_index();
@@ -267,26 +345,25 @@ __global__ void gather_key_column(LO n, I64 const* keys, LO const* perm, int wid
 }  // namespace
 
 LOs offset_scan(Ctx& c, LOs const& counts, LO* total) {
-  LOs out = offset_scan_t<LO>(c, counts, total ? static_cast<LO*>(c.pinned) : nullptr);
+  LOs out = offset_scan_t<LO, false>(c, counts, total ? static_cast<LO*>(c.pinned) : nullptr);
   if (total) *total = pinned_value<LO>(c);
   return out;
 }
 LOs offset_scan_async(Ctx& c, LOs const& counts, int slot) {
   OSH_CHECK(slot >= 0 && slot < Ctx::kSlots, "pinned slot out of range");
-  return offset_scan_t<LO>(c, counts, slot_ptr<LO>(c, slot));
+  return offset_scan_t<LO, false>(c, counts, slot_ptr<LO>(c, slot));
 }
 LOs offset_scan_bytes(Ctx& c, Bytes const& counts, LO* total) {
-  LOs out = offset_scan_t<I8>(c, counts, total ? static_cast<LO*>(c.pinned) : nullptr);
+  LOs out = offset_scan_t<I8, false>(c, counts, total ? static_cast<LO*>(c.pinned) : nullptr);
   if (total) *total = pinned_value<LO>(c);
   return out;
 }
 
 LOs collect_marked(Ctx& c, Bytes const& marks) {
   LO n = marks.size();
-  Bytes flags(c, n);
-  OSH_LAUNCH(nonzero_flags, n, c, marks.data(), flags.data());
-  LO count;
-  LOs offsets = offset_scan_bytes(c, flags, &count);
+  // the scan counts nonzero marks directly
+  LOs offsets = offset_scan_t<I8, true>(c, marks, static_cast<LO*>(c.pinned));
+  LO count = pinned_value<LO>(c);
   LOs out(c, count);
   OSH_LAUNCH(scatter_marked, n, c, marks.data(), offsets.data(), out.data());
   return out;

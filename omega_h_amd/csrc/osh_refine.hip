@@ -271,6 +271,15 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
       Reals pm(c, nkeys * nc);
       mident_metrics(c, m, nkeys, keys2edges.data(), pm.data());
       nw.metric = transfer<Real>(c, mo, nc, m.metric, &pm);
+      for (auto const& t : m.vtags) {
+        Reals pt(c, nkeys * t.ncomps);
+        if (t.kind == 1)
+          mident_values(c, m, nkeys, keys2edges.data(), t.data.data(), t.ncomps, pt.data());
+        else
+          OSH_LAUNCH(midpoint_coords, nkeys, c, t.ncomps, keys2edges.data(), m.down[1].data(),
+              t.data.data(), pt.data());
+        nw.vtags.push_back({t.name, t.ncomps, t.kind, transfer<Real>(c, mo, t.ncomps, t.data, &pt)});
+      }
       transfer_class(c, m, nw, 0, mo, keys2edges, 1, nullptr);
     } else {
       transfer_class(c, m, nw, d, mo, psrc, d, &psrc_dim);

@@ -87,6 +87,13 @@ struct Ctx {
   int* error_word() const { return reinterpret_cast<int*>(static_cast<char*>(pinned) + 8 * kSlots); }
   // per-block partials of reductions
   void* partials = nullptr;
+  // single-pass scans: tile status words (tagged with a per-scan epoch) and
+  // the tile ticket counter, whose running total the host tracks
+  void* scan_status = nullptr;
+  LO scan_status_cap = 0;
+  unsigned scan_epoch = 0;
+  unsigned* tile_counter = nullptr;
+  unsigned tile_tickets = 0;
   // device copy of the edge-swap tables, uploaded on first use
   std::shared_ptr<void> swap_tables;
   std::shared_ptr<Pool> pool;
@@ -168,7 +175,7 @@ inline T* slot_ptr(Ctx& c, int slot) {
 }
 
 // error codes kernels leave in Ctx::error_word()
-enum DeviceError { kNoError = 0, kReflectDownNoMatch = 1 };
+enum DeviceError { kNoError = 0, kReflectDownNoMatch = 1, kScanTimeout = 2, kScanTicket = 3 };
 
 template <class T>
 inline T get_value(Ctx& c, T const* dptr) {

@@ -349,6 +349,63 @@ static void transfer_r(int nnew, int nc, mod_out const* mo, double const* old_da
   *dst = d;
 }
 
+static int metric_dim_of(int ncomps) { return ncomps == 1 ? 1 : (ncomps == 3 ? 2 : 3); }
+
+/* vertex tags of surviving vertices; with mo == NULL every vertex survives
+ * in place (swap) */
+static void copy_vtags(om_mesh const* m, om_mesh* nw, mod_out const* mo) {
+  nw->nvtags = m->nvtags;
+  for (int t = 0; t < m->nvtags; ++t) {
+    om_vtag const* a = &m->vtags[t];
+    om_vtag* b = &nw->vtags[t];
+    memcpy(b->name, a->name, sizeof(b->name));
+    b->ncomps = a->ncomps;
+    b->kind = a->kind;
+    if (mo) {
+      transfer_r(mo->nnew, a->ncomps, mo, a->data, NULL, &b->data);
+    } else {
+      size_t n = (size_t)m->nents[0] * a->ncomps;
+      b->data = (double*)om_malloc(sizeof(double) * (n + 1));
+      memcpy(b->data, a->data, sizeof(double) * n);
+    }
+  }
+}
+
+/* vertex tags at refinement midpoints */
+static void refine_vtags(om_mesh const* m, om_mesh* nw, int nkeys, int const* keys2edges,
+    mod_out const* mo) {
+  int const* ev2v = m->down[1];
+  nw->nvtags = m->nvtags;
+  for (int t = 0; t < m->nvtags; ++t) {
+    om_vtag const* a = &m->vtags[t];
+    om_vtag* b = &nw->vtags[t];
+    memcpy(b->name, a->name, sizeof(b->name));
+    b->ncomps = a->ncomps;
+    b->kind = a->kind;
+    int nc = a->ncomps;
+    double* pd = (double*)om_malloc(sizeof(double) * (size_t)(nkeys + 1) * nc);
+    for (int k = 0; k < nkeys; ++k) {
+      int e = keys2edges[k];
+      double const* x = a->data + (size_t)ev2v[e * 2] * nc;
+      double const* y = a->data + (size_t)ev2v[e * 2 + 1] * nc;
+      if (a->kind == OM_METRIC) {
+        double const* ms[2] = {x, y};
+        om_average_metric(metric_dim_of(nc), 2, ms, pd + (size_t)k * nc);
+      } else {
+        for (int j = 0; j < nc; ++j) {
+          double comp = 0;
+          comp += x[j];
+          comp += y[j];
+          comp /= 2;
+          pd[(size_t)k * nc + j] = comp;
+        }
+      }
+    }
+    transfer_r(mo->nnew, nc, mo, a->data, pd, &b->data);
+    free(pd);
+  }
+}
+
 /* class_id/class_dim of products from a list of old entities of dimension
  * src_dim (one per product) */
 static void transfer_class_from(om_mesh* old, om_mesh* nw, int d, int src_dim,
@@ -639,6 +696,7 @@ int om_refine_by_size(om_mesh* m, om_adapt_opts const* o) {
       mident_metrics(m, nkeys, keys2edges, pm);
       transfer_r(mo.nnew, nc, &mo, m->metric, pm, &nw->metric);
       free(pm);
+      refine_vtags(m, nw, nkeys, keys2edges, &mo);
       /* the midpoint inherits the key edge's classification */
       transfer_class_from(m, nw, 0, 1, keys2edges, &mo);
     } else {
@@ -954,6 +1012,7 @@ static int coarsen(om_mesh* m, om_adapt_opts const* o, signed char* edge_codes,
       memcpy(o2n_verts, mo.old2new, sizeof(int) * (size_t)nv);
       transfer_r(mo.nnew, dim, &mo, m->coords, NULL, &nw->coords);
       transfer_r(mo.nnew, nc, &mo, m->metric, NULL, &nw->metric);
+      copy_vtags(m, nw, &mo);
       transfer_i(mo.nnew, &mo, m->class_id[0], NULL, &nw->class_id[0]);
       transfer_b(mo.nnew, &mo, m->class_dim[0], NULL, &nw->class_dim[0]);
     } else {
@@ -1195,6 +1254,7 @@ static void swap_rebuild(om_mesh* m, int nkeys, int const* keys2edges,
   memcpy(nw->coords, m->coords, sizeof(double) * (size_t)nv * dim);
   nw->metric = (double*)om_malloc(sizeof(double) * (size_t)nv * nc);
   memcpy(nw->metric, m->metric, sizeof(double) * (size_t)nv * nc);
+  copy_vtags(m, nw, NULL);
   nw->class_id[0] = (int*)om_malloc(sizeof(int) * (size_t)nv);
   memcpy(nw->class_id[0], m->class_id[0], sizeof(int) * (size_t)nv);
   nw->class_dim[0] = (signed char*)om_malloc((size_t)nv);

@@ -111,3 +111,136 @@ int om_limit_metric_gradation(om_mesh* m, double max_rate, double tol) {
   free(values2);
   return steps;
 }
+
+/* vertex tags carried through adapt() (add_tag(VERT, name, ncomps, data)
+ * with a LINEAR_INTERP or METRIC transfer type) */
+static om_vtag* find_vtag(om_mesh* m, char const* name) {
+  for (int t = 0; t < m->nvtags; ++t)
+    if (strncmp(m->vtags[t].name, name, sizeof(m->vtags[t].name)) == 0) return &m->vtags[t];
+  return NULL;
+}
+
+int om_add_vtag(om_mesh* m, char const* name, int ncomps, int kind, double const* data) {
+  om_vtag* t = find_vtag(m, name);
+  if (!t) {
+    if (m->nvtags >= OM_MAX_VTAGS) return -1;
+    t = &m->vtags[m->nvtags++];
+    memset(t, 0, sizeof(*t));
+    strncpy(t->name, name, sizeof(t->name) - 1);
+  }
+  free(t->data);
+  t->ncomps = ncomps;
+  t->kind = kind;
+  size_t n = (size_t)m->nents[0] * ncomps;
+  t->data = (double*)om_malloc(sizeof(double) * (n + 1));
+  memcpy(t->data, data, sizeof(double) * n);
+  return 0;
+}
+
+int om_vtag_ncomps(om_mesh* m, char const* name) {
+  om_vtag* t = find_vtag(m, name);
+  return t ? t->ncomps : -1;
+}
+
+void om_get_vtag(om_mesh* m, char const* name, double* out) {
+  om_vtag* t = find_vtag(m, name);
+  if (t) memcpy(out, t->data, sizeof(double) * (size_t)m->nents[0] * t->ncomps);
+}
+
+void om_remove_vtag(om_mesh* m, char const* name) {
+  om_vtag* t = find_vtag(m, name);
+  if (!t) return;
+  free(t->data);
+  int i = (int)(t - m->vtags);
+  for (int j = i; j + 1 < m->nvtags; ++j) m->vtags[j] = m->vtags[j + 1];
+  --m->nvtags;
+}
+
+/* interpolate_between_metrics, src/Omega_h_metric.cpp:106-111: linearize
+ * (log), a (1 - t) + b t per stored component
+ * (src/Omega_h_array_ops.cpp:567-574), delinearize (exp) */
+void om_interpolate_metrics(int mdim, int n, double const* a, double const* b, double t,
+    double* out) {
+  int nc = om_symm_ncomps(mdim);
+  for (int i = 0; i < n; ++i) {
+    om_mat ma, mb, la, lb, r;
+    double va[6], vb[6], vc[6];
+    om_get_symm(mdim, a + (size_t)i * nc, &ma);
+    om_get_symm(mdim, b + (size_t)i * nc, &mb);
+    om_log_spd(mdim, &ma, &la);
+    om_log_spd(mdim, &mb, &lb);
+    om_set_symm(mdim, &la, va);
+    om_set_symm(mdim, &lb, vb);
+    for (int k = 0; k < nc; ++k) vc[k] = va[k] * (1.0 - t) + vb[k] * t;
+    om_mat lc;
+    om_get_symm(mdim, vc, &lc);
+    om_exp_spd(mdim, &lc, &r);
+    om_set_symm(mdim, &r, out + (size_t)i * nc);
+  }
+}
+
+static double min_quality(om_mesh* m) {
+  int ne = m->nents[m->dim];
+  double* q = (double*)om_malloc(sizeof(double) * (size_t)(ne + 1));
+  om_get_qualities(m, q);
+  double r = 1.79769313486231570815e+308;
+  for (int i = 0; i < ne; ++i) r = (q[i] < r) ? q[i] : r;
+  free(q);
+  return r;
+}
+
+static double max_length(om_mesh* m) {
+  int ne = m->nents[1];
+  double* l = (double*)om_malloc(sizeof(double) * (size_t)(ne + 1));
+  om_get_lengths(m, l);
+  double r = -1.79769313486231570815e+308;
+  for (int i = 0; i < ne; ++i) r = (r < l[i]) ? l[i] : r;
+  free(l);
+  return r;
+}
+
+static int okay(om_mesh* m, om_adapt_opts const* o) {
+  return min_quality(m) >= o->min_quality_allowed && !(max_length(m) > o->max_length_allowed);
+}
+
+/* approach_metric, src/Omega_h_approach.cpp:83-126: move "metric" toward the
+ * "target_metric" vertex tag by halving steps of log-Euclidean
+ * interpolation until the mesh is acceptable.  Returns 1 if it moved, 0
+ * without a target, -1 if the mesh is unacceptable to begin with and -2 if
+ * the step fell below min_step (the reference fails in both cases). */
+int om_approach_metric(om_mesh* m, om_adapt_opts const* o, double min_step) {
+  om_vtag* tg = find_vtag(m, "target_metric");
+  if (!tg) return 0;
+  if (min_quality(m) < o->min_quality_allowed) return -1;
+  if (max_length(m) > o->max_length_allowed) return -1;
+  int nc = m->metric_ncomps;
+  if (tg->ncomps != nc) return -1;
+  int nv = m->nents[0];
+  size_t n = (size_t)nv * nc;
+  double* orig = (double*)om_malloc(sizeof(double) * (n + 1));
+  double* target = (double*)om_malloc(sizeof(double) * (n + 1));
+  memcpy(orig, m->metric, sizeof(double) * n);
+  memcpy(target, tg->data, sizeof(double) * n);
+  om_set_metric(m, nc, target);
+  int r = 1;
+  if (okay(m, o)) {
+    om_remove_vtag(m, "target_metric");
+  } else {
+    double factor = 1.0;
+    double* cur = (double*)om_malloc(sizeof(double) * (n + 1));
+    int mdim = (nc == 1) ? 1 : m->dim;
+    do {
+      factor /= 2.0;
+      if (factor < min_step) {
+        r = -2;
+        break;
+      }
+      om_interpolate_metrics(mdim, nv, orig, target, factor, cur);
+      om_set_metric(m, nc, cur);
+    } while (!okay(m, o));
+    free(cur);
+  }
+  free(orig);
+  free(target);
+  return r;
+}

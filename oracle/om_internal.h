@@ -72,8 +72,24 @@ static inline signed char om_make_code(int flipped, int rot, int which_down) {
   return (signed char)((which_down << 3) | (rot << 1) | (flipped ? 1 : 0));
 }
 
+/* a vertex tag carried through adapt(): OM_LINEAR values are averaged at
+ * refinement midpoints (average_field, src/Omega_h_mesh.cpp:822-844), OM_METRIC
+ * values are log-Euclidean midpoint metrics (get_mident_metrics,
+ * src/Omega_h_metric.cpp:56-104); both are copied for surviving vertices
+ * (src/Omega_h_transfer.cpp:391-428,585-619,701-730) */
+#define OM_MAX_VTAGS 8
+enum { OM_LINEAR = 0, OM_METRIC = 1 };
+typedef struct {
+  char name[64];
+  int ncomps;
+  int kind;
+  double* data;
+} om_vtag;
+
 struct om_mesh {
   int dim;
+  int nvtags;
+  om_vtag vtags[OM_MAX_VTAGS];
   int nents[4];
   /* down[d] for d >= 1: nents[d] * (d+1) lower entities of dim d-1 */
   int* down[4];

@@ -168,6 +168,7 @@ void om_mesh_free(om_mesh* m) {
   }
   free(m->coords);
   free(m->metric);
+  for (int t = 0; t < m->nvtags; ++t) free(m->vtags[t].data);
   free(m->length);
   free(m->quality);
   free(m);

@@ -83,6 +83,14 @@ double om_kat_edge_length(int sdim, int mdim, double const* p0,
 double om_kat_aniso_length(double la, double lb);
 void om_kat_average_metric(int mdim, int n, double const* ms, double* out);
 void om_kat_intersect(int dim, double const* m1, double const* m2, double* out);
+/* vertex tags carried through adapt(): kind 0 linear interpolation, 1 metric */
+int om_add_vtag(om_mesh* m, char const* name, int ncomps, int kind, double const* data);
+int om_vtag_ncomps(om_mesh* m, char const* name);
+void om_get_vtag(om_mesh* m, char const* name, double* out);
+void om_remove_vtag(om_mesh* m, char const* name);
+void om_interpolate_metrics(int mdim, int n, double const* a, double const* b, double t,
+    double* out);
+int om_approach_metric(om_mesh* m, om_adapt_opts const* o, double min_step);
 /* limit_metric_gradation on the mesh's metric; returns the sweep count */
 int om_limit_metric_gradation(om_mesh* m, double max_rate, double tol);
 void om_kat_refine_qualities(om_mesh* m, int n, int const* cands, double* out);

@@ -86,6 +86,14 @@ def lib():
             getattr(L, name).argtypes = [P, C.POINTER(AdaptOpts)]
         L.om_adapt.argtypes = [P, C.POINTER(AdaptOpts), C.POINTER(AdaptStats)]
         L.om_limit_metric_gradation.argtypes = [P, C.c_double, C.c_double]
+        L.om_add_vtag.argtypes = [P, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.om_vtag_ncomps.argtypes = [P, C.c_char_p]
+        L.om_get_vtag.argtypes = [P, C.c_char_p, C.POINTER(C.c_double)]
+        L.om_remove_vtag.argtypes = [P, C.c_char_p]
+        L.om_interpolate_metrics.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double), C.c_double,
+                                             C.POINTER(C.c_double)]
+        L.om_approach_metric.argtypes = [P, C.POINTER(AdaptOpts), C.c_double]
         L.om_default_opts.argtypes = [C.c_int, C.POINTER(AdaptOpts)]
         for name in ("om_log", "om_exp", "om_cbrt", "om_acos", "om_cos"):
             getattr(L, name).restype = C.c_double
@@ -247,6 +255,23 @@ class Mesh:
     def limit_metric_gradation(self, max_rate, tol=1e-2):
         return lib().om_limit_metric_gradation(self.h, max_rate, tol)
 
+    LINEAR, METRIC = 0, 1
+
+    def add_vertex_tag(self, name, ncomps, kind, data):
+        a = np.ascontiguousarray(data, dtype=np.float64)
+        assert lib().om_add_vtag(self.h, name.encode(), ncomps, kind, _ptr(a, C.c_double)) == 0
+
+    def vertex_tag(self, name):
+        nc = lib().om_vtag_ncomps(self.h, name.encode())
+        if nc < 0:
+            return None
+        out = np.zeros(self.nents(0) * nc, dtype=np.float64)
+        lib().om_get_vtag(self.h, name.encode(), _ptr(out, C.c_double))
+        return out
+
+    def approach_metric(self, opts, min_step=1e-4):
+        return lib().om_approach_metric(self.h, C.byref(opts), min_step)
+
     def adapt(self, opts):
         st = AdaptStats()
         r = lib().om_adapt(self.h, C.byref(opts), C.byref(st))
@@ -301,6 +326,16 @@ def kat_intersect(dim, m1, m2):
     b = np.ascontiguousarray(m2, dtype=np.float64)
     out = np.zeros(dim * (dim + 1) // 2)
     lib().om_kat_intersect(dim, _ptr(a, C.c_double), _ptr(b, C.c_double), _ptr(out, C.c_double))
+    return out
+
+
+def interpolate_metrics(mdim, a, b, t):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    nc = mdim * (mdim + 1) // 2
+    out = np.zeros_like(a)
+    lib().om_interpolate_metrics(mdim, len(a) // nc, _ptr(a, C.c_double), _ptr(b, C.c_double),
+                                 t, _ptr(out, C.c_double))
     return out
 
 

@@ -64,3 +64,55 @@ def test_limit_metric_gradation_identical(ctx, dim, n):
     assert [p[:3] for p in po] == [p[:3] for p in pg]
     assert np.array_equal(o.verts(dim), g.verts(dim))
     assert np.array_equal(o.coords().view(np.int64), g.coords().view(np.int64))
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_interpolate_metrics_identical(ctx, dim):
+    rng = np.random.default_rng(10 + dim)
+    a, b = metric_pairs(dim, 500, rng)
+    # interpolation needs full-rank metrics: lift the null eigenvalues
+    nc = dim * (dim + 1) // 2
+    eye = O.symm2vector(np.eye(dim))
+    a = a + np.tile(eye, len(a) // nc)
+    b = b + np.tile(eye, len(b) // nc)
+    for t in (0.5, 0.125):
+        got = ctx.interpolate_metrics(dim, a, b, t)
+        want = O.interpolate_metrics(dim, a, b, t)
+        assert np.array_equal(got.view(np.int64), want.view(np.int64))
+
+
+@pytest.mark.parametrize("dim,n", [(2, 8), (3, 3)])
+def test_approach_metric_loop_identical(ctx, dim, n):
+    """The reference's idiom for a target metric (src/aniso_test.cpp:15-27):
+    while (approach_metric) adapt, with a linear vertex tag along for the
+    ride; every step bit for bit."""
+    o = O.Mesh.box(dim, n)
+    g = A.Mesh.box(ctx, dim, n)
+    c = o.coords().reshape(-1, dim)
+    if dim == 2:
+        init, nc = np.full(o.nents(0), 64.0), 1
+        target = 1.0 / (0.03 + 0.1 * c[:, 0]) ** 2
+    else:
+        init, nc = np.tile([9.0, 9.0, 9.0, 0.0, 0.0, 0.0], o.nents(0)), 6
+        target, _ = W.banded_aniso_metric(o.coords(), 3, np.random.default_rng(2))
+    for m in (o, g):
+        m.set_metric(init, nc)
+        m.add_vertex_tag("target_metric", nc, 1, target)
+        m.add_vertex_tag("xyz", dim, 0, o.coords())
+    steps = 0
+    while True:
+        ro = o.approach_metric(O.default_opts(dim))
+        rg = g.approach_metric(ctx.default_opts(dim))
+        assert ro == rg
+        if ro == 0:
+            break
+        o.adapt(O.default_opts(dim))
+        g.adapt(ctx.default_opts(dim))
+        steps += 1
+        assert np.array_equal(o.verts(dim), g.verts(dim))
+        assert np.array_equal(o.metric().view(np.int64), g.metric().view(np.int64))
+        assert np.array_equal(o.vertex_tag("xyz").view(np.int64),
+                              g.vertex_tag("xyz").view(np.int64))
+    assert steps >= 2
+    # the linear tag started as the coordinates and moved with them
+    assert np.array_equal(g.vertex_tag("xyz"), g.coords())

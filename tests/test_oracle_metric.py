@@ -165,3 +165,60 @@ def test_gradation_leaves_a_graded_metric_alone():
     m.set_metric(np.full(m.nents(0), 4.0), 1)
     assert m.limit_metric_gradation(1.0, 1e-6) == 1
     assert np.allclose(m.metric(), 4.0, rtol=1e-13)
+
+
+def test_interpolate_metrics_closed_forms():
+    rng = np.random.default_rng(8)
+    a = np.exp(rng.uniform(-2, 2, 50))
+    b = np.exp(rng.uniform(-2, 2, 50))
+    for t in (0.0, 0.25, 0.5, 1.0):
+        got = O.interpolate_metrics(1, a, b, t)
+        assert np.allclose(got, a ** (1 - t) * b ** t, rtol=1e-13)
+    # full 3D: t = 0 and t = 1 give the end points
+    ms = []
+    for _ in range(2):
+        r = rot3(rng)
+        ms.append(O.symm2vector(r @ np.diag(np.exp(rng.uniform(-3, 3, 3))) @ r.T))
+    for t, want in ((0.0, ms[0]), (1.0, ms[1])):
+        got = O.interpolate_metrics(3, ms[0], ms[1], t)
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("dim,n", [(2, 10), (3, 4)])
+def test_vertex_tags_follow_the_passes(dim, n):
+    from omega_h_amd import workloads as W
+    m = O.Mesh.box(dim, n)
+    met, nc = W.iso_ring_metric_2d(m.coords()) if dim == 2 else W.levelset_metric_3d(m.coords())
+    m.set_metric(met, nc)
+    # coordinates as a linear tag and the metric as a metric tag are moved
+    # exactly as the mesh moves its own coordinates and metric
+    m.add_vertex_tag("xyz", dim, O.Mesh.LINEAR, m.coords())
+    m.add_vertex_tag("m2", nc, O.Mesh.METRIC, met)
+    m.adapt(O.default_opts(dim))
+    assert np.array_equal(m.vertex_tag("xyz"), m.coords())
+    assert np.array_equal(m.vertex_tag("m2"), m.metric())
+
+
+def test_approach_metric_reaches_the_target():
+    m = O.Mesh.box(2, 8)
+    c = m.coords().reshape(-1, 2)
+    h0 = 1.0 / 8
+    m.set_metric(np.full(m.nents(0), 1.0 / h0 ** 2), 1)
+    target = 1.0 / (0.03 + 0.1 * c[:, 0]) ** 2
+    m.add_vertex_tag("target_metric", 1, O.Mesh.METRIC, target)
+    # a copy of the target that is only transported, never consumed
+    m.add_vertex_tag("copy", 1, O.Mesh.METRIC, target)
+    opts = O.default_opts(2)
+    steps = 0
+    while True:
+        r = m.approach_metric(opts)
+        assert r >= 0
+        if r == 0:
+            break
+        m.adapt(opts)
+        steps += 1
+        assert steps < 50
+    assert steps >= 2
+    assert m.vertex_tag("target_metric") is None
+    # the final metric is the target itself, transported to the new vertices
+    assert np.array_equal(m.metric(), m.vertex_tag("copy"))
