@@ -18,6 +18,7 @@
 
 #include <cstdint>
 #include <initializer_list>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -39,6 +40,9 @@ using Real = double;
 enum { VERT = 0, EDGE = 1, FACE = 2, REGION = 3 };
 
 enum Verbosity { SILENT = 0, EACH_ADAPT = 1, EACH_REBUILD = 2, EXTRA_STATS = 3 };
+
+// transfer types of vertex tags carried through adapt() (src/Omega_h_adapt.hpp:30-42)
+enum Omega_h_Transfer { OMEGA_H_LINEAR_INTERP = 0, OMEGA_H_METRIC = 1 };
 
 struct exception : public std::runtime_error {
   explicit exception(std::string const& msg) : std::runtime_error(msg) {}
@@ -261,8 +265,18 @@ class Mesh {
   }
 
   // tags on this path: "coordinates", "metric", "class_id", "class_dim",
-  // "length" (edges), "quality" (elements); only "metric" is set by callers
+  // "length" (edges), "quality" (elements), and caller vertex tags carried
+  // through adapt() ("target_metric" and any field transferred by linear
+  // interpolation or as a metric)
+  int vertex_tag_ncomps(std::string const& name) const {
+    int nc;
+    check_status(osh_mesh_vertex_tag_ncomps(handle(), name.c_str(), &nc));
+    return nc;
+  }
   bool has_tag(Int ent_dim, std::string const& name) const {
+    if (ent_dim == VERT && name != "metric" && name != "coordinates" &&
+        vertex_tag_ncomps(name) >= 0)
+      return true;
     if (name == "coordinates") return ent_dim == VERT;
     if (name == "metric") {
       int nc;
@@ -273,6 +287,9 @@ class Mesh {
     if (name == "length") return ent_dim == EDGE;
     if (name == "quality") return ent_dim == dim();
     return false;
+  }
+  void remove_tag(Int ent_dim, std::string const& name) {
+    if (ent_dim == VERT) check_status(osh_mesh_remove_vertex_tag(handle(), name.c_str()));
   }
   template <class T>
   void add_tag(Int ent_dim, std::string const& name, Int ncomps, Read<T> array,
@@ -288,13 +305,20 @@ class Mesh {
 };
 
 namespace detail {
+// "metric" is the mesh's metric; other vertex tags are carried through
+// adapt() as metrics when their name ends in "metric" ("target_metric"),
+// otherwise by linear interpolation (AdaptOpts::xfer_opts can say otherwise)
 inline void set_metric(Mesh& m, Int ent_dim, std::string const& name, Int ncomps,
     Real const* data, LO size) {
-  if (name != "metric" || ent_dim != VERT)
-    throw exception("only the vertex \"metric\" tag is carried through adapt(), not \"" +
-                    name + "\"");
-  if (size != m.nverts() * ncomps) throw exception("metric array has the wrong size");
-  check_status(osh_mesh_set_metric(m.handle(), ncomps, data));
+  if (ent_dim != VERT) throw exception("only vertex tags are carried through adapt()");
+  if (size != m.nverts() * ncomps) throw exception("tag array has the wrong size");
+  if (name == "metric") {
+    check_status(osh_mesh_set_metric(m.handle(), ncomps, data));
+    return;
+  }
+  bool is_metric = name.size() >= 6 && name.compare(name.size() - 6, 6, "metric") == 0;
+  check_status(osh_mesh_add_vertex_tag(m.handle(), name.c_str(), ncomps,
+      is_metric ? OMEGA_H_METRIC : OMEGA_H_LINEAR_INTERP, data));
 }
 }  // namespace detail
 
@@ -307,6 +331,8 @@ inline void Mesh::add_tag<Real>(Int ent_dim, std::string const& name, Int ncomps
 template <>
 inline void Mesh::set_tag<Real>(Int ent_dim, std::string const& name, Read<Real> array, bool) {
   int nc = (nverts() > 0) ? array.size() / nverts() : 0;
+  if (ent_dim == VERT && name != "metric" && vertex_tag_ncomps(name) > 0)
+    nc = vertex_tag_ncomps(name);
   detail::set_metric(*this, ent_dim, name, nc, array.data(), array.size());
 }
 
@@ -322,6 +348,14 @@ inline Read<Real> Mesh::get_array<Real>(Int ent_dim, std::string const& name) co
   }
   if (name == "length" && ent_dim == EDGE) return ask_lengths();
   if (name == "quality" && ent_dim == dim()) return ask_qualities();
+  if (ent_dim == VERT) {
+    int nc = vertex_tag_ncomps(name);
+    if (nc >= 0) {
+      Write<Real> w(nverts() * nc);
+      check_status(osh_mesh_get_vertex_tag(handle(), name.c_str(), w.data()));
+      return w;
+    }
+  }
   throw exception("no Real tag \"" + name + "\" on dimension " + std::to_string(ent_dim));
 }
 
@@ -358,6 +392,11 @@ inline Mesh build_box(CommPtr comm, Omega_h_Family family, Real x, Real y, Real 
   return Mesh(lib, h);
 }
 
+// src/Omega_h_adapt.hpp:30-42: the transfer type of each caller vertex tag
+struct TransferOpts {
+  std::map<std::string, Omega_h_Transfer> type_map;
+};
+
 // src/Omega_h_adapt.hpp:50-77, defaults src/Omega_h_adapt.cpp:52-85
 struct AdaptOpts {
   AdaptOpts() = default;
@@ -378,6 +417,7 @@ struct AdaptOpts {
   bool should_coarsen = true;
   bool should_swap = true;
   bool should_coarsen_slivers = true;
+  TransferOpts xfer_opts;
 
   osh_adapt_opts to_c() const {
     osh_adapt_opts o;
@@ -409,8 +449,23 @@ struct AdaptOpts {
   }
 };
 
+namespace detail {
+// re-register a vertex tag whose transfer type the options name differently
+inline void apply_transfer_types(Mesh* mesh, AdaptOpts const& opts) {
+  for (auto const& kv : opts.xfer_opts.type_map) {
+    int nc = mesh->vertex_tag_ncomps(kv.first);
+    if (nc < 0) continue;
+    Write<Real> w(mesh->nverts() * nc);
+    check_status(osh_mesh_get_vertex_tag(mesh->handle(), kv.first.c_str(), w.data()));
+    check_status(osh_mesh_add_vertex_tag(mesh->handle(), kv.first.c_str(), nc, kv.second,
+        w.data()));
+  }
+}
+}  // namespace detail
+
 // src/Omega_h_adapt.cpp:272-288: false iff the mesh already met the goals
 inline bool adapt(Mesh* mesh, AdaptOpts const& opts) {
+  detail::apply_transfer_types(mesh, opts);
   osh_adapt_opts o = opts.to_c();
   int did = 0;
   check_status(osh_adapt(mesh->handle(), &o, nullptr, &did));
@@ -438,6 +493,14 @@ inline bool coarsen_slivers(Mesh* mesh, AdaptOpts const& opts) {
 }
 inline bool swap_edges(Mesh* mesh, AdaptOpts const& opts) {
   return detail::run_pass(osh_swap_edges, mesh, opts);
+}
+
+// src/Omega_h_approach.cpp:83-126 (declared src/Omega_h_adapt.hpp:90-92)
+inline bool approach_metric(Mesh* mesh, AdaptOpts const& opts, Real min_step = 1e-4) {
+  osh_adapt_opts o = opts.to_c();
+  int did = 0;
+  check_status(osh_approach_metric(mesh->handle(), &o, min_step, &did));
+  return did != 0;
 }
 
 }  // namespace Omega_h

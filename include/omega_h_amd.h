@@ -153,6 +153,12 @@ int osh_mesh_remove_vertex_tag(osh_mesh* mesh, const char* name);
  * *did = 0 when there is no target.  The reference's idiom
  * (src/aniso_test.cpp:15-27) is: while (approach_metric) adapt. */
 int osh_approach_metric(osh_mesh* mesh, const osh_adapt_opts* opts, double min_step, int* did);
+/* get_implied_metrics(mesh), src/Omega_h_metric.cpp:280-309: the metric each
+ * element implies (all its edges of unit length, src/Omega_h_shape.hpp:240-273),
+ * scaled to a typical unit element and projected to the vertices by
+ * log-Euclidean averaging; out holds nverts * (3 in 2D, 6 in 3D) values, and
+ * with set_as_metric != 0 it also becomes the mesh's metric */
+int osh_get_implied_metrics(osh_mesh* mesh, int set_as_metric, double* out);
 /* interpolate_between_metrics, src/Omega_h_metric.cpp:106-111, for n pairs of
  * metrics of dimension dim in host arrays */
 int osh_interpolate_metrics(osh_ctx* ctx, int dim, int n, const double* a, const double* b,

@@ -133,6 +133,7 @@ def load(path=None):
         "osh_mesh_remove_vertex_tag": [P, C.c_char_p],
         "osh_approach_metric": [P, C.POINTER(AdaptOpts), C.c_double, IP],
         "osh_interpolate_metrics": [P, I, I, DP, DP, C.c_double, DP],
+        "osh_get_implied_metrics": [P, I, DP],
         "osh_server_create": [I, I, C.c_double, C.POINTER(P)],
         "osh_server_destroy": [P],
         "osh_server_submit": [P, I, I, I, I, I, DP, C.POINTER(AdaptOpts), I, IP],
@@ -383,6 +384,14 @@ class Mesh:
 
     def remove_vertex_tag(self, name):
         check(self.lib.osh_mesh_remove_vertex_tag(self.h, name.encode()), self.lib)
+
+    def implied_metrics(self, set_as_metric=False):
+        """get_implied_metrics; optionally make them the mesh's metric."""
+        nc = self.dim * (self.dim + 1) // 2
+        out = np.zeros(self.nents(0) * nc, dtype=np.float64)
+        check(self.lib.osh_get_implied_metrics(self.h, int(set_as_metric), _p(out, C.c_double)),
+              self.lib)
+        return out
 
     def approach_metric(self, opts, min_step=1e-4):
         did = C.c_int()

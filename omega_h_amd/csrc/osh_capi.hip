@@ -510,6 +510,22 @@ int osh_approach_metric(osh_mesh* mesh, const osh_adapt_opts* opts, double min_s
   });
 }
 
+int osh_get_implied_metrics(osh_mesh* mesh, int set_as_metric, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    auto& c = mesh->owner->ctx;
+    osh::Reals r = osh::implied_metrics(c, mesh->m);
+    if (out) download(c, r, out);
+    if (set_as_metric) {
+      mesh->m.metric = r;
+      mesh->m.metric_ncomps = (mesh->m.dim * (mesh->m.dim + 1)) / 2;
+      mesh->m.length = osh::Reals();
+      mesh->m.quality = osh::Reals();
+    }
+    c.sync();
+  });
+}
+
 int osh_interpolate_metrics(osh_ctx* ctx, int dim, int n, const double* a, const double* b,
     double t, double* out) {
   return guarded([&] {

@@ -72,6 +72,8 @@ LOs find_unique(Ctx& c, LO nhigh, int high_dim, int low_dim, LOs const& hv2v);
 int limit_metric_gradation(Ctx& c, DMesh& m, Real max_rate, Real tol);
 void intersect_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals const& b,
     Reals& out);
+// get_implied_metrics (src/Omega_h_metric.cpp:280-309): nverts x (3 or 6)
+Reals implied_metrics(Ctx& c, DMesh& m);
 // interpolate_between_metrics (src/Omega_h_metric.cpp:106-111) of n metrics
 void interpolate_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals const& b, Real t,
     Reals& out);

@@ -71,6 +71,55 @@ __global__ void interpolate_kernel(LO n, Real const* a, Real const* b, Real t, R
   set_symm<M>(exp_spd<M>(get_symm<M>(vc)), out + std::size_t(i) * nc);
 }
 
+// size-scaled implied metric of each element (src/Omega_h_metric.cpp:257-301)
+template <int D>
+__global__ void element_implied_kernel(LO ne, LO const* ev2v, Real const* coords, Real scalar,
+    Real* e2m) {
+  LO e = xcd_thread_index();
+  if (e >= ne) return;
+  constexpr int nc = (D * (D + 1)) / 2;
+  double p[D + 1][3];
+  for (int k = 0; k <= D; ++k) {
+    LO v = ev2v[e * (D + 1) + k];
+    for (int j = 0; j < D; ++j) p[k][j] = coords[std::size_t(v) * D + j];
+  }
+  double x[6];
+  element_implied_metric<D>(p, x);
+  for (int k = 0; k < nc; ++k) e2m[std::size_t(e) * nc + k] = x[k] * scalar;
+}
+
+// project_metrics, src/Omega_h_metric.cpp:221-245: log-Euclidean average of
+// the metrics of a vertex's elements (average_metric_contrib/_finish with
+// degenerate metrics skipped, src/Omega_h_metric.hpp:125-139)
+template <int D>
+__global__ void project_metrics_kernel(LO nv, LO const* a2ab, LO const* ab2b, Real const* e2m,
+    Real* out) {
+  LO v = thread_index();
+  if (v >= nv) return;
+  constexpr int nc = (D * (D + 1)) / 2;
+  Mat am;
+  mat_zero(am);
+  int n = 0;
+  for (LO j = a2ab[v]; j < a2ab[v + 1]; ++j) {
+    Mat em = get_symm<D>(e2m + std::size_t(ab2b[j]) * nc);
+    double mx = 0.0;
+    for (int c = 0; c < D; ++c)
+      for (int r = 0; r < D; ++r) mx = math::max2(mx, math::fabs_(em.m[c][r]));
+    if (mx < kMetricEpsilon) continue;
+    Mat lm = log_spd<D>(em);
+    for (int c = 0; c < D; ++c)
+      for (int r = 0; r < D; ++r) am.m[c][r] = am.m[c][r] + lm.m[c][r];
+    ++n;
+  }
+  Mat r = am;
+  if (n > 0) {
+    for (int c = 0; c < D; ++c)
+      for (int rr = 0; rr < D; ++rr) am.m[c][rr] = am.m[c][rr] / n;
+    r = exp_spd<D>(am);
+  }
+  set_symm<D>(r, out + std::size_t(v) * nc);
+}
+
 Reals limit_gradation_once(Ctx& c, DMesh& m, Reals const& values, Real max_rate) {
   LO nv = m.nents[0];
   Adj const& star = ask_star(c, m, 0);
@@ -117,6 +166,31 @@ int limit_metric_gradation(Ctx& c, DMesh& m, Real max_rate, Real tol) {
   m.length = Reals();
   m.quality = Reals();
   return steps;
+}
+
+Reals implied_metrics(Ctx& c, DMesh& m) {
+  int dim = m.dim;
+  int nc = (dim * (dim + 1)) / 2;
+  LO ne = m.nents[dim];
+  // power(typical_unit_simplex_size / equilateral_simplex_size, 2, dim)
+  double typical = dim == 3 ? 0.0838934100219 : 0.377645136635;
+  double perfect = dim == 3 ? 0.1178511301977579 : 0.4330127018922193;
+  double s = typical / perfect;
+  double scalar = dim == 3 ? math::cbrt_(s * s) : s;
+  Reals e2m(c, ne * nc);
+  LOs const& ev2v = verts_of(c, m, dim);
+  Adj const& v2e = ask_up(c, m, 0, dim);
+  Reals out(c, m.nents[0] * nc);
+  if (dim == 3) {
+    OSH_LAUNCH(element_implied_kernel<3>, ne, c, ev2v.data(), m.coords.data(), scalar, e2m.data());
+    OSH_LAUNCH(project_metrics_kernel<3>, m.nents[0], c, v2e.a2ab.data(), v2e.ab2b.data(),
+        e2m.data(), out.data());
+  } else {
+    OSH_LAUNCH(element_implied_kernel<2>, ne, c, ev2v.data(), m.coords.data(), scalar, e2m.data());
+    OSH_LAUNCH(project_metrics_kernel<2>, m.nents[0], c, v2e.a2ab.data(), v2e.ab2b.data(),
+        e2m.data(), out.data());
+  }
+  return out;
 }
 
 void interpolate_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals const& b, Real t,

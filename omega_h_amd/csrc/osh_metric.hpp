@@ -304,4 +304,86 @@ OSH_HD Mat intersect_in_plane(Mat const& m1, Mat const& m2, double const* u1,
   return matmul_rect<3, 2, 3>(p, matmul_rect<2, 2, 3>(mint, pt));
 }
 
+// Householder least-squares / square solve of an m x n system (n <= m <= 6),
+// src/Omega_h_qr.hpp:18-149; a[col][row]
+OSH_HD void solve_using_qr(int m, int n, double (*a)[6], double const* b_in, double* x) {
+  double v[6][6];
+  for (int k = 0; k < n; ++k) {
+    double norm_x = 0.0;
+    for (int i = k; i < m; ++i) norm_x += a[k][i] * a[k][i];
+    norm_x = math::sqrt_(norm_x);
+    for (int i = k; i < m; ++i) v[k][i] = a[k][i];
+    v[k][k] += ((a[k][k] < 0.0) ? -1.0 : 1.0) * norm_x;
+    double norm_v = 0.0;
+    for (int i = k; i < m; ++i) norm_v += v[k][i] * v[k][i];
+    norm_v = math::sqrt_(norm_v);
+    for (int i = k; i < m; ++i) v[k][i] /= norm_v;
+    for (int j = k; j < n; ++j) {
+      double dot = 0.0;
+      for (int i = k; i < m; ++i) dot += a[j][i] * v[k][i];
+      for (int i = k; i < m; ++i) a[j][i] -= 2.0 * dot * v[k][i];
+    }
+  }
+  double b[6];
+  for (int i = 0; i < m; ++i) b[i] = b_in[i];
+  for (int k = 0; k < n; ++k) {
+    double dot = 0.0;
+    for (int i = k; i < m; ++i) dot += v[k][i] * b[i];
+    for (int i = k; i < m; ++i) b[i] -= 2.0 * dot * v[k][i];
+  }
+  for (int ii = 0; ii < n; ++ii) {
+    int i = n - ii - 1;
+    x[i] = b[i];
+    for (int j = i + 1; j < n; ++j) x[i] -= a[j][i] * x[j];
+    x[i] /= a[i][i];
+  }
+}
+
+// element_implied_metric, src/Omega_h_shape.hpp:240-273: the metric in
+// which every edge of the element has unit length, in symmetric storage
+template <int D>
+OSH_HD void element_implied_metric(double const (*p)[3], double* out) {
+  double b[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  double ev[6][3];
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) b[i][j] = p[i + 1][j] - p[0][j];
+  if constexpr (D == 3) {
+    for (int j = 0; j < 3; ++j) {
+      ev[0][j] = b[0][j];
+      ev[1][j] = p[2][j] - p[1][j];
+      ev[2][j] = -b[1][j];
+      ev[3][j] = b[2][j];
+      ev[4][j] = p[3][j] - p[1][j];
+      ev[5][j] = p[3][j] - p[2][j];
+    }
+    double a[6][6], rhs[6];
+    for (int i = 0; i < 6; ++i) {
+      a[0][i] = ev[i][0] * ev[i][0];
+      a[1][i] = ev[i][1] * ev[i][1];
+      a[2][i] = ev[i][2] * ev[i][2];
+      a[3][i] = 2 * ev[i][0] * ev[i][1];
+      a[4][i] = 2 * ev[i][1] * ev[i][2];
+      a[5][i] = 2 * ev[i][0] * ev[i][2];
+      rhs[i] = 1.0;
+    }
+    solve_using_qr(6, 6, a, rhs, out);
+  } else {
+    for (int j = 0; j < 2; ++j) {
+      ev[0][j] = b[0][j];
+      ev[1][j] = p[2][j] - p[1][j];
+      ev[2][j] = -b[1][j];
+    }
+    Mat a;
+    mat_zero(a);
+    for (int i = 0; i < 3; ++i) {
+      a.m[0][i] = ev[i][0] * ev[i][0];
+      a.m[1][i] = ev[i][1] * ev[i][1];
+      a.m[2][i] = 2 * ev[i][0] * ev[i][1];
+    }
+    Mat inv = invert<3>(a);
+    double rhs[3] = {1.0, 1.0, 1.0};
+    matvec<3>(inv, rhs, out);
+  }
+}
+
 }  // namespace osh

@@ -91,6 +91,9 @@ void om_remove_vtag(om_mesh* m, char const* name);
 void om_interpolate_metrics(int mdim, int n, double const* a, double const* b, double t,
     double* out);
 int om_approach_metric(om_mesh* m, om_adapt_opts const* o, double min_step);
+/* get_implied_metrics: nverts * (3 or 6) values */
+void om_get_implied_metrics(om_mesh* m, double* out);
+void om_element_implied_metric(int dim, double const* const* p, double* out);
 /* limit_metric_gradation on the mesh's metric; returns the sweep count */
 int om_limit_metric_gradation(om_mesh* m, double max_rate, double tol);
 void om_kat_refine_qualities(om_mesh* m, int n, int const* cands, double* out);

@@ -94,6 +94,9 @@ def lib():
                                              C.POINTER(C.c_double), C.c_double,
                                              C.POINTER(C.c_double)]
         L.om_approach_metric.argtypes = [P, C.POINTER(AdaptOpts), C.c_double]
+        L.om_get_implied_metrics.argtypes = [P, C.POINTER(C.c_double)]
+        L.om_element_implied_metric.argtypes = [C.c_int, C.POINTER(C.c_void_p),
+                                                C.POINTER(C.c_double)]
         L.om_default_opts.argtypes = [C.c_int, C.POINTER(AdaptOpts)]
         for name in ("om_log", "om_exp", "om_cbrt", "om_acos", "om_cos"):
             getattr(L, name).restype = C.c_double
@@ -272,6 +275,12 @@ class Mesh:
     def approach_metric(self, opts, min_step=1e-4):
         return lib().om_approach_metric(self.h, C.byref(opts), min_step)
 
+    def implied_metrics(self):
+        nc = self.dim * (self.dim + 1) // 2
+        out = np.zeros(self.nents(0) * nc, dtype=np.float64)
+        lib().om_get_implied_metrics(self.h, _ptr(out, C.c_double))
+        return out
+
     def adapt(self, opts):
         st = AdaptStats()
         r = lib().om_adapt(self.h, C.byref(opts), C.byref(st))
@@ -326,6 +335,16 @@ def kat_intersect(dim, m1, m2):
     b = np.ascontiguousarray(m2, dtype=np.float64)
     out = np.zeros(dim * (dim + 1) // 2)
     lib().om_kat_intersect(dim, _ptr(a, C.c_double), _ptr(b, C.c_double), _ptr(out, C.c_double))
+    return out
+
+
+def element_implied_metric(pts):
+    """pts: (dim+1) x dim vertex coordinates; the symmetric-storage metric."""
+    pts = [np.ascontiguousarray(p, dtype=np.float64) for p in pts]
+    arr = (C.c_void_p * len(pts))(*[p.ctypes.data for p in pts])
+    dim = len(pts) - 1
+    out = np.zeros(dim * (dim + 1) // 2)
+    lib().om_element_implied_metric(dim, arr, _ptr(out, C.c_double))
     return out
 
 

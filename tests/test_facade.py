@@ -50,3 +50,33 @@ def test_facade_adapt_matches_oracle(tmp_path, dim, n):
     for name, want in (("coords", o.coords()), ("quality", o.qualities())):
         got = np.fromfile(prefix + "." + name, dtype=np.float64)
         assert np.array_equal(got.view(np.int64), want.view(np.int64)), name
+
+
+@pytest.mark.gpu
+def test_facade_approach_loop_matches_oracle(tmp_path):
+    prog = os.path.join(ROOT, "omega_h_amd", "_build", "facade_approach")
+    assert os.path.exists(prog), "build the facade programs with __graft_entry__.build()"
+    o = O.Mesh.box(2, 8)
+    c = o.coords().reshape(-1, 2)
+    init = np.full(o.nents(0), 64.0)
+    target = 1.0 / (0.03 + 0.1 * c[:, 0]) ** 2
+    init.tofile(tmp_path / "m.f64")
+    target.tofile(tmp_path / "t.f64")
+    prefix = str(tmp_path / "out")
+    print("facade approach 2D n=8", flush=True)
+    r = subprocess.run([prog, "2", "8", "1", str(tmp_path / "m.f64"), str(tmp_path / "t.f64"),
+                        prefix], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    o.set_metric(init, 1)
+    o.add_vertex_tag("target_metric", 1, O.Mesh.METRIC, target)
+    steps = 0
+    while o.approach_metric(O.default_opts(2)) > 0:
+        o.adapt(O.default_opts(2))
+        steps += 1
+    counts = np.fromfile(prefix + ".counts", dtype=np.int32)
+    assert list(counts[:-1]) == [o.nents(d) for d in range(3)]
+    assert counts[-1] == steps
+    assert np.array_equal(np.fromfile(prefix + ".verts", dtype=np.int32), o.verts(2))
+    for name, want in (("coords", o.coords()), ("metric", o.metric())):
+        got = np.fromfile(prefix + "." + name, dtype=np.float64)
+        assert np.array_equal(got.view(np.int64), want.view(np.int64)), name

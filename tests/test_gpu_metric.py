@@ -116,3 +116,37 @@ def test_approach_metric_loop_identical(ctx, dim, n):
     assert steps >= 2
     # the linear tag started as the coordinates and moved with them
     assert np.array_equal(g.vertex_tag("xyz"), g.coords())
+
+
+@pytest.mark.parametrize("dim,n", [(2, 12), (3, 6)])
+def test_implied_metrics_identical(ctx, dim, n):
+    o = O.Mesh.box(dim, n)
+    g = A.Mesh.box(ctx, dim, n)
+    assert np.array_equal(o.implied_metrics().view(np.int64),
+                          g.implied_metrics().view(np.int64))
+
+
+def test_adapt_to_a_target_from_the_implied_metric(ctx):
+    """src/aniso_test.cpp:15-27 on a 3D box: the implied metric becomes the
+    metric, the level-set metric the target, then approach + adapt until
+    the target is reached; identical to the oracle at every step."""
+    o = O.Mesh.box(3, 3)
+    g = A.Mesh.box(ctx, 3, 3)
+    implied = g.implied_metrics(set_as_metric=True)
+    target, nc = W.levelset_metric_3d(o.coords(), h_min=0.02, h_max=0.2, width=0.1)
+    o.set_metric(implied, 6)
+    for m in (o, g):
+        m.add_vertex_tag("target_metric", nc, 1, target)
+    steps = 0
+    while True:
+        ro = o.approach_metric(O.default_opts(3))
+        rg = g.approach_metric(ctx.default_opts(3))
+        assert ro == rg
+        if ro == 0:
+            break
+        o.adapt(O.default_opts(3))
+        g.adapt(ctx.default_opts(3))
+        steps += 1
+        assert np.array_equal(o.verts(3), g.verts(3))
+        assert np.array_equal(o.metric().view(np.int64), g.metric().view(np.int64))
+    assert steps >= 1

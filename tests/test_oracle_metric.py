@@ -222,3 +222,34 @@ def test_approach_metric_reaches_the_target():
     assert m.vertex_tag("target_metric") is None
     # the final metric is the target itself, transported to the new vertices
     assert np.array_equal(m.metric(), m.vertex_tag("copy"))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_element_implied_metric_gives_unit_edges(dim):
+    rng = np.random.default_rng(12 + dim)
+    for _ in range(50):
+        pts = [rng.random(dim) for _ in range(dim + 1)]
+        m = O.vector2symm(O.element_implied_metric(pts))
+        lens = [math.sqrt((pts[j] - pts[i]) @ m @ (pts[j] - pts[i]))
+                for i in range(dim + 1) for j in range(i + 1, dim + 1)]
+        assert np.allclose(lens, 1.0, rtol=1e-9)
+
+
+def test_implied_metric_of_the_regular_tet_is_the_identity():
+    pts = [np.array([0.0, 0, 0]), np.array([1.0, 0, 0]),
+           np.array([0.5, math.sqrt(3) / 2, 0]),
+           np.array([0.5, math.sqrt(3) / 6, math.sqrt(2.0 / 3.0)])]
+    assert np.allclose(O.element_implied_metric(pts), [1, 1, 1, 0, 0, 0], atol=1e-13)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_implied_metrics_scale_with_the_mesh(dim):
+    # the same box, twice as large: every implied metric is a quarter
+    a = O.Mesh.box(dim, 3)
+    b = O.Mesh.box(dim, 3, x=2.0, y=2.0, z=2.0)
+    ma, mb = a.implied_metrics(), b.implied_metrics()
+    assert np.allclose(mb, ma / 4.0, rtol=1e-12, atol=1e-12)
+    # and they are positive definite
+    nc = dim * (dim + 1) // 2
+    for i in range(0, len(ma), nc):
+        assert np.linalg.eigvalsh(O.vector2symm(ma[i:i + nc])).min() > 0
