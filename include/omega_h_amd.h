@@ -164,6 +164,14 @@ int osh_get_implied_metrics(osh_mesh* mesh, int set_as_metric, double* out);
 int osh_interpolate_metrics(osh_ctx* ctx, int dim, int n, const double* a, const double* b,
     double t, double* out);
 
+/* binary::write(path, mesh) / binary::read(path, ...), src/Omega_h_file.cpp:
+ * 515-600: the .osh directory format (version 9, zlib-compressed arrays),
+ * one part.  Written: down adjacencies with codes, coordinates, metric,
+ * vertex tags, class_id, class_dim, global.  Read: the same; other float64
+ * vertex tags become linear vertex tags, other tags are skipped. */
+int osh_write_osh(osh_mesh* mesh, const char* path);
+int osh_read_osh(osh_ctx* ctx, const char* path, osh_mesh** out);
+
 /* FNV-1a digest of the element vertex lists and coordinate bits (for
  * comparing adapted meshes) */
 int osh_mesh_digest(osh_mesh* mesh, unsigned long long* out);

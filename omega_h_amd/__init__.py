@@ -134,6 +134,8 @@ def load(path=None):
         "osh_approach_metric": [P, C.POINTER(AdaptOpts), C.c_double, IP],
         "osh_interpolate_metrics": [P, I, I, DP, DP, C.c_double, DP],
         "osh_get_implied_metrics": [P, I, DP],
+        "osh_write_osh": [P, C.c_char_p],
+        "osh_read_osh": [P, C.c_char_p, C.POINTER(P)],
         "osh_server_create": [I, I, C.c_double, C.POINTER(P)],
         "osh_server_destroy": [P],
         "osh_server_submit": [P, I, I, I, I, I, DP, C.POINTER(AdaptOpts), I, IP],
@@ -259,6 +261,16 @@ class Mesh:
         check(ctx.lib.osh_build_from_elems(ctx.h, dim, len(a) // (dim + 1), _p(a, C.c_int),
                                            nverts, C.byref(h)), ctx.lib)
         return cls(ctx, h)
+
+    @classmethod
+    def read_osh(cls, ctx, path):
+        """Read a one-part .osh directory (Omega_h's binary format)."""
+        h = C.c_void_p()
+        check(ctx.lib.osh_read_osh(ctx.h, os.fsencode(path), C.byref(h)), ctx.lib)
+        return cls(ctx, h)
+
+    def write_osh(self, path):
+        check(self.lib.osh_write_osh(self.h, os.fsencode(path)), self.lib)
 
     def copy(self):
         h = C.c_void_p()

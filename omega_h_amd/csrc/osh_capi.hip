@@ -96,6 +96,24 @@ extern "C" {
 
 const char* osh_last_error(void) { return osh::last_error_message.c_str(); }
 
+int osh_write_osh(osh_mesh* mesh, const char* path) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(path != nullptr, "null path");
+    osh::write_osh(mesh->owner->ctx, mesh->m, path);
+  });
+}
+
+int osh_read_osh(osh_ctx* ctx, const char* path, osh_mesh** out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && path && out, "null argument");
+    auto m = std::make_unique<osh_mesh>();
+    m->owner = ctx;
+    osh::read_osh(ctx->ctx, m->m, path);
+    *out = m.release();
+  });
+}
+
 int osh_mesh_digest(osh_mesh* mesh, unsigned long long* out) {
   return guarded([&] {
     check_mesh(mesh);

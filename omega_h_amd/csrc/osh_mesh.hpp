@@ -96,6 +96,10 @@ void mident_metrics(Ctx& c, DMesh& m, LO n, LO const* edges, Real* out);
 void mident_values(Ctx& c, DMesh& m, LO n, LO const* edges, Real const* data, int ncomps,
     Real* out);
 
+// the .osh directory format, one rank (osh_file.cpp)
+void write_osh(Ctx& c, DMesh& m, std::string const& dir);
+void read_osh(Ctx& c, DMesh& m, std::string const& dir);
+
 // build_box (osh_build.hip)
 void build_box(Ctx& c, DMesh& m, int dim, Real x, Real y, Real z, LO nx, LO ny, LO nz,
     bool reorder_and_classify);
