@@ -147,6 +147,14 @@ int osh_mesh_add_vertex_tag(osh_mesh* mesh, const char* name, int ncomps, int ki
 int osh_mesh_vertex_tag_ncomps(osh_mesh* mesh, const char* name, int* out); /* -1: absent */
 int osh_mesh_get_vertex_tag(osh_mesh* mesh, const char* name, double* out);
 int osh_mesh_remove_vertex_tag(osh_mesh* mesh, const char* name);
+/* tags on every dimension transferred by inheritance (OMEGA_H_INHERIT,
+ * should_inherit src/Omega_h_transfer.cpp:20-34): products take the value of
+ * the entity they came from, as class_id does; per_dim[d] holds
+ * nents(d) * ncomps values for d = 0..dim */
+int osh_mesh_add_inherit_tag(osh_mesh* mesh, const char* name, int ncomps,
+    const double* const* per_dim);
+int osh_mesh_inherit_tag_ncomps(osh_mesh* mesh, const char* name, int* out); /* -1: absent */
+int osh_mesh_get_inherit_tag(osh_mesh* mesh, const char* name, int ent_dim, double* out);
 /* approach_metric(mesh, opts, min_step), src/Omega_h_approach.cpp:83-126:
  * set the metric to the "target_metric" tag, or halve the log-Euclidean
  * step toward it until the mesh's quality and lengths are acceptable;

@@ -135,6 +135,9 @@ def load(path=None):
         "osh_interpolate_metrics": [P, I, I, DP, DP, C.c_double, DP],
         "osh_get_implied_metrics": [P, I, DP],
         "osh_write_osh": [P, C.c_char_p],
+        "osh_mesh_add_inherit_tag": [P, C.c_char_p, I, C.POINTER(P)],
+        "osh_mesh_inherit_tag_ncomps": [P, C.c_char_p, IP],
+        "osh_mesh_get_inherit_tag": [P, C.c_char_p, I, DP],
         "osh_read_osh": [P, C.c_char_p, C.POINTER(P)],
         "osh_server_create": [I, I, C.c_double, C.POINTER(P)],
         "osh_server_destroy": [P],
@@ -391,6 +394,20 @@ class Mesh:
             return None
         out = np.zeros(self.nents(0) * nc, dtype=np.float64)
         check(self.lib.osh_mesh_get_vertex_tag(self.h, name.encode(), _p(out, C.c_double)),
+              self.lib)
+        return out
+
+    def add_inherit_tag(self, name, ncomps, per_dim):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in per_dim]
+        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        check(self.lib.osh_mesh_add_inherit_tag(self.h, name.encode(), ncomps, ptrs), self.lib)
+
+    def inherit_tag(self, name, d):
+        nc = self._int(self.lib.osh_mesh_inherit_tag_ncomps, name.encode())
+        if nc < 0:
+            return None
+        out = np.zeros(self.nents(d) * nc, dtype=np.float64)
+        check(self.lib.osh_mesh_get_inherit_tag(self.h, name.encode(), d, _p(out, C.c_double)),
               self.lib)
         return out
 

@@ -479,6 +479,55 @@ int osh_mesh_add_vertex_tag(osh_mesh* mesh, const char* name, int ncomps, int ki
   });
 }
 
+int osh_mesh_add_inherit_tag(osh_mesh* mesh, const char* name, int ncomps,
+    const double* const* per_dim) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(name && per_dim, "null argument");
+    OSH_CHECK(ncomps >= 1 && ncomps <= 64, "1 to 64 components");
+    auto& c = mesh->owner->ctx;
+    osh::DMesh::InheritTag t;
+    t.name = name;
+    t.ncomps = ncomps;
+    for (int d = 0; d <= mesh->m.dim; ++d) {
+      OSH_CHECK(per_dim[d] != nullptr, "null array for a dimension");
+      osh::LO n = mesh->m.nents[d] * ncomps;
+      t.data[d] = osh::Reals(c, n);
+      osh::upload(c, t.data[d], per_dim[d], n);
+    }
+    c.sync();
+    for (auto& x : mesh->m.itags)
+      if (x.name == name) {
+        x = t;
+        return;
+      }
+    mesh->m.itags.push_back(t);
+  });
+}
+
+int osh_mesh_inherit_tag_ncomps(osh_mesh* mesh, const char* name, int* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(name && out, "null argument");
+    *out = -1;
+    for (auto const& t : mesh->m.itags)
+      if (t.name == name) *out = t.ncomps;
+  });
+}
+
+int osh_mesh_get_inherit_tag(osh_mesh* mesh, const char* name, int ent_dim, double* out) {
+  return guarded([&] {
+    check_dim(mesh, ent_dim);
+    OSH_CHECK(name && out, "null argument");
+    for (auto const& t : mesh->m.itags)
+      if (t.name == name) {
+        download(mesh->owner->ctx, t.data[ent_dim], out);
+        return;
+      }
+    throw osh::Error(std::string("no inheritance tag ") + name);
+  });
+}
+
 int osh_mesh_vertex_tag_ncomps(osh_mesh* mesh, const char* name, int* out) {
   return guarded([&] {
     check_mesh(mesh);

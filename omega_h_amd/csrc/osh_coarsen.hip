@@ -395,6 +395,9 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
       for (auto const& t : m.vtags)
         nw.vtags.push_back(
             {t.name, t.ncomps, t.kind, transfer<Real>(c, mo, t.ncomps, t.data, nullptr)});
+      nw.itags = m.itags;
+      for (std::size_t t = 0; t < m.itags.size(); ++t)
+        nw.itags[t].data[0] = transfer<Real>(c, mo, m.itags[t].ncomps, m.itags[t].data[0], nullptr);
       nw.class_id[0] = transfer<LO>(c, mo, 1, m.class_id[0], nullptr);
       nw.class_dim[0] = transfer<I8>(c, mo, 1, m.class_dim[0], nullptr);
     } else {

@@ -15,6 +15,7 @@
 #include <sys/stat.h>
 #include <zlib.h>
 
+#include <array>
 #include <cerrno>
 #include <cstring>
 #include <fstream>
@@ -149,12 +150,14 @@ void write_osh(Ctx& c, DMesh& m, std::string const& dir) {
     for (std::size_t i = 0; i < globals.size(); ++i) globals[i] = std::int64_t(i);
     std::int32_t ntags = 1 + (m.class_id[d].exists() ? 1 : 0) + (m.class_dim[d].exists() ? 1 : 0);
     if (d == 0) ntags += 1 + (m.metric.exists() ? 1 : 0) + std::int32_t(m.vtags.size());
+    ntags += std::int32_t(m.itags.size());
     put<std::int32_t>(f, ntags);
     if (d == 0) {
       put_tag(f, "coordinates", m.dim, kF64, to_host(c, m.coords), z);
       if (m.metric.exists()) put_tag(f, "metric", m.metric_ncomps, kF64, to_host(c, m.metric), z);
       for (auto const& t : m.vtags) put_tag(f, t.name, t.ncomps, kF64, to_host(c, t.data), z);
     }
+    for (auto const& t : m.itags) put_tag(f, t.name, t.ncomps, kF64, to_host(c, t.data[d]), z);
     if (m.class_id[d].exists()) put_tag(f, "class_id", 1, kI32, to_host(c, m.class_id[d]), z);
     if (m.class_dim[d].exists()) put_tag(f, "class_dim", 1, kI8, to_host(c, m.class_dim[d]), z);
     put_tag(f, "global", 1, kI64, globals, z);
@@ -206,6 +209,10 @@ void read_osh(Ctx& c, DMesh& m, std::string const& dir) {
       r.down_codes[d] = to_device(c, codes);
     }
   }
+  // float64 tags of each dimension by name: present on every dimension they
+  // are inheritance tags, on vertices only they are vertex tags
+  std::vector<std::pair<std::string, int>> names;
+  std::vector<std::array<Reals, 4>> per_dim;
   for (int d = 0; d <= dim; ++d) {
     std::int32_t ntags = get<std::int32_t>(f);
     for (std::int32_t t = 0; t < ntags; ++t) {
@@ -220,7 +227,11 @@ void read_osh(Ctx& c, DMesh& m, std::string const& dir) {
       if (type == kF64) {
         auto a = get_array<Real>(f, z);
         OSH_CHECK(a.size() == want, "tag " + name + " has the wrong size");
-        if (d != 0) continue;
+        if (d != 0) {
+          for (std::size_t k = 0; k < names.size(); ++k)
+            if (names[k].first == name && names[k].second == ncomps) per_dim[k][d] = to_device(c, a);
+          continue;
+        }
         if (name == "coordinates") {
           OSH_CHECK(ncomps == dim, "coordinates must have dim components");
           r.coords = to_device(c, a);
@@ -228,8 +239,9 @@ void read_osh(Ctx& c, DMesh& m, std::string const& dir) {
           r.metric = to_device(c, a);
           r.metric_ncomps = ncomps;
         } else {
-          bool is_metric = name.size() >= 6 && name.compare(name.size() - 6, 6, "metric") == 0;
-          r.vtags.push_back({name, ncomps, is_metric ? 1 : 0, to_device(c, a)});
+          names.push_back({name, ncomps});
+          per_dim.emplace_back();
+          per_dim.back()[0] = to_device(c, a);
         }
       } else if (type == kI32) {
         auto a = get_array<std::int32_t>(f, z);
@@ -244,6 +256,21 @@ void read_osh(Ctx& c, DMesh& m, std::string const& dir) {
       } else {
         OSH_CHECK(false, "unknown tag type in .osh file");
       }
+    }
+  }
+  for (std::size_t k = 0; k < names.size(); ++k) {
+    bool everywhere = true;
+    for (int d = 1; d <= dim; ++d) everywhere = everywhere && per_dim[k][d].exists();
+    std::string const& name = names[k].first;
+    if (everywhere) {
+      DMesh::InheritTag t;
+      t.name = name;
+      t.ncomps = names[k].second;
+      for (int d = 0; d <= dim; ++d) t.data[d] = per_dim[k][d];
+      r.itags.push_back(t);
+    } else {
+      bool is_metric = name.size() >= 6 && name.compare(name.size() - 6, 6, "metric") == 0;
+      r.vtags.push_back({name, names[k].second, is_metric ? 1 : 0, per_dim[k][0]});
     }
   }
   OSH_CHECK(r.coords.exists(), "the mesh has no coordinates");

@@ -40,6 +40,15 @@ struct DMesh {
     Reals data;
   };
   std::vector<VertexTag> vtags;
+  // tags on every dimension transferred by inheritance (OMEGA_H_INHERIT,
+  // src/Omega_h_transfer.cpp:20-34): products take the value of the entity
+  // they came from, exactly as class_id does
+  struct InheritTag {
+    std::string name;
+    int ncomps;
+    Reals data[4];
+  };
+  std::vector<InheritTag> itags;
   Reals length;   // cached edge lengths (may be absent)
   Reals quality;  // cached element qualities (may be absent)
   // derived caches

@@ -154,6 +154,25 @@ __global__ void transfer_class_kernel(LO n, LO nsame, LO const* same2old, LO con
   dims[w] = second ? dims_b[s] : dims_a[s];
 }
 
+// an inheritance tag of the new entities, by the same sources as
+// transfer_class_kernel
+__global__ void inherit_kernel(LO n, LO nsame, int nc, LO const* same2old, LO const* same2new,
+    Real const* old_vals, LO const* prods2new, LO const* src, int src_dim, I8 const* src_dims,
+    Real const* vals_a, Real const* vals_b, Real* out) {
+  LO i = thread_index();
+  if (i >= n) return;
+  if (i < nsame) {
+    for (int k = 0; k < nc; ++k) out[same2new[i] * nc + k] = old_vals[same2old[i] * nc + k];
+    return;
+  }
+  LO p = i - nsame;
+  int sd = src_dims ? src_dims[p] : src_dim;
+  Real const* from = (sd != src_dim) ? vals_b : vals_a;
+  LO s = src[p];
+  LO w = prods2new[p];
+  for (int k = 0; k < nc; ++k) out[w * nc + k] = from[s * nc + k];
+}
+
 __global__ void indset_init(LO n, I8 const* cands, I8* state) {
   LO i = thread_index();
   if (i < n) state[i] = cands[i] ? 2 : 0;
@@ -345,6 +364,16 @@ void transfer_class(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo,
       ids.data(), dims.data());
   nw.class_id[d] = ids;
   nw.class_dim[d] = dims;
+  if (nw.itags.size() != old.itags.size()) nw.itags = old.itags;
+  for (std::size_t t = 0; t < old.itags.size(); ++t) {
+    auto const& a = old.itags[t];
+    Reals out(c, mo.nnew * a.ncomps);
+    OSH_LAUNCH(inherit_kernel, nsame + nprods, c, nsame, a.ncomps, mo.same2old.data(),
+        mo.same2new.data(), a.data[d].data(), mo.prods2new.data(), prod_src.data(), src_dim,
+        src_dims ? src_dims->data() : (I8 const*)nullptr, a.data[src_dim].data(),
+        a.data[other].data(), out.data());
+    nw.itags[t].data[d] = out;
+  }
 }
 
 void transfer_measures(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo) {

@@ -390,6 +390,7 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
   nw.coords = m.coords;
   nw.metric = m.metric;
   nw.vtags = m.vtags;
+  nw.itags = m.itags;
   nw.class_id[0] = m.class_id[0];
   nw.class_dim[0] = m.class_dim[0];
   LOs old_lows2new_lows = iota(c, nv);

@@ -406,6 +406,32 @@ static void refine_vtags(om_mesh const* m, om_mesh* nw, int nkeys, int const* ke
   }
 }
 
+/* inheritance tags of dimension d: products take the value of entity
+ * prod_src[p] of dimension src_dims[p] (or src_dim when src_dims is NULL);
+ * prod_src == NULL means there are no products */
+static void inherit_itags(om_mesh const* old, om_mesh* nw, int d, mod_out const* mo,
+    int src_dim, int const* src_dims, int const* prod_src) {
+  nw->nitags = old->nitags;
+  for (int t = 0; t < old->nitags; ++t) {
+    om_itag const* a = &old->itags[t];
+    om_itag* b = &nw->itags[t];
+    memcpy(b->name, a->name, sizeof(b->name));
+    b->ncomps = a->ncomps;
+    int nc = a->ncomps;
+    double* pd = NULL;
+    if (prod_src) {
+      pd = (double*)om_malloc(sizeof(double) * (size_t)(mo->nprods + 1) * nc);
+      for (int p = 0; p < mo->nprods; ++p) {
+        int sd = src_dims ? src_dims[p] : src_dim;
+        for (int k = 0; k < nc; ++k)
+          pd[(size_t)p * nc + k] = a->data[sd][(size_t)prod_src[p] * nc + k];
+      }
+    }
+    transfer_r(mo->nnew, nc, mo, a->data[d], pd, &b->data[d]);
+    free(pd);
+  }
+}
+
 /* class_id/class_dim of products from a list of old entities of dimension
  * src_dim (one per product) */
 static void transfer_class_from(om_mesh* old, om_mesh* nw, int d, int src_dim,
@@ -699,24 +725,29 @@ int om_refine_by_size(om_mesh* m, om_adapt_opts const* o) {
       refine_vtags(m, nw, nkeys, keys2edges, &mo);
       /* the midpoint inherits the key edge's classification */
       transfer_class_from(m, nw, 0, 1, keys2edges, &mo);
+      inherit_itags(m, nw, 0, &mo, 1, NULL, keys2edges);
     } else {
       /* pairs inherit from their domain (same dim), cuts from dim+1 */
       int nprods = k2p[nkeys];
       int* pid = (int*)om_malloc(sizeof(int) * (size_t)(nprods + 1));
       signed char* pdim = (signed char*)om_malloc((size_t)(nprods + 1));
+      int* psd = (int*)om_malloc(sizeof(int) * (size_t)(nprods + 1));
       om_graph const* cuts_up = (d < dim) ? om_ask_up(m, 1, d + 1) : NULL;
       for (int k = 0; k < nkeys; ++k) {
         int ncut = cuts_up ? (cuts_up->a2ab[keys2edges[k] + 1] - cuts_up->a2ab[keys2edges[k]]) : 0;
         for (int p = k2p[k]; p < k2p[k + 1]; ++p) {
           int src_dim = (p >= k2p[k + 1] - ncut) ? d + 1 : d;
+          psd[p] = src_dim;
           pid[p] = m->class_id[src_dim][psrc[p]];
           pdim[p] = m->class_dim[src_dim][psrc[p]];
         }
       }
       transfer_i(mo.nnew, &mo, m->class_id[d], pid, &nw->class_id[d]);
       transfer_b(mo.nnew, &mo, m->class_dim[d], pdim, &nw->class_dim[d]);
+      inherit_itags(m, nw, d, &mo, d, psd, psrc);
       free(pid);
       free(pdim);
+      free(psd);
       transfer_lengths_qualities(m, nw, d, &mo);
     }
     free(old_lows2new_lows);
@@ -1015,8 +1046,10 @@ static int coarsen(om_mesh* m, om_adapt_opts const* o, signed char* edge_codes,
       copy_vtags(m, nw, &mo);
       transfer_i(mo.nnew, &mo, m->class_id[0], NULL, &nw->class_id[0]);
       transfer_b(mo.nnew, &mo, m->class_dim[0], NULL, &nw->class_dim[0]);
+      inherit_itags(m, nw, 0, &mo, 0, NULL, NULL);
     } else {
       transfer_class_from(m, nw, d, d, psrc, &mo);
+      inherit_itags(m, nw, d, &mo, d, NULL, psrc);
       transfer_lengths_qualities(m, nw, d, &mo);
     }
     free(old_lows2new_lows);
@@ -1255,6 +1288,14 @@ static void swap_rebuild(om_mesh* m, int nkeys, int const* keys2edges,
   nw->metric = (double*)om_malloc(sizeof(double) * (size_t)nv * nc);
   memcpy(nw->metric, m->metric, sizeof(double) * (size_t)nv * nc);
   copy_vtags(m, nw, NULL);
+  nw->nitags = m->nitags;
+  for (int t = 0; t < m->nitags; ++t) {
+    nw->itags[t] = m->itags[t];
+    for (int d = 0; d < 4; ++d) nw->itags[t].data[d] = NULL;
+    size_t n = (size_t)nv * m->itags[t].ncomps;
+    nw->itags[t].data[0] = (double*)om_malloc(sizeof(double) * (n + 1));
+    memcpy(nw->itags[t].data[0], m->itags[t].data[0], sizeof(double) * n);
+  }
   nw->class_id[0] = (int*)om_malloc(sizeof(int) * (size_t)nv);
   memcpy(nw->class_id[0], m->class_id[0], sizeof(int) * (size_t)nv);
   nw->class_dim[0] = (signed char*)om_malloc((size_t)nv);
@@ -1269,6 +1310,7 @@ static void swap_rebuild(om_mesh* m, int nkeys, int const* keys2edges,
     for (int k = 0; k < nkeys; ++k)
       for (int p = k2p[d][k]; p < k2p[d][k + 1]; ++p) src[p] = keys2edges[k];
     transfer_class_from(m, nw, d, 1, src, &mo);
+    inherit_itags(m, nw, d, &mo, 1, NULL, src);
     free(src);
     transfer_lengths_qualities(m, nw, d, &mo);
     free(old_lows2new_lows);

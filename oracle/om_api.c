@@ -244,3 +244,38 @@ int om_approach_metric(om_mesh* m, om_adapt_opts const* o, double min_step) {
   free(target);
   return r;
 }
+
+/* inheritance tags (OMEGA_H_INHERIT): one array per dimension 0..dim */
+static om_itag* find_itag(om_mesh* m, char const* name) {
+  for (int t = 0; t < m->nitags; ++t)
+    if (strncmp(m->itags[t].name, name, sizeof(m->itags[t].name)) == 0) return &m->itags[t];
+  return NULL;
+}
+
+int om_add_itag(om_mesh* m, char const* name, int ncomps, double const* const* per_dim) {
+  om_itag* t = find_itag(m, name);
+  if (!t) {
+    if (m->nitags >= OM_MAX_ITAGS) return -1;
+    t = &m->itags[m->nitags++];
+    memset(t, 0, sizeof(*t));
+    strncpy(t->name, name, sizeof(t->name) - 1);
+  }
+  t->ncomps = ncomps;
+  for (int d = 0; d <= m->dim; ++d) {
+    free(t->data[d]);
+    size_t n = (size_t)m->nents[d] * ncomps;
+    t->data[d] = (double*)om_malloc(sizeof(double) * (n + 1));
+    memcpy(t->data[d], per_dim[d], sizeof(double) * n);
+  }
+  return 0;
+}
+
+int om_itag_ncomps(om_mesh* m, char const* name) {
+  om_itag* t = find_itag(m, name);
+  return t ? t->ncomps : -1;
+}
+
+void om_get_itag(om_mesh* m, char const* name, int d, double* out) {
+  om_itag* t = find_itag(m, name);
+  if (t) memcpy(out, t->data[d], sizeof(double) * (size_t)m->nents[d] * t->ncomps);
+}

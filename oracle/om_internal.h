@@ -86,10 +86,23 @@ typedef struct {
   double* data;
 } om_vtag;
 
+/* a tag on every dimension transferred by inheritance (OMEGA_H_INHERIT,
+ * should_inherit src/Omega_h_transfer.cpp:20-34): surviving entities keep
+ * their values, a product takes the value of the entity it came from, as
+ * class_id does (src/Omega_h_transfer.cpp:213-267,431-470,656-700) */
+#define OM_MAX_ITAGS 8
+typedef struct {
+  char name[64];
+  int ncomps;
+  double* data[4];
+} om_itag;
+
 struct om_mesh {
   int dim;
   int nvtags;
   om_vtag vtags[OM_MAX_VTAGS];
+  int nitags;
+  om_itag itags[OM_MAX_ITAGS];
   int nents[4];
   /* down[d] for d >= 1: nents[d] * (d+1) lower entities of dim d-1 */
   int* down[4];

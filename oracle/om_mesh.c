@@ -169,6 +169,8 @@ void om_mesh_free(om_mesh* m) {
   free(m->coords);
   free(m->metric);
   for (int t = 0; t < m->nvtags; ++t) free(m->vtags[t].data);
+  for (int t = 0; t < m->nitags; ++t)
+    for (int d = 0; d < 4; ++d) free(m->itags[t].data[d]);
   free(m->length);
   free(m->quality);
   free(m);

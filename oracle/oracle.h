@@ -88,6 +88,10 @@ int om_add_vtag(om_mesh* m, char const* name, int ncomps, int kind, double const
 int om_vtag_ncomps(om_mesh* m, char const* name);
 void om_get_vtag(om_mesh* m, char const* name, double* out);
 void om_remove_vtag(om_mesh* m, char const* name);
+/* inheritance tags: per_dim[d] holds nents(d) * ncomps values, d = 0..dim */
+int om_add_itag(om_mesh* m, char const* name, int ncomps, double const* const* per_dim);
+int om_itag_ncomps(om_mesh* m, char const* name);
+void om_get_itag(om_mesh* m, char const* name, int d, double* out);
 void om_interpolate_metrics(int mdim, int n, double const* a, double const* b, double t,
     double* out);
 int om_approach_metric(om_mesh* m, om_adapt_opts const* o, double min_step);

@@ -95,6 +95,9 @@ def lib():
                                              C.POINTER(C.c_double)]
         L.om_approach_metric.argtypes = [P, C.POINTER(AdaptOpts), C.c_double]
         L.om_get_implied_metrics.argtypes = [P, C.POINTER(C.c_double)]
+        L.om_add_itag.argtypes = [P, C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]
+        L.om_itag_ncomps.argtypes = [P, C.c_char_p]
+        L.om_get_itag.argtypes = [P, C.c_char_p, C.c_int, C.POINTER(C.c_double)]
         L.om_element_implied_metric.argtypes = [C.c_int, C.POINTER(C.c_void_p),
                                                 C.POINTER(C.c_double)]
         L.om_default_opts.argtypes = [C.c_int, C.POINTER(AdaptOpts)]
@@ -274,6 +277,19 @@ class Mesh:
 
     def approach_metric(self, opts, min_step=1e-4):
         return lib().om_approach_metric(self.h, C.byref(opts), min_step)
+
+    def add_inherit_tag(self, name, ncomps, per_dim):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in per_dim]
+        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        assert lib().om_add_itag(self.h, name.encode(), ncomps, ptrs) == 0
+
+    def inherit_tag(self, name, d):
+        nc = lib().om_itag_ncomps(self.h, name.encode())
+        if nc < 0:
+            return None
+        out = np.zeros(self.nents(d) * nc, dtype=np.float64)
+        lib().om_get_itag(self.h, name.encode(), d, _ptr(out, C.c_double))
+        return out
 
     def implied_metrics(self):
         nc = self.dim * (self.dim + 1) // 2

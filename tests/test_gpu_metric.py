@@ -150,3 +150,19 @@ def test_adapt_to_a_target_from_the_implied_metric(ctx):
         assert np.array_equal(o.verts(3), g.verts(3))
         assert np.array_equal(o.metric().view(np.int64), g.metric().view(np.int64))
     assert steps >= 1
+
+
+@pytest.mark.parametrize("dim,n", [(2, 10), (3, 4)])
+def test_inheritance_tags_identical(ctx, dim, n):
+    o = O.Mesh.box(dim, n)
+    g = A.Mesh.box(ctx, dim, n)
+    met, nc = W.iso_ring_metric_2d(o.coords()) if dim == 2 else W.levelset_metric_3d(o.coords())
+    rng = np.random.default_rng(3)
+    tags = [rng.random(o.nents(d) * 2) for d in range(dim + 1)]
+    for m in (o, g):
+        m.set_metric(met, nc)
+        m.add_inherit_tag("field", 2, tags)
+    o.adapt(O.default_opts(dim))
+    g.adapt(ctx.default_opts(dim))
+    for d in range(dim + 1):
+        assert np.array_equal(o.inherit_tag("field", d), g.inherit_tag("field", d)), d

@@ -253,3 +253,18 @@ def test_implied_metrics_scale_with_the_mesh(dim):
     nc = dim * (dim + 1) // 2
     for i in range(0, len(ma), nc):
         assert np.linalg.eigvalsh(O.vector2symm(ma[i:i + nc])).min() > 0
+
+
+@pytest.mark.parametrize("dim,n", [(2, 10), (3, 3)])
+def test_inheritance_tags_follow_the_classification(dim, n):
+    """A tag on every dimension that starts as a copy of class_id is
+    transferred exactly as class_id is (src/Omega_h_transfer.cpp:20-34)."""
+    from omega_h_amd import workloads as W
+    m = O.Mesh.box(dim, n)
+    met, nc = W.iso_ring_metric_2d(m.coords()) if dim == 2 else W.levelset_metric_3d(m.coords())
+    m.set_metric(met, nc)
+    m.add_inherit_tag("cid", 1, [m.classification(d)[0].astype(np.float64)
+                                 for d in range(dim + 1)])
+    m.adapt(O.default_opts(dim))
+    for d in range(dim + 1):
+        assert np.array_equal(m.inherit_tag("cid", d), m.classification(d)[0].astype(np.float64))
