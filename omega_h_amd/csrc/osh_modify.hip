@@ -20,18 +20,41 @@ __global__ void mark_image_kernel(LO n, LO const* a2b, I8* marks) {
   if (i < n) marks[a2b[i]] = 1;
 }
 
-__global__ void mark_up_kernel(LO nh, int deg, LO const* hl2l, I8 const* low_marks, I8* out) {
+// the same entities of dimension > key dimension: those with no key among
+// their lower entities
+__global__ void mark_same_up_kernel(LO nh, int deg, LO const* hl2l, I8 const* low_marks,
+    I8* same) {
   LO h = thread_index();
   if (h >= nh) return;
-  I8 r = 0;
+  I8 r = 1;
   for (int j = 0; j < deg; ++j)
-    if (low_marks[hl2l[h * deg + j]]) r = 1;
-  out[h] = r;
+    if (low_marks[hl2l[h * deg + j]]) r = 0;
+  same[h] = r;
 }
 
-__global__ void invert_marks_kernel(LO n, I8 const* in, I8* out) {
+__global__ void clear_image_kernel(LO n, LO const* a2b, I8* marks) {
   LO i = thread_index();
-  if (i < n) out[i] = in[i] ? 0 : 1;
+  if (i < n) marks[a2b[i]] = 0;
+}
+
+__global__ void counts_from_same(LO n, I8 const* same, LO* counts) {
+  LO i = thread_index();
+  if (i < n) counts[i] = same[i] ? 1 : 0;
+}
+
+// new numbers of the old entities in one sweep: same entities take their
+// offset, the rest -1 (src/Omega_h_modify.cpp:141-161)
+__global__ void number_same(LO n, I8 const* same, LO const* same_offsets, LO const* offsets,
+    LO* same2new, LO* old2new) {
+  LO o = thread_index();
+  if (o >= n) return;
+  if (same && !same[o]) {
+    old2new[o] = -1;
+    return;
+  }
+  LO s = same ? same_offsets[o] : o;
+  same2new[s] = offsets[o];
+  old2new[o] = offsets[o];
 }
 
 __global__ void reps_up_kernel(LO nkeys, LO const* keys2kds, LO const* a2ab, LO const* ab2b,
@@ -43,11 +66,6 @@ __global__ void reps_up_kernel(LO nkeys, LO const* keys2kds, LO const* a2ab, LO 
 __global__ void reps_down_kernel(LO nkeys, LO const* keys2kds, int deg, LO const* down, LO* reps) {
   LO k = thread_index();
   if (k < nkeys) reps[k] = down[keys2kds[k] * deg];
-}
-
-__global__ void set_same_counts(LO nsame, LO const* same2old, LO* counts) {
-  LO s = thread_index();
-  if (s < nsame) counts[same2old[s]] = 1;
 }
 
 __global__ void add_rep_counts(LO nkeys, LO const* reps, LO const* keys2prods, LO* counts) {
@@ -73,15 +91,6 @@ __global__ void rep2md_order_kernel(LO nreps, LO const* a2ab, LO const* ab2b, I8
       offset += keys2prods[mod + 1] - keys2prods[mod];
     }
   }
-}
-
-__global__ void same_numbers(LO nsame, LO const* same2old, LO const* offsets, LO* same2new,
-    LO* old2new) {
-  LO s = thread_index();
-  if (s >= nsame) return;
-  LO o = same2old[s];
-  same2new[s] = offsets[o];
-  old2new[o] = offsets[o];
 }
 
 __global__ void prod_numbers(LO nkeys, LO const* reps, LO const* keys2kds, LO const* offsets,
@@ -249,22 +258,23 @@ void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
     LOs const& old_lows2new_lows, ModOut& out) {
   LO nkeys = keys2kds.size();
   LO nold = old.nents[ent_dim];
-  // same entities: those not adjacent to a key (collect_same)
+  // same entities: those not adjacent to a key (collect_same); without a
+  // `same` array every old entity is the same
+  Bytes same;
+  LOs same_offsets;
   if (ent_dim >= key_dim) {
-    Bytes key_marks = fill_i8(c, old.nents[key_dim], 0);
-    OSH_LAUNCH(mark_image_kernel, nkeys, c, keys2kds.data(), key_marks.data());
-    Bytes mods;
     if (ent_dim == key_dim) {
-      mods = key_marks;
+      same = fill_i8(c, nold, 1);
+      OSH_LAUNCH(clear_image_kernel, nkeys, c, keys2kds.data(), same.data());
     } else {
-      mods = Bytes(c, nold);
+      Bytes key_marks = fill_i8(c, old.nents[key_dim], 0);
+      OSH_LAUNCH(mark_image_kernel, nkeys, c, keys2kds.data(), key_marks.data());
+      same = Bytes(c, nold);
       int deg = simplex_degree(ent_dim, key_dim);
       LOs const& hl2l = ask_down(c, old, ent_dim, key_dim, nullptr);
-      OSH_LAUNCH(mark_up_kernel, nold, c, deg, hl2l.data(), key_marks.data(), mods.data());
+      OSH_LAUNCH(mark_same_up_kernel, nold, c, deg, hl2l.data(), key_marks.data(), same.data());
     }
-    Bytes same(c, nold);
-    OSH_LAUNCH(invert_marks_kernel, nold, c, mods.data(), same.data());
-    out.same2old = collect_marked(c, same);
+    out.same2old = collect_marked(c, same, &same_offsets);
   } else {
     out.same2old = iota(c, nold);
   }
@@ -283,8 +293,13 @@ void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
     reps = LOs(c, nkeys);
     OSH_LAUNCH(reps_down_kernel, nkeys, c, keys2kds.data(), deg, down.data(), reps.data());
   }
-  LOs counts = fill_lo(c, nold, 0);
-  OSH_LAUNCH(set_same_counts, nsame, c, out.same2old.data(), counts.data());
+  LOs counts;
+  if (same.exists()) {
+    counts = LOs(c, nold);
+    OSH_LAUNCH(counts_from_same, nold, c, same.data(), counts.data());
+  } else {
+    counts = fill_lo(c, nold, 1);
+  }
   OSH_LAUNCH(add_rep_counts, nkeys, c, reps.data(), keys2prods.data(), counts.data());
   LOs offsets = offset_scan(c, counts, &out.nnew);
   LOs order;
@@ -298,9 +313,10 @@ void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
         md2mod.data(), keys2prods.data(), order.data());
   }
   out.same2new = LOs(c, nsame);
-  out.old2new = fill_lo(c, nold, -1);
-  OSH_LAUNCH(same_numbers, nsame, c, out.same2old.data(), offsets.data(), out.same2new.data(),
-      out.old2new.data());
+  out.old2new = LOs(c, nold);
+  OSH_LAUNCH(number_same, nold, c, same.exists() ? same.data() : (I8 const*)nullptr,
+      same.exists() ? same_offsets.data() : (LO const*)nullptr, offsets.data(),
+      out.same2new.data(), out.old2new.data());
   out.prods2new = LOs(c, nprods);
   OSH_LAUNCH(prod_numbers, nkeys, c, reps.data(), keys2kds.data(), offsets.data(),
       order.exists() ? order.data() : (LO const*)nullptr, keys2prods.data(),

@@ -359,13 +359,16 @@ LOs offset_scan_bytes(Ctx& c, Bytes const& counts, LO* total) {
   return out;
 }
 
-LOs collect_marked(Ctx& c, Bytes const& marks) {
+LOs collect_marked(Ctx& c, Bytes const& marks) { return collect_marked(c, marks, nullptr); }
+
+LOs collect_marked(Ctx& c, Bytes const& marks, LOs* offsets_out) {
   LO n = marks.size();
   // the scan counts nonzero marks directly
   LOs offsets = offset_scan_t<I8, true>(c, marks, static_cast<LO*>(c.pinned));
   LO count = pinned_value<LO>(c);
   LOs out(c, count);
   OSH_LAUNCH(scatter_marked, n, c, marks.data(), offsets.data(), out.data());
+  if (offsets_out) *offsets_out = offsets;
   return out;
 }
 

@@ -232,6 +232,8 @@ LOs offset_scan(Ctx& c, LOs const& counts, LO* total = nullptr);
 LOs offset_scan_async(Ctx& c, LOs const& counts, int slot);
 LOs offset_scan_bytes(Ctx& c, Bytes const& counts, LO* total = nullptr);
 LOs collect_marked(Ctx& c, Bytes const& marks);       // indices of nonzero
+// the same, also returning the n+1 offsets of the nonzero marks
+LOs collect_marked(Ctx& c, Bytes const& marks, LOs* offsets_out);
 LOs iota(Ctx& c, LO n, LO start = 0, LO stride = 1);
 LOs fill_lo(Ctx& c, LO n, LO value);
 Bytes fill_i8(Ctx& c, LO n, I8 value);
