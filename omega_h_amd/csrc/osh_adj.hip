@@ -7,7 +7,8 @@
 //   reflect_down / find_matches src/Omega_h_adj.cpp:297-441
 //   verts_across_edges, edges_across_tris/tets, add_edges
 //                           src/Omega_h_adj.cpp:512-589, src/Omega_h_graph.cpp:14-40
-//   elements_across_sides   src/Omega_h_adj.cpp:591-657
+//   side_elems (the two elements of each side, for elements_across_sides)
+//                           src/Omega_h_adj.cpp:591-657
 //   find_unique             src/Omega_h_adj.cpp:55-153
 #include "osh_mesh.hpp"
 
@@ -24,7 +25,7 @@ void DMesh::clear_caches() {
     for (int e = 0; e < 4; ++e) up[d][e] = Adj();
   }
   star[0] = star[1] = Adj();
-  dual = Adj();
+  side_elems = LOs();
   t2e = LOs();
   t2e_codes = Bytes();
 }
@@ -145,54 +146,122 @@ Adj invert_adj(Ctx& c, LO nhigh, int deg, LO nlow, LOs const& hl2l, Bytes const*
   return a;
 }
 
-// reflect_down: form the uses of each product and find the matching entity
+// reflect_down: each existing lower entity is listed once, in the row of its
+// smallest vertex, and rows are built only for vertices that are the
+// smallest vertex of some use.  A use's match has the same vertex set, so it
+// sits in the row of the use's smallest vertex; the match is unique, hence it
+// is the entity the walk over the full vertex -> lower-entity adjacency finds
+// (find_matches, src/Omega_h_adj.cpp:297-422).
+template <int NV>
+__device__ __forceinline__ LO min_vertex(LO const* v) {
+  LO m = v[0];
+#pragma unroll
+  for (int k = 1; k < NV; ++k) m = (v[k] < m) ? v[k] : m;
+  return m;
+}
+
+// counts[v] = 1 marks v; a marked row's final slot stays unused
 template <int HIGH>
-__global__ void match_uses(LO nuses, LO const* prod_verts, LO const* lv2v,
-    LO const* v2l_a2ab, LO const* v2l_ab2b, I8 const* v2l_codes, LO* out, I8* out_codes,
-    int* error) {
+__global__ void mark_use_min_verts(LO nuses, LO const* prod_verts, LO* counts) {
+  LO u = thread_index();
+  if (u >= nuses) return;
+  constexpr int LOW = HIGH - 1;
+  LO h = u / (HIGH + 1);
+  int which = u % (HIGH + 1);
+  LO av[3];
+#pragma unroll
+  for (int k = 0; k < LOW + 1; ++k)
+    av[k] = prod_verts[h * (HIGH + 1) + down_template(HIGH, LOW, which, k)];
+  counts[min_vertex<LOW + 1>(av)] = 1;
+}
+
+template <int NVL>
+__global__ void count_min_rows(LO nl, LO const* lv2v, LO* counts, LO* slot) {
+  LO l = thread_index();
+  if (l >= nl) return;
+  LO vm = min_vertex<NVL>(lv2v + l * NVL);
+  // unmarked counts stay 0 throughout this kernel, marked ones are >= 1
+  slot[l] = counts[vm] ? atomicAdd(&counts[vm], 1) - 1 : -1;
+}
+
+template <int NVL>
+__global__ void fill_min_rows(LO nl, LO const* lv2v, LO const* a2ab, LO const* slot, LO* rows) {
+  LO l = thread_index();
+  if (l >= nl) return;
+  LO s = slot[l];
+  if (s < 0) return;
+  rows[a2ab[min_vertex<NVL>(lv2v + l * NVL)] + s] = l;
+}
+
+template <int HIGH>
+__global__ void match_uses(LO nuses, LO const* __restrict__ prod_verts,
+    LO const* __restrict__ lv2v, LO const* __restrict__ a2ab, LO const* __restrict__ rows,
+    LO* __restrict__ out, I8* __restrict__ out_codes, int* error) {
   LO u = thread_index();
   if (u >= nuses) return;
   constexpr int LOW = HIGH - 1;
   constexpr int nvh = HIGH + 1;
-  constexpr int nlh = HIGH + 1;
   constexpr int nvl = LOW + 1;
-  LO h = u / nlh;
-  int which = u % nlh;
+  LO h = u / (HIGH + 1);
+  int which = u % (HIGH + 1);
   LO av[3];
+#pragma unroll
   for (int k = 0; k < nvl; ++k) av[k] = prod_verts[h * nvh + down_template(HIGH, LOW, which, k)];
-  LO fv = av[0];
+  LO vm = min_vertex<nvl>(av);
+  LO k0 = a2ab[vm], k1 = a2ab[vm + 1] - 1;
   int found = 0;
-  for (LO vb = v2l_a2ab[fv]; vb < v2l_a2ab[fv + 1]; ++vb) {
-    LO b = v2l_ab2b[vb];
-    int wd = code_which_down(v2l_codes[vb]);
-    LO const* bv = lv2v + b * nvl;
+  LO fb = 0;
+  I8 fc = 0;
+#pragma unroll 4
+  for (LO k = k0; k < k1; ++k) {
+    LO b = rows[k];
+    LO bv[3];
+#pragma unroll
+    for (int j = 0; j < nvl; ++j) bv[j] = lv2v[b * nvl + j];
+    // wd: the position of the use's first vertex in b
+    int wd = (bv[0] == av[0]) ? 0 : (bv[1] == av[0]) ? 1 : (nvl == 3 && bv[2] == av[0]) ? 2 : -1;
+    if (wd < 0) continue;
     if (nvl == 2) {
       if (av[1] == bv[1 - wd]) {
-        out[u] = b;
-        out_codes[u] = make_code(0, wd, 0);
+        fb = b;
+        fc = make_code(0, wd, 0);
         ++found;
       }
     } else {
       if (av[1] == bv[(wd + 1) % 3] && av[2] == bv[(wd + 2) % 3]) {
-        out[u] = b;
-        out_codes[u] = make_code(0, (3 - wd) % 3, 0);
+        fb = b;
+        fc = make_code(0, (3 - wd) % 3, 0);
         ++found;
       } else if (av[1] == bv[(wd + 2) % 3] && av[2] == bv[(wd + 1) % 3]) {
-        out[u] = b;
-        out_codes[u] = make_code(1, (3 - wd) % 3, 0);
+        fb = b;
+        fc = make_code(1, (3 - wd) % 3, 0);
         ++found;
       }
     }
   }
-  if (found != 1) {
-    // keep every index in range for the kernels that run before the host
-    // sees the error
-    if (found == 0) {
-      out[u] = 0;
-      out_codes[u] = 0;
-    }
-    *error = kReflectDownNoMatch;
-  }
+  // without a unique match every index stays in range for the kernels that
+  // run before the host sees the error
+  out[u] = fb;
+  out_codes[u] = fc;
+  if (found != 1) *error = kReflectDownNoMatch;
+}
+
+// the lower and higher element on each side (equal on the boundary), from
+// the element -> side lists by order-free min/max
+__global__ void side_elems_init(LO ns, LO* se) {
+  LO s = thread_index();
+  if (s >= ns) return;
+  se[2 * s] = 0x7fffffff;
+  se[2 * s + 1] = -1;
+}
+
+__global__ void side_elems_kernel(LO nuses, int nspe, LO const* es2s, LO* se) {
+  LO i = thread_index();
+  if (i >= nuses) return;
+  LO s = es2s[i];
+  LO e = i / nspe;
+  atomicMin(&se[2 * s], e);
+  atomicMax(&se[2 * s + 1], e);
 }
 
 __global__ void verts_across_edges_kernel(LO nve, LO const* ve2e, I8 const* codes,
@@ -229,32 +298,6 @@ __global__ void edge_star_fill(LO ne, LO const* e2f_a2ab, LO const* e2f_ab2b,
       LO r = e2t_ab2b[er];
       int rre = code_which_down(e2t_codes[er]);
       ab2b[k++] = te2e[r * 6 + opposite_template(3, 1, rre)];
-    }
-  }
-}
-
-__global__ void dual_degrees(LO ne, int nspe, LO const* es2s, LO const* s2e_a2ab, LO* deg) {
-  LO e = thread_index();
-  if (e >= ne) return;
-  LO d = 0;
-  for (int k = 0; k < nspe; ++k) {
-    LO s = es2s[e * nspe + k];
-    if (s2e_a2ab[s + 1] - s2e_a2ab[s] >= 2) ++d;
-  }
-  deg[e] = d;
-}
-
-__global__ void dual_fill(LO ne, int nspe, LO const* es2s, LO const* s2e_a2ab,
-    LO const* s2e_ab2b, LO const* a2ab, LO* ab2b) {
-  LO e = thread_index();
-  if (e >= ne) return;
-  LO k = a2ab[e];
-  for (int j = 0; j < nspe; ++j) {
-    LO s = es2s[e * nspe + j];
-    if (s2e_a2ab[s + 1] - s2e_a2ab[s] >= 2) {
-      LO se = s2e_a2ab[s];
-      if (s2e_ab2b[se] == e) ++se;
-      ab2b[k++] = s2e_ab2b[se];
     }
   }
 }
@@ -318,15 +361,30 @@ void reflect_down(Ctx& c, DMesh& m, int high_dim, LO nprods, LOs const& prod_ver
   codes = Bytes(c, nuses);
   if (nuses == 0) return;
   LOs const& lv2v = verts_of(c, m, low_dim);
-  Adj const& v2l = ask_up(c, m, 0, low_dim);
+  LO nv = m.nents[0];
+  LO nl = m.nents[low_dim];
+  LOs counts = fill_lo(c, nv, 0);
+  LOs slot(c, nl);
+  if (high_dim == 2) {
+    OSH_LAUNCH(mark_use_min_verts<2>, nuses, c, prod_verts.data(), counts.data());
+    OSH_LAUNCH(count_min_rows<2>, nl, c, lv2v.data(), counts.data(), slot.data());
+  } else {
+    OSH_LAUNCH(mark_use_min_verts<3>, nuses, c, prod_verts.data(), counts.data());
+    OSH_LAUNCH(count_min_rows<3>, nl, c, lv2v.data(), counts.data(), slot.data());
+  }
+  LOs a2ab = offset_scan(c, counts);
+  // every lower entity in at most one row plus one spare slot per marked row
+  LOs rows(c, nl + nv);
   // a use without a unique match sets the context's error word, which the
   // next stream sync raises
   if (high_dim == 2) {
-    OSH_LAUNCH(match_uses<2>, nuses, c, prod_verts.data(), lv2v.data(), v2l.a2ab.data(),
-        v2l.ab2b.data(), v2l.codes.data(), down.data(), codes.data(), c.error_word());
+    OSH_LAUNCH(fill_min_rows<2>, nl, c, lv2v.data(), a2ab.data(), slot.data(), rows.data());
+    OSH_LAUNCH(match_uses<2>, nuses, c, prod_verts.data(), lv2v.data(), a2ab.data(), rows.data(),
+        down.data(), codes.data(), c.error_word());
   } else {
-    OSH_LAUNCH(match_uses<3>, nuses, c, prod_verts.data(), lv2v.data(), v2l.a2ab.data(),
-        v2l.ab2b.data(), v2l.codes.data(), down.data(), codes.data(), c.error_word());
+    OSH_LAUNCH(fill_min_rows<3>, nl, c, lv2v.data(), a2ab.data(), slot.data(), rows.data());
+    OSH_LAUNCH(match_uses<3>, nuses, c, prod_verts.data(), lv2v.data(), a2ab.data(), rows.data(),
+        down.data(), codes.data(), c.error_word());
   }
 }
 
@@ -364,22 +422,17 @@ Adj const& ask_star(Ctx& c, DMesh& m, int d) {
   return m.star[d];
 }
 
-Adj const& ask_dual(Ctx& c, DMesh& m) {
-  if (!m.dual.exists()) {
+LOs const& ask_side_elems(Ctx& c, DMesh& m) {
+  if (!m.side_elems.exists()) {
     int dim = m.dim;
-    LO ne = m.nents[dim];
-    Adj const& s2e = ask_up(c, m, dim - 1, dim);
-    LOs deg(c, ne);
-    OSH_LAUNCH(dual_degrees, ne, c, dim + 1, m.down[dim].data(), s2e.a2ab.data(), deg.data());
-    Adj g;
-    LO total;
-    g.a2ab = offset_scan(c, deg, &total);
-    g.ab2b = LOs(c, total);
-    OSH_LAUNCH(dual_fill, ne, c, dim + 1, m.down[dim].data(), s2e.a2ab.data(),
-        s2e.ab2b.data(), g.a2ab.data(), g.ab2b.data());
-    m.dual = g;
+    LO ns = m.nents[dim - 1];
+    LOs se(c, 2 * ns);
+    OSH_LAUNCH(side_elems_init, ns, c, se.data());
+    OSH_LAUNCH(side_elems_kernel, m.nents[dim] * (dim + 1), c, dim + 1, m.down[dim].data(),
+        se.data());
+    m.side_elems = se;
   }
-  return m.dual;
+  return m.side_elems;
 }
 
 // ---------------------------------------------------------------- find_unique

@@ -57,7 +57,9 @@ struct DMesh {
   Bytes t2e_codes;
   Adj up[4][4];
   Adj star[2];
-  Adj dual;
+  // side_elems[2s], side_elems[2s+1]: the lower and higher element on side s
+  // (equal on the boundary)
+  LOs side_elems;
   void clear_caches();
 };
 
@@ -67,7 +69,7 @@ LOs const& verts_of(Ctx& c, DMesh& m, int d);
 LOs const& ask_down(Ctx& c, DMesh& m, int high, int low, Bytes const** codes);
 Adj const& ask_up(Ctx& c, DMesh& m, int low, int high);
 Adj const& ask_star(Ctx& c, DMesh& m, int d);
-Adj const& ask_dual(Ctx& c, DMesh& m);
+LOs const& ask_side_elems(Ctx& c, DMesh& m);
 // vertex lists of new entities of dimension high_dim >= 2 matched against the
 // already-built dimension high_dim-1 of m (src/Omega_h_adj.cpp:424-441)
 void reflect_down(Ctx& c, DMesh& m, int high_dim, LO nprods, LOs const& prod_verts,

@@ -211,6 +211,45 @@ __global__ void indset_iteration(LO n, LO const* a2ab, LO const* ab2b, Real cons
   new_state[v] = 1;
 }
 
+// the same round for edges, whose star (edges_across_tris/tets,
+// src/Omega_h_adj.cpp:532-589) is the set of other edges of the elements
+// around the edge: read through the edge -> element adjacency instead of a
+// built star (an edge met twice is checked twice, which changes nothing)
+template <int NEPC>
+__global__ void indset_iteration_edges(LO n, LO const* e2c_a2ab, LO const* e2c_ab2b,
+    LO const* c2e, Real const* q, I8 const* old_state, I8* new_state) {
+  LO v = thread_index();
+  if (v >= n) return;
+  I8 s = old_state[v];
+  new_state[v] = s;
+  if (s != 2) return;
+  LO b = e2c_a2ab[v], e = e2c_a2ab[v + 1];
+  for (LO j = b; j < e; ++j) {
+    LO const* ce = c2e + e2c_ab2b[j] * NEPC;
+#pragma unroll
+    for (int k = 0; k < NEPC; ++k) {
+      LO u = ce[k];
+      if (u != v && old_state[u] == 1) {
+        new_state[v] = 0;
+        return;
+      }
+    }
+  }
+  Real qv = q[v];
+  for (LO j = b; j < e; ++j) {
+    LO const* ce = c2e + e2c_ab2b[j] * NEPC;
+#pragma unroll
+    for (int k = 0; k < NEPC; ++k) {
+      LO u = ce[k];
+      if (u == v || old_state[u] == 0) continue;
+      Real qu = q[u];
+      bool beats = (qu != qv) ? (qu < qv) : (u < v);
+      if (!beats) return;
+    }
+  }
+  new_state[v] = 1;
+}
+
 __global__ void mark_down_kernel(LO nl, LO const* a2ab, LO const* ab2b, I8 const* marks, I8* out) {
   LO l = thread_index();
   if (l >= nl) return;
@@ -225,13 +264,22 @@ __global__ void below_kernel(LO n, Real const* q, Real ceil, I8* out) {
   if (i < n) out[i] = q[i] < ceil;
 }
 
-__global__ void dual_max_kernel(LO n, LO const* a2ab, LO const* ab2b, I8 const* in, I8* out) {
+// graph_reduce over the dual graph with max (src/Omega_h_graph.cpp:62-80 on
+// elements_across_sides): the max over the elements across each interior
+// side, -128 for an element with none
+template <int NSPE>
+__global__ void dual_max_kernel(LO n, LO const* es2s, LO const* side_elems, I8 const* in,
+    I8* out) {
   LO e = thread_index();
   if (e >= n) return;
   I8 r = -128;
-  for (LO j = a2ab[e]; j < a2ab[e + 1]; ++j) {
-    I8 x = in[ab2b[j]];
-    r = (r < x) ? x : r;
+#pragma unroll
+  for (int j = 0; j < NSPE; ++j) {
+    int2 se = reinterpret_cast<int2 const*>(side_elems)[es2s[e * NSPE + j]];
+    if (se.x != se.y) {
+      I8 x = in[se.x == e ? se.y : se.x];
+      r = (r < x) ? x : r;
+    }
   }
   out[e] = r;
 }
@@ -410,7 +458,15 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
     if (any_cands) *any_cands = max_i8(c, cands) > 0;
     return cands;
   }
-  Adj const& g = ask_star(c, m, d);
+  Adj const* g = nullptr;
+  Adj const* e2c = nullptr;
+  LO const* c2e = nullptr;
+  if (d == 1) {
+    e2c = &ask_up(c, m, 1, m.dim);
+    c2e = ask_down(c, m, m.dim, 1, nullptr).data();
+  } else {
+    g = &ask_star(c, m, d);
+  }
   Bytes a(c, n), b(c, n);
   OSH_LAUNCH(indset_init, n, c, cands.data(), a.data());
   // a round leaves decided nodes as they are, so rounds after convergence
@@ -423,8 +479,15 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
     if (!undecided) break;
     OSH_CHECK(round <= n + 1, "independent set does not converge");
     for (int k = 0; k < 2; ++k, ++round) {
-      OSH_LAUNCH(indset_iteration, n, c, g.a2ab.data(), g.ab2b.data(), quality.data(),
-          a.data(), b.data());
+      if (g)
+        OSH_LAUNCH(indset_iteration, n, c, g->a2ab.data(), g->ab2b.data(), quality.data(),
+            a.data(), b.data());
+      else if (m.dim == 3)
+        OSH_LAUNCH(indset_iteration_edges<6>, n, c, e2c->a2ab.data(), e2c->ab2b.data(), c2e,
+            quality.data(), a.data(), b.data());
+      else
+        OSH_LAUNCH(indset_iteration_edges<3>, n, c, e2c->a2ab.data(), e2c->ab2b.data(), c2e,
+            quality.data(), a.data(), b.data());
       std::swap(a, b);
     }
   }
@@ -445,11 +508,15 @@ Bytes mark_sliver_layers(Ctx& c, DMesh& m, Real qual_ceil, int nlayers) {
   Reals const& q = ask_qualities(c, m);
   Bytes marks(c, ne);
   OSH_LAUNCH(below_kernel, ne, c, q.data(), qual_ceil, marks.data());
-  Adj const& dual = ask_dual(c, m);
+  LOs const& se = ask_side_elems(c, m);
   for (int i = 0; i < nlayers; ++i) {
     Bytes next(c, ne);
-    OSH_LAUNCH(dual_max_kernel, ne, c, dual.a2ab.data(), dual.ab2b.data(), marks.data(),
-        next.data());
+    if (m.dim == 3)
+      OSH_LAUNCH(dual_max_kernel<4>, ne, c, m.down[3].data(), se.data(), marks.data(),
+          next.data());
+    else
+      OSH_LAUNCH(dual_max_kernel<3>, ne, c, m.down[2].data(), se.data(), marks.data(),
+          next.data());
     marks = next;
   }
   return marks;

@@ -98,8 +98,8 @@ void Ctx::sync() {
 
 namespace {
 
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kBlock * kScanItems;  // 2048 values per block
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems;  // 4096 values per block
 constexpr int kWave = 64;
 
 __device__ __forceinline__ LO wave_inclusive_scan(LO v) {
@@ -109,6 +109,12 @@ __device__ __forceinline__ LO wave_inclusive_scan(LO v) {
     LO u = __shfl_up(v, d, kWave);
     if (lane >= d) v += u;
   }
+  return v;
+}
+
+__device__ __forceinline__ LO wave_sum(LO v) {
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
   return v;
 }
 
@@ -139,17 +145,21 @@ __global__ void fill_kernel(LO n, T* out, T value) {
   if (i < n) out[i] = value;
 }
 
-// Single-pass exclusive scan with decoupled look-back.  Each workgroup takes
-// the next tile ticket from a counter (so every tile it waits on belongs to
-// a workgroup that is already running), scans its 2048 values, publishes
-// its aggregate, then walks back over the predecessors' published words
-// until it meets an inclusive prefix.  A status word is
-// [epoch:24 | flag:8 | value:32]; the epoch changes every scan, so words
-// left by earlier scans read as "not yet published" and no clearing pass is
-// needed.  The word carries the value itself, so relaxed agent-scope atomics
-// suffice.  A wait that outlasts kSpinLimit polls (it cannot happen on a
-// correct schedule) stops, sets the context's error word and lets every
-// workgroup finish; the next stream sync raises it.
+// Single-pass exclusive scan with decoupled look-back.  Each workgroup of a
+// multi-tile scan takes the next tile ticket from a counter (so every tile it
+// waits on belongs to a workgroup that is already running), scans its 4096
+// values and publishes its aggregate; its first wave then reads the words of
+// the 64 preceding tiles at once, one per lane, and adds the aggregates back
+// to the nearest inclusive prefix, moving the window back 64 tiles at a time
+// until it meets one.  A status word is [epoch:24 | flag:8 | value:32]; the
+// epoch changes every scan, so words left by earlier scans read as "not yet
+// published" and no clearing pass is needed.  The word carries the value
+// itself, so relaxed agent-scope atomics suffice.  A wait that outlasts
+// kSpinLimit polls (it cannot happen on a correct schedule) stops, sets the
+// context's error word and lets every workgroup finish; the next stream sync
+// raises it.  Values pass through LDS so that global loads and stores are
+// coalesced while each thread scans 16 consecutive values; LDS index
+// j + j/16 keeps both access patterns free of bank conflicts.
 constexpr unsigned kFlagAggregate = 1u;
 constexpr unsigned kFlagInclusive = 2u;
 constexpr unsigned kSpinLimit = 1u << 20;
@@ -159,71 +169,104 @@ __device__ __forceinline__ unsigned long long status_word(unsigned epoch, unsign
          (static_cast<unsigned long long>(flag) << 32) | static_cast<unsigned>(v);
 }
 
+// the exclusive prefix of `tile` from its predecessors' status words; run by
+// the 64 lanes of one wave, returns the same value in every lane
+__device__ __forceinline__ LO look_back(unsigned long long* status, unsigned tile,
+    unsigned epoch, int* error) {
+  int lane = threadIdx.x & (kWave - 1);
+  LO prefix = 0;
+  long long window = static_cast<long long>(tile) - 1;  // the tile lane 0 reads
+  unsigned spins = 0;
+  for (;;) {
+    long long j = window - lane;
+    unsigned long long w = status_word(epoch, kFlagInclusive, 0);  // before tile 0
+    if (j >= 0) w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned flag = unsigned(w >> 32) & 0xffu;
+    bool ready = unsigned(w >> 40) == epoch && flag != 0;
+    if (__ballot(!ready) != 0ull) {
+      if (++spins > kSpinLimit) {
+        if (lane == 0) *error = kScanTimeout;
+        return prefix;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    LO v = LO(unsigned(w));
+    unsigned long long inclusive = __ballot(flag == kFlagInclusive);
+    if (inclusive != 0ull) {
+      int nearest = __ffsll(static_cast<long long>(inclusive)) - 1;
+      if (lane > nearest) v = 0;
+      return prefix + wave_sum(v);
+    }
+    prefix += wave_sum(v);
+    window -= kWave;
+  }
+}
+
 template <class T, bool NONZERO>
 __global__ void __launch_bounds__(kBlock) scan_lookback(LO n, T const* in, LO* out,
     unsigned long long* status, unsigned* tile_counter, unsigned ticket_base, unsigned epoch,
     LO* total_out, int* error) {
+  __shared__ LO s_vals[kScanTile + kScanTile / 16];
   __shared__ unsigned s_tile;
   __shared__ LO s_prefix;
-  if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u) - ticket_base;
-  __syncthreads();
-  unsigned tile = s_tile;
+  bool single = gridDim.x == 1;
+  if (!single) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u) - ticket_base;
+    __syncthreads();
+  }
+  unsigned tile = single ? 0u : s_tile;
   if (tile >= gridDim.x) {  // the ticket base is out of step: refuse to write
     if (threadIdx.x == 0) *error = kScanTicket;
     return;
   }
-  LO base = LO(tile) * kScanTile + LO(threadIdx.x) * kScanItems;
+  LO tile_base = LO(tile) * kScanTile;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int j = k * kBlock + int(threadIdx.x);
+    LO i = tile_base + j;
+    LO v = 0;
+    if (i < n) v = NONZERO ? LO(in[i] != 0) : LO(in[i]);
+    s_vals[j + (j >> 4)] = v;
+  }
+  __syncthreads();
+  int row = int(threadIdx.x) * (kScanItems + 1);  // padded index of value 16 * threadIdx.x
   LO vals[kScanItems];
   LO acc = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    LO i = base + k;
-    LO v = 0;
-    if (i < n) v = NONZERO ? LO(in[i] != 0) : LO(in[i]);
-    acc += v;
+    acc += s_vals[row + k];
     vals[k] = acc;
   }
   LO total;
   LO excl = block_exclusive_scan(acc, &total);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < kWave) {
     LO prefix = 0;
-    if (tile == 0) {
-      __hip_atomic_store(&status[0], status_word(epoch, kFlagInclusive, total), __ATOMIC_RELAXED,
-          __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      __hip_atomic_store(&status[tile], status_word(epoch, kFlagAggregate, total),
-          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      unsigned j = tile - 1;
-      unsigned spins = 0;
-      for (;;) {
-        unsigned long long w =
-            __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned flag = unsigned(w >> 32) & 0xffu;
-        if (unsigned(w >> 40) != epoch || flag == 0) {
-          if (++spins > kSpinLimit) {
-            *error = kScanTimeout;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        prefix += LO(unsigned(w));
-        if (flag == kFlagInclusive || j == 0) break;
-        --j;
-      }
-      __hip_atomic_store(&status[tile], status_word(epoch, kFlagInclusive, prefix + total),
-          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tile != 0) {
+      if (threadIdx.x == 0)
+        __hip_atomic_store(&status[tile], status_word(epoch, kFlagAggregate, total),
+            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      prefix = look_back(status, tile, epoch, error);
     }
-    s_prefix = prefix;
-    if (tile == 0) out[0] = 0;
-    if (total_out && tile == gridDim.x - 1) *total_out = prefix + total;
+    if (threadIdx.x == 0) {
+      if (!single)
+        __hip_atomic_store(&status[tile], status_word(epoch, kFlagInclusive, prefix + total),
+            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_prefix = prefix;
+      if (tile == 0) out[0] = 0;
+      if (total_out && tile == gridDim.x - 1) *total_out = prefix + total;
+    }
   }
   __syncthreads();
   LO prefix = s_prefix + excl;
 #pragma unroll
+  for (int k = 0; k < kScanItems; ++k) s_vals[row + k] = prefix + vals[k];
+  __syncthreads();
+#pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    LO i = base + k;
-    if (i < n) out[1 + i] = prefix + vals[k];
+    int j = k * kBlock + int(threadIdx.x);
+    LO i = tile_base + j;
+    if (i < n) out[1 + i] = s_vals[j + (j >> 4)];
   }
 }
 
@@ -267,8 +310,9 @@ LOs offset_scan_t(Ctx& c, DArray<T> const& counts, LO* total_out) {
   LO nb = LO((std::int64_t(n) + kScanTile - 1) / kScanTile);
   unsigned epoch;
   unsigned long long* status = scan_status_words(c, nb, &epoch);
+  // a one-tile scan takes no ticket
   unsigned ticket_base = c.tile_tickets;
-  c.tile_tickets += unsigned(nb);
+  if (nb > 1) c.tile_tickets += unsigned(nb);
   hipLaunchKernelGGL((scan_lookback<T, NONZERO>), dim3(nb), dim3(kBlock), 0, c.stream, n,
       counts.data(), out.data(), status, c.tile_counter, ticket_base, epoch, total_out,
       c.error_word());
