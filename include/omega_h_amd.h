@@ -124,6 +124,13 @@ int osh_measure_qualities(osh_ctx* ctx, int dim, int metric_ncomps, int n,
     const int* elems, const int* ev2v, const double* coords,
     const double* metric, double* out);
 
+/* offset_scan(Read<I32>), src/Omega_h_int_scan.cpp:10-20, for njobs (1 to 4)
+ * arrays at once on device pointers: out[j] (n[j] + 1 ints) receives 0
+ * followed by the running sums of in[j] (n[j] ints).  Asynchronous like
+ * osh_measure_edges_metric. */
+int osh_offset_scans(osh_ctx* ctx, int njobs, const int* const* in, const int* n,
+    int* const* out);
+
 /* limit_metric_gradation(mesh, metric, max_rate, tol),
  * src/Omega_h_metric.cpp:200-220, applied to the mesh's metric in place
  * (each sweep intersects every vertex metric with its neighbours' scaled

@@ -120,6 +120,7 @@ def load(path=None):
         "osh_adapt": [P, C.POINTER(AdaptOpts), C.POINTER(AdaptStats), IP],
         "osh_measure_edges_metric": [P, I, I, I, P, P, P, P, P],
         "osh_measure_qualities": [P, I, I, I, P, P, P, P, P],
+        "osh_offset_scans": [P, I, P, IP, P],
         "osh_refine_qualities": [P, I, IP, DP],
         "osh_find_loop": [P, I, IP, IP, IP],
         "osh_find_indset": [P, I, DP, BP, BP],
@@ -189,6 +190,15 @@ class Context:
         check(self.lib.osh_measure_edges_metric(self.h, dim, metric_ncomps, n, edges_ptr or None,
                                                 ev2v_ptr, coords_ptr, metric_ptr, out_ptr),
               self.lib)
+
+    def offset_scans(self, in_ptrs, sizes, out_ptrs):
+        """offset_scan of up to four int arrays on device pointers, in one launch on
+        this context's stream: out[j] gets n[j] + 1 exclusive offsets."""
+        k = len(in_ptrs)
+        ins = (C.c_void_p * k)(*[p or None for p in in_ptrs])
+        outs = (C.c_void_p * k)(*out_ptrs)
+        ns = (C.c_int * k)(*sizes)
+        check(self.lib.osh_offset_scans(self.h, k, ins, ns, outs), self.lib)
 
     def synchronize(self):
         check(self.lib.osh_ctx_synchronize(self.h), self.lib)

@@ -103,22 +103,28 @@ __global__ void transit_kernel(LO nhigh, LO const* hm2m, I8 const* m2hm_codes,
 //               writes itself at that rank, which sorts every row by high
 //               index (sort_by_high_index, src/Omega_h_adj.cpp:178-200)
 //               and forms the codes of separate_upward_with_codes (:202-217)
-__global__ void count_lows(LO nhl, LO const* hl2l, LO* counts, LO* slot) {
+// With `marks`, only the rows of marked lows are built (the rest are empty).
+__global__ void count_lows(LO nhl, LO const* hl2l, I8 const* marks, LO* counts, LO* slot) {
   LO i = thread_index();
-  if (i < nhl) slot[i] = atomicAdd(&counts[hl2l[i]], 1);
+  if (i >= nhl) return;
+  LO l = hl2l[i];
+  if (!marks || marks[l]) slot[i] = atomicAdd(&counts[l], 1);
 }
 
-__global__ void fill_uses(LO nhl, LO const* hl2l, LO const* a2ab, LO const* slot, LO* ab2hl) {
-  LO hl = thread_index();
-  if (hl >= nhl) return;
-  ab2hl[a2ab[hl2l[hl]] + slot[hl]] = hl;
-}
-
-__global__ void rank_uses(LO nhl, LO const* hl2l, LO const* a2ab, LO const* ab2hl, int deg,
-    I8 const* down_codes, LO* ab2b, I8* codes) {
+__global__ void fill_uses(LO nhl, LO const* hl2l, I8 const* marks, LO const* a2ab,
+    LO const* slot, LO* ab2hl) {
   LO hl = thread_index();
   if (hl >= nhl) return;
   LO l = hl2l[hl];
+  if (!marks || marks[l]) ab2hl[a2ab[l] + slot[hl]] = hl;
+}
+
+__global__ void rank_uses(LO nhl, LO const* hl2l, I8 const* marks, LO const* a2ab,
+    LO const* ab2hl, int deg, I8 const* down_codes, LO* ab2b, I8* codes) {
+  LO hl = thread_index();
+  if (hl >= nhl) return;
+  LO l = hl2l[hl];
+  if (marks && !marks[l]) return;
   LO b = a2ab[l], e = a2ab[l + 1];
   LO rank = 0;
   for (LO j = b; j < e; ++j) rank += (ab2hl[j] < hl);
@@ -130,18 +136,20 @@ __global__ void rank_uses(LO nhl, LO const* hl2l, LO const* a2ab, LO const* ab2h
   codes[b + rank] = c;
 }
 
-Adj invert_adj(Ctx& c, LO nhigh, int deg, LO nlow, LOs const& hl2l, Bytes const* down_codes) {
+Adj invert_adj(Ctx& c, LO nhigh, int deg, LO nlow, LOs const& hl2l, Bytes const* down_codes,
+    I8 const* marks = nullptr) {
   LO nhl = nhigh * deg;
   LOs counts = fill_lo(c, nlow, 0);
   LOs slot(c, nhl);
-  OSH_LAUNCH(count_lows, nhl, c, hl2l.data(), counts.data(), slot.data());
+  OSH_LAUNCH(count_lows, nhl, c, hl2l.data(), marks, counts.data(), slot.data());
   Adj a;
   a.a2ab = offset_scan(c, counts);
+  // with marks the rows hold at most nhl uses; the size stays an upper bound
   LOs ab2hl(c, nhl);
-  OSH_LAUNCH(fill_uses, nhl, c, hl2l.data(), a.a2ab.data(), slot.data(), ab2hl.data());
+  OSH_LAUNCH(fill_uses, nhl, c, hl2l.data(), marks, a.a2ab.data(), slot.data(), ab2hl.data());
   a.ab2b = LOs(c, nhl);
   a.codes = Bytes(c, nhl);
-  OSH_LAUNCH(rank_uses, nhl, c, hl2l.data(), a.a2ab.data(), ab2hl.data(), deg,
+  OSH_LAUNCH(rank_uses, nhl, c, hl2l.data(), marks, a.a2ab.data(), ab2hl.data(), deg,
       down_codes ? down_codes->data() : (I8 const*)nullptr, a.ab2b.data(), a.codes.data());
   return a;
 }
@@ -184,19 +192,23 @@ __global__ void count_min_rows(LO nl, LO const* lv2v, LO* counts, LO* slot) {
   slot[l] = counts[vm] ? atomicAdd(&counts[vm], 1) - 1 : -1;
 }
 
+// a row entry is {entity, its vertices}, so the match reads no other array
 template <int NVL>
-__global__ void fill_min_rows(LO nl, LO const* lv2v, LO const* a2ab, LO const* slot, LO* rows) {
+__global__ void fill_min_rows(LO nl, LO const* lv2v, LO const* a2ab, LO const* slot, int4* rows) {
   LO l = thread_index();
   if (l >= nl) return;
   LO s = slot[l];
   if (s < 0) return;
-  rows[a2ab[min_vertex<NVL>(lv2v + l * NVL)] + s] = l;
+  LO v[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < NVL; ++k) v[k] = lv2v[l * NVL + k];
+  rows[a2ab[min_vertex<NVL>(v)] + s] = make_int4(l, v[0], v[1], v[2]);
 }
 
 template <int HIGH>
 __global__ void match_uses(LO nuses, LO const* __restrict__ prod_verts,
-    LO const* __restrict__ lv2v, LO const* __restrict__ a2ab, LO const* __restrict__ rows,
-    LO* __restrict__ out, I8* __restrict__ out_codes, int* error) {
+    LO const* __restrict__ a2ab, int4 const* __restrict__ rows, LO* __restrict__ out,
+    I8* __restrict__ out_codes, int* error) {
   LO u = thread_index();
   if (u >= nuses) return;
   constexpr int LOW = HIGH - 1;
@@ -214,26 +226,24 @@ __global__ void match_uses(LO nuses, LO const* __restrict__ prod_verts,
   I8 fc = 0;
 #pragma unroll 4
   for (LO k = k0; k < k1; ++k) {
-    LO b = rows[k];
-    LO bv[3];
-#pragma unroll
-    for (int j = 0; j < nvl; ++j) bv[j] = lv2v[b * nvl + j];
-    // wd: the position of the use's first vertex in b
+    int4 r = rows[k];
+    LO bv[3] = {r.y, r.z, r.w};
+    // wd: the position of the use's first vertex in the candidate
     int wd = (bv[0] == av[0]) ? 0 : (bv[1] == av[0]) ? 1 : (nvl == 3 && bv[2] == av[0]) ? 2 : -1;
     if (wd < 0) continue;
     if (nvl == 2) {
       if (av[1] == bv[1 - wd]) {
-        fb = b;
+        fb = r.x;
         fc = make_code(0, wd, 0);
         ++found;
       }
     } else {
       if (av[1] == bv[(wd + 1) % 3] && av[2] == bv[(wd + 2) % 3]) {
-        fb = b;
+        fb = r.x;
         fc = make_code(0, (3 - wd) % 3, 0);
         ++found;
       } else if (av[1] == bv[(wd + 2) % 3] && av[2] == bv[(wd + 1) % 3]) {
-        fb = b;
+        fb = r.x;
         fc = make_code(1, (3 - wd) % 3, 0);
         ++found;
       }
@@ -343,6 +353,13 @@ LOs const& ask_down(Ctx& c, DMesh& m, int high, int low, Bytes const** codes) {
   return m.t2e;
 }
 
+Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks) {
+  Bytes const* codes = nullptr;
+  LOs const& down = ask_down(c, m, high, low, &codes);
+  return invert_adj(c, m.nents[high], simplex_degree(high, low), m.nents[low], down, codes,
+      marks.data());
+}
+
 Adj const& ask_up(Ctx& c, DMesh& m, int low, int high) {
   if (!m.up[low][high].exists()) {
     Bytes const* codes = nullptr;
@@ -374,17 +391,17 @@ void reflect_down(Ctx& c, DMesh& m, int high_dim, LO nprods, LOs const& prod_ver
   }
   LOs a2ab = offset_scan(c, counts);
   // every lower entity in at most one row plus one spare slot per marked row
-  LOs rows(c, nl + nv);
+  DArray<int4> rows(c, nl + nv);
   // a use without a unique match sets the context's error word, which the
   // next stream sync raises
   if (high_dim == 2) {
     OSH_LAUNCH(fill_min_rows<2>, nl, c, lv2v.data(), a2ab.data(), slot.data(), rows.data());
-    OSH_LAUNCH(match_uses<2>, nuses, c, prod_verts.data(), lv2v.data(), a2ab.data(), rows.data(),
-        down.data(), codes.data(), c.error_word());
+    OSH_LAUNCH(match_uses<2>, nuses, c, prod_verts.data(), a2ab.data(), rows.data(), down.data(),
+        codes.data(), c.error_word());
   } else {
     OSH_LAUNCH(fill_min_rows<3>, nl, c, lv2v.data(), a2ab.data(), slot.data(), rows.data());
-    OSH_LAUNCH(match_uses<3>, nuses, c, prod_verts.data(), lv2v.data(), a2ab.data(), rows.data(),
-        down.data(), codes.data(), c.error_word());
+    OSH_LAUNCH(match_uses<3>, nuses, c, prod_verts.data(), a2ab.data(), rows.data(), down.data(),
+        codes.data(), c.error_word());
   }
 }
 

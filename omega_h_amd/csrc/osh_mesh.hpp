@@ -68,6 +68,9 @@ int simplex_degree(int from_dim, int to_dim);
 LOs const& verts_of(Ctx& c, DMesh& m, int d);
 LOs const& ask_down(Ctx& c, DMesh& m, int high, int low, Bytes const** codes);
 Adj const& ask_up(Ctx& c, DMesh& m, int low, int high);
+// the upward adjacency with only the rows of marked lows filled (the others
+// are empty); not cached
+Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks);
 Adj const& ask_star(Ctx& c, DMesh& m, int d);
 LOs const& ask_side_elems(Ctx& c, DMesh& m);
 // vertex lists of new entities of dimension high_dim >= 2 matched against the

@@ -20,15 +20,29 @@ __global__ void mark_image_kernel(LO n, LO const* a2b, I8* marks) {
   if (i < n) marks[a2b[i]] = 1;
 }
 
+__global__ void init_key_marks(LO n, I8* marks, LO* first_up) {
+  LO i = thread_index();
+  if (i >= n) return;
+  marks[i] = 0;
+  first_up[i] = 0x7fffffff;
+}
+
 // the same entities of dimension > key dimension: those with no key among
-// their lower entities
+// their lower entities; each key also receives the lowest index among the
+// entities above it, which is the first entry of its row of the upward
+// adjacency (rows are sorted by index), without building that adjacency
 __global__ void mark_same_up_kernel(LO nh, int deg, LO const* hl2l, I8 const* low_marks,
-    I8* same) {
+    I8* same, LO* first_up) {
   LO h = thread_index();
   if (h >= nh) return;
   I8 r = 1;
-  for (int j = 0; j < deg; ++j)
-    if (low_marks[hl2l[h * deg + j]]) r = 0;
+  for (int j = 0; j < deg; ++j) {
+    LO l = hl2l[h * deg + j];
+    if (low_marks[l]) {
+      r = 0;
+      atomicMin(&first_up[l], h);
+    }
+  }
   same[h] = r;
 }
 
@@ -43,9 +57,10 @@ __global__ void counts_from_same(LO n, I8 const* same, LO* counts) {
 }
 
 // new numbers of the old entities in one sweep: same entities take their
-// offset, the rest -1 (src/Omega_h_modify.cpp:141-161)
+// offset, the rest -1 (src/Omega_h_modify.cpp:141-161); the same entities
+// are also listed in order (collect_same)
 __global__ void number_same(LO n, I8 const* same, LO const* same_offsets, LO const* offsets,
-    LO* same2new, LO* old2new) {
+    LO* same2old, LO* same2new, LO* old2new) {
   LO o = thread_index();
   if (o >= n) return;
   if (same && !same[o]) {
@@ -53,14 +68,21 @@ __global__ void number_same(LO n, I8 const* same, LO const* same_offsets, LO con
     return;
   }
   LO s = same ? same_offsets[o] : o;
+  same2old[s] = o;
   same2new[s] = offsets[o];
   old2new[o] = offsets[o];
 }
 
-__global__ void reps_up_kernel(LO nkeys, LO const* keys2kds, LO const* a2ab, LO const* ab2b,
-    LO* reps) {
+__global__ void reps_up_kernel(LO nkeys, LO const* keys2kds, LO const* first_up, LO* reps,
+    int* error) {
   LO k = thread_index();
-  if (k < nkeys) reps[k] = ab2b[a2ab[keys2kds[k]]];
+  if (k >= nkeys) return;
+  LO r = first_up[keys2kds[k]];
+  if (r == 0x7fffffff) {  // keeps every index in range until the host sees it
+    r = 0;
+    *error = kKeyWithoutUp;
+  }
+  reps[k] = r;
 }
 
 __global__ void reps_down_kernel(LO nkeys, LO const* keys2kds, int deg, LO const* down, LO* reps) {
@@ -259,6 +281,14 @@ __global__ void mark_down_kernel(LO nl, LO const* a2ab, LO const* ab2b, I8 const
   out[l] = r;
 }
 
+// the same marks by scattering from the marked high entities, for when the
+// upward adjacency is not built
+__global__ void mark_down_scatter(LO nh, int deg, LO const* hl2l, I8 const* marks, I8* out) {
+  LO h = thread_index();
+  if (h >= nh || !marks[h]) return;
+  for (int j = 0; j < deg; ++j) out[hl2l[h * deg + j]] = 1;
+}
+
 __global__ void below_kernel(LO n, Real const* q, Real ceil, I8* out) {
   LO i = thread_index();
   if (i < n) out[i] = q[i] < ceil;
@@ -310,31 +340,32 @@ void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
   // `same` array every old entity is the same
   Bytes same;
   LOs same_offsets;
+  LOs first_up;  // ent_dim > key_dim: per key-dimension entity, the lowest entity above it
   if (ent_dim >= key_dim) {
     if (ent_dim == key_dim) {
       same = fill_i8(c, nold, 1);
       OSH_LAUNCH(clear_image_kernel, nkeys, c, keys2kds.data(), same.data());
     } else {
-      Bytes key_marks = fill_i8(c, old.nents[key_dim], 0);
+      LO nkd = old.nents[key_dim];
+      Bytes key_marks(c, nkd);
+      first_up = LOs(c, nkd);
+      OSH_LAUNCH(init_key_marks, nkd, c, key_marks.data(), first_up.data());
       OSH_LAUNCH(mark_image_kernel, nkeys, c, keys2kds.data(), key_marks.data());
       same = Bytes(c, nold);
       int deg = simplex_degree(ent_dim, key_dim);
       LOs const& hl2l = ask_down(c, old, ent_dim, key_dim, nullptr);
-      OSH_LAUNCH(mark_same_up_kernel, nold, c, deg, hl2l.data(), key_marks.data(), same.data());
+      OSH_LAUNCH(mark_same_up_kernel, nold, c, deg, hl2l.data(), key_marks.data(), same.data(),
+          first_up.data());
     }
-    out.same2old = collect_marked(c, same, &same_offsets);
-  } else {
-    out.same2old = iota(c, nold);
   }
-  LO nsame = out.same2old.size();
   // representatives
   LOs reps;
   if (ent_dim == key_dim) {
     reps = keys2kds;
   } else if (ent_dim > key_dim) {
-    Adj const& up = ask_up(c, old, key_dim, ent_dim);
     reps = LOs(c, nkeys);
-    OSH_LAUNCH(reps_up_kernel, nkeys, c, keys2kds.data(), up.a2ab.data(), up.ab2b.data(), reps.data());
+    OSH_LAUNCH(reps_up_kernel, nkeys, c, keys2kds.data(), first_up.data(), reps.data(),
+        c.error_word());
   } else {
     int deg = simplex_degree(key_dim, ent_dim);
     LOs const& down = ask_down(c, old, key_dim, ent_dim, nullptr);
@@ -349,7 +380,20 @@ void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
     counts = fill_lo(c, nold, 1);
   }
   OSH_LAUNCH(add_rep_counts, nkeys, c, reps.data(), keys2prods.data(), counts.data());
-  LOs offsets = offset_scan(c, counts, &out.nnew);
+  // the offsets of the same entities and of all new entities in one launch
+  // and one sync
+  LOs offsets(c, nold + 1);
+  ScanJob jobs[2] = {{counts.data(), kScanLO, nold, offsets.data(), slot_ptr<LO>(c, 1)},
+      {same.data(), kScanNonzero, nold, nullptr, slot_ptr<LO>(c, 2)}};
+  if (same.exists()) {
+    same_offsets = LOs(c, nold + 1);
+    jobs[1].out = same_offsets.data();
+  }
+  offset_scans(c, jobs, same.exists() ? 2 : 1);
+  c.sync();
+  out.nnew = read_slot<LO>(c, 1);
+  LO nsame = same.exists() ? read_slot<LO>(c, 2) : nold;
+  out.same2old = LOs(c, nsame);
   LOs order;
   if (key_dim > ent_dim) {
     LO nkd = old.nents[key_dim];
@@ -364,7 +408,7 @@ void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
   out.old2new = LOs(c, nold);
   OSH_LAUNCH(number_same, nold, c, same.exists() ? same.data() : (I8 const*)nullptr,
       same.exists() ? same_offsets.data() : (LO const*)nullptr, offsets.data(),
-      out.same2new.data(), out.old2new.data());
+      out.same2old.data(), out.same2new.data(), out.old2new.data());
   out.prods2new = LOs(c, nprods);
   OSH_LAUNCH(prod_numbers, nkeys, c, reps.data(), keys2kds.data(), offsets.data(),
       order.exists() ? order.data() : (LO const*)nullptr, keys2prods.data(),
@@ -452,7 +496,7 @@ void transfer_measures(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo) {
 }
 
 Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& cands,
-    bool* any_cands) {
+    bool* any_cands, Adj const* edge_elems) {
   LO n = m.nents[d];
   if (d == m.dim) {  // src/Omega_h_indset.cpp:29
     if (any_cands) *any_cands = max_i8(c, cands) > 0;
@@ -462,7 +506,7 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
   Adj const* e2c = nullptr;
   LO const* c2e = nullptr;
   if (d == 1) {
-    e2c = &ask_up(c, m, 1, m.dim);
+    e2c = edge_elems ? edge_elems : &ask_up(c, m, 1, m.dim);
     c2e = ask_down(c, m, m.dim, 1, nullptr).data();
   } else {
     g = &ask_star(c, m, d);
@@ -496,6 +540,13 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
 
 Bytes mark_down(Ctx& c, DMesh& m, int high, int low, Bytes const& marks) {
   if (high == low) return marks;
+  if (!m.up[low][high].exists()) {
+    Bytes out = fill_i8(c, m.nents[low], 0);
+    LOs const& hl2l = ask_down(c, m, high, low, nullptr);
+    OSH_LAUNCH(mark_down_scatter, m.nents[high], c, simplex_degree(high, low), hl2l.data(),
+        marks.data(), out.data());
+    return out;
+  }
   Adj const& l2h = ask_up(c, m, low, high);
   Bytes out(c, m.nents[low]);
   OSH_LAUNCH(mark_down_kernel, m.nents[low], c, l2h.a2ab.data(), l2h.ab2b.data(), marks.data(),

@@ -92,6 +92,7 @@ void Ctx::sync() {
       throw Error("reflect_down: entity without a unique match");
     if (code == kScanTimeout) throw Error("scan: a tile waited too long for its predecessor");
     if (code == kScanTicket) throw Error("scan: tile tickets out of step");
+    if (code == kKeyWithoutUp) throw Error("modify: a key entity has no adjacent entity above it");
     throw Error("device error " + std::to_string(code));
   }
 }
@@ -203,31 +204,56 @@ __device__ __forceinline__ LO look_back(unsigned long long* status, unsigned til
   }
 }
 
-template <class T, bool NONZERO>
-__global__ void __launch_bounds__(kBlock) scan_lookback(LO n, T const* in, LO* out,
-    unsigned long long* status, unsigned* tile_counter, unsigned ticket_base, unsigned epoch,
-    LO* total_out, int* error) {
+// up to kMaxScanJobs independent scans share one launch: job j owns the
+// tiles [first[j], first[j + 1]) of the launch and the same range of status
+// words, so its look-back never leaves its own tiles
+struct ScanBatch {
+  void const* in[kMaxScanJobs];
+  LO* out[kMaxScanJobs];
+  LO* total[kMaxScanJobs];
+  LO n[kMaxScanJobs];
+  int kind[kMaxScanJobs];
+  unsigned first[kMaxScanJobs + 1];
+  int count;
+};
+
+__device__ __forceinline__ LO scan_input(void const* in, int kind, LO i) {
+  if (kind == kScanLO) return static_cast<LO const*>(in)[i];
+  I8 x = static_cast<I8 const*>(in)[i];
+  return kind == kScanBytes ? LO(x) : LO(x != 0);
+}
+
+// without tickets (every job one tile) a workgroup's tile is its index
+__global__ void __launch_bounds__(kBlock) scan_lookback(ScanBatch b,
+    unsigned long long* status, unsigned* tile_counter, unsigned ticket_base, int tickets,
+    unsigned epoch, int* error) {
   __shared__ LO s_vals[kScanTile + kScanTile / 16];
   __shared__ unsigned s_tile;
   __shared__ LO s_prefix;
-  bool single = gridDim.x == 1;
-  if (!single) {
+  if (tickets) {
     if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u) - ticket_base;
     __syncthreads();
   }
-  unsigned tile = single ? 0u : s_tile;
-  if (tile >= gridDim.x) {  // the ticket base is out of step: refuse to write
+  unsigned t = tickets ? s_tile : blockIdx.x;
+  if (t >= gridDim.x) {  // the ticket base is out of step: refuse to write
     if (threadIdx.x == 0) *error = kScanTicket;
     return;
   }
+  int job = 0;
+  while (job + 1 < b.count && t >= b.first[job + 1]) ++job;
+  unsigned tile = t - b.first[job];
+  unsigned ntiles = b.first[job + 1] - b.first[job];
+  LO n = b.n[job];
+  int kind = b.kind[job];
+  void const* in = b.in[job];
+  LO* out = b.out[job];
+  unsigned long long* st = status + b.first[job];
   LO tile_base = LO(tile) * kScanTile;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     int j = k * kBlock + int(threadIdx.x);
     LO i = tile_base + j;
-    LO v = 0;
-    if (i < n) v = NONZERO ? LO(in[i] != 0) : LO(in[i]);
-    s_vals[j + (j >> 4)] = v;
+    s_vals[j + (j >> 4)] = (i < n) ? scan_input(in, kind, i) : 0;
   }
   __syncthreads();
   int row = int(threadIdx.x) * (kScanItems + 1);  // padded index of value 16 * threadIdx.x
@@ -244,17 +270,17 @@ __global__ void __launch_bounds__(kBlock) scan_lookback(LO n, T const* in, LO* o
     LO prefix = 0;
     if (tile != 0) {
       if (threadIdx.x == 0)
-        __hip_atomic_store(&status[tile], status_word(epoch, kFlagAggregate, total),
+        __hip_atomic_store(&st[tile], status_word(epoch, kFlagAggregate, total),
             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      prefix = look_back(status, tile, epoch, error);
+      prefix = look_back(st, tile, epoch, error);
     }
     if (threadIdx.x == 0) {
-      if (!single)
-        __hip_atomic_store(&status[tile], status_word(epoch, kFlagInclusive, prefix + total),
+      if (ntiles > 1)
+        __hip_atomic_store(&st[tile], status_word(epoch, kFlagInclusive, prefix + total),
             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_prefix = prefix;
       if (tile == 0) out[0] = 0;
-      if (total_out && tile == gridDim.x - 1) *total_out = prefix + total;
+      if (b.total[job] && tile == ntiles - 1) *b.total[job] = prefix + total;
     }
   }
   __syncthreads();
@@ -297,29 +323,51 @@ unsigned long long* scan_status_words(Ctx& c, LO ntiles, unsigned* epoch) {
   return static_cast<unsigned long long*>(c.scan_status);
 }
 
+}  // namespace
+
+void offset_scans(Ctx& c, ScanJob const* jobs, int njobs) {
+  OSH_CHECK(njobs >= 1 && njobs <= kMaxScanJobs, "offset_scans: 1 to 4 jobs per launch");
+  ScanBatch b{};
+  unsigned tiles = 0;
+  bool multi = false;
+  for (int j = 0; j < njobs; ++j) {
+    OSH_CHECK(jobs[j].n >= 0 && jobs[j].out, "offset_scans: bad job");
+    b.in[j] = jobs[j].in;
+    b.out[j] = jobs[j].out;
+    b.total[j] = jobs[j].total;
+    b.n[j] = jobs[j].n;
+    b.kind[j] = jobs[j].kind;
+    b.first[j] = tiles;
+    // an empty job still takes one tile, which writes its 0 and its total
+    unsigned nt = jobs[j].n == 0 ? 1u : unsigned((std::int64_t(jobs[j].n) + kScanTile - 1) / kScanTile);
+    multi = multi || nt > 1;
+    tiles += nt;
+  }
+  b.first[njobs] = tiles;
+  b.count = njobs;
+  unsigned epoch = 0;
+  unsigned long long* status = nullptr;
+  unsigned ticket_base = c.tile_tickets;
+  if (multi) {
+    status = scan_status_words(c, LO(tiles), &epoch);
+    c.tile_tickets += tiles;
+  }
+  hipLaunchKernelGGL(scan_lookback, dim3(tiles), dim3(kBlock), 0, c.stream, b, status,
+      c.tile_counter, ticket_base, multi ? 1 : 0, epoch, c.error_word());
+  OSH_HIP(hipGetLastError());
+}
+
+namespace {
+
 // the grand total lands in *total_out (device-visible) when given
 template <class T, bool NONZERO>
 LOs offset_scan_t(Ctx& c, DArray<T> const& counts, LO* total_out) {
-  LO n = counts.size();
-  LOs out(c, n + 1);
-  if (n == 0) {
-    OSH_LAUNCH(fill_kernel<LO>, 1, c, out.data(), 0);
-    if (total_out) OSH_LAUNCH(fill_kernel<LO>, 1, c, total_out, 0);
-    return out;
-  }
-  LO nb = LO((std::int64_t(n) + kScanTile - 1) / kScanTile);
-  unsigned epoch;
-  unsigned long long* status = scan_status_words(c, nb, &epoch);
-  // a one-tile scan takes no ticket
-  unsigned ticket_base = c.tile_tickets;
-  if (nb > 1) c.tile_tickets += unsigned(nb);
-  hipLaunchKernelGGL((scan_lookback<T, NONZERO>), dim3(nb), dim3(kBlock), 0, c.stream, n,
-      counts.data(), out.data(), status, c.tile_counter, ticket_base, epoch, total_out,
-      c.error_word());
-  OSH_HIP(hipGetLastError());
+  LOs out(c, counts.size() + 1);
+  ScanJob job{counts.data(), sizeof(T) == 1 ? (NONZERO ? kScanNonzero : kScanBytes) : kScanLO,
+      counts.size(), out.data(), total_out};
+  offset_scans(c, &job, 1);
   return out;
 }
-
 
 __global__ void scatter_marked(LO n, I8 const* marks, LO const* offsets, LO* out) {
   LO i = thread_index();
@@ -392,10 +440,6 @@ LOs offset_scan(Ctx& c, LOs const& counts, LO* total) {
   LOs out = offset_scan_t<LO, false>(c, counts, total ? static_cast<LO*>(c.pinned) : nullptr);
   if (total) *total = pinned_value<LO>(c);
   return out;
-}
-LOs offset_scan_async(Ctx& c, LOs const& counts, int slot) {
-  OSH_CHECK(slot >= 0 && slot < Ctx::kSlots, "pinned slot out of range");
-  return offset_scan_t<LO, false>(c, counts, slot_ptr<LO>(c, slot));
 }
 LOs offset_scan_bytes(Ctx& c, Bytes const& counts, LO* total) {
   LOs out = offset_scan_t<I8, false>(c, counts, total ? static_cast<LO*>(c.pinned) : nullptr);

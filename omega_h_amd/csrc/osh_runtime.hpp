@@ -175,7 +175,13 @@ inline T* slot_ptr(Ctx& c, int slot) {
 }
 
 // error codes kernels leave in Ctx::error_word()
-enum DeviceError { kNoError = 0, kReflectDownNoMatch = 1, kScanTimeout = 2, kScanTicket = 3 };
+enum DeviceError {
+  kNoError = 0,
+  kReflectDownNoMatch = 1,
+  kScanTimeout = 2,
+  kScanTicket = 3,
+  kKeyWithoutUp = 4
+};
 
 template <class T>
 inline T get_value(Ctx& c, T const* dptr) {
@@ -225,11 +231,20 @@ __device__ __forceinline__ LO xcd_thread_index() {
 }
 
 // ---- primitives (osh_prims.hip) ----
+// exclusive scans (n + 1 outputs, out[0] = 0) of LO values, I8 values or I8
+// nonzero flags; several share one launch through offset_scans
+enum ScanKind { kScanLO = 0, kScanBytes = 1, kScanNonzero = 2 };
+constexpr int kMaxScanJobs = 4;
+struct ScanJob {
+  void const* in;
+  int kind;
+  LO n;
+  LO* out;    // n + 1 entries
+  LO* total;  // device-visible grand total (a pinned slot), or nullptr
+};
+void offset_scans(Ctx& c, ScanJob const* jobs, int njobs);
 // n+1 exclusive offsets; with `total`, also the sum (read back after a sync)
 LOs offset_scan(Ctx& c, LOs const& counts, LO* total = nullptr);
-// without a sync: the total lands in pinned slot `slot` (read it with
-// read_slot after the next sync)
-LOs offset_scan_async(Ctx& c, LOs const& counts, int slot);
 LOs offset_scan_bytes(Ctx& c, Bytes const& counts, LO* total = nullptr);
 LOs collect_marked(Ctx& c, Bytes const& marks);       // indices of nonzero
 // the same, also returning the n+1 offsets of the nonzero marks

@@ -427,7 +427,8 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   Bytes configs = fill_i8(c, ne, -1);
   Bytes keep = fill_i8(c, ne, 0);
   Reals quals = fill_real(c, ne, -1.0);
-  Adj const& e2t = ask_up(c, m, 1, 3);
+  // every edge -> tet row this pass reads is a candidate's
+  Adj e2t = marked_up(c, m, 1, 3, cand);
   LOs const& tv2v = verts_of(c, m, 3);
   Reals const& eq = ask_qualities(c, m);
   LO ncands = cands.size();
@@ -436,8 +437,10 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   OSH_LAUNCH(swap3d_loops, ncands, c, cands.data(), e2t.a2ab.data(), e2t.ab2b.data(),
       e2t.codes.data(), m.down[1].data(), tv2v.data(), tb, lsize.data(), leev.data(),
       lverts.data(), ntris.data(), nedges.data());
-  LOs toff = offset_scan_async(c, ntris, 1);
-  LOs eoff = offset_scan_async(c, nedges, 2);
+  LOs toff(c, ncands + 1), eoff(c, ncands + 1);
+  ScanJob loop_jobs[2] = {{ntris.data(), kScanLO, ncands, toff.data(), slot_ptr<LO>(c, 1)},
+      {nedges.data(), kScanLO, ncands, eoff.data(), slot_ptr<LO>(c, 2)}};
+  offset_scans(c, loop_jobs, 2);
   c.sync();
   LO ntp = read_slot<LO>(c, 1), nep = read_slot<LO>(c, 2);
   Reals tq(c, ntp), el(c, nep);
@@ -456,7 +459,7 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
       tq.data(), el.data(), e2t.a2ab.data(), e2t.ab2b.data(), eq.data(), o.max_length_allowed,
       tb, configs.data(), quals.data(), keep.data());
   bool any;
-  Bytes keys = find_indset(c, m, 1, quals, keep, &any);
+  Bytes keys = find_indset(c, m, 1, quals, keep, &any, &e2t);
   if (!any) return false;
   LOs keys2edges = collect_marked(c, keys);
   LO nkeys = keys2edges.size();
@@ -465,9 +468,13 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
       cf.data(), ct.data());
   LOs k2p[4], pv[4];
   LO nprods[4] = {0, 0, 0, 0};
-  k2p[1] = offset_scan_async(c, ce, 1);
-  k2p[2] = offset_scan_async(c, cf, 2);
-  k2p[3] = offset_scan_async(c, ct, 3);
+  ScanJob prod_jobs[3];
+  LOs const* pc[4] = {nullptr, &ce, &cf, &ct};
+  for (int d = 1; d <= 3; ++d) {
+    k2p[d] = LOs(c, nkeys + 1);
+    prod_jobs[d - 1] = {pc[d]->data(), kScanLO, nkeys, k2p[d].data(), slot_ptr<LO>(c, d)};
+  }
+  offset_scans(c, prod_jobs, 3);
   c.sync();
   for (int d = 1; d <= 3; ++d) {
     nprods[d] = read_slot<LO>(c, d);
