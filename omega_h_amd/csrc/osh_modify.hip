@@ -28,11 +28,12 @@ __global__ void init_key_marks(LO n, I8* marks, LO* first_up) {
 }
 
 // the same entities of dimension > key dimension: those with no key among
-// their lower entities; each key also receives the lowest index among the
-// entities above it, which is the first entry of its row of the upward
-// adjacency (rows are sorted by index), without building that adjacency
+// their lower entities, each counting one new entity; each key also receives
+// the lowest index among the entities above it, which is the first entry of
+// its row of the upward adjacency (rows are sorted by index), without
+// building that adjacency
 __global__ void mark_same_up_kernel(LO nh, int deg, LO const* hl2l, I8 const* low_marks,
-    I8* same, LO* first_up) {
+    I8* same, LO* counts, LO* first_up) {
   LO h = thread_index();
   if (h >= nh) return;
   I8 r = 1;
@@ -44,16 +45,16 @@ __global__ void mark_same_up_kernel(LO nh, int deg, LO const* hl2l, I8 const* lo
     }
   }
   same[h] = r;
+  counts[h] = r;
 }
 
-__global__ void clear_image_kernel(LO n, LO const* a2b, I8* marks) {
+// entities of the key dimension: all same, each counting one, until the
+// keys are removed
+__global__ void init_same_counts(LO n, I8* same, LO* counts) {
   LO i = thread_index();
-  if (i < n) marks[a2b[i]] = 0;
-}
-
-__global__ void counts_from_same(LO n, I8 const* same, LO* counts) {
-  LO i = thread_index();
-  if (i < n) counts[i] = same[i] ? 1 : 0;
+  if (i >= n) return;
+  same[i] = 1;
+  counts[i] = 1;
 }
 
 // new numbers of the old entities in one sweep: same entities take their
@@ -73,26 +74,34 @@ __global__ void number_same(LO n, I8 const* same, LO const* same_offsets, LO con
   old2new[o] = offsets[o];
 }
 
-__global__ void reps_up_kernel(LO nkeys, LO const* keys2kds, LO const* first_up, LO* reps,
-    int* error) {
+// each key's representative (get_mods2reps, src/Omega_h_modify.cpp:20-45):
+// the key itself, the lowest entity above it (first_up) or its first lower
+// entity (down); the representative counts the key's products as well
+// (get_rep_counts, :47-70).  A key's own entry was cleared from `same`
+// and its count replaced here when it represents itself.
+__global__ void reps_kernel(LO nkeys, LO const* keys2kds, LO const* first_up, LO const* down,
+    int deg, LO const* keys2prods, I8* same, LO* reps, LO* counts, int* error) {
   LO k = thread_index();
   if (k >= nkeys) return;
-  LO r = first_up[keys2kds[k]];
-  if (r == 0x7fffffff) {  // keeps every index in range until the host sees it
-    r = 0;
-    *error = kKeyWithoutUp;
+  LO kd = keys2kds[k];
+  LO r = kd;
+  LO np = keys2prods[k + 1] - keys2prods[k];
+  if (first_up) {
+    r = first_up[kd];
+    if (r == 0x7fffffff) {  // keeps every index in range until the host sees it
+      r = 0;
+      *error = kKeyWithoutUp;
+    }
+  } else if (down) {
+    r = down[kd * deg];
   }
-  reps[k] = r;
-}
-
-__global__ void reps_down_kernel(LO nkeys, LO const* keys2kds, int deg, LO const* down, LO* reps) {
-  LO k = thread_index();
-  if (k < nkeys) reps[k] = down[keys2kds[k] * deg];
-}
-
-__global__ void add_rep_counts(LO nkeys, LO const* reps, LO const* keys2prods, LO* counts) {
-  LO k = thread_index();
-  if (k < nkeys) atomicAdd(&counts[reps[k]], keys2prods[k + 1] - keys2prods[k]);
+  if (reps) reps[k] = r;
+  if (same) {  // the key is its own representative and no longer the same
+    same[kd] = 0;
+    counts[kd] = np;
+  } else {
+    atomicAdd(&counts[r], np);
+  }
 }
 
 __global__ void scatter_index(LO n, LO const* a2b, LO* b2a) {
@@ -340,46 +349,35 @@ void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
   // `same` array every old entity is the same
   Bytes same;
   LOs same_offsets;
-  LOs first_up;  // ent_dim > key_dim: per key-dimension entity, the lowest entity above it
-  if (ent_dim >= key_dim) {
-    if (ent_dim == key_dim) {
-      same = fill_i8(c, nold, 1);
-      OSH_LAUNCH(clear_image_kernel, nkeys, c, keys2kds.data(), same.data());
-    } else {
-      LO nkd = old.nents[key_dim];
-      Bytes key_marks(c, nkd);
-      first_up = LOs(c, nkd);
-      OSH_LAUNCH(init_key_marks, nkd, c, key_marks.data(), first_up.data());
-      OSH_LAUNCH(mark_image_kernel, nkeys, c, keys2kds.data(), key_marks.data());
-      same = Bytes(c, nold);
-      int deg = simplex_degree(ent_dim, key_dim);
-      LOs const& hl2l = ask_down(c, old, ent_dim, key_dim, nullptr);
-      OSH_LAUNCH(mark_same_up_kernel, nold, c, deg, hl2l.data(), key_marks.data(), same.data(),
-          first_up.data());
-    }
-  }
-  // representatives
-  LOs reps;
-  if (ent_dim == key_dim) {
-    reps = keys2kds;
-  } else if (ent_dim > key_dim) {
-    reps = LOs(c, nkeys);
-    OSH_LAUNCH(reps_up_kernel, nkeys, c, keys2kds.data(), first_up.data(), reps.data(),
-        c.error_word());
-  } else {
-    int deg = simplex_degree(key_dim, ent_dim);
-    LOs const& down = ask_down(c, old, key_dim, ent_dim, nullptr);
-    reps = LOs(c, nkeys);
-    OSH_LAUNCH(reps_down_kernel, nkeys, c, keys2kds.data(), deg, down.data(), reps.data());
-  }
   LOs counts;
-  if (same.exists()) {
+  LOs reps(c, nkeys);
+  if (ent_dim == key_dim) {
+    same = Bytes(c, nold);
     counts = LOs(c, nold);
-    OSH_LAUNCH(counts_from_same, nold, c, same.data(), counts.data());
+    OSH_LAUNCH(init_same_counts, nold, c, same.data(), counts.data());
+    OSH_LAUNCH(reps_kernel, nkeys, c, keys2kds.data(), (LO const*)nullptr, (LO const*)nullptr,
+        0, keys2prods.data(), same.data(), reps.data(), counts.data(), c.error_word());
+  } else if (ent_dim > key_dim) {
+    LO nkd = old.nents[key_dim];
+    Bytes key_marks(c, nkd);
+    LOs first_up(c, nkd);  // per key-dimension entity, the lowest entity above it
+    OSH_LAUNCH(init_key_marks, nkd, c, key_marks.data(), first_up.data());
+    OSH_LAUNCH(mark_image_kernel, nkeys, c, keys2kds.data(), key_marks.data());
+    same = Bytes(c, nold);
+    counts = LOs(c, nold);
+    int deg = simplex_degree(ent_dim, key_dim);
+    LOs const& hl2l = ask_down(c, old, ent_dim, key_dim, nullptr);
+    OSH_LAUNCH(mark_same_up_kernel, nold, c, deg, hl2l.data(), key_marks.data(), same.data(),
+        counts.data(), first_up.data());
+    OSH_LAUNCH(reps_kernel, nkeys, c, keys2kds.data(), first_up.data(), (LO const*)nullptr, 0,
+        keys2prods.data(), (I8*)nullptr, reps.data(), counts.data(), c.error_word());
   } else {
     counts = fill_lo(c, nold, 1);
+    int deg = simplex_degree(key_dim, ent_dim);
+    LOs const& down = ask_down(c, old, key_dim, ent_dim, nullptr);
+    OSH_LAUNCH(reps_kernel, nkeys, c, keys2kds.data(), (LO const*)nullptr, down.data(), deg,
+        keys2prods.data(), (I8*)nullptr, reps.data(), counts.data(), c.error_word());
   }
-  OSH_LAUNCH(add_rep_counts, nkeys, c, reps.data(), keys2prods.data(), counts.data());
   // the offsets of the same entities and of all new entities in one launch
   // and one sync
   LOs offsets(c, nold + 1);
@@ -513,14 +511,12 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
   }
   Bytes a(c, n), b(c, n);
   OSH_LAUNCH(indset_init, n, c, cands.data(), a.data());
-  // a round leaves decided nodes as they are, so rounds after convergence
-  // change nothing: check for undecided nodes every second round only
-  bool first = true;
+  // rounds leave decided nodes as they are and take no node from the set, so
+  // extra rounds after convergence change nothing: run two rounds before each
+  // check; afterwards a state of 2 means undecided nodes remain, and a state
+  // of 1 or 2 that there was a candidate (the best candidate joins the set in
+  // the first round)
   for (LO round = 0;;) {
-    bool undecided = max_i8(c, a) == 2;
-    if (first && any_cands) *any_cands = undecided;
-    first = false;
-    if (!undecided) break;
     OSH_CHECK(round <= n + 1, "independent set does not converge");
     for (int k = 0; k < 2; ++k, ++round) {
       if (g)
@@ -534,6 +530,9 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
             quality.data(), a.data(), b.data());
       std::swap(a, b);
     }
+    I8 top = max_i8(c, a);
+    if (any_cands && round == 2) *any_cands = top >= 1;
+    if (top != 2) break;
   }
   return a;
 }

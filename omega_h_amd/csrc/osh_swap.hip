@@ -87,15 +87,17 @@ __device__ double length_between(LO a, LO b, Real const* coords, Real const* met
   return edge_length<S, M>(p0, p1, m0, m1);
 }
 
-// swap3d_qualities (src/Omega_h_swap3d_qualities.cpp:11-75) in four flat
+// swap3d_qualities (src/Omega_h_swap3d_qualities.cpp:11-75) in five flat
 // sweeps instead of one thread per candidate edge:
 //   swap3d_loops:     the loop of each candidate and its counts of unique
 //                     triangles and unique interior edges
 //   swap3d_tri_quals: one thread per (candidate, unique triangle): the min
 //                     quality of the two tets on that triangle
 //   swap3d_edge_lens: one thread per (candidate, unique edge): its length
-//   swap3d_choose:    per candidate, choose() of src/Omega_h_swap3d_choice.hpp
-//                     over those values, in the reference's order
+//   swap3d_mesh_quals: one thread per (candidate, triangulation): its value
+//                     for choose() of src/Omega_h_swap3d_choice.hpp
+//   swap3d_choose:    per candidate, the choice over those values in the
+//                     reference's order
 // Every value is the one the reference computes (each unique triangle and
 // edge evaluated once), so the choice is the same.
 __global__ void swap3d_loops(LO ncands, LO const* cands, LO const* e2t_a2ab, LO const* e2t_ab2b,
@@ -167,10 +169,38 @@ __global__ void swap3d_edge_lens(LO npairs, LO ncands, LO const* eoff, LO const*
       metric);
 }
 
-__global__ void swap3d_choose(LO ncands, LO const* cands, LO const* lsize, LO const* toff,
-    LO const* eoff, Real const* tq, Real const* el, LO const* e2t_a2ab, LO const* e2t_ab2b,
-    Real const* elem_quals, Real max_length, Tables const* tb, I8* configs, Real* quals,
-    I8* keep) {
+// one thread per (candidate, triangulation) on a fixed stride of
+// kMaxMeshes: the triangulation's minimum triangle quality, or -1 when one of
+// its interior edges is longer than max_length.  choose() keeps the first
+// triangulation whose quality exceeds every earlier kept one (starting from
+// 0) and whose edges pass, so it equals the first triangulation attaining
+// the largest of these values, when that is positive; a minimum that went
+// negative (where choose() stops early) is never kept either way.
+constexpr int kMaxMeshes = 42;
+
+__global__ void swap3d_mesh_quals(LO nt, LO const* lsize, LO const* toff, LO const* eoff,
+    Real const* tq, Real const* el, Real max_length, Tables const* tb, Real* mq) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO cand = t / kMaxMeshes;
+  int mesh = t % kMaxMeshes;
+  int n = lsize[cand];
+  if (n > swap::kMaxLoop || mesh >= tb->mesh_counts[n]) return;
+  int nmesh_tris = tb->mesh_sizes[n];
+  int nmesh_edges = tb->nedges[n];
+  Real const* ctq = tq + toff[cand];
+  Real const* cel = el + eoff[cand];
+  double q = 1.0;
+  for (int mt = 0; mt < nmesh_tris; ++mt)
+    q = math::min2(q, ctq[tb->meshes[n][mesh * nmesh_tris + mt]]);
+  for (int me = 0; me < nmesh_edges; ++me)
+    if (cel[tb->medges[n][mesh * nmesh_edges + me]] > max_length) q = -1.0;
+  mq[t] = q;
+}
+
+__global__ void swap3d_choose(LO ncands, LO const* cands, LO const* lsize, Real const* mq,
+    LO const* e2t_a2ab, LO const* e2t_ab2b, Real const* elem_quals, Tables const* tb,
+    I8* configs, Real* quals, I8* keep) {
   LO cand = thread_index();
   if (cand >= ncands) return;
   LO edge = cands[cand];
@@ -180,28 +210,12 @@ __global__ void swap3d_choose(LO ncands, LO const* cands, LO const* lsize, LO co
   if (n <= swap::kMaxLoop) {
     choice_quality = 0.0;
     int nmeshes = tb->mesh_counts[n];
-    int nmesh_tris = tb->mesh_sizes[n];
-    int nmesh_edges = tb->nedges[n];
-    Real const* ctq = tq + toff[cand];
-    Real const* cel = el + eoff[cand];
+    Real const* cmq = mq + LO(cand) * kMaxMeshes;
     for (int mesh = 0; mesh < nmeshes; ++mesh) {
-      double mesh_minqual = 1.0;
-      for (int mt = 0; mt < nmesh_tris; ++mt) {
-        mesh_minqual = math::min2(mesh_minqual, ctq[tb->meshes[n][mesh * nmesh_tris + mt]]);
-        if (mesh_minqual < 0.0) break;
-      }
-      if (mesh_minqual > choice_quality) {
-        bool overshoot = false;
-        for (int me = 0; me < nmesh_edges; ++me) {
-          if (cel[tb->medges[n][mesh * nmesh_edges + me]] > max_length) {
-            overshoot = true;
-            break;
-          }
-        }
-        if (!overshoot) {
-          choice_mesh = mesh;
-          choice_quality = mesh_minqual;
-        }
+      double q = cmq[mesh];
+      if (q > choice_quality) {
+        choice_mesh = mesh;
+        choice_quality = q;
       }
     }
   }
@@ -217,21 +231,54 @@ __global__ void swap3d_choose(LO ncands, LO const* cands, LO const* lsize, LO co
   }
 }
 
+__global__ void swap_init(LO ne, I8* configs, I8* keep, Real* quals) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  if (configs) configs[e] = -1;
+  keep[e] = 0;
+  quals[e] = -1.0;
+}
+
 __global__ void and_interior(LO ne, int dim, I8 const* edim, I8* marks) {
   LO e = thread_index();
   if (e < ne) marks[e] = I8(marks[e] && (edim[e] == dim));
 }
 
-__global__ void swap3d_counts(LO nkeys, LO const* keys2edges, LO const* e2t_a2ab,
-    Tables const* tb, LO* ne, LO* nf, LO* nt) {
-  LO k = thread_index();
-  if (k >= nkeys) return;
-  LO e = keys2edges[k];
-  int ls = e2t_a2ab[e + 1] - e2t_a2ab[e];
-  int npt = tb->mesh_sizes[ls], npe = tb->nedges[ls];
-  ne[k] = npe;
-  nf[k] = npt + 2 * npe;
-  nt[k] = 2 * npt;
+// per edge, the edges, faces and tets a key edge's swap produces (0 for
+// other edges), so that one scan launch numbers the keys and their products
+__global__ void swap3d_counts(LO nedges, I8 const* keys, LO const* e2t_a2ab, Tables const* tb,
+    LO* ne, LO* nf, LO* nt) {
+  LO e = thread_index();
+  if (e >= nedges) return;
+  int npt = 0, npe = 0;
+  if (keys[e]) {
+    int ls = e2t_a2ab[e + 1] - e2t_a2ab[e];
+    npt = tb->mesh_sizes[ls];
+    npe = tb->nedges[ls];
+  }
+  ne[e] = npe;
+  nf[e] = npt + 2 * npe;
+  nt[e] = 2 * npt;
+}
+
+// the key list and each key's product offsets, from the per-edge scans
+__global__ void swap3d_key_offsets(LO nedges, I8 const* keys, LO const* key_off,
+    LO const* eoff, LO const* foff, LO const* toff, LO* keys2edges, LO* k2e, LO* k2f,
+    LO* k2t) {
+  LO e = thread_index();
+  if (e >= nedges) return;
+  if (e == 0) {
+    LO nk = key_off[nedges];
+    k2e[nk] = eoff[nedges];
+    k2f[nk] = foff[nedges];
+    k2t[nk] = toff[nedges];
+  }
+  if (!keys[e]) return;
+  LO k = key_off[e];
+  keys2edges[k] = e;
+  k2e[k] = eoff[e];
+  k2f[k] = foff[e];
+  k2t[k] = toff[e];
 }
 
 __global__ void swap3d_products(LO nkeys, LO const* keys2edges, I8 const* configs,
@@ -424,9 +471,9 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   LO ne = m.nents[1];
   int md = (m.metric_ncomps == 1) ? 1 : 3;
   Tables const* tb = device_tables(c);
-  Bytes configs = fill_i8(c, ne, -1);
-  Bytes keep = fill_i8(c, ne, 0);
-  Reals quals = fill_real(c, ne, -1.0);
+  Bytes configs(c, ne), keep(c, ne);
+  Reals quals(c, ne);
+  OSH_LAUNCH(swap_init, ne, c, configs.data(), keep.data(), quals.data());
   // every edge -> tet row this pass reads is a candidate's
   Adj e2t = marked_up(c, m, 1, 3, cand);
   LOs const& tv2v = verts_of(c, m, 3);
@@ -455,31 +502,40 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
     OSH_LAUNCH(swap3d_edge_lens<1>, nep, c, ncands, eoff.data(), lsize.data(), lverts.data(),
         m.coords.data(), m.metric.data(), tb, el.data());
   }
-  OSH_LAUNCH(swap3d_choose, ncands, c, cands.data(), lsize.data(), toff.data(), eoff.data(),
-      tq.data(), el.data(), e2t.a2ab.data(), e2t.ab2b.data(), eq.data(), o.max_length_allowed,
-      tb, configs.data(), quals.data(), keep.data());
+  Reals mq(c, ncands * kMaxMeshes);
+  OSH_LAUNCH(swap3d_mesh_quals, ncands * kMaxMeshes, c, lsize.data(), toff.data(), eoff.data(),
+      tq.data(), el.data(), o.max_length_allowed, tb, mq.data());
+  OSH_LAUNCH(swap3d_choose, ncands, c, cands.data(), lsize.data(), mq.data(), e2t.a2ab.data(),
+      e2t.ab2b.data(), eq.data(), tb, configs.data(), quals.data(), keep.data());
   bool any;
   Bytes keys = find_indset(c, m, 1, quals, keep, &any, &e2t);
   if (!any) return false;
-  LOs keys2edges = collect_marked(c, keys);
-  LO nkeys = keys2edges.size();
-  LOs ce(c, nkeys), cf(c, nkeys), ct(c, nkeys);
-  OSH_LAUNCH(swap3d_counts, nkeys, c, keys2edges.data(), e2t.a2ab.data(), tb, ce.data(),
-      cf.data(), ct.data());
+  // keys and product counts numbered by one scan launch and one sync
+  LOs counts[4], offs[4];
+  ScanJob jobs[4];
+  offs[0] = LOs(c, ne + 1);
+  jobs[0] = {keys.data(), kScanNonzero, ne, offs[0].data(), slot_ptr<LO>(c, 0)};
+  for (int d = 1; d <= 3; ++d) {
+    counts[d] = LOs(c, ne);
+    offs[d] = LOs(c, ne + 1);
+    jobs[d] = {counts[d].data(), kScanLO, ne, offs[d].data(), slot_ptr<LO>(c, d)};
+  }
+  OSH_LAUNCH(swap3d_counts, ne, c, keys.data(), e2t.a2ab.data(), tb, counts[1].data(),
+      counts[2].data(), counts[3].data());
+  offset_scans(c, jobs, 4);
+  c.sync();
+  LO nkeys = read_slot<LO>(c, 0);
+  LOs keys2edges(c, nkeys);
   LOs k2p[4], pv[4];
   LO nprods[4] = {0, 0, 0, 0};
-  ScanJob prod_jobs[3];
-  LOs const* pc[4] = {nullptr, &ce, &cf, &ct};
-  for (int d = 1; d <= 3; ++d) {
-    k2p[d] = LOs(c, nkeys + 1);
-    prod_jobs[d - 1] = {pc[d]->data(), kScanLO, nkeys, k2p[d].data(), slot_ptr<LO>(c, d)};
-  }
-  offset_scans(c, prod_jobs, 3);
-  c.sync();
   for (int d = 1; d <= 3; ++d) {
     nprods[d] = read_slot<LO>(c, d);
     pv[d] = LOs(c, nprods[d] * (d + 1));
+    k2p[d] = LOs(c, nkeys + 1);
   }
+  OSH_LAUNCH(swap3d_key_offsets, ne, c, keys.data(), offs[0].data(), offs[1].data(),
+      offs[2].data(), offs[3].data(), keys2edges.data(), k2p[1].data(), k2p[2].data(),
+      k2p[3].data());
   OSH_LAUNCH(swap3d_products, nkeys, c, keys2edges.data(), configs.data(), e2t.a2ab.data(),
       e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), tb, k2p[1].data(),
       k2p[2].data(), k2p[3].data(), pv[1].data(), pv[2].data(), pv[3].data());
@@ -494,8 +550,9 @@ bool swap_edges_2d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   if (cands.size() == 0) return false;
   LO ne = m.nents[1];
   int md = (m.metric_ncomps == 1) ? 1 : 2;
-  Bytes keep = fill_i8(c, ne, 0);
-  Reals quals = fill_real(c, ne, -1.0);
+  Bytes keep(c, ne);
+  Reals quals(c, ne);
+  OSH_LAUNCH(swap_init, ne, c, (I8*)nullptr, keep.data(), quals.data());
   Adj const& e2t = ask_up(c, m, 1, 2);
   LOs const& tv2v = verts_of(c, m, 2);
   Reals const& eq = ask_qualities(c, m);
