@@ -25,7 +25,7 @@ void DMesh::clear_caches() {
     for (int e = 0; e < 4; ++e) up[d][e] = Adj();
   }
   star[0] = star[1] = Adj();
-  side_elems = LOs();
+  side_elems = DArray<unsigned long long>();
   t2e = LOs();
   t2e_codes = Bytes();
 }
@@ -256,22 +256,19 @@ __global__ void match_uses(LO nuses, LO const* __restrict__ prod_verts,
   if (found != 1) *error = kReflectDownNoMatch;
 }
 
-// the lower and higher element on each side (equal on the boundary), from
-// the element -> side lists by order-free min/max
-__global__ void side_elems_init(LO ns, LO* se) {
+// per side, the number of elements on it (high 32 bits) and the sum of their
+// indices (low 32 bits) by one 64-bit atomic add per element side: with two
+// elements, either one finds the other as the sum minus itself
+__global__ void side_elems_init(LO ns, unsigned long long* se) {
   LO s = thread_index();
-  if (s >= ns) return;
-  se[2 * s] = 0x7fffffff;
-  se[2 * s + 1] = -1;
+  if (s < ns) se[s] = 0ull;
 }
 
-__global__ void side_elems_kernel(LO nuses, int nspe, LO const* es2s, LO* se) {
+__global__ void side_elems_kernel(LO nuses, int nspe, LO const* es2s, unsigned long long* se) {
   LO i = thread_index();
   if (i >= nuses) return;
-  LO s = es2s[i];
   LO e = i / nspe;
-  atomicMin(&se[2 * s], e);
-  atomicMax(&se[2 * s + 1], e);
+  atomicAdd(&se[es2s[i]], (1ull << 32) + static_cast<unsigned long long>(e));
 }
 
 __global__ void verts_across_edges_kernel(LO nve, LO const* ve2e, I8 const* codes,
@@ -353,11 +350,37 @@ LOs const& ask_down(Ctx& c, DMesh& m, int high, int low, Bytes const** codes) {
   return m.t2e;
 }
 
-Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks) {
+Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks, LOs* marked) {
   Bytes const* codes = nullptr;
   LOs const& down = ask_down(c, m, high, low, &codes);
-  return invert_adj(c, m.nents[high], simplex_degree(high, low), m.nents[low], down, codes,
-      marks.data());
+  if (!marked)
+    return invert_adj(c, m.nents[high], simplex_degree(high, low), m.nents[low], down, codes,
+        marks.data());
+  // the same sweeps as invert_adj, with the list of marked lows scanned in
+  // the same launch
+  int deg = simplex_degree(high, low);
+  LO nlow = m.nents[low];
+  LO nhl = m.nents[high] * deg;
+  LOs counts = fill_lo(c, nlow, 0);
+  LOs slot(c, nhl);
+  OSH_LAUNCH(count_lows, nhl, c, down.data(), marks.data(), counts.data(), slot.data());
+  Adj a;
+  a.a2ab = LOs(c, nlow + 1);
+  LOs marked_offsets(c, nlow + 1);
+  ScanJob jobs[2] = {{counts.data(), kScanLO, nlow, a.a2ab.data(), nullptr},
+      {marks.data(), kScanNonzero, nlow, marked_offsets.data(), slot_ptr<LO>(c, 1)}};
+  offset_scans(c, jobs, 2);
+  c.sync();
+  *marked = LOs(c, read_slot<LO>(c, 1));
+  scatter_marked_list(c, marks, marked_offsets, *marked);
+  LOs ab2hl(c, nhl);
+  OSH_LAUNCH(fill_uses, nhl, c, down.data(), marks.data(), a.a2ab.data(), slot.data(),
+      ab2hl.data());
+  a.ab2b = LOs(c, nhl);
+  a.codes = Bytes(c, nhl);
+  OSH_LAUNCH(rank_uses, nhl, c, down.data(), marks.data(), a.a2ab.data(), ab2hl.data(), deg,
+      codes ? codes->data() : (I8 const*)nullptr, a.ab2b.data(), a.codes.data());
+  return a;
 }
 
 Adj const& ask_up(Ctx& c, DMesh& m, int low, int high) {
@@ -439,11 +462,11 @@ Adj const& ask_star(Ctx& c, DMesh& m, int d) {
   return m.star[d];
 }
 
-LOs const& ask_side_elems(Ctx& c, DMesh& m) {
+DArray<unsigned long long> const& ask_side_elems(Ctx& c, DMesh& m) {
   if (!m.side_elems.exists()) {
     int dim = m.dim;
     LO ns = m.nents[dim - 1];
-    LOs se(c, 2 * ns);
+    DArray<unsigned long long> se(c, ns);
     OSH_LAUNCH(side_elems_init, ns, c, se.data());
     OSH_LAUNCH(side_elems_kernel, m.nents[dim] * (dim + 1), c, dim + 1, m.down[dim].data(),
         se.data());

@@ -368,26 +368,42 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   nw.dim = dim;
   nw.metric_ncomps = nc;
   LOs o2n, old_lows2new_lows;
-  for (int d = 0; d <= dim; ++d) {
-    LOs k2p, pv, psrc;
-    LO nprods = 0;
-    if (d == 0) {
-      k2p = fill_lo(c, nkeys + 1, 0);
-    } else {
+  // products per key of every dimension, numbered by one scan launch
+  LOs k2p[4];
+  LO nprods[4] = {0, 0, 0, 0};
+  k2p[0] = fill_lo(c, nkeys + 1, 0);
+  {
+    LOs counts[4];
+    ScanJob jobs[3];
+    for (int d = 1; d <= dim; ++d) {
       Adj const& v2d = ask_up(c, m, 0, d);
-      LOs counts(c, nkeys);
+      counts[d] = LOs(c, nkeys);
       OSH_LAUNCH(live_counts, nkeys, c, keys2verts.data(), v2d.a2ab.data(), v2d.ab2b.data(),
-          dead[d].data(), counts.data());
-      k2p = offset_scan(c, counts, &nprods);
-      pv = LOs(c, nprods * (d + 1));
-      psrc = LOs(c, nprods);
+          dead[d].data(), counts[d].data());
+      k2p[d] = LOs(c, nkeys + 1);
+      jobs[d - 1] = {counts[d].data(), kScanLO, nkeys, k2p[d].data(), slot_ptr<LO>(c, d)};
+    }
+    offset_scans(c, jobs, dim);
+    c.sync();
+    for (int d = 1; d <= dim; ++d) nprods[d] = read_slot<LO>(c, d);
+  }
+  // every dimension's new numbering by one more scan launch
+  ModPlan plans[4];
+  for (int d = 0; d <= dim; ++d) modify_plan(c, m, d, 0, keys2verts, k2p[d], plans[d]);
+  modify_scan(c, plans, dim + 1);
+  for (int d = 0; d <= dim; ++d) {
+    LOs pv, psrc;
+    if (d > 0) {
+      Adj const& v2d = ask_up(c, m, 0, d);
+      pv = LOs(c, nprods[d] * (d + 1));
+      psrc = LOs(c, nprods[d]);
       LOs const& dv2v = verts_of(c, m, d);
       OSH_LAUNCH(coarsen_products, nkeys, c, d + 1, keys2verts.data(), keys2onto.data(),
           v2d.a2ab.data(), v2d.ab2b.data(), v2d.codes.data(), dead[d].data(), dv2v.data(),
-          o2n.data(), k2p.data(), pv.data(), psrc.data());
+          o2n.data(), k2p[d].data(), pv.data(), psrc.data());
     }
     ModOut mo;
-    modify_ents_adapt(c, m, nw, d, 0, keys2verts, k2p, nprods, pv, old_lows2new_lows, mo);
+    modify_build(c, m, nw, plans[d], keys2verts, k2p[d], nprods[d], pv, old_lows2new_lows, mo);
     if (d == 0) {
       o2n = mo.old2new;
       nw.coords = transfer<Real>(c, mo, dim, m.coords, nullptr);

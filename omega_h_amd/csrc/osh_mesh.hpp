@@ -57,9 +57,9 @@ struct DMesh {
   Bytes t2e_codes;
   Adj up[4][4];
   Adj star[2];
-  // side_elems[2s], side_elems[2s+1]: the lower and higher element on side s
-  // (equal on the boundary)
-  LOs side_elems;
+  // side_elems[s]: the number of elements on side s (high 32 bits) and the
+  // sum of their indices (low 32 bits)
+  DArray<unsigned long long> side_elems;
   void clear_caches();
 };
 
@@ -69,10 +69,11 @@ LOs const& verts_of(Ctx& c, DMesh& m, int d);
 LOs const& ask_down(Ctx& c, DMesh& m, int high, int low, Bytes const** codes);
 Adj const& ask_up(Ctx& c, DMesh& m, int low, int high);
 // the upward adjacency with only the rows of marked lows filled (the others
-// are empty); not cached
-Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks);
+// are empty); not cached.  With `marked`, also the list of marked lows
+// (collect_marked), from the same scan launch
+Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks, LOs* marked = nullptr);
 Adj const& ask_star(Ctx& c, DMesh& m, int d);
-LOs const& ask_side_elems(Ctx& c, DMesh& m);
+DArray<unsigned long long> const& ask_side_elems(Ctx& c, DMesh& m);
 // vertex lists of new entities of dimension high_dim >= 2 matched against the
 // already-built dimension high_dim-1 of m (src/Omega_h_adj.cpp:424-441)
 void reflect_down(Ctx& c, DMesh& m, int high_dim, LO nprods, LOs const& prod_verts,

@@ -304,20 +304,24 @@ __global__ void below_kernel(LO n, Real const* q, Real ceil, I8* out) {
 }
 
 // graph_reduce over the dual graph with max (src/Omega_h_graph.cpp:62-80 on
-// elements_across_sides): the max over the elements across each interior
-// side, -128 for an element with none
+// elements_across_sides, src/Omega_h_adj.cpp:591-657): the max over the
+// elements across each side that has two, -128 for an element with none; a
+// side with more than two elements (not a manifold mesh) is an error
 template <int NSPE>
-__global__ void dual_max_kernel(LO n, LO const* es2s, LO const* side_elems, I8 const* in,
-    I8* out) {
+__global__ void dual_max_kernel(LO n, LO const* es2s, unsigned long long const* side_elems,
+    I8 const* in, I8* out, int* error) {
   LO e = thread_index();
   if (e >= n) return;
   I8 r = -128;
 #pragma unroll
   for (int j = 0; j < NSPE; ++j) {
-    int2 se = reinterpret_cast<int2 const*>(side_elems)[es2s[e * NSPE + j]];
-    if (se.x != se.y) {
-      I8 x = in[se.x == e ? se.y : se.x];
+    unsigned long long w = side_elems[es2s[e * NSPE + j]];
+    unsigned count = unsigned(w >> 32);
+    if (count == 2) {
+      I8 x = in[LO(unsigned(w)) - e];
       r = (r < x) ? x : r;
+    } else if (count > 2) {
+      *error = kNonManifoldSide;
     }
   }
   out[e] = r;
@@ -340,57 +344,80 @@ LOs degrees_of(Ctx& c, LOs const& a2ab, LOs const& rows) {
   return out;
 }
 
-void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
-    LOs const& keys2kds, LOs const& keys2prods, LO nprods, LOs const& prod_verts,
-    LOs const& old_lows2new_lows, ModOut& out) {
+void modify_plan(Ctx& c, DMesh& old, int ent_dim, int key_dim, LOs const& keys2kds,
+    LOs const& keys2prods, ModPlan& plan) {
   LO nkeys = keys2kds.size();
   LO nold = old.nents[ent_dim];
+  plan.ent_dim = ent_dim;
+  plan.key_dim = key_dim;
+  plan.nold = nold;
+  plan.reps = LOs(c, nkeys);
   // same entities: those not adjacent to a key (collect_same); without a
   // `same` array every old entity is the same
-  Bytes same;
-  LOs same_offsets;
-  LOs counts;
-  LOs reps(c, nkeys);
   if (ent_dim == key_dim) {
-    same = Bytes(c, nold);
-    counts = LOs(c, nold);
-    OSH_LAUNCH(init_same_counts, nold, c, same.data(), counts.data());
+    plan.same = Bytes(c, nold);
+    plan.counts = LOs(c, nold);
+    OSH_LAUNCH(init_same_counts, nold, c, plan.same.data(), plan.counts.data());
     OSH_LAUNCH(reps_kernel, nkeys, c, keys2kds.data(), (LO const*)nullptr, (LO const*)nullptr,
-        0, keys2prods.data(), same.data(), reps.data(), counts.data(), c.error_word());
+        0, keys2prods.data(), plan.same.data(), plan.reps.data(), plan.counts.data(),
+        c.error_word());
   } else if (ent_dim > key_dim) {
     LO nkd = old.nents[key_dim];
     Bytes key_marks(c, nkd);
     LOs first_up(c, nkd);  // per key-dimension entity, the lowest entity above it
     OSH_LAUNCH(init_key_marks, nkd, c, key_marks.data(), first_up.data());
     OSH_LAUNCH(mark_image_kernel, nkeys, c, keys2kds.data(), key_marks.data());
-    same = Bytes(c, nold);
-    counts = LOs(c, nold);
+    plan.same = Bytes(c, nold);
+    plan.counts = LOs(c, nold);
     int deg = simplex_degree(ent_dim, key_dim);
     LOs const& hl2l = ask_down(c, old, ent_dim, key_dim, nullptr);
-    OSH_LAUNCH(mark_same_up_kernel, nold, c, deg, hl2l.data(), key_marks.data(), same.data(),
-        counts.data(), first_up.data());
+    OSH_LAUNCH(mark_same_up_kernel, nold, c, deg, hl2l.data(), key_marks.data(),
+        plan.same.data(), plan.counts.data(), first_up.data());
     OSH_LAUNCH(reps_kernel, nkeys, c, keys2kds.data(), first_up.data(), (LO const*)nullptr, 0,
-        keys2prods.data(), (I8*)nullptr, reps.data(), counts.data(), c.error_word());
+        keys2prods.data(), (I8*)nullptr, plan.reps.data(), plan.counts.data(), c.error_word());
   } else {
-    counts = fill_lo(c, nold, 1);
+    plan.counts = fill_lo(c, nold, 1);
     int deg = simplex_degree(key_dim, ent_dim);
     LOs const& down = ask_down(c, old, key_dim, ent_dim, nullptr);
     OSH_LAUNCH(reps_kernel, nkeys, c, keys2kds.data(), (LO const*)nullptr, down.data(), deg,
-        keys2prods.data(), (I8*)nullptr, reps.data(), counts.data(), c.error_word());
+        keys2prods.data(), (I8*)nullptr, plan.reps.data(), plan.counts.data(), c.error_word());
   }
-  // the offsets of the same entities and of all new entities in one launch
-  // and one sync
-  LOs offsets(c, nold + 1);
-  ScanJob jobs[2] = {{counts.data(), kScanLO, nold, offsets.data(), slot_ptr<LO>(c, 1)},
-      {same.data(), kScanNonzero, nold, nullptr, slot_ptr<LO>(c, 2)}};
-  if (same.exists()) {
-    same_offsets = LOs(c, nold + 1);
-    jobs[1].out = same_offsets.data();
+}
+
+void modify_scan(Ctx& c, ModPlan* plans, int nplans) {
+  OSH_CHECK(nplans >= 1 && 2 * nplans <= kMaxScanJobs && 2 * nplans < Ctx::kSlots,
+      "modify_scan: too many dimensions");
+  ScanJob jobs[kMaxScanJobs];
+  int njobs = 0;
+  for (int i = 0; i < nplans; ++i) {
+    ModPlan& p = plans[i];
+    p.offsets = LOs(c, p.nold + 1);
+    jobs[njobs++] = {p.counts.data(), kScanLO, p.nold, p.offsets.data(), slot_ptr<LO>(c, 2 * i + 1)};
+    if (p.same.exists()) {
+      p.same_offsets = LOs(c, p.nold + 1);
+      jobs[njobs++] = {p.same.data(), kScanNonzero, p.nold, p.same_offsets.data(),
+          slot_ptr<LO>(c, 2 * i + 2)};
+    }
   }
-  offset_scans(c, jobs, same.exists() ? 2 : 1);
+  offset_scans(c, jobs, njobs);
   c.sync();
-  out.nnew = read_slot<LO>(c, 1);
-  LO nsame = same.exists() ? read_slot<LO>(c, 2) : nold;
+  for (int i = 0; i < nplans; ++i) {
+    plans[i].nnew = read_slot<LO>(c, 2 * i + 1);
+    plans[i].nsame = plans[i].same.exists() ? read_slot<LO>(c, 2 * i + 2) : plans[i].nold;
+  }
+}
+
+void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const& keys2kds,
+    LOs const& keys2prods, LO nprods, LOs const& prod_verts, LOs const& old_lows2new_lows,
+    ModOut& out) {
+  int ent_dim = plan.ent_dim, key_dim = plan.key_dim;
+  LO nkeys = keys2kds.size();
+  LO nold = plan.nold;
+  LO nsame = plan.nsame;
+  Bytes const& same = plan.same;
+  LOs const& reps = plan.reps;
+  LOs const& offsets = plan.offsets;
+  out.nnew = plan.nnew;
   out.same2old = LOs(c, nsame);
   LOs order;
   if (key_dim > ent_dim) {
@@ -405,7 +432,7 @@ void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
   out.same2new = LOs(c, nsame);
   out.old2new = LOs(c, nold);
   OSH_LAUNCH(number_same, nold, c, same.exists() ? same.data() : (I8 const*)nullptr,
-      same.exists() ? same_offsets.data() : (LO const*)nullptr, offsets.data(),
+      same.exists() ? plan.same_offsets.data() : (LO const*)nullptr, offsets.data(),
       out.same2old.data(), out.same2new.data(), out.old2new.data());
   out.prods2new = LOs(c, nprods);
   OSH_LAUNCH(prod_numbers, nkeys, c, reps.data(), keys2kds.data(), offsets.data(),
@@ -437,6 +464,17 @@ void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
   nw.down[ent_dim] = down;
   nw.down_codes[ent_dim] = codes;
   nw.nents[ent_dim] = nnew;
+}
+
+
+void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
+    LOs const& keys2kds, LOs const& keys2prods, LO nprods, LOs const& prod_verts,
+    LOs const& old_lows2new_lows, ModOut& out) {
+  ModPlan plan;
+  modify_plan(c, old, ent_dim, key_dim, keys2kds, keys2prods, plan);
+  modify_scan(c, &plan, 1);
+  modify_build(c, old, nw, plan, keys2kds, keys2prods, nprods, prod_verts, old_lows2new_lows,
+      out);
 }
 
 template <class T>
@@ -558,15 +596,15 @@ Bytes mark_sliver_layers(Ctx& c, DMesh& m, Real qual_ceil, int nlayers) {
   Reals const& q = ask_qualities(c, m);
   Bytes marks(c, ne);
   OSH_LAUNCH(below_kernel, ne, c, q.data(), qual_ceil, marks.data());
-  LOs const& se = ask_side_elems(c, m);
+  auto const& se = ask_side_elems(c, m);
   for (int i = 0; i < nlayers; ++i) {
     Bytes next(c, ne);
     if (m.dim == 3)
       OSH_LAUNCH(dual_max_kernel<4>, ne, c, m.down[3].data(), se.data(), marks.data(),
-          next.data());
+          next.data(), c.error_word());
     else
       OSH_LAUNCH(dual_max_kernel<3>, ne, c, m.down[2].data(), se.data(), marks.data(),
-          next.data());
+          next.data(), c.error_word());
     marks = next;
   }
   return marks;

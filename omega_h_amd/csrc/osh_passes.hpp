@@ -31,6 +31,23 @@ struct ModOut {
   LOs old2new;
 };
 
+// modify_ents_adapt in three steps, so that a pass numbers all its
+// dimensions with one scan launch and one sync: modify_plan marks the same
+// entities, counts the new entities each old one stands for and finds the
+// representatives (no sync); modify_scan scans the plans of up to four
+// dimensions together; modify_build numbers and connects one dimension
+struct ModPlan {
+  int ent_dim = 0, key_dim = 0;
+  LO nold = 0, nnew = 0, nsame = 0;
+  Bytes same;
+  LOs counts, reps, offsets, same_offsets;
+};
+void modify_plan(Ctx& c, DMesh& old, int ent_dim, int key_dim, LOs const& keys2kds,
+    LOs const& keys2prods, ModPlan& plan);
+void modify_scan(Ctx& c, ModPlan* plans, int nplans);
+void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const& keys2kds,
+    LOs const& keys2prods, LO nprods, LOs const& prod_verts, LOs const& old_lows2new_lows,
+    ModOut& out);
 void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
     LOs const& keys2kds, LOs const& keys2prods, LO nprods, LOs const& prod_verts,
     LOs const& old_lows2new_lows, ModOut& out);

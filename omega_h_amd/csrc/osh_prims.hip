@@ -93,6 +93,7 @@ void Ctx::sync() {
     if (code == kScanTimeout) throw Error("scan: a tile waited too long for its predecessor");
     if (code == kScanTicket) throw Error("scan: tile tickets out of step");
     if (code == kKeyWithoutUp) throw Error("modify: a key entity has no adjacent entity above it");
+    if (code == kNonManifoldSide) throw Error("a side has more than two elements");
     throw Error("device error " + std::to_string(code));
   }
 }
@@ -326,7 +327,7 @@ unsigned long long* scan_status_words(Ctx& c, LO ntiles, unsigned* epoch) {
 }  // namespace
 
 void offset_scans(Ctx& c, ScanJob const* jobs, int njobs) {
-  OSH_CHECK(njobs >= 1 && njobs <= kMaxScanJobs, "offset_scans: 1 to 4 jobs per launch");
+  OSH_CHECK(njobs >= 1 && njobs <= kMaxScanJobs, "offset_scans: 1 to 8 jobs per launch");
   ScanBatch b{};
   unsigned tiles = 0;
   bool multi = false;
@@ -448,6 +449,10 @@ LOs offset_scan_bytes(Ctx& c, Bytes const& counts, LO* total) {
 }
 
 LOs collect_marked(Ctx& c, Bytes const& marks) { return collect_marked(c, marks, nullptr); }
+
+void scatter_marked_list(Ctx& c, Bytes const& marks, LOs const& offsets, LOs& out) {
+  OSH_LAUNCH(scatter_marked, marks.size(), c, marks.data(), offsets.data(), out.data());
+}
 
 LOs collect_marked(Ctx& c, Bytes const& marks, LOs* offsets_out) {
   LO n = marks.size();

@@ -223,34 +223,50 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
   nw.dim = dim;
   nw.metric_ncomps = nc;
   LOs keys2mid, o2n, old_lows2new_lows;
-  for (int d = 0; d <= dim; ++d) {
-    LOs k2p, pv, psrc;
-    Bytes psrc_dim;
-    LO nprods = nkeys;
-    if (d == 0) {
-      k2p = iota(c, nkeys + 1);
-    } else {
-      Adj const* pairs = (d > 1) ? &ask_up(c, m, 1, d) : nullptr;
-      Adj const* cuts = (d < dim) ? &ask_up(c, m, 1, d + 1) : nullptr;
-      LOs counts(c, nkeys);
+  // products per key of every dimension, numbered by one scan launch
+  LOs k2p[4];
+  LO nprods[4] = {nkeys, 0, 0, 0};
+  Adj const* pairs[4] = {nullptr, nullptr, nullptr, nullptr};
+  Adj const* cuts[4] = {nullptr, nullptr, nullptr, nullptr};
+  k2p[0] = iota(c, nkeys + 1);
+  {
+    LOs counts[4];
+    ScanJob jobs[3];
+    for (int d = 1; d <= dim; ++d) {
+      pairs[d] = (d > 1) ? &ask_up(c, m, 1, d) : nullptr;
+      cuts[d] = (d < dim) ? &ask_up(c, m, 1, d + 1) : nullptr;
+      counts[d] = LOs(c, nkeys);
       OSH_LAUNCH(prod_counts, nkeys, c, keys2edges.data(),
-          pairs ? pairs->a2ab.data() : (LO const*)nullptr,
-          cuts ? cuts->a2ab.data() : (LO const*)nullptr, counts.data());
-      k2p = offset_scan(c, counts, &nprods);
-      pv = LOs(c, nprods * (d + 1));
-      psrc = LOs(c, nprods);
-      psrc_dim = Bytes(c, nprods);
+          pairs[d] ? pairs[d]->a2ab.data() : (LO const*)nullptr,
+          cuts[d] ? cuts[d]->a2ab.data() : (LO const*)nullptr, counts[d].data());
+      k2p[d] = LOs(c, nkeys + 1);
+      jobs[d - 1] = {counts[d].data(), kScanLO, nkeys, k2p[d].data(), slot_ptr<LO>(c, d)};
+    }
+    offset_scans(c, jobs, dim);
+    c.sync();
+    for (int d = 1; d <= dim; ++d) nprods[d] = read_slot<LO>(c, d);
+  }
+  // every dimension's new numbering by one more scan launch
+  ModPlan plans[4];
+  for (int d = 0; d <= dim; ++d) modify_plan(c, m, d, 1, keys2edges, k2p[d], plans[d]);
+  modify_scan(c, plans, dim + 1);
+  for (int d = 0; d <= dim; ++d) {
+    LOs pv, psrc;
+    Bytes psrc_dim;
+    if (d > 0) {
+      Adj const* pr = pairs[d];
+      Adj const* ct = cuts[d];
+      pv = LOs(c, nprods[d] * (d + 1));
+      psrc = LOs(c, nprods[d]);
+      psrc_dim = Bytes(c, nprods[d]);
       LO const* pdv = (d > 1) ? verts_of(c, m, d).data() : nullptr;
-      LO const* cdv = cuts ? verts_of(c, m, d + 1).data() : nullptr;
+      LO const* cdv = ct ? verts_of(c, m, d + 1).data() : nullptr;
 #define OSH_RP(ED)                                                                        \
   OSH_LAUNCH(refine_products_kernel<ED>, nkeys, c, keys2edges.data(), keys2mid.data(),       \
-      o2n.data(), m.down[1].data(), k2p.data(), pairs ? pairs->a2ab.data() : (LO const*)nullptr, \
-      pairs ? pairs->ab2b.data() : (LO const*)nullptr,                                       \
-      pairs ? pairs->codes.data() : (I8 const*)nullptr, pdv,                                 \
-      cuts ? cuts->a2ab.data() : (LO const*)nullptr,                                         \
-      cuts ? cuts->ab2b.data() : (LO const*)nullptr,                                         \
-      cuts ? cuts->codes.data() : (I8 const*)nullptr, cdv, pv.data(), psrc.data(),           \
-      psrc_dim.data())
+      o2n.data(), m.down[1].data(), k2p[d].data(), pr ? pr->a2ab.data() : (LO const*)nullptr, \
+      pr ? pr->ab2b.data() : (LO const*)nullptr, pr ? pr->codes.data() : (I8 const*)nullptr,  \
+      pdv, ct ? ct->a2ab.data() : (LO const*)nullptr, ct ? ct->ab2b.data() : (LO const*)nullptr, \
+      ct ? ct->codes.data() : (I8 const*)nullptr, cdv, pv.data(), psrc.data(), psrc_dim.data())
       if (d == 1)
         OSH_RP(1);
       else if (d == 2)
@@ -260,7 +276,7 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
 #undef OSH_RP
     }
     ModOut mo;
-    modify_ents_adapt(c, m, nw, d, 1, keys2edges, k2p, nprods, pv, old_lows2new_lows, mo);
+    modify_build(c, m, nw, plans[d], keys2edges, k2p[d], nprods[d], pv, old_lows2new_lows, mo);
     if (d == 0) {
       keys2mid = mo.prods2new;
       o2n = mo.old2new;

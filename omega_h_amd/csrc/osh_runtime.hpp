@@ -180,7 +180,8 @@ enum DeviceError {
   kReflectDownNoMatch = 1,
   kScanTimeout = 2,
   kScanTicket = 3,
-  kKeyWithoutUp = 4
+  kKeyWithoutUp = 4,
+  kNonManifoldSide = 5
 };
 
 template <class T>
@@ -234,7 +235,7 @@ __device__ __forceinline__ LO xcd_thread_index() {
 // exclusive scans (n + 1 outputs, out[0] = 0) of LO values, I8 values or I8
 // nonzero flags; several share one launch through offset_scans
 enum ScanKind { kScanLO = 0, kScanBytes = 1, kScanNonzero = 2 };
-constexpr int kMaxScanJobs = 4;
+constexpr int kMaxScanJobs = 8;
 struct ScanJob {
   void const* in;
   int kind;
@@ -247,6 +248,8 @@ void offset_scans(Ctx& c, ScanJob const* jobs, int njobs);
 LOs offset_scan(Ctx& c, LOs const& counts, LO* total = nullptr);
 LOs offset_scan_bytes(Ctx& c, Bytes const& counts, LO* total = nullptr);
 LOs collect_marked(Ctx& c, Bytes const& marks);       // indices of nonzero
+// out[offsets[i]] = i for nonzero marks[i], with offsets from a nonzero scan
+void scatter_marked_list(Ctx& c, Bytes const& marks, LOs const& offsets, LOs& out);
 // the same, also returning the n+1 offsets of the nonzero marks
 LOs collect_marked(Ctx& c, Bytes const& marks, LOs* offsets_out);
 LOs iota(Ctx& c, LO n, LO start = 0, LO stride = 1);

@@ -442,10 +442,13 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
   nw.class_dim[0] = m.class_dim[0];
   LOs old_lows2new_lows = iota(c, nv);
   LO nkeys = keys2edges.size();
+  ModPlan plans[3];
+  for (int d = 1; d <= dim; ++d) modify_plan(c, m, d, 1, keys2edges, k2p[d], plans[d - 1]);
+  modify_scan(c, plans, dim);
   for (int d = 1; d <= dim; ++d) {
     ModOut mo;
-    modify_ents_adapt(c, m, nw, d, 1, keys2edges, k2p[d], nprods[d], pv[d], old_lows2new_lows,
-        mo);
+    modify_build(c, m, nw, plans[d - 1], keys2edges, k2p[d], nprods[d], pv[d],
+        old_lows2new_lows, mo);
     LOs src(c, mo.prods2new.size());
     OSH_LAUNCH(key_edge_sources, nkeys, c, keys2edges.data(), k2p[d].data(), src.data());
     transfer_class(c, m, nw, d, mo, src, 1, nullptr);
@@ -466,7 +469,9 @@ Bytes swap_candidates(Ctx& c, DMesh& m, AdaptOptions const& o) {
 bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   ask_lengths(c, m);
   Bytes cand = swap_candidates(c, m, o);
-  LOs cands = collect_marked(c, cand);
+  // every edge -> tet row this pass reads is a candidate's
+  LOs cands;
+  Adj e2t = marked_up(c, m, 1, 3, cand, &cands);
   if (cands.size() == 0) return false;
   LO ne = m.nents[1];
   int md = (m.metric_ncomps == 1) ? 1 : 3;
@@ -474,8 +479,6 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   Bytes configs(c, ne), keep(c, ne);
   Reals quals(c, ne);
   OSH_LAUNCH(swap_init, ne, c, configs.data(), keep.data(), quals.data());
-  // every edge -> tet row this pass reads is a candidate's
-  Adj e2t = marked_up(c, m, 1, 3, cand);
   LOs const& tv2v = verts_of(c, m, 3);
   Reals const& eq = ask_qualities(c, m);
   LO ncands = cands.size();
