@@ -136,21 +136,58 @@ __global__ void rank_uses(LO nhl, LO const* hl2l, I8 const* marks, LO const* a2a
   codes[b + rank] = c;
 }
 
+// rows in atomic order, for uses that do not depend on the order of a row
+__global__ void fill_uses_unsorted(LO nhl, LO const* hl2l, I8 const* marks, LO const* a2ab,
+    LO const* slot, int deg, I8 const* down_codes, LO* ab2b, I8* codes) {
+  LO hl = thread_index();
+  if (hl >= nhl) return;
+  LO l = hl2l[hl];
+  if (marks && !marks[l]) return;
+  LO pos = a2ab[l] + slot[hl];
+  int which_down = hl % deg;
+  ab2b[pos] = hl / deg;
+  codes[pos] = down_codes ? make_code(code_is_flipped(down_codes[hl]),
+                                code_rotation(down_codes[hl]), which_down)
+                          : make_code(0, 0, which_down);
+}
+
+// With `marks`, only marked lows get rows; with `marked`, the list of marked
+// lows comes from the same scan launch; without `sorted`, rows keep the
+// order of the atomics (one sweep less)
 Adj invert_adj(Ctx& c, LO nhigh, int deg, LO nlow, LOs const& hl2l, Bytes const* down_codes,
-    I8 const* marks = nullptr) {
+    Bytes const* marks = nullptr, LOs* marked = nullptr, bool sorted = true) {
   LO nhl = nhigh * deg;
+  I8 const* mk = marks ? marks->data() : nullptr;
   LOs counts = fill_lo(c, nlow, 0);
   LOs slot(c, nhl);
-  OSH_LAUNCH(count_lows, nhl, c, hl2l.data(), marks, counts.data(), slot.data());
+  OSH_LAUNCH(count_lows, nhl, c, hl2l.data(), mk, counts.data(), slot.data());
   Adj a;
-  a.a2ab = offset_scan(c, counts);
+  a.a2ab = LOs(c, nlow + 1);
+  if (marked) {
+    LOs marked_offsets(c, nlow + 1);
+    ScanJob jobs[2] = {{counts.data(), kScanLO, nlow, a.a2ab.data(), nullptr},
+        {mk, kScanNonzero, nlow, marked_offsets.data(), slot_ptr<LO>(c, 1)}};
+    offset_scans(c, jobs, 2);
+    c.sync();
+    *marked = LOs(c, read_slot<LO>(c, 1));
+    scatter_marked_list(c, *marks, marked_offsets, *marked);
+  } else {
+    ScanJob job{counts.data(), kScanLO, nlow, a.a2ab.data(), nullptr};
+    offset_scans(c, &job, 1);
+  }
   // with marks the rows hold at most nhl uses; the size stays an upper bound
-  LOs ab2hl(c, nhl);
-  OSH_LAUNCH(fill_uses, nhl, c, hl2l.data(), marks, a.a2ab.data(), slot.data(), ab2hl.data());
   a.ab2b = LOs(c, nhl);
   a.codes = Bytes(c, nhl);
-  OSH_LAUNCH(rank_uses, nhl, c, hl2l.data(), marks, a.a2ab.data(), ab2hl.data(), deg,
-      down_codes ? down_codes->data() : (I8 const*)nullptr, a.ab2b.data(), a.codes.data());
+  I8 const* dc = down_codes ? down_codes->data() : nullptr;
+  if (sorted) {
+    LOs ab2hl(c, nhl);
+    OSH_LAUNCH(fill_uses, nhl, c, hl2l.data(), mk, a.a2ab.data(), slot.data(), ab2hl.data());
+    OSH_LAUNCH(rank_uses, nhl, c, hl2l.data(), mk, a.a2ab.data(), ab2hl.data(), deg, dc,
+        a.ab2b.data(), a.codes.data());
+  } else {
+    OSH_LAUNCH(fill_uses_unsorted, nhl, c, hl2l.data(), mk, a.a2ab.data(), slot.data(), deg, dc,
+        a.ab2b.data(), a.codes.data());
+  }
   return a;
 }
 
@@ -350,37 +387,12 @@ LOs const& ask_down(Ctx& c, DMesh& m, int high, int low, Bytes const** codes) {
   return m.t2e;
 }
 
-Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks, LOs* marked) {
+Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks, LOs* marked,
+    bool sorted) {
   Bytes const* codes = nullptr;
   LOs const& down = ask_down(c, m, high, low, &codes);
-  if (!marked)
-    return invert_adj(c, m.nents[high], simplex_degree(high, low), m.nents[low], down, codes,
-        marks.data());
-  // the same sweeps as invert_adj, with the list of marked lows scanned in
-  // the same launch
-  int deg = simplex_degree(high, low);
-  LO nlow = m.nents[low];
-  LO nhl = m.nents[high] * deg;
-  LOs counts = fill_lo(c, nlow, 0);
-  LOs slot(c, nhl);
-  OSH_LAUNCH(count_lows, nhl, c, down.data(), marks.data(), counts.data(), slot.data());
-  Adj a;
-  a.a2ab = LOs(c, nlow + 1);
-  LOs marked_offsets(c, nlow + 1);
-  ScanJob jobs[2] = {{counts.data(), kScanLO, nlow, a.a2ab.data(), nullptr},
-      {marks.data(), kScanNonzero, nlow, marked_offsets.data(), slot_ptr<LO>(c, 1)}};
-  offset_scans(c, jobs, 2);
-  c.sync();
-  *marked = LOs(c, read_slot<LO>(c, 1));
-  scatter_marked_list(c, marks, marked_offsets, *marked);
-  LOs ab2hl(c, nhl);
-  OSH_LAUNCH(fill_uses, nhl, c, down.data(), marks.data(), a.a2ab.data(), slot.data(),
-      ab2hl.data());
-  a.ab2b = LOs(c, nhl);
-  a.codes = Bytes(c, nhl);
-  OSH_LAUNCH(rank_uses, nhl, c, down.data(), marks.data(), a.a2ab.data(), ab2hl.data(), deg,
-      codes ? codes->data() : (I8 const*)nullptr, a.ab2b.data(), a.codes.data());
-  return a;
+  return invert_adj(c, m.nents[high], simplex_degree(high, low), m.nents[low], down, codes,
+      &marks, marked, sorted);
 }
 
 Adj const& ask_up(Ctx& c, DMesh& m, int low, int high) {

@@ -56,11 +56,12 @@ __global__ void check_class_kernel(LO ne, LO const* ev2v, I8 const* vdim, I8 con
   codes[e] = code;
 }
 
-__global__ void check_exposure_kernel(LO ne, int cell_dim, LO const* e2c_a2ab,
-    LO const* e2c_ab2b, I8 const* e2c_codes, LO const* cs2s, I8 const* cdim, I8 const* sdim,
-    I8* codes) {
-  LO e = thread_index();
-  if (e >= ne) return;
+__global__ void check_exposure_kernel(LO ncands, LO const* cands, int cell_dim,
+    LO const* e2c_a2ab, LO const* e2c_ab2b, I8 const* e2c_codes, LO const* cs2s,
+    I8 const* cdim, I8 const* sdim, I8* codes) {
+  LO i = thread_index();
+  if (i >= ncands) return;
+  LO e = cands[i];
   I8 code = codes[e];
   if (code == kDont) return;
   int nccs = cell_dim + 1;
@@ -80,61 +81,81 @@ __global__ void check_exposure_kernel(LO ne, int cell_dim, LO const* e2c_a2ab,
   codes[e] = code;
 }
 
+// Each collapse side (candidate edge, direction) gets kLanes threads, which
+// split the collapsing vertex's row between them, so that passes with few
+// candidates still spread their work over the device.
+constexpr int kLanes = 8;
+
+// prevent_coarsen_overshoot: a side is vetoed when any new edge would reach
+// max_length (the order of the row does not matter)
 template <int D, int M>
-__global__ void overshoot_kernel(LO ne, Real max_length, LO const* ev2v, LO const* v2e_a2ab,
-    LO const* v2e_ab2b, I8 const* v2e_codes, Real const* coords, Real const* metric, I8* codes) {
-  LO e = thread_index();
-  if (e >= ne) return;
-  I8 code = codes[e];
-  if (code == kDont) return;
+__global__ void overshoot_lanes(LO nt, LO const* cands, LO const* ev2v, LO const* v2e_a2ab,
+    LO const* v2e_ab2b, I8 const* v2e_codes, Real const* coords, Real const* metric,
+    Real max_length, I8 const* codes, I8* veto) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO side = t / kLanes;
+  int lane = t % kLanes;
+  LO e = cands[side / 2];
+  int col = side % 2;
+  if (!collapses(codes[e], col)) return;
   constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
-  for (int col = 0; col < 2; ++col) {
-    if (!collapses(code, col)) continue;
-    LO v_col = ev2v[e * 2 + col];
-    LO v_onto = ev2v[e * 2 + (1 - col)];
-    for (LO ve = v2e_a2ab[v_col]; ve < v2e_a2ab[v_col + 1]; ++ve) {
-      LO e2 = v2e_ab2b[ve];
-      if (e2 == e) continue;
-      int eev_in = code_which_down(v2e_codes[ve]);
-      int eev_out = 1 - eev_in;
-      LO nv[2];
-      nv[eev_in] = v_onto;
-      nv[eev_out] = ev2v[e2 * 2 + eev_out];
-      double p0[D], p1[D], m0[nc], m1[nc];
-      for (int k = 0; k < D; ++k) {
-        p0[k] = coords[nv[0] * D + k];
-        p1[k] = coords[nv[1] * D + k];
-      }
-      for (int k = 0; k < nc; ++k) {
-        m0[k] = metric[nv[0] * nc + k];
-        m1[k] = metric[nv[1] * nc + k];
-      }
-      if (edge_length<D, M>(p0, p1, m0, m1) >= max_length) {
-        code = dont_collapse(code, col);
-        break;
-      }
+  LO v_col = ev2v[e * 2 + col];
+  LO v_onto = ev2v[e * 2 + (1 - col)];
+  for (LO ve = v2e_a2ab[v_col] + lane; ve < v2e_a2ab[v_col + 1]; ve += kLanes) {
+    LO e2 = v2e_ab2b[ve];
+    if (e2 == e) continue;
+    int eev_in = code_which_down(v2e_codes[ve]);
+    int eev_out = 1 - eev_in;
+    LO nv[2];
+    nv[eev_in] = v_onto;
+    nv[eev_out] = ev2v[e2 * 2 + eev_out];
+    double p0[D], p1[D], m0[nc], m1[nc];
+    for (int k = 0; k < D; ++k) {
+      p0[k] = coords[nv[0] * D + k];
+      p1[k] = coords[nv[1] * D + k];
+    }
+    for (int k = 0; k < nc; ++k) {
+      m0[k] = metric[nv[0] * nc + k];
+      m1[k] = metric[nv[1] * nc + k];
+    }
+    if (edge_length<D, M>(p0, p1, m0, m1) >= max_length) {
+      veto[side] = 1;
+      return;
     }
   }
+}
+
+__global__ void apply_vetoes(LO ncands, LO const* cands, I8 const* veto, I8* codes) {
+  LO i = thread_index();
+  if (i >= ncands) return;
+  LO e = cands[i];
+  I8 code = codes[e];
+  for (int col = 0; col < 2; ++col)
+    if (veto[i * 2 + col]) code = dont_collapse(code, col);
   codes[e] = code;
 }
 
+// coarsen_qualities: each lane folds the qualities of its share of the
+// collapsing vertex's cells (those that survive the collapse, with v_col
+// moved onto v_onto) into a partial minimum
 template <int D, int M>
-__global__ void __launch_bounds__(kBlock) coarsen_qualities_kernel(LO ne, LO const* ev2v,
-    LO const* cv2v, LO const* v2c_a2ab, LO const* v2c_ab2b, I8 const* v2c_codes,
-    Real const* coords, Real const* metric, I8 const* codes, Real* quals) {
-  LO e = thread_index();
-  if (e >= ne) return;
-  I8 code = codes[e];
-  quals[e * 2] = -1.0;
-  quals[e * 2 + 1] = -1.0;
-  if (code == kDont) return;
-  constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
-  for (int col = 0; col < 2; ++col) {
-    if (!collapses(code, col)) continue;
+__global__ void __launch_bounds__(kBlock) coarsen_quality_lanes(LO nt, LO const* cands,
+    LO const* ev2v, LO const* cv2v, LO const* v2c_a2ab, LO const* v2c_ab2b,
+    I8 const* v2c_codes, Real const* coords, Real const* metric, I8 const* codes,
+    Real* partial) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO side = t / kLanes;
+  int lane = t % kLanes;
+  LO e = cands[side / 2];
+  int col = side % 2;
+  double minqual = 1.0;
+  if (collapses(codes[e], col)) {
+    constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
     LO v_col = ev2v[e * 2 + col];
     LO v_onto = ev2v[e * 2 + (1 - col)];
-    double minqual = 1.0;
-    for (LO vc = v2c_a2ab[v_col]; vc < v2c_a2ab[v_col + 1]; ++vc) {
+    for (LO vc = v2c_a2ab[v_col] + lane; vc < v2c_a2ab[v_col + 1]; vc += kLanes) {
       LO c = v2c_ab2b[vc];
       int ccv_col = code_which_down(v2c_codes[vc]);
       LO cv[D + 1];
@@ -157,8 +178,48 @@ __global__ void __launch_bounds__(kBlock) coarsen_qualities_kernel(LO ne, LO con
       }
       minqual = math::min2(minqual, element_quality<D, M>(pp, ms));
     }
-    quals[e * 2 + col] = minqual;
   }
+  partial[t] = minqual;
+}
+
+// the minimum over the lanes: the minimum over the row, as the reference's
+// loop computes it (min2 never takes a NaN, and a zero's sign decides
+// nothing downstream: the filters, rails and independent set compare it only)
+__global__ void combine_quality_lanes(LO ncands, LO const* cands, I8 const* codes,
+    Real const* partial, Real* quals) {
+  LO i = thread_index();
+  if (i >= ncands) return;
+  LO e = cands[i];
+  I8 code = codes[e];
+  for (int col = 0; col < 2; ++col) {
+    double q = -1.0;
+    if (collapses(code, col)) {
+      q = 1.0;
+      for (int l = 0; l < kLanes; ++l) q = math::min2(q, partial[(i * 2 + col) * kLanes + l]);
+    }
+    quals[e * 2 + col] = q;
+  }
+}
+
+// the vertices that would collapse along some candidate edge
+__global__ void mark_collapsing_verts(LO ncands, LO const* cands, LO const* ev2v,
+    I8 const* codes, I8* vmarks) {
+  LO i = thread_index();
+  if (i >= ncands) return;
+  LO e = cands[i];
+  I8 code = codes[e];
+  for (int col = 0; col < 2; ++col)
+    if (collapses(code, col)) vmarks[ev2v[e * 2 + col]] = 1;
+}
+
+// the vertex star (verts_across_edges, src/Omega_h_adj.cpp:512-530) of the
+// vertices whose vertex -> edge rows are built
+__global__ void star_from_v2e(LO nv, LO const* a2ab, LO const* ab2b, I8 const* codes,
+    LO const* ev2v, LO* out) {
+  LO v = thread_index();
+  if (v >= nv) return;
+  for (LO ve = a2ab[v]; ve < a2ab[v + 1]; ++ve)
+    out[ve] = ev2v[ab2b[ve] * 2 + (1 - code_which_down(codes[ve]))];
 }
 
 __global__ void vert_min_quality(LO nv, LO const* a2ab, LO const* ab2b, Real const* q, Real* out) {
@@ -169,10 +230,11 @@ __global__ void vert_min_quality(LO nv, LO const* a2ab, LO const* ab2b, Real con
   out[v] = r;
 }
 
-__global__ void filter_quals_kernel(LO ne, LO const* ev2v, Real const* quals, Real qmin,
-    Real const* vold, I8* codes) {
-  LO e = thread_index();
-  if (e >= ne) return;
+__global__ void filter_quals_kernel(LO ncands, LO const* cands, LO const* ev2v,
+    Real const* quals, Real qmin, Real const* vold, I8* codes) {
+  LO i = thread_index();
+  if (i >= ncands) return;
+  LO e = cands[i];
   I8 code = codes[e];
   if (code == kDont) return;
   for (int col = 0; col < 2; ++col)
@@ -294,18 +356,31 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   LO const* ev2v = m.down[1].data();
   OSH_LAUNCH(check_class_kernel, ne, c, ev2v, m.class_dim[0].data(), m.class_dim[1].data(),
       codes.data());
-  if (max_i8(c, codes) <= kDont) return false;
-  for (int cell_dim = 2; cell_dim <= dim; ++cell_dim) {
-    Adj const& e2c = ask_up(c, m, 1, cell_dim);
-    OSH_LAUNCH(check_exposure_kernel, ne, c, cell_dim, e2c.a2ab.data(), e2c.ab2b.data(),
-        e2c.codes.data(), m.down[cell_dim].data(), m.class_dim[cell_dim].data(),
+  // the candidate edges: every later sweep runs over this list
+  LOs cands = collect_marked(c, codes);
+  LO ncands = cands.size();
+  if (ncands == 0) return false;
+  // upward rows only where the candidates look: the candidate edges' cells
+  // (read by order-free tests), the collapsing vertices' edges (ordered:
+  // choose_rails keeps the first best rail) and cells (order-free minimum)
+  Bytes vmarks = fill_i8(c, nv, 0);
+  OSH_LAUNCH(mark_collapsing_verts, ncands, c, cands.data(), ev2v, codes.data(), vmarks.data());
+  Adj e2c[4];
+  for (int cell_dim = 2; cell_dim <= dim; ++cell_dim)
+    e2c[cell_dim] = marked_up(c, m, 1, cell_dim, codes, nullptr, false);
+  for (int cell_dim = 2; cell_dim <= dim; ++cell_dim)
+    OSH_LAUNCH(check_exposure_kernel, ncands, c, cands.data(), cell_dim,
+        e2c[cell_dim].a2ab.data(), e2c[cell_dim].ab2b.data(), e2c[cell_dim].codes.data(),
+        m.down[cell_dim].data(), m.class_dim[cell_dim].data(),
         m.class_dim[cell_dim - 1].data(), codes.data());
-  }
   Real max_length = overshoot_allowed ? o.max_length_allowed : o.max_length_desired;
-  Adj const& v2e = ask_up(c, m, 0, 1);
-#define OSH_OV(D, M)                                                                    \
-  OSH_LAUNCH((overshoot_kernel<D, M>), ne, c, max_length, ev2v, v2e.a2ab.data(),           \
-      v2e.ab2b.data(), v2e.codes.data(), m.coords.data(), m.metric.data(), codes.data())
+  Adj v2e = marked_up(c, m, 0, 1, vmarks);
+  LO nlanes = ncands * 2 * kLanes;
+  Bytes veto = fill_i8(c, ncands * 2, 0);
+#define OSH_OV(D, M)                                                                     \
+  OSH_LAUNCH((overshoot_lanes<D, M>), nlanes, c, cands.data(), ev2v, v2e.a2ab.data(),       \
+      v2e.ab2b.data(), v2e.codes.data(), m.coords.data(), m.metric.data(), max_length,      \
+      codes.data(), veto.data())
   if (dim == 3 && md == 3)
     OSH_OV(3, 3);
   else if (dim == 3)
@@ -315,14 +390,17 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   else
     OSH_OV(2, 1);
 #undef OSH_OV
-  if (max_i8(c, codes) <= kDont) return false;
+  OSH_LAUNCH(apply_vetoes, ncands, c, cands.data(), veto.data(), codes.data());
+  // sides without a candidate left are skipped by every sweep below, and the
+  // check after the quality filter returns
   Reals quals(c, ne * 2);
   LOs const& cv2v = verts_of(c, m, dim);
-  Adj const& v2c = ask_up(c, m, 0, dim);
-#define OSH_CQ(D, M)                                                                    \
-  OSH_LAUNCH((coarsen_qualities_kernel<D, M>), ne, c, ev2v, cv2v.data(), v2c.a2ab.data(),  \
-      v2c.ab2b.data(), v2c.codes.data(), m.coords.data(), m.metric.data(), codes.data(),    \
-      quals.data())
+  Adj v2c = marked_up(c, m, 0, dim, vmarks, nullptr, false);
+  Reals partial(c, nlanes);
+#define OSH_CQ(D, M)                                                                     \
+  OSH_LAUNCH((coarsen_quality_lanes<D, M>), nlanes, c, cands.data(), ev2v, cv2v.data(),     \
+      v2c.a2ab.data(), v2c.ab2b.data(), v2c.codes.data(), m.coords.data(), m.metric.data(), \
+      codes.data(), partial.data())
   if (dim == 3 && md == 3)
     OSH_CQ(3, 3);
   else if (dim == 3)
@@ -332,14 +410,17 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   else
     OSH_CQ(2, 1);
 #undef OSH_CQ
+  OSH_LAUNCH(combine_quality_lanes, ncands, c, cands.data(), codes.data(), partial.data(),
+      quals.data());
   Reals vold;
   if (improve) {
+    // read only at collapsing vertices, whose rows are built
     vold = Reals(c, nv);
     OSH_LAUNCH(vert_min_quality, nv, c, v2c.a2ab.data(), v2c.ab2b.data(),
         ask_qualities(c, m).data(), vold.data());
   }
-  OSH_LAUNCH(filter_quals_kernel, ne, c, ev2v, quals.data(), o.min_quality_allowed,
-      improve ? vold.data() : (Real const*)nullptr, codes.data());
+  OSH_LAUNCH(filter_quals_kernel, ncands, c, cands.data(), ev2v, quals.data(),
+      o.min_quality_allowed, improve ? vold.data() : (Real const*)nullptr, codes.data());
   if (max_i8(c, codes) <= kDont) return false;
   Reals const& lengths = ask_lengths(c, m);
   Bytes vcand(c, nv);
@@ -347,7 +428,13 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   LOs vrail(c, nv);
   OSH_LAUNCH(choose_rails_kernel, nv, c, v2e.a2ab.data(), v2e.ab2b.data(), v2e.codes.data(),
       codes.data(), lengths.data(), quals.data(), vcand.data(), vqual.data(), vrail.data());
-  Bytes keys = find_indset(c, m, 0, vqual, vcand);
+  // the independent set reads the star rows of candidate vertices only
+  Adj star;
+  star.a2ab = v2e.a2ab;
+  star.ab2b = LOs(c, v2e.ab2b.size());
+  OSH_LAUNCH(star_from_v2e, nv, c, v2e.a2ab.data(), v2e.ab2b.data(), v2e.codes.data(), ev2v,
+      star.ab2b.data());
+  Bytes keys = find_indset(c, m, 0, vqual, vcand, nullptr, nullptr, &star);
   LOs keys2verts = collect_marked(c, keys);
   LO nkeys = keys2verts.size();
   LOs rails2edges(c, nkeys), col_dirs(c, nkeys), keys2onto(c, nkeys);
@@ -359,11 +446,14 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   OSH_LAUNCH(mark_rails, nkeys, c, rails2edges.data(), dead[1].data());
   for (int d = 2; d <= dim; ++d) dead[d] = fill_i8(c, m.nents[d], 0);
   for (int cell_dim = dim; cell_dim > 1; --cell_dim) {
-    Adj const& e2c = ask_up(c, m, 1, cell_dim);
     OSH_LAUNCH(mark_dead_kernel, nkeys, c, cell_dim, rails2edges.data(), col_dirs.data(),
-        e2c.a2ab.data(), e2c.ab2b.data(), e2c.codes.data(), m.down[cell_dim].data(),
-        dead[cell_dim].data(), dead[cell_dim - 1].data());
+        e2c[cell_dim].a2ab.data(), e2c[cell_dim].ab2b.data(), e2c[cell_dim].codes.data(),
+        m.down[cell_dim].data(), dead[cell_dim].data(), dead[cell_dim - 1].data());
   }
+  // the keys' rows of every dimension, in order: products follow them
+  Adj v2d[4];
+  v2d[1] = v2e;
+  for (int d = 2; d <= dim; ++d) v2d[d] = marked_up(c, m, 0, d, keys);
   DMesh nw;
   nw.dim = dim;
   nw.metric_ncomps = nc;
@@ -376,10 +466,9 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
     LOs counts[4];
     ScanJob jobs[3];
     for (int d = 1; d <= dim; ++d) {
-      Adj const& v2d = ask_up(c, m, 0, d);
       counts[d] = LOs(c, nkeys);
-      OSH_LAUNCH(live_counts, nkeys, c, keys2verts.data(), v2d.a2ab.data(), v2d.ab2b.data(),
-          dead[d].data(), counts[d].data());
+      OSH_LAUNCH(live_counts, nkeys, c, keys2verts.data(), v2d[d].a2ab.data(),
+          v2d[d].ab2b.data(), dead[d].data(), counts[d].data());
       k2p[d] = LOs(c, nkeys + 1);
       jobs[d - 1] = {counts[d].data(), kScanLO, nkeys, k2p[d].data(), slot_ptr<LO>(c, d)};
     }
@@ -394,13 +483,12 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   for (int d = 0; d <= dim; ++d) {
     LOs pv, psrc;
     if (d > 0) {
-      Adj const& v2d = ask_up(c, m, 0, d);
       pv = LOs(c, nprods[d] * (d + 1));
       psrc = LOs(c, nprods[d]);
       LOs const& dv2v = verts_of(c, m, d);
       OSH_LAUNCH(coarsen_products, nkeys, c, d + 1, keys2verts.data(), keys2onto.data(),
-          v2d.a2ab.data(), v2d.ab2b.data(), v2d.codes.data(), dead[d].data(), dv2v.data(),
-          o2n.data(), k2p[d].data(), pv.data(), psrc.data());
+          v2d[d].a2ab.data(), v2d[d].ab2b.data(), v2d[d].codes.data(), dead[d].data(),
+          dv2v.data(), o2n.data(), k2p[d].data(), pv.data(), psrc.data());
     }
     ModOut mo;
     modify_build(c, m, nw, plans[d], keys2verts, k2p[d], nprods[d], pv, old_lows2new_lows, mo);

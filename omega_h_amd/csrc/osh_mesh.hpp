@@ -69,9 +69,12 @@ LOs const& verts_of(Ctx& c, DMesh& m, int d);
 LOs const& ask_down(Ctx& c, DMesh& m, int high, int low, Bytes const** codes);
 Adj const& ask_up(Ctx& c, DMesh& m, int low, int high);
 // the upward adjacency with only the rows of marked lows filled (the others
-// are empty); not cached.  With `marked`, also the list of marked lows
-// (collect_marked), from the same scan launch
-Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks, LOs* marked = nullptr);
+// are empty; entries past a2ab[nlow] are unused); not cached.  With
+// `marked`, also the list of marked lows (collect_marked), from the same scan
+// launch.  Without `sorted`, a row's order is arbitrary, which suits uses
+// that do not depend on it (counts, any/all tests, min/max)
+Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks, LOs* marked = nullptr,
+    bool sorted = true);
 Adj const& ask_star(Ctx& c, DMesh& m, int d);
 DArray<unsigned long long> const& ask_side_elems(Ctx& c, DMesh& m);
 // vertex lists of new entities of dimension high_dim >= 2 matched against the

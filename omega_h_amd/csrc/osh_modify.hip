@@ -532,7 +532,7 @@ void transfer_measures(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo) {
 }
 
 Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& cands,
-    bool* any_cands, Adj const* edge_elems) {
+    bool* any_cands, Adj const* edge_elems, Adj const* star) {
   LO n = m.nents[d];
   if (d == m.dim) {  // src/Omega_h_indset.cpp:29
     if (any_cands) *any_cands = max_i8(c, cands) > 0;
@@ -545,7 +545,7 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
     e2c = edge_elems ? edge_elems : &ask_up(c, m, 1, m.dim);
     c2e = ask_down(c, m, m.dim, 1, nullptr).data();
   } else {
-    g = &ask_star(c, m, d);
+    g = star ? star : &ask_star(c, m, d);
   }
   Bytes a(c, n), b(c, n);
   OSH_LAUNCH(indset_init, n, c, cands.data(), a.data());

@@ -65,9 +65,10 @@ void transfer_measures(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo);
 
 // *any_cands (when given) tells whether any entity was a candidate; for
 // edges, `edge_elems` (when given) is an edge -> element adjacency holding at
-// least the rows of the candidate edges
+// least the rows of the candidate edges; for vertices, `star` (when given) is
+// the vertex star holding at least the rows of the candidates
 Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& cands,
-    bool* any_cands = nullptr, Adj const* edge_elems = nullptr);
+    bool* any_cands = nullptr, Adj const* edge_elems = nullptr, Adj const* star = nullptr);
 Bytes mark_down(Ctx& c, DMesh& m, int high, int low, Bytes const& marks);
 Bytes mark_sliver_layers(Ctx& c, DMesh& m, Real qual_ceil, int nlayers);
 LOs degrees_of(Ctx& c, LOs const& a2ab, LOs const& rows);
