@@ -491,7 +491,8 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
           dv2v.data(), o2n.data(), k2p[d].data(), pv.data(), psrc.data());
     }
     ModOut mo;
-    modify_build(c, m, nw, plans[d], keys2verts, k2p[d], nprods[d], pv, old_lows2new_lows, mo);
+    modify_build(c, m, nw, plans[d], keys2verts, k2p[d], nprods[d], pv, old_lows2new_lows, mo,
+        d > 0 ? &o2n : nullptr);
     if (d == 0) {
       o2n = mo.old2new;
       nw.coords = transfer<Real>(c, mo, dim, m.coords, nullptr);

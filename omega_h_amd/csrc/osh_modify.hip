@@ -59,9 +59,13 @@ __global__ void init_same_counts(LO n, I8* same, LO* counts) {
 
 // new numbers of the old entities in one sweep: same entities take their
 // offset, the rest -1 (src/Omega_h_modify.cpp:141-161); the same entities
-// are also listed in order (collect_same)
+// are also listed in order (collect_same) and, when `down` is given, write
+// their connectivity renumbered (modify_conn, :446-466) and, when `verts`
+// is given, their vertex list renumbered
 __global__ void number_same(LO n, I8 const* same, LO const* same_offsets, LO const* offsets,
-    LO* same2old, LO* same2new, LO* old2new) {
+    LO* same2old, LO* same2new, LO* old2new, int deg, LO const* old_down, I8 const* old_codes,
+    LO const* old_lows2new_lows, LO* down, I8* codes, int nvpe, LO const* old_verts,
+    LO const* verts_old2new, LO* verts) {
   LO o = thread_index();
   if (o >= n) return;
   if (same && !same[o]) {
@@ -69,9 +73,20 @@ __global__ void number_same(LO n, I8 const* same, LO const* same_offsets, LO con
     return;
   }
   LO s = same ? same_offsets[o] : o;
+  LO w = offsets[o];
   same2old[s] = o;
-  same2new[s] = offsets[o];
-  old2new[o] = offsets[o];
+  same2new[s] = w;
+  old2new[o] = w;
+  if (down)
+    for (int j = 0; j < deg; ++j) {
+      down[w * deg + j] = old_lows2new_lows[old_down[o * deg + j]];
+      if (codes) codes[w * deg + j] = old_codes[o * deg + j];
+    }
+  if (verts)
+    for (int k = 0; k < nvpe; ++k) {
+      LO v = old_verts[o * nvpe + k];
+      verts[w * nvpe + k] = verts_old2new ? verts_old2new[v] : v;
+    }
 }
 
 // each key's representative (get_mods2reps, src/Omega_h_modify.cpp:20-45):
@@ -124,34 +139,28 @@ __global__ void rep2md_order_kernel(LO nreps, LO const* a2ab, LO const* ab2b, I8
   }
 }
 
+// the products' new numbers, following their representative
+// (src/Omega_h_modify.cpp:141-161), with their connectivity (modify_conn),
+// their vertex lists and the entity each came from when those are asked for
 __global__ void prod_numbers(LO nkeys, LO const* reps, LO const* keys2kds, LO const* offsets,
-    LO const* order, LO const* keys2prods, LO* prods2new) {
+    LO const* order, LO const* keys2prods, LO* prods2new, int deg, LO const* pdown,
+    I8 const* pcodes, LO* down, I8* codes, int nvpe, LO const* pverts, LO* verts,
+    LO const* key_src, LO* src) {
   LO k = thread_index();
   if (k >= nkeys) return;
   LO offset = offsets[reps[k]];
   if (order) offset += order[keys2kds[k]] + 1;
-  for (LO p = keys2prods[k]; p < keys2prods[k + 1]; ++p) prods2new[p] = offset++;
-}
-
-__global__ void same_conn(LO nsame, int deg, LO const* same2old, LO const* same2new,
-    LO const* old_down, I8 const* old_codes, LO const* old_lows2new_lows, LO* down, I8* codes) {
-  LO s = thread_index();
-  if (s >= nsame) return;
-  LO o = same2old[s], n = same2new[s];
-  for (int j = 0; j < deg; ++j) {
-    down[n * deg + j] = old_lows2new_lows[old_down[o * deg + j]];
-    if (codes) codes[n * deg + j] = old_codes[o * deg + j];
-  }
-}
-
-__global__ void prod_conn(LO nprods, int deg, LO const* prods2new, LO const* pdown,
-    I8 const* pcodes, LO* down, I8* codes) {
-  LO p = thread_index();
-  if (p >= nprods) return;
-  LO n = prods2new[p];
-  for (int j = 0; j < deg; ++j) {
-    down[n * deg + j] = pdown[p * deg + j];
-    if (codes) codes[n * deg + j] = pcodes[p * deg + j];
+  for (LO p = keys2prods[k]; p < keys2prods[k + 1]; ++p) {
+    LO w = offset++;
+    prods2new[p] = w;
+    if (down)
+      for (int j = 0; j < deg; ++j) {
+        down[w * deg + j] = pdown[p * deg + j];
+        if (codes) codes[w * deg + j] = pcodes[p * deg + j];
+      }
+    if (verts)
+      for (int j = 0; j < nvpe; ++j) verts[w * nvpe + j] = pverts[p * nvpe + j];
+    if (src) src[p] = key_src[k];
   }
 }
 
@@ -409,16 +418,18 @@ void modify_scan(Ctx& c, ModPlan* plans, int nplans) {
 
 void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const& keys2kds,
     LOs const& keys2prods, LO nprods, LOs const& prod_verts, LOs const& old_lows2new_lows,
-    ModOut& out) {
+    ModOut& out, LOs const* verts_old2new, LOs* prod_src) {
   int ent_dim = plan.ent_dim, key_dim = plan.key_dim;
   LO nkeys = keys2kds.size();
   LO nold = plan.nold;
   LO nsame = plan.nsame;
   Bytes const& same = plan.same;
-  LOs const& reps = plan.reps;
-  LOs const& offsets = plan.offsets;
   out.nnew = plan.nnew;
+  LO nnew = out.nnew;
   out.same2old = LOs(c, nsame);
+  out.same2new = LOs(c, nsame);
+  out.old2new = LOs(c, nold);
+  out.prods2new = LOs(c, nprods);
   LOs order;
   if (key_dim > ent_dim) {
     LO nkd = old.nents[key_dim];
@@ -429,52 +440,51 @@ void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const&
     OSH_LAUNCH(rep2md_order_kernel, nold, c, up.a2ab.data(), up.ab2b.data(), up.codes.data(),
         md2mod.data(), keys2prods.data(), order.data());
   }
-  out.same2new = LOs(c, nsame);
-  out.old2new = LOs(c, nold);
-  OSH_LAUNCH(number_same, nold, c, same.exists() ? same.data() : (I8 const*)nullptr,
-      same.exists() ? plan.same_offsets.data() : (LO const*)nullptr, offsets.data(),
-      out.same2old.data(), out.same2new.data(), out.old2new.data());
-  out.prods2new = LOs(c, nprods);
-  OSH_LAUNCH(prod_numbers, nkeys, c, reps.data(), keys2kds.data(), offsets.data(),
-      order.exists() ? order.data() : (LO const*)nullptr, keys2prods.data(),
-      out.prods2new.data());
-  if (ent_dim == 0) {
-    nw.nents[0] = out.nnew;
-    return;
-  }
   int deg = ent_dim + 1;
-  LO nnew = out.nnew;
-  LOs down(c, nnew * deg);
-  Bytes codes;
   bool with_codes = ent_dim >= 2;
-  if (with_codes) codes = Bytes(c, nnew * deg);
-  OSH_LAUNCH(same_conn, nsame, c, deg, out.same2old.data(), out.same2new.data(),
-      old.down[ent_dim].data(), with_codes ? old.down_codes[ent_dim].data() : (I8 const*)nullptr,
-      old_lows2new_lows.data(), down.data(), with_codes ? codes.data() : (I8*)nullptr);
-  if (with_codes) {
-    LOs pdown;
-    Bytes pcodes;
-    reflect_down(c, nw, ent_dim, nprods, prod_verts, pdown, pcodes);
-    OSH_LAUNCH(prod_conn, nprods, c, deg, out.prods2new.data(), pdown.data(), pcodes.data(),
-        down.data(), codes.data());
-  } else {
-    OSH_LAUNCH(prod_conn, nprods, c, deg, out.prods2new.data(), prod_verts.data(),
-        (I8 const*)nullptr, down.data(), (I8*)nullptr);
+  LOs down;
+  Bytes codes;
+  LOs pdown;
+  Bytes pcodes;
+  // the vertex lists of new faces and elements come with the numbering
+  // (those of same entities renumbered, those of products as built), so the
+  // new mesh does not derive them again
+  LOs verts;
+  LO const* old_verts = nullptr;
+  if (ent_dim >= 1) {
+    down = LOs(c, nnew * deg);
+    if (with_codes) {
+      codes = Bytes(c, nnew * deg);
+      reflect_down(c, nw, ent_dim, nprods, prod_verts, pdown, pcodes);
+      verts = LOs(c, nnew * deg);
+      old_verts = verts_of(c, old, ent_dim).data();
+    } else {
+      pdown = prod_verts;
+    }
   }
+  OSH_LAUNCH(number_same, nold, c, same.exists() ? same.data() : (I8 const*)nullptr,
+      same.exists() ? plan.same_offsets.data() : (LO const*)nullptr, plan.offsets.data(),
+      out.same2old.data(), out.same2new.data(), out.old2new.data(), deg,
+      ent_dim >= 1 ? old.down[ent_dim].data() : (LO const*)nullptr,
+      with_codes ? old.down_codes[ent_dim].data() : (I8 const*)nullptr,
+      ent_dim >= 1 ? old_lows2new_lows.data() : (LO const*)nullptr,
+      ent_dim >= 1 ? down.data() : (LO*)nullptr, with_codes ? codes.data() : (I8*)nullptr, deg,
+      old_verts, verts_old2new ? verts_old2new->data() : (LO const*)nullptr,
+      verts.exists() ? verts.data() : (LO*)nullptr);
+  if (prod_src) *prod_src = LOs(c, nprods);
+  OSH_LAUNCH(prod_numbers, nkeys, c, plan.reps.data(), keys2kds.data(), plan.offsets.data(),
+      order.exists() ? order.data() : (LO const*)nullptr, keys2prods.data(),
+      out.prods2new.data(), deg, ent_dim >= 1 ? pdown.data() : (LO const*)nullptr,
+      with_codes ? pcodes.data() : (I8 const*)nullptr, ent_dim >= 1 ? down.data() : (LO*)nullptr,
+      with_codes ? codes.data() : (I8*)nullptr, deg,
+      verts.exists() ? prod_verts.data() : (LO const*)nullptr,
+      verts.exists() ? verts.data() : (LO*)nullptr, keys2kds.data(),
+      prod_src ? prod_src->data() : (LO*)nullptr);
+  nw.nents[ent_dim] = nnew;
+  if (ent_dim == 0) return;
   nw.down[ent_dim] = down;
   nw.down_codes[ent_dim] = codes;
-  nw.nents[ent_dim] = nnew;
-}
-
-
-void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
-    LOs const& keys2kds, LOs const& keys2prods, LO nprods, LOs const& prod_verts,
-    LOs const& old_lows2new_lows, ModOut& out) {
-  ModPlan plan;
-  modify_plan(c, old, ent_dim, key_dim, keys2kds, keys2prods, plan);
-  modify_scan(c, &plan, 1);
-  modify_build(c, old, nw, plan, keys2kds, keys2prods, nprods, prod_verts, old_lows2new_lows,
-      out);
+  if (verts.exists()) nw.verts[ent_dim] = verts;
 }
 
 template <class T>

@@ -31,11 +31,12 @@ struct ModOut {
   LOs old2new;
 };
 
-// modify_ents_adapt in three steps, so that a pass numbers all its
-// dimensions with one scan launch and one sync: modify_plan marks the same
-// entities, counts the new entities each old one stands for and finds the
-// representatives (no sync); modify_scan scans the plans of up to four
-// dimensions together; modify_build numbers and connects one dimension
+// modify_ents_adapt (src/Omega_h_modify.cpp:468-517) in three steps, so
+// that a pass numbers all its dimensions with one scan launch and one sync:
+// modify_plan marks the same entities, counts the new entities each old one
+// stands for and finds the representatives (no sync); modify_scan scans the
+// plans of up to four dimensions together; modify_build numbers and connects
+// one dimension
 struct ModPlan {
   int ent_dim = 0, key_dim = 0;
   LO nold = 0, nnew = 0, nsame = 0;
@@ -45,12 +46,11 @@ struct ModPlan {
 void modify_plan(Ctx& c, DMesh& old, int ent_dim, int key_dim, LOs const& keys2kds,
     LOs const& keys2prods, ModPlan& plan);
 void modify_scan(Ctx& c, ModPlan* plans, int nplans);
+// verts_old2new: the vertex renumbering (null when vertices keep their
+// numbers); prod_src, when given, receives each product's key entity
 void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const& keys2kds,
     LOs const& keys2prods, LO nprods, LOs const& prod_verts, LOs const& old_lows2new_lows,
-    ModOut& out);
-void modify_ents_adapt(Ctx& c, DMesh& old, DMesh& nw, int ent_dim, int key_dim,
-    LOs const& keys2kds, LOs const& keys2prods, LO nprods, LOs const& prod_verts,
-    LOs const& old_lows2new_lows, ModOut& out);
+    ModOut& out, LOs const* verts_old2new, LOs* prod_src = nullptr);
 
 // new tag data: same entities copy old values, products take prod values
 template <class T>

@@ -276,7 +276,8 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
 #undef OSH_RP
     }
     ModOut mo;
-    modify_build(c, m, nw, plans[d], keys2edges, k2p[d], nprods[d], pv, old_lows2new_lows, mo);
+    modify_build(c, m, nw, plans[d], keys2edges, k2p[d], nprods[d], pv, old_lows2new_lows, mo,
+        d > 0 ? &o2n : nullptr);
     if (d == 0) {
       keys2mid = mo.prods2new;
       o2n = mo.old2new;

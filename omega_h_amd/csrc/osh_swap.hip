@@ -325,11 +325,6 @@ __global__ void swap3d_products(LO nkeys, LO const* keys2edges, I8 const* config
   }
 }
 
-__global__ void key_edge_sources(LO nkeys, LO const* keys2edges, LO const* k2p, LO* src) {
-  LO k = thread_index();
-  if (k >= nkeys) return;
-  for (LO p = k2p[k]; p < k2p[k + 1]; ++p) src[p] = keys2edges[k];
-}
 
 __global__ void loops_kernel(LO n, LO const* edges, LO const* a2ab, LO const* ab2b,
     I8 const* codes, LO const* ev2v, LO const* tv2v, LO* eev2v, LO* verts, LO* sizes) {
@@ -441,16 +436,14 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
   nw.class_id[0] = m.class_id[0];
   nw.class_dim[0] = m.class_dim[0];
   LOs old_lows2new_lows = iota(c, nv);
-  LO nkeys = keys2edges.size();
   ModPlan plans[3];
   for (int d = 1; d <= dim; ++d) modify_plan(c, m, d, 1, keys2edges, k2p[d], plans[d - 1]);
   modify_scan(c, plans, dim);
   for (int d = 1; d <= dim; ++d) {
     ModOut mo;
+    LOs src;  // each product's key edge
     modify_build(c, m, nw, plans[d - 1], keys2edges, k2p[d], nprods[d], pv[d],
-        old_lows2new_lows, mo);
-    LOs src(c, mo.prods2new.size());
-    OSH_LAUNCH(key_edge_sources, nkeys, c, keys2edges.data(), k2p[d].data(), src.data());
+        old_lows2new_lows, mo, nullptr, &src);
     transfer_class(c, m, nw, d, mo, src, 1, nullptr);
     transfer_measures(c, m, nw, d, mo);
     old_lows2new_lows = mo.old2new;
