@@ -315,34 +315,49 @@ __global__ void mark_dead_kernel(LO nkeys, int cell_dim, LO const* rails2edges,
   }
 }
 
-__global__ void live_counts(LO nkeys, LO const* keys2verts, LO const* a2ab, LO const* ab2b,
-    I8 const* dead, LO* counts) {
-  LO k = thread_index();
-  if (k >= nkeys) return;
+// a key's products are the live entries of its vertex's row, in row order;
+// kLanes threads per key share the row (the counts of all dimensions in one
+// launch, added into zeroed counts)
+struct LiveRows {
+  LO const* a2ab[3];
+  LO const* ab2b[3];
+  I8 const* dead[3];
+  LO* counts[3];
+};
+
+__global__ void live_counts(LO nt, LO nkeys, LO const* keys2verts, LiveRows r) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO per = nkeys * kLanes;
+  int d = t / per;
+  LO k = (t % per) / kLanes;
+  int lane = t % kLanes;
   LO v = keys2verts[k];
   LO n = 0;
-  for (LO j = a2ab[v]; j < a2ab[v + 1]; ++j)
-    if (!dead[ab2b[j]]) ++n;
-  counts[k] = n;
+  for (LO j = r.a2ab[d][v] + lane; j < r.a2ab[d][v + 1]; j += kLanes) n += !r.dead[d][r.ab2b[d][j]];
+  if (n) atomicAdd(&r.counts[d][k], n);
 }
 
-__global__ void coarsen_products(LO nkeys, int nvd, LO const* keys2verts, LO const* keys2onto,
+__global__ void coarsen_products(LO nt, int nvd, LO const* keys2verts, LO const* keys2onto,
     LO const* a2ab, LO const* ab2b, I8 const* codes, I8 const* dead, LO const* dv2v,
     LO const* o2n, LO const* keys2prods, LO* pv, LO* psrc) {
-  LO k = thread_index();
-  if (k >= nkeys) return;
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO k = t / kLanes;
+  int lane = t % kLanes;
   LO v = keys2verts[k];
-  LO prod = keys2prods[k];
-  for (LO j = a2ab[v]; j < a2ab[v + 1]; ++j) {
+  LO b = a2ab[v], e = a2ab[v + 1];
+  for (LO j = b + lane; j < e; j += kLanes) {
     LO cell = ab2b[j];
     if (dead[cell]) continue;
+    LO prod = keys2prods[k];
+    for (LO i = b; i < j; ++i) prod += !dead[ab2b[i]];
     int ccv_col = code_which_down(codes[j]);
     for (int ccv = 0; ccv < nvd; ++ccv) {
       LO ov = (ccv == ccv_col) ? keys2onto[k] : dv2v[cell * nvd + ccv];
       pv[prod * nvd + ccv] = o2n[ov];
     }
     psrc[prod] = cell;
-    ++prod;
   }
 }
 
@@ -463,15 +478,18 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   LO nprods[4] = {0, 0, 0, 0};
   k2p[0] = fill_lo(c, nkeys + 1, 0);
   {
-    LOs counts[4];
+    LOs counts = fill_lo(c, dim * nkeys, 0);
+    LiveRows rows{};
     ScanJob jobs[3];
     for (int d = 1; d <= dim; ++d) {
-      counts[d] = LOs(c, nkeys);
-      OSH_LAUNCH(live_counts, nkeys, c, keys2verts.data(), v2d[d].a2ab.data(),
-          v2d[d].ab2b.data(), dead[d].data(), counts[d].data());
+      rows.a2ab[d - 1] = v2d[d].a2ab.data();
+      rows.ab2b[d - 1] = v2d[d].ab2b.data();
+      rows.dead[d - 1] = dead[d].data();
+      rows.counts[d - 1] = counts.data() + (d - 1) * nkeys;
       k2p[d] = LOs(c, nkeys + 1);
-      jobs[d - 1] = {counts[d].data(), kScanLO, nkeys, k2p[d].data(), slot_ptr<LO>(c, d)};
+      jobs[d - 1] = {rows.counts[d - 1], kScanLO, nkeys, k2p[d].data(), slot_ptr<LO>(c, d)};
     }
+    OSH_LAUNCH(live_counts, dim * nkeys * kLanes, c, nkeys, keys2verts.data(), rows);
     offset_scans(c, jobs, dim);
     c.sync();
     for (int d = 1; d <= dim; ++d) nprods[d] = read_slot<LO>(c, d);
@@ -486,7 +504,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
       pv = LOs(c, nprods[d] * (d + 1));
       psrc = LOs(c, nprods[d]);
       LOs const& dv2v = verts_of(c, m, d);
-      OSH_LAUNCH(coarsen_products, nkeys, c, d + 1, keys2verts.data(), keys2onto.data(),
+      OSH_LAUNCH(coarsen_products, nkeys * kLanes, c, d + 1, keys2verts.data(), keys2onto.data(),
           v2d[d].a2ab.data(), v2d[d].ab2b.data(), v2d[d].codes.data(), dead[d].data(),
           dv2v.data(), o2n.data(), k2p[d].data(), pv.data(), psrc.data());
     }

@@ -141,17 +141,26 @@ __global__ void rep2md_order_kernel(LO nreps, LO const* a2ab, LO const* ab2b, I8
 
 // the products' new numbers, following their representative
 // (src/Omega_h_modify.cpp:141-161), with their connectivity (modify_conn),
-// their vertex lists and the entity each came from when those are asked for
-__global__ void prod_numbers(LO nkeys, LO const* reps, LO const* keys2kds, LO const* offsets,
-    LO const* order, LO const* keys2prods, LO* prods2new, int deg, LO const* pdown,
-    I8 const* pcodes, LO* down, I8* codes, int nvpe, LO const* pverts, LO* verts,
-    LO const* key_src, LO* src) {
-  LO k = thread_index();
-  if (k >= nkeys) return;
-  LO offset = offsets[reps[k]];
-  if (order) offset += order[keys2kds[k]] + 1;
-  for (LO p = keys2prods[k]; p < keys2prods[k + 1]; ++p) {
-    LO w = offset++;
+// their vertex lists and the entity each came from when those are asked for.
+// A key's products take consecutive numbers, so kProdLanes threads per key
+// share its products without talking to each other.
+constexpr int kProdLanes = 8;
+
+__global__ void prod_numbers(LO nt, LO const* __restrict__ reps,
+    LO const* __restrict__ keys2kds, LO const* __restrict__ offsets, LO const* __restrict__ order,
+    LO const* __restrict__ keys2prods, LO* __restrict__ prods2new, int deg,
+    LO const* __restrict__ pdown, I8 const* __restrict__ pcodes, LO* __restrict__ down,
+    I8* __restrict__ codes, int nvpe, LO const* __restrict__ pverts, LO* __restrict__ verts,
+    LO const* __restrict__ key_src, LO* __restrict__ src) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO k = t / kProdLanes;
+  int lane = t % kProdLanes;
+  LO first = keys2prods[k];
+  LO base = offsets[reps[k]];
+  if (order) base += order[keys2kds[k]] + 1;
+  for (LO p = first + lane; p < keys2prods[k + 1]; p += kProdLanes) {
+    LO w = base + (p - first);
     prods2new[p] = w;
     if (down)
       for (int j = 0; j < deg; ++j) {
@@ -472,7 +481,8 @@ void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const&
       old_verts, verts_old2new ? verts_old2new->data() : (LO const*)nullptr,
       verts.exists() ? verts.data() : (LO*)nullptr);
   if (prod_src) *prod_src = LOs(c, nprods);
-  OSH_LAUNCH(prod_numbers, nkeys, c, plan.reps.data(), keys2kds.data(), plan.offsets.data(),
+  OSH_LAUNCH(prod_numbers, nkeys * kProdLanes, c, plan.reps.data(), keys2kds.data(),
+      plan.offsets.data(),
       order.exists() ? order.data() : (LO const*)nullptr, keys2prods.data(),
       out.prods2new.data(), deg, ent_dim >= 1 ? pdown.data() : (LO const*)nullptr,
       with_codes ? pcodes.data() : (I8 const*)nullptr, ent_dim >= 1 ? down.data() : (LO*)nullptr,
