@@ -496,7 +496,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   }
   // every dimension's new numbering by one more scan launch
   ModPlan plans[4];
-  for (int d = 0; d <= dim; ++d) modify_plan(c, m, d, 0, keys2verts, k2p[d], plans[d]);
+  modify_plans(c, m, 0, 0, dim, keys2verts, k2p, plans);
   modify_scan(c, plans, dim + 1);
   for (int d = 0; d <= dim; ++d) {
     LOs pv, psrc;

@@ -15,18 +15,6 @@ namespace osh {
 
 namespace {
 
-__global__ void mark_image_kernel(LO n, LO const* a2b, I8* marks) {
-  LO i = thread_index();
-  if (i < n) marks[a2b[i]] = 1;
-}
-
-__global__ void init_key_marks(LO n, I8* marks, LO* first_up) {
-  LO i = thread_index();
-  if (i >= n) return;
-  marks[i] = 0;
-  first_up[i] = 0x7fffffff;
-}
-
 // the same entities of dimension > key dimension: those with no key among
 // their lower entities, each counting one new entity; each key also receives
 // the lowest index among the entities above it, which is the first entry of
@@ -48,20 +36,12 @@ __global__ void mark_same_up_kernel(LO nh, int deg, LO const* hl2l, I8 const* lo
   counts[h] = r;
 }
 
-// entities of the key dimension: all same, each counting one, until the
-// keys are removed
-__global__ void init_same_counts(LO n, I8* same, LO* counts) {
-  LO i = thread_index();
-  if (i >= n) return;
-  same[i] = 1;
-  counts[i] = 1;
-}
-
 // new numbers of the old entities in one sweep: same entities take their
 // offset, the rest -1 (src/Omega_h_modify.cpp:141-161); the same entities
 // are also listed in order (collect_same) and, when `down` is given, write
-// their connectivity renumbered (modify_conn, :446-466) and, when `verts`
-// is given, their vertex list renumbered
+// their connectivity renumbered (modify_conn, :446-466; a null
+// old_lows2new_lows keeps the lower numbers) and, when `verts` is given,
+// their vertex list renumbered
 __global__ void number_same(LO n, I8 const* same, LO const* same_offsets, LO const* offsets,
     LO* same2old, LO* same2new, LO* old2new, int deg, LO const* old_down, I8 const* old_codes,
     LO const* old_lows2new_lows, LO* down, I8* codes, int nvpe, LO const* old_verts,
@@ -79,7 +59,8 @@ __global__ void number_same(LO n, I8 const* same, LO const* same_offsets, LO con
   old2new[o] = w;
   if (down)
     for (int j = 0; j < deg; ++j) {
-      down[w * deg + j] = old_lows2new_lows[old_down[o * deg + j]];
+      LO l = old_down[o * deg + j];
+      down[w * deg + j] = old_lows2new_lows ? old_lows2new_lows[l] : l;
       if (codes) codes[w * deg + j] = old_codes[o * deg + j];
     }
   if (verts)
@@ -89,34 +70,74 @@ __global__ void number_same(LO n, I8 const* same, LO const* same_offsets, LO con
     }
 }
 
-// each key's representative (get_mods2reps, src/Omega_h_modify.cpp:20-45):
-// the key itself, the lowest entity above it (first_up) or its first lower
-// entity (down); the representative counts the key's products as well
-// (get_rep_counts, :47-70).  A key's own entry was cleared from `same`
-// and its count replaced here when it represents itself.
-__global__ void reps_kernel(LO nkeys, LO const* keys2kds, LO const* first_up, LO const* down,
-    int deg, LO const* keys2prods, I8* same, LO* reps, LO* counts, int* error) {
+// One sweep over the key-dimension entities starts the plans of every
+// dimension of a pass: no key marked, no entity above found yet, and (when
+// the key dimension itself is rebuilt) every entity the same, counting one.
+struct KeyDimInit {
+  I8* key_marks;
+  LO* first_up[4];  // per dimension above the keys (null elsewhere)
+  I8* same;         // when the key dimension is rebuilt
+  LO* counts;
+};
+
+__global__ void init_key_dim(LO n, KeyDimInit a) {
+  LO i = thread_index();
+  if (i >= n) return;
+  if (a.key_marks) a.key_marks[i] = 0;
+  for (int d = 0; d < 4; ++d)
+    if (a.first_up[d]) a.first_up[d][i] = 0x7fffffff;
+  if (a.same) {
+    a.same[i] = 1;
+    a.counts[i] = 1;
+  }
+}
+
+// the keys: marked; when the key dimension is rebuilt, a key is its own
+// representative (get_mods2reps, src/Omega_h_modify.cpp:20-45), is no longer
+// the same and counts its products (get_rep_counts, :47-70)
+__global__ void mark_keys(LO nkeys, LO const* keys2kds, LO const* keys2prods, I8* key_marks,
+    I8* same, LO* counts, LO* reps) {
   LO k = thread_index();
   if (k >= nkeys) return;
   LO kd = keys2kds[k];
-  LO r = kd;
-  LO np = keys2prods[k + 1] - keys2prods[k];
-  if (first_up) {
-    r = first_up[kd];
+  if (key_marks) key_marks[kd] = 1;
+  if (same) {
+    same[kd] = 0;
+    counts[kd] = keys2prods[k + 1] - keys2prods[k];
+    reps[k] = kd;
+  }
+}
+
+// representatives of the keys for up to three dimensions in one launch: the
+// lowest entity above the key (first_up) or the key's first lower entity
+// (down); the representative counts the key's products
+struct RepsJob {
+  LO const* first_up[3];
+  LO const* down[3];
+  int deg[3];
+  LO const* keys2prods[3];
+  LO* reps[3];
+  LO* counts[3];
+};
+
+__global__ void reps_kernel(LO nt, LO nkeys, LO const* keys2kds, RepsJob j, int* error) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  int w = t / nkeys;
+  LO k = t % nkeys;
+  LO kd = keys2kds[k];
+  LO r;
+  if (j.first_up[w]) {
+    r = j.first_up[w][kd];
     if (r == 0x7fffffff) {  // keeps every index in range until the host sees it
       r = 0;
       *error = kKeyWithoutUp;
     }
-  } else if (down) {
-    r = down[kd * deg];
-  }
-  if (reps) reps[k] = r;
-  if (same) {  // the key is its own representative and no longer the same
-    same[kd] = 0;
-    counts[kd] = np;
   } else {
-    atomicAdd(&counts[r], np);
+    r = j.down[w][kd * j.deg[w]];
   }
+  j.reps[w][k] = r;
+  atomicAdd(&j.counts[w][r], j.keys2prods[w][k + 1] - j.keys2prods[w][k]);
 }
 
 __global__ void scatter_index(LO n, LO const* a2b, LO* b2a) {
@@ -231,28 +252,34 @@ __global__ void inherit_kernel(LO n, LO nsame, int nc, LO const* same2old, LO co
   for (int k = 0; k < nc; ++k) out[w * nc + k] = from[s * nc + k];
 }
 
-__global__ void indset_init(LO n, I8 const* cands, I8* state) {
-  LO i = thread_index();
-  if (i < n) state[i] = cands[i] ? 2 : 0;
+// one round of indset::find (src/Omega_h_indset.cpp:5-35).  In the first
+// round the states come straight from the candidate marks (2 for a
+// candidate, 0 otherwise), so no node is in the set yet.
+template <bool FIRST>
+__device__ __forceinline__ I8 indset_state(I8 const* old_state, LO x) {
+  I8 s = old_state[x];
+  return FIRST ? I8(s ? 2 : 0) : s;
 }
 
+template <bool FIRST>
 __global__ void indset_iteration(LO n, LO const* a2ab, LO const* ab2b, Real const* q,
     I8 const* old_state, I8* new_state) {
   LO v = thread_index();
   if (v >= n) return;
-  I8 s = old_state[v];
+  I8 s = indset_state<FIRST>(old_state, v);
   new_state[v] = s;
   if (s != 2) return;
   LO b = a2ab[v], e = a2ab[v + 1];
-  for (LO j = b; j < e; ++j)
-    if (old_state[ab2b[j]] == 1) {
-      new_state[v] = 0;
-      return;
-    }
+  if (!FIRST)
+    for (LO j = b; j < e; ++j)
+      if (old_state[ab2b[j]] == 1) {
+        new_state[v] = 0;
+        return;
+      }
   Real qv = q[v];
   for (LO j = b; j < e; ++j) {
     LO u = ab2b[j];
-    if (old_state[u] == 0) continue;
+    if (indset_state<FIRST>(old_state, u) == 0) continue;
     Real qu = q[u];
     bool beats = (qu != qv) ? (qu < qv) : (u < v);
     if (!beats) return;
@@ -264,33 +291,34 @@ __global__ void indset_iteration(LO n, LO const* a2ab, LO const* ab2b, Real cons
 // src/Omega_h_adj.cpp:532-589) is the set of other edges of the elements
 // around the edge: read through the edge -> element adjacency instead of a
 // built star (an edge met twice is checked twice, which changes nothing)
-template <int NEPC>
+template <int NEPC, bool FIRST>
 __global__ void indset_iteration_edges(LO n, LO const* e2c_a2ab, LO const* e2c_ab2b,
     LO const* c2e, Real const* q, I8 const* old_state, I8* new_state) {
   LO v = thread_index();
   if (v >= n) return;
-  I8 s = old_state[v];
+  I8 s = indset_state<FIRST>(old_state, v);
   new_state[v] = s;
   if (s != 2) return;
   LO b = e2c_a2ab[v], e = e2c_a2ab[v + 1];
-  for (LO j = b; j < e; ++j) {
-    LO const* ce = c2e + e2c_ab2b[j] * NEPC;
+  if (!FIRST)
+    for (LO j = b; j < e; ++j) {
+      LO const* ce = c2e + e2c_ab2b[j] * NEPC;
 #pragma unroll
-    for (int k = 0; k < NEPC; ++k) {
-      LO u = ce[k];
-      if (u != v && old_state[u] == 1) {
-        new_state[v] = 0;
-        return;
+      for (int k = 0; k < NEPC; ++k) {
+        LO u = ce[k];
+        if (u != v && old_state[u] == 1) {
+          new_state[v] = 0;
+          return;
+        }
       }
     }
-  }
   Real qv = q[v];
   for (LO j = b; j < e; ++j) {
     LO const* ce = c2e + e2c_ab2b[j] * NEPC;
 #pragma unroll
     for (int k = 0; k < NEPC; ++k) {
       LO u = ce[k];
-      if (u == v || old_state[u] == 0) continue;
+      if (u == v || indset_state<FIRST>(old_state, u) == 0) continue;
       Real qu = q[u];
       bool beats = (qu != qv) ? (qu < qv) : (u < v);
       if (!beats) return;
@@ -324,25 +352,34 @@ __global__ void below_kernel(LO n, Real const* q, Real ceil, I8* out) {
 // graph_reduce over the dual graph with max (src/Omega_h_graph.cpp:62-80 on
 // elements_across_sides, src/Omega_h_adj.cpp:591-657): the max over the
 // elements across each side that has two, -128 for an element with none; a
-// side with more than two elements (not a manifold mesh) is an error
+// side with more than two elements (not a manifold mesh) is an error.  The
+// input is `in`, or, when `q` is given, the sliver test q < ceil.
 template <int NSPE>
-__global__ void dual_max_kernel(LO n, LO const* es2s, unsigned long long const* side_elems,
-    I8 const* in, I8* out, int* error) {
-  LO e = thread_index();
-  if (e >= n) return;
+__device__ __forceinline__ I8 dual_max_at(LO e, LO const* es2s,
+    unsigned long long const* side_elems, I8 const* in, Real const* q, Real ceil, int* error) {
   I8 r = -128;
 #pragma unroll
   for (int j = 0; j < NSPE; ++j) {
     unsigned long long w = side_elems[es2s[e * NSPE + j]];
     unsigned count = unsigned(w >> 32);
     if (count == 2) {
-      I8 x = in[LO(unsigned(w)) - e];
+      LO o = LO(unsigned(w)) - e;
+      I8 x = q ? I8(q[o] < ceil) : in[o];
       r = (r < x) ? x : r;
     } else if (count > 2) {
       *error = kNonManifoldSide;
     }
   }
-  out[e] = r;
+  return r;
+}
+
+__global__ void dual_max_kernel(LO n, int nspe, LO const* es2s,
+    unsigned long long const* side_elems, I8 const* in, Real const* q, Real ceil, I8* out,
+    int* error) {
+  LO e = thread_index();
+  if (e >= n) return;
+  out[e] = nspe == 4 ? dual_max_at<4>(e, es2s, side_elems, in, q, ceil, error)
+                     : dual_max_at<3>(e, es2s, side_elems, in, q, ceil, error);
 }
 
 __global__ void degrees_kernel(LO n, LO const* a2ab, LO const* rows, LO* out) {
@@ -362,44 +399,73 @@ LOs degrees_of(Ctx& c, LOs const& a2ab, LOs const& rows) {
   return out;
 }
 
-void modify_plan(Ctx& c, DMesh& old, int ent_dim, int key_dim, LOs const& keys2kds,
-    LOs const& keys2prods, ModPlan& plan) {
+void modify_plans(Ctx& c, DMesh& old, int key_dim, int d0, int d1, LOs const& keys2kds,
+    LOs const* keys2prods, ModPlan* plans) {
   LO nkeys = keys2kds.size();
-  LO nold = old.nents[ent_dim];
-  plan.ent_dim = ent_dim;
-  plan.key_dim = key_dim;
-  plan.nold = nold;
-  plan.reps = LOs(c, nkeys);
-  // same entities: those not adjacent to a key (collect_same); without a
-  // `same` array every old entity is the same
-  if (ent_dim == key_dim) {
-    plan.same = Bytes(c, nold);
-    plan.counts = LOs(c, nold);
-    OSH_LAUNCH(init_same_counts, nold, c, plan.same.data(), plan.counts.data());
-    OSH_LAUNCH(reps_kernel, nkeys, c, keys2kds.data(), (LO const*)nullptr, (LO const*)nullptr,
-        0, keys2prods.data(), plan.same.data(), plan.reps.data(), plan.counts.data(),
-        c.error_word());
-  } else if (ent_dim > key_dim) {
-    LO nkd = old.nents[key_dim];
-    Bytes key_marks(c, nkd);
-    LOs first_up(c, nkd);  // per key-dimension entity, the lowest entity above it
-    OSH_LAUNCH(init_key_marks, nkd, c, key_marks.data(), first_up.data());
-    OSH_LAUNCH(mark_image_kernel, nkeys, c, keys2kds.data(), key_marks.data());
-    plan.same = Bytes(c, nold);
-    plan.counts = LOs(c, nold);
-    int deg = simplex_degree(ent_dim, key_dim);
-    LOs const& hl2l = ask_down(c, old, ent_dim, key_dim, nullptr);
-    OSH_LAUNCH(mark_same_up_kernel, nold, c, deg, hl2l.data(), key_marks.data(),
-        plan.same.data(), plan.counts.data(), first_up.data());
-    OSH_LAUNCH(reps_kernel, nkeys, c, keys2kds.data(), first_up.data(), (LO const*)nullptr, 0,
-        keys2prods.data(), (I8*)nullptr, plan.reps.data(), plan.counts.data(), c.error_word());
-  } else {
-    plan.counts = fill_lo(c, nold, 1);
-    int deg = simplex_degree(key_dim, ent_dim);
-    LOs const& down = ask_down(c, old, key_dim, ent_dim, nullptr);
-    OSH_LAUNCH(reps_kernel, nkeys, c, keys2kds.data(), (LO const*)nullptr, down.data(), deg,
-        keys2prods.data(), (I8*)nullptr, plan.reps.data(), plan.counts.data(), c.error_word());
+  LO nkd = old.nents[key_dim];
+  KeyDimInit init{};
+  Bytes key_marks;
+  LOs first_up[4];
+  bool above = false;
+  for (int d = d0; d <= d1; ++d) {
+    ModPlan& plan = plans[d - d0];
+    plan.ent_dim = d;
+    plan.key_dim = key_dim;
+    plan.nold = old.nents[d];
+    plan.reps = LOs(c, nkeys);
+    // same entities: those not adjacent to a key (collect_same); without a
+    // `same` array every old entity is the same
+    if (d >= key_dim) {
+      plan.same = Bytes(c, plan.nold);
+      plan.counts = LOs(c, plan.nold);
+    }
+    if (d > key_dim) {
+      first_up[d] = LOs(c, nkd);
+      init.first_up[d] = first_up[d].data();
+      above = true;
+    }
+    if (d == key_dim) {
+      init.same = plan.same.data();
+      init.counts = plan.counts.data();
+    }
   }
+  if (above) {
+    key_marks = Bytes(c, nkd);
+    init.key_marks = key_marks.data();
+  }
+  if (above || init.same) {
+    OSH_LAUNCH(init_key_dim, nkd, c, init);
+    ModPlan* kp = (key_dim >= d0 && key_dim <= d1) ? &plans[key_dim - d0] : nullptr;
+    OSH_LAUNCH(mark_keys, nkeys, c, keys2kds.data(),
+        kp ? keys2prods[key_dim].data() : (LO const*)nullptr, init.key_marks,
+        kp ? kp->same.data() : (I8*)nullptr, kp ? kp->counts.data() : (LO*)nullptr,
+        kp ? kp->reps.data() : (LO*)nullptr);
+  }
+  RepsJob jobs{};
+  int njobs = 0;
+  for (int d = d0; d <= d1; ++d) {
+    ModPlan& plan = plans[d - d0];
+    if (d > key_dim) {
+      int deg = simplex_degree(d, key_dim);
+      LOs const& hl2l = ask_down(c, old, d, key_dim, nullptr);
+      OSH_LAUNCH(mark_same_up_kernel, plan.nold, c, deg, hl2l.data(), key_marks.data(),
+          plan.same.data(), plan.counts.data(), first_up[d].data());
+      jobs.first_up[njobs] = first_up[d].data();
+    } else if (d < key_dim) {
+      plan.counts = fill_lo(c, plan.nold, 1);
+      jobs.deg[njobs] = simplex_degree(key_dim, d);
+      jobs.down[njobs] = ask_down(c, old, key_dim, d, nullptr).data();
+    } else {
+      continue;
+    }
+    jobs.keys2prods[njobs] = keys2prods[d].data();
+    jobs.reps[njobs] = plan.reps.data();
+    jobs.counts[njobs] = plan.counts.data();
+    ++njobs;
+  }
+  OSH_CHECK(njobs <= 3, "modify_plans: too many dimensions");
+  if (njobs) OSH_LAUNCH(reps_kernel, njobs * nkeys, c, nkeys, keys2kds.data(), jobs,
+      c.error_word());
 }
 
 void modify_scan(Ctx& c, ModPlan* plans, int nplans) {
@@ -568,7 +634,6 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
     g = star ? star : &ask_star(c, m, d);
   }
   Bytes a(c, n), b(c, n);
-  OSH_LAUNCH(indset_init, n, c, cands.data(), a.data());
   // rounds leave decided nodes as they are and take no node from the set, so
   // extra rounds after convergence change nothing: run two rounds before each
   // check; afterwards a state of 2 means undecided nodes remain, and a state
@@ -577,15 +642,25 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
   for (LO round = 0;;) {
     OSH_CHECK(round <= n + 1, "independent set does not converge");
     for (int k = 0; k < 2; ++k, ++round) {
-      if (g)
-        OSH_LAUNCH(indset_iteration, n, c, g->a2ab.data(), g->ab2b.data(), quality.data(),
-            a.data(), b.data());
-      else if (m.dim == 3)
-        OSH_LAUNCH(indset_iteration_edges<6>, n, c, e2c->a2ab.data(), e2c->ab2b.data(), c2e,
-            quality.data(), a.data(), b.data());
+      bool first = round == 0;
+      I8 const* in = first ? cands.data() : a.data();
+#define OSH_IS(F)                                                                         \
+  do {                                                                                    \
+    if (g)                                                                                \
+      OSH_LAUNCH(indset_iteration<F>, n, c, g->a2ab.data(), g->ab2b.data(), quality.data(), \
+          in, b.data());                                                                  \
+    else if (m.dim == 3)                                                                  \
+      OSH_LAUNCH((indset_iteration_edges<6, F>), n, c, e2c->a2ab.data(), e2c->ab2b.data(),  \
+          c2e, quality.data(), in, b.data());                                             \
+    else                                                                                  \
+      OSH_LAUNCH((indset_iteration_edges<3, F>), n, c, e2c->a2ab.data(), e2c->ab2b.data(),  \
+          c2e, quality.data(), in, b.data());                                             \
+  } while (0)
+      if (first)
+        OSH_IS(true);
       else
-        OSH_LAUNCH(indset_iteration_edges<3>, n, c, e2c->a2ab.data(), e2c->ab2b.data(), c2e,
-            quality.data(), a.data(), b.data());
+        OSH_IS(false);
+#undef OSH_IS
       std::swap(a, b);
     }
     I8 top = max_i8(c, a);
@@ -614,17 +689,18 @@ Bytes mark_down(Ctx& c, DMesh& m, int high, int low, Bytes const& marks) {
 Bytes mark_sliver_layers(Ctx& c, DMesh& m, Real qual_ceil, int nlayers) {
   LO ne = m.nents[m.dim];
   Reals const& q = ask_qualities(c, m);
-  Bytes marks(c, ne);
-  OSH_LAUNCH(below_kernel, ne, c, q.data(), qual_ceil, marks.data());
+  Bytes marks;
+  if (nlayers == 0) {
+    marks = Bytes(c, ne);
+    OSH_LAUNCH(below_kernel, ne, c, q.data(), qual_ceil, marks.data());
+    return marks;
+  }
   auto const& se = ask_side_elems(c, m);
+  // the first layer reads the sliver test itself
   for (int i = 0; i < nlayers; ++i) {
     Bytes next(c, ne);
-    if (m.dim == 3)
-      OSH_LAUNCH(dual_max_kernel<4>, ne, c, m.down[3].data(), se.data(), marks.data(),
-          next.data(), c.error_word());
-    else
-      OSH_LAUNCH(dual_max_kernel<3>, ne, c, m.down[2].data(), se.data(), marks.data(),
-          next.data(), c.error_word());
+    OSH_LAUNCH(dual_max_kernel, ne, c, m.dim + 1, m.down[m.dim].data(), se.data(), marks.data(),
+        i == 0 ? q.data() : (Real const*)nullptr, qual_ceil, next.data(), c.error_word());
     marks = next;
   }
   return marks;

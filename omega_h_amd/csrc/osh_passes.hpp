@@ -33,7 +33,7 @@ struct ModOut {
 
 // modify_ents_adapt (src/Omega_h_modify.cpp:468-517) in three steps, so
 // that a pass numbers all its dimensions with one scan launch and one sync:
-// modify_plan marks the same entities, counts the new entities each old one
+// modify_plans marks the same entities, counts the new entities each old one
 // stands for and finds the representatives (no sync); modify_scan scans the
 // plans of up to four dimensions together; modify_build numbers and connects
 // one dimension
@@ -43,8 +43,9 @@ struct ModPlan {
   Bytes same;
   LOs counts, reps, offsets, same_offsets;
 };
-void modify_plan(Ctx& c, DMesh& old, int ent_dim, int key_dim, LOs const& keys2kds,
-    LOs const& keys2prods, ModPlan& plan);
+// the plans of dimensions d0..d1 (keys2prods indexed by dimension)
+void modify_plans(Ctx& c, DMesh& old, int key_dim, int d0, int d1, LOs const& keys2kds,
+    LOs const* keys2prods, ModPlan* plans);
 void modify_scan(Ctx& c, ModPlan* plans, int nplans);
 // verts_old2new: the vertex renumbering (null when vertices keep their
 // numbers); prod_src, when given, receives each product's key entity

@@ -248,7 +248,7 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
   }
   // every dimension's new numbering by one more scan launch
   ModPlan plans[4];
-  for (int d = 0; d <= dim; ++d) modify_plan(c, m, d, 1, keys2edges, k2p[d], plans[d]);
+  modify_plans(c, m, 1, 0, dim, keys2edges, k2p, plans);
   modify_scan(c, plans, dim + 1);
   for (int d = 0; d <= dim; ++d) {
     LOs pv, psrc;

@@ -231,17 +231,16 @@ __global__ void swap3d_choose(LO ncands, LO const* cands, LO const* lsize, Real 
   }
 }
 
-__global__ void swap_init(LO ne, I8* configs, I8* keep, Real* quals) {
+// the interior edges among the marked ones (mark_by_class_dim), and the
+// swap choice arrays set to "none"
+__global__ void and_interior(LO ne, int dim, I8 const* edim, I8* marks, I8* configs, I8* keep,
+    Real* quals) {
   LO e = thread_index();
   if (e >= ne) return;
+  marks[e] = I8(marks[e] && (edim[e] == dim));
   if (configs) configs[e] = -1;
   keep[e] = 0;
   quals[e] = -1.0;
-}
-
-__global__ void and_interior(LO ne, int dim, I8 const* edim, I8* marks) {
-  LO e = thread_index();
-  if (e < ne) marks[e] = I8(marks[e] && (edim[e] == dim));
 }
 
 // per edge, the edges, faces and tets a key edge's swap produces (0 for
@@ -435,9 +434,9 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
   nw.itags = m.itags;
   nw.class_id[0] = m.class_id[0];
   nw.class_dim[0] = m.class_dim[0];
-  LOs old_lows2new_lows = iota(c, nv);
+  LOs old_lows2new_lows;  // vertices keep their numbers
   ModPlan plans[3];
-  for (int d = 1; d <= dim; ++d) modify_plan(c, m, d, 1, keys2edges, k2p[d], plans[d - 1]);
+  modify_plans(c, m, 1, 1, dim, keys2edges, k2p, plans);
   modify_scan(c, plans, dim);
   for (int d = 1; d <= dim; ++d) {
     ModOut mo;
@@ -451,17 +450,27 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
   m = nw;
 }
 
-Bytes swap_candidates(Ctx& c, DMesh& m, AdaptOptions const& o) {
+// sliver-layer elements' interior edges (src/Omega_h_swap.cpp:21-30); the
+// same launch sets the choice arrays
+Bytes swap_candidates(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes* configs, Bytes& keep,
+    Reals& quals) {
   int dim = m.dim;
+  LO ne = m.nents[1];
   Bytes elems = mark_sliver_layers(c, m, o.min_quality_desired, o.nsliver_layers);
   Bytes edges = mark_down(c, m, dim, 1, elems);
-  OSH_LAUNCH(and_interior, m.nents[1], c, dim, m.class_dim[1].data(), edges.data());
+  if (configs) *configs = Bytes(c, ne);
+  keep = Bytes(c, ne);
+  quals = Reals(c, ne);
+  OSH_LAUNCH(and_interior, ne, c, dim, m.class_dim[1].data(), edges.data(),
+      configs ? configs->data() : (I8*)nullptr, keep.data(), quals.data());
   return edges;
 }
 
 bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   ask_lengths(c, m);
-  Bytes cand = swap_candidates(c, m, o);
+  Bytes configs, keep;
+  Reals quals;
+  Bytes cand = swap_candidates(c, m, o, &configs, keep, quals);
   // every edge -> tet row this pass reads is a candidate's
   LOs cands;
   Adj e2t = marked_up(c, m, 1, 3, cand, &cands);
@@ -469,9 +478,6 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   LO ne = m.nents[1];
   int md = (m.metric_ncomps == 1) ? 1 : 3;
   Tables const* tb = device_tables(c);
-  Bytes configs(c, ne), keep(c, ne);
-  Reals quals(c, ne);
-  OSH_LAUNCH(swap_init, ne, c, configs.data(), keep.data(), quals.data());
   LOs const& tv2v = verts_of(c, m, 3);
   Reals const& eq = ask_qualities(c, m);
   LO ncands = cands.size();
@@ -541,14 +547,12 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
 
 bool swap_edges_2d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   ask_lengths(c, m);
-  Bytes cand = swap_candidates(c, m, o);
+  Bytes keep;
+  Reals quals;
+  Bytes cand = swap_candidates(c, m, o, nullptr, keep, quals);
   LOs cands = collect_marked(c, cand);
   if (cands.size() == 0) return false;
-  LO ne = m.nents[1];
   int md = (m.metric_ncomps == 1) ? 1 : 2;
-  Bytes keep(c, ne);
-  Reals quals(c, ne);
-  OSH_LAUNCH(swap_init, ne, c, (I8*)nullptr, keep.data(), quals.data());
   Adj const& e2t = ask_up(c, m, 1, 2);
   LOs const& tv2v = verts_of(c, m, 2);
   Reals const& eq = ask_qualities(c, m);
