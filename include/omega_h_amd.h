@@ -124,7 +124,7 @@ int osh_measure_qualities(osh_ctx* ctx, int dim, int metric_ncomps, int n,
     const int* elems, const int* ev2v, const double* coords,
     const double* metric, double* out);
 
-/* offset_scan(Read<I32>), src/Omega_h_int_scan.cpp:10-20, for njobs (1 to 8)
+/* offset_scan(Read<I32>), src/Omega_h_int_scan.cpp:10-20, for njobs (1 to 12)
  * arrays at once on device pointers: out[j] (n[j] + 1 ints) receives 0
  * followed by the running sums of in[j] (n[j] ints).  Asynchronous like
  * osh_measure_edges_metric. */

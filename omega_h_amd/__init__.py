@@ -192,7 +192,7 @@ class Context:
               self.lib)
 
     def offset_scans(self, in_ptrs, sizes, out_ptrs):
-        """offset_scan of up to eight int arrays on device pointers, in one launch on
+        """offset_scan of up to twelve int arrays on device pointers, in one launch on
         this context's stream: out[j] gets n[j] + 1 exclusive offsets."""
         k = len(in_ptrs)
         ins = (C.c_void_p * k)(*[p or None for p in in_ptrs])

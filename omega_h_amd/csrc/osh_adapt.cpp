@@ -13,13 +13,17 @@ namespace osh {
 
 namespace {
 
+constexpr int kQminSlot = Ctx::kSlots - 1;
+
 double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
       .count();
 }
 
+// *qmin receives the minimum element quality after the pass, reduced before
+// the pass's closing sync so that it costs no sync of its own
 bool run_pass(Ctx& c, DMesh& m, AdaptOptions const& o, int kind,
-    std::vector<PassRecord>* log) {
+    std::vector<PassRecord>* log, Real* qmin) {
   OSH_CHECK(!log || log->size() < 4096, "adapt exceeded 4096 passes");
   LO nin = m.nents[m.dim];
   c.sync();
@@ -33,8 +37,10 @@ bool run_pass(Ctx& c, DMesh& m, AdaptOptions const& o, int kind,
     r = swap_edges(c, m, o);
   else
     r = coarsen_slivers(c, m, o);
+  min_real_async(c, ask_qualities(c, m), kQminSlot);
   c.sync();
   double t1 = now();
+  *qmin = read_slot<Real>(c, kQminSlot);
   if (r && log) log->push_back({kind, nin, m.nents[m.dim], t1 - t0});
   return r;
 }
@@ -53,17 +59,18 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
       lmax <= o.max_length_desired)
     return false;
   bool did;
+  Real q = qmin;  // the minimum quality of the current mesh
   do {
     did = false;
-    if (o.should_refine && run_pass(c, m, o, 0, log)) did = true;
-    if (o.should_coarsen && run_pass(c, m, o, 1, log)) did = true;
+    if (o.should_refine && run_pass(c, m, o, 0, log, &q)) did = true;
+    if (o.should_coarsen && run_pass(c, m, o, 1, log, &q)) did = true;
   } while (did);
-  if (min_real(c, ask_qualities(c, m)) >= o.min_quality_desired) return true;
+  if (q >= o.min_quality_desired) return true;
   do {
-    if (o.should_swap && run_pass(c, m, o, 2, log)) continue;
-    if (o.should_coarsen_slivers && run_pass(c, m, o, 3, log)) continue;
+    if (o.should_swap && run_pass(c, m, o, 2, log, &q)) continue;
+    if (o.should_coarsen_slivers && run_pass(c, m, o, 3, log, &q)) continue;
     return true;
-  } while (min_real(c, ask_qualities(c, m)) < o.min_quality_desired);
+  } while (q < o.min_quality_desired);
   return true;
 }
 

@@ -449,7 +449,7 @@ int osh_offset_scans(osh_ctx* ctx, int njobs, const int* const* in, const int* n
     int* const* out) {
   return guarded([&] {
     OSH_CHECK(ctx && in && n && out, "null argument");
-    OSH_CHECK(njobs >= 1 && njobs <= osh::kMaxScanJobs, "1 to 8 scans per call");
+    OSH_CHECK(njobs >= 1 && njobs <= osh::kMaxScanJobs, "1 to 12 scans per call");
     osh::ScanJob jobs[osh::kMaxScanJobs];
     for (int j = 0; j < njobs; ++j) {
       OSH_CHECK(n[j] >= 0, "negative count");

@@ -140,6 +140,41 @@ __global__ void reps_kernel(LO nt, LO nkeys, LO const* keys2kds, RepsJob j, int*
   atomicAdd(&j.counts[w][r], j.keys2prods[w][k + 1] - j.keys2prods[w][k]);
 }
 
+// the same starts with the keys given as a mask over the key dimension and
+// their product counts per key-dimension entity: a key of the rebuilt key
+// dimension is no longer the same and counts its products; for the
+// dimensions above, each key's representative (kept per key-dimension
+// entity) counts the key's products
+struct MaskedReps {
+  LO const* first_up[3];
+  LO const* prod_counts[3];
+  LO* rep_of[3];
+  LO* counts[3];
+};
+
+__global__ void mark_keys_masked(LO n, I8 const* key_mask, LO const* prod_counts, I8* same,
+    LO* counts) {
+  LO i = thread_index();
+  if (i >= n || !key_mask[i]) return;
+  same[i] = 0;
+  counts[i] = prod_counts[i];
+}
+
+__global__ void reps_masked(LO nt, LO nkd, I8 const* key_mask, MaskedReps j, int* error) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  int w = t / nkd;
+  LO kd = t % nkd;
+  if (!key_mask[kd]) return;
+  LO r = j.first_up[w][kd];
+  if (r == 0x7fffffff) {  // keeps every index in range until the host sees it
+    r = 0;
+    *error = kKeyWithoutUp;
+  }
+  j.rep_of[w][kd] = r;
+  atomicAdd(&j.counts[w][r], j.prod_counts[w][kd]);
+}
+
 __global__ void scatter_index(LO n, LO const* a2b, LO* b2a) {
   LO i = thread_index();
   if (i < n) b2a[a2b[i]] = i;
@@ -468,27 +503,78 @@ void modify_plans(Ctx& c, DMesh& old, int key_dim, int d0, int d1, LOs const& ke
       c.error_word());
 }
 
-void modify_scan(Ctx& c, ModPlan* plans, int nplans) {
-  OSH_CHECK(nplans >= 1 && 2 * nplans <= kMaxScanJobs && 2 * nplans < Ctx::kSlots,
-      "modify_scan: too many dimensions");
-  ScanJob jobs[kMaxScanJobs];
+void modify_plans_masked(Ctx& c, DMesh& old, int key_dim, int d1, Bytes const& key_mask,
+    LO const* const* prod_counts, ModPlan* plans, LOs* rep_of) {
+  LO nkd = old.nents[key_dim];
+  KeyDimInit init{};
+  LOs first_up[4];
+  for (int d = key_dim; d <= d1; ++d) {
+    ModPlan& plan = plans[d - key_dim];
+    plan.ent_dim = d;
+    plan.key_dim = key_dim;
+    plan.nold = old.nents[d];
+    plan.same = Bytes(c, plan.nold);
+    plan.counts = LOs(c, plan.nold);
+    if (d > key_dim) {
+      first_up[d] = LOs(c, nkd);
+      init.first_up[d] = first_up[d].data();
+      rep_of[d] = LOs(c, nkd);
+    }
+  }
+  init.same = plans[0].same.data();
+  init.counts = plans[0].counts.data();
+  OSH_LAUNCH(init_key_dim, nkd, c, init);
+  OSH_LAUNCH(mark_keys_masked, nkd, c, key_mask.data(), prod_counts[key_dim],
+      plans[0].same.data(), plans[0].counts.data());
+  MaskedReps j{};
+  int nj = 0;
+  for (int d = key_dim + 1; d <= d1; ++d) {
+    ModPlan& plan = plans[d - key_dim];
+    LOs const& hl2l = ask_down(c, old, d, key_dim, nullptr);
+    OSH_LAUNCH(mark_same_up_kernel, plan.nold, c, simplex_degree(d, key_dim), hl2l.data(),
+        key_mask.data(), plan.same.data(), plan.counts.data(), first_up[d].data());
+    j.first_up[nj] = first_up[d].data();
+    j.prod_counts[nj] = prod_counts[d];
+    j.rep_of[nj] = rep_of[d].data();
+    j.counts[nj] = plan.counts.data();
+    ++nj;
+  }
+  OSH_CHECK(nj <= 3, "modify_plans_masked: too many dimensions");
+  if (nj) OSH_LAUNCH(reps_masked, nj * nkd, c, nkd, key_mask.data(), j, c.error_word());
+}
+
+int modify_scan_jobs(Ctx& c, ModPlan* plans, int nplans, ScanJob* jobs, int first_slot) {
   int njobs = 0;
   for (int i = 0; i < nplans; ++i) {
     ModPlan& p = plans[i];
     p.offsets = LOs(c, p.nold + 1);
-    jobs[njobs++] = {p.counts.data(), kScanLO, p.nold, p.offsets.data(), slot_ptr<LO>(c, 2 * i + 1)};
+    jobs[njobs++] = {p.counts.data(), kScanLO, p.nold, p.offsets.data(),
+        slot_ptr<LO>(c, first_slot + 2 * i)};
     if (p.same.exists()) {
       p.same_offsets = LOs(c, p.nold + 1);
       jobs[njobs++] = {p.same.data(), kScanNonzero, p.nold, p.same_offsets.data(),
-          slot_ptr<LO>(c, 2 * i + 2)};
+          slot_ptr<LO>(c, first_slot + 2 * i + 1)};
     }
   }
+  return njobs;
+}
+
+void modify_scan_totals(Ctx& c, ModPlan* plans, int nplans, int first_slot) {
+  for (int i = 0; i < nplans; ++i) {
+    plans[i].nnew = read_slot<LO>(c, first_slot + 2 * i);
+    plans[i].nsame = plans[i].same.exists() ? read_slot<LO>(c, first_slot + 2 * i + 1)
+                                             : plans[i].nold;
+  }
+}
+
+void modify_scan(Ctx& c, ModPlan* plans, int nplans) {
+  OSH_CHECK(nplans >= 1 && 2 * nplans <= kMaxScanJobs && 2 * nplans < Ctx::kSlots,
+      "modify_scan: too many dimensions");
+  ScanJob jobs[kMaxScanJobs];
+  int njobs = modify_scan_jobs(c, plans, nplans, jobs, 1);
   offset_scans(c, jobs, njobs);
   c.sync();
-  for (int i = 0; i < nplans; ++i) {
-    plans[i].nnew = read_slot<LO>(c, 2 * i + 1);
-    plans[i].nsame = plans[i].same.exists() ? read_slot<LO>(c, 2 * i + 2) : plans[i].nold;
-  }
+  modify_scan_totals(c, plans, nplans, 1);
 }
 
 void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const& keys2kds,

@@ -47,6 +47,18 @@ struct ModPlan {
 void modify_plans(Ctx& c, DMesh& old, int key_dim, int d0, int d1, LOs const& keys2kds,
     LOs const* keys2prods, ModPlan* plans);
 void modify_scan(Ctx& c, ModPlan* plans, int nplans);
+// the same plans for dimensions key_dim..d1 with the keys given as a mask
+// over the key dimension and their product counts per key-dimension entity
+// (prod_counts[d]); plans' reps stay empty and rep_of[d] (d > key_dim)
+// receives each key's representative by key-dimension entity, so that the
+// keys need not be listed before the scan
+void modify_plans_masked(Ctx& c, DMesh& old, int key_dim, int d1, Bytes const& key_mask,
+    LO const* const* prod_counts, ModPlan* plans, LOs* rep_of);
+// modify_scan in two halves, for callers that add the plans' scans to a
+// launch of their own: the jobs (totals in pinned slots first_slot...), then,
+// after the caller's sync, the totals
+int modify_scan_jobs(Ctx& c, ModPlan* plans, int nplans, ScanJob* jobs, int first_slot);
+void modify_scan_totals(Ctx& c, ModPlan* plans, int nplans, int first_slot);
 // verts_old2new: the vertex renumbering (null when vertices keep their
 // numbers); prod_src, when given, receives each product's key entity
 void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const& keys2kds,

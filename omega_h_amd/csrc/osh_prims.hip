@@ -327,7 +327,7 @@ unsigned long long* scan_status_words(Ctx& c, LO ntiles, unsigned* epoch) {
 }  // namespace
 
 void offset_scans(Ctx& c, ScanJob const* jobs, int njobs) {
-  OSH_CHECK(njobs >= 1 && njobs <= kMaxScanJobs, "offset_scans: 1 to 8 jobs per launch");
+  OSH_CHECK(njobs >= 1 && njobs <= kMaxScanJobs, "offset_scans: 1 to 12 jobs per launch");
   ScanBatch b{};
   unsigned tiles = 0;
   bool multi = false;
@@ -407,13 +407,17 @@ __global__ void __launch_bounds__(kBlock) reduce_blocks(LO n, T const* in, T* ou
   if (threadIdx.x == 0) out[blockIdx.x] = buf[0];
 }
 
+// the result lands in `result` (device-visible) once the stream reaches it
 template <class T, int Op>
-T reduce(Ctx& c, DArray<T> const& a, T init) {
+void reduce_to(Ctx& c, DArray<T> const& a, T init, T* result) {
   LO n = a.size();
-  if (n == 0) return init;
+  if (n == 0) {
+    hipLaunchKernelGGL((fill_kernel<T>), dim3(1), dim3(kBlock), 0, c.stream, 1, result, init);
+    OSH_HIP(hipGetLastError());
+    return;
+  }
   LO nb = LO(nblocks(n, kBlock * 4));
   if (nb > 1024) nb = 1024;
-  T* result = static_cast<T*>(c.pinned);
   if (nb == 1) {
     hipLaunchKernelGGL((reduce_blocks<T, Op>), dim3(1), dim3(kBlock), 0, c.stream, n, a.data(),
         result, init);
@@ -426,6 +430,12 @@ T reduce(Ctx& c, DArray<T> const& a, T init) {
         result, init);
   }
   OSH_HIP(hipGetLastError());
+}
+
+template <class T, int Op>
+T reduce(Ctx& c, DArray<T> const& a, T init) {
+  if (a.size() == 0) return init;
+  reduce_to<T, Op>(c, a, init, static_cast<T*>(c.pinned));
   return pinned_value<T>(c);
 }
 
@@ -491,6 +501,10 @@ Reals fill_real(Ctx& c, LO n, Real value) {
 
 I8 max_i8(Ctx& c, Bytes const& a) { return reduce<I8, kMax>(c, a, I8(-128)); }
 Real min_real(Ctx& c, Reals const& a) { return reduce<Real, kMin>(c, a, 1.79769313486231570815e+308); }
+void min_real_async(Ctx& c, Reals const& a, int slot) {
+  OSH_CHECK(slot >= 0 && slot < Ctx::kSlots, "pinned slot out of range");
+  reduce_to<Real, kMin>(c, a, 1.79769313486231570815e+308, slot_ptr<Real>(c, slot));
+}
 Real max_real(Ctx& c, Reals const& a) { return reduce<Real, kMax>(c, a, -1.79769313486231570815e+308); }
 LO sum_lo(Ctx& c, LOs const& a) { return reduce<LO, kSum>(c, a, 0); }
 

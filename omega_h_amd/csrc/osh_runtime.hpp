@@ -235,7 +235,7 @@ __device__ __forceinline__ LO xcd_thread_index() {
 // exclusive scans (n + 1 outputs, out[0] = 0) of LO values, I8 values or I8
 // nonzero flags; several share one launch through offset_scans
 enum ScanKind { kScanLO = 0, kScanBytes = 1, kScanNonzero = 2 };
-constexpr int kMaxScanJobs = 8;
+constexpr int kMaxScanJobs = 12;
 struct ScanJob {
   void const* in;
   int kind;
@@ -258,6 +258,8 @@ Bytes fill_i8(Ctx& c, LO n, I8 value);
 Reals fill_real(Ctx& c, LO n, Real value);
 I8 max_i8(Ctx& c, Bytes const& a);
 Real min_real(Ctx& c, Reals const& a);
+// the same without a sync: the minimum lands in pinned slot `slot`
+void min_real_async(Ctx& c, Reals const& a, int slot);
 Real max_real(Ctx& c, Reals const& a);
 LO sum_lo(Ctx& c, LOs const& a);
 // stable sort of n tuples of `width` int64 keys; returns the permutation

@@ -135,11 +135,12 @@ __device__ LO owner_of(LO i, LO n, LO const* off) {
 }
 
 template <int M>
-__global__ void swap3d_tri_quals(LO npairs, LO ncands, LO const* toff, LO const* lsize,
+__global__ void swap3d_tri_quals(LO nt, LO ncands, LO const* toff, LO const* lsize,
     LO const* leev, LO const* lverts, Real const* coords, Real const* metric, Tables const* tb,
     Real* tq) {
+  // launched over an upper bound; the real count is the offsets' total
   LO p = thread_index();
-  if (p >= npairs) return;
+  if (p >= nt || p >= toff[ncands]) return;
   LO cand = owner_of(p, ncands, toff);
   int ut = p - toff[cand];
   int n = lsize[cand];
@@ -157,10 +158,11 @@ __global__ void swap3d_tri_quals(LO npairs, LO ncands, LO const* toff, LO const*
 }
 
 template <int M>
-__global__ void swap3d_edge_lens(LO npairs, LO ncands, LO const* eoff, LO const* lsize,
+__global__ void swap3d_edge_lens(LO nt, LO ncands, LO const* eoff, LO const* lsize,
     LO const* lverts, Real const* coords, Real const* metric, Tables const* tb, Real* el) {
+  // launched over an upper bound; the real count is the offsets' total
   LO q = thread_index();
-  if (q >= npairs) return;
+  if (q >= nt || q >= eoff[ncands]) return;
   LO cand = owner_of(q, ncands, eoff);
   int ue = q - eoff[cand];
   int n = lsize[cand];
@@ -260,10 +262,11 @@ __global__ void swap3d_counts(LO nedges, I8 const* keys, LO const* e2t_a2ab, Tab
   nt[e] = 2 * npt;
 }
 
-// the key list and each key's product offsets, from the per-edge scans
+// the key list, each key's product offsets and its representatives in the
+// rebuild of faces and tets, from the per-edge scans and sweeps
 __global__ void swap3d_key_offsets(LO nedges, I8 const* keys, LO const* key_off,
-    LO const* eoff, LO const* foff, LO const* toff, LO* keys2edges, LO* k2e, LO* k2f,
-    LO* k2t) {
+    LO const* eoff, LO const* foff, LO const* toff, LO const* frep_of, LO const* trep_of,
+    LO* keys2edges, LO* k2e, LO* k2f, LO* k2t, LO* ereps, LO* freps, LO* treps) {
   LO e = thread_index();
   if (e >= nedges) return;
   if (e == 0) {
@@ -278,6 +281,9 @@ __global__ void swap3d_key_offsets(LO nedges, I8 const* keys, LO const* key_off,
   k2e[k] = eoff[e];
   k2f[k] = foff[e];
   k2t[k] = toff[e];
+  ereps[k] = e;
+  freps[k] = frep_of[e];
+  treps[k] = trep_of[e];
 }
 
 __global__ void swap3d_products(LO nkeys, LO const* keys2edges, I8 const* configs,
@@ -420,8 +426,9 @@ Tables const* device_tables(Ctx& c) {
       static_cast<DArray<unsigned char>*>(c.swap_tables.get())->data());
 }
 
+// `plans` (when given) are the rebuild plans of dimensions 1..dim, scanned
 void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO const* nprods,
-    LOs const* pv) {
+    LOs const* pv, ModPlan* plans = nullptr) {
   int dim = m.dim;
   LO nv = m.nents[0];
   DMesh nw;
@@ -435,9 +442,12 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
   nw.class_id[0] = m.class_id[0];
   nw.class_dim[0] = m.class_dim[0];
   LOs old_lows2new_lows;  // vertices keep their numbers
-  ModPlan plans[3];
-  modify_plans(c, m, 1, 1, dim, keys2edges, k2p, plans);
-  modify_scan(c, plans, dim);
+  ModPlan own[3];
+  if (!plans) {
+    plans = own;
+    modify_plans(c, m, 1, 1, dim, keys2edges, k2p, plans);
+    modify_scan(c, plans, dim);
+  }
   for (int d = 1; d <= dim; ++d) {
     ModOut mo;
     LOs src;  // each product's key edge
@@ -486,12 +496,13 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   OSH_LAUNCH(swap3d_loops, ncands, c, cands.data(), e2t.a2ab.data(), e2t.ab2b.data(),
       e2t.codes.data(), m.down[1].data(), tv2v.data(), tb, lsize.data(), leev.data(),
       lverts.data(), ntris.data(), nedges.data());
+  // the unique triangles and edges of all loops, sized by their upper bounds
+  // so that no sync waits for the totals
   LOs toff(c, ncands + 1), eoff(c, ncands + 1);
-  ScanJob loop_jobs[2] = {{ntris.data(), kScanLO, ncands, toff.data(), slot_ptr<LO>(c, 1)},
-      {nedges.data(), kScanLO, ncands, eoff.data(), slot_ptr<LO>(c, 2)}};
+  ScanJob loop_jobs[2] = {{ntris.data(), kScanLO, ncands, toff.data(), nullptr},
+      {nedges.data(), kScanLO, ncands, eoff.data(), nullptr}};
   offset_scans(c, loop_jobs, 2);
-  c.sync();
-  LO ntp = read_slot<LO>(c, 1), nep = read_slot<LO>(c, 2);
+  LO ntp = ncands * swap::kMaxUniqueTris, nep = ncands * swap::kMaxUniqueEdges;
   Reals tq(c, ntp), el(c, nep);
   if (md == 3) {
     OSH_LAUNCH(swap3d_tri_quals<3>, ntp, c, ncands, toff.data(), lsize.data(), leev.data(),
@@ -512,9 +523,11 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   bool any;
   Bytes keys = find_indset(c, m, 1, quals, keep, &any, &e2t);
   if (!any) return false;
-  // keys and product counts numbered by one scan launch and one sync
-  LOs counts[4], offs[4];
-  ScanJob jobs[4];
+  // per-edge product counts and the rebuild plans from the key mask, then
+  // one scan launch and one sync for the keys, their products and the new
+  // numbering of edges, faces and tets
+  LOs counts[4], offs[4], rep_of[4];
+  ScanJob jobs[kMaxScanJobs];
   offs[0] = LOs(c, ne + 1);
   jobs[0] = {keys.data(), kScanNonzero, ne, offs[0].data(), slot_ptr<LO>(c, 0)};
   for (int d = 1; d <= 3; ++d) {
@@ -524,8 +537,13 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   }
   OSH_LAUNCH(swap3d_counts, ne, c, keys.data(), e2t.a2ab.data(), tb, counts[1].data(),
       counts[2].data(), counts[3].data());
-  offset_scans(c, jobs, 4);
+  ModPlan plans[3];
+  LO const* pc[4] = {nullptr, counts[1].data(), counts[2].data(), counts[3].data()};
+  modify_plans_masked(c, m, 1, 3, keys, pc, plans, rep_of);
+  int njobs = 4 + modify_scan_jobs(c, plans, 3, jobs + 4, 4);
+  offset_scans(c, jobs, njobs);
   c.sync();
+  modify_scan_totals(c, plans, 3, 4);
   LO nkeys = read_slot<LO>(c, 0);
   LOs keys2edges(c, nkeys);
   LOs k2p[4], pv[4];
@@ -534,14 +552,16 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
     nprods[d] = read_slot<LO>(c, d);
     pv[d] = LOs(c, nprods[d] * (d + 1));
     k2p[d] = LOs(c, nkeys + 1);
+    plans[d - 1].reps = LOs(c, nkeys);
   }
   OSH_LAUNCH(swap3d_key_offsets, ne, c, keys.data(), offs[0].data(), offs[1].data(),
-      offs[2].data(), offs[3].data(), keys2edges.data(), k2p[1].data(), k2p[2].data(),
-      k2p[3].data());
+      offs[2].data(), offs[3].data(), rep_of[2].data(), rep_of[3].data(), keys2edges.data(),
+      k2p[1].data(), k2p[2].data(), k2p[3].data(), plans[0].reps.data(), plans[1].reps.data(),
+      plans[2].reps.data());
   OSH_LAUNCH(swap3d_products, nkeys, c, keys2edges.data(), configs.data(), e2t.a2ab.data(),
       e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), tb, k2p[1].data(),
       k2p[2].data(), k2p[3].data(), pv[1].data(), pv[2].data(), pv[3].data());
-  swap_rebuild(c, m, keys2edges, k2p, nprods, pv);
+  swap_rebuild(c, m, keys2edges, k2p, nprods, pv, plans);
   return true;
 }
 

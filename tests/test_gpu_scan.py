@@ -44,7 +44,8 @@ def test_single_scan(ctx, n):
 def test_batched_scans_in_one_launch(ctx):
     rng = np.random.default_rng(7)
     for sizes in ([5, 0, 4096, 1], [300_000, 17, 1_000_000, 4097], [2_500_000, 2_500_000],
-                  [0, 0, 0], [1000, 5000, 0, 70_000, 4096, 3, 250_000, 9000]):
+                  [0, 0, 0], [1000, 5000, 0, 70_000, 4096, 3, 250_000, 9000],
+                  [7, 300_000, 0, 4096, 50_000, 1, 200_000, 4097, 12, 100_000, 0, 65_536]):
         arrays = [rng.integers(0, 5, size=n, dtype=np.int32) for n in sizes]
         got = run_scans(ctx, arrays)
         for a, g in zip(arrays, got):
@@ -65,6 +66,6 @@ def test_bad_scan_arguments_are_refused(ctx):
     a = DeviceArray(np.ones(4, np.int32))
     o = DeviceArray(dtype=np.int32, n=5)
     with pytest.raises(A.OshError):
-        ctx.offset_scans([a.ptr] * 9, [4] * 9, [o.ptr] * 9)
+        ctx.offset_scans([a.ptr] * 13, [4] * 13, [o.ptr] * 13)
     with pytest.raises(A.OshError):
         ctx.offset_scans([a.ptr], [-1], [o.ptr])
