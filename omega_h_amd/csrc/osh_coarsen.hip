@@ -56,28 +56,47 @@ __global__ void check_class_kernel(LO ne, LO const* ev2v, I8 const* vdim, I8 con
   codes[e] = code;
 }
 
-__global__ void check_exposure_kernel(LO ncands, LO const* cands, int cell_dim,
-    LO const* e2c_a2ab, LO const* e2c_ab2b, I8 const* e2c_codes, LO const* cs2s,
-    I8 const* cdim, I8 const* sdim, I8* codes) {
+// check_collapse_exposure from the cells' side: every cell of dimension CD
+// looks at its own edges, and a collapse side whose cell's side opposite the
+// target vertex is classified differently from the cell is vetoed (the
+// order of the cells does not matter, so no edge -> cell rows are built)
+template <int CD>
+__device__ __forceinline__ int local_vertex(LO const* cv, LO v) {
+  int k = 0;
+#pragma unroll
+  for (int j = 1; j < CD + 1; ++j)
+    if (cv[j] == v) k = j;
+  return k;
+}
+
+template <int CD>
+__global__ void check_exposure_cells(LO ncells, LO const* c2e, LO const* cv2v, LO const* cs2s,
+    I8 const* cdim, I8 const* sdim, LO const* ev2v, I8 const* codes, I8* veto) {
+  LO c = thread_index();
+  if (c >= ncells) return;
+  constexpr int nepc = (CD == 3) ? 6 : 3;
+  LO const* cv = cv2v + c * (CD + 1);
+  for (int j = 0; j < nepc; ++j) {
+    LO e = c2e[c * nepc + j];
+    I8 code = codes[e];
+    if (code == kDont) continue;
+    for (int col = 0; col < 2; ++col) {
+      if (!collapses(code, col)) continue;
+      int ccv_onto = local_vertex<CD>(cv, ev2v[e * 2 + (1 - col)]);
+      int ccs_opp = opposite_template(CD, 0, ccv_onto);
+      if (sdim[cs2s[c * (CD + 1) + ccs_opp]] != cdim[c]) veto[e * 2 + col] = 1;
+    }
+  }
+}
+
+// the vetoed sides of the candidate edges leave their codes
+__global__ void apply_edge_vetoes(LO ncands, LO const* cands, I8 const* veto, I8* codes) {
   LO i = thread_index();
   if (i >= ncands) return;
   LO e = cands[i];
   I8 code = codes[e];
-  if (code == kDont) return;
-  int nccs = cell_dim + 1;
-  for (LO ec = e2c_a2ab[e]; ec < e2c_a2ab[e + 1]; ++ec) {
-    LO c = e2c_ab2b[ec];
-    int cce = code_which_down(e2c_codes[ec]);
-    int rot = code_rotation(e2c_codes[ec]);
-    int c_dim = cdim[c];
-    for (int col = 0; col < 2; ++col) {
-      if (!collapses(code, col)) continue;
-      int onto = 1 - col;
-      int ccv_onto = down_template(cell_dim, 1, cce, rot ^ onto);
-      int ccs_opp = opposite_template(cell_dim, 0, ccv_onto);
-      if (sdim[cs2s[c * nccs + ccs_opp]] != c_dim) code = dont_collapse(code, col);
-    }
-  }
+  for (int col = 0; col < 2; ++col)
+    if (veto[e * 2 + col]) code = dont_collapse(code, col);
   codes[e] = code;
 }
 
@@ -291,27 +310,31 @@ __global__ void find_rails_kernel(LO nkeys, LO const* keys2verts, LO const* vrai
   }
 }
 
-__global__ void mark_rails(LO nkeys, LO const* rails2edges, I8* dead_edges) {
-  LO k = thread_index();
-  if (k < nkeys) dead_edges[rails2edges[k]] = 1;
-}
-
-__global__ void mark_dead_kernel(LO nkeys, int cell_dim, LO const* rails2edges,
-    LO const* col_dirs, LO const* e2c_a2ab, LO const* e2c_ab2b, I8 const* e2c_codes,
-    LO const* cs2s, I8* dead_cells, I8* dead_sides) {
+// mark_dead_ents (src/Omega_h_coarsen_topology.cpp:15-60): the rails die,
+// and so do the cells around a rail and each such cell's side opposite the
+// target vertex; rail_col[e] is the collapsing end of rail e (-1 elsewhere)
+__global__ void mark_rails(LO nkeys, LO const* rails2edges, LO const* col_dirs, I8* dead_edges,
+    I8* rail_col) {
   LO k = thread_index();
   if (k >= nkeys) return;
   LO e = rails2edges[k];
-  int onto = 1 - col_dirs[k];
-  int nccs = cell_dim + 1;
-  for (LO ec = e2c_a2ab[e]; ec < e2c_a2ab[e + 1]; ++ec) {
-    LO c = e2c_ab2b[ec];
-    int cce = code_which_down(e2c_codes[ec]);
-    int rot = code_rotation(e2c_codes[ec]);
-    int ccv_onto = down_template(cell_dim, 1, cce, rot ^ onto);
-    int ccs_opp = opposite_template(cell_dim, 0, ccv_onto);
+  dead_edges[e] = 1;
+  rail_col[e] = I8(col_dirs[k]);
+}
+
+template <int CD>
+__global__ void mark_dead_cells(LO ncells, LO const* c2e, LO const* cv2v, LO const* cs2s,
+    LO const* ev2v, I8 const* rail_col, I8* dead_cells, I8* dead_sides) {
+  LO c = thread_index();
+  if (c >= ncells) return;
+  constexpr int nepc = (CD == 3) ? 6 : 3;
+  for (int j = 0; j < nepc; ++j) {
+    LO e = c2e[c * nepc + j];
+    int col = rail_col[e];
+    if (col < 0) continue;
+    int ccv_onto = local_vertex<CD>(cv2v + c * (CD + 1), ev2v[e * 2 + (1 - col)]);
     dead_cells[c] = 1;
-    dead_sides[cs2s[c * nccs + ccs_opp]] = 1;
+    dead_sides[cs2s[c * (CD + 1) + opposite_template(CD, 0, ccv_onto)]] = 1;
   }
 }
 
@@ -380,14 +403,18 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   // choose_rails keeps the first best rail) and cells (order-free minimum)
   Bytes vmarks = fill_i8(c, nv, 0);
   OSH_LAUNCH(mark_collapsing_verts, ncands, c, cands.data(), ev2v, codes.data(), vmarks.data());
-  Adj e2c[4];
-  for (int cell_dim = 2; cell_dim <= dim; ++cell_dim)
-    e2c[cell_dim] = marked_up(c, m, 1, cell_dim, codes, nullptr, false);
-  for (int cell_dim = 2; cell_dim <= dim; ++cell_dim)
-    OSH_LAUNCH(check_exposure_kernel, ncands, c, cands.data(), cell_dim,
-        e2c[cell_dim].a2ab.data(), e2c[cell_dim].ab2b.data(), e2c[cell_dim].codes.data(),
-        m.down[cell_dim].data(), m.class_dim[cell_dim].data(),
-        m.class_dim[cell_dim - 1].data(), codes.data());
+  Bytes eveto = fill_i8(c, 2 * ne, 0);
+  for (int cell_dim = 2; cell_dim <= dim; ++cell_dim) {
+    LO const* c2e = ask_down(c, m, cell_dim, 1, nullptr).data();
+    LO const* cv2v = verts_of(c, m, cell_dim).data();
+    if (cell_dim == 3)
+      OSH_LAUNCH(check_exposure_cells<3>, m.nents[3], c, c2e, cv2v, m.down[3].data(),
+          m.class_dim[3].data(), m.class_dim[2].data(), ev2v, codes.data(), eveto.data());
+    else
+      OSH_LAUNCH(check_exposure_cells<2>, m.nents[2], c, c2e, cv2v, m.down[2].data(),
+          m.class_dim[2].data(), m.class_dim[1].data(), ev2v, codes.data(), eveto.data());
+  }
+  OSH_LAUNCH(apply_edge_vetoes, ncands, c, cands.data(), eveto.data(), codes.data());
   Real max_length = overshoot_allowed ? o.max_length_allowed : o.max_length_desired;
   Adj v2e = marked_up(c, m, 0, 1, vmarks);
   LO nlanes = ncands * 2 * kLanes;
@@ -458,12 +485,19 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
       keys2onto.data());
   Bytes dead[4];
   dead[1] = fill_i8(c, ne, 0);
-  OSH_LAUNCH(mark_rails, nkeys, c, rails2edges.data(), dead[1].data());
+  Bytes rail_col = fill_i8(c, ne, -1);
+  OSH_LAUNCH(mark_rails, nkeys, c, rails2edges.data(), col_dirs.data(), dead[1].data(),
+      rail_col.data());
   for (int d = 2; d <= dim; ++d) dead[d] = fill_i8(c, m.nents[d], 0);
   for (int cell_dim = dim; cell_dim > 1; --cell_dim) {
-    OSH_LAUNCH(mark_dead_kernel, nkeys, c, cell_dim, rails2edges.data(), col_dirs.data(),
-        e2c[cell_dim].a2ab.data(), e2c[cell_dim].ab2b.data(), e2c[cell_dim].codes.data(),
-        m.down[cell_dim].data(), dead[cell_dim].data(), dead[cell_dim - 1].data());
+    LO const* c2e = ask_down(c, m, cell_dim, 1, nullptr).data();
+    LO const* cv2v = verts_of(c, m, cell_dim).data();
+    if (cell_dim == 3)
+      OSH_LAUNCH(mark_dead_cells<3>, m.nents[3], c, c2e, cv2v, m.down[3].data(), ev2v,
+          rail_col.data(), dead[3].data(), dead[2].data());
+    else
+      OSH_LAUNCH(mark_dead_cells<2>, m.nents[2], c, c2e, cv2v, m.down[2].data(), ev2v,
+          rail_col.data(), dead[2].data(), dead[1].data());
   }
   // the keys' rows of every dimension, in order: products follow them
   Adj v2d[4];
