@@ -155,7 +155,8 @@ __global__ void fill_uses_unsorted(LO nhl, LO const* hl2l, I8 const* marks, LO c
 // lows comes from the same scan launch; without `sorted`, rows keep the
 // order of the atomics (one sweep less)
 Adj invert_adj(Ctx& c, LO nhigh, int deg, LO nlow, LOs const& hl2l, Bytes const* down_codes,
-    Bytes const* marks = nullptr, LOs* marked = nullptr, bool sorted = true) {
+    Bytes const* marks = nullptr, LOs* marked = nullptr, bool sorted = true,
+    LOs* marked_index = nullptr) {
   LO nhl = nhigh * deg;
   I8 const* mk = marks ? marks->data() : nullptr;
   LOs counts = fill_lo(c, nlow, 0);
@@ -171,6 +172,7 @@ Adj invert_adj(Ctx& c, LO nhigh, int deg, LO nlow, LOs const& hl2l, Bytes const*
     c.sync();
     *marked = LOs(c, read_slot<LO>(c, 1));
     scatter_marked_list(c, *marks, marked_offsets, *marked);
+    if (marked_index) *marked_index = marked_offsets;
   } else {
     ScanJob job{counts.data(), kScanLO, nlow, a.a2ab.data(), nullptr};
     offset_scans(c, &job, 1);
@@ -205,11 +207,13 @@ __device__ __forceinline__ LO min_vertex(LO const* v) {
   return m;
 }
 
-// counts[v] = 1 marks v; a marked row's final slot stays unused
+// counts[v] = 1 marks v; a marked row's final slot stays unused; each use's
+// match count starts at zero
 template <int HIGH>
-__global__ void mark_use_min_verts(LO nuses, LO const* prod_verts, LO* counts) {
+__global__ void mark_use_min_verts(LO nuses, LO const* prod_verts, LO* counts, LO* found) {
   LO u = thread_index();
   if (u >= nuses) return;
+  found[u] = 0;
   constexpr int LOW = HIGH - 1;
   LO h = u / (HIGH + 1);
   int which = u % (HIGH + 1);
@@ -242,12 +246,19 @@ __global__ void fill_min_rows(LO nl, LO const* lv2v, LO const* a2ab, LO const* s
   rows[a2ab[min_vertex<NVL>(v)] + s] = make_int4(l, v[0], v[1], v[2]);
 }
 
+// kMatchLanes threads per use share its row; a lane that finds a match
+// writes it and counts it in found[u], which the consumer checks (a use needs
+// exactly one match)
+constexpr int kMatchLanes = 4;
+
 template <int HIGH>
-__global__ void match_uses(LO nuses, LO const* __restrict__ prod_verts,
+__global__ void match_uses(LO nt, LO const* __restrict__ prod_verts,
     LO const* __restrict__ a2ab, int4 const* __restrict__ rows, LO* __restrict__ out,
-    I8* __restrict__ out_codes, int* error) {
-  LO u = thread_index();
-  if (u >= nuses) return;
+    I8* __restrict__ out_codes, LO* __restrict__ found) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO u = t / kMatchLanes;
+  int lane = t % kMatchLanes;
   constexpr int LOW = HIGH - 1;
   constexpr int nvh = HIGH + 1;
   constexpr int nvl = LOW + 1;
@@ -258,39 +269,37 @@ __global__ void match_uses(LO nuses, LO const* __restrict__ prod_verts,
   for (int k = 0; k < nvl; ++k) av[k] = prod_verts[h * nvh + down_template(HIGH, LOW, which, k)];
   LO vm = min_vertex<nvl>(av);
   LO k0 = a2ab[vm], k1 = a2ab[vm + 1] - 1;
-  int found = 0;
-  LO fb = 0;
-  I8 fc = 0;
-#pragma unroll 4
-  for (LO k = k0; k < k1; ++k) {
+  for (LO k = k0 + lane; k < k1; k += kMatchLanes) {
     int4 r = rows[k];
     LO bv[3] = {r.y, r.z, r.w};
     // wd: the position of the use's first vertex in the candidate
     int wd = (bv[0] == av[0]) ? 0 : (bv[1] == av[0]) ? 1 : (nvl == 3 && bv[2] == av[0]) ? 2 : -1;
     if (wd < 0) continue;
+    I8 code;
     if (nvl == 2) {
-      if (av[1] == bv[1 - wd]) {
-        fb = r.x;
-        fc = make_code(0, wd, 0);
-        ++found;
-      }
+      if (av[1] != bv[1 - wd]) continue;
+      code = make_code(0, wd, 0);
+    } else if (av[1] == bv[(wd + 1) % 3] && av[2] == bv[(wd + 2) % 3]) {
+      code = make_code(0, (3 - wd) % 3, 0);
+    } else if (av[1] == bv[(wd + 2) % 3] && av[2] == bv[(wd + 1) % 3]) {
+      code = make_code(1, (3 - wd) % 3, 0);
     } else {
-      if (av[1] == bv[(wd + 1) % 3] && av[2] == bv[(wd + 2) % 3]) {
-        fb = r.x;
-        fc = make_code(0, (3 - wd) % 3, 0);
-        ++found;
-      } else if (av[1] == bv[(wd + 2) % 3] && av[2] == bv[(wd + 1) % 3]) {
-        fb = r.x;
-        fc = make_code(1, (3 - wd) % 3, 0);
-        ++found;
-      }
+      continue;
     }
+    out[u] = r.x;
+    out_codes[u] = code;
+    atomicAdd(&found[u], 1);
   }
-  // without a unique match every index stays in range for the kernels that
-  // run before the host sees the error
-  out[u] = fb;
-  out_codes[u] = fc;
-  if (found != 1) *error = kReflectDownNoMatch;
+}
+
+// for callers that use the matches directly: a use without exactly one match
+// sets the error word and gets entity 0, which keeps every index in range
+__global__ void check_matches(LO nuses, LO const* found, LO* out, I8* out_codes, int* error) {
+  LO u = thread_index();
+  if (u >= nuses || found[u] == 1) return;
+  out[u] = 0;
+  out_codes[u] = 0;
+  *error = kReflectDownNoMatch;
 }
 
 // per side, the number of elements on it (high 32 bits) and the sum of their
@@ -388,11 +397,11 @@ LOs const& ask_down(Ctx& c, DMesh& m, int high, int low, Bytes const** codes) {
 }
 
 Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks, LOs* marked,
-    bool sorted) {
+    bool sorted, LOs* marked_index) {
   Bytes const* codes = nullptr;
   LOs const& down = ask_down(c, m, high, low, &codes);
   return invert_adj(c, m.nents[high], simplex_degree(high, low), m.nents[low], down, codes,
-      &marks, marked, sorted);
+      &marks, marked, sorted, marked_index);
 }
 
 Adj const& ask_up(Ctx& c, DMesh& m, int low, int high) {
@@ -406,11 +415,13 @@ Adj const& ask_up(Ctx& c, DMesh& m, int low, int high) {
 }
 
 void reflect_down(Ctx& c, DMesh& m, int high_dim, LO nprods, LOs const& prod_verts,
-    LOs& down, Bytes& codes) {
+    LOs& down, Bytes& codes, LOs* found_out) {
   int low_dim = high_dim - 1;
   LO nuses = nprods * (high_dim + 1);
   down = LOs(c, nuses);
   codes = Bytes(c, nuses);
+  LOs found(c, nuses);
+  if (found_out) *found_out = found;
   if (nuses == 0) return;
   LOs const& lv2v = verts_of(c, m, low_dim);
   LO nv = m.nents[0];
@@ -418,26 +429,30 @@ void reflect_down(Ctx& c, DMesh& m, int high_dim, LO nprods, LOs const& prod_ver
   LOs counts = fill_lo(c, nv, 0);
   LOs slot(c, nl);
   if (high_dim == 2) {
-    OSH_LAUNCH(mark_use_min_verts<2>, nuses, c, prod_verts.data(), counts.data());
+    OSH_LAUNCH(mark_use_min_verts<2>, nuses, c, prod_verts.data(), counts.data(), found.data());
     OSH_LAUNCH(count_min_rows<2>, nl, c, lv2v.data(), counts.data(), slot.data());
   } else {
-    OSH_LAUNCH(mark_use_min_verts<3>, nuses, c, prod_verts.data(), counts.data());
+    OSH_LAUNCH(mark_use_min_verts<3>, nuses, c, prod_verts.data(), counts.data(), found.data());
     OSH_LAUNCH(count_min_rows<3>, nl, c, lv2v.data(), counts.data(), slot.data());
   }
   LOs a2ab = offset_scan(c, counts);
   // every lower entity in at most one row plus one spare slot per marked row
   DArray<int4> rows(c, nl + nv);
-  // a use without a unique match sets the context's error word, which the
-  // next stream sync raises
   if (high_dim == 2) {
     OSH_LAUNCH(fill_min_rows<2>, nl, c, lv2v.data(), a2ab.data(), slot.data(), rows.data());
-    OSH_LAUNCH(match_uses<2>, nuses, c, prod_verts.data(), a2ab.data(), rows.data(), down.data(),
-        codes.data(), c.error_word());
+    OSH_LAUNCH(match_uses<2>, nuses * kMatchLanes, c, prod_verts.data(), a2ab.data(),
+        rows.data(), down.data(), codes.data(), found.data());
   } else {
     OSH_LAUNCH(fill_min_rows<3>, nl, c, lv2v.data(), a2ab.data(), slot.data(), rows.data());
-    OSH_LAUNCH(match_uses<3>, nuses, c, prod_verts.data(), a2ab.data(), rows.data(), down.data(),
-        codes.data(), c.error_word());
+    OSH_LAUNCH(match_uses<3>, nuses * kMatchLanes, c, prod_verts.data(), a2ab.data(),
+        rows.data(), down.data(), codes.data(), found.data());
   }
+  // a use without a unique match sets the context's error word, which the
+  // next stream sync raises; a caller that takes the match counts checks
+  // them where it reads the matches
+  if (!found_out)
+    OSH_LAUNCH(check_matches, nuses, c, found.data(), down.data(), codes.data(),
+        c.error_word());
 }
 
 Adj const& ask_star(Ctx& c, DMesh& m, int d) {

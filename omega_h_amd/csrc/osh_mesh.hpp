@@ -73,14 +73,18 @@ Adj const& ask_up(Ctx& c, DMesh& m, int low, int high);
 // `marked`, also the list of marked lows (collect_marked), from the same scan
 // launch.  Without `sorted`, a row's order is arbitrary, which suits uses
 // that do not depend on it (counts, any/all tests, min/max)
+// `marked_index` (with `marked`) receives, for each marked low, its position
+// in the list (the scan of the marks)
 Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks, LOs* marked = nullptr,
-    bool sorted = true);
+    bool sorted = true, LOs* marked_index = nullptr);
 Adj const& ask_star(Ctx& c, DMesh& m, int d);
 DArray<unsigned long long> const& ask_side_elems(Ctx& c, DMesh& m);
 // vertex lists of new entities of dimension high_dim >= 2 matched against the
 // already-built dimension high_dim-1 of m (src/Omega_h_adj.cpp:424-441)
+// With found_out, the per-use match counts come back unchecked and the caller
+// must treat a use whose count is not 1 as an error (and not read its match).
 void reflect_down(Ctx& c, DMesh& m, int high_dim, LO nprods, LOs const& prod_verts,
-    LOs& down, Bytes& codes);
+    LOs& down, Bytes& codes, LOs* found_out = nullptr);
 // unique lower entities of element vertex lists (src/Omega_h_adj.cpp:133-153)
 LOs find_unique(Ctx& c, LO nhigh, int high_dim, int low_dim, LOs const& hv2v);
 

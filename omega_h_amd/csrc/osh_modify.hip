@@ -205,9 +205,9 @@ constexpr int kProdLanes = 8;
 __global__ void prod_numbers(LO nt, LO const* __restrict__ reps,
     LO const* __restrict__ keys2kds, LO const* __restrict__ offsets, LO const* __restrict__ order,
     LO const* __restrict__ keys2prods, LO* __restrict__ prods2new, int deg,
-    LO const* __restrict__ pdown, I8 const* __restrict__ pcodes, LO* __restrict__ down,
-    I8* __restrict__ codes, int nvpe, LO const* __restrict__ pverts, LO* __restrict__ verts,
-    LO const* __restrict__ key_src, LO* __restrict__ src) {
+    LO const* __restrict__ pdown, I8 const* __restrict__ pcodes, LO const* __restrict__ found,
+    LO* __restrict__ down, I8* __restrict__ codes, int nvpe, LO const* __restrict__ pverts,
+    LO* __restrict__ verts, LO const* __restrict__ key_src, LO* __restrict__ src, int* error) {
   LO t = thread_index();
   if (t >= nt) return;
   LO k = t / kProdLanes;
@@ -220,8 +220,12 @@ __global__ void prod_numbers(LO nt, LO const* __restrict__ reps,
     prods2new[p] = w;
     if (down)
       for (int j = 0; j < deg; ++j) {
-        down[w * deg + j] = pdown[p * deg + j];
-        if (codes) codes[w * deg + j] = pcodes[p * deg + j];
+        // a side matched by reflect_down must have matched exactly once; an
+        // error keeps the index in range until the host sees it
+        bool ok = !found || found[p * deg + j] == 1;
+        if (!ok) *error = kReflectDownNoMatch;
+        down[w * deg + j] = ok ? pdown[p * deg + j] : 0;
+        if (codes) codes[w * deg + j] = ok ? pcodes[p * deg + j] : I8(0);
       }
     if (verts)
       for (int j = 0; j < nvpe; ++j) verts[w * nvpe + j] = pverts[p * nvpe + j];
@@ -605,7 +609,7 @@ void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const&
   bool with_codes = ent_dim >= 2;
   LOs down;
   Bytes codes;
-  LOs pdown;
+  LOs pdown, found;
   Bytes pcodes;
   // the vertex lists of new faces and elements come with the numbering
   // (those of same entities renumbered, those of products as built), so the
@@ -616,7 +620,7 @@ void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const&
     down = LOs(c, nnew * deg);
     if (with_codes) {
       codes = Bytes(c, nnew * deg);
-      reflect_down(c, nw, ent_dim, nprods, prod_verts, pdown, pcodes);
+      reflect_down(c, nw, ent_dim, nprods, prod_verts, pdown, pcodes, &found);
       verts = LOs(c, nnew * deg);
       old_verts = verts_of(c, old, ent_dim).data();
     } else {
@@ -637,11 +641,12 @@ void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const&
       plan.offsets.data(),
       order.exists() ? order.data() : (LO const*)nullptr, keys2prods.data(),
       out.prods2new.data(), deg, ent_dim >= 1 ? pdown.data() : (LO const*)nullptr,
-      with_codes ? pcodes.data() : (I8 const*)nullptr, ent_dim >= 1 ? down.data() : (LO*)nullptr,
+      with_codes ? pcodes.data() : (I8 const*)nullptr,
+      with_codes ? found.data() : (LO const*)nullptr, ent_dim >= 1 ? down.data() : (LO*)nullptr,
       with_codes ? codes.data() : (I8*)nullptr, deg,
       verts.exists() ? prod_verts.data() : (LO const*)nullptr,
       verts.exists() ? verts.data() : (LO*)nullptr, keys2kds.data(),
-      prod_src ? prod_src->data() : (LO*)nullptr);
+      prod_src ? prod_src->data() : (LO*)nullptr, c.error_word());
   nw.nents[ent_dim] = nnew;
   if (ent_dim == 0) return;
   nw.down[ent_dim] = down;

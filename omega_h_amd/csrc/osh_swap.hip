@@ -286,21 +286,29 @@ __global__ void swap3d_key_offsets(LO nedges, I8 const* keys, LO const* key_off,
   treps[k] = trep_of[e];
 }
 
-__global__ void swap3d_products(LO nkeys, LO const* keys2edges, I8 const* configs,
-    LO const* e2t_a2ab, LO const* e2t_ab2b, I8 const* e2t_codes, LO const* ev2v, LO const* tv2v,
-    Tables const* tb, LO const* k2e, LO const* k2f, LO const* k2t, LO* pe, LO* pf, LO* pt) {
-  LO k = thread_index();
-  if (k >= nkeys) return;
+// swap3d_topology (src/Omega_h_swap3d_topology.cpp:12-119): each key's new
+// edges, faces and tets in template order, from the loop its candidate sweep
+// already found; kLanes threads per key share the products
+constexpr int kLanes = 8;
+
+__global__ void swap3d_products(LO nt, LO const* keys2edges, I8 const* configs,
+    LO const* cand_of_edge, LO const* lsize, LO const* leev, LO const* lverts, Tables const* tb,
+    LO const* k2e, LO const* k2f, LO const* k2t, LO* pe, LO* pf, LO* pt) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO k = t / kLanes;
+  int lane = t % kLanes;
   LO edge = keys2edges[k];
   int config = configs[edge];
-  Loop loop;
-  find_loop(edge, e2t_a2ab, e2t_ab2b, e2t_codes, ev2v, tv2v, loop);
-  int n = loop.size;
+  LO cand = cand_of_edge[edge];
+  int n = lsize[cand];
+  LO const* lv = lverts + cand * swap::kMaxLoop;
+  LO eev2v[2] = {leev[cand * 2], leev[cand * 2 + 1]};
   int npt = tb->mesh_sizes[n], npe = tb->nedges[n];
-  for (int pe_i = 0; pe_i < npe; ++pe_i) {
+  for (int pe_i = lane; pe_i < npe; pe_i += kLanes) {
     int ue = tb->medges[n][config * npe + pe_i];
-    LO a = loop.verts[tb->uedges[n][ue][0]];
-    LO b = loop.verts[tb->uedges[n][ue][1]];
+    LO a = lv[tb->uedges[n][ue][0]];
+    LO b = lv[tb->uedges[n][ue][1]];
     LO pedge = k2e[k] + pe_i;
     pe[pedge * 2] = a;
     pe[pedge * 2 + 1] = b;
@@ -308,28 +316,27 @@ __global__ void swap3d_products(LO nkeys, LO const* keys2edges, I8 const* config
       LO ptri = k2f[k] + 2 * pe_i + eev;
       pf[ptri * 3 + 0] = a;
       pf[ptri * 3 + 1] = b;
-      pf[ptri * 3 + 2] = loop.eev2v[eev];
+      pf[ptri * 3 + 2] = eev2v[eev];
     }
   }
-  for (int pt_i = 0; pt_i < npt; ++pt_i) {
+  for (int pt_i = lane; pt_i < npt; pt_i += kLanes) {
     int ut = tb->meshes[n][config * npt + pt_i];
     LO tv[3];
-    for (int j = 0; j < 3; ++j) tv[j] = loop.verts[tb->tris[n][ut][j]];
+    for (int j = 0; j < 3; ++j) tv[j] = lv[tb->tris[n][ut][j]];
     LO ptri = k2f[k] + 2 * npe + pt_i;
     for (int j = 0; j < 3; ++j) pf[ptri * 3 + j] = tv[j];
     for (int eev = 0; eev < 2; ++eev) {
       LO ptet = k2t[k] + 2 * pt_i + eev;
-      LO nt4[4] = {tv[0], tv[1], tv[2], loop.eev2v[eev]};
+      LO nt4[4] = {tv[0], tv[1], tv[2], eev2v[eev]};
       if (eev == 0) {
-        LO t = nt4[1];
+        LO tmp = nt4[1];
         nt4[1] = nt4[2];
-        nt4[2] = t;
+        nt4[2] = tmp;
       }
       for (int j = 0; j < 4; ++j) pt[ptet * 4 + j] = nt4[j];
     }
   }
 }
-
 
 __global__ void loops_kernel(LO n, LO const* edges, LO const* a2ab, LO const* ab2b,
     I8 const* codes, LO const* ev2v, LO const* tv2v, LO* eev2v, LO* verts, LO* sizes) {
@@ -482,8 +489,8 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   Reals quals;
   Bytes cand = swap_candidates(c, m, o, &configs, keep, quals);
   // every edge -> tet row this pass reads is a candidate's
-  LOs cands;
-  Adj e2t = marked_up(c, m, 1, 3, cand, &cands);
+  LOs cands, cand_of_edge;
+  Adj e2t = marked_up(c, m, 1, 3, cand, &cands, true, &cand_of_edge);
   if (cands.size() == 0) return false;
   LO ne = m.nents[1];
   int md = (m.metric_ncomps == 1) ? 1 : 3;
@@ -558,8 +565,8 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
       offs[2].data(), offs[3].data(), rep_of[2].data(), rep_of[3].data(), keys2edges.data(),
       k2p[1].data(), k2p[2].data(), k2p[3].data(), plans[0].reps.data(), plans[1].reps.data(),
       plans[2].reps.data());
-  OSH_LAUNCH(swap3d_products, nkeys, c, keys2edges.data(), configs.data(), e2t.a2ab.data(),
-      e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), tb, k2p[1].data(),
+  OSH_LAUNCH(swap3d_products, nkeys * kLanes, c, keys2edges.data(), configs.data(),
+      cand_of_edge.data(), lsize.data(), leev.data(), lverts.data(), tb, k2p[1].data(),
       k2p[2].data(), k2p[3].data(), pv[1].data(), pv[2].data(), pv[3].data());
   swap_rebuild(c, m, keys2edges, k2p, nprods, pv, plans);
   return true;
