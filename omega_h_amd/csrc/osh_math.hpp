@@ -681,7 +681,8 @@ OSH_HD double element_quality(double const* const* p, double const* const* ms) {
   } else {
     rs = (b[0][0] * b[1][1] - b[0][1] * b[1][0]) / 2.0;
   }
-  double dm = det<M>(m);
+  // the chosen metric's determinant is the maximum found above
+  double dm = maxdet;
   double f;
   if (D == 3 && M == 3)
     f = math::sqrt_(dm);
