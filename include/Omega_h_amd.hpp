@@ -241,6 +241,18 @@ class Mesh {
     check_status(osh_mesh_get_star(handle(), ent_dim, a2ab.data(), ab2b.data()));
     return Graph{a2ab, ab2b};
   }
+  Graph ask_dual() const {
+    int n;
+    check_status(osh_mesh_dual_size(handle(), &n));
+    Write<LO> a2ab(nelems() + 1), ab2b(n);
+    check_status(osh_mesh_get_dual(handle(), a2ab.data(), ab2b.data()));
+    return Graph{a2ab, ab2b};
+  }
+  Reals ask_sizes() const {
+    Write<Real> w(nelems());
+    check_status(osh_mesh_get_sizes(handle(), w.data()));
+    return w;
+  }
   Reals ask_lengths() const {
     Write<Real> w(nedges());
     check_status(osh_mesh_get_lengths(handle(), w.data()));

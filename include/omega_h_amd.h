@@ -87,6 +87,14 @@ int osh_mesh_get_up(osh_mesh* mesh, int low, int high, int* a2ab, int* ab2b,
 /* Mesh::ask_star(dim) */
 int osh_mesh_star_size(osh_mesh* mesh, int ent_dim, int* out);
 int osh_mesh_get_star(osh_mesh* mesh, int ent_dim, int* a2ab, int* ab2b);
+/* Mesh::ask_dual() (src/Omega_h_mesh.cpp:260, elements_across_sides in
+ * src/Omega_h_adj.cpp:591-657): CSR of the elements across each element's
+ * sides, in side order; a side shared by more than two elements is an error */
+int osh_mesh_dual_size(osh_mesh* mesh, int* out);
+int osh_mesh_get_dual(osh_mesh* mesh, int* a2ab, int* ab2b);
+/* Mesh::ask_sizes() (src/Omega_h_mesh.cpp:390-396, measure_elements_real in
+ * src/Omega_h_shape.cpp:55-89): signed element areas (2D) or volumes (3D) */
+int osh_mesh_get_sizes(osh_mesh* mesh, double* out);
 /* tags "coordinates", "metric", "class_id"/"class_dim", "length", "quality" */
 int osh_mesh_get_coords(osh_mesh* mesh, double* out);
 int osh_mesh_set_metric(osh_mesh* mesh, int ncomps, const double* metric);

@@ -106,6 +106,9 @@ def load(path=None):
         "osh_mesh_get_up": [P, I, I, IP, IP, BP],
         "osh_mesh_star_size": [P, I, IP],
         "osh_mesh_get_star": [P, I, IP, IP],
+        "osh_mesh_dual_size": [P, IP],
+        "osh_mesh_get_dual": [P, IP, IP],
+        "osh_mesh_get_sizes": [P, DP],
         "osh_mesh_get_coords": [P, DP],
         "osh_mesh_set_metric": [P, I, DP],
         "osh_mesh_metric_ncomps": [P, IP],
@@ -331,6 +334,21 @@ class Mesh:
         check(self.lib.osh_mesh_get_star(self.h, d, _p(a2ab, C.c_int), _p(ab2b, C.c_int)),
               self.lib)
         return a2ab, ab2b
+
+    def dual(self):
+        """Mesh::ask_dual (src/Omega_h_mesh.cpp:260): elements across sides."""
+        n = self._int(self.lib.osh_mesh_dual_size)
+        a2ab = np.zeros(self.nents(self.dim) + 1, dtype=np.int32)
+        ab2b = np.zeros(n, dtype=np.int32)
+        check(self.lib.osh_mesh_get_dual(self.h, _p(a2ab, C.c_int), _p(ab2b, C.c_int)),
+              self.lib)
+        return a2ab, ab2b
+
+    def sizes(self):
+        """Mesh::ask_sizes (src/Omega_h_mesh.cpp:390-396): signed element sizes."""
+        out = np.zeros(self.nents(self.dim), dtype=np.float64)
+        check(self.lib.osh_mesh_get_sizes(self.h, _p(out, C.c_double)), self.lib)
+        return out
 
     def coords(self):
         out = np.zeros(self.nents(0) * self.dim, dtype=np.float64)

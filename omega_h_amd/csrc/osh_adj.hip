@@ -26,6 +26,8 @@ void DMesh::clear_caches() {
   }
   star[0] = star[1] = Adj();
   side_elems = DArray<unsigned long long>();
+  dual = Adj();
+  size = Reals();
   t2e = LOs();
   t2e_codes = Bytes();
 }
@@ -317,6 +319,33 @@ __global__ void side_elems_kernel(LO nuses, int nspe, LO const* es2s, unsigned l
   atomicAdd(&se[es2s[i]], (1ull << 32) + static_cast<unsigned long long>(e));
 }
 
+// elements_across_sides (src/Omega_h_adj.cpp:591-657) from the per-side
+// words: each element lists the other element across each of its sides that
+// has two, in side order
+__global__ void dual_degrees(LO ne, int nspe, LO const* es2s, unsigned long long const* se,
+    LO* deg, int* error) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  LO d = 0;
+  for (int j = 0; j < nspe; ++j) {
+    unsigned count = unsigned(se[es2s[e * nspe + j]] >> 32);
+    if (count == 2) ++d;
+    if (count > 2) *error = kNonManifoldSide;
+  }
+  deg[e] = d;
+}
+
+__global__ void dual_fill(LO ne, int nspe, LO const* es2s, unsigned long long const* se,
+    LO const* a2ab, LO* ab2b) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  LO k = a2ab[e];
+  for (int j = 0; j < nspe; ++j) {
+    unsigned long long w = se[es2s[e * nspe + j]];
+    if (unsigned(w >> 32) == 2) ab2b[k++] = LO(unsigned(w)) - e;
+  }
+}
+
 __global__ void verts_across_edges_kernel(LO nve, LO const* ve2e, I8 const* codes,
     LO const* ev2v, LO* vv2v) {
   LO ve = thread_index();
@@ -487,6 +516,25 @@ Adj const& ask_star(Ctx& c, DMesh& m, int d) {
     m.star[d] = g;
   }
   return m.star[d];
+}
+
+Adj const& ask_dual(Ctx& c, DMesh& m) {
+  if (!m.dual.exists()) {
+    int dim = m.dim;
+    LO ne = m.nents[dim];
+    auto const& se = ask_side_elems(c, m);
+    LOs deg(c, ne);
+    OSH_LAUNCH(dual_degrees, ne, c, dim + 1, m.down[dim].data(), se.data(), deg.data(),
+        c.error_word());
+    Adj g;
+    LO total;
+    g.a2ab = offset_scan(c, deg, &total);
+    g.ab2b = LOs(c, total);
+    OSH_LAUNCH(dual_fill, ne, c, dim + 1, m.down[dim].data(), se.data(), g.a2ab.data(),
+        g.ab2b.data());
+    m.dual = g;
+  }
+  return m.dual;
 }
 
 DArray<unsigned long long> const& ask_side_elems(Ctx& c, DMesh& m) {

@@ -300,6 +300,33 @@ int osh_mesh_get_star(osh_mesh* mesh, int d, int* a2ab, int* ab2b) {
   });
 }
 
+int osh_mesh_dual_size(osh_mesh* mesh, int* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    auto& c = mesh->owner->ctx;
+    *out = osh::ask_dual(c, mesh->m).ab2b.size();
+    c.sync();
+  });
+}
+
+int osh_mesh_get_dual(osh_mesh* mesh, int* a2ab, int* ab2b) {
+  return guarded([&] {
+    check_mesh(mesh);
+    auto& c = mesh->owner->ctx;
+    auto const& g = osh::ask_dual(c, mesh->m);
+    download(c, g.a2ab, a2ab);
+    download(c, g.ab2b, ab2b);
+  });
+}
+
+int osh_mesh_get_sizes(osh_mesh* mesh, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    auto& c = mesh->owner->ctx;
+    download(c, osh::ask_sizes(c, mesh->m), out);
+  });
+}
+
 int osh_mesh_get_coords(osh_mesh* mesh, double* out) {
   return guarded([&] {
     check_mesh(mesh);

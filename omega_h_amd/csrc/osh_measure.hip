@@ -122,6 +122,26 @@ __global__ void mident_kernel(LO n, LO const* edges, LO const* ev2v, Real const*
   average_metric2<M>(a, b, out + i * nc);
 }
 
+// real_simplex_size (src/Omega_h_shape.hpp:77-100,166-170) of each element
+template <int D>
+__global__ void sizes_kernel(LO n, LO const* ev2v, Real const* coords, Real* out) {
+  LO e = thread_index();
+  if (e >= n) return;
+  double b[3][3];
+  LO v0 = ev2v[e * (D + 1)];
+  for (int i = 0; i < D; ++i) {
+    LO v = ev2v[e * (D + 1) + i + 1];
+    for (int j = 0; j < D; ++j) b[i][j] = coords[v * D + j] - coords[v0 * D + j];
+  }
+  if (D == 3) {
+    double c[3];
+    cross3(b[0], b[1], c);
+    out[e] = dot<3>(c, b[2]) / 6.0;
+  } else {
+    out[e] = (b[0][0] * b[1][1] - b[0][1] * b[1][0]) / 2.0;
+  }
+}
+
 int metric_dim(DMesh const& m) { return (m.metric_ncomps == 1) ? 1 : m.dim; }
 
 }  // namespace
@@ -180,6 +200,19 @@ Reals const& ask_lengths(Ctx& c, DMesh& m) {
     measure_edges(c, m, m.nents[1], nullptr, m.length.data());
   }
   return m.length;
+}
+
+Reals const& ask_sizes(Ctx& c, DMesh& m) {
+  if (!m.size.exists()) {
+    LO n = m.nents[m.dim];
+    m.size = Reals(c, n);
+    LO const* ev2v = verts_of(c, m, m.dim).data();
+    if (m.dim == 3)
+      OSH_LAUNCH(sizes_kernel<3>, n, c, ev2v, m.coords.data(), m.size.data());
+    else
+      OSH_LAUNCH(sizes_kernel<2>, n, c, ev2v, m.coords.data(), m.size.data());
+  }
+  return m.size;
 }
 
 Reals const& ask_qualities(Ctx& c, DMesh& m) {

@@ -60,6 +60,8 @@ struct DMesh {
   // side_elems[s]: the number of elements on side s (high 32 bits) and the
   // sum of their indices (low 32 bits)
   DArray<unsigned long long> side_elems;
+  Adj dual;    // element -> elements across sides (on demand, for callers)
+  Reals size;  // element sizes (on demand)
   void clear_caches();
 };
 
@@ -79,6 +81,11 @@ Adj marked_up(Ctx& c, DMesh& m, int low, int high, Bytes const& marks, LOs* mark
     bool sorted = true, LOs* marked_index = nullptr);
 Adj const& ask_star(Ctx& c, DMesh& m, int d);
 DArray<unsigned long long> const& ask_side_elems(Ctx& c, DMesh& m);
+// Mesh::ask_dual (src/Omega_h_mesh.cpp:260): elements_across_sides
+Adj const& ask_dual(Ctx& c, DMesh& m);
+// Mesh::ask_sizes (src/Omega_h_mesh.cpp:390-396): signed element areas or
+// volumes (measure_elements_real, src/Omega_h_shape.cpp:55-89)
+Reals const& ask_sizes(Ctx& c, DMesh& m);
 // vertex lists of new entities of dimension high_dim >= 2 matched against the
 // already-built dimension high_dim-1 of m (src/Omega_h_adj.cpp:424-441)
 // With found_out, the per-use match counts come back unchecked and the caller

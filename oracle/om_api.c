@@ -77,6 +77,31 @@ void om_get_up(om_mesh* m, int low, int high, int* a2ab, int* ab2b,
   memcpy(codes, g->codes, (size_t)g->a2ab[g->n]);
 }
 
+/* Mesh::ask_dual, src/Omega_h_mesh.cpp:260 (elements_across_sides,
+ * src/Omega_h_adj.cpp:591-657) */
+int om_dual_size(om_mesh* m) {
+  om_graph const* g = om_ask_dual(m);
+  return g->a2ab[g->n];
+}
+
+void om_get_dual(om_mesh* m, int* a2ab, int* ab2b) {
+  om_graph const* g = om_ask_dual(m);
+  memcpy(a2ab, g->a2ab, sizeof(int) * (size_t)(g->n + 1));
+  memcpy(ab2b, g->ab2b, sizeof(int) * (size_t)g->a2ab[g->n]);
+}
+
+/* Mesh::ask_sizes, src/Omega_h_mesh.cpp:390-396 (measure_elements_real,
+ * src/Omega_h_shape.cpp:55-89) */
+void om_get_sizes(om_mesh* m, double* out) {
+  int dim = m->dim;
+  int const* ev2v = om_verts_of(m, dim);
+  for (int e = 0; e < m->nents[dim]; ++e) {
+    double const* p[4];
+    for (int k = 0; k < dim + 1; ++k) p[k] = m->coords + (size_t)ev2v[e * (dim + 1) + k] * dim;
+    out[e] = om_element_size(dim, p);
+  }
+}
+
 int om_star_size(om_mesh* m, int d) {
   om_graph const* g = om_ask_star(m, d);
   return g->a2ab[g->n];

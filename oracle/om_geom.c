@@ -422,6 +422,20 @@ static double metric_size_factor(int dim, int mdim, double det) {
   return det; /* dim 2, iso */
 }
 
+/* real_simplex_size / simplex_size_from_basis, src/Omega_h_shape.hpp:77-100,166-170:
+ * the signed area (2D) or volume (3D) of a simplex from its basis vectors */
+double om_element_size(int dim, double const* const* p) {
+  double b[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int i = 0; i < dim; ++i)
+    for (int j = 0; j < dim; ++j) b[i][j] = p[i + 1][j] - p[0][j];
+  if (dim == 3) {
+    double c[3];
+    cross3(b[0], b[1], c);
+    return dot(3, c, b[2]) / 6.0;
+  }
+  return (b[0][0] * b[1][1] - b[0][1] * b[1][0]) / 2.0;
+}
+
 double om_element_quality(int dim, int mdim, double const* const* p,
     double const* const* ms) {
   /* maxdet_metric, src/Omega_h_metric.hpp:170-181 */

@@ -41,6 +41,7 @@ int om_roots2(double const coeffs[2], double eps, double roots[2], int mults[2])
 /* geometry */
 double om_metric_edge_length(int sdim, int mdim, double const* p0,
     double const* p1, double const* m0, double const* m1);
+double om_element_size(int dim, double const* const* p);
 double om_element_quality(int dim, int mdim, double const* const* p,
     double const* const* ms);
 void om_average_metric(int mdim, int n, double const* const* ms, double* out);

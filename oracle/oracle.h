@@ -57,6 +57,9 @@ void om_get_qualities(om_mesh* m, double* out);
 int om_up_size(om_mesh* m, int low, int high);
 void om_get_up(om_mesh* m, int low, int high, int* a2ab, int* ab2b,
     signed char* codes);
+int om_dual_size(om_mesh* m);
+void om_get_dual(om_mesh* m, int* a2ab, int* ab2b);
+void om_get_sizes(om_mesh* m, double* out);
 int om_star_size(om_mesh* m, int d);
 void om_get_star(om_mesh* m, int d, int* a2ab, int* ab2b);
 

@@ -79,6 +79,9 @@ def lib():
         L.om_up_size.argtypes = [P, C.c_int, C.c_int]
         L.om_get_up.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_int),
                                 C.POINTER(C.c_int), C.POINTER(C.c_byte)]
+        L.om_dual_size.argtypes = [P]
+        L.om_get_dual.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.om_get_sizes.argtypes = [P, C.POINTER(C.c_double)]
         L.om_star_size.argtypes = [P, C.c_int]
         L.om_get_star.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         for name in ("om_refine_by_size", "om_coarsen_by_size", "om_swap_edges",
@@ -218,6 +221,18 @@ class Mesh:
         lib().om_get_up(self.h, low, high, _ptr(a2ab, C.c_int), _ptr(ab2b, C.c_int),
                         _ptr(codes, C.c_byte))
         return a2ab, ab2b, codes
+
+    def dual(self):
+        n = lib().om_dual_size(self.h)
+        a2ab = np.zeros(self.nents(self.dim) + 1, dtype=np.int32)
+        ab2b = np.zeros(n, dtype=np.int32)
+        lib().om_get_dual(self.h, _ptr(a2ab, C.c_int), _ptr(ab2b, C.c_int))
+        return a2ab, ab2b
+
+    def sizes(self):
+        out = np.zeros(self.nents(self.dim), dtype=np.float64)
+        lib().om_get_sizes(self.h, _ptr(out, C.c_double))
+        return out
 
     def star(self, d):
         n = lib().om_star_size(self.h, d)

@@ -50,6 +50,9 @@ def compare_meshes(o, g, tags=True):
         if d >= 2:
             assert_same(oc, gc, f"codes {d}")
     assert_same(o.coords(), g.coords(), "coords")
+    for x, y in zip(o.dual(), g.dual()):
+        assert_same(x, y, "dual")
+    assert_same(o.sizes(), g.sizes(), "sizes")
     if tags:
         for d in range(dim + 1):
             oi, od_ = o.classification(d)

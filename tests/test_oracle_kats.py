@@ -289,6 +289,59 @@ def test_star_graphs():
                        4, 2, 2, 0, 3, 5, 1, 1, 2, 4, 3, 0]
 
 
+def brute_dual(m):
+    """Elements across each element's sides, in side order, found by search."""
+    dim = m.dim
+    down = m.down(dim)[0].reshape(-1, dim + 1)
+    owners = {}
+    for e, sides in enumerate(down):
+        for s in sides:
+            owners.setdefault(int(s), []).append(e)
+    a2ab, ab2b = [0], []
+    for e, sides in enumerate(down):
+        for s in sides:
+            ab2b += [x for x in owners[int(s)] if x != e]
+        a2ab.append(len(ab2b))
+    return a2ab, ab2b
+
+
+@pytest.mark.parametrize("dim,n", [(2, 1), (2, 4), (3, 1), (3, 3)])
+def test_dual_against_search_and_counts(dim, n):
+    # Mesh::ask_dual, src/Omega_h_mesh.cpp:260 (src/Omega_h_adj.cpp:591-657)
+    m = O.Mesh.box(dim, n)
+    a2ab, ab2b = m.dual()
+    want_a, want_b = brute_dual(m)
+    assert list(a2ab) == want_a and list(ab2b) == want_b
+    # every interior side appears twice, boundary sides never
+    nbdry = 4 * n if dim == 2 else 12 * n * n
+    assert len(ab2b) == 2 * (m.nents(dim - 1) - nbdry)
+    pairs = set(zip(np.repeat(np.arange(m.nents(dim)), np.diff(a2ab)), ab2b))
+    assert all((b, a) in pairs for a, b in pairs)
+
+
+def test_dual_single_simplex_is_empty():
+    for dim, ev2v in ((2, [0, 1, 2]), (3, [0, 1, 2, 3])):
+        a2ab, ab2b = O.Mesh.from_elems(dim, ev2v, dim + 1).dual()
+        assert list(a2ab) == [0, 0] and len(ab2b) == 0
+
+
+@pytest.mark.parametrize("dim,n", [(2, 1), (2, 7), (3, 1), (3, 4)])
+def test_sizes_of_box(dim, n):
+    # Mesh::ask_sizes, src/Omega_h_mesh.cpp:390-396: the box splits each
+    # square into 2 triangles and each cube into 6 tets of equal size
+    m = O.Mesh.box(dim, n)
+    s = m.sizes()
+    unit = 1.0 / (2 * n * n) if dim == 2 else 1.0 / (6 * n ** 3)
+    assert np.allclose(s, unit, rtol=1e-12, atol=0)
+    assert abs(s.sum() - 1.0) < 1e-12
+    # against the determinant of the edge basis
+    c = m.coords().reshape(-1, dim)
+    ev = m.verts(dim).reshape(-1, dim + 1)
+    b = c[ev[:, 1:]] - c[ev[:, :1]]
+    fact = 2.0 if dim == 2 else 6.0
+    assert np.allclose(s, np.linalg.det(b) / fact, rtol=1e-12, atol=1e-15)
+
+
 def test_refine_qualities_unit_square():
     # src/unit_mesh.cpp:382-391
     m = O.Mesh.box(2, 1, internal=True)
