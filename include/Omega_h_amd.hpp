@@ -203,6 +203,10 @@ class Mesh {
     check_status(osh_mesh_get_coords(handle(), w.data()));
     return w;
   }
+  void set_coords(Reals const& array) {
+    if (array.size() != nverts() * dim()) throw exception("coordinates have the wrong size");
+    check_status(osh_mesh_set_coords(handle(), array.data()));
+  }
   // one rank: the global number of an entity is its index
   Read<GO> globals(Int ent_dim) const {
     Write<GO> w(nents(ent_dim));
@@ -326,6 +330,11 @@ inline void set_metric(Mesh& m, Int ent_dim, std::string const& name, Int ncomps
   if (size != m.nverts() * ncomps) throw exception("tag array has the wrong size");
   if (name == "metric") {
     check_status(osh_mesh_set_metric(m.handle(), ncomps, data));
+    return;
+  }
+  if (name == "coordinates") {
+    if (ncomps != m.dim()) throw exception("coordinates need dim components");
+    check_status(osh_mesh_set_coords(m.handle(), data));
     return;
   }
   bool is_metric = name.size() >= 6 && name.compare(name.size() - 6, 6, "metric") == 0;
@@ -512,6 +521,14 @@ inline bool approach_metric(Mesh* mesh, AdaptOpts const& opts, Real min_step = 1
   osh_adapt_opts o = opts.to_c();
   int did = 0;
   check_status(osh_approach_metric(mesh->handle(), &o, min_step, &did));
+  return did != 0;
+}
+
+inline bool warp_to_limit(Mesh* mesh, AdaptOpts const& opts, bool exit_on_stall = false,
+    Int max_niters = 40) {
+  osh_adapt_opts o = opts.to_c();
+  int did = 0;
+  check_status(osh_warp_to_limit(mesh->handle(), &o, exit_on_stall ? 1 : 0, max_niters, &did));
   return did != 0;
 }
 

@@ -176,6 +176,20 @@ int osh_mesh_get_inherit_tag(osh_mesh* mesh, const char* name, int ent_dim, doub
  * *did = 0 when there is no target.  The reference's idiom
  * (src/aniso_test.cpp:15-27) is: while (approach_metric) adapt. */
 int osh_approach_metric(osh_mesh* mesh, const osh_adapt_opts* opts, double min_step, int* did);
+/* warp_to_limit(mesh, opts, exit_on_stall, max_niters),
+ * src/Omega_h_approach.cpp:36-81: move the vertices by the "warp" vertex tag
+ * (dim components; add it as a linear tag so adapt() carries it), halving
+ * the move until quality and lengths are acceptable; the part not yet moved
+ * stays in the tag, which is removed once the whole warp is done.  *did = 0
+ * when there is no warp tag.  An unacceptable starting mesh is an error, and
+ * so are more than max_niters halvings unless exit_on_stall is set (then the
+ * tag is dropped).  The reference's idiom (src/bend_test.cpp:55-65) is:
+ * do { adapt } while (warp_to_limit). */
+int osh_warp_to_limit(osh_mesh* mesh, const osh_adapt_opts* opts, int exit_on_stall,
+    int max_niters, int* did);
+/* Mesh::set_coords (src/Omega_h_mesh.cpp:365-368): nverts * dim values;
+ * drops the cached lengths, qualities and sizes */
+int osh_mesh_set_coords(osh_mesh* mesh, const double* data);
 /* get_implied_metrics(mesh), src/Omega_h_metric.cpp:280-309: the metric each
  * element implies (all its edges of unit length, src/Omega_h_shape.hpp:240-273),
  * scaled to a typical unit element and projected to the vertices by

@@ -110,6 +110,8 @@ def load(path=None):
         "osh_mesh_get_dual": [P, IP, IP],
         "osh_mesh_get_sizes": [P, DP],
         "osh_mesh_get_coords": [P, DP],
+        "osh_mesh_set_coords": [P, DP],
+        "osh_warp_to_limit": [P, C.POINTER(AdaptOpts), I, I, IP],
         "osh_mesh_set_metric": [P, I, DP],
         "osh_mesh_metric_ncomps": [P, IP],
         "osh_mesh_get_metric": [P, DP],
@@ -455,6 +457,18 @@ class Mesh:
         check(self.lib.osh_approach_metric(self.h, C.byref(opts), min_step, C.byref(did)),
               self.lib)
         return did.value
+
+    def warp_to_limit(self, opts, exit_on_stall=False, max_niters=40):
+        """warp_to_limit (src/Omega_h_approach.cpp:36-81); 1 if it moved."""
+        did = C.c_int()
+        check(self.lib.osh_warp_to_limit(self.h, C.byref(opts), int(exit_on_stall),
+                                         max_niters, C.byref(did)), self.lib)
+        return did.value
+
+    def set_coords(self, coords):
+        a = np.ascontiguousarray(coords, dtype=np.float64)
+        assert a.size == self.nents(0) * self.dim
+        check(self.lib.osh_mesh_set_coords(self.h, _p(a, C.c_double)), self.lib)
 
     def digest(self):
         """FNV-1a digest of the element vertex lists and coordinate bits."""

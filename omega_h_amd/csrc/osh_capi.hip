@@ -619,6 +619,33 @@ int osh_approach_metric(osh_mesh* mesh, const osh_adapt_opts* opts, double min_s
   });
 }
 
+int osh_warp_to_limit(osh_mesh* mesh, const osh_adapt_opts* opts, int exit_on_stall,
+    int max_niters, int* did) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(opts != nullptr, "null options");
+    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
+    osh::AdaptOptions o;
+    to_options(opts, o);
+    bool r = osh::warp_to_limit(mesh->owner->ctx, mesh->m, o, exit_on_stall != 0, max_niters);
+    mesh->owner->ctx.sync();
+    if (did) *did = r ? 1 : 0;
+  });
+}
+
+int osh_mesh_set_coords(osh_mesh* mesh, const double* data) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(data != nullptr, "null argument");
+    auto& c = mesh->owner->ctx;
+    osh::LO n = mesh->m.nents[0] * mesh->m.dim;
+    osh::Reals d(c, n);
+    osh::upload(c, d, data, n);
+    c.sync();
+    osh::set_coords(mesh->m, d);
+  });
+}
+
 int osh_get_implied_metrics(osh_mesh* mesh, int set_as_metric, double* out) {
   return guarded([&] {
     check_mesh(mesh);

@@ -204,6 +204,27 @@ void interpolate_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals cons
     OSH_LAUNCH(interpolate_kernel<1>, n, c, a.data(), b.data(), t, out.data());
 }
 
+// the first move of warp_to_limit: coordinates + warp
+__global__ void warp_add_kernel(LO n, Real const* coords, Real const* warp, Real* out) {
+  LO i = thread_index();
+  if (i >= n) return;
+  out[i] = coords[i] + warp[i];
+}
+
+// one halving of warp_to_limit (src/Omega_h_approach.cpp:66-70): the warp
+// halves, the remainder (0 before the first halving) gains the half, and
+// the vertices move by the half from where they started
+__global__ void warp_halve_kernel(LO n, Real const* coords, Real const* warp_in,
+    Real* warp_out, Real* rem, bool first, Real* out) {
+  LO i = thread_index();
+  if (i >= n) return;
+  Real h = warp_in[i] / 2.0;
+  warp_out[i] = h;
+  Real r = first ? 0.0 : rem[i];
+  rem[i] = r + h;
+  out[i] = coords[i] + h;
+}
+
 namespace {
 
 void replace_metric(DMesh& m, Reals const& v) {
@@ -250,6 +271,63 @@ bool approach_metric(Ctx& c, DMesh& m, AdaptOptions const& o, Real min_step) {
     interpolate_metric_arrays(c, mdim, m.nents[0], orig, target, factor, cur);
     replace_metric(m, cur);
   } while (!acceptable(c, m, o));
+  return true;
+}
+
+void set_coords(DMesh& m, Reals const& v) {
+  m.coords = v;
+  m.length = Reals();
+  m.quality = Reals();
+  m.size = Reals();
+}
+
+bool warp_to_limit(Ctx& c, DMesh& m, AdaptOptions const& o, bool exit_on_stall, int max_niters) {
+  auto it = m.vtags.begin();
+  for (; it != m.vtags.end(); ++it)
+    if (it->name == "warp") break;
+  if (it == m.vtags.end()) return false;
+  OSH_CHECK(it->ncomps == m.dim, "the warp tag needs one vector per vertex");
+  Real minq = min_real(c, ask_qualities(c, m));
+  OSH_CHECK(minq >= o.min_quality_allowed,
+      "warp_to_limit: minimum element quality " + std::to_string(minq) +
+          " < minimum allowed quality " + std::to_string(o.min_quality_allowed));
+  Real maxl = max_real(c, ask_lengths(c, m));
+  OSH_CHECK(!(maxl > o.max_length_allowed),
+      "warp_to_limit: maximum edge length " + std::to_string(maxl) +
+          " > maximum allowed length " + std::to_string(o.max_length_allowed));
+  LO n = m.nents[0] * m.dim;
+  Reals coords = m.coords;
+  Reals warp = it->data;
+  {
+    Reals moved(c, n);
+    OSH_LAUNCH(warp_add_kernel, n, c, coords.data(), warp.data(), moved.data());
+    set_coords(m, moved);
+  }
+  if (acceptable(c, m, o)) {
+    m.vtags.erase(it);
+    return true;
+  }
+  // the halved warp goes to an array of its own: the tag's array may be
+  // shared with copies of this mesh
+  Reals half(c, n), rem(c, n);
+  int i = 0;
+  do {
+    ++i;
+    if (i > max_niters) {
+      OSH_CHECK(exit_on_stall, "warp step " + std::to_string(i) +
+                                   ": unable to satisfy this warp under this size field");
+      for (it = m.vtags.begin(); it != m.vtags.end(); ++it)
+        if (it->name == "warp") break;
+      m.vtags.erase(it);
+      return true;
+    }
+    Reals moved(c, n);
+    OSH_LAUNCH(warp_halve_kernel, n, c, coords.data(), (i == 1) ? warp.data() : half.data(),
+        half.data(), rem.data(), i == 1, moved.data());
+    set_coords(m, moved);
+  } while (!acceptable(c, m, o));
+  for (auto& t : m.vtags)
+    if (t.name == "warp") t.data = rem;
   return true;
 }
 

@@ -106,5 +106,11 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
 // approach_metric (src/Omega_h_approach.cpp:83-126): move the metric toward
 // the "target_metric" vertex tag; false when there is no target
 bool approach_metric(Ctx& c, DMesh& m, AdaptOptions const& o, Real min_step);
+// Mesh::set_coords (src/Omega_h_mesh.cpp:365-368): drops the cached lengths,
+// qualities and sizes
+void set_coords(DMesh& m, Reals const& v);
+// warp_to_limit (src/Omega_h_approach.cpp:36-81): move the vertices by the
+// "warp" vertex tag as far as quality and lengths allow; false without one
+bool warp_to_limit(Ctx& c, DMesh& m, AdaptOptions const& o, bool exit_on_stall, int max_niters);
 
 }  // namespace osh

@@ -270,6 +270,73 @@ int om_approach_metric(om_mesh* m, om_adapt_opts const* o, double min_step) {
   return r;
 }
 
+/* Mesh::set_coords, src/Omega_h_mesh.cpp:365-368: new coordinates drop the
+ * cached lengths and qualities (src/Omega_h_mesh.cpp:167-177) */
+void om_set_coords(om_mesh* m, double const* coords) {
+  memcpy(m->coords, coords, sizeof(double) * (size_t)m->nents[0] * m->dim);
+  free(m->length);
+  free(m->quality);
+  m->length = NULL;
+  m->quality = NULL;
+}
+
+/* warp_to_limit, src/Omega_h_approach.cpp:36-81: move the vertices by the
+ * "warp" vertex tag, halving the move until the mesh's quality and lengths
+ * are acceptable; what is left of the warp stays in the tag.  Returns 1 if
+ * it moved, 0 without a warp tag, -1 if the mesh is unacceptable to begin
+ * with (or the tag is not one vector per vertex) and -2 when more than
+ * max_niters halvings were needed and exit_on_stall is 0 (the reference
+ * fails in both cases).  On a stall with exit_on_stall the tag is dropped
+ * and the last tried coordinates stay, as in the reference. */
+int om_warp_to_limit(om_mesh* m, om_adapt_opts const* o, int exit_on_stall, int max_niters) {
+  om_vtag* tg = find_vtag(m, "warp");
+  if (!tg) return 0;
+  if (tg->ncomps != m->dim) return -1;
+  if (!okay(m, o)) return -1;
+  size_t n = (size_t)m->nents[0] * m->dim;
+  double* coords = (double*)om_malloc(sizeof(double) * (n + 1));
+  double* warp = (double*)om_malloc(sizeof(double) * (n + 1));
+  double* cur = (double*)om_malloc(sizeof(double) * (n + 1));
+  memcpy(coords, m->coords, sizeof(double) * n);
+  memcpy(warp, tg->data, sizeof(double) * n);
+  for (size_t k = 0; k < n; ++k) cur[k] = coords[k] + warp[k];
+  om_set_coords(m, cur);
+  int r = 1;
+  if (okay(m, o)) {
+    om_remove_vtag(m, "warp");
+  } else {
+    double* rem = (double*)om_malloc(sizeof(double) * (n + 1));
+    for (size_t k = 0; k < n; ++k) rem[k] = 0.0;
+    int i = 0;
+    int stalled = 0;
+    do {
+      ++i;
+      if (i > max_niters) {
+        stalled = 1;
+        break;
+      }
+      for (size_t k = 0; k < n; ++k) {
+        double h = warp[k] / 2.0;
+        warp[k] = h;
+        rem[k] = rem[k] + h;
+        cur[k] = coords[k] + h;
+      }
+      om_set_coords(m, cur);
+    } while (!okay(m, o));
+    if (!stalled)
+      memcpy(tg->data, rem, sizeof(double) * n);
+    else if (exit_on_stall)
+      om_remove_vtag(m, "warp");
+    else
+      r = -2;
+    free(rem);
+  }
+  free(coords);
+  free(warp);
+  free(cur);
+  return r;
+}
+
 /* inheritance tags (OMEGA_H_INHERIT): one array per dimension 0..dim */
 static om_itag* find_itag(om_mesh* m, char const* name) {
   for (int t = 0; t < m->nitags; ++t)

@@ -98,6 +98,8 @@ void om_get_itag(om_mesh* m, char const* name, int d, double* out);
 void om_interpolate_metrics(int mdim, int n, double const* a, double const* b, double t,
     double* out);
 int om_approach_metric(om_mesh* m, om_adapt_opts const* o, double min_step);
+void om_set_coords(om_mesh* m, double const* coords);
+int om_warp_to_limit(om_mesh* m, om_adapt_opts const* o, int exit_on_stall, int max_niters);
 /* get_implied_metrics: nverts * (3 or 6) values */
 void om_get_implied_metrics(om_mesh* m, double* out);
 void om_element_implied_metric(int dim, double const* const* p, double* out);

@@ -1,0 +1,81 @@
+// The reference's idiom for moving a mesh while keeping it valid
+// (src/bend_test.cpp:55-65), written against the C++ facade:
+//
+//   mesh.add_tag(VERT, "warp", dim, ...);
+//   do { adapt(&mesh, opts); } while (warp_to_limit(&mesh, opts));
+//
+//   facade_warp <dim> <n> <ncomps> <metric.f64> <warp.f64> <out_prefix>
+//
+// writes <out_prefix>.counts (int32 nents per dimension, then the number of
+// warp steps), .verts (int32), .coords and .sizes (float64), and .dual
+// (int32 offsets then entries of Mesh::ask_dual).
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "Omega_h_amd.hpp"
+
+static std::vector<double> read_doubles(std::string const& path, std::size_t n) {
+  std::vector<double> v(n);
+  std::ifstream f(path, std::ios::binary);
+  f.read(reinterpret_cast<char*>(v.data()), std::streamsize(n * 8));
+  if (!f) throw std::runtime_error("cannot read " + path);
+  return v;
+}
+
+template <class T>
+static void write_file(std::string const& path, T const* data, std::size_t n) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<char const*>(data), std::streamsize(n * sizeof(T)));
+  if (!f) throw std::runtime_error("cannot write " + path);
+}
+
+int main(int argc, char** argv) {
+  if (argc != 7) {
+    std::cerr << "usage: facade_warp <dim> <n> <ncomps> <metric.f64> <warp.f64> <out>\n";
+    return 2;
+  }
+  int dim = std::atoi(argv[1]);
+  int n = std::atoi(argv[2]);
+  int ncomps = std::atoi(argv[3]);
+  std::string prefix = argv[6];
+  try {
+    Omega_h::Library lib(&argc, &argv);
+    Omega_h::Mesh mesh = Omega_h::build_box(lib.world(), OMEGA_H_SIMPLEX, 1.0, 1.0,
+        dim == 3 ? 1.0 : 0.0, n, n, dim == 3 ? n : 0);
+    std::size_t nv = std::size_t(mesh.nverts());
+    mesh.add_tag(Omega_h::VERT, "metric", ncomps,
+        Omega_h::Reals(Omega_h::Write<Omega_h::Real>(read_doubles(argv[4], nv * ncomps))));
+    mesh.add_tag(Omega_h::VERT, "warp", dim,
+        Omega_h::Reals(Omega_h::Write<Omega_h::Real>(read_doubles(argv[5], nv * dim))));
+    Omega_h::AdaptOpts opts(&mesh);
+    opts.verbosity = Omega_h::SILENT;
+    int steps = 0;
+    do {
+      Omega_h::adapt(&mesh, opts);
+      ++steps;
+    } while (Omega_h::warp_to_limit(&mesh, opts));
+    std::vector<int> counts;
+    for (int d = 0; d <= dim; ++d) counts.push_back(mesh.nents(d));
+    counts.push_back(steps);
+    write_file(prefix + ".counts", counts.data(), counts.size());
+    auto ev2v = mesh.ask_elem_verts();
+    write_file(prefix + ".verts", ev2v.data(), std::size_t(ev2v.size()));
+    auto coords = mesh.coords();
+    write_file(prefix + ".coords", coords.data(), std::size_t(coords.size()));
+    auto sizes = mesh.ask_sizes();
+    write_file(prefix + ".sizes", sizes.data(), std::size_t(sizes.size()));
+    auto dual = mesh.ask_dual();
+    std::vector<int> d(dual.a2ab.data(), dual.a2ab.data() + dual.a2ab.size());
+    d.insert(d.end(), dual.ab2b.data(), dual.ab2b.data() + dual.ab2b.size());
+    write_file(prefix + ".dual", d.data(), d.size());
+    std::printf("warped in %d steps: %d elements\n", steps, mesh.nelems());
+  } catch (Omega_h::exception const& e) {
+    std::cerr << "Omega_h::exception: " << e.what() << "\n";
+    return 1;
+  }
+  return 0;
+}

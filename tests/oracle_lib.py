@@ -97,6 +97,8 @@ def lib():
                                              C.POINTER(C.c_double), C.c_double,
                                              C.POINTER(C.c_double)]
         L.om_approach_metric.argtypes = [P, C.POINTER(AdaptOpts), C.c_double]
+        L.om_warp_to_limit.argtypes = [P, C.POINTER(AdaptOpts), C.c_int, C.c_int]
+        L.om_set_coords.argtypes = [P, C.POINTER(C.c_double)]
         L.om_get_implied_metrics.argtypes = [P, C.POINTER(C.c_double)]
         L.om_add_itag.argtypes = [P, C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]
         L.om_itag_ncomps.argtypes = [P, C.c_char_p]
@@ -292,6 +294,14 @@ class Mesh:
 
     def approach_metric(self, opts, min_step=1e-4):
         return lib().om_approach_metric(self.h, C.byref(opts), min_step)
+
+    def warp_to_limit(self, opts, exit_on_stall=False, max_niters=40):
+        return lib().om_warp_to_limit(self.h, C.byref(opts), int(exit_on_stall), max_niters)
+
+    def set_coords(self, coords):
+        a = np.ascontiguousarray(coords, dtype=np.float64)
+        assert a.size == self.nents(0) * self.dim
+        lib().om_set_coords(self.h, _ptr(a, C.c_double))
 
     def add_inherit_tag(self, name, ncomps, per_dim):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in per_dim]

@@ -80,3 +80,41 @@ def test_facade_approach_loop_matches_oracle(tmp_path):
     for name, want in (("coords", o.coords()), ("metric", o.metric())):
         got = np.fromfile(prefix + "." + name, dtype=np.float64)
         assert np.array_equal(got.view(np.int64), want.view(np.int64)), name
+
+
+@pytest.mark.gpu
+def test_facade_warp_loop_matches_oracle(tmp_path):
+    """do adapt while warp_to_limit (src/bend_test.cpp:55-65) through the
+    facade, then ask_sizes and ask_dual, against the oracle bit for bit."""
+    prog = os.path.join(ROOT, "omega_h_amd", "_build", "facade_warp")
+    assert os.path.exists(prog), "build the facade programs with __graft_entry__.build()"
+    o = O.Mesh.box(3, 3)
+    c = o.coords().reshape(-1, 3)
+    met = np.full(o.nents(0), 9.0)
+    w = np.zeros_like(c)
+    w[:, 0] = 2.0 * c[:, 2] ** 2
+    met.tofile(tmp_path / "m.f64")
+    w.tofile(tmp_path / "w.f64")
+    prefix = str(tmp_path / "out")
+    print("facade warp 3D n=3", flush=True)
+    r = subprocess.run([prog, "3", "3", "1", str(tmp_path / "m.f64"), str(tmp_path / "w.f64"),
+                        prefix], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    o.set_metric(met, 1)
+    o.add_vertex_tag("warp", 3, O.Mesh.LINEAR, w.ravel())
+    steps = 0
+    while True:
+        o.adapt(O.default_opts(3))
+        steps += 1
+        if o.warp_to_limit(O.default_opts(3)) != 1:
+            break
+    counts = np.fromfile(prefix + ".counts", dtype=np.int32)
+    assert list(counts[:-1]) == [o.nents(d) for d in range(4)]
+    assert counts[-1] == steps
+    assert np.array_equal(np.fromfile(prefix + ".verts", dtype=np.int32), o.verts(3))
+    for name, want in (("coords", o.coords()), ("sizes", o.sizes())):
+        got = np.fromfile(prefix + "." + name, dtype=np.float64)
+        assert np.array_equal(got.view(np.int64), want.view(np.int64)), name
+    a2ab, ab2b = o.dual()
+    assert np.array_equal(np.fromfile(prefix + ".dual", dtype=np.int32),
+                          np.concatenate([a2ab, ab2b]))

@@ -166,3 +166,75 @@ def test_inheritance_tags_identical(ctx, dim, n):
     g.adapt(ctx.default_opts(dim))
     for d in range(dim + 1):
         assert np.array_equal(o.inherit_tag("field", d), g.inherit_tag("field", d)), d
+
+
+def same_bits(a, b):
+    return np.array_equal(np.asarray(a).view(np.int64), np.asarray(b).view(np.int64))
+
+
+@pytest.mark.parametrize("dim,n", [(2, 8), (3, 3)])
+def test_warp_and_adapt_loop_identical(ctx, dim, n):
+    """do adapt while warp_to_limit (src/bend_test.cpp:55-65,
+    src/Omega_h_approach.cpp:36-81) for the shear x += 2 z^2: return codes,
+    coordinates, the warp left over and the mesh after every step, bit for bit."""
+    o = O.Mesh.box(dim, n)
+    g = A.Mesh.box(ctx, dim, n)
+    c = o.coords().reshape(-1, dim)
+    w = np.zeros_like(c)
+    w[:, 0] = 2.0 * c[:, dim - 1] ** 2
+    for m in (o, g):
+        m.set_metric(np.full(m.nents(0), float(n * n)), 1)
+        m.add_vertex_tag("warp", dim, 0, w.ravel())
+    steps = 0
+    while True:
+        o.adapt(O.default_opts(dim))
+        g.adapt(ctx.default_opts(dim))
+        ro = o.warp_to_limit(O.default_opts(dim))
+        rg = g.warp_to_limit(ctx.default_opts(dim))
+        assert ro == rg
+        assert np.array_equal(o.verts(dim), g.verts(dim))
+        assert same_bits(o.coords(), g.coords())
+        wo, wg = o.vertex_tag("warp"), g.vertex_tag("warp")
+        assert (wo is None) == (wg is None)
+        if wo is not None:
+            assert same_bits(wo, wg)
+        if ro == 0:
+            break
+        steps += 1
+        assert steps < 10
+    assert steps >= 3
+    assert same_bits(o.qualities(), g.qualities())
+
+
+def test_warp_to_limit_single_vertex_and_errors(ctx):
+    o = O.Mesh.box(2, 4)
+    g = A.Mesh.box(ctx, 2, 4)
+    c = o.coords().reshape(-1, 2)
+    v = int(np.argmin(np.abs(c - 0.5).sum(1)))
+    w = np.zeros_like(c)
+    w[v] = [0.375, 0.0]
+    for m in (o, g):
+        m.set_metric(np.full(m.nents(0), 16.0), 1)
+        m.add_vertex_tag("warp", 2, 0, w.ravel())
+    assert o.warp_to_limit(O.default_opts(2)) == g.warp_to_limit(ctx.default_opts(2)) == 1
+    assert same_bits(o.coords(), g.coords())
+    assert same_bits(o.vertex_tag("warp"), g.vertex_tag("warp"))
+    # exit_on_stall drops the warp where the last try left it
+    g = A.Mesh.box(ctx, 2, 4)
+    g.set_metric(np.full(g.nents(0), 16.0), 1)
+    g.add_vertex_tag("warp", 2, 0, w.ravel())
+    assert g.warp_to_limit(ctx.default_opts(2), exit_on_stall=True, max_niters=1) == 1
+    assert g.vertex_tag("warp") is None
+    assert g.coords().reshape(-1, 2)[v, 0] == 0.5 + 0.375 / 2
+    # without it a stall is an error, as is an unacceptable starting mesh
+    g.add_vertex_tag("warp", 2, 0, w.ravel())
+    g.set_coords(c.ravel())
+    with pytest.raises(A.OshError, match="warp step"):
+        g.warp_to_limit(ctx.default_opts(2), max_niters=1)
+    opts = ctx.default_opts(2)
+    opts.min_quality_allowed = 0.9
+    g.set_coords(c.ravel())
+    with pytest.raises(A.OshError, match="minimum element quality"):
+        g.warp_to_limit(opts)
+    assert g.warp_to_limit(ctx.default_opts(2)) == 1
+    assert g.vertex_tag("warp") is not None

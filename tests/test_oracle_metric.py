@@ -268,3 +268,102 @@ def test_inheritance_tags_follow_the_classification(dim, n):
     m.adapt(O.default_opts(dim))
     for d in range(dim + 1):
         assert np.array_equal(m.inherit_tag("cid", d), m.classification(d)[0].astype(np.float64))
+
+
+def center_vertex_mesh():
+    m = O.Mesh.box(2, 4)
+    m.set_metric(np.full(m.nents(0), 16.0), 1)
+    c = m.coords().reshape(-1, 2)
+    v = int(np.argmin(np.abs(c - 0.5).sum(1)))
+    return m, c, v
+
+
+def okay(m, opts):
+    return (m.qualities().min() >= opts.min_quality_allowed and
+            not m.lengths().max() > opts.max_length_allowed)
+
+
+def test_warp_to_limit_halves_until_acceptable():
+    # src/Omega_h_approach.cpp:36-81: the centre vertex of a 4x4 box is
+    # pushed 3/8 towards its neighbour; the move halves k times until the
+    # mesh is acceptable, and what was not moved stays in the tag (dyadic
+    # values, so every sum is exact)
+    m, c, v = center_vertex_mesh()
+    w = np.zeros_like(c)
+    w[v] = [0.375, 0.0]
+    m.add_vertex_tag("warp", 2, 0, w.ravel())
+    opts = O.default_opts(2)
+    assert m.warp_to_limit(opts) == 1
+    moved = m.coords().reshape(-1, 2) - c
+    k = int(round(math.log2(0.375 / moved[v, 0])))
+    assert k >= 1 and moved[v, 0] == 0.375 / 2 ** k and moved[v, 1] == 0.0
+    assert np.count_nonzero(moved) == 1
+    rem = m.vertex_tag("warp").reshape(-1, 2)
+    assert rem[v, 0] == 0.375 - 0.375 / 2 ** k and np.count_nonzero(rem) == 1
+    assert okay(m, opts)
+    # one halving fewer would not have been acceptable
+    p = O.Mesh.box(2, 4)
+    p.set_metric(np.full(p.nents(0), 16.0), 1)
+    q = c.copy()
+    q[v, 0] += 0.375 / 2 ** (k - 1)
+    p.set_coords(q.ravel())
+    assert not okay(p, opts)
+
+
+def test_warp_to_limit_translation_in_one_step():
+    m, c, _ = center_vertex_mesh()
+    w = np.tile([0.3, -0.7], (len(c), 1))
+    m.add_vertex_tag("warp", 2, 0, w.ravel())
+    assert m.warp_to_limit(O.default_opts(2)) == 1
+    assert np.array_equal(m.coords(), (c + w).ravel())
+    assert m.vertex_tag("warp") is None
+    assert m.warp_to_limit(O.default_opts(2)) == 0
+
+
+def test_warp_to_limit_stall_and_unacceptable_start():
+    m, c, v = center_vertex_mesh()
+    w = np.zeros_like(c)
+    w[v] = [0.375, 0.0]
+    m.add_vertex_tag("warp", 2, 0, w.ravel())
+    opts = O.default_opts(2)
+    # more halvings than allowed: the reference fails ...
+    assert m.warp_to_limit(opts, max_niters=1) == -2
+    # ... or, with exit_on_stall, drops the warp where the last try left it
+    m, c, v = center_vertex_mesh()
+    m.add_vertex_tag("warp", 2, 0, w.ravel())
+    assert m.warp_to_limit(opts, exit_on_stall=True, max_niters=1) == 1
+    assert m.vertex_tag("warp") is None
+    assert m.coords().reshape(-1, 2)[v, 0] == 0.5 + 0.375 / 2
+    opts.min_quality_allowed = 0.9
+    m, c, v = center_vertex_mesh()
+    m.add_vertex_tag("warp", 2, 0, w.ravel())
+    assert m.warp_to_limit(opts) == -1
+
+
+@pytest.mark.parametrize("dim,n", [(2, 8), (3, 3)])
+def test_warp_and_adapt_loop_finishes(dim, n):
+    # the reference's idiom, src/bend_test.cpp:55-65: do adapt while
+    # warp_to_limit, here for the shear x += 2 z^2 (y^2 in 2D)
+    m = O.Mesh.box(dim, n)
+    m.set_metric(np.full(m.nents(0), float(n * n)), 1)
+    c = m.coords().reshape(-1, dim)
+    w = np.zeros_like(c)
+    w[:, 0] = 2.0 * c[:, dim - 1] ** 2
+    m.add_vertex_tag("warp", dim, 0, w.ravel())
+    opts = O.default_opts(dim)
+    steps = 0
+    while True:
+        m.adapt(opts)
+        r = m.warp_to_limit(opts)
+        assert r in (0, 1)
+        if r == 0:
+            break
+        steps += 1
+        assert steps < 10
+    assert steps >= 3
+    assert m.qualities().min() >= opts.min_quality_allowed
+    # the shear moves along x only: the top face went to x in [2, 3]
+    x = m.coords().reshape(-1, dim)
+    top = x[:, dim - 1] == 1.0
+    assert top.any() and np.all(x[top, 0] >= 2.0) and np.all(x[top, 0] <= 3.0)
+    assert np.all((x[:, dim - 1] >= 0.0) & (x[:, dim - 1] <= 1.0))
