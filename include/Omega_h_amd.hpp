@@ -43,6 +43,8 @@ enum Verbosity { SILENT = 0, EACH_ADAPT = 1, EACH_REBUILD = 2, EXTRA_STATS = 3 }
 
 // transfer types of vertex tags carried through adapt() (src/Omega_h_adapt.hpp:30-42)
 enum Omega_h_Transfer { OMEGA_H_LINEAR_INTERP = 0, OMEGA_H_METRIC = 1 };
+// src/Omega_h_defines.h: how a metric is made isotropic
+enum Omega_h_Isotropy { OMEGA_H_ANISOTROPIC = 0, OMEGA_H_ISO_LENGTH = 1, OMEGA_H_ISO_SIZE = 2 };
 
 struct exception : public std::runtime_error {
   explicit exception(std::string const& msg) : std::runtime_error(msg) {}
@@ -530,6 +532,71 @@ inline bool warp_to_limit(Mesh* mesh, AdaptOpts const& opts, bool exit_on_stall 
   int did = 0;
   check_status(osh_warp_to_limit(mesh->handle(), &o, exit_on_stall ? 1 : 0, max_niters, &did));
   return did != 0;
+}
+
+// get_implied_metrics / get_implied_isos (src/Omega_h_metric.cpp:304-321)
+// and add_implied_metric_tag / add_implied_isos_tag
+// (src/Omega_h_metric_input.cpp:271-280)
+inline Reals get_implied_metrics(Mesh* mesh) {
+  Write<Real> w(mesh->nverts() * (mesh->dim() == 3 ? 6 : 3));
+  check_status(osh_get_implied_metrics(mesh->handle(), 0, w.data()));
+  return w;
+}
+inline Reals get_implied_isos(Mesh* mesh) {
+  Write<Real> w(mesh->nverts());
+  check_status(osh_get_implied_isos(mesh->handle(), 0, w.data()));
+  return w;
+}
+inline void add_implied_metric_tag(Mesh* mesh) {
+  check_status(osh_get_implied_metrics(mesh->handle(), 1, nullptr));
+}
+inline void add_implied_isos_tag(Mesh* mesh) {
+  check_status(osh_get_implied_isos(mesh->handle(), 1, nullptr));
+}
+
+// limit_metric_gradation(mesh, values, max_rate, tol), src/Omega_h_metric.cpp:
+// 199-220, for any vertex metric array (the mesh's metric is left as it was)
+inline Reals limit_metric_gradation(Mesh* mesh, Reals values, Real max_rate, Real tol = 1e-2,
+    bool verbose = true) {
+  (void)verbose;
+  LO nv = mesh->nverts();
+  if (nv == 0 || values.size() % nv != 0) throw exception("metric array has the wrong size");
+  int nc = values.size() / nv;
+  int had_nc;
+  check_status(osh_mesh_metric_ncomps(mesh->handle(), &had_nc));
+  Write<Real> had(nv * had_nc);
+  if (had_nc > 0) check_status(osh_mesh_get_metric(mesh->handle(), had.data()));
+  check_status(osh_mesh_set_metric(mesh->handle(), nc, values.data()));
+  int steps;
+  check_status(osh_limit_metric_gradation(mesh->handle(), max_rate, tol, &steps));
+  Write<Real> out(values.size());
+  check_status(osh_mesh_get_metric(mesh->handle(), out.data()));
+  if (had_nc > 0) check_status(osh_mesh_set_metric(mesh->handle(), had_nc, had.data()));
+  return out;
+}
+
+// fix, fix_for_given_metric, grade_fix_adapt (src/Omega_h_fix.cpp:90-176,
+// declared src/Omega_h_adapt.hpp:134-140)
+inline void fix(Mesh* mesh, AdaptOpts const& adapt_opts, Omega_h_Isotropy isotropy,
+    bool verbose) {
+  (void)verbose;
+  osh_adapt_opts o = adapt_opts.to_c();
+  check_status(osh_fix(mesh->handle(), &o, int(isotropy), nullptr));
+}
+inline void fix_for_given_metric(Mesh* mesh, AdaptOpts const& adapt_opts, bool verbose) {
+  (void)verbose;
+  osh_adapt_opts o = adapt_opts.to_c();
+  check_status(osh_fix_for_given_metric(mesh->handle(), &o, nullptr));
+}
+inline void grade_fix_adapt(Mesh* mesh, AdaptOpts const& opts, Reals target_metric,
+    bool verbose) {
+  (void)verbose;
+  LO nv = mesh->nverts();
+  if (nv == 0 || target_metric.size() % nv != 0)
+    throw exception("target metric has the wrong size");
+  osh_adapt_opts o = opts.to_c();
+  check_status(osh_grade_fix_adapt(mesh->handle(), &o, target_metric.size() / nv,
+      target_metric.data(), nullptr));
 }
 
 }  // namespace Omega_h

@@ -196,6 +196,32 @@ int osh_mesh_set_coords(osh_mesh* mesh, const double* data);
  * log-Euclidean averaging; out holds nverts * (3 in 2D, 6 in 3D) values, and
  * with set_as_metric != 0 it also becomes the mesh's metric */
 int osh_get_implied_metrics(osh_mesh* mesh, int set_as_metric, double* out);
+/* get_implied_isos(mesh), src/Omega_h_metric.cpp:316-321 (the implied metric
+ * made isotropic with the same volume; add_implied_isos_tag,
+ * src/Omega_h_metric_input.cpp:276-280, when set_as_metric != 0): nverts values */
+int osh_get_implied_isos(osh_mesh* mesh, int set_as_metric, double* out);
+/* compute_ill_metric(mesh, opts, isotropy), src/Omega_h_fix.cpp:16-79: the
+ * metric implied by the elements (isotropy 0 OMEGA_H_ANISOTROPIC, smoothed
+ * five times; 1 OMEGA_H_ISO_LENGTH, the largest eigenvalue; 2
+ * OMEGA_H_ISO_SIZE), without the elements below opts->min_quality_allowed,
+ * graded at rate 1, becomes the mesh's metric (a stage of osh_fix, static in
+ * the reference) */
+int osh_ill_metric(osh_mesh* mesh, const osh_adapt_opts* opts, int isotropy);
+/* fix_for_given_metric(mesh, opts, verbose), src/Omega_h_fix.cpp:90-118:
+ * adapt() with the allowed quality raised to the mesh's own minimum until
+ * that minimum stops improving by 1e-3; *ncalls = adapt() calls made */
+int osh_fix_for_given_metric(osh_mesh* mesh, const osh_adapt_opts* opts, int* ncalls);
+/* fix(mesh, opts, isotropy, verbose), src/Omega_h_fix.cpp:120-134:
+ * osh_ill_metric then osh_fix_for_given_metric */
+int osh_fix(osh_mesh* mesh, const osh_adapt_opts* opts, int isotropy, int* ncalls);
+/* grade_fix_adapt(mesh, opts, target_metric, verbose),
+ * src/Omega_h_fix.cpp:136-176: the target (nverts * ncomps values, ncomps 1
+ * or full) graded at rate 0.5 becomes "target_metric"; a mesh without a
+ * metric gets the implied one (isotropic for a scalar target); a mesh below
+ * the allowed quality is repaired by osh_fix; then
+ * while (approach_metric) adapt.  *nsteps = adapt() calls of that loop */
+int osh_grade_fix_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, int ncomps,
+    const double* target, int* nsteps);
 /* interpolate_between_metrics, src/Omega_h_metric.cpp:106-111, for n pairs of
  * metrics of dimension dim in host arrays */
 int osh_interpolate_metrics(osh_ctx* ctx, int dim, int n, const double* a, const double* b,

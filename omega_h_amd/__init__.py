@@ -112,6 +112,11 @@ def load(path=None):
         "osh_mesh_get_coords": [P, DP],
         "osh_mesh_set_coords": [P, DP],
         "osh_warp_to_limit": [P, C.POINTER(AdaptOpts), I, I, IP],
+        "osh_get_implied_isos": [P, I, DP],
+        "osh_ill_metric": [P, C.POINTER(AdaptOpts), I],
+        "osh_fix": [P, C.POINTER(AdaptOpts), I, IP],
+        "osh_fix_for_given_metric": [P, C.POINTER(AdaptOpts), IP],
+        "osh_grade_fix_adapt": [P, C.POINTER(AdaptOpts), I, DP, IP],
         "osh_mesh_set_metric": [P, I, DP],
         "osh_mesh_metric_ncomps": [P, IP],
         "osh_mesh_get_metric": [P, DP],
@@ -469,6 +474,37 @@ class Mesh:
         a = np.ascontiguousarray(coords, dtype=np.float64)
         assert a.size == self.nents(0) * self.dim
         check(self.lib.osh_mesh_set_coords(self.h, _p(a, C.c_double)), self.lib)
+
+    def implied_isos(self, set_as_metric=False):
+        """get_implied_isos (src/Omega_h_metric.cpp:316-321)."""
+        out = np.zeros(self.nents(0), dtype=np.float64)
+        check(self.lib.osh_get_implied_isos(self.h, int(set_as_metric), _p(out, C.c_double)),
+              self.lib)
+        return out
+
+    def ill_metric(self, opts, isotropy):
+        """compute_ill_metric (src/Omega_h_fix.cpp:16-79)."""
+        check(self.lib.osh_ill_metric(self.h, C.byref(opts), isotropy), self.lib)
+
+    def fix(self, opts, isotropy):
+        """fix (src/Omega_h_fix.cpp:120-134); the number of adapt() calls."""
+        n = C.c_int()
+        check(self.lib.osh_fix(self.h, C.byref(opts), isotropy, C.byref(n)), self.lib)
+        return n.value
+
+    def fix_for_given_metric(self, opts):
+        n = C.c_int()
+        check(self.lib.osh_fix_for_given_metric(self.h, C.byref(opts), C.byref(n)), self.lib)
+        return n.value
+
+    def grade_fix_adapt(self, opts, target, ncomps):
+        """grade_fix_adapt (src/Omega_h_fix.cpp:136-176); adapt() calls after repair."""
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        assert t.size == self.nents(0) * ncomps
+        n = C.c_int()
+        check(self.lib.osh_grade_fix_adapt(self.h, C.byref(opts), ncomps, _p(t, C.c_double),
+                                           C.byref(n)), self.lib)
+        return n.value
 
     def digest(self):
         """FNV-1a digest of the element vertex lists and coordinate bits."""

@@ -646,6 +646,77 @@ int osh_mesh_set_coords(osh_mesh* mesh, const double* data) {
   });
 }
 
+int osh_get_implied_isos(osh_mesh* mesh, int set_as_metric, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    auto& c = mesh->owner->ctx;
+    osh::Reals r = osh::implied_isos(c, mesh->m);
+    if (out) download(c, r, out);
+    if (set_as_metric) {
+      mesh->m.metric = r;
+      mesh->m.metric_ncomps = 1;
+      mesh->m.length = osh::Reals();
+      mesh->m.quality = osh::Reals();
+    }
+    c.sync();
+  });
+}
+
+int osh_ill_metric(osh_mesh* mesh, const osh_adapt_opts* opts, int isotropy) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(opts != nullptr, "null options");
+    osh::AdaptOptions o;
+    to_options(opts, o);
+    osh::ill_metric(mesh->owner->ctx, mesh->m, o, isotropy);
+    mesh->owner->ctx.sync();
+  });
+}
+
+int osh_fix(osh_mesh* mesh, const osh_adapt_opts* opts, int isotropy, int* ncalls) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(opts != nullptr, "null options");
+    osh::AdaptOptions o;
+    to_options(opts, o);
+    int r = osh::fix(mesh->owner->ctx, mesh->m, o, isotropy);
+    mesh->owner->ctx.sync();
+    if (ncalls) *ncalls = r;
+  });
+}
+
+int osh_fix_for_given_metric(osh_mesh* mesh, const osh_adapt_opts* opts, int* ncalls) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(opts != nullptr, "null options");
+    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
+    osh::AdaptOptions o;
+    to_options(opts, o);
+    int r = osh::fix_for_given_metric(mesh->owner->ctx, mesh->m, o);
+    mesh->owner->ctx.sync();
+    if (ncalls) *ncalls = r;
+  });
+}
+
+int osh_grade_fix_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, int ncomps,
+    const double* target, int* nsteps) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(opts != nullptr && target != nullptr, "null argument");
+    int dim = mesh->m.dim;
+    OSH_CHECK(ncomps == 1 || ncomps == (dim * (dim + 1)) / 2, "target metric size");
+    osh::AdaptOptions o;
+    to_options(opts, o);
+    auto& c = mesh->owner->ctx;
+    osh::LO n = mesh->m.nents[0] * ncomps;
+    osh::Reals t(c, n);
+    osh::upload(c, t, target, n);
+    int r = osh::grade_fix_adapt(c, mesh->m, o, ncomps, t);
+    c.sync();
+    if (nsteps) *nsteps = r;
+  });
+}
+
 int osh_get_implied_metrics(osh_mesh* mesh, int set_as_metric, double* out) {
   return guarded([&] {
     check_mesh(mesh);

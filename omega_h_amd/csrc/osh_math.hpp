@@ -504,6 +504,41 @@ OSH_HD void identity(Mat& a) {
   for (int i = 0; i < N; ++i) a.m[i][i] = 1.0;
 }
 
+// get_max_eigenvalues, src/Omega_h_eigen.cpp:7-40: the root of largest
+// magnitude of the characteristic polynomial (src/Omega_h_eigen.hpp:127-156,
+// tolerance 5e-5, the matrix not normalised); a 1x1 metric is its value
+template <int N>
+OSH_HD double max_eigenvalue(Mat const& m) {
+  if (N == 1) return m.m[0][0];
+  double roots[3] = {0.0, 0.0, 0.0};
+  int mults[3];
+  int n;
+  if (N == 3) {
+    double coeffs[3];
+    double tA = trace<3>(m);
+    Mat AA = matmul<3>(m, m);
+    coeffs[2] = -tA;
+    coeffs[1] = (1. / 2.) * ((tA * tA) - trace<3>(AA));
+    coeffs[0] = -det<3>(m);
+    n = roots3(coeffs, 5e-5, roots, mults);
+  } else {
+    double coeffs[2];
+    coeffs[1] = -trace<2>(m);
+    coeffs[0] = det<2>(m);
+    n = roots2(coeffs, 5e-5, roots, mults);
+  }
+  double out = roots[0];
+  double max_mag = math::fabs_(roots[0]);
+  for (int i = 1; i < n; ++i) {
+    double mag = math::fabs_(roots[i]);
+    if (mag > max_mag) {
+      max_mag = mag;
+      out = roots[i];
+    }
+  }
+  return out;
+}
+
 // decompose_eigen with max-norm scaling, src/Omega_h_eigen.hpp:228-319
 template <int N>
 OSH_HD void decompose_eigen(Mat const& m0, Mat& q, double* l) {

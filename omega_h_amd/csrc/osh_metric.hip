@@ -5,6 +5,12 @@
 //   limit_metric_gradation     src/Omega_h_metric.cpp:200-220
 //   are_close (arrays)         src/Omega_h_array_ops.cpp:75-87,
 //                              src/Omega_h_scalar.hpp:266-277
+// Repairing an initial mesh's quality and the grade-fix-adapt driver:
+//   apply_isotropy             src/Omega_h_metric.cpp:576-612,
+//                              src/Omega_h_eigen.cpp:7-40
+//   compute_ill_metric_dim     src/Omega_h_fix.cpp:16-68
+//   fix_for_given_metric, fix  src/Omega_h_fix.cpp:90-134
+//   grade_fix_adapt            src/Omega_h_fix.cpp:136-176
 #include "osh_metric.hpp"
 #include "osh_passes.hpp"
 
@@ -168,29 +174,43 @@ int limit_metric_gradation(Ctx& c, DMesh& m, Real max_rate, Real tol) {
   return steps;
 }
 
-Reals implied_metrics(Ctx& c, DMesh& m) {
+Reals element_implied_array(Ctx& c, DMesh& m, Real scalar) {
   int dim = m.dim;
   int nc = (dim * (dim + 1)) / 2;
   LO ne = m.nents[dim];
+  Reals e2m(c, ne * nc);
+  LOs const& ev2v = verts_of(c, m, dim);
+  if (dim == 3)
+    OSH_LAUNCH(element_implied_kernel<3>, ne, c, ev2v.data(), m.coords.data(), scalar, e2m.data());
+  else
+    OSH_LAUNCH(element_implied_kernel<2>, ne, c, ev2v.data(), m.coords.data(), scalar, e2m.data());
+  return e2m;
+}
+
+Reals project_metric_array(Ctx& c, DMesh& m, int mdim, Reals const& e2m) {
+  Adj const& v2e = ask_up(c, m, 0, m.dim);
+  int nc = (mdim * (mdim + 1)) / 2;
+  Reals out(c, m.nents[0] * nc);
+  if (mdim == 3)
+    OSH_LAUNCH(project_metrics_kernel<3>, m.nents[0], c, v2e.a2ab.data(), v2e.ab2b.data(),
+        e2m.data(), out.data());
+  else if (mdim == 2)
+    OSH_LAUNCH(project_metrics_kernel<2>, m.nents[0], c, v2e.a2ab.data(), v2e.ab2b.data(),
+        e2m.data(), out.data());
+  else
+    OSH_LAUNCH(project_metrics_kernel<1>, m.nents[0], c, v2e.a2ab.data(), v2e.ab2b.data(),
+        e2m.data(), out.data());
+  return out;
+}
+
+Reals implied_metrics(Ctx& c, DMesh& m) {
+  int dim = m.dim;
   // power(typical_unit_simplex_size / equilateral_simplex_size, 2, dim)
   double typical = dim == 3 ? 0.0838934100219 : 0.377645136635;
   double perfect = dim == 3 ? 0.1178511301977579 : 0.4330127018922193;
   double s = typical / perfect;
   double scalar = dim == 3 ? math::cbrt_(s * s) : s;
-  Reals e2m(c, ne * nc);
-  LOs const& ev2v = verts_of(c, m, dim);
-  Adj const& v2e = ask_up(c, m, 0, dim);
-  Reals out(c, m.nents[0] * nc);
-  if (dim == 3) {
-    OSH_LAUNCH(element_implied_kernel<3>, ne, c, ev2v.data(), m.coords.data(), scalar, e2m.data());
-    OSH_LAUNCH(project_metrics_kernel<3>, m.nents[0], c, v2e.a2ab.data(), v2e.ab2b.data(),
-        e2m.data(), out.data());
-  } else {
-    OSH_LAUNCH(element_implied_kernel<2>, ne, c, ev2v.data(), m.coords.data(), scalar, e2m.data());
-    OSH_LAUNCH(project_metrics_kernel<2>, m.nents[0], c, v2e.a2ab.data(), v2e.ab2b.data(),
-        e2m.data(), out.data());
-  }
-  return out;
+  return project_metric_array(c, m, dim, element_implied_array(c, m, scalar));
 }
 
 void interpolate_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals const& b, Real t,
@@ -202,6 +222,65 @@ void interpolate_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals cons
     OSH_LAUNCH(interpolate_kernel<2>, n, c, a.data(), b.data(), t, out.data());
   else
     OSH_LAUNCH(interpolate_kernel<1>, n, c, a.data(), b.data(), t, out.data());
+}
+
+// apply_isotropy of n metrics of dimension M: 1 (OMEGA_H_ISO_LENGTH) the
+// largest eigenvalue, 2 (OMEGA_H_ISO_SIZE) root<M>(det)
+// (src/Omega_h_metric.cpp:576-612)
+template <int M>
+__global__ void isotropy_kernel(LO n, Real const* in, int isotropy, Real* out) {
+  LO i = thread_index();
+  if (i >= n) return;
+  constexpr int nc = (M * (M + 1)) / 2;
+  Mat m = get_symm<M>(in + std::size_t(i) * nc);
+  if (isotropy == 1) {
+    out[i] = max_eigenvalue<M>(m);
+  } else {
+    double d = det<M>(m);
+    out[i] = (M == 3) ? math::cbrt_(d) : math::sqrt_(d);
+  }
+}
+
+// get_mident_metrics(mesh, dim, v2m, has_degen = true)
+// (src/Omega_h_metric.cpp:56-104): each element's vertex metrics averaged,
+// degenerate ones skipped (src/Omega_h_metric.hpp:125-149)
+template <int D, int M>
+__global__ void element_mident_kernel(LO ne, LO const* ev2v, Real const* v2m, Real* out) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  constexpr int nc = (M * (M + 1)) / 2;
+  Mat am;
+  mat_zero(am);
+  int n = 0;
+  for (int k = 0; k <= D; ++k) {
+    Mat vm = get_symm<M>(v2m + std::size_t(ev2v[e * (D + 1) + k]) * nc);
+    double mx = 0.0;
+    for (int cc = 0; cc < M; ++cc)
+      for (int r = 0; r < M; ++r) mx = math::max2(mx, math::fabs_(vm.m[cc][r]));
+    if (mx < kMetricEpsilon) continue;
+    Mat lm = log_spd<M>(vm);
+    for (int cc = 0; cc < M; ++cc)
+      for (int r = 0; r < M; ++r) am.m[cc][r] = am.m[cc][r] + lm.m[cc][r];
+    ++n;
+  }
+  Mat r = am;
+  if (n > 0) {
+    for (int cc = 0; cc < M; ++cc)
+      for (int rr = 0; rr < M; ++rr) am.m[cc][rr] = am.m[cc][rr] / n;
+    r = exp_spd<M>(am);
+  }
+  set_symm<M>(r, out + std::size_t(e) * nc);
+}
+
+// compute_ill_metric_dim (src/Omega_h_fix.cpp:40-48): the element metrics of
+// elements below the allowed quality stop contributing
+__global__ void zero_low_quality_kernel(LO ne, int nc, Real const* q, Real floor,
+    Real const* in, Real* out) {
+  LO e = thread_index();
+  if (e >= ne) return;
+  bool low = q[e] < floor;
+  for (int k = 0; k < nc; ++k)
+    out[std::size_t(e) * nc + k] = low ? 0. : in[std::size_t(e) * nc + k];
 }
 
 // the first move of warp_to_limit: coordinates + warp
@@ -226,6 +305,30 @@ __global__ void warp_halve_kernel(LO n, Real const* coords, Real const* warp_in,
 }
 
 namespace {
+
+// n metrics of dimension mdim made isotropic (a copy for isotropy 0 or
+// scalar metrics); *out_mdim is the dimension of the result
+Reals apply_isotropy_array(Ctx& c, int mdim, LO n, Reals const& in, int isotropy,
+    int* out_mdim) {
+  if (isotropy == 0 || mdim == 1) {
+    *out_mdim = mdim;
+    return in;
+  }
+  *out_mdim = 1;
+  Reals out(c, n);
+  if (mdim == 3)
+    OSH_LAUNCH(isotropy_kernel<3>, n, c, in.data(), isotropy, out.data());
+  else
+    OSH_LAUNCH(isotropy_kernel<2>, n, c, in.data(), isotropy, out.data());
+  return out;
+}
+
+void set_mesh_metric(DMesh& m, Reals const& v, int ncomps) {
+  m.metric = v;
+  m.metric_ncomps = ncomps;
+  m.length = Reals();
+  m.quality = Reals();
+}
 
 void replace_metric(DMesh& m, Reals const& v) {
   m.metric = v;
@@ -329,6 +432,101 @@ bool warp_to_limit(Ctx& c, DMesh& m, AdaptOptions const& o, bool exit_on_stall, 
   for (auto& t : m.vtags)
     if (t.name == "warp") t.data = rem;
   return true;
+}
+
+Reals implied_isos(Ctx& c, DMesh& m) {
+  int mdim;
+  return apply_isotropy_array(c, m.dim, m.nents[0], implied_metrics(c, m), 2, &mdim);
+}
+
+void ill_metric(Ctx& c, DMesh& m, AdaptOptions const& o, int isotropy) {
+  OSH_CHECK(isotropy >= 0 && isotropy <= 2, "isotropy must be 0, 1 or 2");
+  int dim = m.dim;
+  LO ne = m.nents[dim];
+  int mdim;
+  Reals elem = apply_isotropy_array(c, dim, ne, element_implied_array(c, m, 1.0), isotropy,
+      &mdim);
+  int nc = (mdim * (mdim + 1)) / 2;
+  Reals vm = project_metric_array(c, m, mdim, elem);
+  if (isotropy == 0) {
+    LOs const& ev2v = verts_of(c, m, dim);
+    for (int i = 0; i < 5; ++i) {
+      Reals mid(c, ne * nc);
+      if (dim == 3)
+        OSH_LAUNCH((element_mident_kernel<3, 3>), ne, c, ev2v.data(), vm.data(), mid.data());
+      else
+        OSH_LAUNCH((element_mident_kernel<2, 2>), ne, c, ev2v.data(), vm.data(), mid.data());
+      vm = project_metric_array(c, m, mdim, mid);
+    }
+  }
+  set_mesh_metric(m, vm, nc);
+  Reals const& q = ask_qualities(c, m);
+  Reals kept(c, ne * nc);
+  OSH_LAUNCH(zero_low_quality_kernel, ne, c, nc, q.data(), o.min_quality_allowed, elem.data(),
+      kept.data());
+  set_mesh_metric(m, project_metric_array(c, m, mdim, kept), nc);
+  limit_metric_gradation(c, m, 1.0, 1e-2);
+}
+
+int fix_for_given_metric(Ctx& c, DMesh& m, AdaptOptions const& o) {
+  Real minqual = min_real(c, ask_qualities(c, m));
+  Real maxlen = max_real(c, ask_lengths(c, m));
+  AdaptOptions opts = o;
+  int calls = 0;
+  while (true) {
+    Real minqual_old = minqual;
+    opts.min_quality_allowed = minqual;
+    Real twice = opts.max_length_desired * 2.0;
+    opts.max_length_allowed = (twice > maxlen) ? twice : maxlen;
+    opts.nsliver_layers = 10;
+    Real want = minqual + 0.1;
+    opts.min_quality_desired = (1.0 < want) ? 1.0 : want;
+    adapt(c, m, opts, nullptr);
+    ++calls;
+    minqual = min_real(c, ask_qualities(c, m));
+    maxlen = max_real(c, ask_lengths(c, m));
+    if (minqual < minqual_old + 1e-3) break;
+  }
+  return calls;
+}
+
+int fix(Ctx& c, DMesh& m, AdaptOptions const& o, int isotropy) {
+  ill_metric(c, m, o, isotropy);
+  return fix_for_given_metric(c, m, o);
+}
+
+int grade_fix_adapt(Ctx& c, DMesh& m, AdaptOptions const& o, int ncomps, Reals const& target) {
+  int dim = m.dim;
+  OSH_CHECK(ncomps == 1 || ncomps == (dim * (dim + 1)) / 2, "target metric size");
+  OSH_CHECK(target.size() == m.nents[0] * ncomps, "target metric size");
+  int metric_dim = (ncomps == 1) ? 1 : dim;
+  // limit_metric_gradation(mesh, target, 0.5) over the mesh's own star
+  Reals had = m.metric;
+  int had_ncomps = m.metric_ncomps;
+  set_mesh_metric(m, target, ncomps);
+  limit_metric_gradation(c, m, 0.5, 1e-2);
+  Reals graded = m.metric;
+  bool found = false;
+  for (auto& t : m.vtags)
+    if (t.name == "target_metric") {
+      t = {"target_metric", ncomps, 1, graded};
+      found = true;
+    }
+  if (!found) m.vtags.push_back({"target_metric", ncomps, 1, graded});
+  if (had.exists())
+    set_mesh_metric(m, had, had_ncomps);
+  else if (metric_dim == 1)
+    set_mesh_metric(m, implied_isos(c, m), 1);
+  else
+    set_mesh_metric(m, implied_metrics(c, m), (dim * (dim + 1)) / 2);
+  if (min_real(c, ask_qualities(c, m)) < o.min_quality_allowed)
+    fix(c, m, o, (metric_dim == 1) ? 1 : 0);
+  int steps = 0;
+  while (approach_metric(c, m, o, 1e-4)) {
+    adapt(c, m, o, nullptr);
+    ++steps;
+  }
+  return steps;
 }
 
 void intersect_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals const& b,

@@ -112,5 +112,14 @@ void set_coords(DMesh& m, Reals const& v);
 // warp_to_limit (src/Omega_h_approach.cpp:36-81): move the vertices by the
 // "warp" vertex tag as far as quality and lengths allow; false without one
 bool warp_to_limit(Ctx& c, DMesh& m, AdaptOptions const& o, bool exit_on_stall, int max_niters);
+// get_implied_isos (src/Omega_h_metric.cpp:316-321): nverts values
+Reals implied_isos(Ctx& c, DMesh& m);
+// compute_ill_metric_dim, fix_for_given_metric, fix, grade_fix_adapt
+// (src/Omega_h_fix.cpp:16-176); isotropy 0 anisotropic, 1 iso length,
+// 2 iso size; the fix functions return the number of adapt() calls
+void ill_metric(Ctx& c, DMesh& m, AdaptOptions const& o, int isotropy);
+int fix_for_given_metric(Ctx& c, DMesh& m, AdaptOptions const& o);
+int fix(Ctx& c, DMesh& m, AdaptOptions const& o, int isotropy);
+int grade_fix_adapt(Ctx& c, DMesh& m, AdaptOptions const& o, int ncomps, Reals const& target);
 
 }  // namespace osh

@@ -204,7 +204,7 @@ void om_interpolate_metrics(int mdim, int n, double const* a, double const* b, d
   }
 }
 
-static double min_quality(om_mesh* m) {
+double om_min_quality(om_mesh* m) {
   int ne = m->nents[m->dim];
   double* q = (double*)om_malloc(sizeof(double) * (size_t)(ne + 1));
   om_get_qualities(m, q);
@@ -214,7 +214,7 @@ static double min_quality(om_mesh* m) {
   return r;
 }
 
-static double max_length(om_mesh* m) {
+double om_max_length(om_mesh* m) {
   int ne = m->nents[1];
   double* l = (double*)om_malloc(sizeof(double) * (size_t)(ne + 1));
   om_get_lengths(m, l);
@@ -225,7 +225,7 @@ static double max_length(om_mesh* m) {
 }
 
 static int okay(om_mesh* m, om_adapt_opts const* o) {
-  return min_quality(m) >= o->min_quality_allowed && !(max_length(m) > o->max_length_allowed);
+  return om_min_quality(m) >= o->min_quality_allowed && !(om_max_length(m) > o->max_length_allowed);
 }
 
 /* approach_metric, src/Omega_h_approach.cpp:83-126: move "metric" toward the
@@ -236,8 +236,8 @@ static int okay(om_mesh* m, om_adapt_opts const* o) {
 int om_approach_metric(om_mesh* m, om_adapt_opts const* o, double min_step) {
   om_vtag* tg = find_vtag(m, "target_metric");
   if (!tg) return 0;
-  if (min_quality(m) < o->min_quality_allowed) return -1;
-  if (max_length(m) > o->max_length_allowed) return -1;
+  if (om_min_quality(m) < o->min_quality_allowed) return -1;
+  if (om_max_length(m) > o->max_length_allowed) return -1;
   int nc = m->metric_ncomps;
   if (tg->ncomps != nc) return -1;
   int nv = m->nents[0];

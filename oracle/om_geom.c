@@ -300,6 +300,41 @@ static void decompose_dim(int dim, om_mat const* m, om_mat* q, double* l) {
   }
 }
 
+/* get_max_eigenvalues, src/Omega_h_eigen.cpp:7-40: the root of largest
+ * magnitude of the characteristic polynomial (src/Omega_h_eigen.hpp:127-156,
+ * tolerance 5e-5, the matrix not normalised); a 1x1 metric is its value */
+double om_max_eigenvalue(int dim, om_mat const* m) {
+  if (dim == 1) return m->m[0][0];
+  double roots[3] = {0.0, 0.0, 0.0};
+  int mults[3];
+  int n;
+  if (dim == 3) {
+    double coeffs[3];
+    double tA = trace(3, m);
+    om_mat AA;
+    matmul(3, m, m, &AA);
+    coeffs[2] = -tA;
+    coeffs[1] = (1. / 2.) * ((tA * tA) - trace(3, &AA));
+    coeffs[0] = -om_det(3, m);
+    n = om_roots3(coeffs, 5e-5, roots, mults);
+  } else {
+    double coeffs[2];
+    coeffs[1] = -trace(2, m);
+    coeffs[0] = om_det(2, m);
+    n = om_roots2(coeffs, 5e-5, roots, mults);
+  }
+  double out = roots[0];
+  double max_mag = fabs(roots[0]);
+  for (int i = 1; i < n; ++i) {
+    double mag = fabs(roots[i]);
+    if (mag > max_mag) {
+      max_mag = mag;
+      out = roots[i];
+    }
+  }
+  return out;
+}
+
 void om_decompose_eigen(int dim, om_mat const* m0, om_mat* q, double* l) {
   double nm = max_norm(dim, m0);
   if (nm <= OM_EPSILON) {
@@ -513,6 +548,12 @@ void om_average_metric(int mdim, int n, double const* const* ms, double* out) {
 }
 
 /* ---- known-answer hooks ---- */
+
+double om_kat_max_eigenvalue(int dim, double const* symm) {
+  om_mat m;
+  om_get_symm(dim, symm, &m);
+  return om_max_eigenvalue(dim, &m);
+}
 
 int om_kat_roots3(double const* coeffs, double eps, double* roots, int* mults) {
   return om_roots3(coeffs, eps, roots, mults);

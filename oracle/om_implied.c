@@ -137,15 +137,14 @@ static double max_norm(int n, om_mat const* a) {
   return x;
 }
 
-/* get_implied_metrics: project the size-scaled element implied metrics to
- * the vertices by log-Euclidean averaging over each vertex's elements */
-void om_get_implied_metrics(om_mesh* m, double* out) {
+/* element_implied_length_metrics (src/Omega_h_metric.cpp:257-276) times a
+ * scalar (1 for length metrics, the size scaling of
+ * get_element_implied_size_metrics, src/Omega_h_metric.cpp:295-301) */
+void om_element_implied_metrics(om_mesh* m, double scalar, double* e2m) {
   int dim = m->dim;
   int nc = om_symm_ncomps(dim);
   int ne = m->nents[dim];
   int const* ev2v = om_verts_of(m, dim);
-  double scalar = typical_over_perfect_metric_scalar(dim);
-  double* e2m = (double*)om_malloc(sizeof(double) * (size_t)(ne + 1) * nc);
   for (int e = 0; e < ne; ++e) {
     double const* p[4];
     for (int k = 0; k <= dim; ++k) p[k] = m->coords + (size_t)ev2v[e * (dim + 1) + k] * dim;
@@ -153,6 +152,15 @@ void om_get_implied_metrics(om_mesh* m, double* out) {
     om_element_implied_metric(dim, p, v);
     for (int k = 0; k < nc; ++k) e2m[(size_t)e * nc + k] = v[k] * scalar;
   }
+}
+
+/* project_metrics (src/Omega_h_metric.cpp:221-252): log-Euclidean average
+ * of the metrics (dimension mdim) of each vertex's elements, degenerate
+ * metrics skipped (average_metric_contrib/_finish with has_degen,
+ * src/Omega_h_metric.hpp:125-139) */
+void om_project_metrics(om_mesh* m, int mdim, double const* e2m, double* out) {
+  int dim = m->dim;
+  int nc = om_symm_ncomps(mdim);
   om_graph const* v2e = om_ask_up(m, 0, dim);
   for (int v = 0; v < m->nents[0]; ++v) {
     om_mat am;
@@ -160,22 +168,65 @@ void om_get_implied_metrics(om_mesh* m, double* out) {
     int n = 0;
     for (int j = v2e->a2ab[v]; j < v2e->a2ab[v + 1]; ++j) {
       om_mat em, lm;
-      om_get_symm(dim, e2m + (size_t)v2e->ab2b[j] * nc, &em);
-      if (max_norm(dim, &em) < OM_EPSILON) continue;
-      om_log_spd(dim, &em, &lm);
-      for (int c = 0; c < dim; ++c)
-        for (int r = 0; r < dim; ++r) am.m[c][r] = am.m[c][r] + lm.m[c][r];
+      om_get_symm(mdim, e2m + (size_t)v2e->ab2b[j] * nc, &em);
+      if (max_norm(mdim, &em) < OM_EPSILON) continue;
+      om_log_spd(mdim, &em, &lm);
+      for (int c = 0; c < mdim; ++c)
+        for (int r = 0; r < mdim; ++r) am.m[c][r] = am.m[c][r] + lm.m[c][r];
       ++n;
     }
     om_mat r;
     if (n == 0) {
       r = am;
     } else {
-      for (int c = 0; c < dim; ++c)
-        for (int rr = 0; rr < dim; ++rr) am.m[c][rr] = am.m[c][rr] / n;
-      om_exp_spd(dim, &am, &r);
+      for (int c = 0; c < mdim; ++c)
+        for (int rr = 0; rr < mdim; ++rr) am.m[c][rr] = am.m[c][rr] / n;
+      om_exp_spd(mdim, &am, &r);
     }
-    om_set_symm(dim, &r, out + (size_t)v * nc);
+    om_set_symm(mdim, &r, out + (size_t)v * nc);
   }
+}
+
+/* get_mident_metrics(mesh, dim, v2m, has_degen = true),
+ * src/Omega_h_metric.cpp:56-104: the average of each element's vertex
+ * metrics, degenerate ones skipped */
+void om_element_mident_metrics(om_mesh* m, int mdim, double const* v2m, double* out) {
+  int dim = m->dim;
+  int nc = om_symm_ncomps(mdim);
+  int const* ev2v = om_verts_of(m, dim);
+  for (int e = 0; e < m->nents[dim]; ++e) {
+    om_mat am;
+    memset(&am, 0, sizeof(am));
+    int n = 0;
+    for (int k = 0; k <= dim; ++k) {
+      om_mat vm, lm;
+      om_get_symm(mdim, v2m + (size_t)ev2v[e * (dim + 1) + k] * nc, &vm);
+      if (max_norm(mdim, &vm) < OM_EPSILON) continue;
+      om_log_spd(mdim, &vm, &lm);
+      for (int c = 0; c < mdim; ++c)
+        for (int r = 0; r < mdim; ++r) am.m[c][r] = am.m[c][r] + lm.m[c][r];
+      ++n;
+    }
+    om_mat r;
+    if (n == 0) {
+      r = am;
+    } else {
+      for (int c = 0; c < mdim; ++c)
+        for (int rr = 0; rr < mdim; ++rr) am.m[c][rr] = am.m[c][rr] / n;
+      om_exp_spd(mdim, &am, &r);
+    }
+    om_set_symm(mdim, &r, out + (size_t)e * nc);
+  }
+}
+
+/* get_implied_metrics: project the size-scaled element implied metrics to
+ * the vertices by log-Euclidean averaging over each vertex's elements */
+void om_get_implied_metrics(om_mesh* m, double* out) {
+  int dim = m->dim;
+  int nc = om_symm_ncomps(dim);
+  int ne = m->nents[dim];
+  double* e2m = (double*)om_malloc(sizeof(double) * (size_t)(ne + 1) * nc);
+  om_element_implied_metrics(m, typical_over_perfect_metric_scalar(dim), e2m);
+  om_project_metrics(m, dim, e2m, out);
   free(e2m);
 }

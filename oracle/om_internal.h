@@ -29,6 +29,10 @@ typedef struct {
 
 double om_det(int dim, om_mat const* m);
 void om_decompose_eigen(int dim, om_mat const* m, om_mat* q, double* l);
+double om_max_eigenvalue(int dim, om_mat const* m);
+void om_element_implied_metrics(om_mesh* m, double scalar, double* e2m);
+void om_project_metrics(om_mesh* m, int mdim, double const* e2m, double* out);
+void om_element_mident_metrics(om_mesh* m, int mdim, double const* v2m, double* out);
 void om_compose_ortho(int dim, om_mat const* q, double const* l, om_mat* out);
 void om_log_spd(int dim, om_mat const* m, om_mat* out);
 void om_exp_spd(int dim, om_mat const* m, om_mat* out);
@@ -49,6 +53,8 @@ void om_intersect_metrics(int dim, double const* m1, double const* m2, double* o
 void om_limit_gradation_once(int sdim, int mdim, int nverts, int const* a2ab, int const* ab2b,
     double const* coords, double const* in, double max_rate, double* out);
 int om_are_close(int n, double const* a, double const* b, double tol, double floor);
+double om_min_quality(om_mesh* m);
+double om_max_length(om_mesh* m);
 
 /* growable int arrays */
 typedef struct {

@@ -85,6 +85,7 @@ double om_kat_edge_length(int sdim, int mdim, double const* p0,
     double const* p1, double const* m0, double const* m1);
 double om_kat_aniso_length(double la, double lb);
 void om_kat_average_metric(int mdim, int n, double const* ms, double* out);
+double om_kat_max_eigenvalue(int dim, double const* symm);
 void om_kat_intersect(int dim, double const* m1, double const* m2, double* out);
 /* vertex tags carried through adapt(): kind 0 linear interpolation, 1 metric */
 int om_add_vtag(om_mesh* m, char const* name, int ncomps, int kind, double const* data);
@@ -102,6 +103,14 @@ void om_set_coords(om_mesh* m, double const* coords);
 int om_warp_to_limit(om_mesh* m, om_adapt_opts const* o, int exit_on_stall, int max_niters);
 /* get_implied_metrics: nverts * (3 or 6) values */
 void om_get_implied_metrics(om_mesh* m, double* out);
+/* get_implied_isos: nverts values */
+void om_get_implied_isos(om_mesh* m, double* out);
+/* compute_ill_metric / fix_for_given_metric / fix / grade_fix_adapt
+ * (isotropy 0 anisotropic, 1 iso length, 2 iso size) */
+void om_ill_metric(om_mesh* m, om_adapt_opts const* o, int isotropy);
+int om_fix_for_given_metric(om_mesh* m, om_adapt_opts const* o);
+int om_fix(om_mesh* m, om_adapt_opts const* o, int isotropy);
+int om_grade_fix_adapt(om_mesh* m, om_adapt_opts const* o, int ncomps, double const* target);
 void om_element_implied_metric(int dim, double const* const* p, double* out);
 /* limit_metric_gradation on the mesh's metric; returns the sweep count */
 int om_limit_metric_gradation(om_mesh* m, double max_rate, double tol);

@@ -98,6 +98,14 @@ def lib():
                                              C.POINTER(C.c_double)]
         L.om_approach_metric.argtypes = [P, C.POINTER(AdaptOpts), C.c_double]
         L.om_warp_to_limit.argtypes = [P, C.POINTER(AdaptOpts), C.c_int, C.c_int]
+        L.om_get_implied_isos.argtypes = [P, C.POINTER(C.c_double)]
+        L.om_kat_max_eigenvalue.argtypes = [C.c_int, C.POINTER(C.c_double)]
+        L.om_kat_max_eigenvalue.restype = C.c_double
+        L.om_ill_metric.argtypes = [P, C.POINTER(AdaptOpts), C.c_int]
+        L.om_fix_for_given_metric.argtypes = [P, C.POINTER(AdaptOpts)]
+        L.om_fix.argtypes = [P, C.POINTER(AdaptOpts), C.c_int]
+        L.om_grade_fix_adapt.argtypes = [P, C.POINTER(AdaptOpts), C.c_int,
+                                         C.POINTER(C.c_double)]
         L.om_set_coords.argtypes = [P, C.POINTER(C.c_double)]
         L.om_get_implied_metrics.argtypes = [P, C.POINTER(C.c_double)]
         L.om_add_itag.argtypes = [P, C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]
@@ -303,6 +311,25 @@ class Mesh:
         assert a.size == self.nents(0) * self.dim
         lib().om_set_coords(self.h, _ptr(a, C.c_double))
 
+    @property
+    def metric_ncomps(self):
+        return lib().om_metric_ncomps(self.h)
+
+    def implied_isos(self):
+        out = np.zeros(self.nents(0), dtype=np.float64)
+        lib().om_get_implied_isos(self.h, _ptr(out, C.c_double))
+        return out
+
+    def ill_metric(self, opts, isotropy):
+        lib().om_ill_metric(self.h, C.byref(opts), isotropy)
+
+    def fix(self, opts, isotropy):
+        return lib().om_fix(self.h, C.byref(opts), isotropy)
+
+    def grade_fix_adapt(self, opts, target, ncomps):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        return lib().om_grade_fix_adapt(self.h, C.byref(opts), ncomps, _ptr(t, C.c_double))
+
     def add_inherit_tag(self, name, ncomps, per_dim):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in per_dim]
         ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
@@ -421,3 +448,9 @@ def vector2symm(v):
     if len(v) == 3:
         return np.array([[v[0], v[2]], [v[2], v[1]]])
     return np.array([[v[0], v[3], v[5]], [v[3], v[1], v[4]], [v[5], v[4], v[2]]])
+
+
+def max_eigenvalue(dim, symm):
+    """get_max_eigenvalues of one metric (src/Omega_h_eigen.cpp:7-40)."""
+    a = np.ascontiguousarray(symm, dtype=np.float64)
+    return lib().om_kat_max_eigenvalue(dim, _ptr(a, C.c_double))

@@ -238,3 +238,72 @@ def test_warp_to_limit_single_vertex_and_errors(ctx):
         g.warp_to_limit(opts)
     assert g.warp_to_limit(ctx.default_opts(2)) == 1
     assert g.vertex_tag("warp") is not None
+
+
+def jittered_pair(ctx, dim, n, amp, seed):
+    """The same jittered box (interior vertices moved by up to amp / n per
+    axis) on both sides."""
+    o = O.Mesh.box(dim, n)
+    g = A.Mesh.box(ctx, dim, n)
+    c = o.coords().reshape(-1, dim).copy()
+    inner = np.all((c > 1e-12) & (c < 1 - 1e-12), axis=1)
+    c[inner] += np.random.default_rng(seed).uniform(-amp, amp, (inner.sum(), dim)) / n
+    o.set_coords(c.ravel())
+    g.set_coords(c.ravel())
+    return o, g
+
+
+def assert_meshes_same(o, g, dim):
+    assert [o.nents(d) for d in range(dim + 1)] == [g.nents(d) for d in range(dim + 1)]
+    assert np.array_equal(o.verts(dim), g.verts(dim))
+    assert same_bits(o.coords(), g.coords())
+    assert o.metric_ncomps == g.metric_ncomps
+    assert same_bits(o.metric(), g.metric())
+
+
+@pytest.mark.parametrize("dim,n,amp,seed", [(2, 8, 0.0, 0), (2, 8, 0.4, 2), (3, 4, 0.35, 3)])
+def test_implied_isos_and_ill_metric_identical(ctx, dim, n, amp, seed):
+    """get_implied_isos and compute_ill_metric (src/Omega_h_metric.cpp:316-321,
+    src/Omega_h_fix.cpp:16-79) for both isotropies, bit for bit."""
+    o, g = jittered_pair(ctx, dim, n, amp, seed)
+    assert same_bits(o.implied_isos(), g.implied_isos())
+    for iso in (0, 1):
+        o, g = jittered_pair(ctx, dim, n, amp, seed)
+        o.ill_metric(O.default_opts(dim), iso)
+        g.ill_metric(ctx.default_opts(dim), iso)
+        assert_meshes_same(o, g, dim)
+        assert same_bits(o.qualities(), g.qualities())
+
+
+@pytest.mark.parametrize("dim,n,amp,seed,iso", [(2, 8, 0.4, 2, 0), (2, 8, 0.4, 2, 1),
+                                                (3, 4, 0.35, 3, 0), (3, 4, 0.35, 3, 1)])
+def test_fix_identical(ctx, dim, n, amp, seed, iso):
+    """fix (src/Omega_h_fix.cpp:90-134): the ill metric, then adapt() with the
+    allowed quality raised, call by call the same."""
+    o, g = jittered_pair(ctx, dim, n, amp, seed)
+    co = o.fix(O.default_opts(dim), iso)
+    cg = g.fix(ctx.default_opts(dim), iso)
+    assert co == cg >= 1
+    assert_meshes_same(o, g, dim)
+
+
+@pytest.mark.parametrize("dim,n,amp,seed,nc", [(2, 8, 0.4, 2, 1), (2, 8, 0.4, 2, 3),
+                                               (3, 4, 0.35, 3, 1), (3, 4, 0.35, 3, 6)])
+def test_grade_fix_adapt_identical(ctx, dim, n, amp, seed, nc):
+    """grade_fix_adapt (src/Omega_h_fix.cpp:136-176) from a jittered box with no
+    metric, to a banded target: the repair and the approach loop, bit for bit."""
+    o, g = jittered_pair(ctx, dim, n, amp, seed)
+    c = o.coords().reshape(-1, dim)
+    h = 0.5 / n + 0.5 * np.abs(c[:, 0] - 0.5) / n
+    if nc == 1:
+        target = 1.0 / h ** 2
+    else:
+        t = np.zeros((len(c), nc))
+        t[:, :dim] = (1.0 / h ** 2)[:, None]
+        t[:, 0] *= 2.0
+        target = t.ravel()
+    so = o.grade_fix_adapt(O.default_opts(dim), target, nc)
+    sg = g.grade_fix_adapt(ctx.default_opts(dim), target, nc)
+    assert so == sg >= 1
+    assert_meshes_same(o, g, dim)
+    assert g.vertex_tag("target_metric") is None

@@ -367,3 +367,109 @@ def test_warp_and_adapt_loop_finishes(dim, n):
     top = x[:, dim - 1] == 1.0
     assert top.any() and np.all(x[top, 0] >= 2.0) and np.all(x[top, 0] <= 3.0)
     assert np.all((x[:, dim - 1] >= 0.0) & (x[:, dim - 1] <= 1.0))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_max_eigenvalue_against_numpy(dim):
+    # get_max_eigenvalues (src/Omega_h_eigen.cpp:7-40): the root of largest
+    # magnitude of the characteristic polynomial
+    rng = np.random.default_rng(7)
+    checked = 0
+    for _ in range(300):
+        q, _ = np.linalg.qr(rng.normal(size=(dim, dim)))
+        lam = np.exp(rng.uniform(0, 4, dim))
+        if np.min(np.abs(np.subtract.outer(lam, lam))[~np.eye(dim, dtype=bool)]) < 0.01:
+            continue
+        a = q @ np.diag(lam) @ q.T
+        got = O.max_eigenvalue(dim, O.symm2vector(a))
+        assert abs(got - lam.max()) <= 1e-7 * lam.max()
+        checked += 1
+    assert checked > 200
+    # the tolerance 5e-5 is absolute and the matrix is not normalised: small
+    # eigenvalues closer than it merge into their mean (2D: |disc| < 5e-5)
+    if dim == 2:
+        a = np.diag([0.0233376, 0.0215307])
+        assert O.max_eigenvalue(2, O.symm2vector(a)) == pytest.approx(0.02243415, 1e-9)
+    # equal eigenvalues: the double and triple root branches
+    assert O.max_eigenvalue(dim, O.symm2vector(np.eye(dim) * 3.0)) == pytest.approx(3.0, 1e-12)
+    if dim == 3:
+        a = np.diag([2.0, 5.0, 5.0])
+        assert O.max_eigenvalue(3, O.symm2vector(a)) == pytest.approx(5.0, 1e-12)
+
+
+@pytest.mark.parametrize("dim,n", [(2, 6), (3, 3)])
+def test_implied_isos_keep_the_volume(dim, n):
+    # get_implied_isos (src/Omega_h_metric.cpp:316-321): root<dim>(det) of
+    # the implied metric
+    m = O.Mesh.box(dim, n)
+    full = m.implied_metrics().reshape(m.nents(0), -1)
+    dets = np.array([np.linalg.det(O.vector2symm(v)) for v in full])
+    want = np.cbrt(dets) if dim == 3 else np.sqrt(dets)
+    assert np.allclose(m.implied_isos(), want, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("n", [3, 8])
+def test_ill_metric_of_the_uniform_2d_box(n):
+    # every triangle of the 2D box is a right isoceles triangle with legs 1/n
+    # and the same diagonal, so each implies M = n^2 [[1, -1/2], [-1/2, 1]]
+    # (unit edges (1/n, 0), (0, 1/n), (1/n, 1/n)); projection, smoothing and
+    # grading keep a uniform metric, and every element is perfect in it
+    opts = O.default_opts(2)
+    m = O.Mesh.box(2, n)
+    m.ill_metric(opts, 0)
+    want = np.tile([n * n, n * n, -0.5 * n * n], m.nents(0))
+    assert np.allclose(m.metric(), want, rtol=1e-12, atol=1e-12 * n * n)
+    assert np.allclose(m.qualities(), 1.0, rtol=1e-12)
+    # isotropic by length: the largest eigenvalue, 3/2 n^2
+    m = O.Mesh.box(2, n)
+    m.ill_metric(opts, 1)
+    assert m.metric_ncomps == 1
+    assert np.allclose(m.metric(), 1.5 * n * n, rtol=1e-12)
+
+
+def jittered_box(dim, n, amp, seed):
+    """A box whose interior vertices moved by up to amp / n per axis."""
+    m = O.Mesh.box(dim, n)
+    c = m.coords().reshape(-1, dim).copy()
+    inner = np.all((c > 1e-12) & (c < 1 - 1e-12), axis=1)
+    c[inner] += np.random.default_rng(seed).uniform(-amp, amp, (inner.sum(), dim)) / n
+    m.set_coords(c.ravel())
+    assert m.sizes().min() > 0
+    return m
+
+
+@pytest.mark.parametrize("dim,n,amp,seed,iso", [(2, 8, 0.4, 2, 0), (2, 8, 0.4, 2, 1),
+                                                (3, 4, 0.35, 3, 1)])
+def test_fix_raises_the_minimum_quality(dim, n, amp, seed, iso):
+    # fix (src/Omega_h_fix.cpp:120-134): in the ill metric the mesh is below
+    # the allowed quality; the adapt() calls of fix_for_given_metric raise
+    # its minimum and stop once it improves by less than 1e-3
+    opts = O.default_opts(dim)
+    m = jittered_box(dim, n, amp, seed)
+    m.ill_metric(opts, iso)
+    q0 = m.qualities().min()
+    m = jittered_box(dim, n, amp, seed)
+    calls = m.fix(opts, iso)
+    assert calls >= 2
+    assert m.qualities().min() > q0
+    assert m.sizes().min() > 0
+
+
+@pytest.mark.parametrize("dim,n,amp,seed,nc", [(2, 8, 0.4, 2, 1), (2, 8, 0.4, 2, 3),
+                                               (3, 4, 0.35, 3, 1), (3, 4, 0.35, 3, 6)])
+def test_grade_fix_adapt_reaches_a_uniform_target(dim, n, amp, seed, nc):
+    # grade_fix_adapt (src/Omega_h_fix.cpp:136-176) from a jittered box with
+    # no metric: the implied metric is below the allowed quality (repaired),
+    # then approach_metric + adapt reach the target, which grading left alone
+    opts = O.default_opts(dim)
+    m = jittered_box(dim, n, amp, seed)
+    h2 = float((1.5 * n) ** 2)
+    one = [h2] if nc == 1 else [h2] * dim + [0.0] * (nc - dim)
+    steps = m.grade_fix_adapt(opts, np.tile(one, m.nents(0)), nc)
+    assert steps >= 1
+    assert m.vertex_tag("target_metric") is None
+    assert m.metric_ncomps == nc
+    assert np.allclose(m.metric(), np.tile(one, m.nents(0)), rtol=1e-12, atol=1e-12 * h2)
+    assert m.qualities().min() >= opts.min_quality_allowed
+    assert m.sizes().min() > 0
+    assert abs(m.sizes().sum() - 1.0) < 1e-12
