@@ -42,7 +42,7 @@ enum { VERT = 0, EDGE = 1, FACE = 2, REGION = 3 };
 enum Verbosity { SILENT = 0, EACH_ADAPT = 1, EACH_REBUILD = 2, EXTRA_STATS = 3 };
 
 // transfer types of vertex tags carried through adapt() (src/Omega_h_adapt.hpp:30-42)
-enum Omega_h_Transfer { OMEGA_H_LINEAR_INTERP = 0, OMEGA_H_METRIC = 1 };
+enum Omega_h_Transfer { OMEGA_H_LINEAR_INTERP = 0, OMEGA_H_METRIC = 1, OMEGA_H_POINTWISE = 2 };
 // src/Omega_h_defines.h: how a metric is made isotropic
 enum Omega_h_Isotropy { OMEGA_H_ANISOTROPIC = 0, OMEGA_H_ISO_LENGTH = 1, OMEGA_H_ISO_SIZE = 2 };
 
@@ -291,6 +291,12 @@ class Mesh {
     check_status(osh_mesh_vertex_tag_ncomps(handle(), name.c_str(), &nc));
     return nc;
   }
+  // element tags are carried through adapt() pointwise (OMEGA_H_POINTWISE)
+  int elem_tag_ncomps(std::string const& name) const {
+    int nc;
+    check_status(osh_mesh_elem_tag_ncomps(handle(), name.c_str(), &nc));
+    return nc;
+  }
   bool has_tag(Int ent_dim, std::string const& name) const {
     if (ent_dim == VERT && name != "metric" && name != "coordinates" &&
         vertex_tag_ncomps(name) >= 0)
@@ -304,10 +310,12 @@ class Mesh {
     if (name == "class_id" || name == "class_dim") return ent_dim <= dim();
     if (name == "length") return ent_dim == EDGE;
     if (name == "quality") return ent_dim == dim();
+    if (ent_dim == dim() && elem_tag_ncomps(name) >= 0) return true;
     return false;
   }
   void remove_tag(Int ent_dim, std::string const& name) {
     if (ent_dim == VERT) check_status(osh_mesh_remove_vertex_tag(handle(), name.c_str()));
+    if (ent_dim == dim()) check_status(osh_mesh_remove_elem_tag(handle(), name.c_str()));
   }
   template <class T>
   void add_tag(Int ent_dim, std::string const& name, Int ncomps, Read<T> array,
@@ -328,7 +336,13 @@ namespace detail {
 // otherwise by linear interpolation (AdaptOpts::xfer_opts can say otherwise)
 inline void set_metric(Mesh& m, Int ent_dim, std::string const& name, Int ncomps,
     Real const* data, LO size) {
-  if (ent_dim != VERT) throw exception("only vertex tags are carried through adapt()");
+  if (ent_dim == m.dim() && ent_dim != VERT) {
+    if (size != m.nelems() * ncomps) throw exception("tag array has the wrong size");
+    check_status(osh_mesh_add_elem_tag(m.handle(), name.c_str(), ncomps, data));
+    return;
+  }
+  if (ent_dim != VERT)
+    throw exception("only vertex and element tags are carried through adapt()");
   if (size != m.nverts() * ncomps) throw exception("tag array has the wrong size");
   if (name == "metric") {
     check_status(osh_mesh_set_metric(m.handle(), ncomps, data));
@@ -376,6 +390,14 @@ inline Read<Real> Mesh::get_array<Real>(Int ent_dim, std::string const& name) co
     if (nc >= 0) {
       Write<Real> w(nverts() * nc);
       check_status(osh_mesh_get_vertex_tag(handle(), name.c_str(), w.data()));
+      return w;
+    }
+  }
+  if (ent_dim == dim()) {
+    int nc = elem_tag_ncomps(name);
+    if (nc >= 0) {
+      Write<Real> w(nelems() * nc);
+      check_status(osh_mesh_get_elem_tag(handle(), name.c_str(), w.data()));
       return w;
     }
   }
@@ -476,6 +498,7 @@ namespace detail {
 // re-register a vertex tag whose transfer type the options name differently
 inline void apply_transfer_types(Mesh* mesh, AdaptOpts const& opts) {
   for (auto const& kv : opts.xfer_opts.type_map) {
+    if (kv.second == OMEGA_H_POINTWISE) continue;  // element tags are pointwise already
     int nc = mesh->vertex_tag_ncomps(kv.first);
     if (nc < 0) continue;
     Write<Real> w(mesh->nverts() * nc);

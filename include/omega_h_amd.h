@@ -170,6 +170,16 @@ int osh_mesh_add_inherit_tag(osh_mesh* mesh, const char* name, int ncomps,
     const double* const* per_dim);
 int osh_mesh_inherit_tag_ncomps(osh_mesh* mesh, const char* name, int* out); /* -1: absent */
 int osh_mesh_get_inherit_tag(osh_mesh* mesh, const char* name, int ent_dim, double* out);
+/* element tags carried through adapt() pointwise: Mesh::add_tag(dim, name,
+ * ncomps, array) with OMEGA_H_POINTWISE as its transfer type
+ * (src/Omega_h_transfer.cpp:47-54,515-583): products of refinement take
+ * their parent element's value, products of coarsening and swapping the
+ * value of the old element of their cavity in which their centroid lies
+ * deepest; surviving elements keep theirs.  data: nelems * ncomps values */
+int osh_mesh_add_elem_tag(osh_mesh* mesh, const char* name, int ncomps, const double* data);
+int osh_mesh_elem_tag_ncomps(osh_mesh* mesh, const char* name, int* out); /* -1: absent */
+int osh_mesh_get_elem_tag(osh_mesh* mesh, const char* name, double* out);
+int osh_mesh_remove_elem_tag(osh_mesh* mesh, const char* name);
 /* approach_metric(mesh, opts, min_step), src/Omega_h_approach.cpp:83-126:
  * set the metric to the "target_metric" tag, or halve the log-Euclidean
  * step toward it until the mesh's quality and lengths are acceptable;

@@ -113,6 +113,10 @@ def load(path=None):
         "osh_mesh_set_coords": [P, DP],
         "osh_warp_to_limit": [P, C.POINTER(AdaptOpts), I, I, IP],
         "osh_get_implied_isos": [P, I, DP],
+        "osh_mesh_add_elem_tag": [P, C.c_char_p, I, DP],
+        "osh_mesh_elem_tag_ncomps": [P, C.c_char_p, IP],
+        "osh_mesh_get_elem_tag": [P, C.c_char_p, DP],
+        "osh_mesh_remove_elem_tag": [P, C.c_char_p],
         "osh_ill_metric": [P, C.POINTER(AdaptOpts), I],
         "osh_fix": [P, C.POINTER(AdaptOpts), I, IP],
         "osh_fix_for_given_metric": [P, C.POINTER(AdaptOpts), IP],
@@ -474,6 +478,22 @@ class Mesh:
         a = np.ascontiguousarray(coords, dtype=np.float64)
         assert a.size == self.nents(0) * self.dim
         check(self.lib.osh_mesh_set_coords(self.h, _p(a, C.c_double)), self.lib)
+
+    def add_elem_tag(self, name, ncomps, data):
+        """An element tag carried through adapt() pointwise (OMEGA_H_POINTWISE)."""
+        a = np.ascontiguousarray(data, dtype=np.float64)
+        assert a.size == self.nents(self.dim) * ncomps
+        check(self.lib.osh_mesh_add_elem_tag(self.h, name.encode(), ncomps, _p(a, C.c_double)),
+              self.lib)
+
+    def elem_tag(self, name):
+        nc = self._int(self.lib.osh_mesh_elem_tag_ncomps, name.encode())
+        if nc < 0:
+            return None
+        out = np.zeros(self.nents(self.dim) * nc, dtype=np.float64)
+        check(self.lib.osh_mesh_get_elem_tag(self.h, name.encode(), _p(out, C.c_double)),
+              self.lib)
+        return out
 
     def implied_isos(self, set_as_metric=False):
         """get_implied_isos (src/Omega_h_metric.cpp:316-321)."""

@@ -606,6 +606,61 @@ int osh_mesh_remove_vertex_tag(osh_mesh* mesh, const char* name) {
   });
 }
 
+int osh_mesh_add_elem_tag(osh_mesh* mesh, const char* name, int ncomps, const double* data) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(name && data, "null argument");
+    OSH_CHECK(ncomps >= 1 && ncomps <= 64, "1 to 64 components");
+    auto& c = mesh->owner->ctx;
+    osh::LO n = mesh->m.nents[mesh->m.dim] * ncomps;
+    osh::Reals d(c, n);
+    osh::upload(c, d, data, n);
+    c.sync();
+    for (auto& t : mesh->m.etags)
+      if (t.name == name) {
+        t = {name, ncomps, d};
+        return;
+      }
+    mesh->m.etags.push_back({name, ncomps, d});
+  });
+}
+
+int osh_mesh_elem_tag_ncomps(osh_mesh* mesh, const char* name, int* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(name && out, "null argument");
+    *out = -1;
+    for (auto const& t : mesh->m.etags)
+      if (t.name == name) *out = t.ncomps;
+  });
+}
+
+int osh_mesh_get_elem_tag(osh_mesh* mesh, const char* name, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(name && out, "null argument");
+    for (auto const& t : mesh->m.etags)
+      if (t.name == name) {
+        download(mesh->owner->ctx, t.data, out);
+        return;
+      }
+    throw osh::Error(std::string("no element tag ") + name);
+  });
+}
+
+int osh_mesh_remove_elem_tag(osh_mesh* mesh, const char* name) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(name != nullptr, "null argument");
+    auto& v = mesh->m.etags;
+    for (auto it = v.begin(); it != v.end(); ++it)
+      if (it->name == name) {
+        v.erase(it);
+        return;
+      }
+  });
+}
+
 int osh_approach_metric(osh_mesh* mesh, const osh_adapt_opts* opts, double min_step, int* did) {
   return guarded([&] {
     check_mesh(mesh);

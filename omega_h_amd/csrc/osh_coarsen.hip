@@ -559,6 +559,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
       nw.class_dim[0] = transfer<I8>(c, mo, 1, m.class_dim[0], nullptr);
     } else {
       transfer_class(c, m, nw, d, mo, psrc, d, nullptr);
+      if (d == dim) transfer_pointwise(c, m, nw, mo, 0, keys2verts, k2p[d], pv);
       transfer_measures(c, m, nw, d, mo);
     }
     old_lows2new_lows = mo.old2new;

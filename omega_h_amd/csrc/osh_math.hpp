@@ -691,6 +691,43 @@ OSH_HD double edge_length(double const* p0, double const* p1, double const* m0,
   return aniso_length(l_a, l_b);
 }
 
+// barycentric_from_global (src/Omega_h_shape.hpp:46-63) of point x in the
+// simplex p: the basis b[i] = p[i+1] - p[0] inverted by the cofactor formulas
+// of src/Omega_h_matrix.hpp:332-350, times x - p[0]
+template <int D>
+OSH_HD void barycentric(double const* x, double const (*p)[3], double* xi) {
+  Mat b, inv;
+  mat_zero(b);
+  mat_zero(inv);
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) b.m[i][j] = p[i + 1][j] - p[0][j];
+  double dt = det<D>(b);
+  if (D == 3) {
+    double bc[3][3];
+    cross3(b.m[1], b.m[2], bc[0]);
+    cross3(b.m[2], b.m[0], bc[1]);
+    cross3(b.m[0], b.m[1], bc[2]);
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) inv.m[j][i] = bc[i][j] / dt;
+  } else {
+    double a = b.m[0][0], bb = b.m[1][0], c = b.m[0][1], d = b.m[1][1];
+    inv.m[0][0] = d / dt;
+    inv.m[0][1] = -c / dt;
+    inv.m[1][0] = -bb / dt;
+    inv.m[1][1] = a / dt;
+  }
+  double v[3], lam[3];
+  for (int i = 0; i < D; ++i) v[i] = x[i] - p[0][i];
+  for (int i = 0; i < D; ++i) lam[i] = inv.m[0][i] * v[0];
+  for (int j = 1; j < D; ++j)
+    for (int i = 0; i < D; ++i) lam[i] = lam[i] + inv.m[j][i] * v[j];
+  xi[0] = 1.0;
+  for (int i = 1; i < D + 1; ++i) {
+    xi[i] = lam[i - 1];
+    xi[0] -= lam[i - 1];
+  }
+}
+
 // maxdet_metric + metric_element_quality, src/Omega_h_metric.hpp:170-181,
 // src/Omega_h_quality.hpp:8-37
 template <int D, int M>

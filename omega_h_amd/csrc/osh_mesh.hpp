@@ -49,6 +49,17 @@ struct DMesh {
     Reals data[4];
   };
   std::vector<InheritTag> itags;
+  // element tags transferred pointwise (OMEGA_H_POINTWISE, should_fit
+  // src/Omega_h_transfer.cpp:47-54): refinement products take their parent
+  // element's value; coarsening and swap products take the value of the old
+  // element of the cavity nearest to their centroid
+  // (transfer_pointwise_tmpl, src/Omega_h_transfer.cpp:515-583)
+  struct ElemTag {
+    std::string name;
+    int ncomps;
+    Reals data;
+  };
+  std::vector<ElemTag> etags;
   Reals length;   // cached edge lengths (may be absent)
   Reals quality;  // cached element qualities (may be absent)
   // derived caches

@@ -9,6 +9,7 @@
 //   indset::find / QualityCompare         src/Omega_h_indset_inline.hpp:12-72,
 //                                         src/Omega_h_indset.cpp:5-35
 //   mark_down / mark_dual_layers / mark_sliver_layers src/Omega_h_mark.cpp:12-47,190-212
+#include "osh_math.hpp"
 #include "osh_passes.hpp"
 
 namespace osh {
@@ -270,6 +271,56 @@ __global__ void transfer_class_kernel(LO n, LO nsame, LO const* same2old, LO con
   LO w = prods2new[p];
   ids[w] = second ? ids_b[s] : ids_a[s];
   dims[w] = second ? dims_b[s] : dims_a[s];
+}
+
+// row p of out = row src[p] of in
+__global__ void gather_rows(LO n, int nc, LO const* src, Real const* in, Real* out) {
+  LO p = thread_index();
+  if (p >= n) return;
+  for (int k = 0; k < nc; ++k) out[std::size_t(p) * nc + k] = in[std::size_t(src[p]) * nc + k];
+}
+
+// transfer_pointwise_tmpl (src/Omega_h_transfer.cpp:515-563), one thread per
+// key: each product takes the old element around the key (rows in the order
+// of the upward adjacency) in which its centroid has the largest smallest
+// barycentric coordinate, the first one on ties
+template <int D>
+__global__ void pointwise_kernel(LO nkeys, LO const* keys2kds, LO const* keys2prods,
+    LO const* up_a2ab, LO const* up_ab2b, LO const* old_ev2v, Real const* old_coords,
+    LO const* prod_verts, Real const* new_coords, LO* best) {
+  LO key = thread_index();
+  if (key >= nkeys) return;
+  LO kd = keys2kds[key];
+  for (LO p = keys2prods[key]; p < keys2prods[key + 1]; ++p) {
+    double x[3];
+    LO v0 = prod_verts[std::size_t(p) * (D + 1)];
+    for (int j = 0; j < D; ++j) x[j] = new_coords[std::size_t(v0) * D + j];
+    for (int i = 1; i <= D; ++i) {
+      LO vi = prod_verts[std::size_t(p) * (D + 1) + i];
+      for (int j = 0; j < D; ++j) x[j] = x[j] + new_coords[std::size_t(vi) * D + j];
+    }
+    for (int j = 0; j < D; ++j) x[j] = x[j] / double(D + 1);
+    LO b = -1;
+    double bd = 0.0;
+    for (LO ke = up_a2ab[kd]; ke < up_a2ab[kd + 1]; ++ke) {
+      LO e = up_ab2b[ke];
+      double pp[D + 1][3];
+      for (int i = 0; i <= D; ++i) {
+        LO v = old_ev2v[std::size_t(e) * (D + 1) + i];
+        for (int j = 0; j < D; ++j) pp[i][j] = old_coords[std::size_t(v) * D + j];
+      }
+      double xi[D + 1];
+      barycentric<D>(x, pp, xi);
+      double mn = xi[0];
+      for (int i = 1; i <= D; ++i) mn = (mn < xi[i]) ? mn : xi[i];
+      double dist = -mn;
+      if (b == -1 || dist < bd) {
+        b = e;
+        bd = dist;
+      }
+    }
+    best[p] = b;
+  }
 }
 
 // an inheritance tag of the new entities, by the same sources as
@@ -695,6 +746,40 @@ void transfer_class(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo,
         a.data[other].data(), out.data());
     nw.itags[t].data[d] = out;
   }
+}
+
+void transfer_elem_tags(Ctx& c, DMesh const& old, DMesh& nw, ModOut const& mo,
+    LOs const* prod_src) {
+  nw.etags.clear();
+  LO nprods = mo.prods2new.size();
+  for (auto const& t : old.etags) {
+    Reals pd;
+    if (prod_src) {
+      pd = Reals(c, nprods * t.ncomps);
+      OSH_LAUNCH(gather_rows, nprods, c, t.ncomps, prod_src->data(), t.data.data(), pd.data());
+    }
+    nw.etags.push_back({t.name, t.ncomps, transfer<Real>(c, mo, t.ncomps, t.data,
+        prod_src ? &pd : nullptr)});
+  }
+}
+
+void transfer_pointwise(Ctx& c, DMesh& old, DMesh& nw, ModOut const& mo, int key_dim,
+    LOs const& keys2kds, LOs const& keys2prods, LOs const& prod_verts) {
+  if (old.etags.empty()) return;
+  int dim = old.dim;
+  LO nkeys = keys2kds.size();
+  Adj const& up = ask_up(c, old, key_dim, dim);
+  LOs const& ev2v = verts_of(c, old, dim);
+  LOs best(c, mo.prods2new.size());
+  if (dim == 3)
+    OSH_LAUNCH(pointwise_kernel<3>, nkeys, c, keys2kds.data(), keys2prods.data(),
+        up.a2ab.data(), up.ab2b.data(), ev2v.data(), old.coords.data(), prod_verts.data(),
+        nw.coords.data(), best.data());
+  else
+    OSH_LAUNCH(pointwise_kernel<2>, nkeys, c, keys2kds.data(), keys2prods.data(),
+        up.a2ab.data(), up.ab2b.data(), ev2v.data(), old.coords.data(), prod_verts.data(),
+        nw.coords.data(), best.data());
+  transfer_elem_tags(c, old, nw, mo, &best);
 }
 
 void transfer_measures(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo) {

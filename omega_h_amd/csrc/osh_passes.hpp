@@ -75,6 +75,15 @@ void transfer_class(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo,
     LOs const& prod_src, int src_dim, Bytes const* src_dims);
 // lengths (d == 1) and qualities (d == dim) of products on the new mesh
 void transfer_measures(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo);
+// element tags of the new elements: products take the value of old element
+// prod_src[p] (no products when prod_src is null)
+void transfer_elem_tags(Ctx& c, DMesh const& old, DMesh& nw, ModOut const& mo,
+    LOs const* prod_src);
+// transfer_pointwise (src/Omega_h_transfer.cpp:515-583) of the element tags
+// for coarsening (key_dim 0) and swap (key_dim 1): prod_verts are the
+// products' vertices in the new mesh, whose coordinates are already set
+void transfer_pointwise(Ctx& c, DMesh& old, DMesh& nw, ModOut const& mo, int key_dim,
+    LOs const& keys2kds, LOs const& keys2prods, LOs const& prod_verts);
 
 // *any_cands (when given) tells whether any entity was a candidate; for
 // edges, `edge_elems` (when given) is an edge -> element adjacency holding at

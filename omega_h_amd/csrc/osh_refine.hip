@@ -300,6 +300,7 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
       transfer_class(c, m, nw, 0, mo, keys2edges, 1, nullptr);
     } else {
       transfer_class(c, m, nw, d, mo, psrc, d, &psrc_dim);
+      if (d == dim) transfer_elem_tags(c, m, nw, mo, &psrc);
       transfer_measures(c, m, nw, d, mo);
     }
     old_lows2new_lows = mo.old2new;

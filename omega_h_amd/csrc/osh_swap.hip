@@ -461,6 +461,7 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
     modify_build(c, m, nw, plans[d - 1], keys2edges, k2p[d], nprods[d], pv[d],
         old_lows2new_lows, mo, nullptr, &src);
     transfer_class(c, m, nw, d, mo, src, 1, nullptr);
+    if (d == dim) transfer_pointwise(c, m, nw, mo, 1, keys2edges, k2p[d], pv[d]);
     transfer_measures(c, m, nw, d, mo);
     old_lows2new_lows = mo.old2new;
   }

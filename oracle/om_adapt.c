@@ -432,6 +432,78 @@ static void inherit_itags(om_mesh const* old, om_mesh* nw, int d, mod_out const*
   }
 }
 
+/* element tags: products take the value of old element prod_src[p];
+ * prod_src == NULL means there are no products */
+static void etags_from(om_mesh const* old, om_mesh* nw, mod_out const* mo, int const* prod_src) {
+  nw->netags = old->netags;
+  for (int t = 0; t < old->netags; ++t) {
+    om_etag const* a = &old->etags[t];
+    om_etag* b = &nw->etags[t];
+    memcpy(b->name, a->name, sizeof(b->name));
+    b->ncomps = a->ncomps;
+    int nc = a->ncomps;
+    double* pd = NULL;
+    if (prod_src) {
+      pd = (double*)om_malloc(sizeof(double) * (size_t)(mo->nprods + 1) * nc);
+      for (int p = 0; p < mo->nprods; ++p)
+        for (int k = 0; k < nc; ++k) pd[(size_t)p * nc + k] = a->data[(size_t)prod_src[p] * nc + k];
+    }
+    transfer_r(mo->nnew, nc, mo, a->data, pd, &b->data);
+    free(pd);
+  }
+}
+
+/* transfer_pointwise_tmpl (src/Omega_h_transfer.cpp:515-563): each product
+ * element takes the value of the old element around its key (a vertex or an
+ * edge, rows in the order of the upward adjacency) in which the product's
+ * centroid has the largest smallest barycentric coordinate, the first one on
+ * ties; pv holds the products' vertices in the new mesh, whose coordinates
+ * are already set */
+static void etags_pointwise(om_mesh* old, om_mesh* nw, mod_out const* mo, int nkeys,
+    int const* keys2kds, int key_dim, int const* k2p, int const* pv) {
+  if (old->netags == 0) {
+    nw->netags = 0;
+    return;
+  }
+  int dim = old->dim;
+  int nprods = k2p[nkeys];
+  int* best = (int*)om_malloc(sizeof(int) * (size_t)(nprods + 1));
+  om_graph const* kd2e = om_ask_up(old, key_dim, dim);
+  int const* ev2v = om_verts_of(old, dim);
+  for (int k = 0; k < nkeys; ++k) {
+    int kd = keys2kds[k];
+    for (int p = k2p[k]; p < k2p[k + 1]; ++p) {
+      double x[3];
+      double const* q0 = nw->coords + (size_t)pv[p * (dim + 1)] * dim;
+      for (int j = 0; j < dim; ++j) x[j] = q0[j];
+      for (int i = 1; i <= dim; ++i) {
+        double const* qi = nw->coords + (size_t)pv[p * (dim + 1) + i] * dim;
+        for (int j = 0; j < dim; ++j) x[j] = x[j] + qi[j];
+      }
+      for (int j = 0; j < dim; ++j) x[j] = x[j] / (double)(dim + 1);
+      int b = -1;
+      double bd = 0.0;
+      for (int ke = kd2e->a2ab[kd]; ke < kd2e->a2ab[kd + 1]; ++ke) {
+        int e = kd2e->ab2b[ke];
+        double const* pp[4];
+        for (int i = 0; i <= dim; ++i) pp[i] = old->coords + (size_t)ev2v[e * (dim + 1) + i] * dim;
+        double xi[4];
+        om_barycentric(dim, x, pp, xi);
+        double mn = xi[0];
+        for (int i = 1; i <= dim; ++i) mn = (mn < xi[i]) ? mn : xi[i];
+        double dist = -mn;
+        if (b == -1 || dist < bd) {
+          b = e;
+          bd = dist;
+        }
+      }
+      best[p] = b;
+    }
+  }
+  etags_from(old, nw, mo, best);
+  free(best);
+}
+
 /* class_id/class_dim of products from a list of old entities of dimension
  * src_dim (one per product) */
 static void transfer_class_from(om_mesh* old, om_mesh* nw, int d, int src_dim,
@@ -745,6 +817,7 @@ int om_refine_by_size(om_mesh* m, om_adapt_opts const* o) {
       transfer_i(mo.nnew, &mo, m->class_id[d], pid, &nw->class_id[d]);
       transfer_b(mo.nnew, &mo, m->class_dim[d], pdim, &nw->class_dim[d]);
       inherit_itags(m, nw, d, &mo, d, psd, psrc);
+      if (d == dim) etags_from(m, nw, &mo, psrc);
       free(pid);
       free(pdim);
       free(psd);
@@ -1050,6 +1123,7 @@ static int coarsen(om_mesh* m, om_adapt_opts const* o, signed char* edge_codes,
     } else {
       transfer_class_from(m, nw, d, d, psrc, &mo);
       inherit_itags(m, nw, d, &mo, d, NULL, psrc);
+      if (d == dim) etags_pointwise(m, nw, &mo, nkeys, keys2verts, 0, k2p, pv);
       transfer_lengths_qualities(m, nw, d, &mo);
     }
     free(old_lows2new_lows);
@@ -1311,6 +1385,7 @@ static void swap_rebuild(om_mesh* m, int nkeys, int const* keys2edges,
       for (int p = k2p[d][k]; p < k2p[d][k + 1]; ++p) src[p] = keys2edges[k];
     transfer_class_from(m, nw, d, 1, src, &mo);
     inherit_itags(m, nw, d, &mo, 1, NULL, src);
+    if (d == dim) etags_pointwise(m, nw, &mo, nkeys, keys2edges, 1, k2p[d], pv[d]);
     free(src);
     transfer_lengths_qualities(m, nw, d, &mo);
     free(old_lows2new_lows);

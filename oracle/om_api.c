@@ -273,7 +273,9 @@ int om_approach_metric(om_mesh* m, om_adapt_opts const* o, double min_step) {
 /* Mesh::set_coords, src/Omega_h_mesh.cpp:365-368: new coordinates drop the
  * cached lengths and qualities (src/Omega_h_mesh.cpp:167-177) */
 void om_set_coords(om_mesh* m, double const* coords) {
-  memcpy(m->coords, coords, sizeof(double) * (size_t)m->nents[0] * m->dim);
+  size_t n = (size_t)m->nents[0] * m->dim;
+  if (!m->coords) m->coords = (double*)om_malloc(sizeof(double) * (n + 1));
+  memcpy(m->coords, coords, sizeof(double) * n);
   free(m->length);
   free(m->quality);
   m->length = NULL;
@@ -335,6 +337,39 @@ int om_warp_to_limit(om_mesh* m, om_adapt_opts const* o, int exit_on_stall, int 
   free(warp);
   free(cur);
   return r;
+}
+
+/* element tags transferred pointwise (OMEGA_H_POINTWISE) */
+static om_etag* find_etag(om_mesh* m, char const* name) {
+  for (int t = 0; t < m->netags; ++t)
+    if (strncmp(m->etags[t].name, name, sizeof(m->etags[t].name)) == 0) return &m->etags[t];
+  return NULL;
+}
+
+int om_add_etag(om_mesh* m, char const* name, int ncomps, double const* data) {
+  om_etag* t = find_etag(m, name);
+  if (!t) {
+    if (m->netags >= OM_MAX_ETAGS || strlen(name) >= sizeof(t->name)) return -1;
+    t = &m->etags[m->netags++];
+    memset(t, 0, sizeof(*t));
+    strcpy(t->name, name);
+  }
+  free(t->data);
+  size_t n = (size_t)m->nents[m->dim] * ncomps;
+  t->ncomps = ncomps;
+  t->data = (double*)om_malloc(sizeof(double) * (n + 1));
+  memcpy(t->data, data, sizeof(double) * n);
+  return 0;
+}
+
+int om_etag_ncomps(om_mesh* m, char const* name) {
+  om_etag* t = find_etag(m, name);
+  return t ? t->ncomps : -1;
+}
+
+void om_get_etag(om_mesh* m, char const* name, double* out) {
+  om_etag* t = find_etag(m, name);
+  if (t) memcpy(out, t->data, sizeof(double) * (size_t)m->nents[m->dim] * t->ncomps);
 }
 
 /* inheritance tags (OMEGA_H_INHERIT): one array per dimension 0..dim */

@@ -547,7 +547,49 @@ void om_average_metric(int mdim, int n, double const* const* ms, double* out) {
   om_set_symm(mdim, &e, out);
 }
 
+/* barycentric_from_global (src/Omega_h_shape.hpp:46-63) of point x in the
+ * simplex p: the basis b[i] = p[i+1] - p[0] inverted by the cofactor
+ * formulas of src/Omega_h_matrix.hpp:332-350, times x - p[0] */
+void om_barycentric(int dim, double const* x, double const* const* p, double* xi) {
+  om_mat b, inv;
+  mat_zero(&b);
+  mat_zero(&inv);
+  for (int i = 0; i < dim; ++i)
+    for (int j = 0; j < dim; ++j) b.m[i][j] = p[i + 1][j] - p[0][j];
+  double det = om_det(dim, &b);
+  if (dim == 3) {
+    double bc[3][3];
+    cross3(b.m[1], b.m[2], bc[0]);
+    cross3(b.m[2], b.m[0], bc[1]);
+    cross3(b.m[0], b.m[1], bc[2]);
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) inv.m[j][i] = bc[i][j] / det;
+  } else {
+    double a = b.m[0][0], bb = b.m[1][0], c = b.m[0][1], d = b.m[1][1];
+    inv.m[0][0] = d / det;
+    inv.m[0][1] = -c / det;
+    inv.m[1][0] = -bb / det;
+    inv.m[1][1] = a / det;
+  }
+  double v[3], lam[3];
+  for (int i = 0; i < dim; ++i) v[i] = x[i] - p[0][i];
+  for (int i = 0; i < dim; ++i) lam[i] = inv.m[0][i] * v[0];
+  for (int j = 1; j < dim; ++j)
+    for (int i = 0; i < dim; ++i) lam[i] = lam[i] + inv.m[j][i] * v[j];
+  xi[0] = 1.0;
+  for (int i = 1; i < dim + 1; ++i) {
+    xi[i] = lam[i - 1];
+    xi[0] -= lam[i - 1];
+  }
+}
+
 /* ---- known-answer hooks ---- */
+
+void om_kat_barycentric(int dim, double const* x, double const* simplex, double* xi) {
+  double const* p[4];
+  for (int i = 0; i <= dim; ++i) p[i] = simplex + i * dim;
+  om_barycentric(dim, x, p, xi);
+}
 
 double om_kat_max_eigenvalue(int dim, double const* symm) {
   om_mat m;

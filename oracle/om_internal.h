@@ -49,6 +49,7 @@ double om_element_size(int dim, double const* const* p);
 double om_element_quality(int dim, int mdim, double const* const* p,
     double const* const* ms);
 void om_average_metric(int mdim, int n, double const* const* ms, double* out);
+void om_barycentric(int dim, double const* x, double const* const* p, double* xi);
 void om_intersect_metrics(int dim, double const* m1, double const* m2, double* out);
 void om_limit_gradation_once(int sdim, int mdim, int nverts, int const* a2ab, int const* ab2b,
     double const* coords, double const* in, double max_rate, double* out);
@@ -104,12 +105,26 @@ typedef struct {
   double* data[4];
 } om_itag;
 
+/* an element tag transferred pointwise (OMEGA_H_POINTWISE, should_fit
+ * src/Omega_h_transfer.cpp:47-54): refinement products take their parent
+ * element's value (src/Omega_h_transfer.cpp:324-335); coarsening and swap
+ * products take the value of the old element of the cavity nearest to
+ * their centroid (transfer_pointwise_tmpl, src/Omega_h_transfer.cpp:515-583) */
+#define OM_MAX_ETAGS 8
+typedef struct {
+  char name[64];
+  int ncomps;
+  double* data;
+} om_etag;
+
 struct om_mesh {
   int dim;
   int nvtags;
   om_vtag vtags[OM_MAX_VTAGS];
   int nitags;
   om_itag itags[OM_MAX_ITAGS];
+  int netags;
+  om_etag etags[OM_MAX_ETAGS];
   int nents[4];
   /* down[d] for d >= 1: nents[d] * (d+1) lower entities of dim d-1 */
   int* down[4];

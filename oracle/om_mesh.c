@@ -171,6 +171,7 @@ void om_mesh_free(om_mesh* m) {
   for (int t = 0; t < m->nvtags; ++t) free(m->vtags[t].data);
   for (int t = 0; t < m->nitags; ++t)
     for (int d = 0; d < 4; ++d) free(m->itags[t].data[d]);
+  for (int t = 0; t < m->netags; ++t) free(m->etags[t].data);
   free(m->length);
   free(m->quality);
   free(m);

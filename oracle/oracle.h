@@ -100,6 +100,11 @@ void om_interpolate_metrics(int mdim, int n, double const* a, double const* b, d
     double* out);
 int om_approach_metric(om_mesh* m, om_adapt_opts const* o, double min_step);
 void om_set_coords(om_mesh* m, double const* coords);
+/* element tags transferred pointwise (OMEGA_H_POINTWISE) */
+int om_add_etag(om_mesh* m, char const* name, int ncomps, double const* data);
+int om_etag_ncomps(om_mesh* m, char const* name);
+void om_get_etag(om_mesh* m, char const* name, double* out);
+void om_kat_barycentric(int dim, double const* x, double const* simplex, double* xi);
 int om_warp_to_limit(om_mesh* m, om_adapt_opts const* o, int exit_on_stall, int max_niters);
 /* get_implied_metrics: nverts * (3 or 6) values */
 void om_get_implied_metrics(om_mesh* m, double* out);

@@ -99,6 +99,11 @@ def lib():
         L.om_approach_metric.argtypes = [P, C.POINTER(AdaptOpts), C.c_double]
         L.om_warp_to_limit.argtypes = [P, C.POINTER(AdaptOpts), C.c_int, C.c_int]
         L.om_get_implied_isos.argtypes = [P, C.POINTER(C.c_double)]
+        L.om_add_etag.argtypes = [P, C.c_char_p, C.c_int, C.POINTER(C.c_double)]
+        L.om_etag_ncomps.argtypes = [P, C.c_char_p]
+        L.om_get_etag.argtypes = [P, C.c_char_p, C.POINTER(C.c_double)]
+        L.om_kat_barycentric.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]
         L.om_kat_max_eigenvalue.argtypes = [C.c_int, C.POINTER(C.c_double)]
         L.om_kat_max_eigenvalue.restype = C.c_double
         L.om_ill_metric.argtypes = [P, C.POINTER(AdaptOpts), C.c_int]
@@ -315,6 +320,20 @@ class Mesh:
     def metric_ncomps(self):
         return lib().om_metric_ncomps(self.h)
 
+    def add_elem_tag(self, name, ncomps, data):
+        """An element tag transferred pointwise (OMEGA_H_POINTWISE)."""
+        a = np.ascontiguousarray(data, dtype=np.float64)
+        assert a.size == self.nents(self.dim) * ncomps
+        assert lib().om_add_etag(self.h, name.encode(), ncomps, _ptr(a, C.c_double)) == 0
+
+    def elem_tag(self, name):
+        nc = lib().om_etag_ncomps(self.h, name.encode())
+        if nc < 0:
+            return None
+        out = np.zeros(self.nents(self.dim) * nc, dtype=np.float64)
+        lib().om_get_etag(self.h, name.encode(), _ptr(out, C.c_double))
+        return out
+
     def implied_isos(self):
         out = np.zeros(self.nents(0), dtype=np.float64)
         lib().om_get_implied_isos(self.h, _ptr(out, C.c_double))
@@ -454,3 +473,13 @@ def max_eigenvalue(dim, symm):
     """get_max_eigenvalues of one metric (src/Omega_h_eigen.cpp:7-40)."""
     a = np.ascontiguousarray(symm, dtype=np.float64)
     return lib().om_kat_max_eigenvalue(dim, _ptr(a, C.c_double))
+
+
+def barycentric(dim, x, simplex):
+    """barycentric_from_global (src/Omega_h_shape.hpp:46-63)."""
+    xs = np.ascontiguousarray(x, dtype=np.float64)
+    ps = np.ascontiguousarray(simplex, dtype=np.float64)
+    out = np.zeros(dim + 1, dtype=np.float64)
+    lib().om_kat_barycentric(dim, _ptr(xs, C.c_double), _ptr(ps, C.c_double),
+                             _ptr(out, C.c_double))
+    return out

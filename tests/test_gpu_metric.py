@@ -307,3 +307,25 @@ def test_grade_fix_adapt_identical(ctx, dim, n, amp, seed, nc):
     assert so == sg >= 1
     assert_meshes_same(o, g, dim)
     assert g.vertex_tag("target_metric") is None
+
+
+@pytest.mark.parametrize("dim,n", [(2, 16), (3, 4)])
+def test_pointwise_elem_tags_identical(ctx, dim, n):
+    """An element tag carried pointwise (src/Omega_h_transfer.cpp:324-335,
+    515-583) through a whole adapt(): refinement, coarsening and swap
+    products, bit for bit."""
+    o = O.Mesh.box(dim, n)
+    g = A.Mesh.box(ctx, dim, n)
+    met, nc = W.iso_ring_metric_2d(o.coords()) if dim == 2 else W.levelset_metric_3d(o.coords())
+    vals = np.random.default_rng(3).normal(size=o.nents(dim) * 2)
+    for m in (o, g):
+        m.set_metric(met, nc)
+        m.add_elem_tag("field", 2, vals)
+    ro, po = o.adapt(O.default_opts(dim))
+    rg, pg = g.adapt(ctx.default_opts(dim))
+    assert ro == rg and [p[:3] for p in po] == [p[:3] for p in pg]
+    assert {p[0] for p in po} >= ({0, 1} if dim == 2 else {0, 1, 2, 3})
+    assert_meshes_same(o, g, dim)
+    assert same_bits(o.elem_tag("field"), g.elem_tag("field"))
+    # the values are the original ones, moved around
+    assert np.isin(g.elem_tag("field"), vals).all()

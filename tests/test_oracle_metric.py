@@ -473,3 +473,81 @@ def test_grade_fix_adapt_reaches_a_uniform_target(dim, n, amp, seed, nc):
     assert m.qualities().min() >= opts.min_quality_allowed
     assert m.sizes().min() > 0
     assert abs(m.sizes().sum() - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_barycentric_against_numpy(dim):
+    # barycentric_from_global (src/Omega_h_shape.hpp:46-63)
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        p = rng.normal(size=(dim + 1, dim))
+        x = rng.normal(size=dim)
+        got = O.barycentric(dim, x, p)
+        lam = np.linalg.solve((p[1:] - p[0]).T, x - p[0])
+        want = np.concatenate([[1.0 - lam.sum()], lam])
+        assert np.allclose(got, want, rtol=1e-9, atol=1e-9)
+    p = rng.normal(size=(dim + 1, dim))
+    for i in range(dim + 1):
+        assert np.allclose(O.barycentric(dim, p[i], p), np.eye(dim + 1)[i], atol=1e-12)
+    assert np.allclose(O.barycentric(dim, p.mean(0), p), 1.0 / (dim + 1), atol=1e-12)
+
+
+def containing_elements(old, x):
+    """Brute force: the old elements whose closed simplex holds x."""
+    dim = old.dim
+    ev = old.verts(dim).reshape(-1, dim + 1)
+    c = old.coords().reshape(-1, dim)
+    b = c[ev[:, 1:]] - c[ev[:, :1]]
+    lam = np.linalg.solve(np.transpose(b, (0, 2, 1)), (x - c[ev[:, 0]])[:, :, None])[:, :, 0]
+    bc = np.concatenate([1.0 - lam.sum(1, keepdims=True), lam], axis=1)
+    return np.nonzero(bc.min(1) >= -1e-9)[0]
+
+
+@pytest.mark.parametrize("dim,n,which", [(2, 6, "refine"), (2, 6, "coarsen"), (2, 6, "swap"),
+                                         (3, 3, "refine"), (3, 3, "coarsen"),
+                                         (3, 3, "swap")])
+def test_pointwise_tags_come_from_the_element_under_the_centroid(dim, n, which):
+    # transfer_pointwise_tmpl (src/Omega_h_transfer.cpp:515-583) picks, among
+    # the old elements of the cavity, the one in which the product's centroid
+    # lies deepest; refinement products take their parent's value.  Checked
+    # against a search over every old element for one pass of each kind.
+    # every coordinate off the box's faces jittered (boundary vertices slide
+    # within their faces) so that centroids rarely fall on old faces
+    m = O.Mesh.box(dim, n)
+    c = m.coords().reshape(-1, dim).copy()
+    rng = np.random.default_rng(5)
+    free = (c > 1e-12) & (c < 1 - 1e-12)
+    amp = 0.4 if which == "swap" else 0.3
+    c[free] += rng.uniform(-amp, amp, free.sum()) / n
+    m.set_coords(c.ravel())
+    h = {"refine": 0.5 / n, "coarsen": 2.5 / n, "swap": 1.0 / n}[which]
+    met = np.full(m.nents(0), h ** -2)
+    m.set_metric(met, 1)
+    vals = rng.normal(size=(m.nents(dim), 2))
+    m.add_elem_tag("field", 2, vals.ravel())
+    ev_old = m.verts(dim).reshape(-1, dim + 1)
+    old = O.Mesh.from_elems(dim, ev_old.ravel(), m.nents(0))
+    old.set_coords(m.coords())
+    opts = O.default_opts(dim)
+    did = {"refine": m.refine_by_size, "coarsen": m.coarsen_by_size,
+           "swap": m.swap_edges}[which](opts)
+    assert did
+    new_vals = m.elem_tag("field").reshape(-1, 2)
+    ev = m.verts(dim).reshape(-1, dim + 1)
+    x = m.coords().reshape(-1, dim)[ev].mean(1)
+    # same elements by their vertex positions (coarsening renumbers vertices)
+    oc = old.coords().reshape(-1, dim)
+    nc = m.coords().reshape(-1, dim)
+    old_sets = {tuple(sorted(map(tuple, oc[v]))): i for i, v in enumerate(ev_old)}
+    nprods = checked = 0
+    for e in range(len(ev)):
+        same = old_sets.get(tuple(sorted(map(tuple, nc[ev[e]]))))
+        if same is not None and which != "refine":
+            assert np.array_equal(new_vals[e], vals[same]), e
+            continue
+        nprods += 1
+        hosts = containing_elements(old, x[e:e + 1].repeat(old.nents(dim), 0))
+        if len(hosts) == 1:
+            assert np.array_equal(new_vals[e], vals[hosts[0]]), e
+            checked += 1
+    assert nprods >= 4 and checked >= 0.7 * nprods
