@@ -351,70 +351,113 @@ __device__ __forceinline__ I8 indset_state(I8 const* old_state, LO x) {
   return FIRST ? I8(s ? 2 : 0) : s;
 }
 
+// A node joins the set once every neighbor that beats it (higher quality,
+// or the same quality and a higher index) is out, and leaves once one of
+// them is in: a neighbor it beats can be in only after the node is out, so
+// this is the round above restated over the beating neighbors alone.  The
+// first round records, per candidate, which row positions hold candidates
+// that beat it (bits of `dom`); later rounds read the states at those
+// positions only, not the qualities.  A row longer than 64 positions is
+// read whole every round.
+constexpr int kDomBits = 64;
+
 template <bool FIRST>
 __global__ void indset_iteration(LO n, LO const* a2ab, LO const* ab2b, Real const* q,
-    I8 const* old_state, I8* new_state) {
+    I8 const* old_state, I8* new_state, unsigned long long* dom) {
   LO v = thread_index();
   if (v >= n) return;
   I8 s = indset_state<FIRST>(old_state, v);
   new_state[v] = s;
   if (s != 2) return;
   LO b = a2ab[v], e = a2ab[v + 1];
-  if (!FIRST)
-    for (LO j = b; j < e; ++j)
-      if (old_state[ab2b[j]] == 1) {
+  bool undecided = false;
+  if (!FIRST && e - b <= kDomBits) {
+    unsigned long long mask = dom[v];
+    while (mask) {
+      int j = __ffsll(static_cast<long long>(mask)) - 1;
+      mask &= mask - 1;
+      I8 su = old_state[ab2b[b + j]];
+      if (su == 1) {
         new_state[v] = 0;
         return;
       }
-  Real qv = q[v];
-  for (LO j = b; j < e; ++j) {
-    LO u = ab2b[j];
-    if (indset_state<FIRST>(old_state, u) == 0) continue;
-    Real qu = q[u];
-    bool beats = (qu != qv) ? (qu < qv) : (u < v);
-    if (!beats) return;
+      if (su == 2) undecided = true;
+    }
+  } else {
+    unsigned long long mask = 0;
+    Real qv = q[v];
+    for (LO j = b; j < e; ++j) {
+      LO u = ab2b[j];
+      I8 su = indset_state<FIRST>(old_state, u);
+      if (su == 0) continue;
+      Real qu = q[u];
+      bool beaten = (qu != qv) ? (qu < qv) : (u < v);
+      if (beaten) continue;
+      if (su == 1) {
+        new_state[v] = 0;
+        return;
+      }
+      undecided = true;
+      if (FIRST && j - b < kDomBits) mask |= 1ull << (j - b);
+    }
+    if (FIRST) dom[v] = mask;
   }
-  new_state[v] = 1;
+  if (!undecided) new_state[v] = 1;
 }
 
 // the same round for edges, whose star (edges_across_tris/tets,
 // src/Omega_h_adj.cpp:532-589) is the set of other edges of the elements
 // around the edge: read through the edge -> element adjacency instead of a
-// built star (an edge met twice is checked twice, which changes nothing)
+// built star (an edge met twice is checked twice, which changes nothing);
+// row position (j, k) is bit j * NEPC + k of `dom`
 template <int NEPC, bool FIRST>
 __global__ void indset_iteration_edges(LO n, LO const* e2c_a2ab, LO const* e2c_ab2b,
-    LO const* c2e, Real const* q, I8 const* old_state, I8* new_state) {
+    LO const* c2e, Real const* q, I8 const* old_state, I8* new_state, unsigned long long* dom) {
   LO v = thread_index();
   if (v >= n) return;
   I8 s = indset_state<FIRST>(old_state, v);
   new_state[v] = s;
   if (s != 2) return;
   LO b = e2c_a2ab[v], e = e2c_a2ab[v + 1];
-  if (!FIRST)
+  bool undecided = false;
+  if (!FIRST && (e - b) * NEPC <= kDomBits) {
+    unsigned long long mask = dom[v];
+    while (mask) {
+      int p = __ffsll(static_cast<long long>(mask)) - 1;
+      mask &= mask - 1;
+      I8 su = old_state[c2e[e2c_ab2b[b + p / NEPC] * NEPC + p % NEPC]];
+      if (su == 1) {
+        new_state[v] = 0;
+        return;
+      }
+      if (su == 2) undecided = true;
+    }
+  } else {
+    unsigned long long mask = 0;
+    Real qv = q[v];
     for (LO j = b; j < e; ++j) {
       LO const* ce = c2e + e2c_ab2b[j] * NEPC;
 #pragma unroll
       for (int k = 0; k < NEPC; ++k) {
         LO u = ce[k];
-        if (u != v && old_state[u] == 1) {
+        if (u == v) continue;
+        I8 su = indset_state<FIRST>(old_state, u);
+        if (su == 0) continue;
+        Real qu = q[u];
+        bool beaten = (qu != qv) ? (qu < qv) : (u < v);
+        if (beaten) continue;
+        if (su == 1) {
           new_state[v] = 0;
           return;
         }
+        undecided = true;
+        int p = (j - b) * NEPC + k;
+        if (FIRST && p < kDomBits) mask |= 1ull << p;
       }
     }
-  Real qv = q[v];
-  for (LO j = b; j < e; ++j) {
-    LO const* ce = c2e + e2c_ab2b[j] * NEPC;
-#pragma unroll
-    for (int k = 0; k < NEPC; ++k) {
-      LO u = ce[k];
-      if (u == v || indset_state<FIRST>(old_state, u) == 0) continue;
-      Real qu = q[u];
-      bool beats = (qu != qv) ? (qu < qv) : (u < v);
-      if (!beats) return;
-    }
+    if (FIRST) dom[v] = mask;
   }
-  new_state[v] = 1;
+  if (!undecided) new_state[v] = 1;
 }
 
 __global__ void mark_down_kernel(LO nl, LO const* a2ab, LO const* ab2b, I8 const* marks, I8* out) {
@@ -810,27 +853,32 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
     g = star ? star : &ask_star(c, m, d);
   }
   Bytes a(c, n), b(c, n);
+  DArray<unsigned long long> dom(c, n);
   // rounds leave decided nodes as they are and take no node from the set, so
   // extra rounds after convergence change nothing: run two rounds before each
-  // check; afterwards a state of 2 means undecided nodes remain, and a state
-  // of 1 or 2 that there was a candidate (the best candidate joins the set in
-  // the first round)
+  // of the first four checks, then 4, 8 and 16 (long chains of decreasing
+  // quality, as on the first coarsening passes of a large smooth mesh, take
+  // some 200 rounds and would otherwise spend a sync every two); after the
+  // first check a state of 2 means undecided nodes remain, and a state of 1
+  // or 2 that there was a candidate (the best candidate joins the set in the
+  // first round)
   for (LO round = 0;;) {
     OSH_CHECK(round <= n + 1, "independent set does not converge");
-    for (int k = 0; k < 2; ++k, ++round) {
+    int batch = round < 8 ? 2 : (round < 16 ? 4 : (round < 32 ? 8 : 16));
+    for (int k = 0; k < batch; ++k, ++round) {
       bool first = round == 0;
       I8 const* in = first ? cands.data() : a.data();
 #define OSH_IS(F)                                                                         \
   do {                                                                                    \
     if (g)                                                                                \
       OSH_LAUNCH(indset_iteration<F>, n, c, g->a2ab.data(), g->ab2b.data(), quality.data(), \
-          in, b.data());                                                                  \
+          in, b.data(), dom.data());                                                      \
     else if (m.dim == 3)                                                                  \
       OSH_LAUNCH((indset_iteration_edges<6, F>), n, c, e2c->a2ab.data(), e2c->ab2b.data(),  \
-          c2e, quality.data(), in, b.data());                                             \
+          c2e, quality.data(), in, b.data(), dom.data());                                 \
     else                                                                                  \
       OSH_LAUNCH((indset_iteration_edges<3, F>), n, c, e2c->a2ab.data(), e2c->ab2b.data(),  \
-          c2e, quality.data(), in, b.data());                                             \
+          c2e, quality.data(), in, b.data(), dom.data());                                 \
   } while (0)
       if (first)
         OSH_IS(true);
