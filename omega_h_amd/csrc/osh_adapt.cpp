@@ -23,7 +23,7 @@ double now() {
 // *qmin receives the minimum element quality after the pass, reduced before
 // the pass's closing sync so that it costs no sync of its own
 bool run_pass(Ctx& c, DMesh& m, AdaptOptions const& o, int kind,
-    std::vector<PassRecord>* log, Real* qmin) {
+    std::vector<PassRecord>* log, Real* qmin, SwapCache* cache = nullptr) {
   OSH_CHECK(!log || log->size() < 4096, "adapt exceeded 4096 passes");
   LO nin = m.nents[m.dim];
   c.sync();
@@ -34,7 +34,7 @@ bool run_pass(Ctx& c, DMesh& m, AdaptOptions const& o, int kind,
   else if (kind == 1)
     r = coarsen_by_size(c, m, o);
   else if (kind == 2)
-    r = swap_edges(c, m, o);
+    r = swap_edges(c, m, o, cache);
   else
     r = coarsen_slivers(c, m, o);
   min_real_async(c, ask_qualities(c, m), kQminSlot);
@@ -66,8 +66,12 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
     if (o.should_coarsen && run_pass(c, m, o, 1, log, &q)) did = true;
   } while (did);
   if (q >= o.min_quality_desired) return true;
+  // swap choices carry over between swap passes; a sliver pass rebuilds
+  // the mesh and drops them
+  SwapCache cache;
   do {
-    if (o.should_swap && run_pass(c, m, o, 2, log, &q)) continue;
+    if (o.should_swap && run_pass(c, m, o, 2, log, &q, &cache)) continue;
+    cache.clear();
     if (o.should_coarsen_slivers && run_pass(c, m, o, 3, log, &q)) continue;
     return true;
   } while (q < o.min_quality_desired);

@@ -98,7 +98,24 @@ LOs degrees_of(Ctx& c, LOs const& a2ab, LOs const& rows);
 bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o);
 bool coarsen_by_size(Ctx& c, DMesh& m, AdaptOptions const& o);
 bool coarsen_slivers(Ctx& c, DMesh& m, AdaptOptions const& o);
-bool swap_edges(Ctx& c, DMesh& m, AdaptOptions const& o);
+// The 3D swap choice of each edge (its configuration, whether it improves
+// on the edge's elements and the quality it reaches) is a function of the
+// tets around the edge, the coordinates and metric of their vertices and
+// max_length_allowed only.  Between swap passes of one adapt() nothing else
+// moves, so the choices of edges whose tets a pass did not touch carry over
+// to the next mesh; `valid` marks them, per edge of the current mesh.
+struct SwapCache {
+  Bytes valid;
+  Bytes cfg;
+  Bytes keep;
+  Reals q;
+  void clear() { *this = SwapCache(); }
+};
+
+// with a cache (only for consecutive swap passes of one adapt(), cleared by
+// the caller whenever anything else changes the mesh, its metric or the
+// options) the 3D pass evaluates only the candidates not carried over
+bool swap_edges(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache = nullptr);
 
 // known-answer hooks used by the tests
 Reals refine_qualities(Ctx& c, DMesh& m, LOs const& cands);

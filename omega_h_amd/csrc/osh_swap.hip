@@ -101,8 +101,8 @@ __device__ double length_between(LO a, LO b, Real const* coords, Real const* met
 // Every value is the one the reference computes (each unique triangle and
 // edge evaluated once), so the choice is the same.
 __global__ void swap3d_loops(LO ncands, LO const* cands, LO const* e2t_a2ab, LO const* e2t_ab2b,
-    I8 const* e2t_codes, LO const* ev2v, LO const* tv2v, Tables const* tb, LO* lsize,
-    LO* leev, LO* lverts, LO* ntris, LO* nedges) {
+    I8 const* e2t_codes, LO const* ev2v, LO const* tv2v, Tables const* tb, I8 const* cached,
+    LO* lsize, LO* leev, LO* lverts, LO* ntris, LO* nedges) {
   LO cand = thread_index();
   if (cand >= ncands) return;
   Loop loop;
@@ -117,8 +117,10 @@ __global__ void swap3d_loops(LO ncands, LO const* cands, LO const* e2t_a2ab, LO 
   leev[cand * 2 + 1] = loop.eev2v[1];
   for (int k = 0; k < swap::kMaxLoop; ++k)
     lverts[cand * swap::kMaxLoop + k] = (k < loop.size) ? loop.verts[k] : -1;
-  ntris[cand] = tb->ntris[loop.size];
-  nedges[cand] = tb->nuedges[loop.size];
+  // a carried-over choice needs the loop (for products) but no qualities
+  bool carried = cached && cached[cands[cand]];
+  ntris[cand] = carried ? 0 : tb->ntris[loop.size];
+  nedges[cand] = carried ? 0 : tb->nuedges[loop.size];
 }
 
 // the candidate whose range [off[c], off[c + 1]) holds item i
@@ -188,6 +190,7 @@ __global__ void swap3d_mesh_quals(LO nt, LO const* lsize, LO const* toff, LO con
   int mesh = t % kMaxMeshes;
   int n = lsize[cand];
   if (n > swap::kMaxLoop || mesh >= tb->mesh_counts[n]) return;
+  if (toff[cand + 1] == toff[cand]) return;  // a carried-over choice
   int nmesh_tris = tb->mesh_sizes[n];
   int nmesh_edges = tb->nedges[n];
   Real const* ctq = tq + toff[cand];
@@ -202,10 +205,18 @@ __global__ void swap3d_mesh_quals(LO nt, LO const* lsize, LO const* toff, LO con
 
 __global__ void swap3d_choose(LO ncands, LO const* cands, LO const* lsize, Real const* mq,
     LO const* e2t_a2ab, LO const* e2t_ab2b, Real const* elem_quals, Tables const* tb,
-    I8* configs, Real* quals, I8* keep) {
+    I8* configs, Real* quals, I8* keep, I8* c_valid, I8* c_cfg, I8* c_keep, Real* c_q) {
   LO cand = thread_index();
   if (cand >= ncands) return;
   LO edge = cands[cand];
+  if (c_valid && c_valid[edge]) {
+    configs[edge] = c_cfg[edge];
+    if (c_keep[edge]) {
+      keep[edge] = 1;
+      quals[edge] = c_q[edge];
+    }
+    return;
+  }
   int n = lsize[cand];
   int choice_mesh = -1;
   double choice_quality = -1.0;
@@ -227,9 +238,39 @@ __global__ void swap3d_choose(LO ncands, LO const* cands, LO const* lsize, Real 
     old = math::min2(old, elem_quals[e2t_ab2b[j]]);
   // a key needs a triangulation; with inverted elements the reference can
   // keep an edge without one and then index its tables out of bounds
-  if (choice_mesh >= 0 && choice_quality > old) {
+  bool kept = choice_mesh >= 0 && choice_quality > old;
+  if (kept) {
     keep[edge] = 1;
     quals[edge] = choice_quality;
+  }
+  if (c_valid) {
+    c_cfg[edge] = I8(choice_mesh);
+    c_keep[edge] = I8(kept);
+    c_q[edge] = choice_quality;
+    c_valid[edge] = 1;
+  }
+}
+
+// the edges of the tets the pass replaces: their tet rings change
+__global__ void swap3d_mark_dirty(LO ntets, I8 const* tet_same, LO const* t2e, I8* dirty) {
+  LO t = thread_index();
+  if (t >= ntets || tet_same[t]) return;
+  for (int i = 0; i < 6; ++i) dirty[t2e[t * 6 + i]] = 1;
+}
+
+// the carried-over choices of the same edges whose tets did not change
+__global__ void swap3d_cache_transfer(LO nsame, LO const* same2old, LO const* same2new,
+    I8 const* dirty, I8 const* valid, I8 const* cfg, I8 const* keep, Real const* q,
+    I8* n_valid, I8* n_cfg, I8* n_keep, Real* n_q) {
+  LO i = thread_index();
+  if (i >= nsame) return;
+  LO o = same2old[i], w = same2new[i];
+  bool ok = valid[o] && !dirty[o];
+  n_valid[w] = I8(ok);
+  if (ok) {
+    n_cfg[w] = cfg[o];
+    n_keep[w] = keep[o];
+    n_q[w] = q[o];
   }
 }
 
@@ -435,7 +476,8 @@ Tables const* device_tables(Ctx& c) {
 
 // `plans` (when given) are the rebuild plans of dimensions 1..dim, scanned
 void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO const* nprods,
-    LOs const* pv, ModPlan* plans = nullptr) {
+    LOs const* pv, ModPlan* plans = nullptr, SwapCache* cache = nullptr,
+    Bytes const* dirty = nullptr) {
   int dim = m.dim;
   LO nv = m.nents[0];
   DMesh nw;
@@ -461,6 +503,18 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
     modify_build(c, m, nw, plans[d - 1], keys2edges, k2p[d], nprods[d], pv[d],
         old_lows2new_lows, mo, nullptr, &src);
     transfer_class(c, m, nw, d, mo, src, 1, nullptr);
+    if (d == 1 && cache) {
+      SwapCache next;
+      next.valid = fill_i8(c, mo.nnew, 0);
+      next.cfg = Bytes(c, mo.nnew);
+      next.keep = Bytes(c, mo.nnew);
+      next.q = Reals(c, mo.nnew);
+      LO nsame = mo.same2old.size();
+      OSH_LAUNCH(swap3d_cache_transfer, nsame, c, mo.same2old.data(), mo.same2new.data(),
+          dirty->data(), cache->valid.data(), cache->cfg.data(), cache->keep.data(),
+          cache->q.data(), next.valid.data(), next.cfg.data(), next.keep.data(), next.q.data());
+      *cache = next;
+    }
     if (d == dim) transfer_pointwise(c, m, nw, mo, 1, keys2edges, k2p[d], pv[d]);
     transfer_measures(c, m, nw, d, mo);
     old_lows2new_lows = mo.old2new;
@@ -484,7 +538,7 @@ Bytes swap_candidates(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes* configs, B
   return edges;
 }
 
-bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
+bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
   ask_lengths(c, m);
   Bytes configs, keep;
   Reals quals;
@@ -499,10 +553,17 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   LOs const& tv2v = verts_of(c, m, 3);
   Reals const& eq = ask_qualities(c, m);
   LO ncands = cands.size();
+  if (cache && !cache->valid.exists()) {
+    cache->valid = fill_i8(c, ne, 0);
+    cache->cfg = Bytes(c, ne);
+    cache->keep = Bytes(c, ne);
+    cache->q = Reals(c, ne);
+  }
   LOs lsize(c, ncands), leev(c, ncands * 2), lverts(c, ncands * swap::kMaxLoop);
   LOs ntris(c, ncands), nedges(c, ncands);
   OSH_LAUNCH(swap3d_loops, ncands, c, cands.data(), e2t.a2ab.data(), e2t.ab2b.data(),
-      e2t.codes.data(), m.down[1].data(), tv2v.data(), tb, lsize.data(), leev.data(),
+      e2t.codes.data(), m.down[1].data(), tv2v.data(), tb,
+      cache ? cache->valid.data() : (I8 const*)nullptr, lsize.data(), leev.data(),
       lverts.data(), ntris.data(), nedges.data());
   // the unique triangles and edges of all loops, sized by their upper bounds
   // so that no sync waits for the totals
@@ -527,7 +588,9 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   OSH_LAUNCH(swap3d_mesh_quals, ncands * kMaxMeshes, c, lsize.data(), toff.data(), eoff.data(),
       tq.data(), el.data(), o.max_length_allowed, tb, mq.data());
   OSH_LAUNCH(swap3d_choose, ncands, c, cands.data(), lsize.data(), mq.data(), e2t.a2ab.data(),
-      e2t.ab2b.data(), eq.data(), tb, configs.data(), quals.data(), keep.data());
+      e2t.ab2b.data(), eq.data(), tb, configs.data(), quals.data(), keep.data(),
+      cache ? cache->valid.data() : (I8*)nullptr, cache ? cache->cfg.data() : (I8*)nullptr,
+      cache ? cache->keep.data() : (I8*)nullptr, cache ? cache->q.data() : (Real*)nullptr);
   bool any;
   Bytes keys = find_indset(c, m, 1, quals, keep, &any, &e2t);
   if (!any) return false;
@@ -569,7 +632,13 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   OSH_LAUNCH(swap3d_products, nkeys * kLanes, c, keys2edges.data(), configs.data(),
       cand_of_edge.data(), lsize.data(), leev.data(), lverts.data(), tb, k2p[1].data(),
       k2p[2].data(), k2p[3].data(), pv[1].data(), pv[2].data(), pv[3].data());
-  swap_rebuild(c, m, keys2edges, k2p, nprods, pv, plans);
+  Bytes dirty;
+  if (cache) {
+    dirty = fill_i8(c, ne, 0);
+    OSH_LAUNCH(swap3d_mark_dirty, m.nents[3], c, plans[2].same.data(),
+        ask_down(c, m, 3, 1, nullptr).data(), dirty.data());
+  }
+  swap_rebuild(c, m, keys2edges, k2p, nprods, pv, plans, cache, cache ? &dirty : nullptr);
   return true;
 }
 
@@ -611,8 +680,8 @@ bool swap_edges_2d(Ctx& c, DMesh& m, AdaptOptions const& o) {
 
 }  // namespace
 
-bool swap_edges(Ctx& c, DMesh& m, AdaptOptions const& o) {
-  if (m.dim == 3) return swap_edges_3d(c, m, o);
+bool swap_edges(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
+  if (m.dim == 3) return swap_edges_3d(c, m, o, cache);
   return swap_edges_2d(c, m, o);
 }
 
