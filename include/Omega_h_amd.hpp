@@ -27,7 +27,15 @@
 
 #include "omega_h_amd.h"
 
+// C enums of src/Omega_h_defines.h, at global scope as there
 enum Omega_h_Family { OMEGA_H_SIMPLEX = 0 };
+// transfer types of tags carried through adapt() (src/Omega_h_adapt.hpp:30-42)
+enum Omega_h_Transfer { OMEGA_H_LINEAR_INTERP = 0, OMEGA_H_METRIC = 1, OMEGA_H_POINTWISE = 2 };
+// how a metric is made isotropic
+enum Omega_h_Isotropy { OMEGA_H_ANISOTROPIC = 0, OMEGA_H_ISO_LENGTH = 1, OMEGA_H_ISO_SIZE = 2 };
+// how a distributed mesh is partitioned; one rank holds the whole mesh, so
+// every choice is the same here
+enum Omega_h_Parting { OMEGA_H_ELEM_BASED = 0, OMEGA_H_GHOSTED = 1, OMEGA_H_VERT_BASED = 2 };
 
 namespace Omega_h {
 
@@ -41,10 +49,6 @@ enum { VERT = 0, EDGE = 1, FACE = 2, REGION = 3 };
 
 enum Verbosity { SILENT = 0, EACH_ADAPT = 1, EACH_REBUILD = 2, EXTRA_STATS = 3 };
 
-// transfer types of vertex tags carried through adapt() (src/Omega_h_adapt.hpp:30-42)
-enum Omega_h_Transfer { OMEGA_H_LINEAR_INTERP = 0, OMEGA_H_METRIC = 1, OMEGA_H_POINTWISE = 2 };
-// src/Omega_h_defines.h: how a metric is made isotropic
-enum Omega_h_Isotropy { OMEGA_H_ANISOTROPIC = 0, OMEGA_H_ISO_LENGTH = 1, OMEGA_H_ISO_SIZE = 2 };
 
 struct exception : public std::runtime_error {
   explicit exception(std::string const& msg) : std::runtime_error(msg) {}
@@ -127,7 +131,10 @@ class Library;
 // stands in for the communicator of build_box(lib.world(), ...): one rank
 struct Comm {
   Library* library;
+  int rank() const { return 0; }
+  int size() const { return 1; }
 };
+
 using CommPtr = std::shared_ptr<Comm>;
 
 // one HIP stream on one GPU (the device is the only argument that matters)
@@ -182,6 +189,12 @@ class Mesh {
     return h_.get();
   }
   Omega_h_Family family() const { return OMEGA_H_SIMPLEX; }
+  CommPtr comm() const { return std::make_shared<Comm>(Comm{library_}); }
+  Omega_h_Parting parting() const { return OMEGA_H_ELEM_BASED; }
+  void set_parting(Omega_h_Parting, Int nlayers = 0, bool verbose = false) {
+    (void)nlayers;
+    (void)verbose;
+  }
   Int dim() const {
     int d;
     check_status(osh_mesh_dim(handle(), &d));

@@ -51,6 +51,7 @@ int main(int argc, char** argv) {
         Omega_h::Reals(Omega_h::Write<Omega_h::Real>(read_doubles(argv[4], nv * ncomps))));
     mesh.add_tag(Omega_h::VERT, "warp", dim,
         Omega_h::Reals(Omega_h::Write<Omega_h::Real>(read_doubles(argv[5], nv * dim))));
+    mesh.set_parting(OMEGA_H_GHOSTED);
     Omega_h::AdaptOpts opts(&mesh);
     opts.verbosity = Omega_h::SILENT;
     int steps = 0;
@@ -72,7 +73,8 @@ int main(int argc, char** argv) {
     std::vector<int> d(dual.a2ab.data(), dual.a2ab.data() + dual.a2ab.size());
     d.insert(d.end(), dual.ab2b.data(), dual.ab2b.data() + dual.ab2b.size());
     write_file(prefix + ".dual", d.data(), d.size());
-    std::printf("warped in %d steps: %d elements\n", steps, mesh.nelems());
+    if (mesh.comm()->rank() == 0)
+      std::printf("warped in %d steps: %d elements\n", steps, mesh.nelems());
   } catch (Omega_h::exception const& e) {
     std::cerr << "Omega_h::exception: " << e.what() << "\n";
     return 1;
