@@ -123,28 +123,15 @@ __global__ void swap3d_loops(LO ncands, LO const* cands, LO const* e2t_a2ab, LO 
   nedges[cand] = carried ? 0 : tb->nuedges[loop.size];
 }
 
-// the candidate whose range [off[c], off[c + 1]) holds item i
-__device__ LO owner_of(LO i, LO n, LO const* off) {
-  LO lo = 0, hi = n;
-  while (hi - lo > 1) {
-    LO mid = lo + (hi - lo) / 2;
-    if (off[mid] <= i)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
 template <int M>
-__global__ void swap3d_tri_quals(LO nt, LO ncands, LO const* toff, LO const* lsize,
-    LO const* leev, LO const* lverts, Real const* coords, Real const* metric, Tables const* tb,
-    Real* tq) {
-  // launched over an upper bound; the real count is the offsets' total
+__global__ void swap3d_tri_quals(LO nt, LO const* ntris, LO const* lsize, LO const* leev,
+    LO const* lverts, Real const* coords, Real const* metric, Tables const* tb, Real* tq) {
+  // a fixed stride of kMaxUniqueTris slots per candidate
   LO p = thread_index();
-  if (p >= nt || p >= toff[ncands]) return;
-  LO cand = owner_of(p, ncands, toff);
-  int ut = p - toff[cand];
+  if (p >= nt) return;
+  LO cand = p / swap::kMaxUniqueTris;
+  int ut = p % swap::kMaxUniqueTris;
+  if (ut >= ntris[cand]) return;
   int n = lsize[cand];
   LO tv[4];
   for (int k = 0; k < 3; ++k) tv[k] = lverts[cand * swap::kMaxLoop + tb->tris[n][ut][k]];
@@ -160,13 +147,14 @@ __global__ void swap3d_tri_quals(LO nt, LO ncands, LO const* toff, LO const* lsi
 }
 
 template <int M>
-__global__ void swap3d_edge_lens(LO nt, LO ncands, LO const* eoff, LO const* lsize,
-    LO const* lverts, Real const* coords, Real const* metric, Tables const* tb, Real* el) {
-  // launched over an upper bound; the real count is the offsets' total
+__global__ void swap3d_edge_lens(LO nt, LO const* nedges, LO const* lsize, LO const* lverts,
+    Real const* coords, Real const* metric, Tables const* tb, Real* el) {
+  // a fixed stride of kMaxUniqueEdges slots per candidate
   LO q = thread_index();
-  if (q >= nt || q >= eoff[ncands]) return;
-  LO cand = owner_of(q, ncands, eoff);
-  int ue = q - eoff[cand];
+  if (q >= nt) return;
+  LO cand = q / swap::kMaxUniqueEdges;
+  int ue = q % swap::kMaxUniqueEdges;
+  if (ue >= nedges[cand]) return;
   int n = lsize[cand];
   LO const* lv = lverts + cand * swap::kMaxLoop;
   el[q] = length_between<3, M>(lv[tb->uedges[n][ue][0]], lv[tb->uedges[n][ue][1]], coords,
@@ -182,19 +170,19 @@ __global__ void swap3d_edge_lens(LO nt, LO ncands, LO const* eoff, LO const* lsi
 // negative (where choose() stops early) is never kept either way.
 constexpr int kMaxMeshes = 42;
 
-__global__ void swap3d_mesh_quals(LO nt, LO const* lsize, LO const* toff, LO const* eoff,
-    Real const* tq, Real const* el, Real max_length, Tables const* tb, Real* mq) {
+__global__ void swap3d_mesh_quals(LO nt, LO const* lsize, LO const* ntris, Real const* tq,
+    Real const* el, Real max_length, Tables const* tb, Real* mq) {
   LO t = thread_index();
   if (t >= nt) return;
   LO cand = t / kMaxMeshes;
   int mesh = t % kMaxMeshes;
   int n = lsize[cand];
   if (n > swap::kMaxLoop || mesh >= tb->mesh_counts[n]) return;
-  if (toff[cand + 1] == toff[cand]) return;  // a carried-over choice
+  if (ntris[cand] == 0) return;  // a carried-over choice
   int nmesh_tris = tb->mesh_sizes[n];
   int nmesh_edges = tb->nedges[n];
-  Real const* ctq = tq + toff[cand];
-  Real const* cel = el + eoff[cand];
+  Real const* ctq = tq + LO(cand) * swap::kMaxUniqueTris;
+  Real const* cel = el + LO(cand) * swap::kMaxUniqueEdges;
   double q = 1.0;
   for (int mt = 0; mt < nmesh_tris; ++mt)
     q = math::min2(q, ctq[tb->meshes[n][mesh * nmesh_tris + mt]]);
@@ -565,27 +553,23 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
       e2t.codes.data(), m.down[1].data(), tv2v.data(), tb,
       cache ? cache->valid.data() : (I8 const*)nullptr, lsize.data(), leev.data(),
       lverts.data(), ntris.data(), nedges.data());
-  // the unique triangles and edges of all loops, sized by their upper bounds
-  // so that no sync waits for the totals
-  LOs toff(c, ncands + 1), eoff(c, ncands + 1);
-  ScanJob loop_jobs[2] = {{ntris.data(), kScanLO, ncands, toff.data(), nullptr},
-      {nedges.data(), kScanLO, ncands, eoff.data(), nullptr}};
-  offset_scans(c, loop_jobs, 2);
+  // the unique triangles and edges of all loops at fixed strides per
+  // candidate, so that neither a scan nor a sync sizes them
   LO ntp = ncands * swap::kMaxUniqueTris, nep = ncands * swap::kMaxUniqueEdges;
   Reals tq(c, ntp), el(c, nep);
   if (md == 3) {
-    OSH_LAUNCH(swap3d_tri_quals<3>, ntp, c, ncands, toff.data(), lsize.data(), leev.data(),
+    OSH_LAUNCH(swap3d_tri_quals<3>, ntp, c, ntris.data(), lsize.data(), leev.data(),
         lverts.data(), m.coords.data(), m.metric.data(), tb, tq.data());
-    OSH_LAUNCH(swap3d_edge_lens<3>, nep, c, ncands, eoff.data(), lsize.data(), lverts.data(),
+    OSH_LAUNCH(swap3d_edge_lens<3>, nep, c, nedges.data(), lsize.data(), lverts.data(),
         m.coords.data(), m.metric.data(), tb, el.data());
   } else {
-    OSH_LAUNCH(swap3d_tri_quals<1>, ntp, c, ncands, toff.data(), lsize.data(), leev.data(),
+    OSH_LAUNCH(swap3d_tri_quals<1>, ntp, c, ntris.data(), lsize.data(), leev.data(),
         lverts.data(), m.coords.data(), m.metric.data(), tb, tq.data());
-    OSH_LAUNCH(swap3d_edge_lens<1>, nep, c, ncands, eoff.data(), lsize.data(), lverts.data(),
+    OSH_LAUNCH(swap3d_edge_lens<1>, nep, c, nedges.data(), lsize.data(), lverts.data(),
         m.coords.data(), m.metric.data(), tb, el.data());
   }
   Reals mq(c, ncands * kMaxMeshes);
-  OSH_LAUNCH(swap3d_mesh_quals, ncands * kMaxMeshes, c, lsize.data(), toff.data(), eoff.data(),
+  OSH_LAUNCH(swap3d_mesh_quals, ncands * kMaxMeshes, c, lsize.data(), ntris.data(),
       tq.data(), el.data(), o.max_length_allowed, tb, mq.data());
   OSH_LAUNCH(swap3d_choose, ncands, c, cands.data(), lsize.data(), mq.data(), e2t.a2ab.data(),
       e2t.ab2b.data(), eq.data(), tb, configs.data(), quals.data(), keep.data(),
