@@ -4,7 +4,7 @@
 Workload: build_box unit cube 64^3 cells, 6 tets per cell (1,572,864 tets),
 anisotropic metric around the level set phi = z - 0.5 - 0.1 sin(2 pi x)
 (normal eigen-length 0.002 + 0.048 (1 - exp(-|phi|/0.02)), tangential 0.05),
-one full adapt() with default AdaptOpts, float64 throughout.
+one full adapt() with default AdaptOpts (verbosity SILENT), float64 throughout.
 
 A step is one adapt() of a fresh copy of that input mesh (the copies are made
 before the timed region; the mesh lives in HBM the whole time).  `value` is
@@ -144,6 +144,8 @@ def serve_main(args):
     import omega_h_amd as A
     ctx = A.Context(0)
     opts = {2: ctx.default_opts(2), 3: ctx.default_opts(3)}
+    for o in opts.values():
+        o.verbosity = 0  # SILENT, as the workload specifies
     rng = np.random.default_rng(args.seed)
     out = []
     for rate in args.rates:
@@ -202,6 +204,7 @@ def batch_main(args):
     import omega_h_amd as A
     ctx = A.Context(0)
     opts = ctx.default_opts(3)
+    opts.verbosity = 0  # SILENT, as the workload specifies
     rng = np.random.default_rng(args.seed)
     boxes = []
     for _ in range(args.boxes):
@@ -310,6 +313,7 @@ def main():
     met, nc = W.levelset_metric_3d(base.coords())
     base.set_metric(met, nc)
     opts = ctx.default_opts(3)
+    opts.verbosity = 0  # SILENT, as the workload specifies
     nelems0 = base.nents(3)
     log(f"input: {args.n}^3 box, {nelems0} tets, {base.nents(0)} verts")
     if args.roofline_only:

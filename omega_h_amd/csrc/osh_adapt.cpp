@@ -6,12 +6,47 @@
 // coarsen_slivers call that rebuilt the mesh; each is timed on the host
 // clock with the stream drained at both ends.
 #include <chrono>
+#include <cstdio>
+#include <vector>
 
 #include "osh_passes.hpp"
 
 namespace osh {
 
 namespace {
+
+char const* plural_name(int d) {
+  return d == 3 ? "tets" : (d == 2 ? "triangles" : (d == 1 ? "edges" : "vertices"));
+}
+
+// print_goal_stats, src/Omega_h_histogram.cpp:56-85
+void print_goal_stats(Ctx& c, int d, char const* name, Reals const& values, Real lo, Real hi) {
+  std::vector<Real> v = to_host(c, values);
+  LO nlow = 0, nhigh = 0;
+  Real mn = v.empty() ? 0.0 : v[0], mx = mn;
+  for (Real x : v) {
+    if (x < lo) ++nlow;
+    if (x > hi) ++nhigh;
+    mn = (x < mn) ? x : mn;
+    mx = (x > mx) ? x : mx;
+  }
+  LO nmid = LO(v.size()) - nlow - nhigh;
+  std::printf("%d %s, %s [%.2f,%.2f]", int(v.size()), plural_name(d), name, mn, mx);
+  if (nlow) std::printf(", %d <%.2f", nlow, lo);
+  if (nmid) std::printf(", %d in [%.2f,%.2f]", nmid, lo, hi);
+  if (nhigh) std::printf(", %d >%.2f", nhigh, hi);
+  std::printf("\n");
+}
+
+// print_adapt_status's report, src/Omega_h_adapt.cpp:101-120
+void print_status(Ctx& c, DMesh& m, AdaptOptions const& o) {
+  if (o.verbosity <= 0) return;
+  print_goal_stats(c, m.dim, "quality", ask_qualities(c, m), o.min_quality_allowed,
+      o.min_quality_desired);
+  print_goal_stats(c, 1, "length", ask_lengths(c, m), o.min_length_desired,
+      o.max_length_desired);
+  std::fflush(stdout);
+}
 
 constexpr int kQminSlot = Ctx::kSlots - 1;
 
@@ -42,6 +77,8 @@ bool run_pass(Ctx& c, DMesh& m, AdaptOptions const& o, int kind,
   double t1 = now();
   *qmin = read_slot<Real>(c, kQminSlot);
   if (r && log) log->push_back({kind, nin, m.nents[m.dim], t1 - t0});
+  // post_rebuild, src/Omega_h_adapt.cpp:169-171
+  if (r && o.verbosity >= 2) print_status(c, m, o);
   return r;
 }
 
@@ -52,12 +89,20 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
   OSH_CHECK(o.min_quality_allowed <= o.min_quality_desired, "min_quality_allowed > desired");
   OSH_CHECK(o.min_quality_desired <= 1.0, "min_quality_desired > 1");
   OSH_CHECK(o.nsliver_layers >= 0 && o.nsliver_layers < 100, "nsliver_layers out of range");
+  // the reports of pre_adapt/post_adapt (src/Omega_h_adapt.cpp:153-166,
+  // 244-270) at the reference's verbosity levels: 1 EACH_ADAPT, 2
+  // EACH_REBUILD (with the passes' own lines), 3 EXTRA_STATS (no histograms)
+  double t0 = now();
+  if (o.verbosity >= 1) std::printf("before adapting:\n");
+  print_status(c, m, o);
   Reals const& l = ask_lengths(c, m);
   Real lmin = min_real(c, l), lmax = max_real(c, l);
   Real qmin = min_real(c, ask_qualities(c, m));
   if (qmin >= o.min_quality_desired && lmin >= o.min_length_desired &&
       lmax <= o.max_length_desired)
     return false;
+  if (o.verbosity >= 2) std::printf("addressing edge lengths\n");
+  double t1 = now();
   bool did;
   Real q = qmin;  // the minimum quality of the current mesh
   do {
@@ -65,16 +110,31 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
     if (o.should_refine && run_pass(c, m, o, 0, log, &q)) did = true;
     if (o.should_coarsen && run_pass(c, m, o, 1, log, &q)) did = true;
   } while (did);
-  if (q >= o.min_quality_desired) return true;
-  // swap choices carry over between swap passes; a sliver pass rebuilds
-  // the mesh and drops them
-  SwapCache cache;
-  do {
-    if (o.should_swap && run_pass(c, m, o, 2, log, &q, &cache)) continue;
-    cache.clear();
-    if (o.should_coarsen_slivers && run_pass(c, m, o, 3, log, &q)) continue;
-    return true;
-  } while (q < o.min_quality_desired);
+  double t2 = now();
+  if (q < o.min_quality_desired) {
+    if (o.verbosity >= 2) std::printf("addressing element qualities\n");
+    // swap choices carry over between swap passes; a sliver pass rebuilds
+    // the mesh and drops them
+    SwapCache cache;
+    do {
+      if (o.should_swap && run_pass(c, m, o, 2, log, &q, &cache)) continue;
+      cache.clear();
+      if (o.should_coarsen_slivers && run_pass(c, m, o, 3, log, &q)) continue;
+      if (o.verbosity >= 1) std::printf("could not satisfy quality\n");
+      break;
+    } while (q < o.min_quality_desired);
+  }
+  double t3 = now();
+  if (o.verbosity == 1) {
+    std::printf("after adapting:\n");
+    print_status(c, m, o);
+  }
+  if (o.verbosity >= 1) {
+    std::printf("addressing edge lengths took %g seconds\n", t2 - t1);
+    std::printf("addressing element qualities took %g seconds\n", t3 - t2);
+    std::printf("adapting took %g seconds\n\n", now() - t0);
+    std::fflush(stdout);
+  }
   return true;
 }
 

@@ -11,6 +11,8 @@
 //   mark_dead_ents / find_coarsen_domains / coarsen_topology
 //                                    src/Omega_h_coarsen_topology.cpp:15-122
 //   transfer_coarsen                 src/Omega_h_transfer.cpp:585-619
+#include <cstdio>
+
 #include "osh_math.hpp"
 #include "osh_passes.hpp"
 
@@ -479,6 +481,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   Bytes keys = find_indset(c, m, 0, vqual, vcand, nullptr, nullptr, &star);
   LOs keys2verts = collect_marked(c, keys);
   LO nkeys = keys2verts.size();
+  if (o.verbosity >= 2) std::printf("coarsening %d vertices\n", int(nkeys));
   LOs rails2edges(c, nkeys), col_dirs(c, nkeys), keys2onto(c, nkeys);
   OSH_LAUNCH(find_rails_kernel, nkeys, c, keys2verts.data(), vrail.data(), v2e.a2ab.data(),
       v2e.ab2b.data(), v2e.codes.data(), ev2v, rails2edges.data(), col_dirs.data(),

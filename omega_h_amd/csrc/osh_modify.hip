@@ -942,7 +942,7 @@ void default_options(int dim, AdaptOptions& o) {
     o.min_quality_desired = 0.40;
   }
   o.nsliver_layers = 4;
-  o.verbosity = 0;
+  o.verbosity = 2;  // EACH_REBUILD
   o.should_refine = 1;
   o.should_coarsen = 1;
   o.should_swap = 1;

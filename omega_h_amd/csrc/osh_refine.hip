@@ -6,6 +6,8 @@
 //                                             src/Omega_h_refine_topology.cpp:13-203
 //   transfer_refine (coords, metric, class, length, quality)
 //                                             src/Omega_h_transfer.cpp:182-428
+#include <cstdio>
+
 #include "osh_math.hpp"
 #include "osh_passes.hpp"
 
@@ -218,6 +220,7 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
   if (!any) return false;
   LOs keys2edges = collect_marked(c, keys);
   LO nkeys = keys2edges.size();
+  if (o.verbosity >= 2) std::printf("refining %d edges\n", int(nkeys));
   int nc = m.metric_ncomps;
   DMesh nw;
   nw.dim = dim;

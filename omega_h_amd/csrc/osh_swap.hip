@@ -10,6 +10,8 @@
 //   swap2d_qualities / swap2d_topology           src/Omega_h_swap2d_qualities.cpp:16-72,
 //                                                src/Omega_h_swap2d_topology.cpp:11-79
 //   transfer_swap / transfer_copy                src/Omega_h_transfer.cpp:631-730
+#include <cstdio>
+
 #include "osh_math.hpp"
 #include "osh_passes.hpp"
 #include "osh_swap_tables.hpp"
@@ -600,6 +602,7 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
   c.sync();
   modify_scan_totals(c, plans, 3, 4);
   LO nkeys = read_slot<LO>(c, 0);
+  if (o.verbosity >= 2) std::printf("swapping %d 3D edges\n", int(nkeys));
   LOs keys2edges(c, nkeys);
   LOs k2p[4], pv[4];
   LO nprods[4] = {0, 0, 0, 0};
@@ -650,6 +653,7 @@ bool swap_edges_2d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   if (!any) return false;
   LOs keys2edges = collect_marked(c, keys);
   LO nkeys = keys2edges.size();
+  if (o.verbosity >= 2) std::printf("swapping %d 2D edges\n", int(nkeys));
   LOs k2p[4], pv[4];
   k2p[1] = iota(c, nkeys + 1);
   k2p[2] = iota(c, nkeys + 1, 0, 2);

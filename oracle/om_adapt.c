@@ -65,7 +65,7 @@ void om_default_opts(int dim, om_adapt_opts* o) {
     o->min_quality_desired = 0.40;
   }
   o->nsliver_layers = 4;
-  o->verbosity = 0;
+  o->verbosity = 2; /* EACH_REBUILD, as the reference; the oracle prints nothing */
   o->should_refine = 1;
   o->should_coarsen = 1;
   o->should_swap = 1;

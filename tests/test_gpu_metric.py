@@ -329,3 +329,26 @@ def test_pointwise_elem_tags_identical(ctx, dim, n):
     assert same_bits(o.elem_tag("field"), g.elem_tag("field"))
     # the values are the original ones, moved around
     assert np.isin(g.elem_tag("field"), vals).all()
+
+
+def test_verbosity_reports(ctx, capfd):
+    """The reports of adapt() at the reference's verbosity levels
+    (src/Omega_h_adapt.cpp:101-288, src/Omega_h_histogram.cpp:56-85)."""
+    assert ctx.default_opts(3).verbosity == 2  # EACH_REBUILD, as AdaptOpts(dim)
+    o = O.Mesh.box(2, 16)
+    met, nc = W.iso_ring_metric_2d(o.coords())
+    g = A.Mesh.box(ctx, 2, 16)
+    g.set_metric(met, nc)
+    opts = ctx.default_opts(2)
+    did, passes = g.adapt(opts)
+    out = capfd.readouterr().out
+    assert "before adapting:" in out and "addressing edge lengths" in out
+    assert "512 triangles, quality [" in out and "edges, length [" in out
+    assert ("refining " in out) and ("coarsening " in out)
+    assert "adapting took " in out
+    assert out.count("triangles, quality [") == 1 + len(passes)
+    g = A.Mesh.box(ctx, 2, 16)
+    g.set_metric(met, nc)
+    opts.verbosity = 0
+    g.adapt(opts)
+    assert capfd.readouterr().out == ""
