@@ -11,6 +11,9 @@
 //   compute_ill_metric_dim     src/Omega_h_fix.cpp:16-68
 //   fix_for_given_metric, fix  src/Omega_h_fix.cpp:90-134
 //   grade_fix_adapt            src/Omega_h_fix.cpp:136-176
+#include <cmath>
+#include <cstdio>
+
 #include "osh_metric.hpp"
 #include "osh_passes.hpp"
 
@@ -374,6 +377,7 @@ bool approach_metric(Ctx& c, DMesh& m, AdaptOptions const& o, Real min_step) {
     interpolate_metric_arrays(c, mdim, m.nents[0], orig, target, factor, cur);
     replace_metric(m, cur);
   } while (!acceptable(c, m, o));
+  if (o.verbosity >= 2) std::printf("approach_metric moved by factor %g\n", factor);
   return true;
 }
 
@@ -407,6 +411,7 @@ bool warp_to_limit(Ctx& c, DMesh& m, AdaptOptions const& o, bool exit_on_stall, 
     set_coords(m, moved);
   }
   if (acceptable(c, m, o)) {
+    if (o.verbosity >= 2) std::printf("warp_to_limit completed in one step\n");
     m.vtags.erase(it);
     return true;
   }
@@ -419,6 +424,7 @@ bool warp_to_limit(Ctx& c, DMesh& m, AdaptOptions const& o, bool exit_on_stall, 
     if (i > max_niters) {
       OSH_CHECK(exit_on_stall, "warp step " + std::to_string(i) +
                                    ": unable to satisfy this warp under this size field");
+      std::printf("warp_to_limit stalled, dropping warp field and continuing anyway\n");
       for (it = m.vtags.begin(); it != m.vtags.end(); ++it)
         if (it->name == "warp") break;
       m.vtags.erase(it);
@@ -429,6 +435,7 @@ bool warp_to_limit(Ctx& c, DMesh& m, AdaptOptions const& o, bool exit_on_stall, 
         half.data(), rem.data(), i == 1, moved.data());
     set_coords(m, moved);
   } while (!acceptable(c, m, o));
+  if (o.verbosity >= 2) std::printf("warp_to_limit moved by factor %g\n", std::ldexp(1.0, -i));
   for (auto& t : m.vtags)
     if (t.name == "warp") t.data = rem;
   return true;
