@@ -511,8 +511,14 @@ int grade_fix_adapt(Ctx& c, DMesh& m, AdaptOptions const& o, int ncomps, Reals c
   Reals had = m.metric;
   int had_ncomps = m.metric_ncomps;
   set_mesh_metric(m, target, ncomps);
-  limit_metric_gradation(c, m, 0.5, 1e-2);
-  Reals graded = m.metric;
+  Reals graded;
+  try {
+    limit_metric_gradation(c, m, 0.5, 1e-2);
+    graded = m.metric;
+  } catch (...) {
+    set_mesh_metric(m, had, had_ncomps);  // the caller's mesh as it was
+    throw;
+  }
   bool found = false;
   for (auto& t : m.vtags)
     if (t.name == "target_metric") {
