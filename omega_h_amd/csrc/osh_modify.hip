@@ -677,7 +677,8 @@ void modify_scan(Ctx& c, ModPlan* plans, int nplans) {
 
 void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const& keys2kds,
     LOs const& keys2prods, LO nprods, LOs const& prod_verts, LOs const& old_lows2new_lows,
-    ModOut& out, LOs const* verts_old2new, LOs* prod_src) {
+    ModOut& out, LOs const* verts_old2new, LOs* prod_src, LOs const* given_down,
+    Bytes const* given_codes) {
   int ent_dim = plan.ent_dim, key_dim = plan.key_dim;
   LO nkeys = keys2kds.size();
   LO nold = plan.nold;
@@ -714,7 +715,13 @@ void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const&
     down = LOs(c, nnew * deg);
     if (with_codes) {
       codes = Bytes(c, nnew * deg);
-      reflect_down(c, nw, ent_dim, nprods, prod_verts, pdown, pcodes, &found);
+      // a caller that knows where the products' sides are hands them over
+      if (given_down) {
+        pdown = *given_down;
+        pcodes = *given_codes;
+      } else {
+        reflect_down(c, nw, ent_dim, nprods, prod_verts, pdown, pcodes, &found);
+      }
       verts = LOs(c, nnew * deg);
       old_verts = verts_of(c, old, ent_dim).data();
     } else {

@@ -63,7 +63,8 @@ void modify_scan_totals(Ctx& c, ModPlan* plans, int nplans, int first_slot);
 // numbers); prod_src, when given, receives each product's key entity
 void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const& keys2kds,
     LOs const& keys2prods, LO nprods, LOs const& prod_verts, LOs const& old_lows2new_lows,
-    ModOut& out, LOs const* verts_old2new, LOs* prod_src = nullptr);
+    ModOut& out, LOs const* verts_old2new, LOs* prod_src = nullptr,
+    LOs const* given_down = nullptr, Bytes const* given_codes = nullptr);
 
 // new tag data: same entities copy old values, products take prod values
 template <class T>

@@ -241,6 +241,72 @@ __global__ void swap3d_choose(LO ncands, LO const* cands, LO const* lsize, Real 
   }
 }
 
+// reflect_down for the products of one swap key, searched locally: a side
+// of a product (a face of a new tet, an edge of a new face) is a product of
+// the same key one dimension down or a surviving side of one of the old tets
+// around the key edge, so those are the only candidates (vertices keep their
+// numbers in a swap).  The code is the one match_uses computes.
+template <int HIGH>
+__global__ void swap3d_local_down(LO nuses, LO nkeys, LO const* keys2edges,
+    LO const* k2p_high, LO const* k2p_low, LO const* pv_high, LO const* pv_low,
+    LO const* low_prods2new, LO const* e2t_a2ab, LO const* e2t_ab2b, LO const* t2low,
+    LO const* old_low_verts, LO const* low_old2new, LO* out, I8* out_codes, int* error) {
+  LO u = thread_index();
+  if (u >= nuses) return;
+  constexpr int LOW = HIGH - 1;
+  constexpr int nvh = HIGH + 1;
+  constexpr int nvl = LOW + 1;
+  constexpr int nlpt = (HIGH == 3) ? 4 : 6;  // faces or edges of an old tet
+  LO p = u / nvh;
+  int which = u % nvh;
+  LO lo = 0, hi = nkeys;  // the key whose products hold p
+  while (hi - lo > 1) {
+    LO mid = lo + (hi - lo) / 2;
+    if (k2p_high[mid] <= p)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  LO key = lo;
+  LO av[3];
+#pragma unroll
+  for (int k = 0; k < nvl; ++k) av[k] = pv_high[p * nvh + down_template(HIGH, LOW, which, k)];
+  LO match = -1;
+  I8 code = 0;
+  auto try_side = [&](LO id, LO const* bv) {
+    int wd = (bv[0] == av[0]) ? 0 : (bv[1] == av[0]) ? 1 : (nvl == 3 && bv[2] == av[0]) ? 2 : -1;
+    if (wd < 0) return;
+    if (nvl == 2) {
+      if (av[1] != bv[1 - wd]) return;
+      code = make_code(0, wd, 0);
+    } else if (av[1] == bv[(wd + 1) % 3] && av[2] == bv[(wd + 2) % 3]) {
+      code = make_code(0, (3 - wd) % 3, 0);
+    } else if (av[1] == bv[(wd + 2) % 3] && av[2] == bv[(wd + 1) % 3]) {
+      code = make_code(1, (3 - wd) % 3, 0);
+    } else {
+      return;
+    }
+    match = id;
+  };
+  for (LO q = k2p_low[key]; q < k2p_low[key + 1] && match < 0; ++q)
+    try_side(low_prods2new[q], pv_low + q * nvl);
+  LO e = keys2edges[key];
+  for (LO j = e2t_a2ab[e]; j < e2t_a2ab[e + 1] && match < 0; ++j) {
+    LO t = e2t_ab2b[j];
+    for (int i = 0; i < nlpt && match < 0; ++i) {
+      LO o = t2low[t * nlpt + i];
+      LO w = low_old2new[o];
+      if (w >= 0) try_side(w, old_low_verts + o * nvl);
+    }
+  }
+  if (match < 0) {
+    *error = kReflectDownNoMatch;
+    match = 0;
+  }
+  out[u] = match;
+  out_codes[u] = code;
+}
+
 // the edges of the tets the pass replaces: their tet rings change
 __global__ void swap3d_mark_dirty(LO ntets, I8 const* tet_same, LO const* t2e, I8* dirty) {
   LO t = thread_index();
@@ -467,7 +533,7 @@ Tables const* device_tables(Ctx& c) {
 // `plans` (when given) are the rebuild plans of dimensions 1..dim, scanned
 void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO const* nprods,
     LOs const* pv, ModPlan* plans = nullptr, SwapCache* cache = nullptr,
-    Bytes const* dirty = nullptr) {
+    Bytes const* dirty = nullptr, Adj const* key_tets = nullptr) {
   int dim = m.dim;
   LO nv = m.nents[0];
   DMesh nw;
@@ -487,11 +553,32 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
     modify_plans(c, m, 1, 1, dim, keys2edges, k2p, plans);
     modify_scan(c, plans, dim);
   }
+  ModOut low_mo;  // the previous dimension's numbering
   for (int d = 1; d <= dim; ++d) {
     ModOut mo;
     LOs src;  // each product's key edge
+    // in 3D the products' sides come from a search around each key
+    LOs gdown;
+    Bytes gcodes;
+    if (key_tets && d >= 2) {
+      LO nuses = nprods[d] * (d + 1);
+      gdown = LOs(c, nuses);
+      gcodes = Bytes(c, nuses);
+      LO const* t2low = (d == 3) ? m.down[3].data() : ask_down(c, m, 3, 1, nullptr).data();
+      if (d == 3)
+        OSH_LAUNCH(swap3d_local_down<3>, nuses, c, keys2edges.size(), keys2edges.data(),
+            k2p[3].data(), k2p[2].data(), pv[3].data(), pv[2].data(), low_mo.prods2new.data(),
+            key_tets->a2ab.data(), key_tets->ab2b.data(), t2low, verts_of(c, m, 2).data(),
+            low_mo.old2new.data(), gdown.data(), gcodes.data(), c.error_word());
+      else
+        OSH_LAUNCH(swap3d_local_down<2>, nuses, c, keys2edges.size(), keys2edges.data(),
+            k2p[2].data(), k2p[1].data(), pv[2].data(), pv[1].data(), low_mo.prods2new.data(),
+            key_tets->a2ab.data(), key_tets->ab2b.data(), t2low, verts_of(c, m, 1).data(),
+            low_mo.old2new.data(), gdown.data(), gcodes.data(), c.error_word());
+    }
     modify_build(c, m, nw, plans[d - 1], keys2edges, k2p[d], nprods[d], pv[d],
-        old_lows2new_lows, mo, nullptr, &src);
+        old_lows2new_lows, mo, nullptr, &src, gdown.exists() ? &gdown : nullptr,
+        gcodes.exists() ? &gcodes : nullptr);
     transfer_class(c, m, nw, d, mo, src, 1, nullptr);
     if (d == 1 && cache) {
       SwapCache next;
@@ -508,6 +595,7 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
     if (d == dim) transfer_pointwise(c, m, nw, mo, 1, keys2edges, k2p[d], pv[d]);
     transfer_measures(c, m, nw, d, mo);
     old_lows2new_lows = mo.old2new;
+    low_mo = mo;
   }
   m = nw;
 }
@@ -625,7 +713,7 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
     OSH_LAUNCH(swap3d_mark_dirty, m.nents[3], c, plans[2].same.data(),
         ask_down(c, m, 3, 1, nullptr).data(), dirty.data());
   }
-  swap_rebuild(c, m, keys2edges, k2p, nprods, pv, plans, cache, cache ? &dirty : nullptr);
+  swap_rebuild(c, m, keys2edges, k2p, nprods, pv, plans, cache, cache ? &dirty : nullptr, &e2t);
   return true;
 }
 
