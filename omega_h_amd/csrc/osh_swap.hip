@@ -245,66 +245,75 @@ __global__ void swap3d_choose(LO ncands, LO const* cands, LO const* lsize, Real 
 // of a product (a face of a new tet, an edge of a new face) is a product of
 // the same key one dimension down or a surviving side of one of the old tets
 // around the key edge, so those are the only candidates (vertices keep their
-// numbers in a swap).  The code is the one match_uses computes.
+// numbers in a swap).  kSideLanes threads share one side's candidates; the
+// one that matches writes the side and the code match_uses would compute.
+constexpr int kSideLanes = 16;
+
 template <int HIGH>
-__global__ void swap3d_local_down(LO nuses, LO nkeys, LO const* keys2edges,
-    LO const* k2p_high, LO const* k2p_low, LO const* pv_high, LO const* pv_low,
-    LO const* low_prods2new, LO const* e2t_a2ab, LO const* e2t_ab2b, LO const* t2low,
-    LO const* old_low_verts, LO const* low_old2new, LO* out, I8* out_codes, int* error) {
-  LO u = thread_index();
-  if (u >= nuses) return;
+__global__ void swap3d_local_down(LO nt, LO const* prod_key, LO const* keys2edges,
+    LO const* k2p_low, LO const* pv_high, LO const* pv_low, LO const* low_prods2new,
+    LO const* e2t_a2ab, LO const* e2t_ab2b, LO const* t2low, LO const* old_low_verts,
+    LO const* low_old2new, LO* out, I8* out_codes, int* error) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO u = t / kSideLanes;
+  int lane = t % kSideLanes;
   constexpr int LOW = HIGH - 1;
   constexpr int nvh = HIGH + 1;
   constexpr int nvl = LOW + 1;
   constexpr int nlpt = (HIGH == 3) ? 4 : 6;  // faces or edges of an old tet
   LO p = u / nvh;
   int which = u % nvh;
-  LO lo = 0, hi = nkeys;  // the key whose products hold p
-  while (hi - lo > 1) {
-    LO mid = lo + (hi - lo) / 2;
-    if (k2p_high[mid] <= p)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  LO key = lo;
+  LO key = prod_key[p];
   LO av[3];
 #pragma unroll
   for (int k = 0; k < nvl; ++k) av[k] = pv_high[p * nvh + down_template(HIGH, LOW, which, k)];
-  LO match = -1;
-  I8 code = 0;
-  auto try_side = [&](LO id, LO const* bv) {
+  LO q0 = k2p_low[key];
+  LO nq = k2p_low[key + 1] - q0;
+  LO e = keys2edges[key];
+  LO r0 = e2t_a2ab[e];
+  LO ncand = nq + (e2t_a2ab[e + 1] - r0) * nlpt;
+  bool found = false;
+  for (LO cnd = lane; cnd < ncand; cnd += kSideLanes) {
+    LO id;
+    LO bv[3];
+    if (cnd < nq) {
+      id = low_prods2new[q0 + cnd];
+#pragma unroll
+      for (int k = 0; k < nvl; ++k) bv[k] = pv_low[(q0 + cnd) * nvl + k];
+    } else {
+      LO r = cnd - nq;
+      LO o = t2low[e2t_ab2b[r0 + r / nlpt] * nlpt + r % nlpt];
+      id = low_old2new[o];
+      if (id < 0) continue;
+#pragma unroll
+      for (int k = 0; k < nvl; ++k) bv[k] = old_low_verts[o * nvl + k];
+    }
     int wd = (bv[0] == av[0]) ? 0 : (bv[1] == av[0]) ? 1 : (nvl == 3 && bv[2] == av[0]) ? 2 : -1;
-    if (wd < 0) return;
+    if (wd < 0) continue;
+    I8 code;
     if (nvl == 2) {
-      if (av[1] != bv[1 - wd]) return;
+      if (av[1] != bv[1 - wd]) continue;
       code = make_code(0, wd, 0);
     } else if (av[1] == bv[(wd + 1) % 3] && av[2] == bv[(wd + 2) % 3]) {
       code = make_code(0, (3 - wd) % 3, 0);
     } else if (av[1] == bv[(wd + 2) % 3] && av[2] == bv[(wd + 1) % 3]) {
       code = make_code(1, (3 - wd) % 3, 0);
     } else {
-      return;
+      continue;
     }
-    match = id;
-  };
-  for (LO q = k2p_low[key]; q < k2p_low[key + 1] && match < 0; ++q)
-    try_side(low_prods2new[q], pv_low + q * nvl);
-  LO e = keys2edges[key];
-  for (LO j = e2t_a2ab[e]; j < e2t_a2ab[e + 1] && match < 0; ++j) {
-    LO t = e2t_ab2b[j];
-    for (int i = 0; i < nlpt && match < 0; ++i) {
-      LO o = t2low[t * nlpt + i];
-      LO w = low_old2new[o];
-      if (w >= 0) try_side(w, old_low_verts + o * nvl);
-    }
+    out[u] = id;
+    out_codes[u] = code;
+    found = true;
   }
-  if (match < 0) {
+  // the side's lanes are 16 consecutive lanes of one wave
+  unsigned long long any = __ballot(found);
+  int base = int(threadIdx.x % 64) & ~(kSideLanes - 1);
+  if (lane == 0 && ((any >> base) & 0xffffull) == 0ull) {
     *error = kReflectDownNoMatch;
-    match = 0;
+    out[u] = 0;
+    out_codes[u] = 0;
   }
-  out[u] = match;
-  out_codes[u] = code;
 }
 
 // the edges of the tets the pass replaces: their tet rings change
@@ -390,7 +399,7 @@ constexpr int kLanes = 8;
 
 __global__ void swap3d_products(LO nt, LO const* keys2edges, I8 const* configs,
     LO const* cand_of_edge, LO const* lsize, LO const* leev, LO const* lverts, Tables const* tb,
-    LO const* k2e, LO const* k2f, LO const* k2t, LO* pe, LO* pf, LO* pt) {
+    LO const* k2e, LO const* k2f, LO const* k2t, LO* pe, LO* pf, LO* pt, LO* fkey, LO* tkey) {
   LO t = thread_index();
   if (t >= nt) return;
   LO k = t / kLanes;
@@ -414,6 +423,7 @@ __global__ void swap3d_products(LO nt, LO const* keys2edges, I8 const* configs,
       pf[ptri * 3 + 0] = a;
       pf[ptri * 3 + 1] = b;
       pf[ptri * 3 + 2] = eev2v[eev];
+      fkey[ptri] = k;
     }
   }
   for (int pt_i = lane; pt_i < npt; pt_i += kLanes) {
@@ -422,8 +432,10 @@ __global__ void swap3d_products(LO nt, LO const* keys2edges, I8 const* configs,
     for (int j = 0; j < 3; ++j) tv[j] = lv[tb->tris[n][ut][j]];
     LO ptri = k2f[k] + 2 * npe + pt_i;
     for (int j = 0; j < 3; ++j) pf[ptri * 3 + j] = tv[j];
+    fkey[ptri] = k;
     for (int eev = 0; eev < 2; ++eev) {
       LO ptet = k2t[k] + 2 * pt_i + eev;
+      tkey[ptet] = k;
       LO nt4[4] = {tv[0], tv[1], tv[2], eev2v[eev]};
       if (eev == 0) {
         LO tmp = nt4[1];
@@ -533,7 +545,8 @@ Tables const* device_tables(Ctx& c) {
 // `plans` (when given) are the rebuild plans of dimensions 1..dim, scanned
 void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO const* nprods,
     LOs const* pv, ModPlan* plans = nullptr, SwapCache* cache = nullptr,
-    Bytes const* dirty = nullptr, Adj const* key_tets = nullptr) {
+    Bytes const* dirty = nullptr, Adj const* key_tets = nullptr,
+    LOs const* prod_keys = nullptr) {
   int dim = m.dim;
   LO nv = m.nents[0];
   DMesh nw;
@@ -560,21 +573,23 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
     // in 3D the products' sides come from a search around each key
     LOs gdown;
     Bytes gcodes;
-    if (key_tets && d >= 2) {
+    if (key_tets && prod_keys && d >= 2) {
       LO nuses = nprods[d] * (d + 1);
       gdown = LOs(c, nuses);
       gcodes = Bytes(c, nuses);
       LO const* t2low = (d == 3) ? m.down[3].data() : ask_down(c, m, 3, 1, nullptr).data();
       if (d == 3)
-        OSH_LAUNCH(swap3d_local_down<3>, nuses, c, keys2edges.size(), keys2edges.data(),
-            k2p[3].data(), k2p[2].data(), pv[3].data(), pv[2].data(), low_mo.prods2new.data(),
-            key_tets->a2ab.data(), key_tets->ab2b.data(), t2low, verts_of(c, m, 2).data(),
-            low_mo.old2new.data(), gdown.data(), gcodes.data(), c.error_word());
+        OSH_LAUNCH(swap3d_local_down<3>, nuses * kSideLanes, c, prod_keys[3].data(),
+            keys2edges.data(), k2p[2].data(), pv[3].data(), pv[2].data(),
+            low_mo.prods2new.data(), key_tets->a2ab.data(), key_tets->ab2b.data(), t2low,
+            verts_of(c, m, 2).data(), low_mo.old2new.data(), gdown.data(), gcodes.data(),
+            c.error_word());
       else
-        OSH_LAUNCH(swap3d_local_down<2>, nuses, c, keys2edges.size(), keys2edges.data(),
-            k2p[2].data(), k2p[1].data(), pv[2].data(), pv[1].data(), low_mo.prods2new.data(),
-            key_tets->a2ab.data(), key_tets->ab2b.data(), t2low, verts_of(c, m, 1).data(),
-            low_mo.old2new.data(), gdown.data(), gcodes.data(), c.error_word());
+        OSH_LAUNCH(swap3d_local_down<2>, nuses * kSideLanes, c, prod_keys[2].data(),
+            keys2edges.data(), k2p[1].data(), pv[2].data(), pv[1].data(),
+            low_mo.prods2new.data(), key_tets->a2ab.data(), key_tets->ab2b.data(), t2low,
+            verts_of(c, m, 1).data(), low_mo.old2new.data(), gdown.data(), gcodes.data(),
+            c.error_word());
     }
     modify_build(c, m, nw, plans[d - 1], keys2edges, k2p[d], nprods[d], pv[d],
         old_lows2new_lows, mo, nullptr, &src, gdown.exists() ? &gdown : nullptr,
@@ -704,16 +719,21 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
       offs[2].data(), offs[3].data(), rep_of[2].data(), rep_of[3].data(), keys2edges.data(),
       k2p[1].data(), k2p[2].data(), k2p[3].data(), plans[0].reps.data(), plans[1].reps.data(),
       plans[2].reps.data());
+  LOs prod_keys[4];  // the key of each product face and tet
+  prod_keys[2] = LOs(c, nprods[2]);
+  prod_keys[3] = LOs(c, nprods[3]);
   OSH_LAUNCH(swap3d_products, nkeys * kLanes, c, keys2edges.data(), configs.data(),
       cand_of_edge.data(), lsize.data(), leev.data(), lverts.data(), tb, k2p[1].data(),
-      k2p[2].data(), k2p[3].data(), pv[1].data(), pv[2].data(), pv[3].data());
+      k2p[2].data(), k2p[3].data(), pv[1].data(), pv[2].data(), pv[3].data(),
+      prod_keys[2].data(), prod_keys[3].data());
   Bytes dirty;
   if (cache) {
     dirty = fill_i8(c, ne, 0);
     OSH_LAUNCH(swap3d_mark_dirty, m.nents[3], c, plans[2].same.data(),
         ask_down(c, m, 3, 1, nullptr).data(), dirty.data());
   }
-  swap_rebuild(c, m, keys2edges, k2p, nprods, pv, plans, cache, cache ? &dirty : nullptr, &e2t);
+  swap_rebuild(c, m, keys2edges, k2p, nprods, pv, plans, cache, cache ? &dirty : nullptr, &e2t,
+      prod_keys);
   return true;
 }
 
