@@ -405,6 +405,45 @@ __global__ void indset_iteration(LO n, LO const* a2ab, LO const* ab2b, Real cons
   if (!undecided) new_state[v] = 1;
 }
 
+// the first round for edges with kFirstLanes threads per edge: each takes
+// every kFirstLanes-th row position, and the group ORs its findings, so the
+// row's gathers run side by side instead of one after another
+constexpr int kFirstLanes = 8;
+
+template <int NEPC>
+__global__ void indset_first_edges(LO nt, LO const* e2c_a2ab, LO const* e2c_ab2b,
+    LO const* c2e, Real const* q, I8 const* cands, I8* new_state, unsigned long long* dom) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO v = t / kFirstLanes;
+  int lane = t % kFirstLanes;
+  I8 s = cands[v] ? I8(2) : I8(0);
+  unsigned long long mask = 0;
+  int undecided = 0;
+  if (s == 2) {
+    LO b = e2c_a2ab[v], e = e2c_a2ab[v + 1];
+    Real qv = q[v];
+    LO np = (e - b) * NEPC;
+    for (LO p = lane; p < np; p += kFirstLanes) {
+      LO u = c2e[e2c_ab2b[b + p / NEPC] * NEPC + p % NEPC];
+      if (u == v || !cands[u]) continue;
+      Real qu = q[u];
+      bool beaten = (qu != qv) ? (qu < qv) : (u < v);
+      if (beaten) continue;
+      undecided = 1;
+      if (p < kDomBits) mask |= 1ull << p;
+    }
+  }
+#pragma unroll
+  for (int d = kFirstLanes / 2; d > 0; d >>= 1) {
+    mask |= __shfl_xor(mask, d, 64);
+    undecided |= __shfl_xor(undecided, d, 64);
+  }
+  if (lane != 0) return;
+  new_state[v] = (s == 2 && !undecided) ? I8(1) : s;
+  if (s == 2) dom[v] = mask;
+}
+
 // the same round for edges, whose star (edges_across_tris/tets,
 // src/Omega_h_adj.cpp:532-589) is the set of other edges of the elements
 // around the edge: read through the edge -> element adjacency instead of a
@@ -887,7 +926,13 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
       OSH_LAUNCH((indset_iteration_edges<3, F>), n, c, e2c->a2ab.data(), e2c->ab2b.data(),  \
           c2e, quality.data(), in, b.data(), dom.data());                                 \
   } while (0)
-      if (first)
+      if (first && !g && m.dim == 3)
+        OSH_LAUNCH((indset_first_edges<6>), n * kFirstLanes, c, e2c->a2ab.data(),
+            e2c->ab2b.data(), c2e, quality.data(), cands.data(), b.data(), dom.data());
+      else if (first && !g)
+        OSH_LAUNCH((indset_first_edges<3>), n * kFirstLanes, c, e2c->a2ab.data(),
+            e2c->ab2b.data(), c2e, quality.data(), cands.data(), b.data(), dom.data());
+      else if (first)
         OSH_IS(true);
       else
         OSH_IS(false);

@@ -171,6 +171,7 @@ __global__ void swap3d_edge_lens(LO nt, LO const* nedges, LO const* lsize, LO co
 // the largest of these values, when that is positive; a minimum that went
 // negative (where choose() stops early) is never kept either way.
 constexpr int kMaxMeshes = 42;
+constexpr int kChooseLanes = 8;
 
 __global__ void swap3d_mesh_quals(LO nt, LO const* lsize, LO const* ntris, Real const* tq,
     Real const* el, Real max_length, Tables const* tb, Real* mq) {
@@ -193,28 +194,37 @@ __global__ void swap3d_mesh_quals(LO nt, LO const* lsize, LO const* ntris, Real 
   mq[t] = q;
 }
 
-__global__ void swap3d_choose(LO ncands, LO const* cands, LO const* lsize, Real const* mq,
+__global__ void swap3d_choose(LO nt, LO const* cands, LO const* lsize, Real const* mq,
     LO const* e2t_a2ab, LO const* e2t_ab2b, Real const* elem_quals, Tables const* tb,
     I8* configs, Real* quals, I8* keep, I8* c_valid, I8* c_cfg, I8* c_keep, Real* c_q) {
-  LO cand = thread_index();
-  if (cand >= ncands) return;
+  // kChooseLanes threads per candidate share its triangulations and tets
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO cand = t / kChooseLanes;
+  int lane = t % kChooseLanes;
   LO edge = cands[cand];
   if (c_valid && c_valid[edge]) {
-    configs[edge] = c_cfg[edge];
-    if (c_keep[edge]) {
-      keep[edge] = 1;
-      quals[edge] = c_q[edge];
+    if (lane == 0) {
+      configs[edge] = c_cfg[edge];
+      if (c_keep[edge]) {
+        keep[edge] = 1;
+        quals[edge] = c_q[edge];
+      }
     }
     return;
   }
   int n = lsize[cand];
+  // each lane keeps the first of its triangulations with the largest value
+  // above 0; the group then takes the largest value, the lowest index on
+  // ties: the first triangulation attaining the maximum, as the sequential
+  // choice finds it
   int choice_mesh = -1;
   double choice_quality = -1.0;
   if (n <= swap::kMaxLoop) {
     choice_quality = 0.0;
     int nmeshes = tb->mesh_counts[n];
     Real const* cmq = mq + LO(cand) * kMaxMeshes;
-    for (int mesh = 0; mesh < nmeshes; ++mesh) {
+    for (int mesh = lane; mesh < nmeshes; mesh += kChooseLanes) {
       double q = cmq[mesh];
       if (q > choice_quality) {
         choice_mesh = mesh;
@@ -222,10 +232,22 @@ __global__ void swap3d_choose(LO ncands, LO const* cands, LO const* lsize, Real 
       }
     }
   }
-  configs[edge] = I8(choice_mesh);
   double old = 1.79769313486231570815e+308;
-  for (LO j = e2t_a2ab[edge]; j < e2t_a2ab[edge + 1]; ++j)
+  for (LO j = e2t_a2ab[edge] + lane; j < e2t_a2ab[edge + 1]; j += kChooseLanes)
     old = math::min2(old, elem_quals[e2t_ab2b[j]]);
+#pragma unroll
+  for (int d = kChooseLanes / 2; d > 0; d >>= 1) {
+    double oq = __shfl_xor(choice_quality, d, 64);
+    int om = __shfl_xor(choice_mesh, d, 64);
+    if (oq > choice_quality || (oq == choice_quality && om >= 0 &&
+                                   (choice_mesh < 0 || om < choice_mesh))) {
+      choice_quality = oq;
+      choice_mesh = om;
+    }
+    old = math::min2(old, __shfl_xor(old, d, 64));
+  }
+  if (lane != 0) return;
+  configs[edge] = I8(choice_mesh);
   // a key needs a triangulation; with inverted elements the reference can
   // keep an edge without one and then index its tables out of bounds
   bool kept = choice_mesh >= 0 && choice_quality > old;
@@ -676,7 +698,8 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
   Reals mq(c, ncands * kMaxMeshes);
   OSH_LAUNCH(swap3d_mesh_quals, ncands * kMaxMeshes, c, lsize.data(), ntris.data(),
       tq.data(), el.data(), o.max_length_allowed, tb, mq.data());
-  OSH_LAUNCH(swap3d_choose, ncands, c, cands.data(), lsize.data(), mq.data(), e2t.a2ab.data(),
+  OSH_LAUNCH(swap3d_choose, ncands * kChooseLanes, c, cands.data(), lsize.data(), mq.data(),
+      e2t.a2ab.data(),
       e2t.ab2b.data(), eq.data(), tb, configs.data(), quals.data(), keep.data(),
       cache ? cache->valid.data() : (I8*)nullptr, cache ? cache->cfg.data() : (I8*)nullptr,
       cache ? cache->keep.data() : (I8*)nullptr, cache ? cache->q.data() : (Real*)nullptr);
