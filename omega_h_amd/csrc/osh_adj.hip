@@ -72,16 +72,20 @@ __device__ __forceinline__ void up_template0(int elem_dim, int bdry_dim, int whi
   }
 }
 
+// one thread per (high entity, local low entity), so that the gathers of a
+// high entity's lows run side by side
 template <int HIGH, int LOW>
-__global__ void transit_kernel(LO nhigh, LO const* hm2m, I8 const* m2hm_codes,
+__global__ void transit_kernel(LO nuses, LO const* hm2m, I8 const* m2hm_codes,
     LO const* ml2l, I8 const* ml_codes, LO* hl2l, I8* codes) {
-  LO h = thread_index();
-  if (h >= nhigh) return;
+  LO t = thread_index();
+  if (t >= nuses) return;
   constexpr int MID = LOW + 1;
   constexpr int nmph = (HIGH == 3) ? (MID == 2 ? 4 : 6) : 3;
   constexpr int nlpm = MID + 1;
   constexpr int nlph = (HIGH == 3) ? (LOW == 0 ? 4 : 6) : 3;
-  for (int hl = 0; hl < nlph; ++hl) {
+  LO h = t / nlph;
+  int hl = t % nlph;
+  {
     int hm, hml, canon_flipped;
     up_template0(HIGH, LOW, hl, hm, hml, canon_flipped);
     LO m = hm2m[h * nmph + hm];
@@ -392,12 +396,12 @@ LOs const& verts_of(Ctx& c, DMesh& m, int d) {
     LO n = m.nents[d];
     LOs out(c, n * (d + 1));
     if (d == 2) {
-      OSH_LAUNCH((transit_kernel<2, 0>), n, c, m.down[2].data(), m.down_codes[2].data(),
+      OSH_LAUNCH((transit_kernel<2, 0>), n * 3, c, m.down[2].data(), m.down_codes[2].data(),
           m.down[1].data(), (I8 const*)nullptr, out.data(), (I8*)nullptr);
     } else {
       Bytes const* codes;
       LOs const& t2e = ask_down(c, m, 3, 1, &codes);
-      OSH_LAUNCH((transit_kernel<3, 0>), n, c, t2e.data(), codes->data(), m.down[1].data(),
+      OSH_LAUNCH((transit_kernel<3, 0>), n * 4, c, t2e.data(), codes->data(), m.down[1].data(),
           (I8 const*)nullptr, out.data(), (I8*)nullptr);
     }
     m.verts[d] = out;
@@ -418,7 +422,7 @@ LOs const& ask_down(Ctx& c, DMesh& m, int high, int low, Bytes const** codes) {
     LO n = m.nents[3];
     m.t2e = LOs(c, n * 6);
     m.t2e_codes = Bytes(c, n * 6);
-    OSH_LAUNCH((transit_kernel<3, 1>), n, c, m.down[3].data(), m.down_codes[3].data(),
+    OSH_LAUNCH((transit_kernel<3, 1>), n * 6, c, m.down[3].data(), m.down_codes[3].data(),
         m.down[2].data(), m.down_codes[2].data(), m.t2e.data(), m.t2e_codes.data());
   }
   if (codes) *codes = &m.t2e_codes;
