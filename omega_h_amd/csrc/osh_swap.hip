@@ -22,6 +22,8 @@ namespace {
 
 using swap::Tables;
 
+constexpr int kLoopLanes = 8;  // at least swap::kMaxLoop
+
 struct Loop {
   int size;
   LO eev2v[2];
@@ -102,27 +104,60 @@ __device__ double length_between(LO a, LO b, Real const* coords, Real const* met
 //                     reference's order
 // Every value is the one the reference computes (each unique triangle and
 // edge evaluated once), so the choice is the same.
-__global__ void swap3d_loops(LO ncands, LO const* cands, LO const* e2t_a2ab, LO const* e2t_ab2b,
+__global__ void swap3d_loops(LO nt, LO const* cands, LO const* e2t_a2ab, LO const* e2t_ab2b,
     I8 const* e2t_codes, LO const* ev2v, LO const* tv2v, Tables const* tb, I8 const* cached,
     LO* lsize, LO* leev, LO* lverts, LO* ntris, LO* nedges) {
-  LO cand = thread_index();
-  if (cand >= ncands) return;
-  Loop loop;
-  find_loop(cands[cand], e2t_a2ab, e2t_ab2b, e2t_codes, ev2v, tv2v, loop);
-  lsize[cand] = loop.size;
-  if (loop.size > swap::kMaxLoop) {
+  // kLoopLanes threads per candidate: lane le reads the le-th tet around
+  // the edge; the group then orders the loop as find_loop does
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO cand = t / kLoopLanes;
+  int lane = t % kLoopLanes;
+  LO edge = cands[cand];
+  LO begin = e2t_a2ab[edge];
+  int size = e2t_a2ab[edge + 1] - begin;
+  LO x0 = -1, x1 = -1;
+  if (size <= swap::kMaxLoop && lane < size) {
+    LO tet = e2t_ab2b[begin + lane];
+    I8 code = e2t_codes[begin + lane];
+    int rre_opp = opposite_template(3, 1, code_which_down(code));
+    int rot = code_rotation(code);
+    x0 = tv2v[tet * 4 + down_template(3, 1, rre_opp, rot)];
+    x1 = tv2v[tet * 4 + down_template(3, 1, rre_opp, rot ^ 1)];
+  }
+  int base = int(threadIdx.x % 64) & ~(kLoopLanes - 1);
+  LO tmp[swap::kMaxLoop][2];
+#pragma unroll
+  for (int k = 0; k < swap::kMaxLoop; ++k) {
+    tmp[k][0] = __shfl(x0, base + k, 64);
+    tmp[k][1] = __shfl(x1, base + k, 64);
+  }
+  if (lane != 0) return;
+  lsize[cand] = size;
+  if (size > swap::kMaxLoop) {
     ntris[cand] = 0;
     nedges[cand] = 0;
     return;
   }
-  leev[cand * 2 + 0] = loop.eev2v[0];
-  leev[cand * 2 + 1] = loop.eev2v[1];
+  for (int i = 0; i < size - 1; ++i) {
+    int j;
+    for (j = i + 1; j < size; ++j)
+      if (tmp[j][0] == tmp[i][1]) break;
+    if (j >= size) j = i + 1;  // topological error: the reference aborts here
+    LO t0 = tmp[i + 1][0], t1 = tmp[i + 1][1];
+    tmp[i + 1][0] = tmp[j][0];
+    tmp[i + 1][1] = tmp[j][1];
+    tmp[j][0] = t0;
+    tmp[j][1] = t1;
+  }
+  leev[cand * 2 + 0] = ev2v[edge * 2];
+  leev[cand * 2 + 1] = ev2v[edge * 2 + 1];
   for (int k = 0; k < swap::kMaxLoop; ++k)
-    lverts[cand * swap::kMaxLoop + k] = (k < loop.size) ? loop.verts[k] : -1;
+    lverts[cand * swap::kMaxLoop + k] = (k < size) ? tmp[k][0] : -1;
   // a carried-over choice needs the loop (for products) but no qualities
-  bool carried = cached && cached[cands[cand]];
-  ntris[cand] = carried ? 0 : tb->ntris[loop.size];
-  nedges[cand] = carried ? 0 : tb->nuedges[loop.size];
+  bool carried = cached && cached[edge];
+  ntris[cand] = carried ? 0 : tb->ntris[size];
+  nedges[cand] = carried ? 0 : tb->nuedges[size];
 }
 
 template <int M>
@@ -676,7 +711,7 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
   }
   LOs lsize(c, ncands), leev(c, ncands * 2), lverts(c, ncands * swap::kMaxLoop);
   LOs ntris(c, ncands), nedges(c, ncands);
-  OSH_LAUNCH(swap3d_loops, ncands, c, cands.data(), e2t.a2ab.data(), e2t.ab2b.data(),
+  OSH_LAUNCH(swap3d_loops, ncands * kLoopLanes, c, cands.data(), e2t.a2ab.data(), e2t.ab2b.data(),
       e2t.codes.data(), m.down[1].data(), tv2v.data(), tb,
       cache ? cache->valid.data() : (I8 const*)nullptr, lsize.data(), leev.data(),
       lverts.data(), ntris.data(), nedges.data());
