@@ -368,7 +368,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"build_box unit cube {args.n}^3 x 6 tets ({nelems0} tets), "
                                "level-set anisotropic metric, full adapt() FP64, default "
-                               "AdaptOpts",
+                               "AdaptOpts with verbosity SILENT",
                    "passes_per_step": len(runs[-1]), "final_tets": final},
         "roofline": {"bound": "hbm", "achieved": roof["achieved"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": roof["achieved"] / HBM_PEAK_GBS,
