@@ -161,7 +161,7 @@ __global__ void apply_vetoes(LO ncands, LO const* cands, I8 const* veto, I8* cod
 // collapsing vertex's cells (those that survive the collapse, with v_col
 // moved onto v_onto) into a partial minimum
 template <int D, int M>
-__global__ void __launch_bounds__(kBlock) coarsen_quality_lanes(LO nt, LO const* cands,
+__global__ void __launch_bounds__(kBlock, 5) coarsen_quality_lanes(LO nt, LO const* cands,
     LO const* ev2v, LO const* cv2v, LO const* v2c_a2ab, LO const* v2c_ab2b,
     I8 const* v2c_codes, Real const* coords, Real const* metric, I8 const* codes,
     Real* partial) {
