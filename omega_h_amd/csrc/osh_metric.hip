@@ -227,6 +227,8 @@ void interpolate_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals cons
     OSH_LAUNCH(interpolate_kernel<1>, n, c, a.data(), b.data(), t, out.data());
 }
 
+namespace {
+
 // apply_isotropy of n metrics of dimension M: 1 (OMEGA_H_ISO_LENGTH) the
 // largest eigenvalue, 2 (OMEGA_H_ISO_SIZE) root<M>(det)
 // (src/Omega_h_metric.cpp:576-612)
@@ -306,8 +308,6 @@ __global__ void warp_halve_kernel(LO n, Real const* coords, Real const* warp_in,
   rem[i] = r + h;
   out[i] = coords[i] + h;
 }
-
-namespace {
 
 // n metrics of dimension mdim made isotropic (a copy for isotropy 0 or
 // scalar metrics); *out_mdim is the dimension of the result
