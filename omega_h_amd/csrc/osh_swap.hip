@@ -363,10 +363,10 @@ __global__ void swap3d_local_down(LO nt, LO const* prod_key, LO const* keys2edge
     out_codes[u] = code;
     found = true;
   }
-  // the side's lanes are 16 consecutive lanes of one wave
+  // the side's lanes are kSideLanes consecutive lanes of one wave
   unsigned long long any = __ballot(found);
   int base = int(threadIdx.x % 64) & ~(kSideLanes - 1);
-  if (lane == 0 && ((any >> base) & 0xffffull) == 0ull) {
+  if (lane == 0 && ((any >> base) & ((1ull << kSideLanes) - 1)) == 0ull) {
     *error = kReflectDownNoMatch;
     out[u] = 0;
     out_codes[u] = 0;
