@@ -117,6 +117,18 @@ int osh_coarsen_slivers(osh_mesh* mesh, const osh_adapt_opts* opts, int* did);
 int osh_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats,
     int* did);
 
+/* osh_adapt that also records the mesh after every pass as four FNV-1a
+ * digests (no counterpart in the reference; for comparing runs pass by
+ * pass): digests[4*i + k] after pass i, k = 0 the vertex lists of every
+ * dimension 1..dim (int32) then class_id (int32) and class_dim (int8) of
+ * every dimension 0..dim, 1 the coordinate bits, 2 the metric bits, 3 the
+ * edge length bits then the element quality bits.  digests holds
+ * 4 * OSH_MAX_PASSES values; stats is required. */
+int osh_adapt_digests(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats,
+    unsigned long long* digests, int* did);
+/* the same four digests of the mesh as it is */
+int osh_mesh_digests(osh_mesh* mesh, unsigned long long* out);
+
 /* measure_edges_metric, src/Omega_h_shape.cpp:7-48, on device pointers:
  * out[i] = metric length of edge edges[i] (or i when edges is NULL).
  * Asynchronous: ordered on the context's stream (osh_ctx_stream), so the
@@ -287,6 +299,16 @@ int osh_find_loop(osh_mesh* mesh, int edge, int* eev2v, int* loop_verts,
 /* find_indset src/Omega_h_indset.cpp:27-35 */
 int osh_find_indset(osh_mesh* mesh, int ent_dim, const double* quality,
     const signed char* cands, signed char* out);
+/* the edge-swap triangulation tables in use (the data of
+ * src/Omega_h_swap3d_tables.hpp:8-300) for one loop size 3..7:
+ * counts = {unique triangles, triangulations, unique interior edges,
+ * triangles per triangulation}; tris holds 3 loop vertices per unique
+ * triangle, meshes (size-2) unique-triangle indices per triangulation, uedges
+ * 2 loop vertices per unique interior edge, medges (size-3) unique-edge
+ * indices per triangulation; any output may be NULL.  Host data only: needs
+ * no GPU. */
+int osh_swap3d_tables(int loop_size, int* counts, int* tris, int* meshes, int* uedges,
+    int* medges);
 /* device evaluation of the scalar transcendentals: which = 0 log, 1 exp,
  * 2 cbrt, 3 acos, 4 cos */
 int osh_eval_math(osh_ctx* ctx, int which, int n, const double* in, double* out);

@@ -132,6 +132,9 @@ def load(path=None):
         "osh_swap_edges": [P, C.POINTER(AdaptOpts), IP],
         "osh_coarsen_slivers": [P, C.POINTER(AdaptOpts), IP],
         "osh_adapt": [P, C.POINTER(AdaptOpts), C.POINTER(AdaptStats), IP],
+        "osh_adapt_digests": [P, C.POINTER(AdaptOpts), C.POINTER(AdaptStats),
+                              C.POINTER(C.c_ulonglong), IP],
+        "osh_mesh_digests": [P, C.POINTER(C.c_ulonglong)],
         "osh_measure_edges_metric": [P, I, I, I, P, P, P, P, P],
         "osh_measure_qualities": [P, I, I, I, P, P, P, P, P],
         "osh_offset_scans": [P, I, P, IP, P],
@@ -139,6 +142,7 @@ def load(path=None):
         "osh_find_loop": [P, I, IP, IP, IP],
         "osh_find_indset": [P, I, DP, BP, BP],
         "osh_eval_math": [P, I, I, DP, DP],
+        "osh_swap3d_tables": [I, IP, IP, IP, IP, IP],
         "osh_limit_metric_gradation": [P, C.c_double, C.c_double, IP],
         "osh_intersect_metrics": [P, I, I, DP, DP, DP],
         "osh_mesh_digest": [P, C.POINTER(C.c_ulonglong)],
@@ -168,6 +172,26 @@ def load(path=None):
     if path == LIB_PATH or _lib is None:
         _lib = L
     return L
+
+
+def swap3d_tables(loop_size, lib=None):
+    """The library's edge-swap tables for one loop size (host data, no GPU):
+    (triangles, triangulations, unique interior edges, interior edges of each
+    triangulation) as lists."""
+    L = lib or load()
+    cnt = np.zeros(4, dtype=np.int32)
+    check(L.osh_swap3d_tables(loop_size, _p(cnt, C.c_int), None, None, None, None), L)
+    ntris, nmeshes, nue, per = (int(x) for x in cnt)
+    epm = max(loop_size - 3, 0)
+    tris = np.zeros(3 * ntris, dtype=np.int32)
+    meshes = np.zeros(max(nmeshes * per, 1), dtype=np.int32)
+    ue = np.zeros(max(2 * nue, 1), dtype=np.int32)
+    me = np.zeros(max(nmeshes * epm, 1), dtype=np.int32)
+    check(L.osh_swap3d_tables(loop_size, None, _p(tris, C.c_int), _p(meshes, C.c_int),
+                              _p(ue, C.c_int), _p(me, C.c_int)), L)
+    return (tris.reshape(-1, 3).tolist(), meshes[:nmeshes * per].reshape(nmeshes, per).tolist(),
+            ue[:2 * nue].reshape(-1, 2).tolist(),
+            me[:nmeshes * epm].reshape(nmeshes, epm).tolist())
 
 
 def check(status, lib=None):
@@ -562,6 +586,27 @@ class Mesh:
         passes = [(st.kind[i], st.nelems_in[i], st.nelems_out[i], st.seconds[i])
                   for i in range(st.npasses)]
         return did.value, passes
+
+
+    def adapt_digests(self, opts):
+        """adapt() recording four digests of the mesh after every pass:
+        returns (did, passes, digests) with digests[i] = (topology and
+        classification, coordinates, metric, lengths and qualities)."""
+        st = AdaptStats()
+        did = C.c_int()
+        dg = (C.c_ulonglong * (4 * MAX_PASSES))()
+        check(self.lib.osh_adapt_digests(self.h, C.byref(opts), C.byref(st), dg, C.byref(did)),
+              self.lib)
+        passes = [(st.kind[i], st.nelems_in[i], st.nelems_out[i], st.seconds[i])
+                  for i in range(st.npasses)]
+        digests = [tuple(dg[4 * i + k] for k in range(4)) for i in range(st.npasses)]
+        return did.value, passes, digests
+
+    def digests(self):
+        """The same four digests of the mesh as it is."""
+        dg = (C.c_ulonglong * 4)()
+        check(self.lib.osh_mesh_digests(self.h, dg), self.lib)
+        return tuple(dg)
 
 
 class Server:

@@ -7,6 +7,7 @@
 // clock with the stream drained at both ends.
 #include <chrono>
 #include <cstdio>
+#include <functional>
 #include <vector>
 
 #include "osh_passes.hpp"
@@ -58,7 +59,8 @@ double now() {
 // *qmin receives the minimum element quality after the pass, reduced before
 // the pass's closing sync so that it costs no sync of its own
 bool run_pass(Ctx& c, DMesh& m, AdaptOptions const& o, int kind,
-    std::vector<PassRecord>* log, Real* qmin, SwapCache* cache = nullptr) {
+    std::vector<PassRecord>* log, Real* qmin, PassHook const* hook,
+    SwapCache* cache = nullptr) {
   OSH_CHECK(!log || log->size() < 4096, "adapt exceeded 4096 passes");
   LO nin = m.nents[m.dim];
   c.sync();
@@ -77,6 +79,7 @@ bool run_pass(Ctx& c, DMesh& m, AdaptOptions const& o, int kind,
   double t1 = now();
   *qmin = read_slot<Real>(c, kQminSlot);
   if (r && log) log->push_back({kind, nin, m.nents[m.dim], t1 - t0});
+  if (r && hook && *hook) (*hook)(m);
   // post_rebuild, src/Omega_h_adapt.cpp:169-171
   if (r && o.verbosity >= 2) print_status(c, m, o);
   return r;
@@ -84,7 +87,8 @@ bool run_pass(Ctx& c, DMesh& m, AdaptOptions const& o, int kind,
 
 }  // namespace
 
-bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log) {
+bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log,
+    PassHook const* hook) {
   OSH_CHECK(0.0 <= o.min_quality_allowed, "min_quality_allowed < 0");
   OSH_CHECK(o.min_quality_allowed <= o.min_quality_desired, "min_quality_allowed > desired");
   OSH_CHECK(o.min_quality_desired <= 1.0, "min_quality_desired > 1");
@@ -107,8 +111,8 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
   Real q = qmin;  // the minimum quality of the current mesh
   do {
     did = false;
-    if (o.should_refine && run_pass(c, m, o, 0, log, &q)) did = true;
-    if (o.should_coarsen && run_pass(c, m, o, 1, log, &q)) did = true;
+    if (o.should_refine && run_pass(c, m, o, 0, log, &q, hook)) did = true;
+    if (o.should_coarsen && run_pass(c, m, o, 1, log, &q, hook)) did = true;
   } while (did);
   double t2 = now();
   if (q < o.min_quality_desired) {
@@ -117,9 +121,9 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
     // the mesh and drops them
     SwapCache cache;
     do {
-      if (o.should_swap && run_pass(c, m, o, 2, log, &q, &cache)) continue;
+      if (o.should_swap && run_pass(c, m, o, 2, log, &q, hook, &cache)) continue;
       cache.clear();
-      if (o.should_coarsen_slivers && run_pass(c, m, o, 3, log, &q)) continue;
+      if (o.should_coarsen_slivers && run_pass(c, m, o, 3, log, &q, hook)) continue;
       if (o.verbosity >= 1) std::printf("could not satisfy quality\n");
       break;
     } while (q < o.min_quality_desired);

@@ -7,6 +7,7 @@
 #include "osh_guard.hpp"
 #include "osh_math.hpp"
 #include "osh_passes.hpp"
+#include "osh_swap_tables.hpp"
 
 struct osh_ctx {
   osh::Ctx ctx;
@@ -32,6 +33,45 @@ void to_options(osh_adapt_opts const* o, AdaptOptions& r) {
   r.should_coarsen = o->should_coarsen;
   r.should_swap = o->should_swap;
   r.should_coarsen_slivers = o->should_coarsen_slivers;
+}
+
+namespace {
+std::uint64_t fnv(std::uint64_t h, void const* p, std::size_t n) {
+  auto b = static_cast<unsigned char const*>(p);
+  for (std::size_t i = 0; i < n; ++i) {
+    h ^= b[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+constexpr std::uint64_t kFnvBasis = 1469598103934665603ull;
+}  // namespace
+
+// four FNV-1a digests of a mesh: 0 the vertex lists of every dimension
+// 1..dim (int32) then class_id (int32) and class_dim (int8) of every
+// dimension 0..dim; 1 coordinate bits; 2 metric bits; 3 edge length bits
+// then element quality bits
+void mesh_digests(Ctx& c, DMesh& m, std::uint64_t* out) {
+  std::uint64_t h = kFnvBasis;
+  for (int d = 1; d <= m.dim; ++d) {
+    auto v = to_host(c, verts_of(c, m, d));
+    h = fnv(h, v.data(), v.size() * sizeof(LO));
+  }
+  for (int d = 0; d <= m.dim; ++d) {
+    auto id = to_host(c, m.class_id[d]);
+    auto cd = to_host(c, m.class_dim[d]);
+    h = fnv(h, id.data(), id.size() * sizeof(LO));
+    h = fnv(h, cd.data(), cd.size());
+  }
+  out[0] = h;
+  auto x = to_host(c, m.coords);
+  out[1] = fnv(kFnvBasis, x.data(), x.size() * sizeof(Real));
+  auto mt = to_host(c, m.metric);
+  out[2] = fnv(kFnvBasis, mt.data(), mt.size() * sizeof(Real));
+  auto l = to_host(c, ask_lengths(c, m));
+  auto q = to_host(c, ask_qualities(c, m));
+  out[3] = fnv(fnv(kFnvBasis, l.data(), l.size() * sizeof(Real)), q.data(),
+      q.size() * sizeof(Real));
 }
 
 // FNV-1a over the element vertex lists and the bits of the coordinates
@@ -68,6 +108,33 @@ void download(osh::Ctx& c, osh::DArray<T> const& d, T* out) {
 
 void check_mesh(osh_mesh* m) {
   OSH_CHECK(m != nullptr, "null mesh handle");
+}
+
+// what the passes read: a metric of 1 or dim*(dim+1)/2 components and both
+// classification tags on every dimension (host checks, so a malformed mesh
+// fails with a message before any kernel runs)
+void check_metric(osh_mesh* m) {
+  OSH_CHECK(m != nullptr, "null mesh handle");
+  auto const& d = m->m;
+  OSH_CHECK(d.metric.exists(), "mesh has no metric");
+  OSH_CHECK(d.metric_ncomps == 1 || d.metric_ncomps == d.dim * (d.dim + 1) / 2,
+      "metric must have 1 or dim*(dim+1)/2 components");
+  OSH_CHECK(d.metric.size() == d.nents[0] * d.metric_ncomps, "metric has the wrong size");
+}
+
+void check_classified(osh_mesh* m) {
+  OSH_CHECK(m != nullptr, "null mesh handle");
+  auto const& d = m->m;
+  for (int k = 0; k <= d.dim; ++k) {
+    OSH_CHECK(d.class_id[k].exists() && d.class_dim[k].exists(), "mesh is not classified");
+    OSH_CHECK(d.class_id[k].size() == d.nents[k] && d.class_dim[k].size() == d.nents[k],
+        "classification has the wrong size");
+  }
+}
+
+void check_adaptable(osh_mesh* m) {
+  check_metric(m);
+  check_classified(m);
 }
 
 void check_dim(osh_mesh* m, int d) {
@@ -398,10 +465,8 @@ int osh_mesh_get_qualities(osh_mesh* mesh, double* out) {
 
 static int pass(osh_mesh* mesh, const osh_adapt_opts* opts, int* did, int kind) {
   return guarded([&] {
-    check_mesh(mesh);
+    check_adaptable(mesh);
     OSH_CHECK(opts != nullptr, "null options");
-    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
-    OSH_CHECK(mesh->m.class_id[0].exists(), "mesh is not classified");
     osh::AdaptOptions o;
     to_options(opts, o);
     auto& c = mesh->owner->ctx;
@@ -426,10 +491,8 @@ int osh_coarsen_slivers(osh_mesh* m, const osh_adapt_opts* o, int* did) { return
 
 int osh_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats, int* did) {
   return guarded([&] {
-    check_mesh(mesh);
+    check_adaptable(mesh);
     OSH_CHECK(opts != nullptr, "null options");
-    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
-    OSH_CHECK(mesh->m.class_id[0].exists(), "mesh is not classified");
     osh::AdaptOptions o;
     to_options(opts, o);
     std::vector<osh::PassRecord> log;
@@ -445,6 +508,45 @@ int osh_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats
       }
     }
     if (did) *did = r ? 1 : 0;
+  });
+}
+
+int osh_adapt_digests(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats,
+    unsigned long long* digests, int* did) {
+  return guarded([&] {
+    check_adaptable(mesh);
+    OSH_CHECK(opts != nullptr && stats != nullptr && digests != nullptr, "null argument");
+    osh::AdaptOptions o;
+    to_options(opts, o);
+    auto& c = mesh->owner->ctx;
+    std::vector<osh::PassRecord> log;
+    osh::PassHook hook = [&](osh::DMesh& m) {
+      if (log.size() <= OSH_MAX_PASSES) {
+        std::uint64_t d[4];
+        osh::mesh_digests(c, m, d);
+        for (int k = 0; k < 4; ++k) digests[4 * (log.size() - 1) + k] = d[k];
+      }
+    };
+    bool r = osh::adapt(c, mesh->m, o, &log, &hook);
+    c.sync();
+    stats->npasses = int(std::min<std::size_t>(log.size(), OSH_MAX_PASSES));
+    for (int i = 0; i < stats->npasses; ++i) {
+      stats->kind[i] = log[i].kind;
+      stats->nelems_in[i] = log[i].nelems_in;
+      stats->nelems_out[i] = log[i].nelems_out;
+      stats->seconds[i] = log[i].seconds;
+    }
+    if (did) *did = r ? 1 : 0;
+  });
+}
+
+int osh_mesh_digests(osh_mesh* mesh, unsigned long long* out) {
+  return guarded([&] {
+    check_adaptable(mesh);
+    OSH_CHECK(out != nullptr, "null output pointer");
+    std::uint64_t d[4];
+    osh::mesh_digests(mesh->owner->ctx, mesh->m, d);
+    for (int k = 0; k < 4; ++k) out[k] = d[k];
   });
 }
 
@@ -663,9 +765,8 @@ int osh_mesh_remove_elem_tag(osh_mesh* mesh, const char* name) {
 
 int osh_approach_metric(osh_mesh* mesh, const osh_adapt_opts* opts, double min_step, int* did) {
   return guarded([&] {
-    check_mesh(mesh);
+    check_metric(mesh);
     OSH_CHECK(opts != nullptr, "null options");
-    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
     osh::AdaptOptions o;
     to_options(opts, o);
     bool r = osh::approach_metric(mesh->owner->ctx, mesh->m, o, min_step);
@@ -677,9 +778,8 @@ int osh_approach_metric(osh_mesh* mesh, const osh_adapt_opts* opts, double min_s
 int osh_warp_to_limit(osh_mesh* mesh, const osh_adapt_opts* opts, int exit_on_stall,
     int max_niters, int* did) {
   return guarded([&] {
-    check_mesh(mesh);
+    check_metric(mesh);
     OSH_CHECK(opts != nullptr, "null options");
-    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
     osh::AdaptOptions o;
     to_options(opts, o);
     bool r = osh::warp_to_limit(mesh->owner->ctx, mesh->m, o, exit_on_stall != 0, max_niters);
@@ -730,7 +830,7 @@ int osh_ill_metric(osh_mesh* mesh, const osh_adapt_opts* opts, int isotropy) {
 
 int osh_fix(osh_mesh* mesh, const osh_adapt_opts* opts, int isotropy, int* ncalls) {
   return guarded([&] {
-    check_mesh(mesh);
+    check_classified(mesh);
     OSH_CHECK(opts != nullptr, "null options");
     osh::AdaptOptions o;
     to_options(opts, o);
@@ -742,9 +842,8 @@ int osh_fix(osh_mesh* mesh, const osh_adapt_opts* opts, int isotropy, int* ncall
 
 int osh_fix_for_given_metric(osh_mesh* mesh, const osh_adapt_opts* opts, int* ncalls) {
   return guarded([&] {
-    check_mesh(mesh);
+    check_adaptable(mesh);
     OSH_CHECK(opts != nullptr, "null options");
-    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
     osh::AdaptOptions o;
     to_options(opts, o);
     int r = osh::fix_for_given_metric(mesh->owner->ctx, mesh->m, o);
@@ -756,7 +855,7 @@ int osh_fix_for_given_metric(osh_mesh* mesh, const osh_adapt_opts* opts, int* nc
 int osh_grade_fix_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, int ncomps,
     const double* target, int* nsteps) {
   return guarded([&] {
-    check_mesh(mesh);
+    check_classified(mesh);
     OSH_CHECK(opts != nullptr && target != nullptr, "null argument");
     int dim = mesh->m.dim;
     OSH_CHECK(ncomps == 1 || ncomps == (dim * (dim + 1)) / 2, "target metric size");
@@ -856,6 +955,33 @@ int osh_find_indset(osh_mesh* mesh, int d, const double* quality, const signed c
     osh::upload(c, cb, reinterpret_cast<osh::I8 const*>(cands), n);
     auto r = osh::find_indset(c, mesh->m, d, q, cb);
     download(c, r, reinterpret_cast<osh::I8*>(out));
+  });
+}
+
+int osh_swap3d_tables(int loop_size, int* counts, int* tris, int* meshes, int* uedges,
+    int* medges) {
+  return guarded([&] {
+    OSH_CHECK(loop_size >= 3 && loop_size <= osh::swap::kMaxLoop, "loop size must be 3 to 7");
+    static osh::swap::Tables const t = osh::swap::make_tables();
+    int n = loop_size;
+    int ntris = t.ntris[n], nmeshes = t.mesh_counts[n], nue = t.nuedges[n];
+    int per_mesh = t.mesh_sizes[n], edges_per_mesh = t.nedges[n];
+    if (counts) {
+      counts[0] = ntris;
+      counts[1] = nmeshes;
+      counts[2] = nue;
+      counts[3] = per_mesh;
+    }
+    if (tris)
+      for (int i = 0; i < ntris; ++i)
+        for (int k = 0; k < 3; ++k) tris[i * 3 + k] = t.tris[n][i][k];
+    if (meshes)
+      for (int i = 0; i < nmeshes * per_mesh; ++i) meshes[i] = t.meshes[n][i];
+    if (uedges)
+      for (int i = 0; i < nue; ++i)
+        for (int k = 0; k < 2; ++k) uedges[i * 2 + k] = t.uedges[n][i][k];
+    if (medges)
+      for (int i = 0; i < nmeshes * edges_per_mesh; ++i) medges[i] = t.medges[n][i];
   });
 }
 

@@ -206,6 +206,12 @@ void read_osh(Ctx& c, DMesh& m, std::string const& dir) {
     if (d > 1) {
       auto codes = get_array<I8>(f, z);
       OSH_CHECK(codes.size() == down.size(), "bad alignment code count");
+      // a down code is make_code(flip, rotation, 0) with rotation below the
+      // side's vertex count (d) and no flip for edges (src/Omega_h_align.hpp:43-54)
+      for (I8 code : codes)
+        OSH_CHECK(code >= 0 && code_which_down(code) == 0 && code_rotation(code) < d &&
+                      (d > 2 || !code_is_flipped(code)),
+            "alignment code out of range");
       r.down_codes[d] = to_device(c, codes);
     }
   }
@@ -236,6 +242,8 @@ void read_osh(Ctx& c, DMesh& m, std::string const& dir) {
           OSH_CHECK(ncomps == dim, "coordinates must have dim components");
           r.coords = to_device(c, a);
         } else if (name == "metric") {
+          OSH_CHECK(ncomps == 1 || ncomps == dim * (dim + 1) / 2,
+              "metric must have 1 or dim*(dim+1)/2 components");
           r.metric = to_device(c, a);
           r.metric_ncomps = ncomps;
         } else {
@@ -274,6 +282,16 @@ void read_osh(Ctx& c, DMesh& m, std::string const& dir) {
     }
   }
   OSH_CHECK(r.coords.exists(), "the mesh has no coordinates");
+  // classification is all or nothing: every dimension carries both tags, or
+  // none does (the passes read both on every dimension)
+  bool any_class = false, all_class = true;
+  for (int d = 0; d <= dim; ++d) {
+    for (bool has : {r.class_id[d].exists(), r.class_dim[d].exists()}) {
+      any_class = any_class || has;
+      all_class = all_class && has;
+    }
+  }
+  OSH_CHECK(all_class || !any_class, "class_id/class_dim missing on some dimensions");
   c.sync();
   m = r;
 }

@@ -440,7 +440,9 @@ OSH_HD int roots2(double const* coeffs, double eps, double* roots, int* mults) {
   return 0;
 }
 
-OSH_HD void single_eigenvector3(Mat const& m, double l, double* out) {
+// false when no row pair spans a plane (the reference's check fails,
+// src/Omega_h_eigen.hpp:191); out is then left as it was
+OSH_HD bool single_eigenvector3(Mat const& m, double l, double* out) {
   Mat s0 = m;
   for (int i = 0; i < 3; ++i) s0.m[i][i] -= l;
   Mat s = transpose<3>(s0);
@@ -463,12 +465,15 @@ OSH_HD void single_eigenvector3(Mat const& m, double l, double* out) {
     v[2] = c[2];
     v_norm = c_norm;
   }
-  if (!(v_norm > 1e-10)) return;
+  if (!(v_norm > 1e-10)) return false;
   for (int i = 0; i < 3; ++i) out[i] = v[i] / v_norm;
+  return true;
 }
 
+// false when every row is (numerically) zero (the reference's check fails,
+// src/Omega_h_eigen.hpp:211); out is then left as it was
 template <int N>
-OSH_HD void row_space_1d(Mat const& a, double* out) {
+OSH_HD bool row_space_1d(Mat const& a, double* out) {
   Mat ta = transpose<N>(a);
   int best_row = 0;
   double best_norm = vnorm<N>(ta.m[0]);
@@ -479,8 +484,9 @@ OSH_HD void row_space_1d(Mat const& a, double* out) {
       best_norm = row_norm;
     }
   }
-  if (!(best_norm > 1e-10)) return;
+  if (!(best_norm > 1e-10)) return false;
   for (int i = 0; i < N; ++i) out[i] = ta.m[best_row][i] / best_norm;
+  return true;
 }
 
 OSH_HD void ortho_basis3(double const* v, Mat& a) {
@@ -539,9 +545,11 @@ OSH_HD double max_eigenvalue(Mat const& m) {
   return out;
 }
 
-// decompose_eigen with max-norm scaling, src/Omega_h_eigen.hpp:228-319
+// decompose_eigen with max-norm scaling, src/Omega_h_eigen.hpp:228-319;
+// *bad (when given) is set when an eigenvector cannot be formed, where the
+// reference's checks abort (src/Omega_h_eigen.hpp:191,211)
 template <int N>
-OSH_HD void decompose_eigen(Mat const& m0, Mat& q, double* l) {
+OSH_HD void decompose_eigen(Mat const& m0, Mat& q, double* l, int* bad = nullptr) {
   double nm = 0.0;
   for (int j = 0; j < N; ++j)
     for (int i = 0; i < N; ++i) nm = math::max2(nm, math::fabs_(m0.m[j][i]));
@@ -565,18 +573,19 @@ OSH_HD void decompose_eigen(Mat const& m0, Mat& q, double* l) {
     double roots[3];
     int mults[3];
     int nroots = roots3(coeffs, 5e-5, roots, mults);
+    bool ok = true;
     if (nroots == 3) {
       for (int i = 0; i < 3; ++i) {
-        single_eigenvector3(m, roots[i], q.m[i]);
+        ok = single_eigenvector3(m, roots[i], q.m[i]) && ok;
         l[i] = roots[i];
       }
     } else if (nroots == 2 && mults[1] == 2) {
-      single_eigenvector3(m, roots[0], q.m[0]);
+      ok = single_eigenvector3(m, roots[0], q.m[0]);
       l[0] = roots[0];
       Mat s = m;
       for (int i = 0; i < 3; ++i) s.m[i][i] -= roots[1];
       double n[3] = {0.0, 0.0, 0.0};
-      row_space_1d<3>(s, n);
+      ok = row_space_1d<3>(s, n) && ok;
       Mat b;
       ortho_basis3(n, b);
       for (int i = 0; i < 3; ++i) {
@@ -588,6 +597,7 @@ OSH_HD void decompose_eigen(Mat const& m0, Mat& q, double* l) {
       l[0] = l[1] = l[2] = roots[0];
       identity<3>(q);
     }
+    if (!ok && bad) *bad = 1;
   } else if (N == 2) {
     double coeffs[2];
     coeffs[1] = -trace<2>(m);
@@ -600,7 +610,7 @@ OSH_HD void decompose_eigen(Mat const& m0, Mat& q, double* l) {
         Mat s = m;
         for (int k = 0; k < 2; ++k) s.m[k][k] -= roots[i];
         double r[2] = {0.0, 0.0};
-        row_space_1d<2>(s, r);
+        if (!row_space_1d<2>(s, r) && bad) *bad = 1;
         q.m[i][0] = -r[1];
         q.m[i][1] = r[0];
         l[i] = roots[i];
@@ -627,37 +637,37 @@ OSH_HD Mat compose_ortho(Mat const& q, double const* l) {
 }
 
 template <int N>
-OSH_HD Mat log_spd(Mat const& m) {
+OSH_HD Mat log_spd(Mat const& m, int* bad = nullptr) {
   Mat q;
   double l[3];
-  decompose_eigen<N>(m, q, l);
+  decompose_eigen<N>(m, q, l, bad);
   for (int i = 0; i < N; ++i) l[i] = math::log_(l[i]);
   return compose_ortho<N>(q, l);
 }
 
 template <int N>
-OSH_HD Mat exp_spd(Mat const& m) {
+OSH_HD Mat exp_spd(Mat const& m, int* bad = nullptr) {
   Mat q;
   double l[3];
-  decompose_eigen<N>(m, q, l);
+  decompose_eigen<N>(m, q, l, bad);
   for (int i = 0; i < N; ++i) l[i] = math::exp_(l[i]);
   return compose_ortho<N>(q, l);
 }
 
 // log-Euclidean average of two metrics, src/Omega_h_metric.hpp:125-149
 template <int M>
-OSH_HD void average_metric2(double const* a, double const* b, double* out) {
+OSH_HD void average_metric2(double const* a, double const* b, double* out, int* bad = nullptr) {
   Mat am;
   mat_zero(am);
   double const* ms[2] = {a, b};
   for (int k = 0; k < 2; ++k) {
-    Mat lm = log_spd<M>(get_symm<M>(ms[k]));
+    Mat lm = log_spd<M>(get_symm<M>(ms[k]), bad);
     for (int j = 0; j < M; ++j)
       for (int i = 0; i < M; ++i) am.m[j][i] = am.m[j][i] + lm.m[j][i];
   }
   for (int j = 0; j < M; ++j)
     for (int i = 0; i < M; ++i) am.m[j][i] = am.m[j][i] / 2.0;
-  set_symm<M>(exp_spd<M>(am), out);
+  set_symm<M>(exp_spd<M>(am, bad), out);
 }
 
 // ------------------------------------------------------------ geometry

@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(kBlock) measure_transfer_kernel(LO n, LO nsame
 
 template <int M>
 __global__ void mident_kernel(LO n, LO const* edges, LO const* ev2v, Real const* metric,
-    Real* out) {
+    Real* out, int* error) {
   LO i = thread_index();
   if (i >= n) return;
   LO e = edges[i];
@@ -119,7 +119,9 @@ __global__ void mident_kernel(LO n, LO const* edges, LO const* ev2v, Real const*
     a[k] = metric[ev2v[2 * e] * nc + k];
     b[k] = metric[ev2v[2 * e + 1] * nc + k];
   }
-  average_metric2<M>(a, b, out + i * nc);
+  int bad = 0;
+  average_metric2<M>(a, b, out + i * nc, &bad);
+  if (bad) *error = kDegenerateEigen;
 }
 
 // real_simplex_size (src/Omega_h_shape.hpp:77-100,166-170) of each element
@@ -254,11 +256,11 @@ void mident_values(Ctx& c, DMesh& m, LO n, LO const* edges, Real const* data, in
   int md = (ncomps == 1) ? 1 : (ncomps == 3 ? 2 : 3);
   OSH_CHECK(ncomps == 1 || ncomps == 3 || ncomps == 6, "a metric has 1, 3 or 6 components");
   if (md == 3)
-    OSH_LAUNCH(mident_kernel<3>, n, c, edges, m.down[1].data(), data, out);
+    OSH_LAUNCH(mident_kernel<3>, n, c, edges, m.down[1].data(), data, out, c.error_word());
   else if (md == 2)
-    OSH_LAUNCH(mident_kernel<2>, n, c, edges, m.down[1].data(), data, out);
+    OSH_LAUNCH(mident_kernel<2>, n, c, edges, m.down[1].data(), data, out, c.error_word());
   else
-    OSH_LAUNCH(mident_kernel<1>, n, c, edges, m.down[1].data(), data, out);
+    OSH_LAUNCH(mident_kernel<1>, n, c, edges, m.down[1].data(), data, out, c.error_word());
 }
 
 void mident_metrics(Ctx& c, DMesh& m, LO n, LO const* edges, Real* out) {

@@ -23,9 +23,10 @@ namespace {
 
 template <int S, int M>
 __global__ void limit_gradation_kernel(LO nv, LO const* a2ab, LO const* ab2b,
-    Real const* coords, Real const* in, Real max_rate, Real* out) {
+    Real const* coords, Real const* in, Real max_rate, Real* out, int* error) {
   LO v = xcd_thread_index();
   if (v >= nv) return;
+  int bad = 0;
   constexpr int nc = (M * (M + 1)) / 2;
   Mat m = get_symm<M>(in + std::size_t(v) * nc);
   double x[S];
@@ -40,9 +41,10 @@ __global__ void limit_gradation_kernel(LO nv, LO const* a2ab, LO const* ab2b,
     double factor = 1.0 / (h * h);
     for (int c = 0; c < M; ++c)
       for (int r = 0; r < M; ++r) am.m[c][r] = am.m[c][r] * factor;
-    m = intersect_metrics<M>(m, am);
+    m = intersect_metrics<M>(m, am, &bad);
   }
   set_symm<M>(m, out + std::size_t(v) * nc);
+  if (bad) *error = kDegenerateEigen;
 }
 
 __global__ void not_close_kernel(LO n, Real const* a, Real const* b, Real tol, Real floor,
@@ -56,28 +58,33 @@ __global__ void not_close_kernel(LO n, Real const* a, Real const* b, Real tol, R
 }
 
 template <int M>
-__global__ void intersect_kernel(LO n, Real const* a, Real const* b, Real* out) {
+__global__ void intersect_kernel(LO n, Real const* a, Real const* b, Real* out, int* error) {
   LO i = thread_index();
   if (i >= n) return;
   constexpr int nc = (M * (M + 1)) / 2;
+  int bad = 0;
   Mat r = intersect_metrics<M>(get_symm<M>(a + std::size_t(i) * nc),
-      get_symm<M>(b + std::size_t(i) * nc));
+      get_symm<M>(b + std::size_t(i) * nc), &bad);
   set_symm<M>(r, out + std::size_t(i) * nc);
+  if (bad) *error = kDegenerateEigen;
 }
 
 // interpolate_between_metrics, src/Omega_h_metric.cpp:106-111: linearize
 // (log), a (1 - t) + b t per stored component
 // (src/Omega_h_array_ops.cpp:567-574), delinearize (exp)
 template <int M>
-__global__ void interpolate_kernel(LO n, Real const* a, Real const* b, Real t, Real* out) {
+__global__ void interpolate_kernel(LO n, Real const* a, Real const* b, Real t, Real* out,
+    int* error) {
   LO i = thread_index();
   if (i >= n) return;
   constexpr int nc = (M * (M + 1)) / 2;
   double va[6], vb[6], vc[6];
-  set_symm<M>(log_spd<M>(get_symm<M>(a + std::size_t(i) * nc)), va);
-  set_symm<M>(log_spd<M>(get_symm<M>(b + std::size_t(i) * nc)), vb);
+  int bad = 0;
+  set_symm<M>(log_spd<M>(get_symm<M>(a + std::size_t(i) * nc), &bad), va);
+  set_symm<M>(log_spd<M>(get_symm<M>(b + std::size_t(i) * nc), &bad), vb);
   for (int k = 0; k < nc; ++k) vc[k] = va[k] * (1.0 - t) + vb[k] * t;
-  set_symm<M>(exp_spd<M>(get_symm<M>(vc)), out + std::size_t(i) * nc);
+  set_symm<M>(exp_spd<M>(get_symm<M>(vc), &bad), out + std::size_t(i) * nc);
+  if (bad) *error = kDegenerateEigen;
 }
 
 // size-scaled implied metric of each element (src/Omega_h_metric.cpp:257-301)
@@ -102,9 +109,10 @@ __global__ void element_implied_kernel(LO ne, LO const* ev2v, Real const* coords
 // degenerate metrics skipped, src/Omega_h_metric.hpp:125-139)
 template <int D>
 __global__ void project_metrics_kernel(LO nv, LO const* a2ab, LO const* ab2b, Real const* e2m,
-    Real* out) {
+    Real* out, int* error) {
   LO v = thread_index();
   if (v >= nv) return;
+  int bad = 0;
   constexpr int nc = (D * (D + 1)) / 2;
   Mat am;
   mat_zero(am);
@@ -115,7 +123,7 @@ __global__ void project_metrics_kernel(LO nv, LO const* a2ab, LO const* ab2b, Re
     for (int c = 0; c < D; ++c)
       for (int r = 0; r < D; ++r) mx = math::max2(mx, math::fabs_(em.m[c][r]));
     if (mx < kMetricEpsilon) continue;
-    Mat lm = log_spd<D>(em);
+    Mat lm = log_spd<D>(em, &bad);
     for (int c = 0; c < D; ++c)
       for (int r = 0; r < D; ++r) am.m[c][r] = am.m[c][r] + lm.m[c][r];
     ++n;
@@ -124,9 +132,10 @@ __global__ void project_metrics_kernel(LO nv, LO const* a2ab, LO const* ab2b, Re
   if (n > 0) {
     for (int c = 0; c < D; ++c)
       for (int rr = 0; rr < D; ++rr) am.m[c][rr] = am.m[c][rr] / n;
-    r = exp_spd<D>(am);
+    r = exp_spd<D>(am, &bad);
   }
   set_symm<D>(r, out + std::size_t(v) * nc);
+  if (bad) *error = kDegenerateEigen;
 }
 
 Reals limit_gradation_once(Ctx& c, DMesh& m, Reals const& values, Real max_rate) {
@@ -136,7 +145,7 @@ Reals limit_gradation_once(Ctx& c, DMesh& m, Reals const& values, Real max_rate)
   int md = (m.metric_ncomps == 1) ? 1 : m.dim;
 #define OSH_LG(S, M)                                                                       \
   OSH_LAUNCH((limit_gradation_kernel<S, M>), nv, c, star.a2ab.data(), star.ab2b.data(),    \
-      m.coords.data(), values.data(), max_rate, out.data())
+      m.coords.data(), values.data(), max_rate, out.data(), c.error_word())
   if (m.dim == 3 && md == 3)
     OSH_LG(3, 3);
   else if (m.dim == 3)
@@ -196,13 +205,13 @@ Reals project_metric_array(Ctx& c, DMesh& m, int mdim, Reals const& e2m) {
   Reals out(c, m.nents[0] * nc);
   if (mdim == 3)
     OSH_LAUNCH(project_metrics_kernel<3>, m.nents[0], c, v2e.a2ab.data(), v2e.ab2b.data(),
-        e2m.data(), out.data());
+        e2m.data(), out.data(), c.error_word());
   else if (mdim == 2)
     OSH_LAUNCH(project_metrics_kernel<2>, m.nents[0], c, v2e.a2ab.data(), v2e.ab2b.data(),
-        e2m.data(), out.data());
+        e2m.data(), out.data(), c.error_word());
   else
     OSH_LAUNCH(project_metrics_kernel<1>, m.nents[0], c, v2e.a2ab.data(), v2e.ab2b.data(),
-        e2m.data(), out.data());
+        e2m.data(), out.data(), c.error_word());
   return out;
 }
 
@@ -220,11 +229,11 @@ void interpolate_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals cons
     Reals& out) {
   out = Reals(c, a.size());
   if (dim == 3)
-    OSH_LAUNCH(interpolate_kernel<3>, n, c, a.data(), b.data(), t, out.data());
+    OSH_LAUNCH(interpolate_kernel<3>, n, c, a.data(), b.data(), t, out.data(), c.error_word());
   else if (dim == 2)
-    OSH_LAUNCH(interpolate_kernel<2>, n, c, a.data(), b.data(), t, out.data());
+    OSH_LAUNCH(interpolate_kernel<2>, n, c, a.data(), b.data(), t, out.data(), c.error_word());
   else
-    OSH_LAUNCH(interpolate_kernel<1>, n, c, a.data(), b.data(), t, out.data());
+    OSH_LAUNCH(interpolate_kernel<1>, n, c, a.data(), b.data(), t, out.data(), c.error_word());
 }
 
 namespace {
@@ -250,9 +259,11 @@ __global__ void isotropy_kernel(LO n, Real const* in, int isotropy, Real* out) {
 // (src/Omega_h_metric.cpp:56-104): each element's vertex metrics averaged,
 // degenerate ones skipped (src/Omega_h_metric.hpp:125-149)
 template <int D, int M>
-__global__ void element_mident_kernel(LO ne, LO const* ev2v, Real const* v2m, Real* out) {
+__global__ void element_mident_kernel(LO ne, LO const* ev2v, Real const* v2m, Real* out,
+    int* error) {
   LO e = thread_index();
   if (e >= ne) return;
+  int bad = 0;
   constexpr int nc = (M * (M + 1)) / 2;
   Mat am;
   mat_zero(am);
@@ -263,7 +274,7 @@ __global__ void element_mident_kernel(LO ne, LO const* ev2v, Real const* v2m, Re
     for (int cc = 0; cc < M; ++cc)
       for (int r = 0; r < M; ++r) mx = math::max2(mx, math::fabs_(vm.m[cc][r]));
     if (mx < kMetricEpsilon) continue;
-    Mat lm = log_spd<M>(vm);
+    Mat lm = log_spd<M>(vm, &bad);
     for (int cc = 0; cc < M; ++cc)
       for (int r = 0; r < M; ++r) am.m[cc][r] = am.m[cc][r] + lm.m[cc][r];
     ++n;
@@ -272,9 +283,10 @@ __global__ void element_mident_kernel(LO ne, LO const* ev2v, Real const* v2m, Re
   if (n > 0) {
     for (int cc = 0; cc < M; ++cc)
       for (int rr = 0; rr < M; ++rr) am.m[cc][rr] = am.m[cc][rr] / n;
-    r = exp_spd<M>(am);
+    r = exp_spd<M>(am, &bad);
   }
   set_symm<M>(r, out + std::size_t(e) * nc);
+  if (bad) *error = kDegenerateEigen;
 }
 
 // compute_ill_metric_dim (src/Omega_h_fix.cpp:40-48): the element metrics of
@@ -460,9 +472,11 @@ void ill_metric(Ctx& c, DMesh& m, AdaptOptions const& o, int isotropy) {
     for (int i = 0; i < 5; ++i) {
       Reals mid(c, ne * nc);
       if (dim == 3)
-        OSH_LAUNCH((element_mident_kernel<3, 3>), ne, c, ev2v.data(), vm.data(), mid.data());
+        OSH_LAUNCH((element_mident_kernel<3, 3>), ne, c, ev2v.data(), vm.data(), mid.data(),
+            c.error_word());
       else
-        OSH_LAUNCH((element_mident_kernel<2, 2>), ne, c, ev2v.data(), vm.data(), mid.data());
+        OSH_LAUNCH((element_mident_kernel<2, 2>), ne, c, ev2v.data(), vm.data(), mid.data(),
+            c.error_word());
       vm = project_metric_array(c, m, mdim, mid);
     }
   }
@@ -546,11 +560,11 @@ void intersect_metric_arrays(Ctx& c, int dim, LO n, Reals const& a, Reals const&
     Reals& out) {
   out = Reals(c, a.size());
   if (dim == 3)
-    OSH_LAUNCH(intersect_kernel<3>, n, c, a.data(), b.data(), out.data());
+    OSH_LAUNCH(intersect_kernel<3>, n, c, a.data(), b.data(), out.data(), c.error_word());
   else if (dim == 2)
-    OSH_LAUNCH(intersect_kernel<2>, n, c, a.data(), b.data(), out.data());
+    OSH_LAUNCH(intersect_kernel<2>, n, c, a.data(), b.data(), out.data(), c.error_word());
   else
-    OSH_LAUNCH(intersect_kernel<1>, n, c, a.data(), b.data(), out.data());
+    OSH_LAUNCH(intersect_kernel<1>, n, c, a.data(), b.data(), out.data(), c.error_word());
 }
 
 }  // namespace osh

@@ -102,8 +102,10 @@ OSH_HD void normalize3(double* v) {
   for (int i = 0; i < 3; ++i) v[i] = v[i] / n;
 }
 
+// *bad (when given) is set when an eigendecomposition fails (the reference
+// aborts there)
 template <int N>
-OSH_HD Mat intersect_metrics(Mat const& m1_in, Mat const& m2_in);
+OSH_HD Mat intersect_metrics(Mat const& m1_in, Mat const& m2_in, int* bad = nullptr);
 
 // cases 3.a and 4.a: the intersection within the plane of u1 and v1
 OSH_HD Mat intersect_in_plane(Mat const& m1, Mat const& m2, double const* u1,
@@ -227,11 +229,11 @@ OSH_HD Mat intersect_degenerate3(Mat const& m1, Mat const& q1, double const* l1v
 }
 
 template <int N>
-OSH_HD Mat intersect_metrics(Mat const& m1_in, Mat const& m2_in) {
+OSH_HD Mat intersect_metrics(Mat const& m1_in, Mat const& m2_in, int* bad) {
   Mat m1 = m1_in, m2 = m2_in, q1, q2;
   double l1[3] = {0, 0, 0}, l2[3] = {0, 0, 0};
-  decompose_eigen<N>(m1, q1, l1);
-  decompose_eigen<N>(m2, q2, l2);
+  decompose_eigen<N>(m1, q1, l1, bad);
+  decompose_eigen<N>(m2, q2, l2, bad);
   int d1[3] = {0, 0, 0}, d2[3] = {0, 0, 0};
   int n1 = 0, n2 = 0;
   for (int i = 0; i < N; ++i) {
@@ -275,7 +277,7 @@ OSH_HD Mat intersect_metrics(Mat const& m1_in, Mat const& m2_in) {
     Mat m2bar = matmul_rect<N, N, N>(matmul_rect<N, N, N>(transpose_rect<N, N>(is), m2), is);
     Mat p;
     double lb[3] = {0, 0, 0};
-    decompose_eigen<N>(m2bar, p, lb);
+    decompose_eigen<N>(m2bar, p, lb, bad);
     for (int i = 0; i < N; ++i) lb[i] = math::max2(lb[i], 1.0);
     Mat mib = compose_ortho<N>(p, lb);
     return matmul_rect<N, N, N>(matmul_rect<N, N, N>(transpose_rect<N, N>(s), mib), s);

@@ -1,6 +1,8 @@
 // Internal interfaces shared by the adapt() passes.
 #pragma once
 
+#include <functional>
+
 #include "osh_mesh.hpp"
 
 namespace osh {
@@ -129,7 +131,11 @@ struct PassRecord {
   double seconds;
 };
 
-bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log);
+// called with the mesh after every pass that changed it (outside the pass's
+// timing), for callers that record the mesh pass by pass
+using PassHook = std::function<void(DMesh&)>;
+bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log,
+    PassHook const* hook = nullptr);
 // approach_metric (src/Omega_h_approach.cpp:83-126): move the metric toward
 // the "target_metric" vertex tag; false when there is no target
 bool approach_metric(Ctx& c, DMesh& m, AdaptOptions const& o, Real min_step);

@@ -94,6 +94,10 @@ void Ctx::sync() {
     if (code == kScanTicket) throw Error("scan: tile tickets out of step");
     if (code == kKeyWithoutUp) throw Error("modify: a key entity has no adjacent entity above it");
     if (code == kNonManifoldSide) throw Error("a side has more than two elements");
+    if (code == kBrokenSwapLoop)
+      throw Error("swap: the elements around an edge do not close a loop");
+    if (code == kDegenerateEigen)
+      throw Error("eigen: no eigenvector for a simple eigenvalue (degenerate metric)");
     throw Error("device error " + std::to_string(code));
   }
 }
@@ -303,10 +307,11 @@ __global__ void zero_u64(LO n, unsigned long long* a) {
 }
 
 // the context's tile status words, grown (and cleared) when a scan needs
-// more tiles than they hold; a new epoch per scan, with a clearing pass when
-// the 24-bit epoch wraps
+// more tiles than they hold; a new epoch per scan, with a clearing pass
+// before the 24-bit epoch field would overflow (epochs run 1 .. 2^24 - 1;
+// cleared words carry epoch 0)
 unsigned long long* scan_status_words(Ctx& c, LO ntiles, unsigned* epoch) {
-  if (ntiles > c.scan_status_cap || c.scan_epoch >= (1u << 24)) {
+  if (ntiles > c.scan_status_cap || c.scan_epoch >= Ctx::kMaxScanEpoch) {
     LO cap = ntiles > c.scan_status_cap ? (ntiles > 2 * c.scan_status_cap ? ntiles
                                                                           : 2 * c.scan_status_cap)
                                         : c.scan_status_cap;

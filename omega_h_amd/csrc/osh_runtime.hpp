@@ -92,6 +92,8 @@ struct Ctx {
   void* scan_status = nullptr;
   LO scan_status_cap = 0;
   unsigned scan_epoch = 0;
+  // the largest epoch a scan may use: status words keep it in 24 bits
+  static constexpr unsigned kMaxScanEpoch = (1u << 24) - 1;
   unsigned* tile_counter = nullptr;
   unsigned tile_tickets = 0;
   // device copy of the edge-swap tables, uploaded on first use
@@ -181,7 +183,9 @@ enum DeviceError {
   kScanTimeout = 2,
   kScanTicket = 3,
   kKeyWithoutUp = 4,
-  kNonManifoldSide = 5
+  kNonManifoldSide = 5,
+  kBrokenSwapLoop = 6,
+  kDegenerateEigen = 7
 };
 
 template <class T>

@@ -30,6 +30,9 @@ struct Loop {
   LO verts[swap::kMaxLoop];
 };
 
+// a ring of tets that does not close is a broken mesh: the reference aborts
+// (src/Omega_h_swap3d_loop.hpp:71); here the loop is marked broken (size
+// -1) and the caller raises kBrokenSwapLoop
 __device__ void find_loop(LO edge, LO const* a2ab, LO const* ab2b, I8 const* codes,
     LO const* ev2v, LO const* tv2v, Loop& loop) {
   LO begin = a2ab[edge], end = a2ab[edge + 1];
@@ -50,7 +53,10 @@ __device__ void find_loop(LO edge, LO const* a2ab, LO const* ab2b, I8 const* cod
     int j;
     for (j = i + 1; j < loop.size; ++j)
       if (tmp[j][0] == tmp[i][1]) break;
-    if (j >= loop.size) j = i + 1;  // topological error: the reference aborts here
+    if (j >= loop.size) {
+      loop.size = -1;
+      return;
+    }
     LO t0 = tmp[i + 1][0], t1 = tmp[i + 1][1];
     tmp[i + 1][0] = tmp[j][0];
     tmp[i + 1][1] = tmp[j][1];
@@ -106,7 +112,7 @@ __device__ double length_between(LO a, LO b, Real const* coords, Real const* met
 // edge evaluated once), so the choice is the same.
 __global__ void swap3d_loops(LO nt, LO const* cands, LO const* e2t_a2ab, LO const* e2t_ab2b,
     I8 const* e2t_codes, LO const* ev2v, LO const* tv2v, Tables const* tb, I8 const* cached,
-    LO* lsize, LO* leev, LO* lverts, LO* ntris, LO* nedges) {
+    LO* lsize, LO* leev, LO* lverts, LO* ntris, LO* nedges, int* error) {
   // kLoopLanes threads per candidate: lane le reads the le-th tet around
   // the edge; the group then orders the loop as find_loop does
   LO t = thread_index();
@@ -143,7 +149,15 @@ __global__ void swap3d_loops(LO nt, LO const* cands, LO const* e2t_a2ab, LO cons
     int j;
     for (j = i + 1; j < size; ++j)
       if (tmp[j][0] == tmp[i][1]) break;
-    if (j >= size) j = i + 1;  // topological error: the reference aborts here
+    if (j >= size) {
+      // the tets around the edge do not close a ring (the reference aborts,
+      // src/Omega_h_swap3d_loop.hpp:71): no configuration, and an error
+      *error = kBrokenSwapLoop;
+      ntris[cand] = 0;
+      nedges[cand] = 0;
+      lsize[cand] = swap::kMaxLoop + 1;
+      return;
+    }
     LO t0 = tmp[i + 1][0], t1 = tmp[i + 1][1];
     tmp[i + 1][0] = tmp[j][0];
     tmp[i + 1][1] = tmp[j][1];
@@ -505,11 +519,17 @@ __global__ void swap3d_products(LO nt, LO const* keys2edges, I8 const* configs,
 }
 
 __global__ void loops_kernel(LO n, LO const* edges, LO const* a2ab, LO const* ab2b,
-    I8 const* codes, LO const* ev2v, LO const* tv2v, LO* eev2v, LO* verts, LO* sizes) {
+    I8 const* codes, LO const* ev2v, LO const* tv2v, LO* eev2v, LO* verts, LO* sizes,
+    int* error) {
   LO i = thread_index();
   if (i >= n) return;
   Loop loop;
   find_loop(edges[i], a2ab, ab2b, codes, ev2v, tv2v, loop);
+  if (loop.size < 0) {
+    *error = kBrokenSwapLoop;
+    sizes[i] = 0;
+    return;
+  }
   sizes[i] = loop.size;
   if (loop.size > swap::kMaxLoop) return;
   eev2v[i * 2] = loop.eev2v[0];
@@ -714,7 +734,7 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
   OSH_LAUNCH(swap3d_loops, ncands * kLoopLanes, c, cands.data(), e2t.a2ab.data(), e2t.ab2b.data(),
       e2t.codes.data(), m.down[1].data(), tv2v.data(), tb,
       cache ? cache->valid.data() : (I8 const*)nullptr, lsize.data(), leev.data(),
-      lverts.data(), ntris.data(), nedges.data());
+      lverts.data(), ntris.data(), nedges.data(), c.error_word());
   // the unique triangles and edges of all loops at fixed strides per
   // candidate, so that neither a scan nor a sync sizes them
   LO ntp = ncands * swap::kMaxUniqueTris, nep = ncands * swap::kMaxUniqueEdges;
@@ -848,7 +868,7 @@ void find_loops(Ctx& c, DMesh& m, LOs const& edges, LOs& eev2v, LOs& loop_verts,
   LOs const& tv2v = verts_of(c, m, 3);
   OSH_LAUNCH(loops_kernel, n, c, edges.data(), e2t.a2ab.data(), e2t.ab2b.data(),
       e2t.codes.data(), m.down[1].data(), tv2v.data(), eev2v.data(), loop_verts.data(),
-      sizes.data());
+      sizes.data(), c.error_word());
 }
 
 }  // namespace osh

@@ -159,6 +159,20 @@ static signed char* mark_up(om_mesh* m, int low, int high,
   return out;
 }
 
+void om_kat_mark_down(om_mesh* m, int high, int low, signed char const* marks,
+    signed char* out) {
+  signed char* r = mark_down(m, high, low, marks);
+  memcpy(out, r, (size_t)m->nents[low]);
+  free(r);
+}
+
+void om_kat_mark_up(om_mesh* m, int low, int high, signed char const* marks,
+    signed char* out) {
+  signed char* r = mark_up(m, low, high, marks);
+  memcpy(out, r, (size_t)m->nents[high]);
+  free(r);
+}
+
 static signed char* mark_sliver_layers(om_mesh* m, double qual_ceil, int nlayers) {
   int dim = m->dim;
   int ne = m->nents[dim];
@@ -1505,6 +1519,38 @@ static int swap_edges_3d(om_mesh* m, om_adapt_opts const* o) {
 }
 
 /* ------------------------------------------------------------ swap 2D */
+/* swap2d_topology, src/Omega_h_swap2d_topology.cpp:62-79: per key edge one
+ * new edge (the quadrilateral's other diagonal, its vertices in the order
+ * of the rotations of the edge's triangles) and two new triangles */
+static void swap2d_products(om_mesh* m, int nkeys, int const* keys2edges, int* edge_verts,
+    int* tri_verts) {
+  om_graph const* e2t = om_ask_up(m, 1, 2);
+  int const* tv2v = om_verts_of(m, 2);
+  int const* ev2v = m->down[1];
+  for (int k = 0; k < nkeys; ++k) {
+    int e = keys2edges[k];
+    int* ov = edge_verts + k * 2;
+    for (int i = 0; i < 2; ++i) {
+      int et = e2t->a2ab[e] + i;
+      int tte = om_code_which_down(e2t->codes[et]);
+      int rot = om_code_rotation(e2t->codes[et]);
+      int t = e2t->ab2b[et];
+      ov[rot] = tv2v[t * 3 + om_opposite_template(2, 1, tte)];
+    }
+    for (int i = 0; i < 2; ++i) {
+      int* ntv = tri_verts + (k * 2 + i) * 3;
+      ntv[0] = ev2v[e * 2 + (1 - i)];
+      ntv[1] = ov[i];
+      ntv[2] = ov[1 - i];
+    }
+  }
+}
+
+void om_kat_swap2d_topology(om_mesh* m, int nkeys, int const* keys2edges, int* edge_verts,
+    int* tri_verts) {
+  swap2d_products(m, nkeys, keys2edges, edge_verts, tri_verts);
+}
+
 static int swap_edges_2d(om_mesh* m, om_adapt_opts const* o) {
   om_ask_lengths(m);
   signed char* cand = swap_candidates(m, o);
@@ -1577,23 +1623,7 @@ static int swap_edges_2d(om_mesh* m, om_adapt_opts const* o) {
   }
   pv[1] = (int*)om_malloc(sizeof(int) * (size_t)(nkeys + 1) * 2);
   pv[2] = (int*)om_malloc(sizeof(int) * (size_t)(nkeys + 1) * 6);
-  for (int k = 0; k < nkeys; ++k) {
-    int e = keys2edges[k];
-    int* ov = pv[1] + k * 2;
-    for (int i = 0; i < 2; ++i) {
-      int et = e2t->a2ab[e] + i;
-      int tte = om_code_which_down(e2t->codes[et]);
-      int rot = om_code_rotation(e2t->codes[et]);
-      int t = e2t->ab2b[et];
-      ov[rot] = tv2v[t * 3 + om_opposite_template(2, 1, tte)];
-    }
-    for (int i = 0; i < 2; ++i) {
-      int* ntv = pv[2] + (k * 2 + i) * 3;
-      ntv[0] = ev2v[e * 2 + (1 - i)];
-      ntv[1] = ov[i];
-      ntv[2] = ov[1 - i];
-    }
-  }
+  swap2d_products(m, nkeys, keys2edges, pv[1], pv[2]);
   swap_rebuild(m, nkeys, keys2edges, k2p, pv);
   for (int d = 0; d < 4; ++d) {
     free(k2p[d]);
@@ -1631,8 +1661,38 @@ static void record(om_adapt_stats* s, int kind, int nin, om_mesh* m, double t) {
   s->npasses++;
 }
 
+/* FNV-1a (64-bit) of the mesh after a pass, four digests:
+ *   0: vertex lists of every dimension 1..dim (int32), then class_id (int32)
+ *      and class_dim (int8) of every dimension 0..dim
+ *   1: coordinate bits, 2: metric bits,
+ *   3: edge length bits, then element quality bits */
+static uint64_t fnv(uint64_t h, void const* p, size_t n) {
+  unsigned char const* b = (unsigned char const*)p;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= b[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+void om_mesh_digests(om_mesh* m, uint64_t* out) {
+  uint64_t const basis = 1469598103934665603ull;
+  uint64_t h = basis;
+  for (int d = 1; d <= m->dim; ++d)
+    h = fnv(h, om_verts_of(m, d), sizeof(int) * (size_t)m->nents[d] * (size_t)(d + 1));
+  for (int d = 0; d <= m->dim; ++d) {
+    h = fnv(h, m->class_id[d], sizeof(int) * (size_t)m->nents[d]);
+    h = fnv(h, m->class_dim[d], (size_t)m->nents[d]);
+  }
+  out[0] = h;
+  out[1] = fnv(basis, m->coords, sizeof(double) * (size_t)m->nents[0] * (size_t)m->dim);
+  out[2] = fnv(basis, m->metric, sizeof(double) * (size_t)m->nents[0] * (size_t)m->metric_ncomps);
+  h = fnv(basis, om_ask_lengths(m), sizeof(double) * (size_t)m->nents[1]);
+  out[3] = fnv(h, om_ask_qualities(m), sizeof(double) * (size_t)m->nents[m->dim]);
+}
+
 static int run_pass(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* s,
-    int kind) {
+    int kind, uint64_t* digests) {
   if (s && s->npasses >= OM_MAX_PASSES) {
     fprintf(stderr, "oracle: adapt exceeded %d passes\n", OM_MAX_PASSES);
     abort();
@@ -1649,11 +1709,16 @@ static int run_pass(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* s,
   else
     r = om_coarsen_slivers(m, o);
   double t1 = now_seconds();
-  if (r) record(s, kind, nin, m, t1 - t0);
+  if (r) {
+    if (digests && s && s->npasses < OM_MAX_PASSES)
+      om_mesh_digests(m, digests + 4 * (size_t)s->npasses);
+    record(s, kind, nin, m, t1 - t0);
+  }
   return r;
 }
 
-int om_adapt(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* s) {
+static int adapt_impl(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* s,
+    uint64_t* digests) {
   if (s) s->npasses = 0;
   /* pre_adapt: nothing to do if quality and lengths already satisfy goals */
   double const* l = om_ask_lengths(m);
@@ -1669,15 +1734,24 @@ int om_adapt(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* s) {
   int did;
   do {
     did = 0;
-    if (o->should_refine && run_pass(m, o, s, 0)) did = 1;
-    if (o->should_coarsen && run_pass(m, o, s, 1)) did = 1;
+    if (o->should_refine && run_pass(m, o, s, 0, digests)) did = 1;
+    if (o->should_coarsen && run_pass(m, o, s, 1, digests)) did = 1;
   } while (did);
   /* satisfy_quality */
   if (min_quality(m) >= o->min_quality_desired) return 1;
   do {
-    if (o->should_swap && run_pass(m, o, s, 2)) continue;
-    if (o->should_coarsen_slivers && run_pass(m, o, s, 3)) continue;
+    if (o->should_swap && run_pass(m, o, s, 2, digests)) continue;
+    if (o->should_coarsen_slivers && run_pass(m, o, s, 3, digests)) continue;
     return 1;
   } while (min_quality(m) < o->min_quality_desired);
   return 1;
+}
+
+int om_adapt(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* s) {
+  return adapt_impl(m, o, s, NULL);
+}
+
+int om_adapt_digests(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* s,
+    uint64_t* digests) {
+  return adapt_impl(m, o, s, digests);
 }

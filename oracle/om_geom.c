@@ -16,6 +16,8 @@
  *   metric_edge_length   src/Omega_h_shape.hpp:112-125
  */
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "om_internal.h"
@@ -238,6 +240,15 @@ static void ortho_basis3(double const* v, om_mat* a) {
   a->m[2][2] = -v[1];
 }
 
+/* the reference's OMEGA_H_CHECKs in single_eigenvector and
+ * get_1d_row_space (src/Omega_h_eigen.hpp:191,211) abort */
+static void eigen_check(int status) {
+  if (status != 0) {
+    fprintf(stderr, "oracle: no eigenvector for an eigenvalue\n");
+    abort();
+  }
+}
+
 static void decompose_dim(int dim, om_mat const* m, om_mat* q, double* l) {
   mat_zero(q);
   if (dim == 3) {
@@ -253,16 +264,16 @@ static void decompose_dim(int dim, om_mat const* m, om_mat* q, double* l) {
     int nroots = om_roots3(coeffs, 5e-5, roots, mults);
     if (nroots == 3) {
       for (int i = 0; i < 3; ++i) {
-        single_eigenvector3(m, roots[i], q->m[i]);
+        eigen_check(single_eigenvector3(m, roots[i], q->m[i]));
         l[i] = roots[i];
       }
     } else if (nroots == 2 && mults[1] == 2) {
-      single_eigenvector3(m, roots[0], q->m[0]);
+      eigen_check(single_eigenvector3(m, roots[0], q->m[0]));
       l[0] = roots[0];
       om_mat s = *m;
       subtract_from_diag(3, &s, roots[1]);
       double n[3] = {0.0, 0.0, 0.0};
-      row_space_1d(3, &s, n);
+      eigen_check(row_space_1d(3, &s, n));
       om_mat b;
       ortho_basis3(n, &b);
       memcpy(q->m[1], b.m[1], 3 * sizeof(double));
@@ -284,7 +295,7 @@ static void decompose_dim(int dim, om_mat const* m, om_mat* q, double* l) {
         om_mat s = *m;
         subtract_from_diag(2, &s, roots[i]);
         double r[2] = {0.0, 0.0};
-        row_space_1d(2, &s, r);
+        eigen_check(row_space_1d(2, &s, r));
         q->m[i][0] = -r[1];
         q->m[i][1] = r[0];
         l[i] = roots[i];

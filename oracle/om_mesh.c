@@ -347,21 +347,65 @@ static void find_matches(int deg, int na, int const* av2v, int const* bv2v,
   }
 }
 
+/* form_uses, src/Omega_h_adj.cpp:155-176: the vertex lists of the
+ * low_dim-dimensional sides of each high entity, in template order */
+static int* form_uses(int nhigh, int high_dim, int low_dim, int const* hv2v) {
+  int nvh = high_dim + 1;
+  int nlh = om_simplex_degree(high_dim, low_dim);
+  int nvl = low_dim + 1;
+  int* uv2v = (int*)om_malloc(sizeof(int) * ((size_t)nhigh * nlh * nvl + 1));
+  for (int h = 0; h < nhigh; ++h)
+    for (int u = 0; u < nlh; ++u)
+      for (int uv = 0; uv < nvl; ++uv)
+        uv2v[(h * nlh + u) * nvl + uv] =
+            hv2v[h * nvh + om_down_template(high_dim, low_dim, u, uv)];
+  return uv2v;
+}
+
+void om_kat_form_uses(int nhigh, int high_dim, int low_dim, int const* hv2v, int* out) {
+  int* u = form_uses(nhigh, high_dim, low_dim, hv2v);
+  size_t n = (size_t)nhigh * om_simplex_degree(high_dim, low_dim) * (low_dim + 1);
+  memcpy(out, u, sizeof(int) * n);
+  free(u);
+}
+
+/* reflect_down(hv2v, lv2v, nverts, high_dim, low_dim),
+ * src/Omega_h_adj.cpp:424-441: the low entities (given by vertex lists)
+ * on each high entity's sides, with alignment codes */
+void om_kat_reflect_down(int high_dim, int low_dim, int nhigh, int const* hv2v, int nlow,
+    int const* lv2v, int nverts, int* ab2b, signed char* codes) {
+  int nvl = low_dim + 1;
+  int nuses = nhigh * om_simplex_degree(high_dim, low_dim);
+  int* uv2v = form_uses(nhigh, high_dim, low_dim, hv2v);
+  om_graph* v2l = invert_adj(nlow, nvl, nverts, lv2v, NULL);
+  find_matches(nvl, nuses, uv2v, lv2v, v2l, ab2b, codes);
+  om_graph_free(v2l);
+  free(uv2v);
+}
+
+/* invert_map_by_atomics / invert_map_by_sorting (src/Omega_h_map.cpp:238-280):
+ * rows of the inverse sorted by the high index */
+void om_kat_invert_map(int n, int const* hl2l, int nlow, int* a2ab, int* ab2b) {
+  om_graph* g = invert_adj(n, 1, nlow, hl2l, NULL);
+  memcpy(a2ab, g->a2ab, sizeof(int) * (size_t)(nlow + 1));
+  memcpy(ab2b, g->ab2b, sizeof(int) * (size_t)n);
+  om_graph_free(g);
+}
+
+/* index alignment of src/Omega_h_align.hpp:65-85 */
+int om_kat_align_vert_index(int n, int index, int code) {
+  return align_vert_index(n, index, (signed char)code);
+}
+
 /* reflect_down of new entities of dimension high_dim (>= 2) given by their
  * vertex lists, against the already-built lower dimension of m_new. */
 void om_reflect_down(om_mesh* m_new, int high_dim, int nprods,
     int const* prod_verts, int* out_down, signed char* out_codes) {
   int low_dim = high_dim - 1;
-  int nvh = high_dim + 1;
   int nlh = high_dim + 1;
   int nvl = low_dim + 1;
   int nuses = nprods * nlh;
-  int* uv2v = (int*)om_malloc(sizeof(int) * (size_t)nuses * nvl);
-  for (int h = 0; h < nprods; ++h)
-    for (int u = 0; u < nlh; ++u)
-      for (int uv = 0; uv < nvl; ++uv)
-        uv2v[(h * nlh + u) * nvl + uv] =
-            prod_verts[h * nvh + om_down_template(high_dim, low_dim, u, uv)];
+  int* uv2v = form_uses(nprods, high_dim, low_dim, prod_verts);
   int const* lv2v = om_verts_of(m_new, low_dim);
   om_graph const* v2l = om_ask_up(m_new, 0, low_dim);
   find_matches(nvl, nuses, uv2v, lv2v, v2l, out_down, out_codes);

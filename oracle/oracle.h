@@ -68,6 +68,13 @@ int om_coarsen_by_size(om_mesh* m, om_adapt_opts const* o);
 int om_swap_edges(om_mesh* m, om_adapt_opts const* o);
 int om_coarsen_slivers(om_mesh* m, om_adapt_opts const* o);
 int om_adapt(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* stats);
+/* om_adapt that also records four FNV-1a digests of the mesh after every
+ * pass (digests holds 4 * OM_MAX_PASSES values): 0 vertex lists of every
+ * dimension and classification, 1 coordinate bits, 2 metric bits, 3 edge
+ * length and element quality bits */
+int om_adapt_digests(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* stats,
+    uint64_t* digests);
+void om_mesh_digests(om_mesh* m, uint64_t* out);
 
 /* known-answer hooks for the reference's unit tests */
 double om_log(double x);
@@ -123,6 +130,18 @@ void om_kat_refine_qualities(om_mesh* m, int n, int const* cands, double* out);
 int om_kat_find_loop(om_mesh* m, int edge, int* eev2v, int* loop_verts);
 void om_kat_indset(om_mesh* m, int d, double const* q, signed char const* c,
     signed char* out);
+/* topology known answers (src/unit_mesh.cpp, src/unit_array_algs.cpp) */
+void om_kat_form_uses(int nhigh, int high_dim, int low_dim, int const* hv2v, int* out);
+void om_kat_reflect_down(int high_dim, int low_dim, int nhigh, int const* hv2v, int nlow,
+    int const* lv2v, int nverts, int* ab2b, signed char* codes);
+void om_kat_invert_map(int n, int const* hl2l, int nlow, int* a2ab, int* ab2b);
+int om_kat_align_vert_index(int n, int index, int code);
+void om_kat_mark_down(om_mesh* m, int high, int low, signed char const* marks,
+    signed char* out);
+void om_kat_mark_up(om_mesh* m, int low, int high, signed char const* marks,
+    signed char* out);
+void om_kat_swap2d_topology(om_mesh* m, int nkeys, int const* keys2edges, int* edge_verts,
+    int* tri_verts);
 
 #ifdef __cplusplus
 }

@@ -88,6 +88,26 @@ def lib():
                      "om_coarsen_slivers"):
             getattr(L, name).argtypes = [P, C.POINTER(AdaptOpts)]
         L.om_adapt.argtypes = [P, C.POINTER(AdaptOpts), C.POINTER(AdaptStats)]
+        L.om_adapt_digests.argtypes = [P, C.POINTER(AdaptOpts), C.POINTER(AdaptStats),
+                                       C.POINTER(C.c_uint64)]
+        L.om_mesh_digests.argtypes = [P, C.POINTER(C.c_uint64)]
+        IPt, BPt = C.POINTER(C.c_int), C.POINTER(C.c_byte)
+        L.om_down_template.argtypes = [C.c_int] * 4
+        L.om_opposite_template.argtypes = [C.c_int] * 3
+        L.om_kat_form_uses.argtypes = [C.c_int, C.c_int, C.c_int, IPt, IPt]
+        L.om_kat_reflect_down.argtypes = [C.c_int, C.c_int, C.c_int, IPt, C.c_int, IPt,
+                                          C.c_int, IPt, BPt]
+        L.om_kat_invert_map.argtypes = [C.c_int, IPt, C.c_int, IPt, IPt]
+        L.om_kat_align_vert_index.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.om_kat_mark_down.argtypes = [P, C.c_int, C.c_int, BPt, BPt]
+        L.om_kat_mark_up.argtypes = [P, C.c_int, C.c_int, BPt, BPt]
+        L.om_kat_swap2d_topology.argtypes = [P, C.c_int, IPt, IPt, IPt]
+        L.om_swap_triangle.restype = IPt
+        L.om_swap_triangle.argtypes = [C.c_int, C.c_int]
+        L.om_swap_unique_edge.restype = IPt
+        L.om_swap_unique_edge.argtypes = [C.c_int, C.c_int]
+        L.om_swap_mesh_tri.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.om_swap_mesh_edge.argtypes = [C.c_int, C.c_int, C.c_int]
         L.om_limit_metric_gradation.argtypes = [P, C.c_double, C.c_double]
         L.om_add_vtag.argtypes = [P, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.om_vtag_ncomps.argtypes = [P, C.c_char_p]
@@ -268,6 +288,26 @@ class Mesh:
         n = lib().om_kat_find_loop(self.h, edge, _ptr(eev, C.c_int), _ptr(lv, C.c_int))
         return eev, lv[:n]
 
+    def mark_down(self, high, low, marks):
+        mk = np.ascontiguousarray(marks, dtype=np.int8)
+        out = np.zeros(self.nents(low), dtype=np.int8)
+        lib().om_kat_mark_down(self.h, high, low, _ptr(mk, C.c_byte), _ptr(out, C.c_byte))
+        return out
+
+    def mark_up(self, low, high, marks):
+        mk = np.ascontiguousarray(marks, dtype=np.int8)
+        out = np.zeros(self.nents(high), dtype=np.int8)
+        lib().om_kat_mark_up(self.h, low, high, _ptr(mk, C.c_byte), _ptr(out, C.c_byte))
+        return out
+
+    def swap2d_topology(self, keys2edges):
+        k = _ints(keys2edges)
+        ev = np.zeros(2 * len(k), dtype=np.int32)
+        tv = np.zeros(6 * len(k), dtype=np.int32)
+        lib().om_kat_swap2d_topology(self.h, len(k), _ptr(k, C.c_int), _ptr(ev, C.c_int),
+                                     _ptr(tv, C.c_int))
+        return ev, tv
+
     def indset(self, d, quality, cands):
         q = np.ascontiguousarray(quality, dtype=np.float64)
         c = np.ascontiguousarray(cands, dtype=np.int8)
@@ -368,12 +408,91 @@ class Mesh:
         lib().om_get_implied_metrics(self.h, _ptr(out, C.c_double))
         return out
 
+    def adapt_digests(self, opts):
+        """adapt() with four digests of the mesh after every pass (topology and
+        classification, coordinates, metric, lengths and qualities)."""
+        st = AdaptStats()
+        dg = (C.c_uint64 * (4 * MAX_PASSES))()
+        r = lib().om_adapt_digests(self.h, C.byref(opts), C.byref(st), dg)
+        passes = [(st.kind[i], st.nelems_in[i], st.nelems_out[i], st.seconds[i])
+                  for i in range(st.npasses)]
+        digests = [tuple(dg[4 * i + k] for k in range(4)) for i in range(st.npasses)]
+        return r, passes, digests
+
+    def digests(self):
+        dg = (C.c_uint64 * 4)()
+        lib().om_mesh_digests(self.h, dg)
+        return tuple(dg)
+
     def adapt(self, opts):
         st = AdaptStats()
         r = lib().om_adapt(self.h, C.byref(opts), C.byref(st))
         passes = [(st.kind[i], st.nelems_in[i], st.nelems_out[i], st.seconds[i])
                   for i in range(st.npasses)]
         return r, passes
+
+
+def _ints(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def form_uses(high_dim, low_dim, hv2v):
+    """form_uses(hv2v, SIMPLEX, high_dim, low_dim), src/Omega_h_adj.cpp:155-176."""
+    h = _ints(hv2v)
+    nhigh = len(h) // (high_dim + 1)
+    deg = lib().om_simplex_degree(high_dim, low_dim)
+    out = np.zeros(max(nhigh * deg * (low_dim + 1), 1), dtype=np.int32)
+    lib().om_kat_form_uses(nhigh, high_dim, low_dim, _ptr(h, C.c_int), _ptr(out, C.c_int))
+    return out[:nhigh * deg * (low_dim + 1)]
+
+
+def reflect_down(hv2v, lv2v, nverts, high_dim, low_dim):
+    """reflect_down(hv2v, lv2v, SIMPLEX, nverts, high_dim, low_dim),
+    src/Omega_h_adj.cpp:424-441: (ab2b, codes)."""
+    h, lo = _ints(hv2v), _ints(lv2v)
+    nhigh = len(h) // (high_dim + 1)
+    nlow = len(lo) // (low_dim + 1)
+    deg = lib().om_simplex_degree(high_dim, low_dim)
+    ab2b = np.zeros(max(nhigh * deg, 1), dtype=np.int32)
+    codes = np.zeros(max(nhigh * deg, 1), dtype=np.int8)
+    hb = h if len(h) else np.zeros(1, dtype=np.int32)
+    lb = lo if len(lo) else np.zeros(1, dtype=np.int32)
+    lib().om_kat_reflect_down(high_dim, low_dim, nhigh, _ptr(hb, C.c_int), nlow,
+                              _ptr(lb, C.c_int), nverts, _ptr(ab2b, C.c_int),
+                              _ptr(codes, C.c_byte))
+    return ab2b[:nhigh * deg], codes[:nhigh * deg]
+
+
+def invert_map(hl2l, nlow):
+    """invert_map_by_atomics / _by_sorting, src/Omega_h_map.cpp:238-280."""
+    h = _ints(hl2l)
+    a2ab = np.zeros(nlow + 1, dtype=np.int32)
+    ab2b = np.zeros(max(len(h), 1), dtype=np.int32)
+    hb = h if len(h) else np.zeros(1, dtype=np.int32)
+    lib().om_kat_invert_map(len(h), _ptr(hb, C.c_int), nlow, _ptr(a2ab, C.c_int),
+                            _ptr(ab2b, C.c_int))
+    return a2ab, ab2b[:len(h)]
+
+
+def align_vert_index(n, index, code):
+    return lib().om_kat_align_vert_index(n, index, code)
+
+
+def make_code(flipped, rotation, which_down):
+    """make_code, src/Omega_h_align.hpp:50-53."""
+    return (which_down << 3) | (rotation << 1) | int(bool(flipped))
+
+
+def swap3d_tables_of_oracle(n):
+    L = lib()
+    ntris = {3: 1, 4: 4, 5: 10, 6: 20, 7: 35}[n]
+    nue = {3: 0, 4: 2, 5: 7, 6: 15, 7: 26}[n]
+    nm = (C.c_int * 8).in_dll(L, "om_swap_mesh_counts")[n]
+    tris = [[L.om_swap_triangle(n, t)[k] for k in range(3)] for t in range(ntris)]
+    meshes = [[L.om_swap_mesh_tri(n, m, k) for k in range(n - 2)] for m in range(nm)]
+    ue = [[L.om_swap_unique_edge(n, e)[k] for k in range(2)] for e in range(nue)]
+    me = [[L.om_swap_mesh_edge(n, m, k) for k in range(n - 3)] for m in range(nm)]
+    return tris, meshes, ue, me
 
 
 def kat_quality(dim, mdim, pts, ms):

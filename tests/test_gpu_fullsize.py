@@ -2,15 +2,20 @@
 
 The oracle runs the first two mesh-changing passes of adapt() on the same input
 (about 20 s of host time) and the device mesh must match it bit for bit.  The
-whole adapt() is then checked through properties that do not need the oracle:
-a second run on a fresh copy gives the same mesh digest and pass sequence, every
-element keeps a positive quality, the Euler characteristic of the ball is 1, every
-face has one or two tets and a face with one tet is classified on the boundary.
-The final counts are pinned as regression values of this implementation.
+whole adapt() (226 passes) is compared with the oracle's own whole run, pass by
+pass, through the digests the oracle recorded in
+tests/golden/adapt_levelset_3d_64.json (tests/golden/make_adapt_digests.py):
+every pass's kind and element counts, and the vertex lists, classification,
+coordinate, metric, length and quality bits after it.  Properties that need no
+oracle are checked too: a second run on a fresh copy gives the same mesh and
+pass sequence, every element keeps a positive quality, the Euler characteristic
+of the ball is 1, every face has one or two tets and a face with one tet is
+classified on the boundary.
 """
 import numpy as np
 import pytest
 
+import adapt_fixtures as F
 import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
@@ -50,7 +55,20 @@ def test_first_passes_match_the_oracle(ctx):
     compare_meshes(o, g)
 
 
+def test_full_adapt_matches_oracle_pass_by_pass(ctx):
+    fx = F.load("levelset_3d_64")
+    m = levelset_box(A, ctx)
+    opts = ctx.default_opts(3)
+    opts.verbosity = 0
+    did, passes, digests = m.adapt_digests(opts)
+    assert did == fx["adapted"]
+    msg = F.first_difference(fx["passes"], F.rows_of(passes, digests))
+    assert msg is None, msg
+    assert [m.nents(d) for d in range(4)] == fx["final_nents"]
+
+
 def test_full_adapt_properties(ctx):
+    fx = F.load("levelset_3d_64")
     base = levelset_box(A, ctx)
     opts = ctx.default_opts(3)
     runs = []
@@ -61,11 +79,11 @@ def test_full_adapt_properties(ctx):
         runs.append((m.digest(), [p[:3] for p in passes], m))
     assert runs[0][0] == runs[1][0]
     assert runs[0][1] == runs[1][1]
+    # osh_adapt takes the same passes as the oracle's run
+    assert [list(p) for p in runs[0][1]] == [r[:3] for r in fx["passes"]]
     m = runs[0][2]
     nt = m.nents(3)
-    # regression values of this implementation on this input
-    assert len(runs[0][1]) == 226
-    assert nt == 163062
+    assert nt == fx["final_nents"][3]
     assert m.qualities().min() > 0.0
     assert m.nents(0) - m.nents(1) + m.nents(2) - nt == 1
     a2ab, _, _ = m.up(2, 3)
