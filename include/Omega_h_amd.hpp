@@ -29,8 +29,18 @@
 
 // C enums of src/Omega_h_defines.h, at global scope as there
 enum Omega_h_Family { OMEGA_H_SIMPLEX = 0 };
-// transfer types of tags carried through adapt() (src/Omega_h_adapt.hpp:30-42)
-enum Omega_h_Transfer { OMEGA_H_LINEAR_INTERP = 0, OMEGA_H_METRIC = 1, OMEGA_H_POINTWISE = 2 };
+// transfer types of tags carried through adapt() (src/Omega_h_defines.hpp:29-37,
+// same order); OMEGA_H_DENSITY, OMEGA_H_CONSERVE and OMEGA_H_MOMENTUM_VELOCITY
+// are not built here and adapt() throws when a tag asks for them
+enum Omega_h_Transfer {
+  OMEGA_H_INHERIT,
+  OMEGA_H_LINEAR_INTERP,
+  OMEGA_H_METRIC,
+  OMEGA_H_DENSITY,
+  OMEGA_H_CONSERVE,
+  OMEGA_H_MOMENTUM_VELOCITY,
+  OMEGA_H_POINTWISE,
+};
 // how a metric is made isotropic
 enum Omega_h_Isotropy { OMEGA_H_ANISOTROPIC = 0, OMEGA_H_ISO_LENGTH = 1, OMEGA_H_ISO_SIZE = 2 };
 // how a distributed mesh is partitioned; one rank holds the whole mesh, so
@@ -368,7 +378,7 @@ inline void set_metric(Mesh& m, Int ent_dim, std::string const& name, Int ncomps
   }
   bool is_metric = name.size() >= 6 && name.compare(name.size() - 6, 6, "metric") == 0;
   check_status(osh_mesh_add_vertex_tag(m.handle(), name.c_str(), ncomps,
-      is_metric ? OMEGA_H_METRIC : OMEGA_H_LINEAR_INTERP, data));
+      is_metric ? 1 : 0, data));
 }
 }  // namespace detail
 
@@ -450,9 +460,22 @@ inline Mesh build_box(CommPtr comm, Omega_h_Family family, Real x, Real y, Real 
   return Mesh(lib, h);
 }
 
-// src/Omega_h_adapt.hpp:30-42: the transfer type of each caller vertex tag
+// src/Omega_h_compare.hpp:14-21
+struct VarCompareOpts {
+  enum { NONE, RELATIVE, ABSOLUTE } type = RELATIVE;
+  Real tolerance = 1e-6;
+  Real floor = 0.0;
+};
+
+// src/Omega_h_adapt.hpp:30-42: the transfer type of each caller tag, and the
+// maps of conservative and momentum transfer (which are not built here:
+// adapt() throws when any of them is filled)
 struct TransferOpts {
-  std::map<std::string, Omega_h_Transfer> type_map;
+  std::map<std::string, Omega_h_Transfer> type_map;              // "density" -> CONSERVE
+  std::map<std::string, std::string> integral_map;               // "density" -> "mass"
+  std::map<std::string, std::string> velocity_density_map;       // "velocity" -> "density"
+  std::map<std::string, std::string> velocity_momentum_map;      // "velocity" -> "momentum"
+  std::map<std::string, VarCompareOpts> integral_diffuse_map;    // "mass" -> tolerance
 };
 
 // src/Omega_h_adapt.hpp:50-77, defaults src/Omega_h_adapt.cpp:52-85
@@ -475,6 +498,14 @@ struct AdaptOpts {
   bool should_coarsen = true;
   bool should_swap = true;
   bool should_coarsen_slivers = true;
+  Real length_histogram_min = 0.0;
+  Real length_histogram_max = 3.0;
+  Int nlength_histogram_bins = 10;
+  Int nquality_histogram_bins = 10;
+  // not built here: adapt() with this set throws (src/Omega_h_coarsen_flip.cpp)
+  bool should_prevent_coarsen_flip = false;
+  // no counterpart in the reference: OSH_FP64 or OSH_MIXED cavity screens
+  Int precision = OSH_FP64;
   TransferOpts xfer_opts;
 
   osh_adapt_opts to_c() const {
@@ -490,6 +521,12 @@ struct AdaptOpts {
     o.should_coarsen = should_coarsen;
     o.should_swap = should_swap;
     o.should_coarsen_slivers = should_coarsen_slivers;
+    o.length_histogram_min = length_histogram_min;
+    o.length_histogram_max = length_histogram_max;
+    o.nlength_histogram_bins = nlength_histogram_bins;
+    o.nquality_histogram_bins = nquality_histogram_bins;
+    o.should_prevent_coarsen_flip = should_prevent_coarsen_flip;
+    o.precision = precision;
     return o;
   }
   void from_c(osh_adapt_opts const& o) {
@@ -504,20 +541,34 @@ struct AdaptOpts {
     should_coarsen = o.should_coarsen != 0;
     should_swap = o.should_swap != 0;
     should_coarsen_slivers = o.should_coarsen_slivers != 0;
+    length_histogram_min = o.length_histogram_min;
+    length_histogram_max = o.length_histogram_max;
+    nlength_histogram_bins = o.nlength_histogram_bins;
+    nquality_histogram_bins = o.nquality_histogram_bins;
+    should_prevent_coarsen_flip = o.should_prevent_coarsen_flip != 0;
+    precision = o.precision;
   }
 };
 
 namespace detail {
 // re-register a vertex tag whose transfer type the options name differently
 inline void apply_transfer_types(Mesh* mesh, AdaptOpts const& opts) {
-  for (auto const& kv : opts.xfer_opts.type_map) {
+  auto const& x = opts.xfer_opts;
+  if (!x.integral_map.empty() || !x.velocity_density_map.empty() ||
+      !x.velocity_momentum_map.empty() || !x.integral_diffuse_map.empty())
+    throw exception("conservative and momentum transfer are not supported");
+  for (auto const& kv : x.type_map) {
     if (kv.second == OMEGA_H_POINTWISE) continue;  // element tags are pointwise already
+    if (kv.second == OMEGA_H_INHERIT) continue;    // inheritance tags are inherited already
+    if (kv.second != OMEGA_H_LINEAR_INTERP && kv.second != OMEGA_H_METRIC)
+      throw exception("transfer type of \"" + kv.first + "\" is not supported");
     int nc = mesh->vertex_tag_ncomps(kv.first);
     if (nc < 0) continue;
     Write<Real> w(mesh->nverts() * nc);
     check_status(osh_mesh_get_vertex_tag(mesh->handle(), kv.first.c_str(), w.data()));
-    check_status(osh_mesh_add_vertex_tag(mesh->handle(), kv.first.c_str(), nc, kv.second,
-        w.data()));
+    // the C ABI's vertex tag kinds: 0 linear interpolation, 1 metric
+    check_status(osh_mesh_add_vertex_tag(mesh->handle(), kv.first.c_str(), nc,
+        kv.second == OMEGA_H_METRIC ? 1 : 0, w.data()));
   }
 }
 }  // namespace detail

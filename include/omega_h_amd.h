@@ -33,7 +33,24 @@ typedef struct {
   int should_coarsen;
   int should_swap;
   int should_coarsen_slivers;
+  /* the histograms printed at verbosity EXTRA_STATS (3),
+   * print_adapt_histograms src/Omega_h_adapt.cpp:122-138 */
+  double length_histogram_min;
+  double length_histogram_max;
+  int nlength_histogram_bins;
+  int nquality_histogram_bins;
+  /* prevent_coarsen_flip (src/Omega_h_coarsen_flip.cpp:68-150) is not
+   * built: a pass or adapt() with this set returns a nonzero status */
+  int should_prevent_coarsen_flip;
+  /* arithmetic of the cavity quality and length screens (no counterpart in
+   * the reference): OSH_FP64, or OSH_MIXED, which screens cavities in FP32
+   * and re-decides in FP64 every cavity not rejected by a margin, so the
+   * adapted mesh is the same bit for bit */
+  int precision;
 } osh_adapt_opts;
+
+#define OSH_FP64 0
+#define OSH_MIXED 1
 
 #define OSH_MAX_PASSES 4096
 /* per-pass record of an adapt() run: kind 0 refine, 1 coarsen, 2 swap,
@@ -72,6 +89,20 @@ int osh_build_from_elems(osh_ctx* ctx, int dim, int nelems, const int* ev2v,
     int nverts, osh_mesh** out);
 int osh_mesh_destroy(osh_mesh* mesh);
 int osh_mesh_copy(osh_mesh* mesh, osh_mesh** out);
+
+/* The batched form of adapt() (no counterpart in the reference, whose
+ * adapt() takes one mesh): osh_mesh_join makes one disjoint mesh of nparts
+ * meshes of the context (same dimension and metric layout, classified, no
+ * inheritance or element tags), part k's entities of every dimension
+ * numbered after those of parts < k.  Every pass and osh_adapt on the joined
+ * mesh does to each part what it does to that part alone (all decisions are
+ * cavity-local and new entities are numbered in old order), with one kernel
+ * launch for all parts.  osh_mesh_split gives back nparts new meshes (the
+ * joined one stays valid); osh_mesh_part_counts writes nparts * (dim + 1)
+ * entity counts.  Both fail unless every part is still one block. */
+int osh_mesh_join(osh_ctx* ctx, int nparts, osh_mesh* const* parts, osh_mesh** out);
+int osh_mesh_split(osh_mesh* joined, int nparts, osh_mesh** out);
+int osh_mesh_part_counts(osh_mesh* joined, int nparts, int* counts);
 
 /* Mesh::dim / Mesh::nents, src/Omega_h_mesh.hpp */
 int osh_mesh_dim(osh_mesh* mesh, int* out);

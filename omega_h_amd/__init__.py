@@ -30,6 +30,12 @@ class AdaptOpts(C.Structure):
         ("should_coarsen", C.c_int),
         ("should_swap", C.c_int),
         ("should_coarsen_slivers", C.c_int),
+        ("length_histogram_min", C.c_double),
+        ("length_histogram_max", C.c_double),
+        ("nlength_histogram_bins", C.c_int),
+        ("nquality_histogram_bins", C.c_int),
+        ("should_prevent_coarsen_flip", C.c_int),
+        ("precision", C.c_int),
     ]
 
 
@@ -99,6 +105,9 @@ def load(path=None):
         "osh_mesh_destroy": [P],
         "osh_mesh_copy": [P, C.POINTER(P)],
         "osh_mesh_dim": [P, IP],
+        "osh_mesh_join": [P, I, C.POINTER(P), C.POINTER(P)],
+        "osh_mesh_split": [P, I, C.POINTER(P)],
+        "osh_mesh_part_counts": [P, I, IP],
         "osh_mesh_nents": [P, I, IP],
         "osh_mesh_get_verts": [P, I, IP],
         "osh_mesh_get_down": [P, I, IP, BP],
@@ -319,6 +328,26 @@ class Mesh:
         h = C.c_void_p()
         check(ctx.lib.osh_read_osh(ctx.h, os.fsencode(path), C.byref(h)), ctx.lib)
         return cls(ctx, h)
+
+    @classmethod
+    def join(cls, ctx, parts):
+        """One disjoint mesh of the parts (the batched form of adapt())."""
+        arr = (C.c_void_p * len(parts))(*[p.h for p in parts])
+        h = C.c_void_p()
+        check(ctx.lib.osh_mesh_join(ctx.h, len(parts), arr, C.byref(h)), ctx.lib)
+        return cls(ctx, h)
+
+    def split(self, nparts):
+        """The parts of a joined mesh, as new meshes."""
+        arr = (C.c_void_p * nparts)()
+        check(self.lib.osh_mesh_split(self.h, nparts, arr), self.lib)
+        return [Mesh(self.ctx, C.c_void_p(arr[k])) for k in range(nparts)]
+
+    def part_counts(self, nparts):
+        """Entity counts of each part of a joined mesh: nparts x (dim + 1)."""
+        out = np.zeros(nparts * (self.dim + 1), dtype=np.int32)
+        check(self.lib.osh_mesh_part_counts(self.h, nparts, _p(out, C.c_int)), self.lib)
+        return out.reshape(nparts, self.dim + 1)
 
     def write_osh(self, path):
         check(self.lib.osh_write_osh(self.h, os.fsencode(path)), self.lib)

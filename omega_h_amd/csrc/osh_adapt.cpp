@@ -49,6 +49,36 @@ void print_status(Ctx& c, DMesh& m, AdaptOptions const& o) {
   std::fflush(stdout);
 }
 
+// get_histogram + print_histogram, src/Omega_h_histogram.cpp:15-52: bin i
+// holds the values in [lo + i w, lo + (i + 1) w), the last bin [.., hi]
+void print_histogram(std::vector<Real> const& v, int nbins, Real lo, Real hi,
+    char const* name) {
+  Real w = (hi - lo) / Real(nbins);
+  std::printf("%s histogram:\n", name);
+  for (int i = 0; i < nbins; ++i) {
+    Real floor = w * i + lo;
+    Real ceil = (i == nbins - 1) ? hi : w * (i + 1) + lo;
+    long long n = 0;
+    for (Real x : v)
+      if (x >= floor && (i == nbins - 1 ? x <= ceil : x < ceil)) ++n;
+    std::printf("%.2f-%.2f: %lld\n", w * i + lo, w * (i + 1) + lo, n);
+  }
+}
+
+// print_adapt_histograms, src/Omega_h_adapt.cpp:122-138 (verbosity
+// EXTRA_STATS)
+void print_histograms(Ctx& c, DMesh& m, AdaptOptions const& o) {
+  std::vector<Real> q = to_host(c, ask_qualities(c, m));
+  std::vector<Real> l = to_host(c, ask_lengths(c, m));
+  print_histogram(q, o.nquality_histogram_bins, 0.0, 1.0, "quality");
+  print_histogram(l, o.nlength_histogram_bins, o.length_histogram_min, o.length_histogram_max,
+      "length");
+  Real sum = 0;
+  for (Real x : q) sum += x;
+  std::printf("average quality: %g\n", q.empty() ? 0.0 : sum / Real(q.size()));
+  std::fflush(stdout);
+}
+
 constexpr int kQminSlot = Ctx::kSlots - 1;
 
 double now() {
@@ -95,7 +125,7 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
   OSH_CHECK(o.nsliver_layers >= 0 && o.nsliver_layers < 100, "nsliver_layers out of range");
   // the reports of pre_adapt/post_adapt (src/Omega_h_adapt.cpp:153-166,
   // 244-270) at the reference's verbosity levels: 1 EACH_ADAPT, 2
-  // EACH_REBUILD (with the passes' own lines), 3 EXTRA_STATS (no histograms)
+  // EACH_REBUILD (with the passes' own lines), 3 EXTRA_STATS (histograms)
   double t0 = now();
   if (o.verbosity >= 1) std::printf("before adapting:\n");
   print_status(c, m, o);
@@ -105,6 +135,7 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
   if (qmin >= o.min_quality_desired && lmin >= o.min_length_desired &&
       lmax <= o.max_length_desired)
     return false;
+  if (o.verbosity >= 3) print_histograms(c, m, o);
   if (o.verbosity >= 2) std::printf("addressing edge lengths\n");
   double t1 = now();
   bool did;
@@ -133,6 +164,7 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
     std::printf("after adapting:\n");
     print_status(c, m, o);
   }
+  if (o.verbosity >= 3) print_histograms(c, m, o);
   if (o.verbosity >= 1) {
     std::printf("addressing edge lengths took %g seconds\n", t2 - t1);
     std::printf("addressing element qualities took %g seconds\n", t3 - t2);

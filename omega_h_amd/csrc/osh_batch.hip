@@ -1,0 +1,220 @@
+// Many independent meshes adapted as one: the batched form of adapt().
+//
+// Omega_h's adapt() (src/Omega_h_adapt.cpp:272-288) takes one mesh.  Every
+// decision inside a pass is local to a cavity, the independent set is a
+// fixed point of a rule over star neighbours (src/Omega_h_indset_inline.hpp),
+// and modify_ents numbers new entities in the order of the old ones
+// (src/Omega_h_modify.cpp:141-517).  So for a mesh made of disjoint parts,
+// with part k's entities numbered after those of parts < k in every
+// dimension, one pass does to each part exactly what the same pass does to
+// that part alone, and the parts stay contiguous blocks.  Passes that find
+// nothing to do in a part leave it as it is, so the extra passes a part sees
+// while other parts still change are no-ops, and each part ends where its
+// own adapt() ends.  The batched form joins the meshes, runs adapt() on the
+// joined mesh (one launch per kernel for all meshes of the batch) and splits
+// the result.
+//
+// Each part's vertices carry their part number in the vertex tag
+// kPartTag (a linear tag: refinement midpoints average two equal numbers, so
+// the value is exact); an entity of any dimension belongs to the part of its
+// first vertex.
+#include <algorithm>
+
+#include "osh_passes.hpp"
+
+namespace osh {
+
+char const* const kPartTag = "omega_h_amd_part";
+
+namespace {
+
+__global__ void copy_offset(LO n, LO const* in, LO offset, LO* out) {
+  LO i = thread_index();
+  if (i < n) out[i] = in[i] + offset;
+}
+
+template <class T>
+void copy_into(Ctx& c, DArray<T> const& src, DArray<T>& dst, LO at, LO n) {
+  if (n > 0)
+    OSH_HIP(hipMemcpyAsync(dst.data() + at, src.data(), sizeof(T) * std::size_t(n),
+        hipMemcpyDeviceToDevice, c.stream));
+}
+
+template <class T>
+DArray<T> slice(Ctx& c, DArray<T> const& src, LO at, LO n) {
+  DArray<T> out(c, n);
+  if (n > 0)
+    OSH_HIP(hipMemcpyAsync(out.data(), src.data() + at, sizeof(T) * std::size_t(n),
+        hipMemcpyDeviceToDevice, c.stream));
+  return out;
+}
+
+// first and last entity of each part, and its entity count, from the part
+// number of each entity's first vertex
+__global__ void part_bounds(LO n, LO const* ev2v, int deg, Real const* vpart, int nparts,
+    LO* first, LO* last, LO* count, int* error) {
+  LO e = thread_index();
+  if (e >= n) return;
+  LO v = ev2v ? ev2v[std::size_t(e) * deg] : e;
+  Real pr = vpart[v];
+  int p = int(pr);
+  if (!(pr >= 0.0) || p >= nparts || Real(p) != pr) {
+    *error = kBadPart;
+    return;
+  }
+  atomicMin(first + p, e);
+  atomicMax(last + p, e);
+  atomicAdd(count + p, 1);
+}
+
+}  // namespace
+
+void join_meshes(Ctx& c, std::vector<DMesh const*> const& parts, DMesh& out) {
+  OSH_CHECK(!parts.empty(), "join: no meshes");
+  DMesh const& p0 = *parts[0];
+  int dim = p0.dim;
+  for (DMesh const* p : parts) {
+    OSH_CHECK(p->dim == dim, "join: meshes of different dimensions");
+    OSH_CHECK(p->metric.exists() == p0.metric.exists() && p->metric_ncomps == p0.metric_ncomps,
+        "join: meshes with different metric layouts");
+    for (int d = 0; d <= dim; ++d)
+      OSH_CHECK(p->class_id[d].exists() && p->class_dim[d].exists(),
+          "join: every mesh must be classified");
+    OSH_CHECK(p->itags.empty() && p->etags.empty(),
+        "join: inheritance and element tags are not carried");
+    OSH_CHECK(p->vtags.size() == p0.vtags.size(), "join: meshes with different vertex tags");
+    for (std::size_t t = 0; t < p->vtags.size(); ++t)
+      OSH_CHECK(p->vtags[t].name == p0.vtags[t].name && p->vtags[t].ncomps == p0.vtags[t].ncomps &&
+                    p->vtags[t].kind == p0.vtags[t].kind && p->vtags[t].name != kPartTag,
+          "join: meshes with different vertex tags");
+  }
+  DMesh r;
+  r.dim = dim;
+  r.metric_ncomps = p0.metric_ncomps;
+  std::int64_t tot[4] = {0, 0, 0, 0};
+  for (DMesh const* p : parts)
+    for (int d = 0; d <= dim; ++d) tot[d] += p->nents[d];
+  for (int d = 0; d <= dim; ++d) {
+    OSH_CHECK(tot[d] * (d + 1) < (std::int64_t(1) << 31), "join: too many entities");
+    r.nents[d] = LO(tot[d]);
+  }
+  for (int d = 1; d <= dim; ++d) {
+    r.down[d] = LOs(c, r.nents[d] * (d + 1));
+    if (d >= 2) r.down_codes[d] = Bytes(c, r.nents[d] * (d + 1));
+  }
+  r.coords = Reals(c, r.nents[0] * dim);
+  if (p0.metric.exists()) r.metric = Reals(c, r.nents[0] * r.metric_ncomps);
+  for (int d = 0; d <= dim; ++d) {
+    r.class_id[d] = LOs(c, r.nents[d]);
+    r.class_dim[d] = Bytes(c, r.nents[d]);
+  }
+  for (auto const& t : p0.vtags) r.vtags.push_back({t.name, t.ncomps, t.kind, Reals(c, r.nents[0] * t.ncomps)});
+  Reals part(c, r.nents[0]);
+  std::vector<Real> part_h(std::size_t(r.nents[0]));
+  LO at[4] = {0, 0, 0, 0};
+  for (std::size_t k = 0; k < parts.size(); ++k) {
+    DMesh const& p = *parts[k];
+    for (int d = 1; d <= dim; ++d) {
+      LO n = p.nents[d] * (d + 1);
+      if (n > 0)
+        hipLaunchKernelGGL(copy_offset, dim3(nblocks(n)), dim3(kBlock), 0, c.stream, n,
+            p.down[d].data(), at[d - 1], r.down[d].data() + std::size_t(at[d]) * (d + 1));
+      OSH_HIP(hipGetLastError());
+      if (d >= 2) copy_into(c, p.down_codes[d], r.down_codes[d], at[d] * (d + 1), n);
+    }
+    copy_into(c, p.coords, r.coords, at[0] * dim, p.nents[0] * dim);
+    if (p.metric.exists())
+      copy_into(c, p.metric, r.metric, at[0] * r.metric_ncomps, p.nents[0] * r.metric_ncomps);
+    for (int d = 0; d <= dim; ++d) {
+      copy_into(c, p.class_id[d], r.class_id[d], at[d], p.nents[d]);
+      copy_into(c, p.class_dim[d], r.class_dim[d], at[d], p.nents[d]);
+    }
+    for (std::size_t t = 0; t < p.vtags.size(); ++t)
+      copy_into(c, p.vtags[t].data, r.vtags[t].data, at[0] * p.vtags[t].ncomps,
+          p.nents[0] * p.vtags[t].ncomps);
+    std::fill(part_h.begin() + at[0], part_h.begin() + at[0] + p.nents[0], Real(k));
+    for (int d = 0; d <= dim; ++d) at[d] += p.nents[d];
+  }
+  upload(c, part, part_h.data(), r.nents[0]);
+  r.vtags.push_back({kPartTag, 1, 0, part});
+  c.sync();
+  out = r;
+}
+
+void part_ranges(Ctx& c, DMesh& m, int nparts, std::vector<LO> first[4],
+    std::vector<LO> count[4]) {
+  Reals const* part = nullptr;
+  for (auto const& t : m.vtags)
+    if (t.name == kPartTag) part = &t.data;
+  OSH_CHECK(part != nullptr, "split: the mesh was not joined");
+  OSH_CHECK(nparts >= 1, "split: no parts");
+  for (int d = 0; d <= m.dim; ++d) {
+    std::vector<LO> f(std::size_t(nparts), 0x7fffffff), l(std::size_t(nparts), -1),
+        n(std::size_t(nparts), 0);
+    LOs df = to_device(c, f), dl = to_device(c, l), dn = to_device(c, n);
+    LO const* ev2v = d == 0 ? nullptr : verts_of(c, m, d).data();
+    if (m.nents[d] > 0)
+      hipLaunchKernelGGL(part_bounds, dim3(nblocks(m.nents[d])), dim3(kBlock), 0, c.stream,
+          m.nents[d], ev2v, d + 1, part->data(), nparts, df.data(), dl.data(), dn.data(),
+          c.error_word());
+    OSH_HIP(hipGetLastError());
+    f = to_host(c, df);
+    l = to_host(c, dl);
+    n = to_host(c, dn);
+    LO next = 0;
+    for (int k = 0; k < nparts; ++k) {
+      if (n[k] == 0) {
+        f[k] = next;
+        continue;
+      }
+      // each part is one block, the blocks in part order
+      OSH_CHECK(f[k] == next && l[k] - f[k] + 1 == n[k], "split: parts are not contiguous");
+      next += n[k];
+    }
+    OSH_CHECK(next == m.nents[d], "split: entities outside the parts");
+    first[d] = f;
+    count[d] = n;
+  }
+}
+
+void split_mesh(Ctx& c, DMesh& m, int nparts, std::vector<DMesh>& out) {
+  std::vector<LO> first[4], count[4];
+  part_ranges(c, m, nparts, first, count);
+  int dim = m.dim;
+  out.assign(std::size_t(nparts), DMesh());
+  Reals const& len = m.length;
+  Reals const& qual = m.quality;
+  for (int k = 0; k < nparts; ++k) {
+    DMesh& r = out[std::size_t(k)];
+    r.dim = dim;
+    r.metric_ncomps = m.metric_ncomps;
+    for (int d = 0; d <= dim; ++d) r.nents[d] = count[d][k];
+    for (int d = 1; d <= dim; ++d) {
+      LO n = r.nents[d] * (d + 1);
+      r.down[d] = LOs(c, n);
+      if (n > 0)
+        hipLaunchKernelGGL(copy_offset, dim3(nblocks(n)), dim3(kBlock), 0, c.stream, n,
+            m.down[d].data() + std::size_t(first[d][k]) * (d + 1), -first[d - 1][k],
+            r.down[d].data());
+      OSH_HIP(hipGetLastError());
+      if (d >= 2) r.down_codes[d] = slice(c, m.down_codes[d], first[d][k] * (d + 1), n);
+    }
+    r.coords = slice(c, m.coords, first[0][k] * dim, r.nents[0] * dim);
+    if (m.metric.exists())
+      r.metric = slice(c, m.metric, first[0][k] * m.metric_ncomps, r.nents[0] * m.metric_ncomps);
+    for (int d = 0; d <= dim; ++d) {
+      r.class_id[d] = slice(c, m.class_id[d], first[d][k], r.nents[d]);
+      r.class_dim[d] = slice(c, m.class_dim[d], first[d][k], r.nents[d]);
+    }
+    for (auto const& t : m.vtags) {
+      if (t.name == kPartTag) continue;
+      r.vtags.push_back({t.name, t.ncomps, t.kind,
+          slice(c, t.data, first[0][k] * t.ncomps, r.nents[0] * t.ncomps)});
+    }
+    if (len.exists()) r.length = slice(c, len, first[1][k], r.nents[1]);
+    if (qual.exists()) r.quality = slice(c, qual, first[dim][k], r.nents[dim]);
+  }
+  c.sync();
+}
+
+}  // namespace osh

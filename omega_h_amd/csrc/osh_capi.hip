@@ -33,6 +33,19 @@ void to_options(osh_adapt_opts const* o, AdaptOptions& r) {
   r.should_coarsen = o->should_coarsen;
   r.should_swap = o->should_swap;
   r.should_coarsen_slivers = o->should_coarsen_slivers;
+  r.length_histogram_min = o->length_histogram_min;
+  r.length_histogram_max = o->length_histogram_max;
+  r.nlength_histogram_bins = o->nlength_histogram_bins;
+  r.nquality_histogram_bins = o->nquality_histogram_bins;
+  r.should_prevent_coarsen_flip = o->should_prevent_coarsen_flip;
+  r.precision = o->precision;
+  OSH_CHECK(!r.should_prevent_coarsen_flip,
+      "should_prevent_coarsen_flip is not supported (src/Omega_h_coarsen_flip.cpp:68-150 "
+      "is not built)");
+  OSH_CHECK(r.precision == 0 || r.precision == 1, "precision must be OSH_FP64 or OSH_MIXED");
+  OSH_CHECK(r.nlength_histogram_bins >= 1 && r.nquality_histogram_bins >= 1 &&
+                r.length_histogram_max > r.length_histogram_min,
+      "histogram options out of range");
 }
 
 namespace {
@@ -228,6 +241,12 @@ int osh_default_opts(int dim, osh_adapt_opts* out) {
     out->should_coarsen = o.should_coarsen;
     out->should_swap = o.should_swap;
     out->should_coarsen_slivers = o.should_coarsen_slivers;
+    out->length_histogram_min = o.length_histogram_min;
+    out->length_histogram_max = o.length_histogram_max;
+    out->nlength_histogram_bins = o.nlength_histogram_bins;
+    out->nquality_histogram_bins = o.nquality_histogram_bins;
+    out->should_prevent_coarsen_flip = o.should_prevent_coarsen_flip;
+    out->precision = o.precision;
   });
 }
 
@@ -283,6 +302,50 @@ int osh_mesh_copy(osh_mesh* mesh, osh_mesh** out) {
     check_mesh(mesh);
     auto m = std::make_unique<osh_mesh>(*mesh);
     *out = m.release();
+  });
+}
+
+int osh_mesh_join(osh_ctx* ctx, int nparts, osh_mesh* const* parts, osh_mesh** out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && parts && out && nparts >= 1, "null argument");
+    std::vector<osh::DMesh const*> ps;
+    for (int k = 0; k < nparts; ++k) {
+      check_mesh(parts[k]);
+      OSH_CHECK(parts[k]->owner == ctx, "join: every mesh must belong to the context");
+      ps.push_back(&parts[k]->m);
+    }
+    auto m = std::make_unique<osh_mesh>();
+    m->owner = ctx;
+    osh::join_meshes(ctx->ctx, ps, m->m);
+    *out = m.release();
+  });
+}
+
+int osh_mesh_split(osh_mesh* joined, int nparts, osh_mesh** out) {
+  return guarded([&] {
+    check_mesh(joined);
+    OSH_CHECK(out != nullptr && nparts >= 1, "null argument");
+    std::vector<osh::DMesh> parts;
+    osh::split_mesh(joined->owner->ctx, joined->m, nparts, parts);
+    std::vector<std::unique_ptr<osh_mesh>> made;
+    for (auto& p : parts) {
+      made.push_back(std::make_unique<osh_mesh>());
+      made.back()->owner = joined->owner;
+      made.back()->m = p;
+    }
+    for (int k = 0; k < nparts; ++k) out[k] = made[std::size_t(k)].release();
+  });
+}
+
+int osh_mesh_part_counts(osh_mesh* joined, int nparts, int* counts) {
+  return guarded([&] {
+    check_mesh(joined);
+    OSH_CHECK(counts != nullptr && nparts >= 1, "null argument");
+    std::vector<osh::LO> first[4], count[4];
+    osh::part_ranges(joined->owner->ctx, joined->m, nparts, first, count);
+    int dim = joined->m.dim;
+    for (int k = 0; k < nparts; ++k)
+      for (int d = 0; d <= dim; ++d) counts[k * (dim + 1) + d] = count[d][std::size_t(k)];
   });
 }
 

@@ -999,6 +999,12 @@ void default_options(int dim, AdaptOptions& o) {
   o.should_coarsen = 1;
   o.should_swap = 1;
   o.should_coarsen_slivers = 1;
+  o.length_histogram_min = 0.0;
+  o.length_histogram_max = 3.0;
+  o.nlength_histogram_bins = 10;
+  o.nquality_histogram_bins = 10;
+  o.should_prevent_coarsen_flip = 0;
+  o.precision = 0;
 }
 
 }  // namespace osh

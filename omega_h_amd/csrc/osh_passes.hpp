@@ -20,6 +20,14 @@ struct AdaptOptions {
   int should_coarsen;
   int should_swap;
   int should_coarsen_slivers;
+  Real length_histogram_min;
+  Real length_histogram_max;
+  int nlength_histogram_bins;
+  int nquality_histogram_bins;
+  int should_prevent_coarsen_flip;
+  // 0 FP64; 1 mixed: cavity screens in FP32, every cavity the screen does
+  // not reject by a margin re-evaluated in FP64 (same mesh, bit for bit)
+  int precision;
 };
 
 void default_options(int dim, AdaptOptions& o);
@@ -123,6 +131,17 @@ bool swap_edges(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache = null
 // known-answer hooks used by the tests
 Reals refine_qualities(Ctx& c, DMesh& m, LOs const& cands);
 void find_loops(Ctx& c, DMesh& m, LOs const& edges, LOs& eev2v, LOs& loop_verts, LOs& sizes);
+
+// the batched form (osh_batch.hip): meshes joined into one disjoint mesh
+// (part k's entities of every dimension after those of parts < k; the
+// vertex tag kPartTag holds each vertex's part), adapted as one, split back
+extern char const* const kPartTag;
+void join_meshes(Ctx& c, std::vector<DMesh const*> const& parts, DMesh& out);
+// first entity and entity count of each part in every dimension; an error
+// unless each part is one block and the blocks follow the part order
+void part_ranges(Ctx& c, DMesh& m, int nparts, std::vector<LO> first[4],
+    std::vector<LO> count[4]);
+void split_mesh(Ctx& c, DMesh& m, int nparts, std::vector<DMesh>& out);
 
 struct PassRecord {
   int kind;  // 0 refine, 1 coarsen, 2 swap, 3 coarsen slivers

@@ -96,6 +96,7 @@ void Ctx::sync() {
     if (code == kNonManifoldSide) throw Error("a side has more than two elements");
     if (code == kBrokenSwapLoop)
       throw Error("swap: the elements around an edge do not close a loop");
+    if (code == kBadPart) throw Error("split: a vertex without a valid part number");
     if (code == kDegenerateEigen)
       throw Error("eigen: no eigenvector for a simple eigenvalue (degenerate metric)");
     throw Error("device error " + std::to_string(code));

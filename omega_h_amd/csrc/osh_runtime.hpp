@@ -185,7 +185,8 @@ enum DeviceError {
   kKeyWithoutUp = 4,
   kNonManifoldSide = 5,
   kBrokenSwapLoop = 6,
-  kDegenerateEigen = 7
+  kDegenerateEigen = 7,
+  kBadPart = 8
 };
 
 template <class T>

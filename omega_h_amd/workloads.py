@@ -85,3 +85,69 @@ def banded_aniso_metric(coords, dim, rng):
     else:
         v = np.stack([m[:, 0, 0], m[:, 1, 1], m[:, 0, 1]], 1)
     return v.ravel(), v.shape[1]
+
+
+# volume of the unit-edge regular simplex (typical_unit_simplex_size,
+# src/Omega_h_metric.cpp:288-290)
+UNIT_SIMPLEX_SIZE = {2: math.sqrt(3.0) / 4.0, 3: 1.0 / (6.0 * math.sqrt(2.0))}
+
+
+def symm_det(v):
+    """Determinant of one metric in Omega_h's symmetric layout."""
+    if len(v) == 3:
+        return v[0] * v[1] - v[2] * v[2]
+    a, b, c, d, e, f = v
+    return a * (b * c - e * e) - d * (d * c - e * f) + f * (d * e - b * f)
+
+
+def expected_nelems_unit_box(v, dim):
+    """get_expected_nelems (src/Omega_h_metric.cpp:464-519) of a constant
+    metric on the unit box: sqrt(det M) / the unit simplex's size."""
+    return math.sqrt(symm_det(v)) / UNIT_SIMPLEX_SIZE[dim]
+
+
+def clamp_metric(v, dim, max_nelems):
+    """The reference's element-count limit (should_limit_element_count,
+    src/Omega_h_metric_input.cpp:226-236 with get_metric_scalar_for_nelems,
+    src/Omega_h_metric.cpp:521-526): a metric expected to give more than
+    max_nelems elements is scaled by (max_nelems / expected)^(2/dim)."""
+    n = expected_nelems_unit_box(v, dim)
+    if n <= max_nelems:
+        return np.asarray(v, dtype=np.float64), n, 1.0
+    s = (max_nelems / n) ** (2.0 / dim)
+    return np.asarray(v, dtype=np.float64) * s, n, s
+
+
+def batch_boxes(nboxes, seed, n_range=(8, 48), h_range=(0.005, 0.1), max_nelems=None,
+                dim=3):
+    """Independent boxes with their own constant rotated metrics: cells per
+    side n = round(exp(U(ln n0, ln n1))), a random unit quaternion (a
+    normalised 4-D Gaussian) for the axes and eigen-lengths exp(U(ln h0, ln
+    h1)); numpy's PCG64 generator seeded with `seed`.  Returns a list of
+    (n, metric components, expected elements before the clamp, clamp scale)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(nboxes):
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        if dim == 3:
+            q = rng.normal(size=4)
+            q /= np.linalg.norm(q)
+            w, x, y, z = q
+            R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                          [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                          [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        else:
+            a = rng.uniform(0, 2 * math.pi)
+            R = np.array([[math.cos(a), -math.sin(a)], [math.sin(a), math.cos(a)]])
+        h = np.exp(rng.uniform(math.log(h_range[0]), math.log(h_range[1]), dim))
+        M = R @ np.diag(1.0 / (h * h)) @ R.T
+        if dim == 3:
+            v = [M[0, 0], M[1, 1], M[2, 2], M[0, 1], M[1, 2], M[0, 2]]
+        else:
+            v = [M[0, 0], M[1, 1], M[0, 1]]
+        if max_nelems is None:
+            out.append((n, np.asarray(v), expected_nelems_unit_box(v, dim), 1.0))
+        else:
+            vc, e, s = clamp_metric(v, dim, max_nelems)
+            out.append((n, vc, e, s))
+    return out

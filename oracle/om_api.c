@@ -406,3 +406,49 @@ void om_get_itag(om_mesh* m, char const* name, int d, double* out) {
   om_itag* t = find_itag(m, name);
   if (t) memcpy(out, t->data[d], sizeof(double) * (size_t)m->nents[d] * t->ncomps);
 }
+
+/* one disjoint mesh of nparts meshes (same dimension and metric layout,
+ * classified, no vertex tags): part k's entities of every dimension follow
+ * those of parts < k, and the vertex tag "part" holds each vertex's part */
+om_mesh* om_mesh_join(int nparts, om_mesh* const* parts) {
+  int dim = parts[0]->dim;
+  int nc = parts[0]->metric_ncomps;
+  om_mesh* r = om_mesh_new(dim);
+  r->metric_ncomps = nc;
+  for (int d = 0; d <= dim; ++d) {
+    r->nents[d] = 0;
+    for (int k = 0; k < nparts; ++k) r->nents[d] += parts[k]->nents[d];
+  }
+  for (int d = 1; d <= dim; ++d) {
+    r->down[d] = (int*)om_malloc(sizeof(int) * ((size_t)r->nents[d] * (d + 1) + 1));
+    if (d >= 2) r->down_codes[d] = (signed char*)om_malloc((size_t)r->nents[d] * (d + 1) + 1);
+  }
+  r->coords = (double*)om_malloc(sizeof(double) * ((size_t)r->nents[0] * dim + 1));
+  r->metric = (double*)om_malloc(sizeof(double) * ((size_t)r->nents[0] * nc + 1));
+  for (int d = 0; d <= dim; ++d) {
+    r->class_id[d] = (int*)om_malloc(sizeof(int) * ((size_t)r->nents[d] + 1));
+    r->class_dim[d] = (signed char*)om_malloc((size_t)r->nents[d] + 1);
+  }
+  double* part = (double*)om_malloc(sizeof(double) * ((size_t)r->nents[0] + 1));
+  int at[4] = {0, 0, 0, 0};
+  for (int k = 0; k < nparts; ++k) {
+    om_mesh const* p = parts[k];
+    for (int d = 1; d <= dim; ++d) {
+      size_t n = (size_t)p->nents[d] * (d + 1);
+      for (size_t i = 0; i < n; ++i)
+        r->down[d][(size_t)at[d] * (d + 1) + i] = p->down[d][i] + at[d - 1];
+      if (d >= 2) memcpy(r->down_codes[d] + (size_t)at[d] * (d + 1), p->down_codes[d], n);
+    }
+    memcpy(r->coords + (size_t)at[0] * dim, p->coords, sizeof(double) * (size_t)p->nents[0] * dim);
+    memcpy(r->metric + (size_t)at[0] * nc, p->metric, sizeof(double) * (size_t)p->nents[0] * nc);
+    for (int d = 0; d <= dim; ++d) {
+      memcpy(r->class_id[d] + at[d], p->class_id[d], sizeof(int) * (size_t)p->nents[d]);
+      memcpy(r->class_dim[d] + at[d], p->class_dim[d], (size_t)p->nents[d]);
+    }
+    for (int v = 0; v < p->nents[0]; ++v) part[at[0] + v] = (double)k;
+    for (int d = 0; d <= dim; ++d) at[d] += p->nents[d];
+  }
+  om_add_vtag(r, "part", 1, OM_LINEAR, part);
+  free(part);
+  return r;
+}

@@ -43,6 +43,9 @@ om_mesh* om_build_box_internal(int dim, double x, double y, double z, int nx,
     int ny, int nz);
 om_mesh* om_build_from_elems(int dim, int nelems, int const* ev2v, int nverts);
 void om_mesh_free(om_mesh* m);
+/* one disjoint mesh of nparts meshes, part k's entities after those of
+ * parts < k, the vertex tag "part" holding each vertex's part */
+om_mesh* om_mesh_join(int nparts, om_mesh* const* parts);
 int om_mesh_dim(om_mesh* m);
 int om_mesh_nents(om_mesh* m, int d);
 void om_get_verts(om_mesh* m, int d, int* out);

@@ -65,6 +65,8 @@ def lib():
         L.om_build_from_elems.restype = P
         L.om_build_from_elems.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int]
         L.om_mesh_free.argtypes = [P]
+        L.om_mesh_join.restype = P
+        L.om_mesh_join.argtypes = [C.c_int, C.POINTER(P)]
         L.om_mesh_dim.argtypes = [P]
         L.om_mesh_nents.argtypes = [P, C.c_int]
         L.om_get_verts.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
@@ -197,6 +199,12 @@ class Mesh:
     def from_elems(cls, dim, ev2v, nverts):
         a = np.ascontiguousarray(ev2v, dtype=np.int32)
         return cls(lib().om_build_from_elems(dim, len(a) // (dim + 1), _ptr(a, C.c_int), nverts))
+
+    @classmethod
+    def join(cls, parts):
+        """One disjoint mesh of the parts (vertex tag "part" = part number)."""
+        arr = (C.c_void_p * len(parts))(*[p.h for p in parts])
+        return cls(lib().om_mesh_join(len(parts), arr))
 
     @property
     def dim(self):
