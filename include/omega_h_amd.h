@@ -157,6 +157,14 @@ int osh_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats
  * 4 * OSH_MAX_PASSES values; stats is required. */
 int osh_adapt_digests(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats,
     unsigned long long* digests, int* did);
+/* the FP32 estimates that the mixed-precision form (precision OSH_MIXED)
+ * screens cavities with, for every edge length (nedges values) and element
+ * quality (nelems values), and the elements whose max-determinant vertex
+ * metric is not clear in FP32 (unsure = 1; the screen re-decides those in
+ * FP64); no counterpart in the reference, for reporting the estimates'
+ * error against osh_mesh_get_lengths / osh_mesh_get_qualities */
+int osh_mesh_screen_estimates(osh_mesh* mesh, double* lengths32, double* qualities32,
+    signed char* unsure);
 /* the same four digests of the mesh as it is */
 int osh_mesh_digests(osh_mesh* mesh, unsigned long long* out);
 

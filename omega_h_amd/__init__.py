@@ -144,6 +144,7 @@ def load(path=None):
         "osh_adapt_digests": [P, C.POINTER(AdaptOpts), C.POINTER(AdaptStats),
                               C.POINTER(C.c_ulonglong), IP],
         "osh_mesh_digests": [P, C.POINTER(C.c_ulonglong)],
+        "osh_mesh_screen_estimates": [P, DP, DP, BP],
         "osh_measure_edges_metric": [P, I, I, I, P, P, P, P, P],
         "osh_measure_qualities": [P, I, I, I, P, P, P, P, P],
         "osh_offset_scans": [P, I, P, IP, P],
@@ -630,6 +631,17 @@ class Mesh:
                   for i in range(st.npasses)]
         digests = [tuple(dg[4 * i + k] for k in range(4)) for i in range(st.npasses)]
         return did.value, passes, digests
+
+    def screen_estimates(self):
+        """FP32 estimates of the edge lengths and element qualities (the
+        mixed-precision screens) and the elements marked unsure."""
+        l32 = np.zeros(self.nents(1), dtype=np.float64)
+        q32 = np.zeros(self.nents(self.dim), dtype=np.float64)
+        u = np.zeros(self.nents(self.dim), dtype=np.int8)
+        check(self.lib.osh_mesh_screen_estimates(self.h, _p(l32, C.c_double),
+                                                 _p(q32, C.c_double), _p(u, C.c_byte)),
+              self.lib)
+        return l32, q32, u
 
     def digests(self):
         """The same four digests of the mesh as it is."""

@@ -574,6 +574,21 @@ int osh_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats
   });
 }
 
+int osh_mesh_screen_estimates(osh_mesh* mesh, double* lengths32, double* qualities32,
+    signed char* unsure) {
+  return guarded([&] {
+    check_metric(mesh);
+    OSH_CHECK(lengths32 && qualities32 && unsure, "null argument");
+    auto& c = mesh->owner->ctx;
+    osh::Reals l, q;
+    osh::Bytes u;
+    osh::screen_estimates(c, mesh->m, l, q, u);
+    download(c, l, lengths32);
+    download(c, q, qualities32);
+    download(c, u, reinterpret_cast<osh::I8*>(unsure));
+  });
+}
+
 int osh_adapt_digests(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats,
     unsigned long long* digests, int* did) {
   return guarded([&] {

@@ -108,11 +108,15 @@ __global__ void apply_edge_vetoes(LO ncands, LO const* cands, I8 const* veto, I8
 constexpr int kLanes = 8;
 
 // prevent_coarsen_overshoot: a side is vetoed when any new edge would reach
-// max_length (the order of the row does not matter)
-template <int D, int M>
+// max_length (the order of the row does not matter).  F32 (mixed
+// precision): the lengths are FP32 estimates; a side is vetoed when one is
+// over max_length by more than the screening band, and sides with a length
+// inside the band are marked in `need` for the FP64 sweep.  With `need`
+// given, the FP64 sweep looks only at marked sides not already vetoed.
+template <int D, int M, bool F32>
 __global__ void overshoot_lanes(LO nt, LO const* cands, LO const* ev2v, LO const* v2e_a2ab,
     LO const* v2e_ab2b, I8 const* v2e_codes, Real const* coords, Real const* metric,
-    Real max_length, I8 const* codes, I8* veto) {
+    Real max_length, I8 const* codes, I8* veto, I8* need) {
   LO t = thread_index();
   if (t >= nt) return;
   LO side = t / kLanes;
@@ -120,6 +124,7 @@ __global__ void overshoot_lanes(LO nt, LO const* cands, LO const* ev2v, LO const
   LO e = cands[side / 2];
   int col = side % 2;
   if (!collapses(codes[e], col)) return;
+  if (!F32 && need && (!need[side] || veto[side])) return;
   constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
   LO v_col = ev2v[e * 2 + col];
   LO v_onto = ev2v[e * 2 + (1 - col)];
@@ -140,9 +145,19 @@ __global__ void overshoot_lanes(LO nt, LO const* cands, LO const* ev2v, LO const
       m0[k] = metric[nv[0] * nc + k];
       m1[k] = metric[nv[1] * nc + k];
     }
-    if (edge_length<D, M>(p0, p1, m0, m1) >= max_length) {
-      veto[side] = 1;
-      return;
+    if constexpr (F32) {
+      float l = f32::edge_length<D, M>(p0, p1, m0, m1);
+      float lmax = float(max_length);
+      if (l >= lmax * (1.0f + f32::kScreenBand)) {
+        veto[side] = 1;
+        return;
+      }
+      if (!(l < lmax * (1.0f - f32::kScreenBand))) need[side] = 1;
+    } else {
+      if (edge_length<D, M>(p0, p1, m0, m1) >= max_length) {
+        veto[side] = 1;
+        return;
+      }
     }
   }
 }
@@ -159,12 +174,15 @@ __global__ void apply_vetoes(LO ncands, LO const* cands, I8 const* veto, I8* cod
 
 // coarsen_qualities: each lane folds the qualities of its share of the
 // collapsing vertex's cells (those that survive the collapse, with v_col
-// moved onto v_onto) into a partial minimum
-template <int D, int M>
+// moved onto v_onto) into a partial minimum.  F32 (mixed precision): the
+// qualities are FP32 estimates, and a lane that meets an element whose
+// max-determinant metric is not clear marks its side in `unsure`.  With
+// `need` given, the FP64 sweep evaluates only the marked sides.
+template <int D, int M, bool F32>
 __global__ void __launch_bounds__(kBlock, 5) coarsen_quality_lanes(LO nt, LO const* cands,
     LO const* ev2v, LO const* cv2v, LO const* v2c_a2ab, LO const* v2c_ab2b,
     I8 const* v2c_codes, Real const* coords, Real const* metric, I8 const* codes,
-    Real* partial) {
+    I8 const* need, I8* unsure, Real* partial) {
   LO t = thread_index();
   if (t >= nt) return;
   LO side = t / kLanes;
@@ -172,7 +190,9 @@ __global__ void __launch_bounds__(kBlock, 5) coarsen_quality_lanes(LO nt, LO con
   LO e = cands[side / 2];
   int col = side % 2;
   double minqual = 1.0;
-  if (collapses(codes[e], col)) {
+  float minq32 = 1.0f;
+  bool unclear = false;
+  if (collapses(codes[e], col) && (F32 || !need || need[side])) {
     constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
     LO v_col = ev2v[e * 2 + col];
     LO v_onto = ev2v[e * 2 + (1 - col)];
@@ -197,29 +217,60 @@ __global__ void __launch_bounds__(kBlock, 5) coarsen_quality_lanes(LO nt, LO con
         pp[k] = p[k];
         ms[k] = m[k];
       }
-      minqual = math::min2(minqual, element_quality<D, M>(pp, ms));
+      if constexpr (F32) {
+        float q = f32::element_quality<D, M>(pp, ms, &unclear);
+        minq32 = (q < minq32) ? q : minq32;
+      } else {
+        minqual = math::min2(minqual, element_quality<D, M>(pp, ms));
+      }
     }
   }
-  partial[t] = minqual;
+  if constexpr (F32) {
+    if (unclear) unsure[side] = 1;
+    partial[t] = double(minq32);
+  } else {
+    partial[t] = minqual;
+  }
 }
 
 // the minimum over the lanes: the minimum over the row, as the reference's
 // loop computes it (min2 never takes a NaN, and a zero's sign decides
-// nothing downstream: the filters, rails and independent set compare it only)
+// nothing downstream: the filters, rails and independent set compare it
+// only).  After a mixed-precision screen (`need` given) a side the screen
+// rejected gets -1, below every quality threshold.
 __global__ void combine_quality_lanes(LO ncands, LO const* cands, I8 const* codes,
-    Real const* partial, Real* quals) {
+    I8 const* need, Real const* partial, Real* quals) {
   LO i = thread_index();
   if (i >= ncands) return;
   LO e = cands[i];
   I8 code = codes[e];
   for (int col = 0; col < 2; ++col) {
     double q = -1.0;
-    if (collapses(code, col)) {
+    if (collapses(code, col) && (!need || need[i * 2 + col])) {
       q = 1.0;
       for (int l = 0; l < kLanes; ++l) q = math::min2(q, partial[(i * 2 + col) * kLanes + l]);
     }
     quals[e * 2 + col] = q;
   }
+}
+
+// the screen's verdict per side: FP64 is needed unless the FP32 minimum is
+// below qmin by more than the band and every element's metric was clear
+// (a NaN estimate compares false and is evaluated again)
+__global__ void screen_quality_sides(LO nsides, LO const* cands, I8 const* codes,
+    I8 const* unsure, Real const* partial, Real qmin, I8* need) {
+  LO side = thread_index();
+  if (side >= nsides) return;
+  LO e = cands[side / 2];
+  int col = side % 2;
+  if (!collapses(codes[e], col)) {
+    need[side] = 0;
+    return;
+  }
+  double q = 1.0;
+  for (int l = 0; l < kLanes; ++l) q = math::min2(q, partial[side * kLanes + l]);
+  bool reject = !unsure[side] && q < qmin * (1.0 - double(f32::kScreenBand));
+  need[side] = reject ? 0 : 1;
 }
 
 // the vertices that would collapse along some candidate edge
@@ -421,18 +472,29 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   Adj v2e = marked_up(c, m, 0, 1, vmarks);
   LO nlanes = ncands * 2 * kLanes;
   Bytes veto = fill_i8(c, ncands * 2, 0);
-#define OSH_OV(D, M)                                                                     \
-  OSH_LAUNCH((overshoot_lanes<D, M>), nlanes, c, cands.data(), ev2v, v2e.a2ab.data(),       \
+  // mixed precision: an FP32 screen first, then FP64 only for the sides it
+  // left open
+  bool mixed = o.precision == 1;
+  Bytes need;
+  if (mixed) need = fill_i8(c, ncands * 2, 0);
+#define OSH_OV(D, M, F32)                                                                \
+  OSH_LAUNCH((overshoot_lanes<D, M, F32>), nlanes, c, cands.data(), ev2v, v2e.a2ab.data(),  \
       v2e.ab2b.data(), v2e.codes.data(), m.coords.data(), m.metric.data(), max_length,      \
-      codes.data(), veto.data())
-  if (dim == 3 && md == 3)
-    OSH_OV(3, 3);
-  else if (dim == 3)
-    OSH_OV(3, 1);
-  else if (md == 2)
-    OSH_OV(2, 2);
-  else
-    OSH_OV(2, 1);
+      codes.data(), veto.data(), mixed ? need.data() : (I8*)nullptr)
+#define OSH_OV_ALL(F32)      \
+  if (dim == 3 && md == 3)   \
+    OSH_OV(3, 3, F32);       \
+  else if (dim == 3)         \
+    OSH_OV(3, 1, F32);       \
+  else if (md == 2)          \
+    OSH_OV(2, 2, F32);       \
+  else                       \
+    OSH_OV(2, 1, F32)
+  if (mixed) {
+    OSH_OV_ALL(true);
+  }
+  OSH_OV_ALL(false);
+#undef OSH_OV_ALL
 #undef OSH_OV
   OSH_LAUNCH(apply_vetoes, ncands, c, cands.data(), veto.data(), codes.data());
   // sides without a candidate left are skipped by every sweep below, and the
@@ -441,21 +503,32 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   LOs const& cv2v = verts_of(c, m, dim);
   Adj v2c = marked_up(c, m, 0, dim, vmarks, nullptr, false);
   Reals partial(c, nlanes);
-#define OSH_CQ(D, M)                                                                     \
-  OSH_LAUNCH((coarsen_quality_lanes<D, M>), nlanes, c, cands.data(), ev2v, cv2v.data(),     \
-      v2c.a2ab.data(), v2c.ab2b.data(), v2c.codes.data(), m.coords.data(), m.metric.data(), \
-      codes.data(), partial.data())
-  if (dim == 3 && md == 3)
-    OSH_CQ(3, 3);
-  else if (dim == 3)
-    OSH_CQ(3, 1);
-  else if (md == 2)
-    OSH_CQ(2, 2);
-  else
-    OSH_CQ(2, 1);
+  Bytes unsure;
+  if (mixed) unsure = fill_i8(c, ncands * 2, 0);
+#define OSH_CQ(D, M, F32)                                                                \
+  OSH_LAUNCH((coarsen_quality_lanes<D, M, F32>), nlanes, c, cands.data(), ev2v,             \
+      cv2v.data(), v2c.a2ab.data(), v2c.ab2b.data(), v2c.codes.data(), m.coords.data(),     \
+      m.metric.data(), codes.data(), mixed ? need.data() : (I8 const*)nullptr,              \
+      mixed ? unsure.data() : (I8*)nullptr, partial.data())
+#define OSH_CQ_ALL(F32)      \
+  if (dim == 3 && md == 3)   \
+    OSH_CQ(3, 3, F32);       \
+  else if (dim == 3)         \
+    OSH_CQ(3, 1, F32);       \
+  else if (md == 2)          \
+    OSH_CQ(2, 2, F32);       \
+  else                       \
+    OSH_CQ(2, 1, F32)
+  if (mixed) {
+    OSH_CQ_ALL(true);
+    OSH_LAUNCH(screen_quality_sides, ncands * 2, c, cands.data(), codes.data(), unsure.data(),
+        partial.data(), o.min_quality_allowed, need.data());
+  }
+  OSH_CQ_ALL(false);
+#undef OSH_CQ_ALL
 #undef OSH_CQ
-  OSH_LAUNCH(combine_quality_lanes, ncands, c, cands.data(), codes.data(), partial.data(),
-      quals.data());
+  OSH_LAUNCH(combine_quality_lanes, ncands, c, cands.data(), codes.data(),
+      mixed ? need.data() : (I8 const*)nullptr, partial.data(), quals.data());
   Reals vold;
   if (improve) {
     // read only at collapsing vertices, whose rows are built

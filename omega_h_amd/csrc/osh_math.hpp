@@ -803,4 +803,144 @@ OSH_HD double element_quality(double const* const* p, double const* const* ms) {
   return num / msl;
 }
 
+// ------------------------------------------------------------ FP32 screens
+// Single-precision estimates of element_quality and edge_length for the
+// mixed-precision form (no counterpart in the reference).  They only screen:
+// a cavity is rejected on an estimate that falls short of its threshold by
+// more than kScreenBand (relative), and every other cavity is evaluated
+// again in FP64, so decisions are those of the FP64 path.  Vertex
+// differences are taken in FP64 and rounded once (coordinates near 1 would
+// otherwise lose the digits of short edges), the rest runs in FP32 with the
+// hardware reciprocal, square root, log2 and exp2.  Their relative error
+// stays below 1e-4 for elements and metrics of condition number up to 1e3,
+// a tenth of the band.
+namespace f32 {
+
+constexpr float kScreenBand = 1e-3f;
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// symmetric metric of M = 1 (isotropic), 2 or 3 from Omega_h's layout
+template <int M>
+__device__ __forceinline__ void load_symm(double const* v, float* out) {
+  constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
+#pragma unroll
+  for (int k = 0; k < nc; ++k) out[k] = float(v[k]);
+}
+
+template <int M>
+__device__ __forceinline__ float symm_det(float const* m) {
+  if (M == 1) return m[0];
+  if (M == 2) return m[0] * m[1] - m[2] * m[2];
+  // (xx, yy, zz, xy, yz, xz)
+  return m[0] * (m[1] * m[2] - m[4] * m[4]) - m[3] * (m[3] * m[2] - m[4] * m[5]) +
+         m[5] * (m[3] * m[4] - m[1] * m[5]);
+}
+
+template <int S, int M>
+__device__ __forceinline__ float metric_product(float const* m, float const* v) {
+  if (M == 1) {
+    float d = v[0] * v[0];
+    for (int i = 1; i < S; ++i) d += v[i] * v[i];
+    return m[0] * d;
+  }
+  if (M == 2) return m[0] * v[0] * v[0] + m[1] * v[1] * v[1] + 2.0f * m[2] * v[0] * v[1];
+  return m[0] * v[0] * v[0] + m[1] * v[1] * v[1] + m[2] * v[2] * v[2] +
+         2.0f * (m[3] * v[0] * v[1] + m[4] * v[1] * v[2] + m[5] * v[0] * v[2]);
+}
+
+// element_quality in FP32; *unsure is set when the max-determinant vertex
+// metric is not clear (another vertex's different metric within the band),
+// since the FP64 choice could then differ
+template <int D, int M>
+__device__ __forceinline__ float element_quality(double const* const* p, double const* const* ms,
+    bool* unsure) {
+  constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
+  float m[nc], mi[nc];
+  load_symm<M>(ms[0], m);
+  float maxdet = symm_det<M>(m);
+  float second = -3.4e38f;
+  for (int i = 1; i < D + 1; ++i) {
+    load_symm<M>(ms[i], mi);
+    float d = symm_det<M>(mi);
+    bool same = true;
+    for (int k = 0; k < nc; ++k) same = same && (mi[k] == m[k]);
+    if (same) continue;
+    if (d > maxdet) {
+      second = maxdet;
+      for (int k = 0; k < nc; ++k) m[k] = mi[k];
+      maxdet = d;
+    } else if (d > second) {
+      second = d;
+    }
+  }
+  if (!(maxdet - second > kScreenBand * maxdet)) *unsure = true;
+  float b[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) b[i][j] = float(p[i + 1][j] - p[0][j]);
+  float rs;
+  if (D == 3) {
+    float c0 = b[0][1] * b[1][2] - b[0][2] * b[1][1];
+    float c1 = b[0][2] * b[1][0] - b[0][0] * b[1][2];
+    float c2 = b[0][0] * b[1][1] - b[0][1] * b[1][0];
+    rs = (c0 * b[2][0] + c1 * b[2][1] + c2 * b[2][2]) * (1.0f / 6.0f);
+  } else {
+    rs = (b[0][0] * b[1][1] - b[0][1] * b[1][0]) * 0.5f;
+  }
+  float dm = maxdet;
+  float f;
+  if (D == 3 && M == 3)
+    f = sqrt(dm);
+  else if (D == 3 && M == 1)
+    f = sqrt(dm * (dm * dm));
+  else if (D == 2 && M == 2)
+    f = sqrt(dm);
+  else
+    f = dm;
+  float s = rs * f;
+  if (s < 0.0f) return s;
+  constexpr int nedges = (D == 3) ? 6 : 3;
+  float ev[6][3];
+  for (int j = 0; j < 3; ++j) {
+    ev[0][j] = b[0][j];
+    ev[1][j] = b[1][j] - b[0][j];
+    ev[2][j] = -b[1][j];
+    ev[3][j] = b[2][j];
+    ev[4][j] = b[2][j] - b[0][j];
+    ev[5][j] = b[2][j] - b[1][j];
+  }
+  float msl = 0.0f;
+  for (int i = 0; i < nedges; ++i) msl += metric_product<D, M>(m, ev[i]);
+  msl = msl * (1.0f / nedges);
+  constexpr float eq = (D == 3) ? 0.1178511301977579f : 0.4330127018922193f;
+  float x = s * (1.0f / eq);
+  // cbrt(x * x) = 2^(2/3 log2 x); x = 0 gives 0
+  float num = (D == 3) ? exp2((2.0f / 3.0f) * log2(x)) : x;
+  return num * rcp(msl);
+}
+
+// metric_edge_length in FP32: near-equal end lengths (within 2 %) take the
+// arithmetic mean, which is within 3.4e-5 of the logarithmic mean there
+template <int S, int M>
+__device__ __forceinline__ float edge_length(double const* p0, double const* p1,
+    double const* m0, double const* m1) {
+  constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
+  float v[3] = {0, 0, 0};
+  for (int i = 0; i < S; ++i) v[i] = float(p1[i] - p0[i]);
+  float a[nc], b[nc];
+  load_symm<M>(m0, a);
+  load_symm<M>(m1, b);
+  float la = sqrt(metric_product<S, M>(a, v));
+  float lb = sqrt(metric_product<S, M>(b, v));
+  float hi = la > lb ? la : lb;
+  float d = la - lb;
+  if (!(d > 0.02f * hi) && !(-d > 0.02f * hi)) return 0.5f * (la + lb);
+  return d * rcp(log2(la * rcp(lb)) * 0.6931471805599453f);
+}
+
+}  // namespace f32
+
 }  // namespace osh

@@ -87,6 +87,40 @@ __global__ void __launch_bounds__(kBlock) elem_qualities_kernel(LO n, LO const* 
   out[i] = quality_of<D, M>(elems ? elems[i] : i, ev2v, coords, metric);
 }
 
+// the FP32 screening estimates of every edge length and element quality
+// (the mixed-precision form's screens; reported against the FP64 values)
+template <int S, int M>
+__global__ void edge_lengths32_kernel(LO n, LO const* ev2v, Real const* coords,
+    Real const* metric, Real* out) {
+  LO e = thread_index();
+  if (e >= n) return;
+  constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
+  double p0[S], p1[S], m0[nc], m1[nc];
+  load_vertex<S, nc>(coords, metric, ev2v[2 * e], p0, m0);
+  load_vertex<S, nc>(coords, metric, ev2v[2 * e + 1], p1, m1);
+  out[e] = double(f32::edge_length<S, M>(p0, p1, m0, m1));
+}
+
+template <int D, int M>
+__global__ void elem_qualities32_kernel(LO n, LO const* ev2v, Real const* coords,
+    Real const* metric, Real* out, I8* unsure) {
+  LO e = thread_index();
+  if (e >= n) return;
+  constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
+  double p[D + 1][D];
+  double m[D + 1][nc];
+  double const* pp[D + 1];
+  double const* mm[D + 1];
+  for (int k = 0; k < D + 1; ++k) {
+    load_vertex<D, nc>(coords, metric, ev2v[e * (D + 1) + k], p[k], m[k]);
+    pp[k] = p[k];
+    mm[k] = m[k];
+  }
+  bool u = false;
+  out[e] = double(f32::element_quality<D, M>(pp, mm, &u));
+  unsure[e] = u ? 1 : 0;
+}
+
 // the length (WHICH 1) or quality (WHICH 0) tag of a rebuilt mesh in one
 // launch: threads [0, nsame) copy the old value of a same entity, the rest
 // measure a product on the new mesh
@@ -183,6 +217,32 @@ void launch_elem_qualities(Ctx& c, int dim, int md, LO n, LO const* elems, LO co
     OSH_LAUNCH((elem_qualities_kernel<2, 2>), n, c, elems, ev2v, coords, metric, out);
   else
     OSH_LAUNCH((elem_qualities_kernel<2, 1>), n, c, elems, ev2v, coords, metric, out);
+}
+
+void screen_estimates(Ctx& c, DMesh& m, Reals& lengths32, Reals& quals32, Bytes& unsure) {
+  int dim = m.dim, md = metric_dim(m);
+  LO ne = m.nents[1], nt = m.nents[dim];
+  lengths32 = Reals(c, ne);
+  quals32 = Reals(c, nt);
+  unsure = Bytes(c, nt);
+  LO const* ev2v = m.down[1].data();
+  LO const* tv2v = verts_of(c, m, dim).data();
+  Real const* x = m.coords.data();
+  Real const* mt = m.metric.data();
+#define OSH_EST(D, M)                                                                    \
+  OSH_LAUNCH((edge_lengths32_kernel<D, M>), ne, c, ev2v, x, mt, lengths32.data());          \
+  OSH_LAUNCH((elem_qualities32_kernel<D, M>), nt, c, tv2v, x, mt, quals32.data(),           \
+      unsure.data())
+  if (dim == 3 && md == 3) {
+    OSH_EST(3, 3);
+  } else if (dim == 3) {
+    OSH_EST(3, 1);
+  } else if (md == 2) {
+    OSH_EST(2, 2);
+  } else {
+    OSH_EST(2, 1);
+  }
+#undef OSH_EST
 }
 
 void measure_edges(Ctx& c, DMesh& m, LO n, LO const* edges, Real* out) {

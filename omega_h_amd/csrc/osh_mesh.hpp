@@ -124,6 +124,10 @@ void launch_edge_lengths(Ctx& c, int dim, int md, LO n, LO const* edges, LO cons
 void launch_elem_qualities(Ctx& c, int dim, int md, LO n, LO const* elems, LO const* ev2v,
     Real const* coords, Real const* metric, Real* out);
 void measure_edges(Ctx& c, DMesh& m, LO n, LO const* edges, Real* out);
+// the FP32 screening estimates of the mixed-precision form for every edge
+// length and element quality, with the elements whose max-determinant
+// metric was not clear marked in `unsure`
+void screen_estimates(Ctx& c, DMesh& m, Reals& lengths32, Reals& quals32, Bytes& unsure);
 void measure_elems(Ctx& c, DMesh& m, LO n, LO const* elems, Real* out);
 // the length (d = 1) or quality (d = dim) tag of a rebuilt mesh: same
 // entities copy old values, products are measured on the new mesh

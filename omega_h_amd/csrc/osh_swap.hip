@@ -82,6 +82,40 @@ __device__ double tet_quality(LO const* v, Real const* coords, Real const* metri
   return element_quality<3, M>(pp, ms);
 }
 
+// the FP32 screen of tet_quality (mixed precision); *unsure as in
+// f32::element_quality
+template <int M>
+__device__ float tet_quality32(LO const* v, Real const* coords, Real const* metric,
+    bool* unsure) {
+  constexpr int nc = (M == 1) ? 1 : 6;
+  double p[4][3];
+  double m[4][nc];
+  double const* pp[4];
+  double const* ms[4];
+  for (int k = 0; k < 4; ++k) {
+    for (int j = 0; j < 3; ++j) p[k][j] = coords[v[k] * 3 + j];
+    for (int j = 0; j < nc; ++j) m[k][j] = metric[v[k] * nc + j];
+    pp[k] = p[k];
+    ms[k] = m[k];
+  }
+  return f32::element_quality<3, M>(pp, ms, unsure);
+}
+
+template <int S, int M>
+__device__ float length_between32(LO a, LO b, Real const* coords, Real const* metric) {
+  constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
+  double p0[S], p1[S], m0[nc], m1[nc];
+  for (int k = 0; k < S; ++k) {
+    p0[k] = coords[a * S + k];
+    p1[k] = coords[b * S + k];
+  }
+  for (int k = 0; k < nc; ++k) {
+    m0[k] = metric[a * nc + k];
+    m1[k] = metric[b * nc + k];
+  }
+  return f32::edge_length<S, M>(p0, p1, m0, m1);
+}
+
 template <int S, int M>
 __device__ double length_between(LO a, LO b, Real const* coords, Real const* metric) {
   constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
@@ -174,42 +208,63 @@ __global__ void swap3d_loops(LO nt, LO const* cands, LO const* e2t_a2ab, LO cons
   nedges[cand] = carried ? 0 : tb->nuedges[size];
 }
 
-template <int M>
+// F32 (mixed precision): FP32 estimates, and a candidate with an element
+// whose max-determinant metric is not clear is marked in `unsure`; with
+// `need` given, the FP64 sweep evaluates only the marked candidates
+template <int M, bool F32>
 __global__ void swap3d_tri_quals(LO nt, LO const* ntris, LO const* lsize, LO const* leev,
-    LO const* lverts, Real const* coords, Real const* metric, Tables const* tb, Real* tq) {
+    LO const* lverts, Real const* coords, Real const* metric, Tables const* tb,
+    I8 const* need, I8* unsure, Real* tq) {
   // a fixed stride of kMaxUniqueTris slots per candidate
   LO p = thread_index();
   if (p >= nt) return;
   LO cand = p / swap::kMaxUniqueTris;
   int ut = p % swap::kMaxUniqueTris;
   if (ut >= ntris[cand]) return;
+  if (!F32 && need && !need[cand]) return;
   int n = lsize[cand];
   LO tv[4];
   for (int k = 0; k < 3; ++k) tv[k] = lverts[cand * swap::kMaxLoop + tb->tris[n][ut][k]];
   double tri_minqual = 1.0;
+  float minq32 = 1.0f;
+  bool unclear = false;
   for (int tri_tet = 0; tri_tet < 2; ++tri_tet) {
     tv[3] = leev[cand * 2 + 1 - tri_tet];
-    tri_minqual = math::min2(tri_minqual, tet_quality<M>(tv, coords, metric));
+    if constexpr (F32) {
+      float q = tet_quality32<M>(tv, coords, metric, &unclear);
+      minq32 = (q < minq32) ? q : minq32;
+    } else {
+      tri_minqual = math::min2(tri_minqual, tet_quality<M>(tv, coords, metric));
+    }
     LO t = tv[1];
     tv[1] = tv[2];
     tv[2] = t;
   }
-  tq[p] = tri_minqual;
+  if constexpr (F32) {
+    if (unclear) unsure[cand] = 1;
+    tq[p] = double(minq32);
+  } else {
+    tq[p] = tri_minqual;
+  }
 }
 
-template <int M>
+template <int M, bool F32>
 __global__ void swap3d_edge_lens(LO nt, LO const* nedges, LO const* lsize, LO const* lverts,
-    Real const* coords, Real const* metric, Tables const* tb, Real* el) {
+    Real const* coords, Real const* metric, Tables const* tb, I8 const* need, Real* el) {
   // a fixed stride of kMaxUniqueEdges slots per candidate
   LO q = thread_index();
   if (q >= nt) return;
   LO cand = q / swap::kMaxUniqueEdges;
   int ue = q % swap::kMaxUniqueEdges;
   if (ue >= nedges[cand]) return;
+  if (!F32 && need && !need[cand]) return;
   int n = lsize[cand];
   LO const* lv = lverts + cand * swap::kMaxLoop;
-  el[q] = length_between<3, M>(lv[tb->uedges[n][ue][0]], lv[tb->uedges[n][ue][1]], coords,
-      metric);
+  LO a = lv[tb->uedges[n][ue][0]], b = lv[tb->uedges[n][ue][1]];
+  if constexpr (F32)
+    el[q] = double(length_between32<3, M>(a, b, coords, metric));
+  else
+    el[q] = length_between<3, M>(a, b, coords, metric);
 }
 
 // one thread per (candidate, triangulation) on a fixed stride of
@@ -223,7 +278,7 @@ constexpr int kMaxMeshes = 42;
 constexpr int kChooseLanes = 8;
 
 __global__ void swap3d_mesh_quals(LO nt, LO const* lsize, LO const* ntris, Real const* tq,
-    Real const* el, Real max_length, Tables const* tb, Real* mq) {
+    Real const* el, Real max_length, Tables const* tb, I8 const* need, Real* mq) {
   LO t = thread_index();
   if (t >= nt) return;
   LO cand = t / kMaxMeshes;
@@ -231,6 +286,7 @@ __global__ void swap3d_mesh_quals(LO nt, LO const* lsize, LO const* ntris, Real 
   int n = lsize[cand];
   if (n > swap::kMaxLoop || mesh >= tb->mesh_counts[n]) return;
   if (ntris[cand] == 0) return;  // a carried-over choice
+  if (need && !need[cand]) return;  // rejected by the mixed-precision screen
   int nmesh_tris = tb->mesh_sizes[n];
   int nmesh_edges = tb->nedges[n];
   Real const* ctq = tq + LO(cand) * swap::kMaxUniqueTris;
@@ -243,9 +299,57 @@ __global__ void swap3d_mesh_quals(LO nt, LO const* lsize, LO const* ntris, Real 
   mq[t] = q;
 }
 
+// the mixed-precision screen of one candidate (kChooseLanes threads): the
+// best FP32 triangulation value, where an interior edge counts as too long
+// only when its FP32 length exceeds max_length by more than the band, is an
+// upper estimate of the FP64 choice's quality; the candidate needs FP64
+// unless that estimate is below the edge's current minimum quality by more
+// than the band (with every element's metric clear and that minimum
+// positive)
+__global__ void swap3d_screen(LO nt, LO const* cands, LO const* lsize, LO const* ntris,
+    Real const* tq, Real const* el, Real max_length, Tables const* tb, LO const* e2t_a2ab,
+    LO const* e2t_ab2b, Real const* elem_quals, I8 const* unsure, I8* need) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  LO cand = t / kChooseLanes;
+  int lane = t % kChooseLanes;
+  LO edge = cands[cand];
+  int n = lsize[cand];
+  bool open = ntris[cand] != 0 && n <= swap::kMaxLoop;  // not carried over, a loop
+  double best = -1.0;
+  if (open) {
+    int nmesh_tris = tb->mesh_sizes[n];
+    int nmesh_edges = tb->nedges[n];
+    Real const* ctq = tq + LO(cand) * swap::kMaxUniqueTris;
+    Real const* cel = el + LO(cand) * swap::kMaxUniqueEdges;
+    double lmax = max_length * (1.0 + double(f32::kScreenBand));
+    for (int mesh = lane; mesh < tb->mesh_counts[n]; mesh += kChooseLanes) {
+      double q = 1.0;
+      for (int mt = 0; mt < nmesh_tris; ++mt)
+        q = math::min2(q, ctq[tb->meshes[n][mesh * nmesh_tris + mt]]);
+      for (int me = 0; me < nmesh_edges; ++me)
+        if (cel[tb->medges[n][mesh * nmesh_edges + me]] > lmax) q = -1.0;
+      best = math::max2(best, q);
+    }
+  }
+  double old = 1.79769313486231570815e+308;
+  for (LO j = e2t_a2ab[edge] + lane; j < e2t_a2ab[edge + 1]; j += kChooseLanes)
+    old = math::min2(old, elem_quals[e2t_ab2b[j]]);
+#pragma unroll
+  for (int d = kChooseLanes / 2; d > 0; d >>= 1) {
+    best = math::max2(best, __shfl_xor(best, d, 64));
+    old = math::min2(old, __shfl_xor(old, d, 64));
+  }
+  if (lane != 0) return;
+  bool reject = open && !unsure[cand] && old > 0.0 &&
+                best < old * (1.0 - double(f32::kScreenBand));
+  need[cand] = reject ? 0 : 1;
+}
+
 __global__ void swap3d_choose(LO nt, LO const* cands, LO const* lsize, Real const* mq,
     LO const* e2t_a2ab, LO const* e2t_ab2b, Real const* elem_quals, Tables const* tb,
-    I8* configs, Real* quals, I8* keep, I8* c_valid, I8* c_cfg, I8* c_keep, Real* c_q) {
+    I8 const* need, I8* configs, Real* quals, I8* keep, I8* c_valid, I8* c_cfg, I8* c_keep,
+    Real* c_q) {
   // kChooseLanes threads per candidate share its triangulations and tets
   LO t = thread_index();
   if (t >= nt) return;
@@ -258,6 +362,20 @@ __global__ void swap3d_choose(LO nt, LO const* cands, LO const* lsize, Real cons
       if (c_keep[edge]) {
         keep[edge] = 1;
         quals[edge] = c_q[edge];
+      }
+    }
+    return;
+  }
+  if (need && !need[cand]) {
+    // the mixed-precision screen showed that no triangulation improves on
+    // the edge's elements: not kept, as the FP64 choice would decide
+    if (lane == 0) {
+      configs[edge] = -1;
+      if (c_valid) {
+        c_cfg[edge] = -1;
+        c_keep[edge] = 0;
+        c_q[edge] = -1.0;
+        c_valid[edge] = 1;
       }
     }
     return;
@@ -739,23 +857,43 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
   // candidate, so that neither a scan nor a sync sizes them
   LO ntp = ncands * swap::kMaxUniqueTris, nep = ncands * swap::kMaxUniqueEdges;
   Reals tq(c, ntp), el(c, nep);
-  if (md == 3) {
-    OSH_LAUNCH(swap3d_tri_quals<3>, ntp, c, ntris.data(), lsize.data(), leev.data(),
-        lverts.data(), m.coords.data(), m.metric.data(), tb, tq.data());
-    OSH_LAUNCH(swap3d_edge_lens<3>, nep, c, nedges.data(), lsize.data(), lverts.data(),
-        m.coords.data(), m.metric.data(), tb, el.data());
-  } else {
-    OSH_LAUNCH(swap3d_tri_quals<1>, ntp, c, ntris.data(), lsize.data(), leev.data(),
-        lverts.data(), m.coords.data(), m.metric.data(), tb, tq.data());
-    OSH_LAUNCH(swap3d_edge_lens<1>, nep, c, nedges.data(), lsize.data(), lverts.data(),
-        m.coords.data(), m.metric.data(), tb, el.data());
+  // mixed precision: an FP32 screen of every candidate first, then the FP64
+  // sweeps, choice included, only for the candidates it left open
+  bool mixed = o.precision == 1;
+  Bytes need, unsure;
+  if (mixed) {
+    need = Bytes(c, ncands);
+    unsure = fill_i8(c, ncands, 0);
   }
+  I8 const* needp = mixed ? need.data() : nullptr;
+#define OSH_SWQ(M, F32)                                                                  \
+  OSH_LAUNCH((swap3d_tri_quals<M, F32>), ntp, c, ntris.data(), lsize.data(), leev.data(),  \
+      lverts.data(), m.coords.data(), m.metric.data(), tb, needp,                         \
+      mixed ? unsure.data() : (I8*)nullptr, tq.data());                                   \
+  OSH_LAUNCH((swap3d_edge_lens<M, F32>), nep, c, nedges.data(), lsize.data(),              \
+      lverts.data(), m.coords.data(), m.metric.data(), tb, needp, el.data())
+  if (mixed) {
+    if (md == 3) {
+      OSH_SWQ(3, true);
+    } else {
+      OSH_SWQ(1, true);
+    }
+    OSH_LAUNCH(swap3d_screen, ncands * kChooseLanes, c, cands.data(), lsize.data(),
+        ntris.data(), tq.data(), el.data(), o.max_length_allowed, tb, e2t.a2ab.data(),
+        e2t.ab2b.data(), eq.data(), unsure.data(), need.data());
+  }
+  if (md == 3) {
+    OSH_SWQ(3, false);
+  } else {
+    OSH_SWQ(1, false);
+  }
+#undef OSH_SWQ
   Reals mq(c, ncands * kMaxMeshes);
   OSH_LAUNCH(swap3d_mesh_quals, ncands * kMaxMeshes, c, lsize.data(), ntris.data(),
-      tq.data(), el.data(), o.max_length_allowed, tb, mq.data());
+      tq.data(), el.data(), o.max_length_allowed, tb, needp, mq.data());
   OSH_LAUNCH(swap3d_choose, ncands * kChooseLanes, c, cands.data(), lsize.data(), mq.data(),
       e2t.a2ab.data(),
-      e2t.ab2b.data(), eq.data(), tb, configs.data(), quals.data(), keep.data(),
+      e2t.ab2b.data(), eq.data(), tb, needp, configs.data(), quals.data(), keep.data(),
       cache ? cache->valid.data() : (I8*)nullptr, cache ? cache->cfg.data() : (I8*)nullptr,
       cache ? cache->keep.data() : (I8*)nullptr, cache ? cache->q.data() : (Real*)nullptr);
   bool any;

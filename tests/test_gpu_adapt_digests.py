@@ -6,7 +6,9 @@ mesh-changing pass: its kind, the element counts in and out, and FNV-1a
 digests of the vertex lists and classification, the coordinate bits, the
 metric bits and the edge length and element quality bits.  The device's
 adapt() (osh_adapt_digests, the same loop as osh_adapt with the swap choices
-carried between swap passes) must reproduce every record bit for bit.
+carried between swap passes) must reproduce every record bit for bit, in
+FP64 and in the mixed-precision form (FP32 cavity screens with every cavity
+not rejected by the screening band re-evaluated in FP64).
 """
 import pytest
 
@@ -27,8 +29,9 @@ def ctx():
     return A.Context(0)
 
 
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("name", list(METRICS))
-def test_adapt_matches_oracle_pass_by_pass(ctx, name):
+def test_adapt_matches_oracle_pass_by_pass(ctx, name, precision):
     dim, n, metric = METRICS[name]
     fx = F.load(name)
     m = A.Mesh.box(ctx, dim, n)
@@ -36,6 +39,7 @@ def test_adapt_matches_oracle_pass_by_pass(ctx, name):
     m.set_metric(met, nc)
     opts = ctx.default_opts(dim)
     opts.verbosity = 0
+    opts.precision = precision
     did, passes, digests = m.adapt_digests(opts)
     assert did == fx["adapted"]
     msg = F.first_difference(fx["passes"], F.rows_of(passes, digests))

@@ -55,11 +55,13 @@ def test_first_passes_match_the_oracle(ctx):
     compare_meshes(o, g)
 
 
-def test_full_adapt_matches_oracle_pass_by_pass(ctx):
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_full_adapt_matches_oracle_pass_by_pass(ctx, precision):
     fx = F.load("levelset_3d_64")
     m = levelset_box(A, ctx)
     opts = ctx.default_opts(3)
     opts.verbosity = 0
+    opts.precision = precision
     did, passes, digests = m.adapt_digests(opts)
     assert did == fx["adapted"]
     msg = F.first_difference(fx["passes"], F.rows_of(passes, digests))
