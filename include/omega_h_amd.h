@@ -332,6 +332,19 @@ int osh_server_wait(osh_server* server, int ticket, osh_serve_result* out);
 /* cavity kernels on host arrays, for tests and diagnostics:
  * refine_qualities src/Omega_h_refine_qualities.cpp:88-108 */
 int osh_refine_qualities(osh_mesh* mesh, int n, const int* cands, double* out);
+/* coarsen_qualities(mesh, cands2edges, cand_codes),
+ * src/Omega_h_coarsen_qualities.cpp:65-88, on device pointers: out[2i + col]
+ * is the minimum quality of the elements around the collapsing end of edge
+ * cands[i] (direction col: bit col of cand_codes[i], as in
+ * src/Omega_h_collapse.hpp:11-28) after the collapse, -1 where that
+ * direction is not asked for.  Asynchronous like osh_measure_edges_metric
+ * (the first call on a mesh builds its vertex -> element adjacency and
+ * waits).  No two candidates may name the same edge.  precision OSH_MIXED
+ * screens in FP32 first and leaves -1 where the estimate is below
+ * min_quality by more than the screening band (1e-3 relative); every other
+ * value is the FP64 one. */
+int osh_coarsen_qualities(osh_mesh* mesh, int ncands, const int* cands,
+    const signed char* cand_codes, double* out, int precision, double min_quality);
 /* swap3d::find_loop src/Omega_h_swap3d_loop.hpp:24-78 (loop_verts holds 7) */
 int osh_find_loop(osh_mesh* mesh, int edge, int* eev2v, int* loop_verts,
     int* size);

@@ -150,6 +150,7 @@ def load(path=None):
         "osh_offset_scans": [P, I, P, IP, P],
         "osh_refine_qualities": [P, I, IP, DP],
         "osh_find_loop": [P, I, IP, IP, IP],
+        "osh_coarsen_qualities": [P, I, P, P, P, I, C.c_double],
         "osh_find_indset": [P, I, DP, BP, BP],
         "osh_eval_math": [P, I, I, DP, DP],
         "osh_swap3d_tables": [I, IP, IP, IP, IP, IP],
@@ -457,6 +458,23 @@ class Mesh:
         check(self.lib.osh_refine_qualities(self.h, len(c), _p(c, C.c_int),
                                             _p(out, C.c_double)), self.lib)
         return out
+
+    def coarsen_qualities_ptr(self, ncands, cands_ptr, codes_ptr, out_ptr, precision=0,
+                              min_quality=0.0):
+        """coarsen_qualities on device pointers, ordered on the context's stream."""
+        check(self.lib.osh_coarsen_qualities(self.h, ncands, cands_ptr, codes_ptr, out_ptr,
+                                             precision, min_quality), self.lib)
+
+    def coarsen_qualities(self, cands, codes, precision=0, min_quality=0.0):
+        """coarsen_qualities (src/Omega_h_coarsen_qualities.cpp:65-88) of host
+        arrays: two values per candidate edge."""
+        from .hip import DeviceArray
+        c = DeviceArray(np.ascontiguousarray(cands, dtype=np.int32))
+        k = DeviceArray(np.ascontiguousarray(codes, dtype=np.int8))
+        out = DeviceArray(dtype=np.float64, n=2 * len(cands))
+        self.coarsen_qualities_ptr(len(cands), c.ptr, k.ptr, out.ptr, precision, min_quality)
+        self.ctx.synchronize()
+        return out.numpy()
 
     def find_loop(self, edge):
         eev = np.zeros(2, dtype=np.int32)

@@ -1007,6 +1007,17 @@ int osh_refine_qualities(osh_mesh* mesh, int n, const int* cands, double* out) {
   });
 }
 
+int osh_coarsen_qualities(osh_mesh* mesh, int ncands, const int* cands,
+    const signed char* cand_codes, double* out, int precision, double min_quality) {
+  return guarded([&] {
+    check_metric(mesh);
+    OSH_CHECK(ncands >= 0 && (ncands == 0 || (cands && cand_codes && out)), "null argument");
+    OSH_CHECK(precision == 0 || precision == 1, "precision must be OSH_FP64 or OSH_MIXED");
+    osh::coarsen_qualities(mesh->owner->ctx, mesh->m, ncands, cands,
+        reinterpret_cast<osh::I8 const*>(cand_codes), out, precision, min_quality);
+  });
+}
+
 int osh_find_loop(osh_mesh* mesh, int edge, int* eev2v, int* loop_verts, int* size) {
   return guarded([&] {
     check_mesh(mesh);

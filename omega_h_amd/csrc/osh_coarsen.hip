@@ -273,6 +273,28 @@ __global__ void screen_quality_sides(LO nsides, LO const* cands, I8 const* codes
   need[side] = reject ? 0 : 1;
 }
 
+// combine_quality_lanes with one output row per candidate (the layout of
+// the reference's coarsen_qualities, two values per candidate)
+__global__ void combine_to_cands(LO ncands, LO const* cands, I8 const* codes, I8 const* need,
+    Real const* partial, Real* out) {
+  LO i = thread_index();
+  if (i >= ncands) return;
+  I8 code = codes[cands[i]];
+  for (int col = 0; col < 2; ++col) {
+    double q = -1.0;
+    if (collapses(code, col) && (!need || need[i * 2 + col])) {
+      q = 1.0;
+      for (int l = 0; l < kLanes; ++l) q = math::min2(q, partial[(i * 2 + col) * kLanes + l]);
+    }
+    out[i * 2 + col] = q;
+  }
+}
+
+__global__ void scatter_codes(LO n, LO const* cands, I8 const* cand_codes, I8* codes) {
+  LO i = thread_index();
+  if (i < n) codes[cands[i]] = cand_codes[i];
+}
+
 // the vertices that would collapse along some candidate edge
 __global__ void mark_collapsing_verts(LO ncands, LO const* cands, LO const* ev2v,
     I8 const* codes, I8* vmarks) {
@@ -670,6 +692,50 @@ bool coarsen_slivers(Ctx& c, DMesh& m, AdaptOptions const& o) {
   Bytes marks = mark_sliver_layers(c, m, o.min_quality_desired, o.nsliver_layers);
   Bytes vm = mark_down(c, m, m.dim, 0, marks);
   return coarsen_verts(c, m, o, vm, true, true);
+}
+
+void coarsen_qualities(Ctx& c, DMesh& m, LO ncands, LO const* cands, I8 const* cand_codes,
+    Real* out, int precision, Real qmin) {
+  int dim = m.dim;
+  int md = (m.metric_ncomps == 1) ? 1 : dim;
+  if (ncands == 0) return;
+  LO const* ev2v = m.down[1].data();
+  LOs const& cv2v = verts_of(c, m, dim);
+  Adj const& v2c = ask_up(c, m, 0, dim);
+  Bytes codes = fill_i8(c, m.nents[1], kDont);
+  OSH_LAUNCH(scatter_codes, ncands, c, cands, cand_codes, codes.data());
+  LO nlanes = ncands * 2 * kLanes;
+  Reals partial(c, nlanes);
+  bool mixed = precision == 1;
+  Bytes need, unsure;
+  if (mixed) {
+    need = Bytes(c, ncands * 2);
+    unsure = fill_i8(c, ncands * 2, 0);
+  }
+#define OSH_CQ(D, M, F32)                                                                \
+  OSH_LAUNCH((coarsen_quality_lanes<D, M, F32>), nlanes, c, cands, ev2v, cv2v.data(),       \
+      v2c.a2ab.data(), v2c.ab2b.data(), v2c.codes.data(), m.coords.data(), m.metric.data(), \
+      codes.data(), mixed ? need.data() : (I8 const*)nullptr,                               \
+      mixed ? unsure.data() : (I8*)nullptr, partial.data())
+#define OSH_CQ_ALL(F32)      \
+  if (dim == 3 && md == 3)   \
+    OSH_CQ(3, 3, F32);       \
+  else if (dim == 3)         \
+    OSH_CQ(3, 1, F32);       \
+  else if (md == 2)          \
+    OSH_CQ(2, 2, F32);       \
+  else                       \
+    OSH_CQ(2, 1, F32)
+  if (mixed) {
+    OSH_CQ_ALL(true);
+    OSH_LAUNCH(screen_quality_sides, ncands * 2, c, cands, codes.data(), unsure.data(),
+        partial.data(), qmin, need.data());
+  }
+  OSH_CQ_ALL(false);
+#undef OSH_CQ_ALL
+#undef OSH_CQ
+  OSH_LAUNCH(combine_to_cands, ncands, c, cands, codes.data(),
+      mixed ? need.data() : (I8 const*)nullptr, partial.data(), out);
 }
 
 }  // namespace osh

@@ -130,6 +130,14 @@ bool swap_edges(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache = null
 
 // known-answer hooks used by the tests
 Reals refine_qualities(Ctx& c, DMesh& m, LOs const& cands);
+// coarsen_qualities (src/Omega_h_coarsen_qualities.cpp:65-88) on device
+// arrays, stream-ordered: out[2 i + col] is the quality after collapsing
+// candidate i's edge in direction col (-1 where cand_codes[i] does not
+// collapse that way).  precision 1 screens in FP32 first: a direction whose
+// estimate falls below qmin by more than the band gets -1 instead of its
+// FP64 value (the filter that follows rejects it either way).
+void coarsen_qualities(Ctx& c, DMesh& m, LO ncands, LO const* cands, I8 const* cand_codes,
+    Real* out, int precision, Real qmin);
 void find_loops(Ctx& c, DMesh& m, LOs const& edges, LOs& eev2v, LOs& loop_verts, LOs& sizes);
 
 // the batched form (osh_batch.hip): meshes joined into one disjoint mesh

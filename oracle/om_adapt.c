@@ -859,6 +859,53 @@ static signed char dont_collapse(signed char code, int col_v) {
   return (signed char)(code & ~(1 << col_v));
 }
 
+/* coarsen_qualities for one edge, src/Omega_h_coarsen_qualities.cpp:16-63:
+ * per collapse direction the minimum quality of the collapsing vertex's
+ * elements that survive, with that vertex moved onto the other end; -1 for
+ * a direction that does not collapse */
+static void coarsen_edge_qualities(om_mesh* m, int e, signed char code, double* out) {
+  int dim = m->dim;
+  int nc = m->metric_ncomps;
+  int mdim = (nc == 1) ? 1 : dim;
+  int const* ev2v = m->down[1];
+  int const* cv2v = om_verts_of(m, dim);
+  om_graph const* v2c = om_ask_up(m, 0, dim);
+  out[0] = out[1] = -1.0;
+  if (code == DONT_COLLAPSE) return;
+  for (int col = 0; col < 2; ++col) {
+    if (!collapses(code, col)) continue;
+    int v_col = ev2v[e * 2 + col];
+    int v_onto = ev2v[e * 2 + (1 - col)];
+    double minqual = 1.0;
+    for (int vc = v2c->a2ab[v_col]; vc < v2c->a2ab[v_col + 1]; ++vc) {
+      int c = v2c->ab2b[vc];
+      int ccv_col = om_code_which_down(v2c->codes[vc]);
+      int cv[4];
+      int will_die = 0;
+      for (int k = 0; k < dim + 1; ++k) {
+        cv[k] = cv2v[c * (dim + 1) + k];
+        if (k != ccv_col && cv[k] == v_onto) will_die = 1;
+      }
+      if (will_die) continue;
+      cv[ccv_col] = v_onto;
+      double const* p[4];
+      double const* ms[4];
+      for (int k = 0; k < dim + 1; ++k) {
+        p[k] = m->coords + cv[k] * dim;
+        ms[k] = m->metric + cv[k] * nc;
+      }
+      double q = om_element_quality(dim, mdim, p, ms);
+      minqual = min2(minqual, q);
+    }
+    out[col] = minqual;
+  }
+}
+
+void om_kat_coarsen_qualities(om_mesh* m, int n, int const* cands, signed char const* codes,
+    double* out) {
+  for (int i = 0; i < n; ++i) coarsen_edge_qualities(m, cands[i], codes[i], out + 2 * i);
+}
+
 static int coarsen(om_mesh* m, om_adapt_opts const* o, signed char* edge_codes,
     int overshoot_allowed, int improve) {
   int dim = m->dim;
@@ -944,40 +991,8 @@ static int coarsen(om_mesh* m, om_adapt_opts const* o, signed char* edge_codes,
   if (!anyc) return 0;
   /* coarsen_qualities */
   double* cq = (double*)om_malloc(sizeof(double) * (size_t)ne * 2);
-  int const* cv2v = om_verts_of(m, dim);
   om_graph const* v2c = om_ask_up(m, 0, dim);
-  for (int e = 0; e < ne; ++e) {
-    cq[e * 2] = cq[e * 2 + 1] = -1.0;
-    signed char code = edge_codes[e];
-    if (code == DONT_COLLAPSE) continue;
-    for (int col = 0; col < 2; ++col) {
-      if (!collapses(code, col)) continue;
-      int v_col = ev2v[e * 2 + col];
-      int v_onto = ev2v[e * 2 + (1 - col)];
-      double minqual = 1.0;
-      for (int vc = v2c->a2ab[v_col]; vc < v2c->a2ab[v_col + 1]; ++vc) {
-        int c = v2c->ab2b[vc];
-        int ccv_col = om_code_which_down(v2c->codes[vc]);
-        int cv[4];
-        int will_die = 0;
-        for (int k = 0; k < dim + 1; ++k) {
-          cv[k] = cv2v[c * (dim + 1) + k];
-          if (k != ccv_col && cv[k] == v_onto) will_die = 1;
-        }
-        if (will_die) continue;
-        cv[ccv_col] = v_onto;
-        double const* p[4];
-        double const* ms[4];
-        for (int k = 0; k < dim + 1; ++k) {
-          p[k] = m->coords + cv[k] * dim;
-          ms[k] = m->metric + cv[k] * nc;
-        }
-        double q = om_element_quality(dim, mdim, p, ms);
-        minqual = min2(minqual, q);
-      }
-      cq[e * 2 + col] = minqual;
-    }
-  }
+  for (int e = 0; e < ne; ++e) coarsen_edge_qualities(m, e, edge_codes[e], cq + e * 2);
   /* filter_coarsen_min_qual and filter_coarsen_improve */
   double const* eq = om_ask_qualities(m);
   double* vold = NULL;

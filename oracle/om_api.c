@@ -452,3 +452,34 @@ om_mesh* om_mesh_join(int nparts, om_mesh* const* parts) {
   free(part);
   return r;
 }
+
+/* a mesh from its arrays: nents[0..dim], the down adjacencies (and alignment
+ * codes for dim >= 2) of every dimension 1..dim, coordinates, metric and the
+ * classification of every dimension (copied) */
+om_mesh* om_mesh_from_arrays(int dim, int const* nents, int const* const* down,
+    signed char const* const* codes, double const* coords, int metric_ncomps,
+    double const* metric, int const* const* class_id, signed char const* const* class_dim) {
+  om_mesh* m = om_mesh_new(dim);
+  for (int d = 0; d <= dim; ++d) m->nents[d] = nents[d];
+  for (int d = 1; d <= dim; ++d) {
+    size_t n = (size_t)nents[d] * (d + 1);
+    m->down[d] = (int*)om_malloc(sizeof(int) * (n + 1));
+    memcpy(m->down[d], down[d], sizeof(int) * n);
+    if (d >= 2) {
+      m->down_codes[d] = (signed char*)om_malloc(n + 1);
+      memcpy(m->down_codes[d], codes[d], n);
+    }
+  }
+  m->coords = (double*)om_malloc(sizeof(double) * ((size_t)nents[0] * dim + 1));
+  memcpy(m->coords, coords, sizeof(double) * (size_t)nents[0] * dim);
+  m->metric_ncomps = metric_ncomps;
+  m->metric = (double*)om_malloc(sizeof(double) * ((size_t)nents[0] * metric_ncomps + 1));
+  memcpy(m->metric, metric, sizeof(double) * (size_t)nents[0] * metric_ncomps);
+  for (int d = 0; d <= dim; ++d) {
+    m->class_id[d] = (int*)om_malloc(sizeof(int) * ((size_t)nents[d] + 1));
+    memcpy(m->class_id[d], class_id[d], sizeof(int) * (size_t)nents[d]);
+    m->class_dim[d] = (signed char*)om_malloc((size_t)nents[d] + 1);
+    memcpy(m->class_dim[d], class_dim[d], (size_t)nents[d]);
+  }
+  return m;
+}

@@ -46,6 +46,11 @@ void om_mesh_free(om_mesh* m);
 /* one disjoint mesh of nparts meshes, part k's entities after those of
  * parts < k, the vertex tag "part" holding each vertex's part */
 om_mesh* om_mesh_join(int nparts, om_mesh* const* parts);
+/* a mesh from its arrays (down adjacencies and codes per dimension 1..dim,
+ * coordinates, metric, classification per dimension 0..dim) */
+om_mesh* om_mesh_from_arrays(int dim, int const* nents, int const* const* down,
+    signed char const* const* codes, double const* coords, int metric_ncomps,
+    double const* metric, int const* const* class_id, signed char const* const* class_dim);
 int om_mesh_dim(om_mesh* m);
 int om_mesh_nents(om_mesh* m, int d);
 void om_get_verts(om_mesh* m, int d, int* out);
@@ -130,6 +135,9 @@ void om_element_implied_metric(int dim, double const* const* p, double* out);
 /* limit_metric_gradation on the mesh's metric; returns the sweep count */
 int om_limit_metric_gradation(om_mesh* m, double max_rate, double tol);
 void om_kat_refine_qualities(om_mesh* m, int n, int const* cands, double* out);
+/* coarsen_qualities(mesh, cands2edges, cand_codes): two values per candidate */
+void om_kat_coarsen_qualities(om_mesh* m, int n, int const* cands, signed char const* codes,
+    double* out);
 int om_kat_find_loop(om_mesh* m, int edge, int* eev2v, int* loop_verts);
 void om_kat_indset(om_mesh* m, int d, double const* q, signed char const* c,
     signed char* out);

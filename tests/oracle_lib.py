@@ -67,6 +67,10 @@ def lib():
         L.om_mesh_free.argtypes = [P]
         L.om_mesh_join.restype = P
         L.om_mesh_join.argtypes = [C.c_int, C.POINTER(P)]
+        L.om_mesh_from_arrays.restype = P
+        L.om_mesh_from_arrays.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(P),
+                                          C.POINTER(P), C.POINTER(C.c_double), C.c_int,
+                                          C.POINTER(C.c_double), C.POINTER(P), C.POINTER(P)]
         L.om_mesh_dim.argtypes = [P]
         L.om_mesh_nents.argtypes = [P, C.c_int]
         L.om_get_verts.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
@@ -164,6 +168,8 @@ def lib():
         L.om_kat_intersect.argtypes = [C.c_int] + [C.POINTER(C.c_double)] * 3
         L.om_kat_refine_qualities.argtypes = [P, C.c_int, C.POINTER(C.c_int),
                                               C.POINTER(C.c_double)]
+        L.om_kat_coarsen_qualities.argtypes = [P, C.c_int, C.POINTER(C.c_int),
+                                                C.POINTER(C.c_byte), C.POINTER(C.c_double)]
         L.om_kat_find_loop.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.om_kat_indset.argtypes = [P, C.c_int, C.POINTER(C.c_double),
                                     C.POINTER(C.c_byte), C.POINTER(C.c_byte)]
@@ -205,6 +211,37 @@ class Mesh:
         """One disjoint mesh of the parts (vertex tag "part" = part number)."""
         arr = (C.c_void_p * len(parts))(*[p.h for p in parts])
         return cls(lib().om_mesh_join(len(parts), arr))
+
+    @classmethod
+    def copy_of(cls, other):
+        """An oracle mesh with the arrays of another mesh object (the library's
+        or the oracle's): down adjacencies and codes, coordinates, metric and
+        classification."""
+        d = other.dim
+        nents = (C.c_int * 4)(*[other.nents(k) if k <= d else 0 for k in range(4)])
+        keep = []
+        downs = (C.c_void_p * 4)()
+        codes = (C.c_void_p * 4)()
+        for k in range(1, d + 1):
+            dn, cd = other.down(k)
+            dn = np.ascontiguousarray(dn, dtype=np.int32)
+            cd = np.ascontiguousarray(cd, dtype=np.int8)
+            keep += [dn, cd]
+            downs[k], codes[k] = dn.ctypes.data, cd.ctypes.data
+        ids = (C.c_void_p * 4)()
+        dims = (C.c_void_p * 4)()
+        for k in range(d + 1):
+            i, c = other.classification(k)
+            i = np.ascontiguousarray(i, dtype=np.int32)
+            c = np.ascontiguousarray(c, dtype=np.int8)
+            keep += [i, c]
+            ids[k], dims[k] = i.ctypes.data, c.ctypes.data
+        x = np.ascontiguousarray(other.coords(), dtype=np.float64)
+        mt = np.ascontiguousarray(other.metric(), dtype=np.float64)
+        nc = len(mt) // max(other.nents(0), 1)
+        h = lib().om_mesh_from_arrays(d, nents, downs, codes, _ptr(x, C.c_double), nc,
+                                      _ptr(mt, C.c_double), ids, dims)
+        return cls(h)
 
     @property
     def dim(self):
@@ -288,6 +325,14 @@ class Mesh:
         c = np.ascontiguousarray(cands, dtype=np.int32)
         out = np.zeros(len(c), dtype=np.float64)
         lib().om_kat_refine_qualities(self.h, len(c), _ptr(c, C.c_int), _ptr(out, C.c_double))
+        return out
+
+    def coarsen_qualities(self, cands, codes):
+        c = _ints(cands)
+        k = np.ascontiguousarray(codes, dtype=np.int8)
+        out = np.zeros(2 * len(c), dtype=np.float64)
+        lib().om_kat_coarsen_qualities(self.h, len(c), _ptr(c, C.c_int), _ptr(k, C.c_byte),
+                                       _ptr(out, C.c_double))
         return out
 
     def find_loop(self, edge):

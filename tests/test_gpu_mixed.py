@@ -59,3 +59,32 @@ def test_rotated_metric_estimates(ctx, seed):
     lrel, qrel, frac = errors(m)
     print("seed %d: length %.2e, quality %.2e, unsure %.4f" % (seed, lrel, qrel, frac))
     assert lrel < TOL and qrel < TOL
+
+
+@pytest.mark.parametrize("n", [12, 20])
+def test_coarsen_qualities_match_the_oracle(ctx, n):
+    """osh_coarsen_qualities (the coarsen pass's dominant sweep) on every edge
+    shorter than 1/sqrt(2), both directions, bit for bit against the oracle;
+    the mixed form gives the same values except -1 where the FP64 value is
+    below the threshold."""
+    import oracle_lib as O
+    o = O.Mesh.box(3, n)
+    met, nc = W.levelset_metric_3d(o.coords())
+    o.set_metric(met, nc)
+    g = A.Mesh.box(ctx, 3, n)
+    g.set_metric(met, nc)
+    cands = np.nonzero(g.lengths() < 2 ** -0.5)[0].astype(np.int32)
+    assert len(cands) > 100
+    codes = np.full(len(cands), 3, dtype=np.int8)
+    codes[::5] = 1
+    codes[1::5] = 2
+    want = o.coarsen_qualities(cands, codes)
+    got = g.coarsen_qualities(cands, codes)
+    assert np.array_equal(want.view(np.int64), got.view(np.int64))
+    qmin = 0.2
+    mixed = g.coarsen_qualities(cands, codes, precision=1, min_quality=qmin)
+    kept = mixed != -1.0
+    assert np.array_equal(mixed[kept].view(np.int64), want[kept].view(np.int64))
+    assert np.all(want[~kept] < qmin)
+    print("screen rejected %d of %d directions" % ((~kept & (want != -1.0)).sum(),
+                                                     (want != -1.0).sum()))
