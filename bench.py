@@ -14,17 +14,34 @@ those passes (each pass timed on the host clock with the stream drained at
 both ends), in millions per second.  `ms_per_step` is the wall time of a
 whole adapt() call, including the calls that changed nothing.
 
-The `roofline` object covers the element-quality kernel
-(measure_qualities, 3D, 6-component metric) on the input mesh, timed with HIP
-events on the library's own stream.  Algorithmic bytes per launch:
-24 B per tet (4 vertex indices + one FP64 quality) + 72 B per vertex
-(3 FP64 coordinates + 6 FP64 metric components, each read once).  The edge
-length kernel is timed the same way (16 B per edge + 72 B per vertex) and
-reported on stderr.
+`mixed` times the same steps with the mixed-precision form (FP32 cavity
+screens, every cavity not rejected by the 1e-3 band re-evaluated in FP64),
+checks that it produces the FP64 mesh bit for bit, and reports the FP32
+estimates' largest relative errors against FP64 over the input mesh.
 
-The `cpu_baseline` object times the CPU restatement of the same algorithm
-(single thread) on the first passes of the same workload, bounded to about
-`--cpu-seconds` seconds of pass time.
+The `roofline` object covers the adapt's most expensive kernel, the
+coarsen quality sweep (coarsen_quality_lanes, through osh_coarsen_qualities)
+over every edge of the input shorter than 1/sqrt(2), both collapse
+directions, timed with HIP events on the library's own stream.  It is bound
+by FP64 arithmetic: achieved = 222 FP64 operations per cavity element
+evaluated (element_quality<3,3>: four 3x3 determinants, the volume, six
+metric products and the mean-ratio formula) times the elements evaluated,
+over the launch time, against MI355X's 78.6 TFLOP/s.  `roofline_measure`
+keeps the element-quality measure kernel against HBM (24 B per tet + 72 B per
+vertex).
+
+`levelset_drivers` compares, once each outside the timed steps, the bare
+adapt() with the reference's idiom for a strongly anisotropic target
+(src/aniso_test.cpp:15-27: the implied metric becomes the metric, the
+level-set metric the target, then while (approach_metric) adapt): passes by
+kind, time, final element count, minimum quality and the fraction of edges
+with metric length in [1/sqrt(2), sqrt(2)].
+
+The `cpu_baseline` object times the oracle (the CPU restatement, one
+thread) on a bounded sample of the same workload's passes: the first coarsen
+pass from the input, then swap and sliver passes from the mesh the device
+reached after its last coarsen pass (the device's mesh is the oracle's, bit
+for bit, at every pass), about `--cpu-seconds` seconds in all.
 """
 import argparse
 import json
@@ -36,6 +53,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# FP64 operations (add, sub, mul, div, sqrt, cbrt: one each) of one
+# element_quality<3,3> (maxdet_metric + metric_element_quality): 4 x 14 for
+# the vertex metrics' determinants, 9 + 15 for the basis and volume, 2 for the
+# size, 9 for the other edge vectors, 6 x 20 + 6 + 1 for the mean squared
+# metric length, 5 for the mean ratio
+FLOPS_PER_QUALITY = 222
 PASS_NAMES = ("refine_by_size", "coarsen_by_size", "swap_edges", "coarsen_slivers")
 
 
@@ -81,6 +105,96 @@ def kernel_rooflines(ctx, mesh, launches):
         out[name] = {"kernel": name, "launch_ms": ms, "bytes": nbytes,
                      "achieved": nbytes / (ms * 1e-3) / 1e9}
     return out
+
+
+def coarsen_roofline(ctx, mesh, launches, precision=0):
+    """HIP-event timing, on the library's stream, of the coarsen quality sweep
+    (osh_coarsen_qualities) over every edge shorter than 1/sqrt(2) of the
+    mesh, both collapse directions."""
+    import numpy as np
+    from omega_h_amd.hip import DeviceArray, Event
+    lengths = mesh.lengths()
+    cands = np.nonzero(lengths < 2 ** -0.5)[0].astype(np.int32)
+    codes = np.full(len(cands), 3, dtype=np.int8)
+    ev2v = mesh.verts(1).reshape(-1, 2)
+    v2t = np.diff(mesh.up(0, 3)[0])
+    e2t = np.diff(mesh.up(1, 3)[0])
+    # the elements around the collapsing vertex that survive the collapse:
+    # all of its elements except those around the edge
+    evals = int((v2t[ev2v[cands, 0]] + v2t[ev2v[cands, 1]] - 2 * e2t[cands]).sum())
+    dc, dk = DeviceArray(cands), DeviceArray(codes)
+    out = DeviceArray(dtype=np.float64, n=2 * len(cands))
+
+    def launch():
+        mesh.coarsen_qualities_ptr(len(cands), dc.ptr, dk.ptr, out.ptr, precision, 0.2)
+
+    for _ in range(2):
+        launch()
+    ctx.synchronize()
+    stream = ctx.stream
+    t0, t1 = Event(), Event()
+    t0.record(stream)
+    for _ in range(launches):
+        launch()
+    t1.record(stream)
+    t1.synchronize()
+    ms = t0.elapsed_ms(t1) / launches
+    flops = FLOPS_PER_QUALITY * evals
+    return {"kernel": "coarsen_quality_lanes<3,3>", "launch_ms": ms, "sides": 2 * len(cands),
+            "evaluations": evals, "flops": flops, "tflops": flops / (ms * 1e-3) / 1e12}
+
+
+def quality_stats(mesh):
+    """Minimum element quality and the fraction of edges of metric length in
+    [1/sqrt(2), sqrt(2)]."""
+    import numpy as np
+    q = mesh.qualities()
+    lengths = mesh.lengths()
+    ok = (lengths >= 2 ** -0.5) & (lengths <= 2 ** 0.5)
+    return {"min_quality": float(q.min()), "edges_in_length_goal": float(ok.mean()),
+            "max_length": float(lengths.max())}
+
+
+def levelset_drivers(ctx, base, opts, n):
+    """The bare adapt() (as timed) against the reference's idiom for a strongly
+    anisotropic target (implied metric + while (approach_metric) adapt)."""
+    import numpy as np
+    from omega_h_amd import workloads as W
+    out = {}
+    m = base.copy()
+    t0 = time.perf_counter()
+    _, passes = m.adapt(opts)
+    out["bare_adapt"] = {"adapt_calls": 1, "seconds": time.perf_counter() - t0,
+                         "passes": [sum(1 for p in passes if p[0] == k) for k in range(4)],
+                         "final_tets": m.nents(3), **quality_stats(m)}
+    m = A_mod().Mesh.box(ctx, 3, n)
+    target, nc = W.levelset_metric_3d(m.coords())
+    t0 = time.perf_counter()
+    m.implied_metrics(set_as_metric=True)
+    m.add_vertex_tag("target_metric", nc, 1, target)
+    calls, kinds = 0, [0, 0, 0, 0]
+    nin = 0
+    pass_secs = 0.0
+    while m.approach_metric(opts):
+        _, passes = m.adapt(opts)
+        calls += 1
+        for p in passes:
+            kinds[p[0]] += 1
+            nin += p[1]
+            pass_secs += p[3]
+    secs = time.perf_counter() - t0
+    out["implied_then_approach"] = {
+        "adapt_calls": calls, "seconds": secs, "passes": kinds, "final_tets": m.nents(3),
+        "Mtets_per_s_per_pass": nin / pass_secs / 1e6 if pass_secs > 0 else None,
+        **quality_stats(m)}
+    for k, v in out.items():
+        log(f"driver {k}: {v}")
+    return out
+
+
+def A_mod():
+    import omega_h_amd as A
+    return A
 
 
 def roofline_only(ctx, mesh, launches):
@@ -197,75 +311,155 @@ def serve_main(args):
 
 
 def batch_main(args):
-    """Many independent boxes of uneven size (cells per side log-uniform in
-    [8, 48] in 3D, constant rotated metrics) adapted concurrently by the
-    request server, against the same boxes one at a time."""
+    """The batched form: --boxes independent tet boxes (cells per side n
+    log-uniform in [8, 48]; each with a constant metric rotated by a random
+    unit quaternion, eigen-lengths log-uniform in [0.005, 0.1]; numpy PCG64
+    seeded with --batch-seed), each metric clamped by the reference's
+    element-count rule to --max-box-elems expected elements.  The boxes are
+    joined into one disjoint mesh and adapted as one (one launch per kernel
+    for all boxes), in FP64 and in the mixed form; the same boxes adapted one
+    after another give the unbatched time; sampled boxes adapted alone must
+    equal their parts of the joined result bit for bit."""
     import numpy as np
     import omega_h_amd as A
+    from omega_h_amd import workloads as W
     ctx = A.Context(0)
-    opts = ctx.default_opts(3)
-    opts.verbosity = 0  # SILENT, as the workload specifies
-    rng = np.random.default_rng(args.seed)
-    boxes = []
-    for _ in range(args.boxes):
-        n = int(round(np.exp(rng.uniform(np.log(8), np.log(48)))))
-        d, _, v = constant_metric_request(rng, 3, 6 * n ** 3, args.h_lo, args.h_hi)
-        boxes.append((n, v))
-    out = {}
-    for slots in (1, args.slots):
-        server = A.Server(nslots=slots, hbm_budget_bytes=args.budget)
-        server.wait(server.submit(3, 4, boxes[0][1], opts))
+    boxes = W.batch_boxes(args.boxes, args.batch_seed, n_range=(8, 48), h_range=(0.005, 0.1),
+                          max_nelems=args.max_box_elems)
+    nclamped = sum(1 for b in boxes if b[3] < 1.0)
+    expected_raw = sum(b[2] for b in boxes)
+    expected = sum(min(b[2], args.max_box_elems) for b in boxes)
+    log(f"{len(boxes)} boxes, {sum(6 * b[0] ** 3 for b in boxes)} tets in, expected "
+        f"{expected_raw:.3g} tets unclamped, {expected:.3g} after clamping {nclamped} boxes")
+
+    def make(n, v):
+        m = A.Mesh.box(ctx, 3, n)
+        m.set_metric(np.tile(v, m.nents(0)), 6)
+        return m
+
+    t0 = time.perf_counter()
+    joined = A.Mesh.join(ctx, [make(n, v) for n, v, _, _ in boxes])
+    ctx.synchronize()
+    log(f"built and joined in {time.perf_counter() - t0:.2f} s: {joined.nents(3)} tets")
+    res = {}
+    for precision in (0, 1):
+        o = ctx.default_opts(3)
+        o.verbosity = 0
+        o.precision = precision
+        copies = [joined.copy() for _ in range(args.batch_warmup + args.batch_steps)]
+        for i in range(args.batch_warmup):
+            copies[i].adapt(o)
+        ctx.synchronize()
         t0 = time.perf_counter()
-        tickets = [server.submit(3, n, v, opts) for n, v in boxes]
-        res = [server.wait(t) for t in tickets]
+        runs = [copies[args.batch_warmup + i].adapt(o)[1] for i in range(args.batch_steps)]
+        ctx.synchronize()
         wall = time.perf_counter() - t0
-        server.close()
-        elems = sum(r.pass_elems for r in res)
-        out[slots] = {"wall_s": wall, "Mtets_per_s": elems / wall / 1e6,
-                      "tets_in": int(sum(r.nelems_in for r in res)),
-                      "tets_out": int(sum(r.nelems_out for r in res))}
-        log(f"batch slots {slots}: {out[slots]}")
+        nin = sum(p[1] for r in runs for p in r)
+        secs = sum(p[3] for r in runs for p in r)
+        last = copies[-1]
+        res[precision] = {"Mtets_per_s_per_pass": nin / secs / 1e6,
+                          "s_per_step": wall / args.batch_steps,
+                          "passes": [sum(1 for p in runs[-1] if p[0] == k) for k in range(4)],
+                          "tets_out": last.nents(3), "mesh": last}
+        log(f"batch {'mixed' if precision else 'fp64'}: "
+            f"{ {k: v for k, v in res[precision].items() if k != 'mesh'} }")
+    same = res[0]["mesh"].digests() == res[1]["mesh"].digests()
+    # one box at a time
+    o = ctx.default_opts(3)
+    o.verbosity = 0
+    singles = [make(n, v) for n, v, _, _ in boxes]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    nin, secs = 0, 0.0
+    for m in singles:
+        for p in m.adapt(o)[1]:
+            nin += p[1]
+            secs += p[3]
+    ctx.synchronize()
+    one_wall = time.perf_counter() - t0
+    # sampled boxes alone against their parts of the joined result
+    parts = res[0]["mesh"].split(len(boxes))
+    order = np.argsort([b[0] for b in boxes])
+    sample = sorted(set(order[:3].tolist() + order[-3:].tolist()))
+    verified = all(parts[k].digests() == singles[k].digests() for k in sample)
     line = {
-        "metric": "Mtets adapted/s per pass, batched boxes",
-        "value": out[args.slots]["Mtets_per_s"], "unit": "Mtets/s", "n_gpus": 1,
+        "metric": "Mtets adapted/s per pass, batched boxes (one disjoint mesh)",
+        "value": res[0]["Mtets_per_s_per_pass"], "unit": "Mtets/s", "n_gpus": 1,
+        "steps": args.batch_steps, "warmup": args.batch_warmup,
+        "ms_per_step": res[0]["s_per_step"] * 1e3,
         "higher_is_better": True, "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.boxes} tet boxes, cells per side log-uniform in [8, 48], "
-                               f"constant metrics with eigen-lengths log-uniform in "
-                               f"[{args.h_lo}, {args.h_hi}]", "slots": args.slots},
-        "one_slot": out[1], "concurrent": out[args.slots],
+        "config": {"workload": f"{len(boxes)} tet boxes, cells per side log-uniform in "
+                               f"[8, 48] (seed {args.batch_seed:#x}), constant metrics rotated "
+                               "by random unit quaternions, eigen-lengths log-uniform in "
+                               f"[0.005, 0.1], clamped to {args.max_box_elems:g} expected "
+                               "elements per box", "boxes_clamped": nclamped,
+                   "expected_tets_unclamped": expected_raw, "expected_tets": expected},
+        "tets_in": joined.nents(3),
+        "fp64": {k: v for k, v in res[0].items() if k != "mesh"},
+        "mixed": {k: v for k, v in res[1].items() if k != "mesh"},
+        "mixed_same_mesh": same,
+        "one_at_a_time": {"s": one_wall, "Mtets_per_s_per_pass": nin / secs / 1e6},
+        "batched_speedup": one_wall / res[0]["s_per_step"],
+        "sample_boxes_alone_equal_parts": verified, "sample_boxes": sample,
     }
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(n, seconds):
-    """Oracle passes, in adapt() order, on the same workload until `seconds`
-    of pass time have been spent."""
+def cpu_baseline(ctx, base, opts_gpu, n, seconds):
+    """The oracle on a bounded sample of the workload's passes: the first
+    coarsen pass from the input, then (from the mesh the device reached after
+    satisfy_lengths, which is the oracle's own mesh there, bit for bit) swap
+    and sliver passes in satisfy_quality's order, until `seconds` of pass
+    time have been spent."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from omega_h_amd import workloads as W
+    opts = O.default_opts(3)
+    opts.verbosity = 0
+    done = []
+    spent = 0.0
+    nin = 0
     m = O.Mesh.box(3, n)
     met, nc = W.levelset_metric_3d(m.coords())
     m.set_metric(met, nc)
-    opts = O.default_opts(3)
-    spent = 0.0
-    nin = 0
-    done = []
-    order = [0, 1] * 64
-    for kind in order:
-        n0 = m.nents(3)
-        t0 = time.perf_counter()
-        changed = getattr(m, PASS_NAMES[kind])(opts)
-        dt = time.perf_counter() - t0
-        if changed:
-            spent += dt
-            nin += n0
-            done.append(PASS_NAMES[kind])
-        if spent >= seconds:
+    n0 = m.nents(3)
+    t0 = time.perf_counter()
+    changed = m.coarsen_by_size(opts)
+    dt = time.perf_counter() - t0
+    if changed:
+        spent += dt
+        nin += n0
+        done.append("coarsen_by_size")
+    # satisfy_lengths on the device, then the same mesh in the oracle
+    g = base.copy()
+    while True:
+        r = g.refine_by_size(opts_gpu)
+        c = g.coarsen_by_size(opts_gpu)
+        if not (r or c):
             break
+    m = O.Mesh.copy_of(g)
+    qdesired = opts.min_quality_desired
+    while spent < seconds and m.qualities().min() < qdesired:
+        for kind in (2, 3):
+            n0 = m.nents(3)
+            t0 = time.perf_counter()
+            changed = getattr(m, PASS_NAMES[kind])(opts)
+            dt = time.perf_counter() - t0
+            if changed:
+                spent += dt
+                nin += n0
+                done.append(PASS_NAMES[kind])
+                break
+        else:
+            break
+    counts = {k: done.count(k) for k in dict.fromkeys(done)}
     return {"value": nin / spent / 1e6 if spent > 0 else None, "unit": "Mtets/s",
             "cores": 1, "kind": "port",
-            "sample": f"first {len(done)} mesh-changing passes ({', '.join(done)}) of the "
-                      f"{n}^3 level-set adapt, {spent:.1f} s of pass time, one thread"}
+            "sample": f"{len(done)} mesh-changing passes of the {n}^3 level-set adapt "
+                      f"({', '.join(f'{v} {k}' for k, v in counts.items())}): the first "
+                      f"coarsen pass from the input, then satisfy_quality's passes from "
+                      f"the mesh after the last coarsen pass; {spent:.1f} s of pass time, "
+                      f"one thread"}
 
 
 def main():
@@ -276,6 +470,8 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="cells per side of the box")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-drivers", action="store_true")
     ap.add_argument("--roofline-launches", type=int, default=20)
     ap.add_argument("--mode", choices=["adapt", "serve", "batch"], default="adapt",
                     help="adapt: the 64^3 level-set adapt (default); serve: Poisson request "
@@ -287,7 +483,11 @@ def main():
     ap.add_argument("--max-elems", type=float, default=2e5)
     ap.add_argument("--h-lo", type=float, default=0.02)
     ap.add_argument("--h-hi", type=float, default=0.1)
-    ap.add_argument("--boxes", type=int, default=64)
+    ap.add_argument("--boxes", type=int, default=256)
+    ap.add_argument("--batch-seed", type=lambda x: int(x, 0), default=0x0A5A17E5EED5EED5)
+    ap.add_argument("--max-box-elems", type=float, default=2.5e5)
+    ap.add_argument("--batch-steps", type=int, default=2)
+    ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the measure-kernel sweeps and a 64 MiB device copy (the "
@@ -305,6 +505,8 @@ def main():
         batch_main(args)
         return
 
+    import numpy as np
+
     import omega_h_amd as A
     from omega_h_amd import workloads as W
 
@@ -320,33 +522,62 @@ def main():
         roofline_only(ctx, base, args.roofline_launches)
         return
 
-    copies = [base.copy() for _ in range(args.warmup + args.steps)]
-    for i in range(args.warmup):
+    def timed(precision):
+        o = ctx.default_opts(3)
+        o.verbosity = 0  # SILENT, as the workload specifies
+        o.precision = precision
+        copies = [base.copy() for _ in range(args.warmup + args.steps)]
+        for i in range(args.warmup):
+            t0 = time.perf_counter()
+            copies[i].adapt(o)
+            log(f"{'mixed' if precision else 'fp64'} warmup {i}: {copies[i].nents(3)} tets, "
+                f"{time.perf_counter() - t0:.2f} s")
+        ctx.synchronize()
         t0 = time.perf_counter()
-        res, passes = copies[i].adapt(opts)
-        log(f"warmup {i}: {len(passes)} passes, {copies[i].nents(3)} tets, "
-            f"{time.perf_counter() - t0:.2f} s")
+        runs = []
+        for i in range(args.steps):
+            runs.append(copies[args.warmup + i].adapt(o)[1])
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        nin = sum(p[1] for passes in runs for p in passes)
+        secs = sum(p[3] for passes in runs for p in passes)
+        return nin / secs / 1e6, wall, runs, copies[-1]
 
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    runs = []
-    for i in range(args.steps):
-        m = copies[args.warmup + i]
-        runs.append(m.adapt(opts)[1])
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-
-    nin = sum(p[1] for passes in runs for p in passes)
-    secs = sum(p[3] for passes in runs for p in passes)
-    value = nin / secs / 1e6
-    final = copies[-1].nents(3)
+    value, wall, runs, last = timed(0)
+    final = last.nents(3)
     for kind in range(4):
         ps = [p for p in runs[-1] if p[0] == kind]
         if ps:
             log(f"{PASS_NAMES[kind]}: {len(ps)} passes, {sum(p[1] for p in ps)} tets in, "
                 f"{sum(p[3] for p in ps):.3f} s")
     log(f"step: {len(runs[-1])} passes, {nelems0} -> {final} tets, {wall / args.steps:.3f} s")
+    mixed = None
+    if not args.no_mixed:
+        mvalue, mwall, mruns, mlast = timed(1)
+        l32, q32, unsure = base.screen_estimates()
+        l64, q64 = base.lengths(), base.qualities()
+        ok = unsure == 0
+        mixed = {
+            "value": mvalue, "unit": "Mtets/s", "ms_per_step": mwall / args.steps * 1e3,
+            "speedup_vs_fp64": mvalue / value,
+            "step_speedup_vs_fp64": wall / mwall,
+            "same_mesh_as_fp64": mlast.digests() == last.digests(),
+            "same_passes_as_fp64": [p[:3] for p in mruns[-1]] == [p[:3] for p in runs[-1]],
+            "screen_band_rel": 1e-3,
+            "fp32_estimate_error_on_input": {
+                "length_rel_max": float((np.abs(l32 - l64) / np.abs(l64)).max()),
+                "quality_rel_max": float((np.abs(q32 - q64)[ok] /
+                                          np.maximum(np.abs(q64[ok]), 1e-3)).max()),
+                "quality_unsure_frac": float(1.0 - ok.mean()),
+                "log_exp": "not on this workload's path (no refinement passes)"},
+        }
+        log(f"mixed: {mixed}")
 
+    cq = coarsen_roofline(ctx, base, args.roofline_launches)
+    cq32 = coarsen_roofline(ctx, base, args.roofline_launches, precision=1)
+    log(f"{cq['kernel']}: {cq['launch_ms'] * 1e3:.1f} us per launch, {cq['evaluations']} "
+        f"element qualities, {cq['tflops']:.2f} TFLOP/s; mixed "
+        f"{cq32['launch_ms'] * 1e3:.1f} us")
     roofs = kernel_rooflines(ctx, base, args.roofline_launches)
     for r in roofs.values():
         log(f"{r['kernel']}: {r['launch_ms'] * 1e3:.1f} us per launch, {r['bytes']} B, "
@@ -370,19 +601,29 @@ def main():
                                "level-set anisotropic metric, full adapt() FP64, default "
                                "AdaptOpts with verbosity SILENT",
                    "passes_per_step": len(runs[-1]), "final_tets": final},
-        "roofline": {"bound": "hbm", "achieved": roof["achieved"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": roof["achieved"] / HBM_PEAK_GBS,
-                     "traffic": traffic_per_launch(roof["kernel"]) if args.n == 64 else None,
-                     "kernel": roof["kernel"],
-                     # the counters show HBM traffic within 6% of the algorithmic
-                     # bytes; the kernel is bound by its FP64 instruction issue
-                     "binding": "fp64 valu",
-                     "launch_us": roof["launch_ms"] * 1e3, "bytes": roof["bytes"]},
+        "roofline": {"bound": "fp64", "achieved": cq["tflops"], "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": cq["tflops"] / FP64_PEAK_TFLOPS,
+                     "traffic": traffic_per_launch(cq["kernel"]) if args.n == 64 else None,
+                     "kernel": cq["kernel"], "launch_us": cq["launch_ms"] * 1e3,
+                     "evaluations": cq["evaluations"], "flops_per_evaluation": FLOPS_PER_QUALITY,
+                     "mixed_launch_us": cq32["launch_ms"] * 1e3},
+        "roofline_measure": {"bound": "hbm", "achieved": roof["achieved"], "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": roof["achieved"] / HBM_PEAK_GBS,
+                             "traffic": traffic_per_launch(roof["kernel"])
+                             if args.n == 64 else None,
+                             "kernel": roof["kernel"], "launch_us": roof["launch_ms"] * 1e3,
+                             "bytes": roof["bytes"]},
+        "mixed": mixed,
+        "levelset_drivers": None,
         "cpu_baseline": None,
     }
+    if not args.no_drivers:
+        o = ctx.default_opts(3)
+        o.verbosity = 0
+        line["levelset_drivers"] = levelset_drivers(ctx, base, o, args.n)
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()
-        line["cpu_baseline"] = cpu_baseline(args.n, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(ctx, base, opts, args.n, args.cpu_seconds)
         log(f"cpu baseline: {time.perf_counter() - t0:.1f} s")
     print(json.dumps(line), flush=True)
 

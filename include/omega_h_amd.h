@@ -328,6 +328,11 @@ int osh_server_destroy(osh_server* server);
 int osh_server_submit(osh_server* server, int dim, int nx, int ny, int nz, int metric_ncomps,
     const double* metric, const osh_adapt_opts* opts, int want_digest, int* ticket);
 int osh_server_wait(osh_server* server, int ticket, osh_serve_result* out);
+/* per-request time limit for requests submitted afterwards (0: none): a
+ * request fails with status 1 and frees its slot once `seconds` have passed
+ * since its submission, checked before it starts and after every pass (a
+ * pass is not interrupted) */
+int osh_server_set_timeout(osh_server* server, double seconds);
 
 /* cavity kernels on host arrays, for tests and diagnostics:
  * refine_qualities src/Omega_h_refine_qualities.cpp:88-108 */

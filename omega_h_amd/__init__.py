@@ -173,6 +173,7 @@ def load(path=None):
         "osh_server_destroy": [P],
         "osh_server_submit": [P, I, I, I, I, I, DP, C.POINTER(AdaptOpts), I, IP],
         "osh_server_wait": [P, I, C.POINTER(ServeResult)],
+        "osh_server_set_timeout": [P, C.c_double],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -688,6 +689,10 @@ class Server:
                                          _p(m, C.c_double), C.byref(opts), int(want_digest),
                                          C.byref(t)), self.lib)
         return t.value
+
+    def set_timeout(self, seconds):
+        """Per-request time limit for later submissions (0: none)."""
+        check(self.lib.osh_server_set_timeout(self.h, float(seconds)), self.lib)
 
     def wait(self, ticket):
         r = ServeResult()

@@ -60,6 +60,7 @@ struct Request {
   bool want_digest;
   double est_bytes;
   double t_submit;
+  double deadline;  // wall clock; 0 for none
 };
 
 }  // namespace
@@ -102,11 +103,22 @@ class Server {
     r.est_bytes = kBytesPerElement * (nin > nexp ? nin : nexp);
     r.t_submit = wall();
     std::lock_guard<std::mutex> g(mu_);
+    r.deadline = timeout_ > 0 ? r.t_submit + timeout_ : 0.0;
     r.ticket = next_ticket_++;
     results_[r.ticket].done = false;
     queue_.push_back(r);
     cv_work_.notify_all();
     return r.ticket;
+  }
+
+  // requests submitted from now on fail (status 1) once `seconds` have
+  // passed since their submission: before they start, or at the end of the
+  // pass that crosses the deadline (a pass itself is not interrupted); 0
+  // turns the limit off
+  void set_timeout(double seconds) {
+    OSH_CHECK(seconds >= 0, "the timeout must not be negative");
+    std::lock_guard<std::mutex> g(mu_);
+    timeout_ = seconds;
   }
 
   void wait(int ticket, osh_serve_result* out) {
@@ -156,6 +168,8 @@ class Server {
       double t_start = wall();
       try {
         OSH_CHECK(ctx != nullptr, "this slot has no device context: " + ctx_error);
+        OSH_CHECK(r.deadline == 0 || t_start <= r.deadline,
+            "request timed out before it started");
         Ctx& c = *ctx;
         DMesh m;
         build_box(c, m, r.dim, 1.0, 1.0, r.dim == 3 ? 1.0 : 0.0, r.nx, r.ny, r.nz, true);
@@ -167,7 +181,13 @@ class Server {
         m.metric_ncomps = r.ncomps;
         res.nelems_in = m.nents[r.dim];
         std::vector<PassRecord> log;
-        adapt(c, m, r.opts, &log);
+        PassHook hook;
+        if (r.deadline > 0)
+          hook = [&](DMesh&) {
+            OSH_CHECK(wall() <= r.deadline, "request timed out after " +
+                                                std::to_string(log.size()) + " passes");
+          };
+        adapt(c, m, r.opts, &log, &hook);
         c.sync();
         res.nelems_out = m.nents[r.dim];
         res.npasses = int(log.size());
@@ -207,6 +227,7 @@ class Server {
   int running_ = 0;
   bool stop_ = false;
   int next_ticket_ = 0;
+  double timeout_ = 0;
   std::vector<std::thread> threads_;
 };
 
@@ -237,6 +258,13 @@ int osh_server_submit(osh_server* server, int dim, int nx, int ny, int nz, int m
     osh::AdaptOptions o;
     osh::to_options(opts, o);
     *ticket = server->s.submit(dim, nx, ny, nz, metric_ncomps, metric, o, want_digest != 0);
+  });
+}
+
+int osh_server_set_timeout(osh_server* server, double seconds) {
+  return osh::guarded([&] {
+    OSH_CHECK(server != nullptr, "null argument");
+    server->s.set_timeout(seconds);
   });
 }
 

@@ -87,9 +87,9 @@ def banded_aniso_metric(coords, dim, rng):
     return v.ravel(), v.shape[1]
 
 
-# volume of the unit-edge regular simplex (typical_unit_simplex_size,
-# src/Omega_h_metric.cpp:288-290)
-UNIT_SIMPLEX_SIZE = {2: math.sqrt(3.0) / 4.0, 3: 1.0 / (6.0 * math.sqrt(2.0))}
+# the reference's empirical volume of an element of a unit-edge-length mesh
+# (typical_unit_simplex_size, src/Omega_h_metric.cpp:286-290)
+UNIT_SIMPLEX_SIZE = {2: 0.377645136635, 3: 0.0838934100219}
 
 
 def symm_det(v):

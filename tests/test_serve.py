@@ -81,3 +81,21 @@ def test_admission_under_a_tiny_budget_runs_requests_one_at_a_time():
     for i in range(1, len(res)):
         assert res[i].queue_seconds >= res[i - 1].run_seconds * 0.9
     assert all(r.status == 0 for r in res)
+
+
+@pytest.mark.gpu
+def test_timed_out_request_fails_and_frees_its_slot():
+    """A request past its time limit fails with a status (it does not hold
+    its slot); requests after the limit is lifted run normally."""
+    ctx = A.Context(0)
+    reqs = requests()[:3]
+    server = A.Server(nslots=1, hbm_budget_bytes=50e9)
+    server.set_timeout(1e-9)
+    late = [server.submit(d, n, v, ctx.default_opts(d)) for d, n, v in reqs]
+    for t in late:
+        with pytest.raises(A.OshError, match="timed out"):
+            server.wait(t)
+    server.set_timeout(0)
+    ok = [server.wait(server.submit(d, n, v, ctx.default_opts(d))) for d, n, v in reqs]
+    server.close()
+    assert all(r.status == 0 and r.nelems_out > 0 for r in ok)
