@@ -502,7 +502,7 @@ struct AdaptOpts {
   Real length_histogram_max = 3.0;
   Int nlength_histogram_bins = 10;
   Int nquality_histogram_bins = 10;
-  // not built here: adapt() with this set throws (src/Omega_h_coarsen_flip.cpp)
+  // prevent_coarsen_flip, src/Omega_h_coarsen_flip.cpp:68-150
   bool should_prevent_coarsen_flip = false;
   // no counterpart in the reference: OSH_FP64 or OSH_MIXED cavity screens
   Int precision = OSH_FP64;

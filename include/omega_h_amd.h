@@ -39,8 +39,9 @@ typedef struct {
   double length_histogram_max;
   int nlength_histogram_bins;
   int nquality_histogram_bins;
-  /* prevent_coarsen_flip (src/Omega_h_coarsen_flip.cpp:68-150) is not
-   * built: a pass or adapt() with this set returns a nonzero status */
+  /* vetoes collapses that would turn an exposed side against its
+   * vertices' normals, prevent_coarsen_flip src/Omega_h_coarsen_flip.cpp:68-150
+   * (called from src/Omega_h_coarsen.cpp:73-77) */
   int should_prevent_coarsen_flip;
   /* arithmetic of the cavity quality and length screens (no counterpart in
    * the reference): OSH_FP64, or OSH_MIXED, which screens cavities in FP32

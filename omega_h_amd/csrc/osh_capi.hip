@@ -39,9 +39,6 @@ void to_options(osh_adapt_opts const* o, AdaptOptions& r) {
   r.nquality_histogram_bins = o->nquality_histogram_bins;
   r.should_prevent_coarsen_flip = o->should_prevent_coarsen_flip;
   r.precision = o->precision;
-  OSH_CHECK(!r.should_prevent_coarsen_flip,
-      "should_prevent_coarsen_flip is not supported (src/Omega_h_coarsen_flip.cpp:68-150 "
-      "is not built)");
   OSH_CHECK(r.precision == 0 || r.precision == 1, "precision must be OSH_FP64 or OSH_MIXED");
   OSH_CHECK(r.verbosity < 3 || (r.nlength_histogram_bins >= 1 && r.nquality_histogram_bins >= 1 &&
                                     r.length_histogram_max > r.length_histogram_min),

@@ -5,6 +5,8 @@
 //   check_collapse_class / check_collapse_exposure
 //                                    src/Omega_h_check_collapse.cpp:10-117
 //   prevent_coarsen_overshoot        src/Omega_h_coarsen_overshoot.cpp:12-49
+//   prevent_coarsen_flip             src/Omega_h_coarsen_flip.cpp:14-150,
+//                                    src/Omega_h_most_normal.hpp:15-93
 //   coarsen_qualities, filter_coarsen_min_qual / _improve
 //                                    src/Omega_h_coarsen_qualities.cpp:16-122
 //   choose_rails / find_rails        src/Omega_h_collapse_rail.cpp:15-101
@@ -459,6 +461,256 @@ __global__ void coarsen_products(LO nt, int nvd, LO const* keys2verts, LO const*
   }
 }
 
+// ---- prevent_coarsen_flip (src/Omega_h_coarsen_flip.cpp:68-150), taken
+// only when AdaptOpts::should_prevent_coarsen_flip is set.  Exposed sides
+// (fewer than two elements) around a vertex of a candidate edge "matter", and
+// so do their vertices; each such vertex gets a normal, and a collapse
+// direction is vetoed when a surviving side that matters around the
+// collapsing vertex would turn so that its normal has a dot product below
+// 1e-10 with the normal of one of its vertices.
+
+// get_side_vector (src/Omega_h_shape.hpp:276-298): the triangle's
+// (b - a) x (c - a) in 3D, -perp(b - a) in 2D
+template <int D>
+__device__ __forceinline__ void side_vector(double const* const* p, double* out) {
+  if (D == 3) {
+    double u[3], w[3];
+    for (int i = 0; i < 3; ++i) {
+      u[i] = p[1][i] - p[0][i];
+      w[i] = p[2][i] - p[0][i];
+    }
+    cross3(u, w, out);
+  } else {
+    double v0 = p[1][0] - p[0][0], v1 = p[1][1] - p[0][1];
+    out[0] = -(-v1);
+    out[1] = -v0;
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void normalize_vec(double* v) {
+  double l = vnorm<D>(v);
+  for (int i = 0; i < D; ++i) v[i] = v[i] / l;
+}
+
+__device__ __forceinline__ bool close_rel(double a, double b) {
+  double am = math::fabs_(a), bm = math::fabs_(b);
+  if (am <= 1e-10 && bm <= 1e-10) return true;
+  return math::fabs_(b - a) / math::max2(am, bm) <= 1e-10;
+}
+
+// get_most_normal_normal (src/Omega_h_most_normal.hpp:15-93; Aubry and
+// Lohner 2008): the centre of the smallest spherical cap holding the n unit
+// vectors, from pairs (their normalized bisector) and then from triplets
+// (the direction with one dot product with all three, from the inverse of
+// the matrix whose rows are the three).  In 2D the normalized sum of the
+// first two.  False where the reference's checks fail.
+template <int D>
+__device__ bool most_normal(int n, double const* N, double* c) {
+  if (D == 2) {
+    c[0] = N[0] + N[2];
+    c[1] = N[1] + N[3];
+    normalize_vec<2>(c);
+    return true;
+  }
+  double const tol = 1e-10;
+  double scalmin = -1.0;
+  c[0] = c[1] = c[2] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    for (int j = i + 1; j < n; ++j) {
+      double b[3];
+      for (int k = 0; k < 3; ++k) b[k] = N[i * 3 + k] + N[j * 3 + k];
+      normalize_vec<3>(b);
+      double scal = dot<3>(b, N + i * 3);
+      if (scal < scalmin) continue;
+      int l;
+      for (l = 0; l < n; ++l)
+        if (dot<3>(N + l * 3, b) + tol < scal) break;
+      if (l < n) continue;
+      for (int k = 0; k < 3; ++k) c[k] = b[k];
+      scalmin = scal;
+    }
+  }
+  if (scalmin != -1.0) return true;
+  for (int i = 0; i < n; ++i) {
+    for (int j = i + 1; j < n; ++j) {
+      for (int k = j + 1; k < n; ++k) {
+        Mat M;  // columns N_i, N_j, N_k
+        for (int r = 0; r < 3; ++r) {
+          M.m[0][r] = N[i * 3 + r];
+          M.m[1][r] = N[j * 3 + r];
+          M.m[2][r] = N[k * 3 + r];
+        }
+        if (math::fabs_(det<3>(M)) < 1e-6) continue;
+        // invert(transpose(M)) * (1, 1, 1) (src/Omega_h_matrix.hpp:89-94,
+        // 344-350): rows of the cofactor matrix from the cross products of
+        // the transpose's columns, divided by its determinant, summed
+        // column by column
+        Mat T = transpose<3>(M);
+        double bm[3][3];
+        for (int a = 0; a < 3; ++a) cross3(T.m[(a + 1) % 3], T.m[(a + 2) % 3], bm[a]);
+        double dt = det<3>(T);
+        double nc[3];
+        for (int r = 0; r < 3; ++r) {
+          double s = (bm[r][0] / dt) * 1.0;
+          s = s + (bm[r][1] / dt) * 1.0;
+          s = s + (bm[r][2] / dt) * 1.0;
+          nc[r] = s;
+        }
+        normalize_vec<3>(nc);
+        double scal = dot<3>(nc, N + i * 3);
+        if (!close_rel(scal, dot<3>(nc, N + j * 3)) || !close_rel(scal, dot<3>(nc, N + k * 3)))
+          return false;
+        if (scal < scalmin) continue;
+        int l;
+        for (l = 0; l < n; ++l)
+          if (dot<3>(N + l * 3, nc) + tol < scal) break;
+        if (l < n) continue;
+        for (int q = 0; q < 3; ++q) c[q] = nc[q];
+        scalmin = scal;
+      }
+    }
+  }
+  return scalmin > -1.0;
+}
+
+__global__ void flip_vert_cands(LO ncands, LO const* cands, LO const* ev2v, I8 const* codes,
+    I8* vcand) {
+  LO i = thread_index();
+  if (i >= ncands) return;
+  LO e = cands[i];
+  if (codes[e] == kDont) return;
+  vcand[ev2v[e * 2]] = 1;
+  vcand[ev2v[e * 2 + 1]] = 1;
+}
+
+// mark_exposed_sides (src/Omega_h_mark.cpp:12-19) and the sides and
+// vertices that matter (src/Omega_h_coarsen_flip.cpp:128-134)
+__global__ void flip_sides(LO ns, int nsv, LO const* sv2v, unsigned long long const* side_elems,
+    I8 const* vcand, I8* exposed, I8* smatter, I8* vmatter) {
+  LO s = thread_index();
+  if (s >= ns) return;
+  I8 ex = (side_elems[s] >> 32) < 2 ? 1 : 0;
+  bool adj = false;
+  for (int k = 0; k < nsv; ++k) adj = adj || vcand[sv2v[s * nsv + k]] != 0;
+  I8 mt = (adj && ex) ? 1 : 0;
+  exposed[s] = ex;
+  smatter[s] = mt;
+  if (mt)
+    for (int k = 0; k < nsv; ++k) vmatter[sv2v[s * nsv + k]] = 1;
+}
+
+// compute_flip_normals_dim (src/Omega_h_coarsen_flip.cpp:14-64): zero for
+// vertices that do not matter
+template <int D>
+__global__ void flip_normals(LO nv, LO const* a2ab, LO const* ab2b, LO const* sv2v,
+    Real const* coords, I8 const* exposed, I8 const* vmatter, Real* normals, int* error) {
+  constexpr int kMaxAdj = (D == 3) ? 24 : 8;
+  LO v = thread_index();
+  if (v >= nv) return;
+  double Nc[D];
+  for (int k = 0; k < D; ++k) Nc[k] = 0.0;
+  if (vmatter[v]) {
+    double N[kMaxAdj * D];
+    int n = 0;
+    for (LO vs = a2ab[v]; vs < a2ab[v + 1]; ++vs) {
+      LO s = ab2b[vs];
+      if (!exposed[s]) continue;
+      if (n >= kMaxAdj) {
+        *error = kFlipNormals;
+        return;
+      }
+      double const* p[D];
+      for (int k = 0; k < D; ++k) p[k] = coords + size_t(sv2v[s * D + k]) * D;
+      double sv[3];
+      side_vector<D>(p, sv);
+      for (int k = 0; k < D; ++k) Nc[k] = Nc[k] + sv[k];
+      double l = vnorm<D>(sv);
+      for (int k = 0; k < D; ++k) N[n * D + k] = sv[k] / l;
+      ++n;
+    }
+    normalize_vec<D>(Nc);
+    int i;
+    for (i = 0; i < n; ++i)
+      if (dot<D>(Nc, N + i * D) < 0.95) break;
+    if (i < n && !most_normal<D>(n, N, Nc)) {
+      *error = kFlipNormals;
+      return;
+    }
+  }
+  for (int k = 0; k < D; ++k) normals[size_t(v) * D + k] = Nc[k];
+}
+
+// prevent_coarsen_flip2_dim (src/Omega_h_coarsen_flip.cpp:70-121)
+template <int D>
+__global__ void flip_veto(LO ncands, LO const* cands, LO const* ev2v, LO const* a2ab,
+    LO const* ab2b, I8 const* vs_codes, LO const* sv2v, Real const* coords,
+    Real const* normals, I8 const* smatter, I8 const* vmatter, I8* codes) {
+  LO i = thread_index();
+  if (i >= ncands) return;
+  LO e = cands[i];
+  I8 code = codes[e];
+  if (code == kDont) return;
+  for (int col = 0; col < 2; ++col) {
+    if (!collapses(code, col)) continue;
+    LO v_col = ev2v[e * 2 + col];
+    if (!vmatter[v_col]) continue;
+    LO v_onto = ev2v[e * 2 + (1 - col)];
+    for (LO vs = a2ab[v_col]; vs < a2ab[v_col + 1]; ++vs) {
+      LO s = ab2b[vs];
+      if (!smatter[s]) continue;
+      int ssv_col = vs_codes[vs] >> 3;
+      LO sv[D];
+      int k;
+      for (k = 0; k < D; ++k) sv[k] = sv2v[s * D + k];
+      for (k = 0; k < D; ++k)
+        if (sv[k] == v_onto) break;
+      if (k != D) continue;
+      sv[ssv_col] = v_onto;
+      double const* p[D];
+      for (k = 0; k < D; ++k) p[k] = coords + size_t(sv[k]) * D;
+      double sn[3];
+      side_vector<D>(p, sn);
+      for (k = 0; k < D; ++k)
+        if (dot<D>(normals + size_t(sv[k]) * D, sn) < 1e-10) break;
+      if (k < D) code = dont_collapse(code, col);
+    }
+  }
+  codes[e] = code;
+}
+
+// the whole check on the candidates' codes, in place
+void prevent_coarsen_flip(Ctx& c, DMesh& m, LOs const& cands, Bytes& codes) {
+  int dim = m.dim;
+  int sd = dim - 1;
+  LO nv = m.nents[0], ns = m.nents[sd];
+  LO ncands = cands.size();
+  LO const* ev2v = m.down[1].data();
+  LOs const& sv2v = verts_of(c, m, sd);
+  Adj const& v2s = ask_up(c, m, 0, sd);
+  auto const& se = ask_side_elems(c, m);
+  Bytes vcand = fill_i8(c, nv, 0);
+  OSH_LAUNCH(flip_vert_cands, ncands, c, cands.data(), ev2v, codes.data(), vcand.data());
+  Bytes exposed(c, ns), smatter(c, ns);
+  Bytes vmatter = fill_i8(c, nv, 0);
+  OSH_LAUNCH(flip_sides, ns, c, sd + 1, sv2v.data(), se.data(), vcand.data(), exposed.data(),
+      smatter.data(), vmatter.data());
+  Reals normals(c, LO(nv * dim));
+  if (dim == 3) {
+    OSH_LAUNCH(flip_normals<3>, nv, c, v2s.a2ab.data(), v2s.ab2b.data(), sv2v.data(),
+        m.coords.data(), exposed.data(), vmatter.data(), normals.data(), c.error_word());
+    OSH_LAUNCH(flip_veto<3>, ncands, c, cands.data(), ev2v, v2s.a2ab.data(), v2s.ab2b.data(),
+        v2s.codes.data(), sv2v.data(), m.coords.data(), normals.data(), smatter.data(),
+        vmatter.data(), codes.data());
+  } else {
+    OSH_LAUNCH(flip_normals<2>, nv, c, v2s.a2ab.data(), v2s.ab2b.data(), sv2v.data(),
+        m.coords.data(), exposed.data(), vmatter.data(), normals.data(), c.error_word());
+    OSH_LAUNCH(flip_veto<2>, ncands, c, cands.data(), ev2v, v2s.a2ab.data(), v2s.ab2b.data(),
+        v2s.codes.data(), sv2v.data(), m.coords.data(), normals.data(), smatter.data(),
+        vmatter.data(), codes.data());
+  }
+}
+
 bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool overshoot_allowed,
     bool improve) {
   int dim = m.dim;
@@ -519,6 +771,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
 #undef OSH_OV_ALL
 #undef OSH_OV
   OSH_LAUNCH(apply_vetoes, ncands, c, cands.data(), veto.data(), codes.data());
+  if (o.should_prevent_coarsen_flip) prevent_coarsen_flip(c, m, cands, codes);
   // sides without a candidate left are skipped by every sweep below, and the
   // check after the quality filter returns
   Reals quals(c, ne * 2);

@@ -97,6 +97,9 @@ void Ctx::sync() {
     if (code == kBrokenSwapLoop)
       throw Error("swap: the elements around an edge do not close a loop");
     if (code == kBadPart) throw Error("split: a vertex without a valid part number");
+    if (code == kFlipNormals)
+      throw Error("prevent_coarsen_flip: too many exposed sides at a vertex, or no most normal "
+                  "normal");
     if (code == kDegenerateEigen)
       throw Error("eigen: no eigenvector for a simple eigenvalue (degenerate metric)");
     throw Error("device error " + std::to_string(code));

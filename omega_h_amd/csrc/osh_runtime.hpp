@@ -186,7 +186,8 @@ enum DeviceError {
   kNonManifoldSide = 5,
   kBrokenSwapLoop = 6,
   kDegenerateEigen = 7,
-  kBadPart = 8
+  kBadPart = 8,
+  kFlipNormals = 9
 };
 
 template <class T>

@@ -8,6 +8,8 @@
  *   coarsen_by_size / coarsen_slivers       src/Omega_h_coarsen.cpp:29-220
  *   check_collapse_class / _exposure        src/Omega_h_check_collapse.cpp:10-117
  *   prevent_coarsen_overshoot               src/Omega_h_coarsen_overshoot.cpp:12-49
+ *   prevent_coarsen_flip / most normal      src/Omega_h_coarsen_flip.cpp:14-150,
+ *                                           src/Omega_h_most_normal.hpp:15-88
  *   coarsen_qualities / filters             src/Omega_h_coarsen_qualities.cpp:16-122
  *   choose_rails / find_rails               src/Omega_h_collapse_rail.cpp:15-101
  *   mark_dead_ents / find_coarsen_domains / coarsen_topology
@@ -70,6 +72,7 @@ void om_default_opts(int dim, om_adapt_opts* o) {
   o->should_coarsen = 1;
   o->should_swap = 1;
   o->should_coarsen_slivers = 1;
+  o->should_prevent_coarsen_flip = 0;
 }
 
 /* ------------------------------------------------------------ indset */
@@ -906,6 +909,129 @@ void om_kat_coarsen_qualities(om_mesh* m, int n, int const* cands, signed char c
   for (int i = 0; i < n; ++i) coarsen_edge_qualities(m, cands[i], codes[i], out + 2 * i);
 }
 
+/* prevent_coarsen_flip, src/Omega_h_coarsen_flip.cpp:68-150.  Exposed sides
+ * (fewer than two elements, mark_exposed_sides src/Omega_h_mark.cpp:12-19)
+ * that touch a vertex of a candidate edge "matter", and so do their
+ * vertices.  Each vertex that matters gets a normal
+ * (compute_flip_normals_dim, :14-64): the normalized sum of its exposed
+ * sides' area vectors, replaced by the most normal normal of their unit
+ * normals when one of those is more than acos(0.95) away from it.  A
+ * collapse direction is vetoed when a side that matters around the
+ * collapsing vertex, and does not hold the vertex it collapses onto, would
+ * after the collapse have a normal whose dot product with the normal of one
+ * of its vertices falls below 1e-10 (:70-121).  Returns 1 where the
+ * reference's checks would fail. */
+static int prevent_coarsen_flip(om_mesh* m, signed char* edge_codes) {
+  int dim = m->dim;
+  int sd = dim - 1;
+  int nv = m->nents[0], ne = m->nents[1], ns = m->nents[sd];
+  int const* ev2v = m->down[1];
+  int const* sv2v = om_verts_of(m, sd);
+  om_graph const* v2s = om_ask_up(m, 0, sd);
+  om_graph const* s2e = om_ask_up(m, sd, dim);
+  signed char* vcand = (signed char*)om_calloc((size_t)nv, 1);
+  for (int e = 0; e < ne; ++e) {
+    if (edge_codes[e] == DONT_COLLAPSE) continue;
+    vcand[ev2v[e * 2]] = 1;
+    vcand[ev2v[e * 2 + 1]] = 1;
+  }
+  signed char* exposed = (signed char*)om_calloc((size_t)ns, 1);
+  signed char* smatter = (signed char*)om_calloc((size_t)ns, 1);
+  signed char* vmatter = (signed char*)om_calloc((size_t)nv, 1);
+  for (int s = 0; s < ns; ++s) {
+    exposed[s] = (s2e->a2ab[s + 1] - s2e->a2ab[s]) < 2;
+    int adj = 0;
+    for (int k = 0; k < sd + 1; ++k) adj |= vcand[sv2v[s * (sd + 1) + k]];
+    smatter[s] = (adj && exposed[s]) ? 1 : 0;
+    if (smatter[s])
+      for (int k = 0; k < sd + 1; ++k) vmatter[sv2v[s * (sd + 1) + k]] = 1;
+  }
+  int const max_adj = (dim == 3) ? 24 : 8;
+  double* normals = (double*)om_calloc((size_t)nv * dim, sizeof(double));
+  double N[24 * 3];
+  int bad = 0;
+  for (int v = 0; v < nv && !bad; ++v) {
+    if (!vmatter[v]) continue;
+    double Nc[3] = {0.0, 0.0, 0.0};
+    int n = 0;
+    for (int vs = v2s->a2ab[v]; vs < v2s->a2ab[v + 1]; ++vs) {
+      int s = v2s->ab2b[vs];
+      if (!exposed[s]) continue;
+      if (n >= max_adj) {
+        bad = 1;
+        break;
+      }
+      double const* p[3];
+      for (int k = 0; k < sd + 1; ++k) p[k] = m->coords + (size_t)sv2v[s * (sd + 1) + k] * dim;
+      double sv[3];
+      om_side_vector(dim, p, sv);
+      double l = 0.0;
+      for (int k = 0; k < dim; ++k) Nc[k] = Nc[k] + sv[k];
+      l = sv[0] * sv[0];
+      for (int k = 1; k < dim; ++k) l = l + sv[k] * sv[k];
+      l = om_sqrt(l);
+      for (int k = 0; k < dim; ++k) N[n * dim + k] = sv[k] / l;
+      ++n;
+    }
+    if (bad) break;
+    double l = Nc[0] * Nc[0];
+    for (int k = 1; k < dim; ++k) l = l + Nc[k] * Nc[k];
+    l = om_sqrt(l);
+    for (int k = 0; k < dim; ++k) Nc[k] = Nc[k] / l;
+    int i;
+    for (i = 0; i < n; ++i) {
+      double d = Nc[0] * N[i * dim];
+      for (int k = 1; k < dim; ++k) d = d + Nc[k] * N[i * dim + k];
+      if (d < 0.95) break;
+    }
+    if (i < n && om_most_normal(dim, n, N, Nc)) {
+      bad = 1;
+      break;
+    }
+    for (int k = 0; k < dim; ++k) normals[(size_t)v * dim + k] = Nc[k];
+  }
+  for (int e = 0; e < ne && !bad; ++e) {
+    signed char code = edge_codes[e];
+    if (code == DONT_COLLAPSE) continue;
+    for (int col = 0; col < 2; ++col) {
+      if (!collapses(code, col)) continue;
+      int v_col = ev2v[e * 2 + col];
+      if (!vmatter[v_col]) continue;
+      int v_onto = ev2v[e * 2 + (1 - col)];
+      for (int vs = v2s->a2ab[v_col]; vs < v2s->a2ab[v_col + 1]; ++vs) {
+        int s = v2s->ab2b[vs];
+        if (!smatter[s]) continue;
+        int ssv_col = om_code_which_down(v2s->codes[vs]);
+        int sv[3];
+        int k;
+        for (k = 0; k < sd + 1; ++k) sv[k] = sv2v[s * (sd + 1) + k];
+        for (k = 0; k < sd + 1; ++k)
+          if (sv[k] == v_onto) break;
+        if (k != sd + 1) continue;
+        sv[ssv_col] = v_onto;
+        double const* p[3];
+        for (k = 0; k < sd + 1; ++k) p[k] = m->coords + (size_t)sv[k] * dim;
+        double sn[3];
+        om_side_vector(dim, p, sn);
+        for (k = 0; k < sd + 1; ++k) {
+          double const* vn = normals + (size_t)sv[k] * dim;
+          double d = vn[0] * sn[0];
+          for (int q = 1; q < dim; ++q) d = d + vn[q] * sn[q];
+          if (d < 1e-10) break;
+        }
+        if (k < sd + 1) code = dont_collapse(code, col);
+      }
+    }
+    edge_codes[e] = code;
+  }
+  free(vcand);
+  free(exposed);
+  free(smatter);
+  free(vmatter);
+  free(normals);
+  return bad;
+}
+
 static int coarsen(om_mesh* m, om_adapt_opts const* o, signed char* edge_codes,
     int overshoot_allowed, int improve) {
   int dim = m->dim;
@@ -989,6 +1115,16 @@ static int coarsen(om_mesh* m, om_adapt_opts const* o, signed char* edge_codes,
     if (code != DONT_COLLAPSE) anyc = 1;
   }
   if (!anyc) return 0;
+  if (o->should_prevent_coarsen_flip) {
+    if (prevent_coarsen_flip(m, edge_codes)) {
+      fprintf(stderr, "oracle: prevent_coarsen_flip: a check of the reference failed\n");
+      abort();
+    }
+    anyc = 0;
+    for (int e = 0; e < ne; ++e)
+      if (edge_codes[e] != DONT_COLLAPSE) anyc = 1;
+    if (!anyc) return 0;
+  }
   /* coarsen_qualities */
   double* cq = (double*)om_malloc(sizeof(double) * (size_t)ne * 2);
   om_graph const* v2c = om_ask_up(m, 0, dim);

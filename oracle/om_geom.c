@@ -665,3 +665,132 @@ void om_kat_average_metric(int mdim, int n, double const* ms, double* out) {
   for (int i = 0; i < n && i < 16; ++i) p[i] = ms + i * nc;
   om_average_metric(mdim, n, p, out);
 }
+
+/* ------------------------------------------------ side normals, most normal */
+
+static double dot3(double const* a, double const* b, int n) {
+  double out = a[0] * b[0];
+  for (int i = 1; i < n; ++i) out = out + (a[i] * b[i]);
+  return out;
+}
+
+/* normalize, src/Omega_h_vector.hpp:152-154: each component divided by the
+ * norm sqrt(v . v) */
+static void normalize_n(double* v, int n) {
+  double l = om_sqrt(dot3(v, v, n));
+  for (int i = 0; i < n; ++i) v[i] = v[i] / l;
+}
+
+/* get_side_vector, src/Omega_h_shape.hpp:276-298: in 3D the cross product
+ * (b - a) x (c - a) of the triangle (a, b, c); in 2D -perp(b - a) */
+void om_side_vector(int dim, double const* const* p, double* out) {
+  if (dim == 3) {
+    double u[3], w[3];
+    for (int i = 0; i < 3; ++i) {
+      u[i] = p[1][i] - p[0][i];
+      w[i] = p[2][i] - p[0][i];
+    }
+    out[0] = u[1] * w[2] - u[2] * w[1];
+    out[1] = u[2] * w[0] - u[0] * w[2];
+    out[2] = u[0] * w[1] - u[1] * w[0];
+  } else {
+    double v0 = p[1][0] - p[0][0], v1 = p[1][1] - p[0][1];
+    out[0] = -(-v1);
+    out[1] = -v0;
+  }
+}
+
+/* get_most_normal_normal, src/Omega_h_most_normal.hpp:15-88 (Aubry and
+ * Lohner, "On the 'most normal' normal", 2008): the centre of the smallest
+ * circle on the unit sphere holding the n unit vectors N, tried first
+ * through pairs (the normalized bisector), then through triplets (the
+ * direction with equal dot products with all three, from the inverse of the
+ * matrix whose rows are the three vectors).  In 2D it is the normalized sum
+ * of the first two (src/Omega_h_most_normal.hpp:90-93).  Returns 0 on
+ * success, 1 where the reference's checks fail (a triplet centre not
+ * equidistant, or no circle found). */
+int om_most_normal(int dim, int n, double const* N, double* c) {
+  double const inside_tol = 1e-10;
+  if (dim == 2) {
+    c[0] = N[0] + N[2];
+    c[1] = N[1] + N[3];
+    normalize_n(c, 2);
+    return 0;
+  }
+  double scalmin = -1.0;
+  c[0] = c[1] = c[2] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    for (int j = i + 1; j < n; ++j) {
+      double b[3];
+      for (int k = 0; k < 3; ++k) b[k] = N[i * 3 + k] + N[j * 3 + k];
+      normalize_n(b, 3);
+      double scal = dot3(b, N + i * 3, 3);
+      if (scal < scalmin) continue;
+      int l;
+      for (l = 0; l < n; ++l) {
+        double scalt = dot3(N + l * 3, b, 3);
+        if (scalt + inside_tol < scal) break;
+      }
+      if (l < n) continue;
+      for (int k = 0; k < 3; ++k) c[k] = b[k];
+      scalmin = scal;
+    }
+  }
+  if (scalmin != -1.0) return 0;
+  for (int i = 0; i < n; ++i) {
+    for (int j = i + 1; j < n; ++j) {
+      for (int k = j + 1; k < n; ++k) {
+        /* M's columns are N_i, N_j, N_k (src/Omega_h_matrix.hpp:14) */
+        om_mat M;
+        for (int r = 0; r < 3; ++r) {
+          M.m[0][r] = N[i * 3 + r];
+          M.m[1][r] = N[j * 3 + r];
+          M.m[2][r] = N[k * 3 + r];
+        }
+        if (fabs(om_det(3, &M)) < 1e-6) continue;
+        /* invert(transpose(M)) * (1, 1, 1), src/Omega_h_matrix.hpp:89-94,
+         * 281-293,344-350: T = transpose(M); b = rows of cross products of
+         * T's columns; inverse = transpose(b) / det(T) */
+        om_mat T;
+        for (int a = 0; a < 3; ++a)
+          for (int r = 0; r < 3; ++r) T.m[a][r] = M.m[r][a];
+        double bm[3][3];
+        for (int a = 0; a < 3; ++a) {
+          double const* x = T.m[(a + 1) % 3];
+          double const* y = T.m[(a + 2) % 3];
+          bm[a][0] = x[1] * y[2] - x[2] * y[1];
+          bm[a][1] = x[2] * y[0] - x[0] * y[2];
+          bm[a][2] = x[0] * y[1] - x[1] * y[0];
+        }
+        double dt = om_det(3, &T);
+        double nc[3];
+        for (int r = 0; r < 3; ++r) {
+          /* column q of the inverse is (b[0][q], b[1][q], b[2][q]) / det;
+           * the product with (1, 1, 1) sums the columns in order */
+          double s = (bm[r][0] / dt) * 1.0;
+          s = s + (bm[r][1] / dt) * 1.0;
+          s = s + (bm[r][2] / dt) * 1.0;
+          nc[r] = s;
+        }
+        /* the sum above is column-wise: component r of A * v is
+         * A[0][r] v0 + A[1][r] v1 + A[2][r] v2 with A[q][r] = b[r][q] / det */
+        normalize_n(nc, 3);
+        double scal = dot3(nc, N + i * 3, 3);
+        double sj = dot3(nc, N + j * 3, 3), sk = dot3(nc, N + k * 3, 3);
+        if (!om_are_close(1, &scal, &sj, 1e-10, 1e-10) ||
+            !om_are_close(1, &scal, &sk, 1e-10, 1e-10))
+          return 1;
+        if (scal < scalmin) continue;
+        int l;
+        for (l = 0; l < n; ++l) {
+          double scalt = dot3(N + l * 3, nc, 3);
+          if (scalt + inside_tol < scal) break;
+        }
+        if (l < n) continue;
+        for (int q = 0; q < 3; ++q) c[q] = nc[q];
+        scalmin = scal;
+      }
+    }
+  }
+  return (scalmin > -1.0) ? 0 : 1;
+}

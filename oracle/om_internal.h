@@ -55,6 +55,8 @@ void om_limit_gradation_once(int sdim, int mdim, int nverts, int const* a2ab, in
     double const* coords, double const* in, double max_rate, double* out);
 int om_are_close(int n, double const* a, double const* b, double tol, double floor);
 double om_min_quality(om_mesh* m);
+void om_side_vector(int dim, double const* const* p, double* out);
+int om_most_normal(int dim, int n, double const* N, double* c);
 double om_max_length(om_mesh* m);
 
 /* growable int arrays */

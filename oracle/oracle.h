@@ -25,6 +25,7 @@ typedef struct {
   int should_coarsen;
   int should_swap;
   int should_coarsen_slivers;
+  int should_prevent_coarsen_flip;
 } om_adapt_opts;
 
 #define OM_MAX_PASSES 4096

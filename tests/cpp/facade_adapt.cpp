@@ -2,7 +2,11 @@
 // (src/aniso_test.cpp:15-27): build_box, add the vertex metric tag, adapt,
 // read the result back.  Used by tests/test_facade.py.
 //
-//   facade_adapt <dim> <n> <ncomps> <metric.f64> <out_prefix>
+//   facade_adapt <dim> <n> <ncomps> <metric.f64> <out_prefix> [every]
+//
+// With "every", the call site assigns every field of the reference's
+// AdaptOpts (src/Omega_h_adapt.hpp:50-77) and of its TransferOpts
+// (:30-42) explicitly, with should_prevent_coarsen_flip on.
 //
 // writes <out_prefix>.counts (int32 nents per dimension, then the adapt()
 // return value), .verts (int32 element vertices), .coords and .quality
@@ -24,14 +28,15 @@ static void write_file(std::string const& path, T const* data, std::size_t n) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 6) {
-    std::cerr << "usage: facade_adapt <dim> <n> <ncomps> <metric.f64> <out_prefix>\n";
+  if (argc != 6 && argc != 7) {
+    std::cerr << "usage: facade_adapt <dim> <n> <ncomps> <metric.f64> <out_prefix> [every]\n";
     return 2;
   }
   int dim = std::atoi(argv[1]);
   int n = std::atoi(argv[2]);
   int ncomps = std::atoi(argv[3]);
   std::string prefix = argv[5];
+  bool every = argc == 7 && std::string(argv[6]) == "every";
   try {
     Omega_h::Library lib(&argc, &argv);
     Omega_h::Mesh mesh = Omega_h::build_box(lib.world(), OMEGA_H_SIMPLEX, 1.0, 1.0,
@@ -44,6 +49,30 @@ int main(int argc, char** argv) {
         Omega_h::Reals(Omega_h::Write<Omega_h::Real>(metric)));
     Omega_h::AdaptOpts opts(&mesh);
     opts.verbosity = Omega_h::SILENT;
+    if (every) {
+      Omega_h::AdaptOpts d(dim);
+      opts.min_length_desired = d.min_length_desired;
+      opts.max_length_desired = d.max_length_desired;
+      opts.max_length_allowed = d.max_length_allowed;
+      opts.min_quality_allowed = d.min_quality_allowed;
+      opts.min_quality_desired = d.min_quality_desired;
+      opts.nsliver_layers = d.nsliver_layers;
+      opts.verbosity = Omega_h::SILENT;
+      opts.length_histogram_min = 0.0;
+      opts.length_histogram_max = 3.0;
+      opts.nlength_histogram_bins = 10;
+      opts.nquality_histogram_bins = 10;
+      opts.should_refine = true;
+      opts.should_coarsen = true;
+      opts.should_swap = true;
+      opts.should_coarsen_slivers = true;
+      opts.should_prevent_coarsen_flip = true;
+      opts.xfer_opts.type_map.clear();
+      opts.xfer_opts.integral_map.clear();
+      opts.xfer_opts.velocity_density_map.clear();
+      opts.xfer_opts.velocity_momentum_map.clear();
+      opts.xfer_opts.integral_diffuse_map.clear();
+    }
     bool did = Omega_h::adapt(&mesh, opts);
     std::vector<int> counts;
     for (int d = 0; d <= dim; ++d) counts.push_back(mesh.nents(d));

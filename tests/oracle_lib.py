@@ -30,6 +30,7 @@ class AdaptOpts(C.Structure):
         ("should_coarsen", C.c_int),
         ("should_swap", C.c_int),
         ("should_coarsen_slivers", C.c_int),
+        ("should_prevent_coarsen_flip", C.c_int),
     ]
 
 

@@ -28,8 +28,9 @@ def test_facade_header_compiles(tmp_path, std):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("every", [False, True], ids=["defaults", "every_field"])
 @pytest.mark.parametrize("dim,n", [(2, 16), (3, 6)])
-def test_facade_adapt_matches_oracle(tmp_path, dim, n):
+def test_facade_adapt_matches_oracle(tmp_path, dim, n, every):
     from omega_h_amd import workloads as W
     assert os.path.exists(BIN), "build the facade driver with __graft_entry__.build()"
     o = O.Mesh.box(dim, n)
@@ -37,12 +38,15 @@ def test_facade_adapt_matches_oracle(tmp_path, dim, n):
     mfile = tmp_path / "metric.f64"
     np.ascontiguousarray(met, dtype=np.float64).tofile(mfile)
     prefix = str(tmp_path / "out")
-    print(f"facade adapt dim={dim} n={n}", flush=True)
-    r = subprocess.run([BIN, str(dim), str(n), str(nc), str(mfile), prefix],
+    print(f"facade adapt dim={dim} n={n} every={every}", flush=True)
+    r = subprocess.run([BIN, str(dim), str(n), str(nc), str(mfile), prefix] +
+                       (["every"] if every else []),
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     o.set_metric(met, nc)
-    did, _ = o.adapt(O.default_opts(dim))
+    oo = O.default_opts(dim)
+    oo.should_prevent_coarsen_flip = int(every)
+    did, _ = o.adapt(oo)
     counts = np.fromfile(prefix + ".counts", dtype=np.int32)
     assert list(counts[:-1]) == [o.nents(d) for d in range(dim + 1)]
     assert counts[-1] == did
