@@ -375,9 +375,9 @@ static void copy_vtags(om_mesh const* m, om_mesh* nw, mod_out const* mo) {
   for (int t = 0; t < m->nvtags; ++t) {
     om_vtag const* a = &m->vtags[t];
     om_vtag* b = &nw->vtags[t];
-    memcpy(b->name, a->name, sizeof(b->name));
-    b->ncomps = a->ncomps;
-    b->kind = a->kind;
+    *b = *a;
+    b->data = NULL;
+    b->err = NULL;
     if (mo) {
       transfer_r(mo->nnew, a->ncomps, mo, a->data, NULL, &b->data);
     } else {
@@ -396,9 +396,9 @@ static void refine_vtags(om_mesh const* m, om_mesh* nw, int nkeys, int const* ke
   for (int t = 0; t < m->nvtags; ++t) {
     om_vtag const* a = &m->vtags[t];
     om_vtag* b = &nw->vtags[t];
-    memcpy(b->name, a->name, sizeof(b->name));
-    b->ncomps = a->ncomps;
-    b->kind = a->kind;
+    *b = *a;
+    b->data = NULL;
+    b->err = NULL;
     int nc = a->ncomps;
     double* pd = (double*)om_malloc(sizeof(double) * (size_t)(nkeys + 1) * nc);
     for (int k = 0; k < nkeys; ++k) {
@@ -456,8 +456,10 @@ static void etags_from(om_mesh const* old, om_mesh* nw, mod_out const* mo, int c
   for (int t = 0; t < old->netags; ++t) {
     om_etag const* a = &old->etags[t];
     om_etag* b = &nw->etags[t];
-    memcpy(b->name, a->name, sizeof(b->name));
-    b->ncomps = a->ncomps;
+    *b = *a;
+    b->data = NULL;
+    b->err = NULL;
+    b->old = NULL;
     int nc = a->ncomps;
     double* pd = NULL;
     if (prod_src) {
@@ -834,7 +836,11 @@ int om_refine_by_size(om_mesh* m, om_adapt_opts const* o) {
       transfer_i(mo.nnew, &mo, m->class_id[d], pid, &nw->class_id[d]);
       transfer_b(mo.nnew, &mo, m->class_dim[d], pdim, &nw->class_dim[d]);
       inherit_itags(m, nw, d, &mo, d, psd, psrc);
-      if (d == dim) etags_from(m, nw, &mo, psrc);
+      if (d == dim) {
+        etags_from(m, nw, &mo, psrc);
+        om_conserve_pass(m, nw, 0, 1, nkeys, keys2edges, k2p, mo.prods2new, mo.nsame,
+            mo.same2old, mo.same2new);
+      }
       free(pid);
       free(pdim);
       free(psd);
@@ -1288,7 +1294,11 @@ static int coarsen(om_mesh* m, om_adapt_opts const* o, signed char* edge_codes,
     } else {
       transfer_class_from(m, nw, d, d, psrc, &mo);
       inherit_itags(m, nw, d, &mo, d, NULL, psrc);
-      if (d == dim) etags_pointwise(m, nw, &mo, nkeys, keys2verts, 0, k2p, pv);
+      if (d == dim) {
+        etags_pointwise(m, nw, &mo, nkeys, keys2verts, 0, k2p, pv);
+        om_conserve_pass(m, nw, 1, 0, nkeys, keys2verts, k2p, mo.prods2new, mo.nsame,
+            mo.same2old, mo.same2new);
+      }
       transfer_lengths_qualities(m, nw, d, &mo);
     }
     free(old_lows2new_lows);
@@ -1550,7 +1560,11 @@ static void swap_rebuild(om_mesh* m, int nkeys, int const* keys2edges,
       for (int p = k2p[d][k]; p < k2p[d][k + 1]; ++p) src[p] = keys2edges[k];
     transfer_class_from(m, nw, d, 1, src, &mo);
     inherit_itags(m, nw, d, &mo, 1, NULL, src);
-    if (d == dim) etags_pointwise(m, nw, &mo, nkeys, keys2edges, 1, k2p[d], pv[d]);
+    if (d == dim) {
+      etags_pointwise(m, nw, &mo, nkeys, keys2edges, 1, k2p[d], pv[d]);
+      om_conserve_pass(m, nw, 2, 1, nkeys, keys2edges, k2p[d], mo.prods2new, mo.nsame,
+          mo.same2old, mo.same2new);
+    }
     free(src);
     transfer_lengths_qualities(m, nw, d, &mo);
     free(old_lows2new_lows);
@@ -1881,6 +1895,8 @@ static int adapt_impl(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* s,
   if (min_quality(m) >= o->min_quality_desired && lmin >= o->min_length_desired &&
       lmax <= o->max_length_desired)
     return 0;
+  /* setup_conservation_tags */
+  om_conserve_setup(m);
   /* satisfy_lengths */
   int did;
   do {
@@ -1889,12 +1905,18 @@ static int adapt_impl(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* s,
     if (o->should_coarsen && run_pass(m, o, s, 1, digests)) did = 1;
   } while (did);
   /* satisfy_quality */
-  if (min_quality(m) >= o->min_quality_desired) return 1;
-  do {
-    if (o->should_swap && run_pass(m, o, s, 2, digests)) continue;
-    if (o->should_coarsen_slivers && run_pass(m, o, s, 3, digests)) continue;
-    return 1;
-  } while (min_quality(m) < o->min_quality_desired);
+  if (min_quality(m) < o->min_quality_desired) {
+    do {
+      if (o->should_swap && run_pass(m, o, s, 2, digests)) continue;
+      if (o->should_coarsen_slivers && run_pass(m, o, s, 3, digests)) continue;
+      break;
+    } while (min_quality(m) < o->min_quality_desired);
+  }
+  /* correct_integral_errors */
+  if (om_conserve_correct(m, OM_MAX_DIFFUSION_SWEEPS)) {
+    fprintf(stderr, "oracle: correct_integral_errors did not finish\n");
+    abort();
+  }
   return 1;
 }
 

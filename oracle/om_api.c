@@ -362,6 +362,37 @@ int om_add_etag(om_mesh* m, char const* name, int ncomps, double const* data) {
   return 0;
 }
 
+/* the transfer of an element tag: OM_POINTWISE, OM_CONSERVE or OM_DENSITY
+ * (TransferOpts::type_map); for OM_CONSERVE the diffusion of its integral's
+ * error (integral_diffuse_map[integral_map[name]]) */
+int om_set_etag_transfer(om_mesh* m, char const* name, int kind, int diffuse_type, double tol,
+    double floor) {
+  om_etag* t = find_etag(m, name);
+  if (!t || kind < OM_POINTWISE || kind > OM_DENSITY) return -1;
+  t->kind = kind;
+  t->diffuse.type = diffuse_type;
+  t->diffuse.tol = tol;
+  t->diffuse.floor = floor;
+  return 0;
+}
+
+/* a vertex tag as a momentum velocity (OMEGA_H_MOMENTUM_VELOCITY) with its
+ * conserved element density (velocity_density_map) and the diffusion of
+ * its momentum's error (integral_diffuse_map[velocity_momentum_map[name]]) */
+int om_set_momentum_velocity(om_mesh* m, char const* name, char const* density,
+    int diffuse_type, double tol, double floor) {
+  om_vtag* t = find_vtag(m, name);
+  om_etag* d = find_etag(m, density);
+  if (!t || !d || d->ncomps != 1 || d->kind != OM_CONSERVE || t->ncomps != m->dim) return -1;
+  t->kind = OM_MOMENTUM;
+  memset(t->density, 0, sizeof(t->density));
+  strncpy(t->density, density, sizeof(t->density) - 1);
+  t->diffuse.type = diffuse_type;
+  t->diffuse.tol = tol;
+  t->diffuse.floor = floor;
+  return 0;
+}
+
 int om_etag_ncomps(om_mesh* m, char const* name) {
   om_etag* t = find_etag(m, name);
   return t ? t->ncomps : -1;

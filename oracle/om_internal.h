@@ -88,12 +88,30 @@ static inline signed char om_make_code(int flipped, int rot, int which_down) {
  * src/Omega_h_metric.cpp:56-104); both are copied for surviving vertices
  * (src/Omega_h_transfer.cpp:391-428,585-619,701-730) */
 #define OM_MAX_VTAGS 8
-enum { OM_LINEAR = 0, OM_METRIC = 1 };
+enum { OM_LINEAR = 0, OM_METRIC = 1, OM_MOMENTUM = 2 };
+
+/* how an integral's error is diffused before it is corrected
+ * (VarCompareOpts src/Omega_h_compare.hpp:14-21, as used by
+ * diffuse_integrals_weighted src/Omega_h_conserve.cpp:708-734) */
+enum { OM_DIFFUSE_NONE = 0, OM_DIFFUSE_RELATIVE = 1, OM_DIFFUSE_ABSOLUTE = 2 };
+typedef struct {
+  int type;
+  double tol;
+  double floor;
+} om_diffuse;
+
+/* OM_MOMENTUM: a velocity interpolated like OM_LINEAR whose momentum (with
+ * the conserved element density named `density`) is conserved
+ * (OMEGA_H_MOMENTUM_VELOCITY); err holds the elements' momentum error
+ * during adapt() */
 typedef struct {
   char name[64];
   int ncomps;
   int kind;
   double* data;
+  char density[64];
+  om_diffuse diffuse;
+  double* err;
 } om_vtag;
 
 /* a tag on every dimension transferred by inheritance (OMEGA_H_INHERIT,
@@ -113,10 +131,20 @@ typedef struct {
  * products take the value of the old element of the cavity nearest to
  * their centroid (transfer_pointwise_tmpl, src/Omega_h_transfer.cpp:515-583) */
 #define OM_MAX_ETAGS 8
+/* element tag kinds: OM_POINTWISE (OMEGA_H_POINTWISE), OM_CONSERVE (a
+ * density whose integral is conserved, OMEGA_H_CONSERVE), OM_DENSITY (a
+ * density transferred by intersection, OMEGA_H_DENSITY); err holds the
+ * integral error of a conserved density during adapt(), old its values
+ * before the final correction */
+enum { OM_POINTWISE = 0, OM_CONSERVE = 1, OM_DENSITY = 2 };
 typedef struct {
   char name[64];
   int ncomps;
   double* data;
+  int kind;
+  om_diffuse diffuse;
+  double* err;
+  double* old;
 } om_etag;
 
 struct om_mesh {
@@ -173,6 +201,21 @@ void* om_malloc(size_t n);
 void* om_calloc(size_t n, size_t sz);
 int* om_offset_scan(int n, int const* counts); /* n+1 */
 void om_stable_sort_keys(int n, int width, int64_t const* keys, int* perm);
+
+
+/* intersection of two simplices (om_r3d.c) */
+double om_intersect_size(int dim, double const* a, double const* b, int* bad);
+
+/* conservative and momentum transfer (om_conserve.c) */
+int om_conserve_any(om_mesh* m);
+void om_conserve_setup(om_mesh* m);
+void om_conserve_pass(om_mesh* old, om_mesh* nw, int op, int key_dim, int nkeys,
+    int const* keys2kds, int const* k2p, int const* prods2new, int nsame,
+    int const* same2old, int const* same2new);
+int om_conserve_correct(om_mesh* m, int max_iters);
+/* diffusion sweeps after which correct_integral_errors gives up (the
+ * reference's loop has no bound) */
+#define OM_MAX_DIFFUSION_SWEEPS 100000
 
 /* swap3d tables (om_swap3d_tables.c) */
 #define OM_MAX_EDGE_SWAP 7

@@ -168,10 +168,17 @@ void om_mesh_free(om_mesh* m) {
   }
   free(m->coords);
   free(m->metric);
-  for (int t = 0; t < m->nvtags; ++t) free(m->vtags[t].data);
+  for (int t = 0; t < m->nvtags; ++t) {
+    free(m->vtags[t].data);
+    free(m->vtags[t].err);
+  }
   for (int t = 0; t < m->nitags; ++t)
     for (int d = 0; d < 4; ++d) free(m->itags[t].data[d]);
-  for (int t = 0; t < m->netags; ++t) free(m->etags[t].data);
+  for (int t = 0; t < m->netags; ++t) {
+    free(m->etags[t].data);
+    free(m->etags[t].err);
+    free(m->etags[t].old);
+  }
   free(m->length);
   free(m->quality);
   free(m);

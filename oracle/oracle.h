@@ -76,6 +76,12 @@ int om_refine_by_size(om_mesh* m, om_adapt_opts const* o);
 int om_coarsen_by_size(om_mesh* m, om_adapt_opts const* o);
 int om_swap_edges(om_mesh* m, om_adapt_opts const* o);
 int om_coarsen_slivers(om_mesh* m, om_adapt_opts const* o);
+/* conservative transfer: kind 0 pointwise, 1 conserve, 2 density;
+ * diffuse_type 0 none, 1 relative, 2 absolute */
+int om_set_etag_transfer(om_mesh* m, char const* name, int kind, int diffuse_type, double tol,
+    double floor);
+int om_set_momentum_velocity(om_mesh* m, char const* name, char const* density,
+    int diffuse_type, double tol, double floor);
 int om_adapt(om_mesh* m, om_adapt_opts const* o, om_adapt_stats* stats);
 /* om_adapt that also records four FNV-1a digests of the mesh after every
  * pass (digests holds 4 * OM_MAX_PASSES values): 0 vertex lists of every

@@ -129,6 +129,13 @@ def lib():
         L.om_add_etag.argtypes = [P, C.c_char_p, C.c_int, C.POINTER(C.c_double)]
         L.om_etag_ncomps.argtypes = [P, C.c_char_p]
         L.om_get_etag.argtypes = [P, C.c_char_p, C.POINTER(C.c_double)]
+        L.om_set_etag_transfer.argtypes = [P, C.c_char_p, C.c_int, C.c_int, C.c_double,
+                                           C.c_double]
+        L.om_set_momentum_velocity.argtypes = [P, C.c_char_p, C.c_char_p, C.c_int, C.c_double,
+                                               C.c_double]
+        L.om_intersect_size.restype = C.c_double
+        L.om_intersect_size.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int)]
         L.om_kat_barycentric.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]
         L.om_kat_max_eigenvalue.argtypes = [C.c_int, C.POINTER(C.c_double)]
@@ -420,6 +427,21 @@ class Mesh:
         assert a.size == self.nents(self.dim) * ncomps
         assert lib().om_add_etag(self.h, name.encode(), ncomps, _ptr(a, C.c_double)) == 0
 
+    POINTWISE, CONSERVE, DENSITY = 0, 1, 2
+    DIFFUSE_NONE, DIFFUSE_RELATIVE, DIFFUSE_ABSOLUTE = 0, 1, 2
+
+    def set_elem_transfer(self, name, kind, diffuse=(0, 0.0, 0.0)):
+        """The transfer of an element tag (POINTWISE, CONSERVE or DENSITY) and,
+        for CONSERVE, the diffusion (type, tolerance, floor) of its integral's
+        error."""
+        t, tol, fl = diffuse
+        assert lib().om_set_etag_transfer(self.h, name.encode(), kind, t, tol, fl) == 0
+
+    def set_momentum_velocity(self, name, density, diffuse=(0, 0.0, 0.0)):
+        t, tol, fl = diffuse
+        assert lib().om_set_momentum_velocity(self.h, name.encode(), density.encode(), t, tol,
+                                              fl) == 0
+
     def elem_tag(self, name):
         nc = lib().om_etag_ncomps(self.h, name.encode())
         if nc < 0:
@@ -656,3 +678,14 @@ def barycentric(dim, x, simplex):
     lib().om_kat_barycentric(dim, _ptr(xs, C.c_double), _ptr(ps, C.c_double),
                              _ptr(out, C.c_double))
     return out
+
+
+def intersect_size(dim, a, b):
+    """The size of the intersection of simplex a with simplex b (points
+    (dim + 1) x dim), as the oracle's restatement of r3d computes it."""
+    pa = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    pb = np.ascontiguousarray(b, dtype=np.float64).ravel()
+    bad = C.c_int(0)
+    r = lib().om_intersect_size(dim, _ptr(pa, C.c_double), _ptr(pb, C.c_double), C.byref(bad))
+    assert bad.value == 0
+    return r
