@@ -30,8 +30,7 @@
 // C enums of src/Omega_h_defines.h, at global scope as there
 enum Omega_h_Family { OMEGA_H_SIMPLEX = 0 };
 // transfer types of tags carried through adapt() (src/Omega_h_defines.hpp:29-37,
-// same order); OMEGA_H_DENSITY, OMEGA_H_CONSERVE and OMEGA_H_MOMENTUM_VELOCITY
-// are not built here and adapt() throws when a tag asks for them
+// same order)
 enum Omega_h_Transfer {
   OMEGA_H_INHERIT,
   OMEGA_H_LINEAR_INTERP,
@@ -468,8 +467,7 @@ struct VarCompareOpts {
 };
 
 // src/Omega_h_adapt.hpp:30-42: the transfer type of each caller tag, and the
-// maps of conservative and momentum transfer (which are not built here:
-// adapt() throws when any of them is filled)
+// maps of conservative and momentum transfer
 struct TransferOpts {
   std::map<std::string, Omega_h_Transfer> type_map;              // "density" -> CONSERVE
   std::map<std::string, std::string> integral_map;               // "density" -> "mass"
@@ -551,15 +549,42 @@ struct AdaptOpts {
 };
 
 namespace detail {
-// re-register a vertex tag whose transfer type the options name differently
+// the diffusion of a conserved integral's error, by the integral's name
+inline VarCompareOpts diffusion_of(TransferOpts const& x, std::string const& integral) {
+  auto it = x.integral_diffuse_map.find(integral);
+  if (it == x.integral_diffuse_map.end())
+    throw exception("no integral_diffuse_map entry for \"" + integral + "\"");
+  return it->second;
+}
+inline std::string mapped(std::map<std::string, std::string> const& m, std::string const& k,
+    char const* what) {
+  auto it = m.find(k);
+  if (it == m.end()) throw exception(std::string("no ") + what + " entry for \"" + k + "\"");
+  return it->second;
+}
+// give the caller's tags the transfers the options name: vertex tags are
+// re-registered as linear or metric, element tags marked conserved or
+// intersection-transferred, velocities marked as momentum velocities (after
+// their densities)
 inline void apply_transfer_types(Mesh* mesh, AdaptOpts const& opts) {
   auto const& x = opts.xfer_opts;
-  if (!x.integral_map.empty() || !x.velocity_density_map.empty() ||
-      !x.velocity_momentum_map.empty() || !x.integral_diffuse_map.empty())
-    throw exception("conservative and momentum transfer are not supported");
   for (auto const& kv : x.type_map) {
-    if (kv.second == OMEGA_H_POINTWISE) continue;  // element tags are pointwise already
-    if (kv.second == OMEGA_H_INHERIT) continue;    // inheritance tags are inherited already
+    if (kv.second == OMEGA_H_POINTWISE) {
+      if (mesh->elem_tag_ncomps(kv.first) >= 0)
+        check_status(osh_mesh_set_elem_transfer(mesh->handle(), kv.first.c_str(), 0, 0, 0, 0));
+      continue;
+    }
+    if (kv.second == OMEGA_H_INHERIT) continue;  // inheritance tags are inherited already
+    if (kv.second == OMEGA_H_CONSERVE || kv.second == OMEGA_H_DENSITY) {
+      VarCompareOpts d;
+      d.type = VarCompareOpts::NONE;
+      if (kv.second == OMEGA_H_CONSERVE)
+        d = diffusion_of(x, mapped(x.integral_map, kv.first, "integral_map"));
+      check_status(osh_mesh_set_elem_transfer(mesh->handle(), kv.first.c_str(),
+          kv.second == OMEGA_H_CONSERVE ? 1 : 2, int(d.type), d.tolerance, d.floor));
+      continue;
+    }
+    if (kv.second == OMEGA_H_MOMENTUM_VELOCITY) continue;  // below, after the densities
     if (kv.second != OMEGA_H_LINEAR_INTERP && kv.second != OMEGA_H_METRIC)
       throw exception("transfer type of \"" + kv.first + "\" is not supported");
     int nc = mesh->vertex_tag_ncomps(kv.first);
@@ -569,6 +594,14 @@ inline void apply_transfer_types(Mesh* mesh, AdaptOpts const& opts) {
     // the C ABI's vertex tag kinds: 0 linear interpolation, 1 metric
     check_status(osh_mesh_add_vertex_tag(mesh->handle(), kv.first.c_str(), nc,
         kv.second == OMEGA_H_METRIC ? 1 : 0, w.data()));
+  }
+  for (auto const& kv : x.type_map) {
+    if (kv.second != OMEGA_H_MOMENTUM_VELOCITY) continue;
+    auto density = mapped(x.velocity_density_map, kv.first, "velocity_density_map");
+    auto momentum = mapped(x.velocity_momentum_map, kv.first, "velocity_momentum_map");
+    VarCompareOpts d = diffusion_of(x, momentum);
+    check_status(osh_mesh_set_momentum_velocity(mesh->handle(), kv.first.c_str(),
+        density.c_str(), int(d.type), d.tolerance, d.floor));
   }
 }
 }  // namespace detail

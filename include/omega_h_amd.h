@@ -133,6 +133,10 @@ int osh_mesh_set_metric(osh_mesh* mesh, int ncomps, const double* metric);
 int osh_mesh_metric_ncomps(osh_mesh* mesh, int* out);
 int osh_mesh_get_metric(osh_mesh* mesh, double* out);
 int osh_mesh_get_class(osh_mesh* mesh, int ent_dim, int* ids, signed char* dims);
+/* Mesh::set_tag(ent_dim, "class_id" / "class_dim", ...)
+ * (src/Omega_h_mesh.cpp:167-177): a caller's own classification of one
+ * dimension (dims[i] between ent_dim and the mesh dimension) */
+int osh_mesh_set_class(osh_mesh* mesh, int ent_dim, const int* ids, const signed char* dims);
 /* Mesh::ask_lengths / ask_qualities, src/Omega_h_mesh.cpp:374-388 */
 int osh_mesh_get_lengths(osh_mesh* mesh, double* out);
 int osh_mesh_get_qualities(osh_mesh* mesh, double* out);
@@ -232,6 +236,26 @@ int osh_mesh_add_elem_tag(osh_mesh* mesh, const char* name, int ncomps, const do
 int osh_mesh_elem_tag_ncomps(osh_mesh* mesh, const char* name, int* out); /* -1: absent */
 int osh_mesh_get_elem_tag(osh_mesh* mesh, const char* name, double* out);
 int osh_mesh_remove_elem_tag(osh_mesh* mesh, const char* name);
+/* the transfer of an element tag through adapt() (TransferOpts::type_map,
+ * src/Omega_h_adapt.hpp:30-42): kind 0 = OMEGA_H_POINTWISE (the default),
+ * 1 = OMEGA_H_CONSERVE (a density whose integral adapt() conserves:
+ * src/Omega_h_conserve.cpp:372-537 through the passes and
+ * correct_integral_errors :799-829 at the end), 2 = OMEGA_H_DENSITY (a
+ * density transferred by intersection, transfer_by_intersection
+ * :391-455).  For kind 1, how the integral's error is diffused over the
+ * elements before the correction (TransferOpts::integral_diffuse_map, a
+ * VarCompareOpts src/Omega_h_compare.hpp:14-21): diffuse_type 0 none,
+ * 1 relative, 2 absolute, with its tolerance and floor.  Re-adding the
+ * tag's values keeps its transfer. */
+int osh_mesh_set_elem_transfer(osh_mesh* mesh, const char* name, int kind, int diffuse_type,
+    double tol, double floor);
+/* a vertex tag of dim components as a momentum velocity
+ * (OMEGA_H_MOMENTUM_VELOCITY): interpolated like kind 0, and its momentum
+ * with the conserved element density `density` (TransferOpts::
+ * velocity_density_map) is conserved through adapt(), its error diffused as
+ * above (integral_diffuse_map of its velocity_momentum_map name) */
+int osh_mesh_set_momentum_velocity(osh_mesh* mesh, const char* name, const char* density,
+    int diffuse_type, double tol, double floor);
 /* approach_metric(mesh, opts, min_step), src/Omega_h_approach.cpp:83-126:
  * set the metric to the "target_metric" tag, or halve the log-Euclidean
  * step toward it until the mesh's quality and lengths are acceptable;

@@ -126,6 +126,10 @@ def load(path=None):
         "osh_mesh_elem_tag_ncomps": [P, C.c_char_p, IP],
         "osh_mesh_get_elem_tag": [P, C.c_char_p, DP],
         "osh_mesh_remove_elem_tag": [P, C.c_char_p],
+        "osh_mesh_set_class": [P, I, IP, C.POINTER(C.c_byte)],
+        "osh_mesh_set_elem_transfer": [P, C.c_char_p, I, I, C.c_double, C.c_double],
+        "osh_mesh_set_momentum_velocity": [P, C.c_char_p, C.c_char_p, I, C.c_double,
+                                           C.c_double],
         "osh_ill_metric": [P, C.POINTER(AdaptOpts), I],
         "osh_fix": [P, C.POINTER(AdaptOpts), I, IP],
         "osh_fix_for_given_metric": [P, C.POINTER(AdaptOpts), IP],
@@ -443,6 +447,13 @@ class Mesh:
               self.lib)
         return ids, dims
 
+    def set_class(self, d, ids, dims):
+        """The classification of dimension d (class_id, class_dim)."""
+        i = np.ascontiguousarray(ids, dtype=np.int32)
+        c = np.ascontiguousarray(dims, dtype=np.int8)
+        assert len(i) == len(c) == self.nents(d)
+        check(self.lib.osh_mesh_set_class(self.h, d, _p(i, C.c_int), _p(c, C.c_byte)), self.lib)
+
     def lengths(self):
         out = np.zeros(self.nents(1), dtype=np.float64)
         check(self.lib.osh_mesh_get_lengths(self.h, _p(out, C.c_double)), self.lib)
@@ -553,11 +564,30 @@ class Mesh:
         check(self.lib.osh_mesh_set_coords(self.h, _p(a, C.c_double)), self.lib)
 
     def add_elem_tag(self, name, ncomps, data):
-        """An element tag carried through adapt() pointwise (OMEGA_H_POINTWISE)."""
+        """An element tag carried through adapt(), pointwise (OMEGA_H_POINTWISE)
+        unless set_elem_transfer names another transfer."""
         a = np.ascontiguousarray(data, dtype=np.float64)
         assert a.size == self.nents(self.dim) * ncomps
         check(self.lib.osh_mesh_add_elem_tag(self.h, name.encode(), ncomps, _p(a, C.c_double)),
               self.lib)
+
+    POINTWISE, CONSERVE, DENSITY = 0, 1, 2
+    DIFFUSE_NONE, DIFFUSE_RELATIVE, DIFFUSE_ABSOLUTE = 0, 1, 2
+
+    def set_elem_transfer(self, name, kind, diffuse=(0, 0.0, 0.0)):
+        """The transfer of an element tag (POINTWISE, CONSERVE or DENSITY) and,
+        for CONSERVE, the diffusion (type, tolerance, floor) of its integral's
+        error (TransferOpts::type_map and integral_diffuse_map)."""
+        t, tol, fl = diffuse
+        check(self.lib.osh_mesh_set_elem_transfer(self.h, name.encode(), kind, t, tol, fl),
+              self.lib)
+
+    def set_momentum_velocity(self, name, density, diffuse=(0, 0.0, 0.0)):
+        """A vertex tag as a momentum velocity whose momentum with the conserved
+        element density `density` adapt() conserves (OMEGA_H_MOMENTUM_VELOCITY)."""
+        t, tol, fl = diffuse
+        check(self.lib.osh_mesh_set_momentum_velocity(self.h, name.encode(), density.encode(),
+                                                      t, tol, fl), self.lib)
 
     def elem_tag(self, name):
         nc = self._int(self.lib.osh_mesh_elem_tag_ncomps, name.encode())

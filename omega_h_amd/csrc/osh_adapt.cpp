@@ -136,6 +136,7 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
       lmax <= o.max_length_desired)
     return false;
   if (o.verbosity >= 3) print_histograms(c, m, o);
+  conserve_setup(c, m);
   if (o.verbosity >= 2) std::printf("addressing edge lengths\n");
   double t1 = now();
   bool did;
@@ -160,6 +161,9 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
     } while (q < o.min_quality_desired);
   }
   double t3 = now();
+  bool conserving = conserve_any(m);
+  conserve_correct(c, m, kMaxDiffusionSweeps);
+  double t4 = now();
   if (o.verbosity == 1) {
     std::printf("after adapting:\n");
     print_status(c, m, o);
@@ -168,6 +172,7 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
   if (o.verbosity >= 1) {
     std::printf("addressing edge lengths took %g seconds\n", t2 - t1);
     std::printf("addressing element qualities took %g seconds\n", t3 - t2);
+    if (conserving) std::printf("correcting integral errors took %g seconds\n", t4 - t3);
     std::printf("adapting took %g seconds\n\n", now() - t0);
     std::fflush(stdout);
   }

@@ -167,6 +167,16 @@ __global__ void eval_math_kernel(osh::LO n, int which, double const* in, double*
   out[i] = r;
 }
 
+osh::Diffusion diffusion_of(int type, double tol, double floor) {
+  OSH_CHECK(type >= 0 && type <= 2, "diffuse_type must be 0 (none), 1 or 2");
+  OSH_CHECK(tol >= 0.0 && floor >= 0.0, "negative diffusion tolerance or floor");
+  osh::Diffusion d;
+  d.type = type;
+  d.tol = tol;
+  d.floor = floor;
+  return d;
+}
+
 }  // namespace
 
 extern "C" {
@@ -505,6 +515,26 @@ int osh_mesh_get_class(osh_mesh* mesh, int d, int* ids, signed char* dims) {
   });
 }
 
+int osh_mesh_set_class(osh_mesh* mesh, int d, const int* ids, const signed char* dims) {
+  return guarded([&] {
+    check_dim(mesh, d);
+    OSH_CHECK(ids && dims, "null argument");
+    int dim = mesh->m.dim;
+    osh::LO n = mesh->m.nents[d];
+    for (osh::LO i = 0; i < n; ++i)
+      OSH_CHECK(dims[i] >= d && dims[i] <= dim,
+          "an entity classified on a model entity of lower dimension than itself");
+    auto& c = mesh->owner->ctx;
+    osh::LOs id(c, n);
+    osh::Bytes cd(c, n);
+    osh::upload(c, id, ids, n);
+    osh::upload(c, cd, reinterpret_cast<const osh::I8*>(dims), n);
+    c.sync();
+    mesh->m.class_id[d] = id;
+    mesh->m.class_dim[d] = cd;
+  });
+}
+
 int osh_mesh_get_lengths(osh_mesh* mesh, double* out) {
   return guarded([&] {
     check_mesh(mesh);
@@ -794,11 +824,55 @@ int osh_mesh_add_elem_tag(osh_mesh* mesh, const char* name, int ncomps, const do
     osh::upload(c, d, data, n);
     c.sync();
     for (auto& t : mesh->m.etags)
-      if (t.name == name) {
-        t = {name, ncomps, d};
+      if (t.name == name) {  // new values; the transfer set for the tag stays
+        t = t.with_data(d);
+        t.ncomps = ncomps;
         return;
       }
     mesh->m.etags.push_back({name, ncomps, d});
+  });
+}
+
+int osh_mesh_set_elem_transfer(osh_mesh* mesh, const char* name, int kind, int diffuse_type,
+    double tol, double floor) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(name, "null argument");
+    OSH_CHECK(kind >= 0 && kind <= 2, "kind must be 0 (pointwise), 1 (conserve) or 2 (density)");
+    osh::Diffusion d = diffusion_of(diffuse_type, tol, floor);
+    for (auto& t : mesh->m.etags)
+      if (t.name == name) {
+        t.kind = kind;
+        t.diffuse = d;
+        return;
+      }
+    throw osh::Error(std::string("no element tag ") + name);
+  });
+}
+
+int osh_mesh_set_momentum_velocity(osh_mesh* mesh, const char* name, const char* density,
+    int diffuse_type, double tol, double floor) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(name && density, "null argument");
+    osh::Diffusion d = diffusion_of(diffuse_type, tol, floor);
+    bool found = false;
+    for (auto const& e : mesh->m.etags)
+      if (e.name == density) {
+        OSH_CHECK(e.ncomps == 1 && e.kind == 1,
+            "a momentum velocity needs a conserved density of one component");
+        found = true;
+      }
+    OSH_CHECK(found, std::string("no element tag ") + density);
+    for (auto& t : mesh->m.vtags)
+      if (t.name == name) {
+        OSH_CHECK(t.ncomps == mesh->m.dim, "a momentum velocity has one component per dimension");
+        t.kind = 2;
+        t.density = density;
+        t.diffuse = d;
+        return;
+      }
+    throw osh::Error(std::string("no vertex tag ") + name);
   });
 }
 

@@ -901,8 +901,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
       nw.coords = transfer<Real>(c, mo, dim, m.coords, nullptr);
       nw.metric = transfer<Real>(c, mo, nc, m.metric, nullptr);
       for (auto const& t : m.vtags)
-        nw.vtags.push_back(
-            {t.name, t.ncomps, t.kind, transfer<Real>(c, mo, t.ncomps, t.data, nullptr)});
+        nw.vtags.push_back(t.with_data(transfer<Real>(c, mo, t.ncomps, t.data, nullptr)));
       nw.itags = m.itags;
       for (std::size_t t = 0; t < m.itags.size(); ++t)
         nw.itags[t].data[0] = transfer<Real>(c, mo, m.itags[t].ncomps, m.itags[t].data[0], nullptr);
@@ -910,7 +909,10 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
       nw.class_dim[0] = transfer<I8>(c, mo, 1, m.class_dim[0], nullptr);
     } else {
       transfer_class(c, m, nw, d, mo, psrc, d, nullptr);
-      if (d == dim) transfer_pointwise(c, m, nw, mo, 0, keys2verts, k2p[d], pv);
+      if (d == dim) {
+        transfer_pointwise(c, m, nw, mo, 0, keys2verts, k2p[d], pv);
+        conserve_pass(c, m, nw, 1, 0, keys2verts, k2p[d], mo);
+      }
       transfer_measures(c, m, nw, d, mo);
     }
     old_lows2new_lows = mo.old2new;

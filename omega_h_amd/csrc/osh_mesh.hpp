@@ -18,6 +18,17 @@ struct Adj {
   bool exists() const { return ab2b.exists(); }
 };
 
+// how the integral error of a conserved quantity is diffused over the
+// element graph before adapt() corrects the field (VarCompareOpts
+// src/Omega_h_compare.hpp:14-21 as diffuse_integrals_weighted
+// src/Omega_h_conserve.cpp:708-734 uses it): type 0 none, 1 relative,
+// 2 absolute; sweeps run until every weighted error is at most `tol`
+struct Diffusion {
+  int type = 0;
+  double tol = 0.0;
+  double floor = 0.0;
+};
+
 struct DMesh {
   int dim = 0;
   LO nents[4] = {-1, -1, -1, -1};
@@ -33,11 +44,24 @@ struct DMesh {
   // refinement midpoints (average_field, src/Omega_h_mesh.cpp:822-844), kind 1
   // values get log-Euclidean midpoint metrics (get_mident_metrics,
   // src/Omega_h_metric.cpp:56-104); surviving vertices keep theirs
+  // kind 2: a momentum velocity (OMEGA_H_MOMENTUM_VELOCITY), interpolated
+  // like kind 0, whose momentum with the conserved element density `density`
+  // is conserved; `err` holds the elements' momentum errors during adapt()
   struct VertexTag {
     std::string name;
     int ncomps;
     int kind;
     Reals data;
+    std::string density;
+    Diffusion diffuse;
+    Reals err;
+    // the same tag carrying other values (errors are not carried along)
+    VertexTag with_data(Reals d) const {
+      VertexTag t = *this;
+      t.data = d;
+      t.err = Reals();
+      return t;
+    }
   };
   std::vector<VertexTag> vtags;
   // tags on every dimension transferred by inheritance (OMEGA_H_INHERIT,
@@ -54,10 +78,25 @@ struct DMesh {
   // element's value; coarsening and swap products take the value of the old
   // element of the cavity nearest to their centroid
   // (transfer_pointwise_tmpl, src/Omega_h_transfer.cpp:515-583)
+  // kind 0: pointwise; kind 1: a density whose integral is conserved
+  // (OMEGA_H_CONSERVE; `err` holds the elements' integral errors during
+  // adapt() and `old` its values before the final correction); kind 2: a
+  // density transferred by intersection (OMEGA_H_DENSITY)
   struct ElemTag {
     std::string name;
     int ncomps;
     Reals data;
+    int kind = 0;
+    Diffusion diffuse;
+    Reals err;
+    Reals old;
+    ElemTag with_data(Reals d) const {
+      ElemTag t = *this;
+      t.data = d;
+      t.err = Reals();
+      t.old = Reals();
+      return t;
+    }
   };
   std::vector<ElemTag> etags;
   Reals length;   // cached edge lengths (may be absent)

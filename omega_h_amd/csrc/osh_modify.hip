@@ -847,8 +847,8 @@ void transfer_elem_tags(Ctx& c, DMesh const& old, DMesh& nw, ModOut const& mo,
       pd = Reals(c, nprods * t.ncomps);
       OSH_LAUNCH(gather_rows, nprods, c, t.ncomps, prod_src->data(), t.data.data(), pd.data());
     }
-    nw.etags.push_back({t.name, t.ncomps, transfer<Real>(c, mo, t.ncomps, t.data,
-        prod_src ? &pd : nullptr)});
+    nw.etags.push_back(t.with_data(transfer<Real>(c, mo, t.ncomps, t.data,
+        prod_src ? &pd : nullptr)));
   }
 }
 

@@ -96,6 +96,20 @@ void transfer_elem_tags(Ctx& c, DMesh const& old, DMesh& nw, ModOut const& mo,
 void transfer_pointwise(Ctx& c, DMesh& old, DMesh& nw, ModOut const& mo, int key_dim,
     LOs const& keys2kds, LOs const& keys2prods, LOs const& prod_verts);
 
+// conservative and momentum transfer (osh_conserve.hip): setup_conservation_tags
+// (src/Omega_h_conserve.cpp:578-606) zeroes the integral errors at the start
+// of adapt(); conserve_pass transfers densities by intersection and the
+// errors through one pass (op 0 refine, 1 coarsen, 2 swap; keys of
+// dimension key_dim, k2p their element products, after the other element
+// transfers of the pass); conserve_correct is correct_integral_errors
+// (src/Omega_h_conserve.cpp:799-829) at the end of adapt()
+bool conserve_any(DMesh const& m);
+void conserve_setup(Ctx& c, DMesh& m);
+void conserve_pass(Ctx& c, DMesh& old, DMesh& nw, int op, int key_dim, LOs const& keys2kds,
+    LOs const& k2p, ModOut const& mo);
+constexpr int kMaxDiffusionSweeps = 100000;
+void conserve_correct(Ctx& c, DMesh& m, int max_sweeps);
+
 // *any_cands (when given) tells whether any entity was a candidate; for
 // edges, `edge_elems` (when given) is an edge -> element adjacency holding at
 // least the rows of the candidate edges; for vertices, `star` (when given) is

@@ -100,6 +100,10 @@ void Ctx::sync() {
     if (code == kFlipNormals)
       throw Error("prevent_coarsen_flip: too many exposed sides at a vertex, or no most normal "
                   "normal");
+    if (code == kZeroSizeCavity)
+      throw Error("conservation: an integral error in a cavity of zero size");
+    if (code == kPolytopeOverflow)
+      throw Error("conservation: a simplex intersection left the polytope buffer");
     if (code == kDegenerateEigen)
       throw Error("eigen: no eigenvector for a simple eigenvalue (degenerate metric)");
     throw Error("device error " + std::to_string(code));

@@ -298,12 +298,15 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
         else
           OSH_LAUNCH(midpoint_coords, nkeys, c, t.ncomps, keys2edges.data(), m.down[1].data(),
               t.data.data(), pt.data());
-        nw.vtags.push_back({t.name, t.ncomps, t.kind, transfer<Real>(c, mo, t.ncomps, t.data, &pt)});
+        nw.vtags.push_back(t.with_data(transfer<Real>(c, mo, t.ncomps, t.data, &pt)));
       }
       transfer_class(c, m, nw, 0, mo, keys2edges, 1, nullptr);
     } else {
       transfer_class(c, m, nw, d, mo, psrc, d, &psrc_dim);
-      if (d == dim) transfer_elem_tags(c, m, nw, mo, &psrc);
+      if (d == dim) {
+        transfer_elem_tags(c, m, nw, mo, &psrc);
+        conserve_pass(c, m, nw, 0, 1, keys2edges, k2p[d], mo);
+      }
       transfer_measures(c, m, nw, d, mo);
     }
     old_lows2new_lows = mo.old2new;

@@ -187,7 +187,9 @@ enum DeviceError {
   kBrokenSwapLoop = 6,
   kDegenerateEigen = 7,
   kBadPart = 8,
-  kFlipNormals = 9
+  kFlipNormals = 9,
+  kZeroSizeCavity = 10,
+  kPolytopeOverflow = 11
 };
 
 template <class T>

@@ -802,7 +802,10 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
           cache->q.data(), next.valid.data(), next.cfg.data(), next.keep.data(), next.q.data());
       *cache = next;
     }
-    if (d == dim) transfer_pointwise(c, m, nw, mo, 1, keys2edges, k2p[d], pv[d]);
+    if (d == dim) {
+      transfer_pointwise(c, m, nw, mo, 1, keys2edges, k2p[d], pv[d]);
+      conserve_pass(c, m, nw, 2, 1, keys2edges, k2p[d], mo);
+    }
     transfer_measures(c, m, nw, d, mo);
     old_lows2new_lows = mo.old2new;
     low_mo = mo;

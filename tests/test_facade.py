@@ -122,3 +122,51 @@ def test_facade_warp_loop_matches_oracle(tmp_path):
     a2ab, ab2b = o.dual()
     assert np.array_equal(np.fromfile(prefix + ".dual", dtype=np.int32),
                           np.concatenate([a2ab, ab2b]))
+
+
+def sequential_centroids(coords, ev2v, dim):
+    """Element centroids summed vertex by vertex from 0.0, then divided, as
+    average_field (src/Omega_h_mesh.cpp:822-844) forms them."""
+    c = coords.reshape(-1, dim)
+    ev = ev2v.reshape(-1, dim + 1)
+    s = np.zeros((len(ev), dim))
+    for k in range(dim + 1):
+        s = s + c[ev[:, k]]
+    return s / (dim + 1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,n", [(2, 10), (3, 6)])
+def test_facade_conserve_matches_oracle(tmp_path, dim, n):
+    """src/2d_conserve_test.cpp and src/3d_conserve_test.cpp through the
+    facade's TransferOpts maps, against the oracle bit for bit."""
+    prog = os.path.join(ROOT, "omega_h_amd", "_build", "facade_conserve")
+    assert os.path.exists(prog), "build the facade programs with __graft_entry__.build()"
+    o = O.Mesh.box(dim, n)
+    metric = o.implied_isos() * (1.0 / (1.3 * 1.3))
+    metric.tofile(tmp_path / "m.f64")
+    prefix = str(tmp_path / "out")
+    print(f"facade conserve dim={dim} n={n}", flush=True)
+    r = subprocess.run([prog, str(dim), str(n), str(tmp_path / "m.f64"), prefix],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    o.set_metric(metric, 1)
+    cen = sequential_centroids(o.coords(), o.verts(dim), dim)
+    o.add_elem_tag("density", 1, np.where(cen[:, 1] > 0.5, 1e-4, 1.0))
+    c = o.coords().reshape(-1, dim)
+    x2 = c[:, 0] - 0.3
+    vel = np.zeros((len(c), dim))
+    vel[:, 0] = np.sign(x2) * np.sqrt(np.abs(x2))
+    o.add_vertex_tag("velocity", dim, O.Mesh.LINEAR, vel.ravel())
+    o.set_elem_transfer("density", O.Mesh.CONSERVE, (1, 0.9, 0.0))
+    o.set_momentum_velocity("velocity", "density", (1, 0.02, 1e-6))
+    opts = O.default_opts(dim)
+    opts.verbosity = 0
+    o.adapt(opts)
+    counts = np.fromfile(prefix + ".counts", dtype=np.int32)
+    assert list(counts) == [o.nents(d) for d in range(dim + 1)]
+    assert np.array_equal(np.fromfile(prefix + ".verts", dtype=np.int32), o.verts(dim))
+    for name, want in (("coords", o.coords()), ("density", o.elem_tag("density")),
+                       ("velocity", o.vertex_tag("velocity"))):
+        got = np.fromfile(prefix + "." + name, dtype=np.float64)
+        assert np.array_equal(got.view(np.int64), want.view(np.int64)), name
