@@ -203,6 +203,9 @@ def roofline_only(ctx, mesh, launches):
     for r in kernel_rooflines(ctx, mesh, launches).values():
         log(f"{r['kernel']}: {r['launch_ms'] * 1e3:.1f} us per launch, {r['bytes']} B, "
             f"{r['achieved']:.0f} GB/s")
+    cq = coarsen_roofline(ctx, mesh, launches)
+    log(f"{cq['kernel']}: {cq['launch_ms'] * 1e3:.1f} us per launch, {cq['evaluations']} "
+        f"element qualities, {cq['tflops']:.2f} TFLOP/s")
     n = COPY_BYTES // 8
     a = DeviceArray(np.arange(n, dtype=np.float64))
     b = DeviceArray(dtype=np.float64, n=n)
@@ -211,27 +214,35 @@ def roofline_only(ctx, mesh, launches):
     log(f"copied {COPY_BYTES} B 5 times")
 
 
+TRAFFIC_SUMMARIES = ("measure_kernels_64cube_hbm_traffic.txt",
+                     "coarsen_quality_lanes_64cube_counters.txt")
+
+
 def traffic_per_launch(kernel):
     """HBM bytes per launch of `kernel` over the whole 64^3 input, from the
-    counter summary in profiles/ (FETCH_SIZE + WRITE_SIZE, each scaled by
+    counter summaries in profiles/ (FETCH_SIZE + WRITE_SIZE, each scaled by
     what the counter reads for a device copy of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "measure_kernels_64cube_hbm_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        parts = line.split()
-        if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == kernel:
-            return float(parts[2])
+    for name in TRAFFIC_SUMMARIES:
+        path = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(path):
+            continue
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == kernel:
+                return float(parts[2])
     return None
 
 
 COPY_BYTES = 64 << 20
 
 
-def constant_metric_request(rng, dim, elems, h_lo, h_hi):
+def constant_metric_request(rng, dim, elems, h_lo, h_hi, clamp=None):
     """A box of about `elems` elements under a constant anisotropic metric:
-    a uniformly random rotation and eigen-lengths log-uniform in [h_lo, h_hi]."""
+    a uniformly random rotation and eigen-lengths log-uniform in [h_lo, h_hi],
+    the metric scaled by the reference's element-count rule when it is
+    expected to give more than `clamp` elements (workloads.clamp_metric)."""
     import numpy as np
+    from omega_h_amd import workloads as W
     n = max(1, round((elems / 2) ** 0.5)) if dim == 2 else max(1, round((elems / 6) ** (1 / 3)))
     g = rng.normal(size=(dim, dim))
     q, r = np.linalg.qr(g)
@@ -242,6 +253,8 @@ def constant_metric_request(rng, dim, elems, h_lo, h_hi):
         v = [m[0, 0], m[1, 1], m[2, 2], m[0, 1], m[1, 2], m[0, 2]]
     else:
         v = [m[0, 0], m[1, 1], m[0, 1]]
+    if clamp is not None:
+        v = list(W.clamp_metric(v, dim, clamp)[0])
     return dim, n, v
 
 
@@ -253,7 +266,9 @@ def percentile(xs, p):
 def serve_main(args):
     """Poisson stream of adapt requests (30% 2D, 70% 3D boxes, element counts
     log-uniform in [1e3, --max-elems], constant metrics as in the batched
-    form) against the request server; p50/p99 latency and throughput."""
+    form, each clamped by the reference's element-count rule to
+    --clamp-elems expected elements) against the request server; p50/p99
+    latency and throughput."""
     import numpy as np
     import omega_h_amd as A
     ctx = A.Context(0)
@@ -266,8 +281,8 @@ def serve_main(args):
         server = A.Server(nslots=args.slots, hbm_budget_bytes=args.budget)
         # one request of each dimension first, so that code paths are warm
         for d in (2, 3):
-            server.wait(server.submit(*constant_metric_request(rng, d, 1e3, args.h_lo, args.h_hi),
-                                      opts[d]))
+            server.wait(server.submit(*constant_metric_request(rng, d, 1e3, args.h_lo, args.h_hi,
+                                                               args.clamp_elems), opts[d]))
         t = 0.0
         arrivals = []
         while True:
@@ -276,7 +291,8 @@ def serve_main(args):
                 break
             dim = 2 if rng.random() < 0.3 else 3
             elems = float(np.exp(rng.uniform(np.log(1e3), np.log(args.max_elems))))
-            arrivals.append((t, constant_metric_request(rng, dim, elems, args.h_lo, args.h_hi)))
+            arrivals.append((t, constant_metric_request(rng, dim, elems, args.h_lo, args.h_hi,
+                                                        args.clamp_elems)))
         t0 = time.perf_counter()
         tickets = []
         for at, (dim, n, v) in arrivals:
@@ -302,8 +318,9 @@ def serve_main(args):
         "data": "synthetic",
         "config": {"workload": f"Poisson adapt requests, 30% 2D / 70% 3D boxes, elements "
                                f"log-uniform in [1e3, {args.max_elems:g}], constant metrics with "
-                               f"eigen-lengths log-uniform in [{args.h_lo}, {args.h_hi}], "
-                               f"{args.duration:g} s per rate",
+                               f"eigen-lengths log-uniform in [{args.h_lo}, {args.h_hi}] "
+                               f"clamped to {args.clamp_elems:g} expected elements by the "
+                               f"reference's element-count rule, {args.duration:g} s per rate",
                    "slots": args.slots, "hbm_budget_bytes": args.budget},
         "rates": out,
     }
@@ -480,8 +497,9 @@ def main():
     ap.add_argument("--duration", type=float, default=20.0)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget", type=float, default=200e9)
-    ap.add_argument("--max-elems", type=float, default=2e5)
-    ap.add_argument("--h-lo", type=float, default=0.02)
+    ap.add_argument("--max-elems", type=float, default=2e6)
+    ap.add_argument("--clamp-elems", type=float, default=2.5e5)
+    ap.add_argument("--h-lo", type=float, default=0.005)
     ap.add_argument("--h-hi", type=float, default=0.1)
     ap.add_argument("--boxes", type=int, default=256)
     ap.add_argument("--batch-seed", type=lambda x: int(x, 0), default=0x0A5A17E5EED5EED5)
