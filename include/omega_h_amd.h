@@ -194,6 +194,19 @@ int osh_measure_qualities(osh_ctx* ctx, int dim, int metric_ncomps, int n,
  * osh_measure_edges_metric. */
 int osh_offset_scans(osh_ctx* ctx, int njobs, const int* const* in, const int* n,
     int* const* out);
+/* osh_offset_scans without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The look-back
+ * state lives in a fixed 128 KiB scratch array per GPU, set up when a context
+ * is created on that GPU; one call holds at most 16384 tiles of 4096 values
+ * (67,108,864 values over all its jobs), and a larger call is refused with a
+ * nonzero status before anything runs.  Calls that overlap on one GPU (two
+ * streams, two threads) race for the scratch; the next sync of a context then
+ * reports a scan error.  Same arguments, results and stream ordering as
+ * osh_offset_scans; errors found on the device surface at the next
+ * osh_ctx_synchronize. */
+int osh_offset_scans_noalloc(osh_ctx* ctx, int njobs, const int* const* in, const int* n,
+    int* const* out);
 
 /* limit_metric_gradation(mesh, metric, max_rate, tol),
  * src/Omega_h_metric.cpp:200-220, applied to the mesh's metric in place

@@ -152,6 +152,7 @@ def load(path=None):
         "osh_measure_edges_metric": [P, I, I, I, P, P, P, P, P],
         "osh_measure_qualities": [P, I, I, I, P, P, P, P, P],
         "osh_offset_scans": [P, I, P, IP, P],
+        "osh_offset_scans_noalloc": [P, I, P, IP, P],
         "osh_refine_qualities": [P, I, IP, DP],
         "osh_find_loop": [P, I, IP, IP, IP],
         "osh_coarsen_qualities": [P, I, P, P, P, I, C.c_double],
@@ -253,6 +254,15 @@ class Context:
         outs = (C.c_void_p * k)(*out_ptrs)
         ns = (C.c_int * k)(*sizes)
         check(self.lib.osh_offset_scans(self.h, k, ins, ns, outs), self.lib)
+
+    def offset_scans_noalloc(self, in_ptrs, sizes, out_ptrs):
+        """offset_scans without any allocation (a fixed per-GPU scratch; at most
+        16384 tiles of 4096 values per call)."""
+        k = len(in_ptrs)
+        ins = (C.c_void_p * k)(*[p or None for p in in_ptrs])
+        outs = (C.c_void_p * k)(*out_ptrs)
+        ns = (C.c_int * k)(*sizes)
+        check(self.lib.osh_offset_scans_noalloc(self.h, k, ins, ns, outs), self.lib)
 
     def synchronize(self):
         check(self.lib.osh_ctx_synchronize(self.h), self.lib)

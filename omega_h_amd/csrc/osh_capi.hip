@@ -694,6 +694,21 @@ int osh_offset_scans(osh_ctx* ctx, int njobs, const int* const* in, const int* n
   });
 }
 
+int osh_offset_scans_noalloc(osh_ctx* ctx, int njobs, const int* const* in, const int* n,
+    int* const* out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && in && n && out, "null argument");
+    OSH_CHECK(njobs >= 1 && njobs <= osh::kMaxScanJobs, "1 to 12 scans per call");
+    osh::ScanJob jobs[osh::kMaxScanJobs];
+    for (int j = 0; j < njobs; ++j) {
+      OSH_CHECK(n[j] >= 0, "negative count");
+      OSH_CHECK(out[j] && (in[j] || n[j] == 0), "null array");
+      jobs[j] = {in[j], osh::kScanLO, n[j], out[j], nullptr};
+    }
+    osh::offset_scans_noalloc(ctx->ctx, jobs, njobs);
+  });
+}
+
 int osh_limit_metric_gradation(osh_mesh* mesh, double max_rate, double tol, int* nsteps) {
   return guarded([&] {
     check_mesh(mesh);

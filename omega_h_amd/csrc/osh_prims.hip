@@ -5,6 +5,14 @@
 
 #include "osh_runtime.hpp"
 
+// Scratch of the scans that allocate nothing (offset_scans_noalloc): the
+// tile status words of up to kNoAllocTiles tiles, then the tile ticket
+// counter.  Each GPU holds its own copy (128 KiB) for the life of the
+// process; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device.
+static constexpr unsigned kNoAllocTiles = 16384;
+static __device__ __align__(256) unsigned long long scratch[kNoAllocTiles + 1];
+
 namespace osh {
 
 std::size_t Pool::round_up(std::size_t bytes) {
@@ -68,6 +76,7 @@ Ctx::Ctx() : pool(std::make_shared<Pool>()) {
   OSH_HIP(hipMalloc((void**)&tile_counter, sizeof(unsigned)));
   unsigned zero = 0;
   OSH_HIP(hipMemcpy(tile_counter, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  noalloc_setup(device);
 }
 
 Ctx::~Ctx() {
@@ -368,6 +377,84 @@ void offset_scans(Ctx& c, ScanJob const* jobs, int njobs) {
   }
   hipLaunchKernelGGL(scan_lookback, dim3(tiles), dim3(kBlock), 0, c.stream, b, status,
       c.tile_counter, ticket_base, multi ? 1 : 0, epoch, c.error_word());
+  OSH_HIP(hipGetLastError());
+}
+
+namespace {
+
+// per device: the address of its `scratch`, and the epoch and ticket count
+// of the scans made in it so far
+struct NoAllocDevice {
+  unsigned long long* words = nullptr;
+  unsigned epoch = 0;
+  unsigned tickets = 0;
+  bool cleared = false;
+};
+constexpr int kMaxNoAllocDevices = 64;
+std::mutex noalloc_mu;
+NoAllocDevice noalloc_devices[kMaxNoAllocDevices];
+
+}  // namespace
+
+void noalloc_setup(int device) {
+  OSH_CHECK(device >= 0 && device < kMaxNoAllocDevices, "device number out of range");
+  std::lock_guard<std::mutex> g(noalloc_mu);
+  NoAllocDevice& d = noalloc_devices[device];
+  if (d.words) return;
+  void* p = nullptr;
+  OSH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  OSH_CHECK(p != nullptr, "no device address for the scan scratch");
+  d.words = static_cast<unsigned long long*>(p);
+}
+
+void offset_scans_noalloc(Ctx& c, ScanJob const* jobs, int njobs) {
+  OSH_CHECK(njobs >= 1 && njobs <= kMaxScanJobs, "offset_scans: 1 to 12 jobs per launch");
+  ScanBatch b{};
+  std::int64_t tiles = 0;
+  bool multi = false;
+  for (int j = 0; j < njobs; ++j) {
+    OSH_CHECK(jobs[j].n >= 0 && jobs[j].out, "offset_scans: bad job");
+    b.in[j] = jobs[j].in;
+    b.out[j] = jobs[j].out;
+    b.total[j] = jobs[j].total;
+    b.n[j] = jobs[j].n;
+    b.kind[j] = jobs[j].kind;
+    b.first[j] = unsigned(tiles);
+    std::int64_t nt = jobs[j].n == 0 ? 1 : (std::int64_t(jobs[j].n) + kScanTile - 1) / kScanTile;
+    multi = multi || nt > 1;
+    tiles += nt;
+    OSH_CHECK(tiles <= std::int64_t(kNoAllocTiles),
+        "offset_scans_noalloc: more than 16384 tiles of 4096 values in one call");
+  }
+  b.first[njobs] = unsigned(tiles);
+  b.count = njobs;
+  unsigned epoch = 0, ticket_base = 0;
+  unsigned long long* words = nullptr;
+  {
+    std::lock_guard<std::mutex> g(noalloc_mu);
+    OSH_CHECK(c.device >= 0 && c.device < kMaxNoAllocDevices, "device number out of range");
+    NoAllocDevice& d = noalloc_devices[c.device];
+    OSH_CHECK(d.words, "the scan scratch of this device is not set up");
+    words = d.words;
+    if (multi) {
+      // status words left by earlier scans read as unpublished under a new
+      // epoch; before the 24-bit epoch field wraps (and before the first
+      // use) a fill kernel clears them
+      if (!d.cleared || d.epoch >= Ctx::kMaxScanEpoch) {
+        hipLaunchKernelGGL(zero_u64, dim3(nblocks(LO(kNoAllocTiles))), dim3(kBlock), 0,
+            c.stream, LO(kNoAllocTiles), words);
+        OSH_HIP(hipGetLastError());
+        d.epoch = 0;
+        d.cleared = true;
+      }
+      epoch = ++d.epoch;
+      ticket_base = d.tickets;
+      d.tickets += unsigned(tiles);
+    }
+  }
+  hipLaunchKernelGGL(scan_lookback, dim3(unsigned(tiles)), dim3(kBlock), 0, c.stream, b,
+      multi ? words : nullptr, reinterpret_cast<unsigned*>(words + kNoAllocTiles), ticket_base,
+      multi ? 1 : 0, epoch, c.error_word());
   OSH_HIP(hipGetLastError());
 }
 

@@ -252,6 +252,13 @@ struct ScanJob {
   LO* total;  // device-visible grand total (a pinned slot), or nullptr
 };
 void offset_scans(Ctx& c, ScanJob const* jobs, int njobs);
+// the same scans with no allocation: the tile status words and ticket
+// counter live in a fixed per-device scratch array (at most 16384 tiles of
+// 4096 values per call, all jobs together); noalloc_setup finds that
+// array's address on a device (once, with the device current).  Calls that
+// overlap on one GPU race for the scratch.
+void noalloc_setup(int device);
+void offset_scans_noalloc(Ctx& c, ScanJob const* jobs, int njobs);
 // n+1 exclusive offsets; with `total`, also the sum (read back after a sync)
 LOs offset_scan(Ctx& c, LOs const& counts, LO* total = nullptr);
 LOs offset_scan_bytes(Ctx& c, Bytes const& counts, LO* total = nullptr);
