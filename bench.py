@@ -237,25 +237,23 @@ COPY_BYTES = 64 << 20
 
 
 def constant_metric_request(rng, dim, elems, h_lo, h_hi, clamp=None):
-    """A box of about `elems` elements under a constant anisotropic metric:
-    a uniformly random rotation and eigen-lengths log-uniform in [h_lo, h_hi],
-    the metric scaled by the reference's element-count rule when it is
-    expected to give more than `clamp` elements (workloads.clamp_metric)."""
-    import numpy as np
     from omega_h_amd import workloads as W
-    n = max(1, round((elems / 2) ** 0.5)) if dim == 2 else max(1, round((elems / 6) ** (1 / 3)))
-    g = rng.normal(size=(dim, dim))
-    q, r = np.linalg.qr(g)
-    q = q * np.sign(np.diag(r))
-    h = np.exp(rng.uniform(np.log(h_lo), np.log(h_hi), dim))
-    m = q @ np.diag(1.0 / h ** 2) @ q.T
-    if dim == 3:
-        v = [m[0, 0], m[1, 1], m[2, 2], m[0, 1], m[1, 2], m[0, 2]]
+    return W.serve_request(rng, dim, elems, h_lo, h_hi, clamp)
+
+
+def apply_tuning(ctx, args, workload):
+    """The lane counts bench runs with (see --tuning); returns them."""
+    from omega_h_amd import autotune
+    if args.tuning == "none":
+        chosen = None
+    elif args.tuning == "auto":
+        chosen = autotune.load_tuning(workload) if workload else None
     else:
-        v = [m[0, 0], m[1, 1], m[0, 1]]
-    if clamp is not None:
-        v = list(W.clamp_metric(v, dim, clamp)[0])
-    return dim, n, v
+        with open(args.tuning) as f:
+            chosen = json.load(f)["tuning"]
+    if chosen:
+        ctx.apply_tuning(chosen)
+    return {"lanes": ctx.tuning(), "source": "autotuner" if chosen else "defaults"}
 
 
 def percentile(xs, p):
@@ -271,6 +269,7 @@ def serve_main(args):
     latency and throughput."""
     import numpy as np
     import omega_h_amd as A
+    from omega_h_amd import workloads as W
     ctx = A.Context(0)
     opts = {2: ctx.default_opts(2), 3: ctx.default_opts(3)}
     for o in opts.values():
@@ -283,16 +282,8 @@ def serve_main(args):
         for d in (2, 3):
             server.wait(server.submit(*constant_metric_request(rng, d, 1e3, args.h_lo, args.h_hi,
                                                                args.clamp_elems), opts[d]))
-        t = 0.0
-        arrivals = []
-        while True:
-            t += rng.exponential(1.0 / rate)
-            if t > args.duration:
-                break
-            dim = 2 if rng.random() < 0.3 else 3
-            elems = float(np.exp(rng.uniform(np.log(1e3), np.log(args.max_elems))))
-            arrivals.append((t, constant_metric_request(rng, dim, elems, args.h_lo, args.h_hi,
-                                                        args.clamp_elems)))
+        arrivals = W.serve_stream(rng, rate, args.duration, args.max_elems, args.h_lo,
+                                  args.h_hi, args.clamp_elems)
         t0 = time.perf_counter()
         tickets = []
         for at, (dim, n, v) in arrivals:
@@ -341,6 +332,7 @@ def batch_main(args):
     import omega_h_amd as A
     from omega_h_amd import workloads as W
     ctx = A.Context(0)
+    tuning = apply_tuning(ctx, args, "batch")
     boxes = W.batch_boxes(args.boxes, args.batch_seed, n_range=(8, 48), h_range=(0.005, 0.1),
                           max_nelems=args.max_box_elems)
     nclamped = sum(1 for b in boxes if b[3] < 1.0)
@@ -410,7 +402,8 @@ def batch_main(args):
                                "by random unit quaternions, eigen-lengths log-uniform in "
                                f"[0.005, 0.1], clamped to {args.max_box_elems:g} expected "
                                "elements per box", "boxes_clamped": nclamped,
-                   "expected_tets_unclamped": expected_raw, "expected_tets": expected},
+                   "expected_tets_unclamped": expected_raw, "expected_tets": expected,
+                   "tuning": tuning},
         "tets_in": joined.nents(3),
         "fp64": {k: v for k, v in res[0].items() if k != "mesh"},
         "mixed": {k: v for k, v in res[1].items() if k != "mesh"},
@@ -507,6 +500,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=20260101)
+    ap.add_argument("--tuning", default="auto",
+                    help="lane counts: 'auto' (omega_h_amd/tuning/<workload>.json when "
+                         "the autotuner wrote one), 'none' (the defaults) or a JSON path")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the measure-kernel sweeps and a 64 MiB device copy (the "
                          "reference kernel for calibrating HBM byte counters)")
@@ -529,6 +525,7 @@ def main():
     from omega_h_amd import workloads as W
 
     ctx = A.Context(0)
+    tuning = apply_tuning(ctx, args, "levelset64" if args.n == 64 else None)
     base = A.Mesh.box(ctx, 3, args.n)
     met, nc = W.levelset_metric_3d(base.coords())
     base.set_metric(met, nc)
@@ -618,7 +615,8 @@ def main():
         "config": {"workload": f"build_box unit cube {args.n}^3 x 6 tets ({nelems0} tets), "
                                "level-set anisotropic metric, full adapt() FP64, default "
                                "AdaptOpts with verbosity SILENT",
-                   "passes_per_step": len(runs[-1]), "final_tets": final},
+                   "passes_per_step": len(runs[-1]), "final_tets": final,
+                   "tuning": tuning},
         "roofline": {"bound": "fp64", "achieved": cq["tflops"], "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": cq["tflops"] / FP64_PEAK_TFLOPS,
                      "traffic": traffic_per_launch(cq["kernel"]) if args.n == 64 else None,

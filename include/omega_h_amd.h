@@ -72,6 +72,14 @@ int osh_ctx_destroy(osh_ctx* ctx);
 int osh_ctx_synchronize(osh_ctx* ctx);
 /* the hipStream_t the context orders its work on (for events and interop) */
 int osh_ctx_stream(osh_ctx* ctx, void** out);
+/* the context's lane counts (no counterpart in the reference): "coarsen_lanes",
+ * "side_lanes", "choose_lanes" and "first_lanes", each a power of two from 1
+ * to 64, the threads that share one collapse side, one swap product side,
+ * one swap candidate and one edge of the first independent-set round.  They
+ * change no result, only speed; omega_h_amd/autotune.py picks them for a
+ * workload. */
+int osh_ctx_set_tuning(osh_ctx* ctx, const char* name, int value);
+int osh_ctx_get_tuning(osh_ctx* ctx, const char* name, int* out);
 
 /* AdaptOpts::AdaptOpts(Int dim), src/Omega_h_adapt.cpp:52-85 */
 int osh_default_opts(int dim, osh_adapt_opts* out);

@@ -97,6 +97,8 @@ def load(path=None):
         "osh_ctx_destroy": [P],
         "osh_ctx_synchronize": [P],
         "osh_ctx_stream": [P, C.POINTER(P)],
+        "osh_ctx_set_tuning": [P, C.c_char_p, I],
+        "osh_ctx_get_tuning": [P, C.c_char_p, IP],
         "osh_default_opts": [I, C.POINTER(AdaptOpts)],
         "osh_build_box": [P, I, C.c_double, C.c_double, C.c_double, I, I, I, C.POINTER(P)],
         "osh_build_box_internal": [P, I, C.c_double, C.c_double, C.c_double, I, I, I,
@@ -232,6 +234,25 @@ class Context:
         s = C.c_void_p()
         check(self.lib.osh_ctx_stream(self.h, C.byref(s)), self.lib)
         return s.value or 0
+
+    TUNING = ("coarsen_lanes", "side_lanes", "choose_lanes", "first_lanes")
+
+    def set_tuning(self, name, value):
+        """One of the context's lane counts (TUNING; a power of two, 1 to 64)."""
+        check(self.lib.osh_ctx_set_tuning(self.h, name.encode(), int(value)), self.lib)
+
+    def tuning(self):
+        """The context's lane counts, by name."""
+        out = {}
+        for name in self.TUNING:
+            v = C.c_int()
+            check(self.lib.osh_ctx_get_tuning(self.h, name.encode(), C.byref(v)), self.lib)
+            out[name] = v.value
+        return out
+
+    def apply_tuning(self, settings):
+        for name, value in settings.items():
+            self.set_tuning(name, value)
 
     def measure_qualities(self, dim, metric_ncomps, n, ev2v_ptr, coords_ptr, metric_ptr,
                           out_ptr, elems_ptr=0):

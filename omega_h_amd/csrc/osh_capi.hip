@@ -167,6 +167,15 @@ __global__ void eval_math_kernel(osh::LO n, int which, double const* in, double*
   out[i] = r;
 }
 
+int* tuning_field(osh::Tuning& t, const char* name) {
+  std::string n(name);
+  if (n == "coarsen_lanes") return &t.coarsen_lanes;
+  if (n == "side_lanes") return &t.side_lanes;
+  if (n == "choose_lanes") return &t.choose_lanes;
+  if (n == "first_lanes") return &t.first_lanes;
+  throw osh::Error("unknown tuning setting " + n);
+}
+
 osh::Diffusion diffusion_of(int type, double tol, double floor) {
   OSH_CHECK(type >= 0 && type <= 2, "diffuse_type must be 0 (none), 1 or 2");
   OSH_CHECK(tol >= 0.0 && floor >= 0.0, "negative diffusion tolerance or floor");
@@ -691,6 +700,22 @@ int osh_offset_scans(osh_ctx* ctx, int njobs, const int* const* in, const int* n
       jobs[j] = {in[j], osh::kScanLO, n[j], out[j], nullptr};
     }
     osh::offset_scans(ctx->ctx, jobs, njobs);
+  });
+}
+
+int osh_ctx_set_tuning(osh_ctx* ctx, const char* name, int value) {
+  return guarded([&] {
+    OSH_CHECK(ctx && name, "null argument");
+    OSH_CHECK(value >= 1 && value <= 64 && (value & (value - 1)) == 0,
+        "a lane count is a power of two from 1 to 64");
+    *tuning_field(ctx->ctx.tuning, name) = value;
+  });
+}
+
+int osh_ctx_get_tuning(osh_ctx* ctx, const char* name, int* out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && name && out, "null argument");
+    *out = *tuning_field(ctx->ctx.tuning, name);
   });
 }
 

@@ -104,9 +104,11 @@ __global__ void apply_edge_vetoes(LO ncands, LO const* cands, I8 const* veto, I8
   codes[e] = code;
 }
 
-// Each collapse side (candidate edge, direction) gets kLanes threads, which
-// split the collapsing vertex's row between them, so that passes with few
-// candidates still spread their work over the device.
+// Each collapse side (candidate edge, direction) gets 2^lshift threads
+// (Tuning::coarsen_lanes), which split the collapsing vertex's row between
+// them, so that passes with few candidates still spread their work over the
+// device.  Minima do not depend on how a row is split, so the lane count
+// changes no result.  The product builders below use kLanes threads per key.
 constexpr int kLanes = 8;
 
 // prevent_coarsen_overshoot: a side is vetoed when any new edge would reach
@@ -116,13 +118,13 @@ constexpr int kLanes = 8;
 // inside the band are marked in `need` for the FP64 sweep.  With `need`
 // given, the FP64 sweep looks only at marked sides not already vetoed.
 template <int D, int M, bool F32>
-__global__ void overshoot_lanes(LO nt, LO const* cands, LO const* ev2v, LO const* v2e_a2ab,
-    LO const* v2e_ab2b, I8 const* v2e_codes, Real const* coords, Real const* metric,
-    Real max_length, I8 const* codes, I8* veto, I8* need) {
+__global__ void overshoot_lanes(LO nt, int lshift, LO const* cands, LO const* ev2v,
+    LO const* v2e_a2ab, LO const* v2e_ab2b, I8 const* v2e_codes, Real const* coords,
+    Real const* metric, Real max_length, I8 const* codes, I8* veto, I8* need) {
   LO t = thread_index();
   if (t >= nt) return;
-  LO side = t / kLanes;
-  int lane = t % kLanes;
+  LO side = t >> lshift;
+  int lane = t & ((1 << lshift) - 1);
   LO e = cands[side / 2];
   int col = side % 2;
   if (!collapses(codes[e], col)) return;
@@ -130,7 +132,7 @@ __global__ void overshoot_lanes(LO nt, LO const* cands, LO const* ev2v, LO const
   constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
   LO v_col = ev2v[e * 2 + col];
   LO v_onto = ev2v[e * 2 + (1 - col)];
-  for (LO ve = v2e_a2ab[v_col] + lane; ve < v2e_a2ab[v_col + 1]; ve += kLanes) {
+  for (LO ve = v2e_a2ab[v_col] + lane; ve < v2e_a2ab[v_col + 1]; ve += (1 << lshift)) {
     LO e2 = v2e_ab2b[ve];
     if (e2 == e) continue;
     int eev_in = code_which_down(v2e_codes[ve]);
@@ -181,14 +183,14 @@ __global__ void apply_vetoes(LO ncands, LO const* cands, I8 const* veto, I8* cod
 // max-determinant metric is not clear marks its side in `unsure`.  With
 // `need` given, the FP64 sweep evaluates only the marked sides.
 template <int D, int M, bool F32>
-__global__ void __launch_bounds__(kBlock, 5) coarsen_quality_lanes(LO nt, LO const* cands,
-    LO const* ev2v, LO const* cv2v, LO const* v2c_a2ab, LO const* v2c_ab2b,
+__global__ void __launch_bounds__(kBlock, 5) coarsen_quality_lanes(LO nt, int lshift,
+    LO const* cands, LO const* ev2v, LO const* cv2v, LO const* v2c_a2ab, LO const* v2c_ab2b,
     I8 const* v2c_codes, Real const* coords, Real const* metric, I8 const* codes,
     I8 const* need, I8* unsure, Real* partial) {
   LO t = thread_index();
   if (t >= nt) return;
-  LO side = t / kLanes;
-  int lane = t % kLanes;
+  LO side = t >> lshift;
+  int lane = t & ((1 << lshift) - 1);
   LO e = cands[side / 2];
   int col = side % 2;
   double minqual = 1.0;
@@ -198,7 +200,7 @@ __global__ void __launch_bounds__(kBlock, 5) coarsen_quality_lanes(LO nt, LO con
     constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
     LO v_col = ev2v[e * 2 + col];
     LO v_onto = ev2v[e * 2 + (1 - col)];
-    for (LO vc = v2c_a2ab[v_col] + lane; vc < v2c_a2ab[v_col + 1]; vc += kLanes) {
+    for (LO vc = v2c_a2ab[v_col] + lane; vc < v2c_a2ab[v_col + 1]; vc += (1 << lshift)) {
       LO c = v2c_ab2b[vc];
       int ccv_col = code_which_down(v2c_codes[vc]);
       LO cv[D + 1];
@@ -240,7 +242,7 @@ __global__ void __launch_bounds__(kBlock, 5) coarsen_quality_lanes(LO nt, LO con
 // nothing downstream: the filters, rails and independent set compare it
 // only).  After a mixed-precision screen (`need` given) a side the screen
 // rejected gets -1, below every quality threshold.
-__global__ void combine_quality_lanes(LO ncands, LO const* cands, I8 const* codes,
+__global__ void combine_quality_lanes(LO ncands, int lanes, LO const* cands, I8 const* codes,
     I8 const* need, Real const* partial, Real* quals) {
   LO i = thread_index();
   if (i >= ncands) return;
@@ -250,7 +252,7 @@ __global__ void combine_quality_lanes(LO ncands, LO const* cands, I8 const* code
     double q = -1.0;
     if (collapses(code, col) && (!need || need[i * 2 + col])) {
       q = 1.0;
-      for (int l = 0; l < kLanes; ++l) q = math::min2(q, partial[(i * 2 + col) * kLanes + l]);
+      for (int l = 0; l < lanes; ++l) q = math::min2(q, partial[(i * 2 + col) * lanes + l]);
     }
     quals[e * 2 + col] = q;
   }
@@ -259,7 +261,7 @@ __global__ void combine_quality_lanes(LO ncands, LO const* cands, I8 const* code
 // the screen's verdict per side: FP64 is needed unless the FP32 minimum is
 // below qmin by more than the band and every element's metric was clear
 // (a NaN estimate compares false and is evaluated again)
-__global__ void screen_quality_sides(LO nsides, LO const* cands, I8 const* codes,
+__global__ void screen_quality_sides(LO nsides, int lanes, LO const* cands, I8 const* codes,
     I8 const* unsure, Real const* partial, Real qmin, I8* need) {
   LO side = thread_index();
   if (side >= nsides) return;
@@ -270,15 +272,15 @@ __global__ void screen_quality_sides(LO nsides, LO const* cands, I8 const* codes
     return;
   }
   double q = 1.0;
-  for (int l = 0; l < kLanes; ++l) q = math::min2(q, partial[side * kLanes + l]);
+  for (int l = 0; l < lanes; ++l) q = math::min2(q, partial[side * lanes + l]);
   bool reject = !unsure[side] && q < qmin * (1.0 - double(f32::kScreenBand));
   need[side] = reject ? 0 : 1;
 }
 
 // combine_quality_lanes with one output row per candidate (the layout of
 // the reference's coarsen_qualities, two values per candidate)
-__global__ void combine_to_cands(LO ncands, LO const* cands, I8 const* codes, I8 const* need,
-    Real const* partial, Real* out) {
+__global__ void combine_to_cands(LO ncands, int lanes, LO const* cands, I8 const* codes,
+    I8 const* need, Real const* partial, Real* out) {
   LO i = thread_index();
   if (i >= ncands) return;
   I8 code = codes[cands[i]];
@@ -286,7 +288,7 @@ __global__ void combine_to_cands(LO ncands, LO const* cands, I8 const* codes, I8
     double q = -1.0;
     if (collapses(code, col) && (!need || need[i * 2 + col])) {
       q = 1.0;
-      for (int l = 0; l < kLanes; ++l) q = math::min2(q, partial[(i * 2 + col) * kLanes + l]);
+      for (int l = 0; l < lanes; ++l) q = math::min2(q, partial[(i * 2 + col) * lanes + l]);
     }
     out[i * 2 + col] = q;
   }
@@ -744,7 +746,9 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   OSH_LAUNCH(apply_edge_vetoes, ncands, c, cands.data(), eveto.data(), codes.data());
   Real max_length = overshoot_allowed ? o.max_length_allowed : o.max_length_desired;
   Adj v2e = marked_up(c, m, 0, 1, vmarks);
-  LO nlanes = ncands * 2 * kLanes;
+  int lanes = fit_lanes(ncands * 2, c.tuning.coarsen_lanes);
+  int lshift = lanes_shift(lanes);
+  LO nlanes = ncands * 2 * lanes;
   Bytes veto = fill_i8(c, ncands * 2, 0);
   // mixed precision: an FP32 screen first, then FP64 only for the sides it
   // left open
@@ -752,9 +756,9 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   Bytes need;
   if (mixed) need = fill_i8(c, ncands * 2, 0);
 #define OSH_OV(D, M, F32)                                                                \
-  OSH_LAUNCH((overshoot_lanes<D, M, F32>), nlanes, c, cands.data(), ev2v, v2e.a2ab.data(),  \
-      v2e.ab2b.data(), v2e.codes.data(), m.coords.data(), m.metric.data(), max_length,      \
-      codes.data(), veto.data(), mixed ? need.data() : (I8*)nullptr)
+  OSH_LAUNCH((overshoot_lanes<D, M, F32>), nlanes, c, lshift, cands.data(), ev2v,          \
+      v2e.a2ab.data(), v2e.ab2b.data(), v2e.codes.data(), m.coords.data(), m.metric.data(), \
+      max_length, codes.data(), veto.data(), mixed ? need.data() : (I8*)nullptr)
 #define OSH_OV_ALL(F32)      \
   if (dim == 3 && md == 3)   \
     OSH_OV(3, 3, F32);       \
@@ -781,7 +785,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   Bytes unsure;
   if (mixed) unsure = fill_i8(c, ncands * 2, 0);
 #define OSH_CQ(D, M, F32)                                                                \
-  OSH_LAUNCH((coarsen_quality_lanes<D, M, F32>), nlanes, c, cands.data(), ev2v,             \
+  OSH_LAUNCH((coarsen_quality_lanes<D, M, F32>), nlanes, c, lshift, cands.data(), ev2v,     \
       cv2v.data(), v2c.a2ab.data(), v2c.ab2b.data(), v2c.codes.data(), m.coords.data(),     \
       m.metric.data(), codes.data(), mixed ? need.data() : (I8 const*)nullptr,              \
       mixed ? unsure.data() : (I8*)nullptr, partial.data())
@@ -796,13 +800,13 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
     OSH_CQ(2, 1, F32)
   if (mixed) {
     OSH_CQ_ALL(true);
-    OSH_LAUNCH(screen_quality_sides, ncands * 2, c, cands.data(), codes.data(), unsure.data(),
-        partial.data(), o.min_quality_allowed, need.data());
+    OSH_LAUNCH(screen_quality_sides, ncands * 2, c, lanes, cands.data(), codes.data(),
+        unsure.data(), partial.data(), o.min_quality_allowed, need.data());
   }
   OSH_CQ_ALL(false);
 #undef OSH_CQ_ALL
 #undef OSH_CQ
-  OSH_LAUNCH(combine_quality_lanes, ncands, c, cands.data(), codes.data(),
+  OSH_LAUNCH(combine_quality_lanes, ncands, c, lanes, cands.data(), codes.data(),
       mixed ? need.data() : (I8 const*)nullptr, partial.data(), quals.data());
   Reals vold;
   if (improve) {
@@ -959,7 +963,9 @@ void coarsen_qualities(Ctx& c, DMesh& m, LO ncands, LO const* cands, I8 const* c
   Adj const& v2c = ask_up(c, m, 0, dim);
   Bytes codes = fill_i8(c, m.nents[1], kDont);
   OSH_LAUNCH(scatter_codes, ncands, c, cands, cand_codes, codes.data());
-  LO nlanes = ncands * 2 * kLanes;
+  int lanes = fit_lanes(ncands * 2, c.tuning.coarsen_lanes);
+  int lshift = lanes_shift(lanes);
+  LO nlanes = ncands * 2 * lanes;
   Reals partial(c, nlanes);
   bool mixed = precision == 1;
   Bytes need, unsure;
@@ -968,9 +974,9 @@ void coarsen_qualities(Ctx& c, DMesh& m, LO ncands, LO const* cands, I8 const* c
     unsure = fill_i8(c, ncands * 2, 0);
   }
 #define OSH_CQ(D, M, F32)                                                                \
-  OSH_LAUNCH((coarsen_quality_lanes<D, M, F32>), nlanes, c, cands, ev2v, cv2v.data(),       \
-      v2c.a2ab.data(), v2c.ab2b.data(), v2c.codes.data(), m.coords.data(), m.metric.data(), \
-      codes.data(), mixed ? need.data() : (I8 const*)nullptr,                               \
+  OSH_LAUNCH((coarsen_quality_lanes<D, M, F32>), nlanes, c, lshift, cands, ev2v,            \
+      cv2v.data(), v2c.a2ab.data(), v2c.ab2b.data(), v2c.codes.data(), m.coords.data(),     \
+      m.metric.data(), codes.data(), mixed ? need.data() : (I8 const*)nullptr,              \
       mixed ? unsure.data() : (I8*)nullptr, partial.data())
 #define OSH_CQ_ALL(F32)      \
   if (dim == 3 && md == 3)   \
@@ -983,13 +989,13 @@ void coarsen_qualities(Ctx& c, DMesh& m, LO ncands, LO const* cands, I8 const* c
     OSH_CQ(2, 1, F32)
   if (mixed) {
     OSH_CQ_ALL(true);
-    OSH_LAUNCH(screen_quality_sides, ncands * 2, c, cands, codes.data(), unsure.data(),
+    OSH_LAUNCH(screen_quality_sides, ncands * 2, c, lanes, cands, codes.data(), unsure.data(),
         partial.data(), qmin, need.data());
   }
   OSH_CQ_ALL(false);
 #undef OSH_CQ_ALL
 #undef OSH_CQ
-  OSH_LAUNCH(combine_to_cands, ncands, c, cands, codes.data(),
+  OSH_LAUNCH(combine_to_cands, ncands, c, lanes, cands, codes.data(),
       mixed ? need.data() : (I8 const*)nullptr, partial.data(), out);
 }
 

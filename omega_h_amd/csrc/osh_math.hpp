@@ -852,9 +852,31 @@ __device__ __forceinline__ float metric_product(float const* m, float const* v) 
          2.0f * (m[3] * v[0] * v[1] + m[4] * v[1] * v[2] + m[5] * v[0] * v[2]);
 }
 
+// Metrics whose components differ by at most kSameMetric of the largest
+// component give qualities within a few kSameMetric of each other (the
+// mean squared metric length is linear in the metric and the determinant
+// enters through a root), so which of them is the max-determinant metric
+// does not move the estimate beyond its error budget.
+constexpr float kSameMetric = 1e-5f;
+
+template <int M>
+__device__ __forceinline__ bool near_same_metric(float const* a, float const* b) {
+  constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
+  float scale = 0.0f, diff = 0.0f;
+#pragma unroll
+  for (int k = 0; k < nc; ++k) {
+    float x = a[k] < 0.0f ? -a[k] : a[k];
+    float y = a[k] - b[k];
+    y = y < 0.0f ? -y : y;
+    scale = x > scale ? x : scale;
+    diff = y > diff ? y : diff;
+  }
+  return diff <= kSameMetric * scale;
+}
+
 // element_quality in FP32; *unsure is set when the max-determinant vertex
-// metric is not clear (another vertex's different metric within the band),
-// since the FP64 choice could then differ
+// metric is not clear (another vertex's metric, not nearly the same, with a
+// determinant within the band), since the FP64 choice could then differ
 template <int D, int M>
 __device__ __forceinline__ float element_quality(double const* const* p, double const* const* ms,
     bool* unsure) {
@@ -866,9 +888,7 @@ __device__ __forceinline__ float element_quality(double const* const* p, double 
   for (int i = 1; i < D + 1; ++i) {
     load_symm<M>(ms[i], mi);
     float d = symm_det<M>(mi);
-    bool same = true;
-    for (int k = 0; k < nc; ++k) same = same && (mi[k] == m[k]);
-    if (same) continue;
+    if (near_same_metric<M>(m, mi)) continue;
     if (d > maxdet) {
       second = maxdet;
       for (int k = 0; k < nc; ++k) m[k] = mi[k];

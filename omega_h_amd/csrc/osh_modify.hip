@@ -405,18 +405,17 @@ __global__ void indset_iteration(LO n, LO const* a2ab, LO const* ab2b, Real cons
   if (!undecided) new_state[v] = 1;
 }
 
-// the first round for edges with kFirstLanes threads per edge: each takes
-// every kFirstLanes-th row position, and the group ORs its findings, so the
-// row's gathers run side by side instead of one after another
-constexpr int kFirstLanes = 8;
-
+// the first round for edges with `lanes` threads per edge (a power of two,
+// Tuning::first_lanes): each takes every lanes-th row position, and the
+// group ORs its findings, so the row's gathers run side by side instead of
+// one after another
 template <int NEPC>
-__global__ void indset_first_edges(LO nt, LO const* e2c_a2ab, LO const* e2c_ab2b,
+__global__ void indset_first_edges(LO nt, int lanes, LO const* e2c_a2ab, LO const* e2c_ab2b,
     LO const* c2e, Real const* q, I8 const* cands, I8* new_state, unsigned long long* dom) {
   LO t = thread_index();
   if (t >= nt) return;
-  LO v = t / kFirstLanes;
-  int lane = t % kFirstLanes;
+  LO v = t / lanes;
+  int lane = t % lanes;
   I8 s = cands[v] ? I8(2) : I8(0);
   unsigned long long mask = 0;
   int undecided = 0;
@@ -424,7 +423,7 @@ __global__ void indset_first_edges(LO nt, LO const* e2c_a2ab, LO const* e2c_ab2b
     LO b = e2c_a2ab[v], e = e2c_a2ab[v + 1];
     Real qv = q[v];
     LO np = (e - b) * NEPC;
-    for (LO p = lane; p < np; p += kFirstLanes) {
+    for (LO p = lane; p < np; p += lanes) {
       LO u = c2e[e2c_ab2b[b + p / NEPC] * NEPC + p % NEPC];
       if (u == v || !cands[u]) continue;
       Real qu = q[u];
@@ -434,8 +433,7 @@ __global__ void indset_first_edges(LO nt, LO const* e2c_a2ab, LO const* e2c_ab2b
       if (p < kDomBits) mask |= 1ull << p;
     }
   }
-#pragma unroll
-  for (int d = kFirstLanes / 2; d > 0; d >>= 1) {
+  for (int d = lanes / 2; d > 0; d >>= 1) {
     mask |= __shfl_xor(mask, d, 64);
     undecided |= __shfl_xor(undecided, d, 64);
   }
@@ -908,6 +906,7 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
   // first check a state of 2 means undecided nodes remain, and a state of 1
   // or 2 that there was a candidate (the best candidate joins the set in the
   // first round)
+  int first_lanes = fit_lanes(n, c.tuning.first_lanes);
   for (LO round = 0;;) {
     OSH_CHECK(round <= n + 1, "independent set does not converge");
     int batch = round < 8 ? 2 : (round < 16 ? 4 : (round < 32 ? 8 : 16));
@@ -927,10 +926,10 @@ Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& ca
           c2e, quality.data(), in, b.data(), dom.data());                                 \
   } while (0)
       if (first && !g && m.dim == 3)
-        OSH_LAUNCH((indset_first_edges<6>), n * kFirstLanes, c, e2c->a2ab.data(),
+        OSH_LAUNCH((indset_first_edges<6>), n * first_lanes, c, first_lanes, e2c->a2ab.data(),
             e2c->ab2b.data(), c2e, quality.data(), cands.data(), b.data(), dom.data());
       else if (first && !g)
-        OSH_LAUNCH((indset_first_edges<3>), n * kFirstLanes, c, e2c->a2ab.data(),
+        OSH_LAUNCH((indset_first_edges<3>), n * first_lanes, c, first_lanes, e2c->a2ab.data(),
             e2c->ab2b.data(), c2e, quality.data(), cands.data(), b.data(), dom.data());
       else if (first)
         OSH_IS(true);

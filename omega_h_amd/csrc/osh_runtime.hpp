@@ -76,6 +76,30 @@ class Pool {
   bool closed_ = false;
 };
 
+// Lane counts of the kernels that give one row of work several threads
+// (powers of two, 1 to 64).  No setting changes a result, only how the work
+// is spread over the device; omega_h_amd/autotune.py picks them for a
+// workload by timing adapt() and checking that the mesh stays the same.
+struct Tuning {
+  int coarsen_lanes = 8;  // threads per collapse side (overshoot and quality sweeps)
+  int side_lanes = 16;    // threads per swap product side searched for its match
+  int choose_lanes = 8;   // threads per swap candidate choosing its triangulation
+  int first_lanes = 8;    // threads per edge in the first independent-set round
+};
+
+// the lane count for `rows` rows: `lanes`, halved while rows x lanes would
+// not fit a launch's 32-bit thread count (results do not depend on it)
+inline int fit_lanes(LO rows, int lanes) {
+  while (lanes > 1 && std::int64_t(rows) * lanes > std::int64_t(0x7fffffff)) lanes /= 2;
+  return lanes;
+}
+
+inline int lanes_shift(int lanes) {
+  int s = 0;
+  while ((1 << s) < lanes) ++s;
+  return s;
+}
+
 struct Ctx {
   hipStream_t stream = nullptr;
   int device = 0;
@@ -96,6 +120,7 @@ struct Ctx {
   static constexpr unsigned kMaxScanEpoch = (1u << 24) - 1;
   unsigned* tile_counter = nullptr;
   unsigned tile_tickets = 0;
+  Tuning tuning;
   // device copy of the edge-swap tables, uploaded on first use
   std::shared_ptr<void> swap_tables;
   std::shared_ptr<Pool> pool;

@@ -275,7 +275,6 @@ __global__ void swap3d_edge_lens(LO nt, LO const* nedges, LO const* lsize, LO co
 // the largest of these values, when that is positive; a minimum that went
 // negative (where choose() stops early) is never kept either way.
 constexpr int kMaxMeshes = 42;
-constexpr int kChooseLanes = 8;
 
 __global__ void swap3d_mesh_quals(LO nt, LO const* lsize, LO const* ntris, Real const* tq,
     Real const* el, Real max_length, Tables const* tb, I8 const* need, Real* mq) {
@@ -299,20 +298,21 @@ __global__ void swap3d_mesh_quals(LO nt, LO const* lsize, LO const* ntris, Real 
   mq[t] = q;
 }
 
-// the mixed-precision screen of one candidate (kChooseLanes threads): the
+// the mixed-precision screen of one candidate (Tuning::choose_lanes threads): the
 // best FP32 triangulation value, where an interior edge counts as too long
 // only when its FP32 length exceeds max_length by more than the band, is an
 // upper estimate of the FP64 choice's quality; the candidate needs FP64
 // unless that estimate is below the edge's current minimum quality by more
 // than the band (with every element's metric clear and that minimum
 // positive)
-__global__ void swap3d_screen(LO nt, LO const* cands, LO const* lsize, LO const* ntris,
-    Real const* tq, Real const* el, Real max_length, Tables const* tb, LO const* e2t_a2ab,
-    LO const* e2t_ab2b, Real const* elem_quals, I8 const* unsure, I8* need) {
+__global__ void swap3d_screen(LO nt, int lanes, LO const* cands, LO const* lsize,
+    LO const* ntris, Real const* tq, Real const* el, Real max_length, Tables const* tb,
+    LO const* e2t_a2ab, LO const* e2t_ab2b, Real const* elem_quals, I8 const* unsure,
+    I8* need) {
   LO t = thread_index();
   if (t >= nt) return;
-  LO cand = t / kChooseLanes;
-  int lane = t % kChooseLanes;
+  LO cand = t / lanes;
+  int lane = t % lanes;
   LO edge = cands[cand];
   int n = lsize[cand];
   bool open = ntris[cand] != 0 && n <= swap::kMaxLoop;  // not carried over, a loop
@@ -323,7 +323,7 @@ __global__ void swap3d_screen(LO nt, LO const* cands, LO const* lsize, LO const*
     Real const* ctq = tq + LO(cand) * swap::kMaxUniqueTris;
     Real const* cel = el + LO(cand) * swap::kMaxUniqueEdges;
     double lmax = max_length * (1.0 + double(f32::kScreenBand));
-    for (int mesh = lane; mesh < tb->mesh_counts[n]; mesh += kChooseLanes) {
+    for (int mesh = lane; mesh < tb->mesh_counts[n]; mesh += lanes) {
       double q = 1.0;
       for (int mt = 0; mt < nmesh_tris; ++mt)
         q = math::min2(q, ctq[tb->meshes[n][mesh * nmesh_tris + mt]]);
@@ -333,10 +333,9 @@ __global__ void swap3d_screen(LO nt, LO const* cands, LO const* lsize, LO const*
     }
   }
   double old = 1.79769313486231570815e+308;
-  for (LO j = e2t_a2ab[edge] + lane; j < e2t_a2ab[edge + 1]; j += kChooseLanes)
+  for (LO j = e2t_a2ab[edge] + lane; j < e2t_a2ab[edge + 1]; j += lanes)
     old = math::min2(old, elem_quals[e2t_ab2b[j]]);
-#pragma unroll
-  for (int d = kChooseLanes / 2; d > 0; d >>= 1) {
+  for (int d = lanes / 2; d > 0; d >>= 1) {
     best = math::max2(best, __shfl_xor(best, d, 64));
     old = math::min2(old, __shfl_xor(old, d, 64));
   }
@@ -346,15 +345,16 @@ __global__ void swap3d_screen(LO nt, LO const* cands, LO const* lsize, LO const*
   need[cand] = reject ? 0 : 1;
 }
 
-__global__ void swap3d_choose(LO nt, LO const* cands, LO const* lsize, Real const* mq,
-    LO const* e2t_a2ab, LO const* e2t_ab2b, Real const* elem_quals, Tables const* tb,
-    I8 const* need, I8* configs, Real* quals, I8* keep, I8* c_valid, I8* c_cfg, I8* c_keep,
-    Real* c_q) {
-  // kChooseLanes threads per candidate share its triangulations and tets
+__global__ void swap3d_choose(LO nt, int lanes, LO const* cands, LO const* lsize,
+    Real const* mq, LO const* e2t_a2ab, LO const* e2t_ab2b, Real const* elem_quals,
+    Tables const* tb, I8 const* need, I8* configs, Real* quals, I8* keep, I8* c_valid,
+    I8* c_cfg, I8* c_keep, Real* c_q) {
+  // `lanes` threads (a power of two, Tuning::choose_lanes) per candidate
+  // share its triangulations and tets
   LO t = thread_index();
   if (t >= nt) return;
-  LO cand = t / kChooseLanes;
-  int lane = t % kChooseLanes;
+  LO cand = t / lanes;
+  int lane = t % lanes;
   LO edge = cands[cand];
   if (c_valid && c_valid[edge]) {
     if (lane == 0) {
@@ -391,7 +391,7 @@ __global__ void swap3d_choose(LO nt, LO const* cands, LO const* lsize, Real cons
     choice_quality = 0.0;
     int nmeshes = tb->mesh_counts[n];
     Real const* cmq = mq + LO(cand) * kMaxMeshes;
-    for (int mesh = lane; mesh < nmeshes; mesh += kChooseLanes) {
+    for (int mesh = lane; mesh < nmeshes; mesh += lanes) {
       double q = cmq[mesh];
       if (q > choice_quality) {
         choice_mesh = mesh;
@@ -400,10 +400,9 @@ __global__ void swap3d_choose(LO nt, LO const* cands, LO const* lsize, Real cons
     }
   }
   double old = 1.79769313486231570815e+308;
-  for (LO j = e2t_a2ab[edge] + lane; j < e2t_a2ab[edge + 1]; j += kChooseLanes)
+  for (LO j = e2t_a2ab[edge] + lane; j < e2t_a2ab[edge + 1]; j += lanes)
     old = math::min2(old, elem_quals[e2t_ab2b[j]]);
-#pragma unroll
-  for (int d = kChooseLanes / 2; d > 0; d >>= 1) {
+  for (int d = lanes / 2; d > 0; d >>= 1) {
     double oq = __shfl_xor(choice_quality, d, 64);
     int om = __shfl_xor(choice_mesh, d, 64);
     if (oq > choice_quality || (oq == choice_quality && om >= 0 &&
@@ -434,19 +433,18 @@ __global__ void swap3d_choose(LO nt, LO const* cands, LO const* lsize, Real cons
 // of a product (a face of a new tet, an edge of a new face) is a product of
 // the same key one dimension down or a surviving side of one of the old tets
 // around the key edge, so those are the only candidates (vertices keep their
-// numbers in a swap).  kSideLanes threads share one side's candidates; the
-// one that matches writes the side and the code match_uses would compute.
-constexpr int kSideLanes = 16;
-
+// numbers in a swap).  `lanes` threads (a power of two up to 64,
+// Tuning::side_lanes) share one side's candidates; the one that matches
+// writes the side and the code match_uses would compute.
 template <int HIGH>
-__global__ void swap3d_local_down(LO nt, LO const* prod_key, LO const* keys2edges,
+__global__ void swap3d_local_down(LO nt, int lanes, LO const* prod_key, LO const* keys2edges,
     LO const* k2p_low, LO const* pv_high, LO const* pv_low, LO const* low_prods2new,
     LO const* e2t_a2ab, LO const* e2t_ab2b, LO const* t2low, LO const* old_low_verts,
     LO const* low_old2new, LO* out, I8* out_codes, int* error) {
   LO t = thread_index();
   if (t >= nt) return;
-  LO u = t / kSideLanes;
-  int lane = t % kSideLanes;
+  LO u = t / lanes;
+  int lane = t % lanes;
   constexpr int LOW = HIGH - 1;
   constexpr int nvh = HIGH + 1;
   constexpr int nvl = LOW + 1;
@@ -463,7 +461,7 @@ __global__ void swap3d_local_down(LO nt, LO const* prod_key, LO const* keys2edge
   LO r0 = e2t_a2ab[e];
   LO ncand = nq + (e2t_a2ab[e + 1] - r0) * nlpt;
   bool found = false;
-  for (LO cnd = lane; cnd < ncand; cnd += kSideLanes) {
+  for (LO cnd = lane; cnd < ncand; cnd += lanes) {
     LO id;
     LO bv[3];
     if (cnd < nq) {
@@ -495,10 +493,11 @@ __global__ void swap3d_local_down(LO nt, LO const* prod_key, LO const* keys2edge
     out_codes[u] = code;
     found = true;
   }
-  // the side's lanes are kSideLanes consecutive lanes of one wave
+  // the side's lanes are `lanes` consecutive lanes of one wave
   unsigned long long any = __ballot(found);
-  int base = int(threadIdx.x % 64) & ~(kSideLanes - 1);
-  if (lane == 0 && ((any >> base) & ((1ull << kSideLanes) - 1)) == 0ull) {
+  int base = int(threadIdx.x % 64) & ~(lanes - 1);
+  unsigned long long group = (lanes >= 64) ? ~0ull : ((1ull << lanes) - 1);
+  if (lane == 0 && ((any >> base) & group) == 0ull) {
     *error = kReflectDownNoMatch;
     out[u] = 0;
     out_codes[u] = 0;
@@ -770,17 +769,18 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
     Bytes gcodes;
     if (key_tets && prod_keys && d >= 2) {
       LO nuses = nprods[d] * (d + 1);
+      int side_lanes = fit_lanes(nuses, c.tuning.side_lanes);
       gdown = LOs(c, nuses);
       gcodes = Bytes(c, nuses);
       LO const* t2low = (d == 3) ? m.down[3].data() : ask_down(c, m, 3, 1, nullptr).data();
       if (d == 3)
-        OSH_LAUNCH(swap3d_local_down<3>, nuses * kSideLanes, c, prod_keys[3].data(),
+        OSH_LAUNCH(swap3d_local_down<3>, nuses * side_lanes, c, side_lanes, prod_keys[3].data(),
             keys2edges.data(), k2p[2].data(), pv[3].data(), pv[2].data(),
             low_mo.prods2new.data(), key_tets->a2ab.data(), key_tets->ab2b.data(), t2low,
             verts_of(c, m, 2).data(), low_mo.old2new.data(), gdown.data(), gcodes.data(),
             c.error_word());
       else
-        OSH_LAUNCH(swap3d_local_down<2>, nuses * kSideLanes, c, prod_keys[2].data(),
+        OSH_LAUNCH(swap3d_local_down<2>, nuses * side_lanes, c, side_lanes, prod_keys[2].data(),
             keys2edges.data(), k2p[1].data(), pv[2].data(), pv[1].data(),
             low_mo.prods2new.data(), key_tets->a2ab.data(), key_tets->ab2b.data(), t2low,
             verts_of(c, m, 1).data(), low_mo.old2new.data(), gdown.data(), gcodes.data(),
@@ -844,6 +844,7 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
   LOs const& tv2v = verts_of(c, m, 3);
   Reals const& eq = ask_qualities(c, m);
   LO ncands = cands.size();
+  int choose_lanes = fit_lanes(ncands, c.tuning.choose_lanes);
   if (cache && !cache->valid.exists()) {
     cache->valid = fill_i8(c, ne, 0);
     cache->cfg = Bytes(c, ne);
@@ -881,7 +882,8 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
     } else {
       OSH_SWQ(1, true);
     }
-    OSH_LAUNCH(swap3d_screen, ncands * kChooseLanes, c, cands.data(), lsize.data(),
+    OSH_LAUNCH(swap3d_screen, ncands * choose_lanes, c, choose_lanes, cands.data(),
+        lsize.data(),
         ntris.data(), tq.data(), el.data(), o.max_length_allowed, tb, e2t.a2ab.data(),
         e2t.ab2b.data(), eq.data(), unsure.data(), need.data());
   }
@@ -894,7 +896,8 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
   Reals mq(c, ncands * kMaxMeshes);
   OSH_LAUNCH(swap3d_mesh_quals, ncands * kMaxMeshes, c, lsize.data(), ntris.data(),
       tq.data(), el.data(), o.max_length_allowed, tb, needp, mq.data());
-  OSH_LAUNCH(swap3d_choose, ncands * kChooseLanes, c, cands.data(), lsize.data(), mq.data(),
+  OSH_LAUNCH(swap3d_choose, ncands * choose_lanes, c, choose_lanes, cands.data(),
+      lsize.data(), mq.data(),
       e2t.a2ab.data(),
       e2t.ab2b.data(), eq.data(), tb, needp, configs.data(), quals.data(), keep.data(),
       cache ? cache->valid.data() : (I8*)nullptr, cache ? cache->cfg.data() : (I8*)nullptr,

@@ -151,3 +151,40 @@ def batch_boxes(nboxes, seed, n_range=(8, 48), h_range=(0.005, 0.1), max_nelems=
             vc, e, s = clamp_metric(v, dim, max_nelems)
             out.append((n, vc, e, s))
     return out
+
+
+def serve_request(rng, dim, elems, h_lo, h_hi, clamp=None):
+    """One request of the serving workload: a box of about `elems` elements
+    (2 n^2 triangles or 6 n^3 tets) under a constant anisotropic metric with
+    a uniformly random rotation and eigen-lengths log-uniform in [h_lo, h_hi],
+    scaled by the reference's element-count rule when it is expected to give
+    more than `clamp` elements.  Returns (dim, n, metric components)."""
+    n = max(1, round((elems / 2) ** 0.5)) if dim == 2 else max(1, round((elems / 6) ** (1 / 3)))
+    g = rng.normal(size=(dim, dim))
+    q, r = np.linalg.qr(g)
+    q = q * np.sign(np.diag(r))
+    h = np.exp(rng.uniform(np.log(h_lo), np.log(h_hi), dim))
+    m = q @ np.diag(1.0 / h ** 2) @ q.T
+    if dim == 3:
+        v = [m[0, 0], m[1, 1], m[2, 2], m[0, 1], m[1, 2], m[0, 2]]
+    else:
+        v = [m[0, 0], m[1, 1], m[0, 1]]
+    if clamp is not None:
+        v = list(clamp_metric(v, dim, clamp)[0])
+    return dim, n, v
+
+
+def serve_stream(rng, rate, duration, max_elems, h_lo, h_hi, clamp=None, min_elems=1e3,
+                 frac_2d=0.3):
+    """Poisson arrivals at `rate` per second for `duration` seconds: a list of
+    (arrival time, request) with 30 % 2D and 70 % 3D boxes and element
+    counts log-uniform in [min_elems, max_elems]."""
+    t = 0.0
+    out = []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            return out
+        dim = 2 if rng.random() < frac_2d else 3
+        elems = float(np.exp(rng.uniform(np.log(min_elems), np.log(max_elems))))
+        out.append((t, serve_request(rng, dim, elems, h_lo, h_hi, clamp)))

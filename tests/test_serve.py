@@ -99,3 +99,51 @@ def test_timed_out_request_fails_and_frees_its_slot():
     ok = [server.wait(server.submit(d, n, v, ctx.default_opts(d))) for d, n, v in reqs]
     server.close()
     assert all(r.status == 0 and r.nelems_out > 0 for r in ok)
+
+
+@pytest.mark.gpu
+def test_serving_stream_alone_equals_mix():
+    """A short Poisson stream of the serving workload (30 % 2D / 70 % 3D
+    boxes, element counts log-uniform in [1e3, 2e6], rotated metrics with
+    eigen-lengths log-uniform in [0.005, 0.1], each clamped to 2e4 expected
+    elements by the reference's element-count rule) submitted at its arrival
+    times to a 4-slot server: every result equals the same request adapted
+    alone, and the smallest ones equal the oracle's."""
+    import time
+
+    from omega_h_amd import workloads as W
+    ctx = A.Context(0)
+    rng = np.random.default_rng(2024)
+    stream = W.serve_stream(rng, 20.0, 0.6, 2e6, 0.005, 0.1, clamp=2e4)
+    assert len(stream) >= 6
+    opts = {d: ctx.default_opts(d) for d in (2, 3)}
+    for o in opts.values():
+        o.verbosity = 0
+    server = A.Server(nslots=4, hbm_budget_bytes=100e9)
+    print(f"serving a stream of {len(stream)} requests", flush=True)
+    t0 = time.perf_counter()
+    tickets = []
+    for at, (d, n, v) in stream:
+        delay = t0 + at - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(server.submit(d, n, v, opts[d], want_digest=True))
+    mixed = [server.wait(t) for t in tickets]
+    server.close()
+    sizes = []
+    for (_, (d, n, v)), r in zip(stream, mixed):
+        g = A.Mesh.box(ctx, d, n)
+        g.set_metric(np.tile(v, g.nents(0)), len(v))
+        g.adapt(opts[d])
+        assert r.digest == g.digest(), (d, n)
+        assert r.nelems_out == g.nents(d)
+        sizes.append(g.nents(d))
+    smallest = sorted(range(len(stream)), key=lambda i: stream[i][1][1])[:2]
+    for i in smallest:
+        d, n, v = stream[i][1]
+        o = O.Mesh.box(d, n)
+        o.set_metric(np.tile(v, o.nents(0)), len(v))
+        oo = O.default_opts(d)
+        oo.verbosity = 0
+        o.adapt(oo)
+        assert mixed[i].digest == fnv1a(o.verts(d), o.coords()), (d, n)
