@@ -112,6 +112,17 @@ int osh_mesh_copy(osh_mesh* mesh, osh_mesh** out);
 int osh_mesh_join(osh_ctx* ctx, int nparts, osh_mesh* const* parts, osh_mesh** out);
 int osh_mesh_split(osh_mesh* joined, int nparts, osh_mesh** out);
 int osh_mesh_part_counts(osh_mesh* joined, int nparts, int* counts);
+/* adapt() of nmeshes meshes of one context at once (the batched form, no
+ * counterpart in the reference): they are joined as above and adapted as
+ * one, and whenever a quarter of the parts can no longer change in the
+ * current phase (unchanged over a run of passes holding every enabled pass
+ * kind, or in the quality phase already at min_quality_desired) the joined
+ * mesh is split and the rest joined again.  Each mesh ends exactly where
+ * osh_adapt alone leaves it; did[i] is osh_adapt's result for mesh i, and
+ * stats lists the passes of the joined meshes.  Same requirements as
+ * osh_mesh_join; the options' verbosity is ignored (silent). */
+int osh_adapt_batch(int nmeshes, osh_mesh* const* meshes, const osh_adapt_opts* opts,
+    osh_adapt_stats* stats, int* did);
 
 /* Mesh::dim / Mesh::nents, src/Omega_h_mesh.hpp */
 int osh_mesh_dim(osh_mesh* mesh, int* out);

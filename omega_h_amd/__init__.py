@@ -147,6 +147,7 @@ def load(path=None):
         "osh_swap_edges": [P, C.POINTER(AdaptOpts), IP],
         "osh_coarsen_slivers": [P, C.POINTER(AdaptOpts), IP],
         "osh_adapt": [P, C.POINTER(AdaptOpts), C.POINTER(AdaptStats), IP],
+        "osh_adapt_batch": [I, C.POINTER(P), C.POINTER(AdaptOpts), C.POINTER(AdaptStats), IP],
         "osh_adapt_digests": [P, C.POINTER(AdaptOpts), C.POINTER(AdaptStats),
                               C.POINTER(C.c_ulonglong), IP],
         "osh_mesh_digests": [P, C.POINTER(C.c_ulonglong)],
@@ -329,6 +330,21 @@ class Context:
         check(self.lib.osh_eval_math(self.h, which, len(x), _p(x, C.c_double),
                                      _p(out, C.c_double)), self.lib)
         return out
+
+
+def adapt_batch(meshes, opts):
+    """adapt() of many meshes of one context at once (the batched form):
+    each mesh ends where its own adapt() would leave it.  Returns the
+    per-mesh results and the passes of the joined meshes as (kind, elements
+    in, elements out, seconds)."""
+    lib = meshes[0].lib
+    arr = (C.c_void_p * len(meshes))(*[m.h for m in meshes])
+    st = AdaptStats()
+    did = (C.c_int * len(meshes))()
+    check(lib.osh_adapt_batch(len(meshes), arr, C.byref(opts), C.byref(st), did), lib)
+    passes = [(st.kind[i], st.nelems_in[i], st.nelems_out[i], st.seconds[i])
+              for i in range(st.npasses)]
+    return [int(x) for x in did], passes
 
 
 class Mesh:

@@ -179,4 +179,187 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
   return true;
 }
 
+namespace {
+
+// The batched form runs each phase of adapt() on joined meshes.  Every pass
+// marks the parts it changes (Ctx::part_changed); a pass acts on a part as
+// a deterministic function of that part alone, so a pass kind that left a
+// part unchanged leaves it unchanged again until another kind changes it.
+// Parts are taken out of a joined mesh (split off, the rest joined again)
+// once a quarter of them no longer follow the joined mesh's pass sequence:
+//
+// * lengths (refine, then coarsen, until neither changes anything): a part
+//   that one round of both left unchanged is finished;
+// * quality (swap while swapping changes something, else coarsen slivers,
+//   while the minimum quality is below min_quality_desired): a part that a
+//   swap pass left unchanged would do a sliver pass next, while the joined
+//   mesh goes on swapping.  Such parts are set aside (those already at
+//   min_quality_desired are finished) and form a group of their own later,
+//   whose first swap pass changes none of them, so that they all go on to
+//   their sliver pass together.
+//
+// Each part thus sees its own pass sequence, with extra passes that leave it
+// unchanged, and ends where adapt() alone leaves it.
+
+struct PartFlags {
+  Ctx& c;
+  LOs flags;
+  int n;
+  PartFlags(Ctx& ctx, int nparts) : c(ctx), n(nparts) { reset(); }
+  ~PartFlags() { c.part_changed = nullptr; }
+  void reset() {
+    flags = fill_lo(c, n, 0);
+    c.part_changed = flags.data();
+  }
+  // which parts the passes since the last reset changed
+  std::vector<LO> read() { return to_host(c, flags); }
+};
+
+void put_back(Ctx& c, DMesh& joined, std::vector<DMesh>& meshes, std::vector<int> const& ids) {
+  std::vector<DMesh> parts;
+  split_mesh(c, joined, int(ids.size()), parts);
+  for (std::size_t k = 0; k < ids.size(); ++k) meshes[std::size_t(ids[k])] = parts[k];
+}
+
+DMesh joined_of(Ctx& c, std::vector<DMesh>& meshes, std::vector<int> const& ids) {
+  std::vector<DMesh const*> ps;
+  for (int i : ids) ps.push_back(&meshes[std::size_t(i)]);
+  DMesh joined;
+  join_meshes(c, ps, joined);
+  return joined;
+}
+
+void batch_lengths(Ctx& c, std::vector<DMesh>& meshes, std::vector<int> active,
+    AdaptOptions const& o, std::vector<PassRecord>* log) {
+  if (!o.should_refine && !o.should_coarsen) return;
+  while (!active.empty()) {
+    int np = int(active.size());
+    DMesh joined = joined_of(c, meshes, active);
+    PartFlags pf(c, np);
+    Real q = 0.0;
+    std::vector<char> frozen;
+    bool finished = false;
+    for (;;) {
+      bool did = false;
+      if (o.should_refine) did = run_pass(c, joined, o, 0, log, &q, nullptr) || did;
+      if (o.should_coarsen) did = run_pass(c, joined, o, 1, log, &q, nullptr) || did;
+      if (!did) {
+        finished = true;
+        break;
+      }
+      std::vector<LO> changed = pf.read();
+      pf.reset();
+      int nfrozen = 0;
+      for (LO x : changed) nfrozen += (x == 0);
+      if (nfrozen > 0 && 4 * nfrozen >= np) {
+        frozen.assign(std::size_t(np), 0);
+        for (int k = 0; k < np; ++k) frozen[std::size_t(k)] = changed[std::size_t(k)] == 0;
+        break;
+      }
+    }
+    put_back(c, joined, meshes, active);
+    if (finished) break;
+    std::vector<int> next;
+    for (int k = 0; k < np; ++k)
+      if (!frozen[std::size_t(k)]) next.push_back(active[std::size_t(k)]);
+    active = next;
+  }
+}
+
+void batch_quality(Ctx& c, std::vector<DMesh>& meshes, std::vector<int> group,
+    AdaptOptions const& o, std::vector<PassRecord>* log) {
+  if (!o.should_swap && !o.should_coarsen_slivers) return;
+  std::vector<std::vector<int>> groups;
+  if (!group.empty()) groups.push_back(group);
+  while (!groups.empty()) {
+    std::vector<int> active = groups.back();
+    groups.pop_back();
+    std::vector<int> stalled;
+    while (!active.empty()) {
+      int np = int(active.size());
+      DMesh joined = joined_of(c, meshes, active);
+      PartFlags pf(c, np);
+      SwapCache cache;
+      Real q = min_real(c, ask_qualities(c, joined));
+      std::vector<char> out;
+      bool finished = false;
+      while (q < o.min_quality_desired) {
+        if (o.should_swap) {
+          bool did = run_pass(c, joined, o, 2, log, &q, nullptr, &cache);
+          std::vector<LO> changed = pf.read();
+          pf.reset();
+          if (did) {
+            int nstalled = 0;
+            for (LO x : changed) nstalled += (x == 0);
+            if (nstalled > 0 && 4 * nstalled >= np) {
+              out.assign(std::size_t(np), 0);
+              for (int k = 0; k < np; ++k) out[std::size_t(k)] = changed[std::size_t(k)] == 0;
+              break;
+            }
+            continue;
+          }
+        }
+        // no part swapped: every part goes on to its sliver pass
+        cache.clear();
+        if (o.should_coarsen_slivers && run_pass(c, joined, o, 3, log, &q, nullptr)) {
+          pf.reset();
+          continue;
+        }
+        break;  // could not satisfy quality
+      }
+      if (out.empty()) finished = true;
+      put_back(c, joined, meshes, active);
+      if (finished) break;
+      std::vector<int> next;
+      for (int k = 0; k < np; ++k) {
+        int id = active[std::size_t(k)];
+        if (!out[std::size_t(k)]) {
+          next.push_back(id);
+        } else if (min_real(c, ask_qualities(c, meshes[std::size_t(id)])) <
+                   o.min_quality_desired) {
+          stalled.push_back(id);  // its next pass is a sliver pass
+        }
+      }
+      active = next;
+    }
+    if (!stalled.empty()) groups.push_back(stalled);
+  }
+}
+
+}  // namespace
+
+std::vector<int> adapt_batch(Ctx& c, std::vector<DMesh>& meshes, AdaptOptions const& o,
+    std::vector<PassRecord>* log) {
+  OSH_CHECK(0.0 <= o.min_quality_allowed, "min_quality_allowed < 0");
+  OSH_CHECK(o.min_quality_allowed <= o.min_quality_desired, "min_quality_allowed > desired");
+  OSH_CHECK(o.min_quality_desired <= 1.0, "min_quality_desired > 1");
+  OSH_CHECK(o.nsliver_layers >= 0 && o.nsliver_layers < 100, "nsliver_layers out of range");
+  int n = int(meshes.size());
+  std::vector<int> did(std::size_t(n), 0);
+  std::vector<int> todo;
+  for (int i = 0; i < n; ++i) {
+    DMesh& m = meshes[std::size_t(i)];
+    OSH_CHECK(m.etags.empty() && m.itags.empty(),
+        "batched adapt: meshes with element or inheritance tags");
+    // pre_adapt (src/Omega_h_adapt.cpp:153-167) per mesh
+    Reals const& l = ask_lengths(c, m);
+    Real lmin = min_real(c, l), lmax = max_real(c, l);
+    Real qmin = min_real(c, ask_qualities(c, m));
+    if (qmin >= o.min_quality_desired && lmin >= o.min_length_desired &&
+        lmax <= o.max_length_desired)
+      continue;
+    did[std::size_t(i)] = 1;
+    todo.push_back(i);
+  }
+  AdaptOptions quiet = o;
+  quiet.verbosity = 0;
+  batch_lengths(c, meshes, todo, quiet, log);
+  std::vector<int> low;
+  for (int i : todo)
+    if (min_real(c, ask_qualities(c, meshes[std::size_t(i)])) < o.min_quality_desired)
+      low.push_back(i);
+  batch_quality(c, meshes, low, quiet, log);
+  return did;
+}
+
 }  // namespace osh

@@ -28,6 +28,8 @@ char const* const kPartTag = "omega_h_amd_part";
 
 namespace {
 
+constexpr int kWave = 64;
+
 __global__ void copy_offset(LO n, LO const* in, LO offset, LO* out) {
   LO i = thread_index();
   if (i < n) out[i] = in[i] + offset;
@@ -50,21 +52,47 @@ DArray<T> slice(Ctx& c, DArray<T> const& src, LO at, LO n) {
 }
 
 // first and last entity of each part, and its entity count, from the part
-// number of each entity's first vertex
+// number of each entity's first vertex.  Parts are contiguous blocks, so a
+// wave's entities nearly always belong to one part: the wave then combines
+// its lanes and makes one atomic update per counter instead of 64 (lanes of
+// different parts update one by one), which keeps the few counters of a
+// batch from serialising the whole mesh's atomics.
 __global__ void part_bounds(LO n, LO const* ev2v, int deg, Real const* vpart, int nparts,
     LO* first, LO* last, LO* count, int* error) {
   LO e = thread_index();
-  if (e >= n) return;
-  LO v = ev2v ? ev2v[std::size_t(e) * deg] : e;
-  Real pr = vpart[v];
-  int p = int(pr);
-  if (!(pr >= 0.0) || p >= nparts || Real(p) != pr) {
-    *error = kBadPart;
-    return;
+  int lane = int(threadIdx.x) & (kWave - 1);
+  bool valid = e < n;
+  int p = -1;
+  if (valid) {
+    LO v = ev2v ? ev2v[std::size_t(e) * deg] : e;
+    Real pr = vpart[v];
+    p = int(pr);
+    if (!(pr >= 0.0) || p >= nparts || Real(p) != pr) {
+      *error = kBadPart;
+      valid = false;
+    }
   }
-  atomicMin(first + p, e);
-  atomicMax(last + p, e);
-  atomicAdd(count + p, 1);
+  unsigned long long vmask = __ballot(valid);
+  if (vmask == 0ull) return;
+  int leader = __ffsll(static_cast<long long>(vmask)) - 1;
+  int p0 = __shfl(p, leader, kWave);
+  if (__ballot(valid && p != p0) == 0ull) {
+    LO lo = valid ? e : 0x7fffffff, hi = valid ? e : -1;
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+      LO olo = __shfl_xor(lo, d, kWave), ohi = __shfl_xor(hi, d, kWave);
+      lo = olo < lo ? olo : lo;
+      hi = ohi > hi ? ohi : hi;
+    }
+    if (lane == leader) {
+      atomicMin(first + p0, lo);
+      atomicMax(last + p0, hi);
+      atomicAdd(count + p0, LO(__popcll(vmask)));
+    }
+  } else if (valid) {
+    atomicMin(first + p, e);
+    atomicMax(last + p, e);
+    atomicAdd(count + p, 1);
+  }
 }
 
 }  // namespace
@@ -215,6 +243,89 @@ void split_mesh(Ctx& c, DMesh& m, int nparts, std::vector<DMesh>& out) {
     if (qual.exists()) r.quality = slice(c, qual, first[dim][k], r.nents[dim]);
   }
   c.sync();
+}
+
+namespace {
+
+__global__ void note_parts(LO nkeys, LO const* keys2kds, LO const* ev2v, Real const* vpart,
+    int* changed) {
+  LO k = thread_index();
+  if (k >= nkeys) return;
+  LO kd = keys2kds[k];
+  LO v = ev2v ? ev2v[std::size_t(kd) * 2] : kd;
+  changed[int(vpart[v])] = 1;
+}
+
+}  // namespace
+
+namespace {
+
+// an order-preserving integer key of a double (negative values flipped)
+__device__ __forceinline__ unsigned long long order_key(double x) {
+  unsigned long long u = __double_as_longlong(x);
+  return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+
+__device__ __forceinline__ double from_order_key(unsigned long long k) {
+  unsigned long long u = (k >> 63) ? (k & ~(1ull << 63)) : ~k;
+  return __longlong_as_double(u);
+}
+
+// per-part minima with one atomic per wave when the wave lies in one part
+// (the parts of a joined mesh are contiguous blocks)
+__global__ void part_min_kernel(LO ne, int deg, LO const* ev2v, Real const* vpart,
+    Real const* quality, unsigned long long* mins) {
+  LO e = thread_index();
+  int lane = int(threadIdx.x) & (kWave - 1);
+  bool valid = e < ne;
+  int p = valid ? int(vpart[ev2v[std::size_t(e) * deg]]) : -1;
+  unsigned long long key = valid ? order_key(quality[e]) : ~0ull;
+  unsigned long long vmask = __ballot(valid);
+  if (vmask == 0ull) return;
+  int leader = __ffsll(static_cast<long long>(vmask)) - 1;
+  int p0 = __shfl(p, leader, kWave);
+  if (__ballot(valid && p != p0) == 0ull) {
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+      unsigned long long o = __shfl_xor(key, d, kWave);
+      key = o < key ? o : key;
+    }
+    if (lane == leader) atomicMin(mins + p0, key);
+  } else if (valid) {
+    atomicMin(mins + p, key);
+  }
+}
+
+__global__ void decode_mins(LO n, unsigned long long const* keys, Real* out) {
+  LO i = thread_index();
+  if (i < n) out[i] = from_order_key(keys[i]);
+}
+
+}  // namespace
+
+std::vector<Real> part_min_quality(Ctx& c, DMesh& m, int nparts) {
+  Reals const* part = nullptr;
+  for (auto const& t : m.vtags)
+    if (t.name == kPartTag) part = &t.data;
+  OSH_CHECK(part != nullptr, "part_min_quality: the mesh was not joined");
+  // the key of +infinity
+  std::vector<unsigned long long> init(std::size_t(nparts), 0xfff0000000000000ull);
+  DArray<unsigned long long> keys = to_device(c, init);
+  int dim = m.dim;
+  OSH_LAUNCH(part_min_kernel, m.nents[dim], c, dim + 1, verts_of(c, m, dim).data(),
+      part->data(), ask_qualities(c, m).data(), keys.data());
+  Reals out(c, nparts);
+  OSH_LAUNCH(decode_mins, nparts, c, keys.data(), out.data());
+  return to_host(c, out);
+}
+
+void note_changed_parts(Ctx& c, DMesh const& m, int key_dim, LOs const& keys2kds) {
+  if (!c.part_changed) return;
+  Real const* part = nullptr;
+  for (auto const& t : m.vtags)
+    if (t.name == kPartTag) part = t.data.data();
+  if (!part) return;
+  OSH_LAUNCH(note_parts, keys2kds.size(), c, keys2kds.data(),
+      key_dim == 1 ? m.down[1].data() : (LO const*)nullptr, part, c.part_changed);
 }
 
 }  // namespace osh

@@ -610,6 +610,41 @@ int osh_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats
   });
 }
 
+int osh_adapt_batch(int nmeshes, osh_mesh* const* meshes, const osh_adapt_opts* opts,
+    osh_adapt_stats* stats, int* did) {
+  return guarded([&] {
+    OSH_CHECK(nmeshes >= 1 && meshes && opts && did, "null argument");
+    osh_ctx* owner = nullptr;
+    std::vector<osh::DMesh> ms;
+    for (int i = 0; i < nmeshes; ++i) {
+      check_adaptable(meshes[i]);
+      if (!owner) owner = meshes[i]->owner;
+      OSH_CHECK(meshes[i]->owner == owner, "batched adapt: every mesh must share one context");
+      for (int j = 0; j < i; ++j)
+        OSH_CHECK(meshes[j] != meshes[i], "batched adapt: a mesh listed twice");
+      ms.push_back(meshes[i]->m);
+    }
+    osh::AdaptOptions o;
+    to_options(opts, o);
+    std::vector<osh::PassRecord> log;
+    std::vector<int> r = osh::adapt_batch(owner->ctx, ms, o, &log);
+    owner->ctx.sync();
+    for (int i = 0; i < nmeshes; ++i) {
+      meshes[i]->m = ms[std::size_t(i)];
+      did[i] = r[std::size_t(i)];
+    }
+    if (stats) {
+      stats->npasses = int(std::min<std::size_t>(log.size(), OSH_MAX_PASSES));
+      for (int i = 0; i < stats->npasses; ++i) {
+        stats->kind[i] = log[i].kind;
+        stats->nelems_in[i] = log[i].nelems_in;
+        stats->nelems_out[i] = log[i].nelems_out;
+        stats->seconds[i] = log[i].seconds;
+      }
+    }
+  });
+}
+
 int osh_mesh_screen_estimates(osh_mesh* mesh, double* lengths32, double* qualities32,
     signed char* unsure) {
   return guarded([&] {

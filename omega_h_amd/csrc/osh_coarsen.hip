@@ -833,6 +833,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   Bytes keys = find_indset(c, m, 0, vqual, vcand, nullptr, nullptr, &star);
   LOs keys2verts = collect_marked(c, keys);
   LO nkeys = keys2verts.size();
+  note_changed_parts(c, m, 0, keys2verts);
   if (o.verbosity >= 2) std::printf("coarsening %d vertices\n", int(nkeys));
   LOs rails2edges(c, nkeys), col_dirs(c, nkeys), keys2onto(c, nkeys);
   OSH_LAUNCH(find_rails_kernel, nkeys, c, keys2verts.data(), vrail.data(), v2e.a2ab.data(),

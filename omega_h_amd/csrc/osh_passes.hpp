@@ -164,6 +164,12 @@ void join_meshes(Ctx& c, std::vector<DMesh const*> const& parts, DMesh& out);
 void part_ranges(Ctx& c, DMesh& m, int nparts, std::vector<LO> first[4],
     std::vector<LO> count[4]);
 void split_mesh(Ctx& c, DMesh& m, int nparts, std::vector<DMesh>& out);
+// with Ctx::part_changed set, marks the parts of the pass's keys (the part
+// of a key's first vertex) as changed; called by every pass with its keys
+void note_changed_parts(Ctx& c, DMesh const& m, int key_dim, LOs const& keys2kds);
+// the minimum element quality of each part of a joined mesh (a part
+// without elements gets +infinity)
+std::vector<Real> part_min_quality(Ctx& c, DMesh& m, int nparts);
 
 struct PassRecord {
   int kind;  // 0 refine, 1 coarsen, 2 swap, 3 coarsen slivers
@@ -177,6 +183,12 @@ struct PassRecord {
 using PassHook = std::function<void(DMesh&)>;
 bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log,
     PassHook const* hook = nullptr);
+// adapt() of many meshes at once (the batched form): joined into one
+// disjoint mesh whose passes cover every mesh in one launch per kernel, with
+// the parts that can no longer change set aside as they appear; each mesh
+// ends where adapt() alone leaves it.  Returns adapt()'s result per mesh.
+std::vector<int> adapt_batch(Ctx& c, std::vector<DMesh>& meshes, AdaptOptions const& o,
+    std::vector<PassRecord>* log);
 // approach_metric (src/Omega_h_approach.cpp:83-126): move the metric toward
 // the "target_metric" vertex tag; false when there is no target
 bool approach_metric(Ctx& c, DMesh& m, AdaptOptions const& o, Real min_step);

@@ -220,6 +220,7 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
   if (!any) return false;
   LOs keys2edges = collect_marked(c, keys);
   LO nkeys = keys2edges.size();
+  note_changed_parts(c, m, 1, keys2edges);
   if (o.verbosity >= 2) std::printf("refining %d edges\n", int(nkeys));
   int nc = m.metric_ncomps;
   DMesh nw;

@@ -121,6 +121,9 @@ struct Ctx {
   unsigned* tile_counter = nullptr;
   unsigned tile_tickets = 0;
   Tuning tuning;
+  // when set (by the batched driver), each pass marks here the parts of a
+  // joined mesh that it changes
+  int* part_changed = nullptr;
   // device copy of the edge-swap tables, uploaded on first use
   std::shared_ptr<void> swap_tables;
   std::shared_ptr<Pool> pool;

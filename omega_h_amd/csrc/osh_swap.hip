@@ -941,6 +941,7 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
       offs[2].data(), offs[3].data(), rep_of[2].data(), rep_of[3].data(), keys2edges.data(),
       k2p[1].data(), k2p[2].data(), k2p[3].data(), plans[0].reps.data(), plans[1].reps.data(),
       plans[2].reps.data());
+  note_changed_parts(c, m, 1, keys2edges);
   LOs prod_keys[4];  // the key of each product face and tet
   prod_keys[2] = LOs(c, nprods[2]);
   prod_keys[3] = LOs(c, nprods[3]);
@@ -983,6 +984,7 @@ bool swap_edges_2d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   if (!any) return false;
   LOs keys2edges = collect_marked(c, keys);
   LO nkeys = keys2edges.size();
+  note_changed_parts(c, m, 1, keys2edges);
   if (o.verbosity >= 2) std::printf("swapping %d 2D edges\n", int(nkeys));
   LOs k2p[4], pv[4];
   k2p[1] = iota(c, nkeys + 1);

@@ -58,3 +58,39 @@ def test_joined_adapt_equals_separate_adapts(ctx, dim, seed, nparts, n_range, h_
         o = make(O, None, dim, n, v)
         o.adapt(oo)
         assert o.digests() == parts[k].digests(), f"part {k} against the oracle"
+
+
+@pytest.mark.parametrize("dim,seed,nparts,n_range,h_range,cap", [
+    (3, 0x0A5A17E5EED5EED5, 16, (4, 14), (0.005, 0.1), 20000),
+    (2, 7, 12, (4, 40), (0.005, 0.1), 8000),
+])
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_compacting_batch_equals_separate_adapts(ctx, dim, seed, nparts, n_range, h_range, cap,
+                                                 precision):
+    """osh_adapt_batch (joined meshes, frozen parts set aside as they appear)
+    leaves every mesh exactly where its own adapt() leaves it; a mesh that
+    already meets the goals is left alone and reported as such."""
+    boxes = W.batch_boxes(nparts, seed, n_range=n_range, h_range=h_range, max_nelems=cap,
+                          dim=dim)
+    # a box whose edges are all within the length goals (metric edge
+    # lengths 0.8, 1.13 and 1.39 for the box's three edge directions) and
+    # whose elements are good: adapt() has nothing to do
+    n_ok = 5
+    h = 1.25 / n_ok
+    ok = np.array([1.0 / h ** 2] + [1.0 / h ** 2] * (dim - 1) + [0.0] * (3 if dim == 3 else 1))
+    boxes.insert(3, (n_ok, ok, 0.0, 1.0))
+    opts = ctx.default_opts(dim)
+    opts.verbosity = 0
+    opts.precision = precision
+    alone, want_did = [], []
+    for n, v, _, _ in boxes:
+        m = make(A, ctx, dim, n, v)
+        want_did.append(m.adapt(opts)[0])
+        alone.append(m)
+    assert want_did[3] == 0 and sum(want_did) >= len(boxes) - 1
+    batch = [make(A, ctx, dim, n, v) for n, v, _, _ in boxes]
+    did, passes = A.adapt_batch(batch, opts)
+    assert did == want_did
+    assert len(passes) > 0
+    for k, (b, m) in enumerate(zip(batch, alone)):
+        assert b.digests() == m.digests(), f"mesh {k}"
