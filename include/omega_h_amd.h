@@ -55,13 +55,17 @@ typedef struct {
 
 #define OSH_MAX_PASSES 4096
 /* per-pass record of an adapt() run: kind 0 refine, 1 coarsen, 2 swap,
- * 3 coarsen slivers */
+ * 3 coarsen slivers; the first OSH_MAX_PASSES passes are listed, and the
+ * totals cover every pass */
 typedef struct {
   int npasses;
   int kind[OSH_MAX_PASSES];
   int nelems_in[OSH_MAX_PASSES];
   int nelems_out[OSH_MAX_PASSES];
   double seconds[OSH_MAX_PASSES];
+  long long total_passes;
+  long long total_nelems_in;
+  double total_seconds;
 } osh_adapt_stats;
 
 /* message of the last failed call on this thread */
@@ -80,6 +84,34 @@ int osh_ctx_stream(osh_ctx* ctx, void** out);
  * workload. */
 int osh_ctx_set_tuning(osh_ctx* ctx, const char* name, int value);
 int osh_ctx_get_tuning(osh_ctx* ctx, const char* name, int* out);
+
+/* Named ranges of a context, the timer tree of Omega_h::profile::History
+ * with ScopedTimer/OMEGA_H_TIME_FUNCTION (src/Omega_h_profile.hpp:47-232),
+ * per context instead of the reference's process-wide singleton.  flags: 1
+ * keeps the tree, 2 emits roctx ranges (rocprofv3 --marker-trace), 4 makes
+ * each range wait for the context's stream at its end so that its time
+ * covers the device work it enqueued (profiling runs only: it serialises the
+ * host and the device).  Ranges are named after the reference's timed
+ * functions (adapt, satisfy_lengths, satisfy_quality, refine_by_size,
+ * coarsen_by_size, swap_edges, coarsen_slivers, ...).  Flags change only
+ * while no range is open.  The environment variables OSH_TIME, OSH_TIME_PERCENT
+ * and OSH_TIME_CHOP stand for Omega_h::Library's --osh-time options
+ * (src/Omega_h_library.cpp:99-131): a context opened with OSH_TIME=1 profiles
+ * and prints its listings when it closes. */
+int osh_ctx_set_profile(osh_ctx* ctx, int flags);
+/* forget every range recorded so far */
+int osh_ctx_profile_clear(osh_ctx* ctx);
+/* the tree's frames: *parent is -1 for a root; seconds and calls sum over
+ * every time the range opened under the same chain of ranges */
+int osh_ctx_profile_nframes(osh_ctx* ctx, int* n);
+int osh_ctx_profile_frame(osh_ctx* ctx, int i, int* parent, double* seconds, long long* calls,
+    char* name, int name_cap);
+/* the TOP-DOWN and BOTTOM-UP listings of print_top_down_and_bottom_up
+ * (src/Omega_h_profile.cpp:151-214) as text; *len receives the full length
+ * (write at most cap bytes, the last a NUL).  chop: leave out lines under
+ * this percentage of the roots' total; percent: times as percentages. */
+int osh_ctx_profile_report(osh_ctx* ctx, double chop, int percent, char* buf, long long cap,
+    long long* len);
 
 /* AdaptOpts::AdaptOpts(Int dim), src/Omega_h_adapt.cpp:52-85 */
 int osh_default_opts(int dim, osh_adapt_opts* out);

@@ -46,6 +46,9 @@ class AdaptStats(C.Structure):
         ("nelems_in", C.c_int * MAX_PASSES),
         ("nelems_out", C.c_int * MAX_PASSES),
         ("seconds", C.c_double * MAX_PASSES),
+        ("total_passes", C.c_longlong),
+        ("total_nelems_in", C.c_longlong),
+        ("total_seconds", C.c_double),
     ]
 
 
@@ -99,6 +102,12 @@ def load(path=None):
         "osh_ctx_stream": [P, C.POINTER(P)],
         "osh_ctx_set_tuning": [P, C.c_char_p, I],
         "osh_ctx_get_tuning": [P, C.c_char_p, IP],
+        "osh_ctx_set_profile": [P, I],
+        "osh_ctx_profile_clear": [P],
+        "osh_ctx_profile_nframes": [P, IP],
+        "osh_ctx_profile_frame": [P, I, IP, DP, C.POINTER(C.c_longlong), C.c_char_p, I],
+        "osh_ctx_profile_report": [P, C.c_double, I, C.c_char_p, C.c_longlong,
+                                   C.POINTER(C.c_longlong)],
         "osh_default_opts": [I, C.POINTER(AdaptOpts)],
         "osh_build_box": [P, I, C.c_double, C.c_double, C.c_double, I, I, I, C.POINTER(P)],
         "osh_build_box_internal": [P, I, C.c_double, C.c_double, C.c_double, I, I, I,
@@ -255,6 +264,41 @@ class Context:
         for name, value in settings.items():
             self.set_tuning(name, value)
 
+    PROFILE_TREE, PROFILE_MARKERS, PROFILE_SYNC = 1, 2, 4
+
+    def set_profile(self, flags):
+        """Named ranges (Omega_h::profile): PROFILE_TREE keeps the timer tree,
+        PROFILE_MARKERS emits roctx ranges, PROFILE_SYNC makes each range wait
+        for the stream at its end so that it is charged its device work."""
+        check(self.lib.osh_ctx_set_profile(self.h, int(flags)), self.lib)
+
+    def profile_clear(self):
+        check(self.lib.osh_ctx_profile_clear(self.h), self.lib)
+
+    def profile_frames(self):
+        """The timer tree as a list of (name, parent index or -1, seconds,
+        calls), parents before their children."""
+        n = C.c_int()
+        check(self.lib.osh_ctx_profile_nframes(self.h, C.byref(n)), self.lib)
+        out = []
+        name = C.create_string_buffer(256)
+        parent, secs, calls = C.c_int(), C.c_double(), C.c_longlong()
+        for i in range(n.value):
+            check(self.lib.osh_ctx_profile_frame(self.h, i, C.byref(parent), C.byref(secs),
+                                                 C.byref(calls), name, len(name)), self.lib)
+            out.append((name.value.decode(), parent.value, secs.value, calls.value))
+        return out
+
+    def profile_report(self, chop=0.0, percent=False):
+        """The TOP-DOWN and BOTTOM-UP listings of the timer tree."""
+        n = C.c_longlong()
+        check(self.lib.osh_ctx_profile_report(self.h, float(chop), int(percent), None, 0,
+                                              C.byref(n)), self.lib)
+        buf = C.create_string_buffer(n.value + 1)
+        check(self.lib.osh_ctx_profile_report(self.h, float(chop), int(percent), buf,
+                                              len(buf), C.byref(n)), self.lib)
+        return buf.value.decode()
+
     def measure_qualities(self, dim, metric_ncomps, n, ev2v_ptr, coords_ptr, metric_ptr,
                           out_ptr, elems_ptr=0):
         """measure_qualities on device pointers (ints), on this context's stream."""
@@ -335,8 +379,9 @@ class Context:
 def adapt_batch(meshes, opts):
     """adapt() of many meshes of one context at once (the batched form):
     each mesh ends where its own adapt() would leave it.  Returns the
-    per-mesh results and the passes of the joined meshes as (kind, elements
-    in, elements out, seconds)."""
+    per-mesh results, the first passes of the joined meshes as (kind,
+    elements in, elements out, seconds) and the totals over every pass
+    (passes, elements in, seconds)."""
     lib = meshes[0].lib
     arr = (C.c_void_p * len(meshes))(*[m.h for m in meshes])
     st = AdaptStats()
@@ -344,7 +389,7 @@ def adapt_batch(meshes, opts):
     check(lib.osh_adapt_batch(len(meshes), arr, C.byref(opts), C.byref(st), did), lib)
     passes = [(st.kind[i], st.nelems_in[i], st.nelems_out[i], st.seconds[i])
               for i in range(st.npasses)]
-    return [int(x) for x in did], passes
+    return [int(x) for x in did], passes, (st.total_passes, st.total_nelems_in, st.total_seconds)
 
 
 class Mesh:
@@ -706,11 +751,16 @@ class Mesh:
         return self._pass(self.lib.osh_coarsen_slivers, opts)
 
     def adapt(self, opts):
+        """adapt(); returns (did, passes) with the first OSH_MAX_PASSES
+        passes as (kind, elements in, elements out, seconds) and keeps the
+        totals over all passes as last_adapt_totals = (passes, elements in,
+        seconds)."""
         st = AdaptStats()
         did = C.c_int()
         check(self.lib.osh_adapt(self.h, C.byref(opts), C.byref(st), C.byref(did)), self.lib)
         passes = [(st.kind[i], st.nelems_in[i], st.nelems_out[i], st.seconds[i])
                   for i in range(st.npasses)]
+        self.last_adapt_totals = (st.total_passes, st.total_nelems_in, st.total_seconds)
         return did.value, passes
 
 

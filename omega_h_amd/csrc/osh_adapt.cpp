@@ -88,12 +88,21 @@ double now() {
 
 // *qmin receives the minimum element quality after the pass, reduced before
 // the pass's closing sync so that it costs no sync of its own
+// the most passes one adapt() records: a guard on the log, far above real
+// runs (the reference's loops have no limit; one 1.3M-tet box of the serving
+// workload under a constant anisotropic metric takes 4,639 swap and 283
+// sliver passes before it stops, unable to reach the quality goal)
+constexpr std::size_t kMaxLoggedPasses = std::size_t(1) << 22;
+
 bool run_pass(Ctx& c, DMesh& m, AdaptOptions const& o, int kind,
     std::vector<PassRecord>* log, Real* qmin, PassHook const* hook,
     SwapCache* cache = nullptr) {
-  OSH_CHECK(!log || log->size() < 4096, "adapt exceeded 4096 passes");
+  OSH_CHECK(!log || log->size() < kMaxLoggedPasses, "adapt exceeded its pass limit");
+  static char const* const kPassNames[4] = {"refine_by_size", "coarsen_by_size", "swap_edges",
+      "coarsen_slivers"};
   LO nin = m.nents[m.dim];
   c.sync();
+  Range range(c, kPassNames[kind]);
   double t0 = now();
   bool r;
   if (kind == 0)
@@ -107,6 +116,7 @@ bool run_pass(Ctx& c, DMesh& m, AdaptOptions const& o, int kind,
   min_real_async(c, ask_qualities(c, m), kQminSlot);
   c.sync();
   double t1 = now();
+  range.end();
   *qmin = read_slot<Real>(c, kQminSlot);
   if (r && log) log->push_back({kind, nin, m.nents[m.dim], t1 - t0});
   if (r && hook && *hook) (*hook)(m);
@@ -127,6 +137,11 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
   // 244-270) at the reference's verbosity levels: 1 EACH_ADAPT, 2
   // EACH_REBUILD (with the passes' own lines), 3 EXTRA_STATS (histograms)
   double t0 = now();
+  // the reference's ScopedTimer adapt_timer("adapt") and the
+  // OMEGA_H_TIME_FUNCTION timers of the phases (src/Omega_h_adapt.cpp:
+  // 111, 174, 190, 273)
+  Range adapt_range(c, "adapt");
+  Range pre_range(c, "pre_adapt");
   if (o.verbosity >= 1) std::printf("before adapting:\n");
   print_status(c, m, o);
   Reals const& l = ask_lengths(c, m);
@@ -137,16 +152,20 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
     return false;
   if (o.verbosity >= 3) print_histograms(c, m, o);
   conserve_setup(c, m);
+  pre_range.end();
   if (o.verbosity >= 2) std::printf("addressing edge lengths\n");
   double t1 = now();
   bool did;
   Real q = qmin;  // the minimum quality of the current mesh
+  Range lengths_range(c, "satisfy_lengths");
   do {
     did = false;
     if (o.should_refine && run_pass(c, m, o, 0, log, &q, hook)) did = true;
     if (o.should_coarsen && run_pass(c, m, o, 1, log, &q, hook)) did = true;
   } while (did);
+  lengths_range.end();
   double t2 = now();
+  Range quality_range(c, "satisfy_quality");
   if (q < o.min_quality_desired) {
     if (o.verbosity >= 2) std::printf("addressing element qualities\n");
     // swap choices carry over between swap passes; a sliver pass rebuilds
@@ -160,10 +179,12 @@ bool adapt(Ctx& c, DMesh& m, AdaptOptions const& o, std::vector<PassRecord>* log
       break;
     } while (q < o.min_quality_desired);
   }
+  quality_range.end();
   double t3 = now();
   bool conserving = conserve_any(m);
   conserve_correct(c, m, kMaxDiffusionSweeps);
   double t4 = now();
+  adapt_range.end();
   if (o.verbosity == 1) {
     std::printf("after adapting:\n");
     print_status(c, m, o);
@@ -241,8 +262,10 @@ void batch_lengths(Ctx& c, std::vector<DMesh>& meshes, std::vector<int> active,
     bool finished = false;
     for (;;) {
       bool did = false;
-      if (o.should_refine) did = run_pass(c, joined, o, 0, log, &q, nullptr) || did;
-      if (o.should_coarsen) did = run_pass(c, joined, o, 1, log, &q, nullptr) || did;
+      if (o.should_refine)
+        did = run_pass(c, joined, o, 0, log, &q, nullptr) || did;
+      if (o.should_coarsen)
+        did = run_pass(c, joined, o, 1, log, &q, nullptr) || did;
       if (!did) {
         finished = true;
         break;
@@ -301,7 +324,8 @@ void batch_quality(Ctx& c, std::vector<DMesh>& meshes, std::vector<int> group,
         }
         // no part swapped: every part goes on to its sliver pass
         cache.clear();
-        if (o.should_coarsen_slivers && run_pass(c, joined, o, 3, log, &q, nullptr)) {
+        if (o.should_coarsen_slivers &&
+            run_pass(c, joined, o, 3, log, &q, nullptr)) {
           pf.reset();
           continue;
         }

@@ -176,6 +176,23 @@ int* tuning_field(osh::Tuning& t, const char* name) {
   throw osh::Error("unknown tuning setting " + n);
 }
 
+void fill_stats(std::vector<osh::PassRecord> const& log, osh_adapt_stats* stats) {
+  stats->npasses = int(std::min<std::size_t>(log.size(), OSH_MAX_PASSES));
+  stats->total_passes = static_cast<long long>(log.size());
+  stats->total_nelems_in = 0;
+  stats->total_seconds = 0.0;
+  for (std::size_t i = 0; i < log.size(); ++i) {
+    if (i < std::size_t(stats->npasses)) {
+      stats->kind[i] = log[i].kind;
+      stats->nelems_in[i] = log[i].nelems_in;
+      stats->nelems_out[i] = log[i].nelems_out;
+      stats->seconds[i] = log[i].seconds;
+    }
+    stats->total_nelems_in += log[i].nelems_in;
+    stats->total_seconds += log[i].seconds;
+  }
+}
+
 osh::Diffusion diffusion_of(int type, double tol, double floor) {
   OSH_CHECK(type >= 0 && type <= 2, "diffuse_type must be 0 (none), 1 or 2");
   OSH_CHECK(tol >= 0.0 && floor >= 0.0, "negative diffusion tolerance or floor");
@@ -598,13 +615,7 @@ int osh_adapt(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stats* stats
     bool r = osh::adapt(mesh->owner->ctx, mesh->m, o, &log);
     mesh->owner->ctx.sync();
     if (stats) {
-      stats->npasses = int(std::min<std::size_t>(log.size(), OSH_MAX_PASSES));
-      for (int i = 0; i < stats->npasses; ++i) {
-        stats->kind[i] = log[i].kind;
-        stats->nelems_in[i] = log[i].nelems_in;
-        stats->nelems_out[i] = log[i].nelems_out;
-        stats->seconds[i] = log[i].seconds;
-      }
+      fill_stats(log, stats);
     }
     if (did) *did = r ? 1 : 0;
   });
@@ -634,13 +645,7 @@ int osh_adapt_batch(int nmeshes, osh_mesh* const* meshes, const osh_adapt_opts* 
       did[i] = r[std::size_t(i)];
     }
     if (stats) {
-      stats->npasses = int(std::min<std::size_t>(log.size(), OSH_MAX_PASSES));
-      for (int i = 0; i < stats->npasses; ++i) {
-        stats->kind[i] = log[i].kind;
-        stats->nelems_in[i] = log[i].nelems_in;
-        stats->nelems_out[i] = log[i].nelems_out;
-        stats->seconds[i] = log[i].seconds;
-      }
+      fill_stats(log, stats);
     }
   });
 }
@@ -678,13 +683,7 @@ int osh_adapt_digests(osh_mesh* mesh, const osh_adapt_opts* opts, osh_adapt_stat
     };
     bool r = osh::adapt(c, mesh->m, o, &log, &hook);
     c.sync();
-    stats->npasses = int(std::min<std::size_t>(log.size(), OSH_MAX_PASSES));
-    for (int i = 0; i < stats->npasses; ++i) {
-      stats->kind[i] = log[i].kind;
-      stats->nelems_in[i] = log[i].nelems_in;
-      stats->nelems_out[i] = log[i].nelems_out;
-      stats->seconds[i] = log[i].seconds;
-    }
+    fill_stats(log, stats);
     if (did) *did = r ? 1 : 0;
   });
 }
@@ -751,6 +750,58 @@ int osh_ctx_get_tuning(osh_ctx* ctx, const char* name, int* out) {
   return guarded([&] {
     OSH_CHECK(ctx && name && out, "null argument");
     *out = *tuning_field(ctx->ctx.tuning, name);
+  });
+}
+
+int osh_ctx_set_profile(osh_ctx* ctx, int flags) {
+  return guarded([&] {
+    OSH_CHECK(ctx, "null argument");
+    OSH_CHECK(flags >= 0 && flags <= (osh::kProfileTree | osh::kProfileMarkers | osh::kProfileSync),
+        "profile flags are 1 (timer tree), 2 (roctx ranges) and 4 (wait at range ends)");
+    OSH_CHECK(ctx->ctx.profile.current == osh::Profile::kNone,
+        "profile flags change only while no range is open");
+    ctx->ctx.profile.flags = flags;
+  });
+}
+
+int osh_ctx_profile_clear(osh_ctx* ctx) {
+  return guarded([&] {
+    OSH_CHECK(ctx, "null argument");
+    OSH_CHECK(ctx->ctx.profile.current == osh::Profile::kNone, "a range is open");
+    ctx->ctx.profile.clear();
+  });
+}
+
+int osh_ctx_profile_nframes(osh_ctx* ctx, int* n) {
+  return guarded([&] {
+    OSH_CHECK(ctx && n, "null argument");
+    *n = int(ctx->ctx.profile.frames.size());
+  });
+}
+
+int osh_ctx_profile_frame(osh_ctx* ctx, int i, int* parent, double* seconds, long long* calls,
+    char* name, int name_cap) {
+  return guarded([&] {
+    OSH_CHECK(ctx && parent && seconds && calls && name && name_cap > 0, "null argument");
+    auto const& fr = ctx->ctx.profile.frames;
+    OSH_CHECK(i >= 0 && std::size_t(i) < fr.size(), "frame index out of range");
+    auto const& f = fr[std::size_t(i)];
+    *parent = (f.parent == osh::Profile::kNone) ? -1 : int(f.parent);
+    *seconds = f.seconds;
+    *calls = f.calls;
+    std::snprintf(name, std::size_t(name_cap), "%s", f.name.c_str());
+  });
+}
+
+int osh_ctx_profile_report(osh_ctx* ctx, double chop, int percent, char* buf, long long cap,
+    long long* len) {
+  return guarded([&] {
+    OSH_CHECK(ctx && len, "null argument");
+    OSH_CHECK(cap >= 0 && (buf || cap == 0), "null buffer");
+    auto const& p = ctx->ctx.profile;
+    std::string r = p.report(chop, percent != 0, p.root_seconds());
+    *len = static_cast<long long>(r.size());
+    if (cap > 0) std::snprintf(buf, std::size_t(cap), "%s", r.c_str());
   });
 }
 

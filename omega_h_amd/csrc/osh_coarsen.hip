@@ -683,6 +683,7 @@ __global__ void flip_veto(LO ncands, LO const* cands, LO const* ev2v, LO const* 
 
 // the whole check on the candidates' codes, in place
 void prevent_coarsen_flip(Ctx& c, DMesh& m, LOs const& cands, Bytes& codes) {
+  Range range(c, "prevent_coarsen_flip");
   int dim = m.dim;
   int sd = dim - 1;
   LO nv = m.nents[0], ns = m.nents[sd];
@@ -749,6 +750,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   int lanes = fit_lanes(ncands * 2, c.tuning.coarsen_lanes);
   int lshift = lanes_shift(lanes);
   LO nlanes = ncands * 2 * lanes;
+  Range overshoot_range(c, "prevent_overshoot");
   Bytes veto = fill_i8(c, ncands * 2, 0);
   // mixed precision: an FP32 screen first, then FP64 only for the sides it
   // left open
@@ -775,9 +777,11 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
 #undef OSH_OV_ALL
 #undef OSH_OV
   OSH_LAUNCH(apply_vetoes, ncands, c, cands.data(), veto.data(), codes.data());
+  overshoot_range.end();
   if (o.should_prevent_coarsen_flip) prevent_coarsen_flip(c, m, cands, codes);
   // sides without a candidate left are skipped by every sweep below, and the
   // check after the quality filter returns
+  Range quality_range(c, "coarsen_qualities");
   Reals quals(c, ne * 2);
   LOs const& cv2v = verts_of(c, m, dim);
   Adj v2c = marked_up(c, m, 0, dim, vmarks, nullptr, false);
@@ -817,6 +821,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   }
   OSH_LAUNCH(filter_quals_kernel, ncands, c, cands.data(), ev2v, quals.data(),
       o.min_quality_allowed, improve ? vold.data() : (Real const*)nullptr, codes.data());
+  quality_range.end();
   if (max_i8(c, codes) <= kDont) return false;
   Reals const& lengths = ask_lengths(c, m);
   Bytes vcand(c, nv);
@@ -835,6 +840,7 @@ bool coarsen(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes& codes, bool oversho
   LO nkeys = keys2verts.size();
   note_changed_parts(c, m, 0, keys2verts);
   if (o.verbosity >= 2) std::printf("coarsening %d vertices\n", int(nkeys));
+  Range modify_range(c, "modify_ents");
   LOs rails2edges(c, nkeys), col_dirs(c, nkeys), keys2onto(c, nkeys);
   OSH_LAUNCH(find_rails_kernel, nkeys, c, keys2verts.data(), vrail.data(), v2e.a2ab.data(),
       v2e.ab2b.data(), v2e.codes.data(), ev2v, rails2edges.data(), col_dirs.data(),

@@ -874,6 +874,11 @@ void conserve_pass(Ctx& c, DMesh& old, DMesh& nw, int op, int key_dim, LOs const
     for (auto& t : nw.vtags) t.err = Reals();
     return;
   }
+  // the reference's names for the three transfers (src/Omega_h_conserve.cpp:
+  // 372, 490, 457)
+  Range range(c, op == 0 ? "transfer_conserve_refine"
+                         : (op == 1 ? "transfer_densities_and_conserve_coarsen"
+                                    : "transfer_densities_and_conserve_swap"));
   if (old.dim == 3)
     pass_tmpl<3>(c, old, nw, op, key_dim, keys2kds, k2p, mo);
   else
@@ -882,6 +887,7 @@ void conserve_pass(Ctx& c, DMesh& old, DMesh& nw, int op, int key_dim, LOs const
 
 void conserve_correct(Ctx& c, DMesh& m, int max_sweeps) {
   if (!conserve_any(m)) return;
+  Range range(c, "correct_integral_errors");
   if (m.dim == 3)
     correct_tmpl<3>(c, m, max_sweeps);
   else

@@ -258,6 +258,7 @@ void measure_elems(Ctx& c, DMesh& m, LO n, LO const* elems, Real* out) {
 
 Reals const& ask_lengths(Ctx& c, DMesh& m) {
   if (!m.length.exists()) {
+    Range range(c, "measure_edges_metric");
     m.length = Reals(c, m.nents[1]);
     measure_edges(c, m, m.nents[1], nullptr, m.length.data());
   }
@@ -279,6 +280,7 @@ Reals const& ask_sizes(Ctx& c, DMesh& m) {
 
 Reals const& ask_qualities(Ctx& c, DMesh& m) {
   if (!m.quality.exists()) {
+    Range range(c, "measure_qualities");
     m.quality = Reals(c, m.nents[m.dim]);
     measure_elems(c, m, m.nents[m.dim], nullptr, m.quality.data());
   }

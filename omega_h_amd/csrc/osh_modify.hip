@@ -882,6 +882,7 @@ void transfer_measures(Ctx& c, DMesh& old, DMesh& nw, int d, ModOut const& mo) {
 
 Bytes find_indset(Ctx& c, DMesh& m, int d, Reals const& quality, Bytes const& cands,
     bool* any_cands, Adj const* edge_elems, Adj const* star) {
+  Range range(c, "find_indset");
   LO n = m.nents[d];
   if (d == m.dim) {  // src/Omega_h_indset.cpp:29
     if (any_cands) *any_cands = max_i8(c, cands) > 0;

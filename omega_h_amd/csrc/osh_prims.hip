@@ -1,7 +1,8 @@
 // Stream-ordered primitives: offset scans, compaction, fills, reductions and
 // the stable tuple sort used when building meshes.
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include "osh_runtime.hpp"
 
@@ -77,9 +78,15 @@ Ctx::Ctx() : pool(std::make_shared<Pool>()) {
   unsigned zero = 0;
   OSH_HIP(hipMemcpy(tile_counter, &zero, sizeof(zero), hipMemcpyHostToDevice));
   noalloc_setup(device);
+  profile.from_environment();
 }
 
 Ctx::~Ctx() {
+  if (profile.print_at_close) {
+    std::string r = profile.report(profile.chop, profile.percent, profile_now() - profile.origin);
+    std::fputs(r.c_str(), stdout);
+    std::fflush(stdout);
+  }
   if (stream) {
     (void)hipStreamSynchronize(stream);
     (void)hipStreamDestroy(stream);
@@ -89,6 +96,19 @@ Ctx::~Ctx() {
   if (pool) pool->close();
   if (partials) (void)hipFree(partials);
   if (pinned) (void)hipHostFree(pinned);
+}
+
+void range_begin(Ctx& c, char const* name) {
+  if (c.profile.flags & kProfileMarkers) roctxRangePushA(name);
+  if (c.profile.flags & kProfileTree) c.profile.start(name, profile_now());
+}
+
+void range_end(Ctx& c) {
+  // a failed wait is left for the next sync() to report: a range may end
+  // while an error unwinds the stack
+  if (c.profile.flags & kProfileSync) (void)hipStreamSynchronize(c.stream);
+  if (c.profile.flags & kProfileTree) c.profile.stop(profile_now());
+  if (c.profile.flags & kProfileMarkers) roctxRangePop();
 }
 
 void Ctx::sync() {
@@ -608,23 +628,154 @@ void min_real_async(Ctx& c, Reals const& a, int slot) {
 Real max_real(Ctx& c, Reals const& a) { return reduce<Real, kMax>(c, a, -1.79769313486231570815e+308); }
 LO sum_lo(Ctx& c, LOs const& a) { return reduce<LO, kSum>(c, a, 0); }
 
+namespace {
+
+// ---- stable LSD radix sort of (64-bit key, 32-bit value) pairs
+//
+// Keys are signed; flipping the sign bit orders them as unsigned words.  A
+// pass sorts by one 8-bit digit: a count kernel histograms the digits of each
+// 4096-item tile into counts[digit * ntiles + tile], one exclusive scan of
+// those counts (digit-major) gives every (digit, tile) its first output
+// position, and a scatter kernel places each item at that position plus its
+// rank among the tile's items with the same digit that precede it, so equal
+// digits keep their order.  A tile's items are read striped (item r * 256 +
+// t is thread t's r-th), so that loads coalesce and the tile order is
+// "round, then thread": the rank is the digit's count in earlier rounds (a
+// running LDS counter) plus its count in earlier waves of this round (LDS)
+// plus the lanes below in the same wave with the same digit (eight ballots).
+// Passes run only for the digits in which the keys differ.
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kBlock * kSortItems;
+constexpr int kSortBins = 256;
+constexpr int kSortWaves = kBlock / kWave;
+
+__device__ __forceinline__ unsigned long long sort_word(I64 k) {
+  return static_cast<unsigned long long>(k) ^ (1ull << 63);
+}
+
+// the OR and the AND of every key word (their XOR shows the bits that
+// differ); each wave combines its lanes first, so the two words see one
+// atomic per wave
+__global__ void sort_bits_kernel(LO n, I64 const* keys, unsigned long long* or_and) {
+  LO i = thread_index();
+  bool valid = i < n;
+  unsigned long long w = valid ? sort_word(keys[i]) : 0ull;
+  unsigned long long o = w, a = valid ? w : ~0ull;
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    o |= __shfl_xor(o, d, kWave);
+    a &= __shfl_xor(a, d, kWave);
+  }
+  // a wave past the end holds o = 0 and a = ~0, which change nothing
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicOr(or_and, o);
+    atomicAnd(or_and + 1, a);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) sort_count(LO n, I64 const* keys, int shift,
+    LO ntiles, LO* counts) {
+  __shared__ LO hist[kSortBins];
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  LO base = LO(blockIdx.x) * kSortTile;
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    LO i = base + r * kBlock + LO(threadIdx.x);
+    if (i < n) atomicAdd(&hist[int(sort_word(keys[i]) >> shift) & (kSortBins - 1)], 1);
+  }
+  __syncthreads();
+  counts[LO(threadIdx.x) * ntiles + LO(blockIdx.x)] = hist[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kBlock) sort_scatter(LO n, I64 const* keys, LO const* vals,
+    int shift, LO ntiles, LO const* offsets, I64* out_keys, LO* out_vals) {
+  __shared__ LO running[kSortBins];
+  __shared__ LO wave_counts[kSortWaves][kSortBins];
+  int t = int(threadIdx.x);
+  int lane = t & (kWave - 1);
+  int wave = t / kWave;
+  running[t] = offsets[LO(t) * ntiles + LO(blockIdx.x)];
+  for (int w = 0; w < kSortWaves; ++w) wave_counts[w][t] = 0;
+  __syncthreads();
+  LO base = LO(blockIdx.x) * kSortTile;
+  unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+  for (int r = 0; r < kSortItems; ++r) {
+    LO i = base + r * kBlock + t;
+    bool valid = i < n;
+    I64 k = valid ? keys[i] : 0;
+    int d = valid ? int(sort_word(k) >> shift) & (kSortBins - 1) : 0;
+    // the lanes of this wave holding the same digit
+    unsigned long long same = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      unsigned long long bit = __ballot(valid && ((d >> b) & 1));
+      same &= ((d >> b) & 1) ? bit : ~bit;
+    }
+    int rank_in_wave = __popcll(same & below);
+    bool leader = valid && rank_in_wave == 0;
+    if (leader) wave_counts[wave][d] = __popcll(same);
+    __syncthreads();
+    if (valid) {
+      LO pos = running[d] + rank_in_wave;
+      for (int w = 0; w < wave; ++w) pos += wave_counts[w][d];
+      out_keys[pos] = k;
+      out_vals[pos] = vals[i];
+    }
+    __syncthreads();
+    // advance the running offsets by this round's counts and clear them
+    LO add = 0;
+    for (int w = 0; w < kSortWaves; ++w) {
+      add += wave_counts[w][t];
+      wave_counts[w][t] = 0;
+    }
+    running[t] += add;
+    __syncthreads();
+  }
+}
+
+// sorts (keys, vals) by key in place (stable), using the second pair of
+// buffers; returns with the result in the first pair
+void radix_sort_pairs(Ctx& c, DArray<I64>& keys, LOs& vals, DArray<I64>& keys_tmp,
+    LOs& vals_tmp) {
+  LO n = keys.size();
+  if (n <= 1) return;
+  std::vector<unsigned long long> init = {0ull, ~0ull};
+  DArray<unsigned long long> or_and = to_device(c, init);
+  OSH_LAUNCH(sort_bits_kernel, n, c, keys.data(), or_and.data());
+  std::vector<unsigned long long> oa = to_host(c, or_and);
+  unsigned long long differ = oa[0] ^ oa[1];
+  LO ntiles = LO((std::int64_t(n) + kSortTile - 1) / kSortTile);
+  OSH_CHECK(std::int64_t(ntiles) * kSortBins < std::int64_t(0x7fffffff), "sort: too many items");
+  LOs counts(c, ntiles * kSortBins), offsets(c, ntiles * kSortBins + 1);
+  for (int shift = 0; shift < 64; shift += 8) {
+    if (((differ >> shift) & 0xffull) == 0ull) continue;
+    hipLaunchKernelGGL(sort_count, dim3(unsigned(ntiles)), dim3(kBlock), 0, c.stream, n,
+        keys.data(), shift, ntiles, counts.data());
+    OSH_HIP(hipGetLastError());
+    ScanJob job{counts.data(), kScanLO, ntiles * kSortBins, offsets.data(), nullptr};
+    offset_scans(c, &job, 1);
+    hipLaunchKernelGGL(sort_scatter, dim3(unsigned(ntiles)), dim3(kBlock), 0, c.stream, n,
+        keys.data(), vals.data(), shift, ntiles, offsets.data(), keys_tmp.data(),
+        vals_tmp.data());
+    OSH_HIP(hipGetLastError());
+    std::swap(keys, keys_tmp);
+    std::swap(vals, vals_tmp);
+  }
+}
+
+}  // namespace
+
 // LSD over tuple components, each a stable radix sort of (key, position):
 // the last component first, so earlier components take precedence.
 LOs sort_by_keys(Ctx& c, DArray<I64> const& keys, int width) {
   LO n = keys.size() / width;
   LOs perm = iota(c, n);
   if (n <= 1) return perm;
-  DArray<I64> col(c, n), col_out(c, n);
-  LOs perm_out(c, n);
+  DArray<I64> col(c, n), col_tmp(c, n);
+  LOs perm_tmp(c, n);
   for (int k = width - 1; k >= 0; --k) {
     OSH_LAUNCH(gather_key_column, n, c, keys.data(), perm.data(), width, k, col.data());
-    std::size_t temp_bytes = 0;
-    OSH_HIP(rocprim::radix_sort_pairs(nullptr, temp_bytes, col.data(), col_out.data(),
-        perm.data(), perm_out.data(), unsigned(n), 0, 64, c.stream));
-    DArray<unsigned char> temp(c, LO(temp_bytes + 1));
-    OSH_HIP(rocprim::radix_sort_pairs(temp.data(), temp_bytes, col.data(), col_out.data(),
-        perm.data(), perm_out.data(), unsigned(n), 0, 64, c.stream));
-    std::swap(perm, perm_out);
+    radix_sort_pairs(c, col, perm, col_tmp, perm_tmp);
   }
   return perm;
 }

@@ -176,6 +176,7 @@ __global__ void midpoint_coords(LO nkeys, int dim, LO const* keys2edges, LO cons
 }  // namespace
 
 Reals refine_qualities(Ctx& c, DMesh& m, LOs const& cands) {
+  Range range(c, "refine_qualities");
   LO n = cands.size();
   int dim = m.dim;
   int nc = m.metric_ncomps;
@@ -218,6 +219,7 @@ bool refine_by_size(Ctx& c, DMesh& m, AdaptOptions const& o) {
   bool any;
   Bytes keys = find_indset(c, m, 1, edge_quals, initial, &any);
   if (!any) return false;
+  Range modify_range(c, "modify_ents");
   LOs keys2edges = collect_marked(c, keys);
   LO nkeys = keys2edges.size();
   note_changed_parts(c, m, 1, keys2edges);

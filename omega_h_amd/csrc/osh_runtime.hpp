@@ -14,6 +14,8 @@
 #include <string>
 #include <vector>
 
+#include "osh_profile.hpp"
+
 namespace osh {
 
 using LO = std::int32_t;
@@ -127,11 +129,36 @@ struct Ctx {
   // device copy of the edge-swap tables, uploaded on first use
   std::shared_ptr<void> swap_tables;
   std::shared_ptr<Pool> pool;
+  // named ranges (off unless profiling flags are set)
+  Profile profile;
   Ctx();
   ~Ctx();
   Ctx(Ctx const&) = delete;
   Ctx& operator=(Ctx const&) = delete;
   void sync();
+};
+
+void range_begin(Ctx& c, char const* name);
+void range_end(Ctx& c);
+
+// A named range from construction to end() or destruction (ScopedTimer,
+// src/Omega_h_profile.hpp:217-226); ranges of one context nest, so they end
+// in the reverse order of their starts
+class Range {
+ public:
+  Range(Ctx& c, char const* name) : c_(c.profile.flags ? &c : nullptr) {
+    if (c_) range_begin(*c_, name);
+  }
+  ~Range() { end(); }
+  void end() {
+    if (c_) range_end(*c_);
+    c_ = nullptr;
+  }
+  Range(Range const&) = delete;
+  Range& operator=(Range const&) = delete;
+
+ private:
+  Ctx* c_;
 };
 
 // reference-counted device buffer from the context's pool, used in stream

@@ -851,6 +851,7 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
     cache->keep = Bytes(c, ne);
     cache->q = Reals(c, ne);
   }
+  Range quality_range(c, "swap3d_qualities");
   LOs lsize(c, ncands), leev(c, ncands * 2), lverts(c, ncands * swap::kMaxLoop);
   LOs ntris(c, ncands), nedges(c, ncands);
   OSH_LAUNCH(swap3d_loops, ncands * kLoopLanes, c, cands.data(), e2t.a2ab.data(), e2t.ab2b.data(),
@@ -902,9 +903,11 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
       e2t.ab2b.data(), eq.data(), tb, needp, configs.data(), quals.data(), keep.data(),
       cache ? cache->valid.data() : (I8*)nullptr, cache ? cache->cfg.data() : (I8*)nullptr,
       cache ? cache->keep.data() : (I8*)nullptr, cache ? cache->q.data() : (Real*)nullptr);
+  quality_range.end();
   bool any;
   Bytes keys = find_indset(c, m, 1, quals, keep, &any, &e2t);
   if (!any) return false;
+  Range modify_range(c, "modify_ents");
   // per-edge product counts and the rebuild plans from the key mask, then
   // one scan launch and one sync for the keys, their products and the new
   // numbering of edges, faces and tets
@@ -971,6 +974,7 @@ bool swap_edges_2d(Ctx& c, DMesh& m, AdaptOptions const& o) {
   Adj const& e2t = ask_up(c, m, 1, 2);
   LOs const& tv2v = verts_of(c, m, 2);
   Reals const& eq = ask_qualities(c, m);
+  Range quality_range(c, "swap2d_qualities");
   if (md == 2)
     OSH_LAUNCH(swap2d_qualities_kernel<2>, cands.size(), c, cands.data(), e2t.a2ab.data(),
         e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), m.coords.data(),
@@ -979,9 +983,11 @@ bool swap_edges_2d(Ctx& c, DMesh& m, AdaptOptions const& o) {
     OSH_LAUNCH(swap2d_qualities_kernel<1>, cands.size(), c, cands.data(), e2t.a2ab.data(),
         e2t.ab2b.data(), e2t.codes.data(), m.down[1].data(), tv2v.data(), m.coords.data(),
         m.metric.data(), eq.data(), o.max_length_allowed, keep.data(), quals.data());
+  quality_range.end();
   bool any;
   Bytes keys = find_indset(c, m, 1, quals, keep, &any);
   if (!any) return false;
+  Range modify_range(c, "modify_ents");
   LOs keys2edges = collect_marked(c, keys);
   LO nkeys = keys2edges.size();
   note_changed_parts(c, m, 1, keys2edges);

@@ -89,7 +89,7 @@ def test_compacting_batch_equals_separate_adapts(ctx, dim, seed, nparts, n_range
         alone.append(m)
     assert want_did[3] == 0 and sum(want_did) >= len(boxes) - 1
     batch = [make(A, ctx, dim, n, v) for n, v, _, _ in boxes]
-    did, passes = A.adapt_batch(batch, opts)
+    did, passes, _ = A.adapt_batch(batch, opts)
     assert did == want_did
     assert len(passes) > 0
     for k, (b, m) in enumerate(zip(batch, alone)):

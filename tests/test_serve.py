@@ -147,3 +147,33 @@ def test_serving_stream_alone_equals_mix():
         oo.verbosity = 0
         o.adapt(oo)
         assert mixed[i].digest == fnv1a(o.verts(d), o.coords()), (d, n)
+
+
+# request 1386 of the serving workload's 20 requests/s stream (seed 20260101,
+# after the 5 requests/s stream): a 60^3 box (1,296,000 tets) under a constant
+# anisotropic metric, which adapt() leaves after 4,639 swap and 283 sliver
+# passes, unable to reach min_quality_desired
+LONG_REQUEST = (3, 60, [5790.258056459252, 2135.76209205912, 494.46986710579586,
+                        3207.769618264246, 774.5229738639886, 1204.9854838345045])
+
+
+@pytest.mark.gpu
+def test_adapt_with_more_passes_than_the_stats_list():
+    """An adapt() of more passes than osh_adapt_stats lists (OSH_MAX_PASSES)
+    runs to its end, alone and as a served request, and the totals count
+    every pass."""
+    dim, n, v = LONG_REQUEST
+    ctx = A.Context(0)
+    opts = ctx.default_opts(dim)
+    opts.verbosity = 0
+    m = A.Mesh.box(ctx, dim, n)
+    m.set_metric(np.tile(v, m.nents(0)), len(v))
+    did, passes = m.adapt(opts)
+    total, elems_in, _ = m.last_adapt_totals
+    assert did == 1 and len(passes) == 4096
+    assert total == 4922 and elems_in >= sum(p[1] for p in passes)
+    assert np.bincount([p[0] for p in passes], minlength=4)[:2].tolist() == [0, 0]
+    server = A.Server(nslots=1)
+    r = server.wait(server.submit(dim, n, v, opts, want_digest=True))
+    server.close()
+    assert r.npasses == 4922 and r.nelems_out == m.nents(dim)
