@@ -10,8 +10,9 @@
 //   refine_by_size, coarsen_by_size, coarsen_slivers, swap_edges
 // so that a call site such as src/aniso_test.cpp:15-27 compiles against it
 // by changing the include.  The mesh lives on the GPU behind an osh_mesh
-// handle; Read<T>/Write<T> here are reference-counted host arrays that cross
-// the boundary when a tag is set or read.  Errors throw Omega_h::exception,
+// handle; Read<T>/Write<T> live in device memory too (tags are read and set
+// by device-to-device copies), and HostRead/HostWrite are their host
+// mirrors.  Errors throw Omega_h::exception,
 // the reference's behaviour under Omega_h_THROW (src/Omega_h_fail.hpp:33-83).
 #ifndef OMEGA_H_AMD_HPP
 #define OMEGA_H_AMD_HPP
@@ -67,56 +68,179 @@ inline void check_status(int status) {
   if (status != 0) throw exception(osh_last_error());
 }
 
+namespace detail {
+// one context (HIP stream) of a Library; arrays and meshes hold it, so it
+// closes after the last of them
+struct Device {
+  osh_ctx* ctx = nullptr;
+  explicit Device(int device) { check_status(osh_ctx_create(device, &ctx)); }
+  ~Device() {
+    if (ctx) osh_ctx_destroy(ctx);
+  }
+  Device(Device const&) = delete;
+  Device& operator=(Device const&) = delete;
+};
+// the context new arrays are allocated on: the most recent live Library's
+inline std::shared_ptr<Device>& current_device() {
+  static std::shared_ptr<Device> d;
+  return d;
+}
+inline std::shared_ptr<Device> live_device() {
+  std::shared_ptr<Device> d = current_device();
+  if (!d) throw exception("Omega_h arrays live on the GPU of an Omega_h::Library; none is open");
+  return d;
+}
+// a block of device memory, shared by the arrays that view it
+struct Block {
+  std::shared_ptr<Device> dev;
+  void* ptr = nullptr;
+  long long bytes = 0;
+  Block(std::shared_ptr<Device> d, long long b) : dev(std::move(d)), bytes(b) {
+    if (b > 0) check_status(osh_device_alloc(dev->ctx, b, &ptr));
+  }
+  ~Block() {
+    if (ptr) osh_device_free(dev->ctx, ptr, bytes);
+  }
+  Block(Block const&) = delete;
+  Block& operator=(Block const&) = delete;
+};
+template <class T>
+std::shared_ptr<Block> device_block(LO n) {
+  if (n < 0) throw exception("negative array size");
+  return std::make_shared<Block>(live_device(), (long long)n * (long long)sizeof(T));
+}
+inline void copy_bytes(Block const& b, void* dst, void const* src, long long bytes) {
+  if (bytes > 0) check_status(osh_copy(b.dev->ctx, dst, src, bytes));
+}
+template <class T>
+void fill_items(Block const& b, LO n, T value) {
+  static_assert(sizeof(T) == 1 || sizeof(T) == 4 || sizeof(T) == 8, "items of 1, 4 or 8 bytes");
+  if (n > 0) check_status(osh_device_fill(b.dev->ctx, b.ptr, n, int(sizeof(T)), &value));
+}
+}  // namespace detail
+
+// Write<T>/Read<T> (src/Omega_h_array.hpp:21-154) live in device memory of
+// the open Library's GPU, as the reference's do in a GPU build: data() is a
+// device pointer, copies share the block, and set/get/last move one item.
+// HostRead/HostWrite are the host mirrors for host-side loops.
 template <class T>
 class Write {
  public:
   Write() = default;
   explicit Write(LO size, std::string const& = "")
-      : data_(std::make_shared<std::vector<T>>(std::size_t(size))) {}
-  Write(LO size, T value, std::string const& = "")
-      : data_(std::make_shared<std::vector<T>>(std::size_t(size), value)) {}
-  Write(std::initializer_list<T> l, std::string const& = "")
-      : data_(std::make_shared<std::vector<T>>(l)) {}
-  explicit Write(std::vector<T> v) : data_(std::make_shared<std::vector<T>>(std::move(v))) {}
-  LO size() const { return data_ ? LO(data_->size()) : 0; }
-  bool exists() const { return bool(data_); }
-  T* data() const { return data_ ? data_->data() : nullptr; }
-  T& operator[](LO i) const { return (*data_)[std::size_t(i)]; }
-  void set(LO i, T v) const { (*data_)[std::size_t(i)] = v; }
-  T get(LO i) const { return (*data_)[std::size_t(i)]; }
+      : block_(detail::device_block<T>(size)), size_(size) {}
+  Write(LO size, T value, std::string const& = "") : Write(size) {
+    detail::fill_items(*block_, size, value);
+  }
+  Write(LO size, T offset, T stride, std::string const& = "") : Write(size) {
+    std::vector<T> h(static_cast<std::size_t>(size));
+    for (LO i = 0; i < size; ++i) h[std::size_t(i)] = T(offset + stride * T(i));
+    detail::copy_bytes(*block_, data(), h.data(), bytes());
+  }
+  Write(std::initializer_list<T> l, std::string const& = "") : Write(LO(l.size())) {
+    detail::copy_bytes(*block_, data(), l.begin(), bytes());
+  }
+  LO size() const { return size_; }
+  bool exists() const { return bool(block_); }
+  T* data() const { return block_ ? static_cast<T*>(block_->ptr) : nullptr; }
+  // the context (stream) the array's block belongs to
+  osh_ctx* context() const { return block_ ? block_->dev->ctx : nullptr; }
+  void set(LO i, T v) const {
+    check_index(i);
+    detail::copy_bytes(*block_, data() + i, &v, sizeof(T));
+  }
+  T get(LO i) const {
+    check_index(i);
+    T v;
+    detail::copy_bytes(*block_, &v, data() + i, sizeof(T));
+    return v;
+  }
 
  private:
   template <class U>
   friend class Read;
-  std::shared_ptr<std::vector<T>> data_;
+  void check_index(LO i) const {
+    if (i < 0 || i >= size_) throw exception("array index out of range");
+  }
+  long long bytes() const { return (long long)size_ * (long long)sizeof(T); }
+  std::shared_ptr<detail::Block> block_;
+  LO size_ = 0;
 };
-
-// the reference's host-side arrays are the same thing here
-template <class T>
-using HostWrite = Write<T>;
 
 template <class T>
 class Read {
  public:
   Read() = default;
-  Read(Write<T> w) : data_(w.data_) {}
-  Read(std::initializer_list<T> l, std::string const& = "")
-      : data_(std::make_shared<std::vector<T>>(l)) {}
-  Read(LO size, T value, std::string const& = "")
-      : data_(std::make_shared<std::vector<T>>(std::size_t(size), value)) {}
-  LO size() const { return data_ ? LO(data_->size()) : 0; }
-  bool exists() const { return bool(data_); }
-  T const* data() const { return data_ ? data_->data() : nullptr; }
-  T const& operator[](LO i) const { return (*data_)[std::size_t(i)]; }
-  T get(LO i) const { return (*data_)[std::size_t(i)]; }
-  T last() const { return data_->back(); }
+  Read(Write<T> w) : w_(std::move(w)) {}
+  Read(std::initializer_list<T> l, std::string const& name = "") : w_(l, name) {}
+  Read(LO size, T value, std::string const& name = "") : w_(size, value, name) {}
+  Read(LO size, T offset, T stride, std::string const& name = "")
+      : w_(size, offset, stride, name) {}
+  LO size() const { return w_.size(); }
+  bool exists() const { return w_.exists(); }
+  T const* data() const { return w_.data(); }
+  osh_ctx* context() const { return w_.context(); }
+  T get(LO i) const { return w_.get(i); }
+  T last() const { return w_.get(size() - 1); }
 
  private:
-  std::shared_ptr<std::vector<T>> data_;
+  Write<T> w_;
 };
 
+// host mirror of a Read (src/Omega_h_array.hpp:155-188)
 template <class T>
-using HostRead = Read<T>;
+class HostRead {
+ public:
+  HostRead() = default;
+  HostRead(Read<T> r) : h_(std::size_t(r.size())) {
+    if (r.size())
+      check_status(osh_copy(r.context(), h_.data(), r.data(), (long long)h_.size() * sizeof(T)));
+  }
+  LO size() const { return LO(h_.size()); }
+  T const& operator[](LO i) const { return h_[std::size_t(i)]; }
+  T const* data() const { return h_.data(); }
+  T get(LO i) const { return h_.at(std::size_t(i)); }
+  T last() const { return h_.back(); }
+
+ private:
+  std::vector<T> h_;
+};
+
+// host array written on the host and sent to the device with write()
+// (src/Omega_h_array.hpp:190-230)
+template <class T>
+class HostWrite {
+ public:
+  HostWrite() = default;
+  explicit HostWrite(LO size, std::string const& = "") : h_(std::size_t(size)) {}
+  HostWrite(LO size, T offset, T stride, std::string const& = "") : h_(std::size_t(size)) {
+    for (LO i = 0; i < size; ++i) h_[std::size_t(i)] = T(offset + stride * T(i));
+  }
+  HostWrite(std::initializer_list<T> l, std::string const& = "") : h_(l) {}
+  HostWrite(Write<T> w) : w_(w), h_(std::size_t(w.size())) {
+    if (w.size())
+      check_status(osh_copy(w.context(), h_.data(), w.data(), (long long)h_.size() * sizeof(T)));
+  }
+  // the device array holding what was written on the host (the array this
+  // mirror was made from, else a new one on the open Library's GPU)
+  Write<T> write() const {
+    if (!w_.exists() || w_.size() != size()) w_ = Write<T>(size());
+    if (size())
+      check_status(osh_copy(w_.context(), w_.data(), h_.data(), (long long)h_.size() * sizeof(T)));
+    return w_;
+  }
+  LO size() const { return LO(h_.size()); }
+  T& operator[](LO i) { return h_[std::size_t(i)]; }
+  T const& operator[](LO i) const { return h_[std::size_t(i)]; }
+  T* data() { return h_.data(); }
+  bool exists() const { return true; }
+  void set(LO i, T v) { h_.at(std::size_t(i)) = v; }
+  T get(LO i) const { return h_.at(std::size_t(i)); }
+
+ private:
+  mutable Write<T> w_;
+  std::vector<T> h_;
+};
 
 using LOs = Read<LO>;
 using GOs = Read<GO>;
@@ -146,45 +270,53 @@ struct Comm {
 
 using CommPtr = std::shared_ptr<Comm>;
 
-// one HIP stream on one GPU (the device is the only argument that matters)
+// one HIP stream on one GPU (the device is the only argument that matters);
+// arrays made while it is open live on its GPU.  The reference's --osh-time
+// options are the environment variables OSH_TIME, OSH_TIME_PERCENT and
+// OSH_TIME_CHOP here.
 class Library {
  public:
-  explicit Library(int* argc = nullptr, char*** argv = nullptr, int device = 0) {
+  explicit Library(int* argc = nullptr, char*** argv = nullptr, int device = 0)
+      : dev_(std::make_shared<detail::Device>(device)) {
     (void)argc;
     (void)argv;
-    check_status(osh_ctx_create(device, &ctx_));
+    detail::current_device() = dev_;
   }
   Library(Library const&) = delete;
   Library& operator=(Library const&) = delete;
-  ~Library() { osh_ctx_destroy(ctx_); }
-  osh_ctx* ctx() const { return ctx_; }
+  ~Library() {
+    if (detail::current_device() == dev_) detail::current_device().reset();
+  }
+  osh_ctx* ctx() const { return dev_->ctx; }
+  std::shared_ptr<detail::Device> const& device() const { return dev_; }
   CommPtr world() { return std::make_shared<Comm>(Comm{this}); }
   CommPtr self() { return world(); }
 
  private:
-  osh_ctx* ctx_ = nullptr;
+  std::shared_ptr<detail::Device> dev_;
 };
 
 class Mesh {
  public:
   Mesh() = default;
-  explicit Mesh(Library* library) : library_(library) {}
+  explicit Mesh(Library* library) : library_(library), dev_(library->device()) {}
   // wraps a handle this object now owns
   Mesh(Library* library, osh_mesh* h)
-      : library_(library), h_(h, [](osh_mesh* m) { osh_mesh_destroy(m); }) {}
+      : library_(library), dev_(library->device()), h_(own(h)) {}
   // copies share the device arrays; passes build new ones (like assigning
   // an Omega_h::Mesh)
-  Mesh(Mesh const& o) : library_(o.library_) {
+  Mesh(Mesh const& o) : library_(o.library_), dev_(o.dev_) {
     if (o.h_) {
       osh_mesh* c = nullptr;
       check_status(osh_mesh_copy(o.h_.get(), &c));
-      h_.reset(c, [](osh_mesh* m) { osh_mesh_destroy(m); });
+      h_ = own(c);
     }
   }
   Mesh& operator=(Mesh const& o) {
     if (this != &o) {
       Mesh t(o);
       library_ = t.library_;
+      dev_ = t.dev_;
       h_ = std::move(t.h_);
     }
     return *this;
@@ -232,11 +364,7 @@ class Mesh {
     check_status(osh_mesh_set_coords(handle(), array.data()));
   }
   // one rank: the global number of an entity is its index
-  Read<GO> globals(Int ent_dim) const {
-    Write<GO> w(nents(ent_dim));
-    for (LO i = 0; i < w.size(); ++i) w[i] = i;
-    return w;
-  }
+  Read<GO> globals(Int ent_dim) const { return Read<GO>(nents(ent_dim), GO(0), GO(1)); }
   LOs ask_verts_of(Int ent_dim) const {
     Write<LO> w(nents(ent_dim) * (ent_dim + 1));
     check_status(osh_mesh_get_verts(handle(), ent_dim, w.data()));
@@ -292,15 +420,13 @@ class Mesh {
     return w;
   }
   Real min_quality() const {
-    Reals q = ask_qualities();
-    Real r = 1.0;
-    for (LO i = 0; i < q.size(); ++i) r = (q[i] < r) ? q[i] : r;
+    Real r;
+    check_status(osh_mesh_min_quality(handle(), &r));
     return r;
   }
   Real max_length() const {
-    Reals l = ask_lengths();
-    Real r = 0.0;
-    for (LO i = 0; i < l.size(); ++i) r = (l[i] > r) ? l[i] : r;
+    Real r;
+    check_status(osh_mesh_max_length(handle(), &r));
     return r;
   }
 
@@ -348,7 +474,13 @@ class Mesh {
   Read<T> get_array(Int ent_dim, std::string const& name) const;
 
  private:
+  // the mesh handle keeps its context open
+  std::shared_ptr<osh_mesh> own(osh_mesh* h) const {
+    std::shared_ptr<detail::Device> d = dev_;
+    return std::shared_ptr<osh_mesh>(h, [d](osh_mesh* m) { osh_mesh_destroy(m); });
+  }
   Library* library_ = nullptr;
+  std::shared_ptr<detail::Device> dev_;
   std::shared_ptr<osh_mesh> h_;
 };
 

@@ -7,7 +7,10 @@
  * Every function returns 0 on success and a nonzero status otherwise, never
  * aborts, and leaves a message for osh_last_error().  All work is ordered on
  * the context's HIP stream; functions that hand data to the host return
- * after it has arrived.
+ * after it has arrived.  Array arguments of the mesh, metric and pass
+ * functions may point to host memory or to device memory (osh_device_alloc,
+ * any hipMalloc'd block): copies let the HIP runtime tell the two apart, so
+ * data that already lives on the GPU never passes through the host.
  */
 #ifndef OMEGA_H_AMD_H
 #define OMEGA_H_AMD_H
@@ -76,6 +79,21 @@ int osh_ctx_destroy(osh_ctx* ctx);
 int osh_ctx_synchronize(osh_ctx* ctx);
 /* the hipStream_t the context orders its work on (for events and interop) */
 int osh_ctx_stream(osh_ctx* ctx, void** out);
+/* Device memory a caller keeps on the GPU: the storage of the C++ facade's
+ * Read<T>/Write<T>, which live on the device as the reference's do in a GPU
+ * build (src/Omega_h_array.hpp:21-204, allocated through Kokkos views).
+ * Blocks come from the context's caching pool and are used in its stream
+ * order; osh_device_free takes the size given to osh_device_alloc. */
+int osh_device_alloc(osh_ctx* ctx, long long bytes, void** out);
+int osh_device_free(osh_ctx* ctx, void* p, long long bytes);
+/* copy `bytes` between any host or device buffers after the context's
+ * earlier work; returns when the copy is done (Omega_h's deep_copy) */
+int osh_copy(osh_ctx* ctx, void* dst, const void* src, long long bytes);
+/* set n items of item_bytes (1, 4 or 8) bytes at dst (device memory) to
+ * *value; returns when done (the fill of Write<T>(size, value)) */
+int osh_device_fill(osh_ctx* ctx, void* dst, long long n, int item_bytes, const void* value);
+/* *out = 1 when p is device memory */
+int osh_pointer_is_device(const void* p, int* out);
 /* the context's lane counts (no counterpart in the reference): "coarsen_lanes",
  * "side_lanes", "choose_lanes" and "first_lanes", each a power of two from 1
  * to 64, the threads that share one collapse side, one swap product side,
@@ -191,6 +209,10 @@ int osh_mesh_set_class(osh_mesh* mesh, int ent_dim, const int* ids, const signed
 /* Mesh::ask_lengths / ask_qualities, src/Omega_h_mesh.cpp:374-388 */
 int osh_mesh_get_lengths(osh_mesh* mesh, double* out);
 int osh_mesh_get_qualities(osh_mesh* mesh, double* out);
+/* Mesh::min_quality / max_length (src/Omega_h_mesh.cpp:698-707): reduced on
+ * the device */
+int osh_mesh_min_quality(osh_mesh* mesh, double* out);
+int osh_mesh_max_length(osh_mesh* mesh, double* out);
 
 /* passes: refine_by_size src/Omega_h_refine.cpp:92, coarsen_by_size
  * src/Omega_h_coarsen.cpp:198, swap_edges src/Omega_h_swap.cpp:45,

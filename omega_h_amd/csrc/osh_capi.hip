@@ -1,5 +1,6 @@
 // The C ABI (include/omega_h_amd.h): status codes, thread-local error
 // messages, handles over the device runtime.
+#include <cstring>
 #include <memory>
 #include <string>
 
@@ -108,12 +109,43 @@ using osh::guarded;
 using osh::to_options;
 
 
+// array arguments of the entry points may be host memory or device memory
+// (a facade Read/Write, any hipMalloc'd or pool block): copies let the
+// runtime tell them apart
 template <class T>
 void download(osh::Ctx& c, osh::DArray<T> const& d, T* out) {
   if (d.size())
-    OSH_HIP(hipMemcpyAsync(out, d.data(), sizeof(T) * size_t(d.size()), hipMemcpyDeviceToHost,
+    OSH_HIP(hipMemcpyAsync(out, d.data(), sizeof(T) * size_t(d.size()), hipMemcpyDefault,
         c.stream));
   c.sync();
+}
+
+// a host copy of a caller array (host or device) that host code reads
+template <class T>
+std::vector<T> host_copy(osh::Ctx& c, T const* p, std::size_t n) {
+  std::vector<T> h(n);
+  if (n) {
+    OSH_HIP(hipMemcpyAsync(h.data(), p, sizeof(T) * n, hipMemcpyDefault, c.stream));
+    c.sync();
+  }
+  return h;
+}
+
+template <class T>
+__global__ void fill_items(long long n, T* p, T v) {
+  long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+template <class T>
+void fill_device(osh::Ctx& c, void* dst, long long n, void const* value) {
+  T v;
+  std::memcpy(&v, value, sizeof(T));
+  long long blocks = (n + 255) / 256;
+  OSH_CHECK(blocks <= 0x7fffffffLL, "fill: too many items");
+  hipLaunchKernelGGL(fill_items<T>, dim3(unsigned(blocks)), dim3(256), 0, c.stream, n,
+      static_cast<T*>(dst), v);
+  OSH_HIP(hipGetLastError());
 }
 
 void check_mesh(osh_mesh* m) {
@@ -316,7 +348,7 @@ int osh_build_from_elems(osh_ctx* ctx, int dim, int nelems, const int* ev2v, int
     OSH_CHECK(ctx && out && (ev2v || nelems == 0), "null argument");
     OSH_CHECK(dim == 2 || dim == 3, "dimension must be 2 or 3");
     auto& c = ctx->ctx;
-    std::vector<int> h(ev2v, ev2v + std::size_t(nelems) * (dim + 1));
+    std::vector<int> h = host_copy(c, ev2v, std::size_t(nelems) * (dim + 1));
     for (int v : h) OSH_CHECK(v >= 0 && v < nverts, "vertex index out of range");
     auto m = std::make_unique<osh_mesh>();
     m->owner = ctx;
@@ -401,7 +433,7 @@ int osh_mesh_get_verts(osh_mesh* mesh, int d, int* out) {
     check_dim(mesh, d);
     auto& c = mesh->owner->ctx;
     if (d == 0) {
-      for (int i = 0; i < mesh->m.nents[0]; ++i) out[i] = i;
+      download(c, osh::iota(c, mesh->m.nents[0]), out);
       return;
     }
     download(c, osh::verts_of(c, mesh->m, d), out);
@@ -547,14 +579,15 @@ int osh_mesh_set_class(osh_mesh* mesh, int d, const int* ids, const signed char*
     OSH_CHECK(ids && dims, "null argument");
     int dim = mesh->m.dim;
     osh::LO n = mesh->m.nents[d];
-    for (osh::LO i = 0; i < n; ++i)
-      OSH_CHECK(dims[i] >= d && dims[i] <= dim,
-          "an entity classified on a model entity of lower dimension than itself");
     auto& c = mesh->owner->ctx;
+    std::vector<signed char> hd = host_copy(c, dims, std::size_t(n));
+    for (osh::LO i = 0; i < n; ++i)
+      OSH_CHECK(hd[std::size_t(i)] >= d && hd[std::size_t(i)] <= dim,
+          "an entity classified on a model entity of lower dimension than itself");
     osh::LOs id(c, n);
     osh::Bytes cd(c, n);
     osh::upload(c, id, ids, n);
-    osh::upload(c, cd, reinterpret_cast<const osh::I8*>(dims), n);
+    osh::upload(c, cd, reinterpret_cast<const osh::I8*>(hd.data()), n);
     c.sync();
     mesh->m.class_id[d] = id;
     mesh->m.class_dim[d] = cd;
@@ -750,6 +783,81 @@ int osh_ctx_get_tuning(osh_ctx* ctx, const char* name, int* out) {
   return guarded([&] {
     OSH_CHECK(ctx && name && out, "null argument");
     *out = *tuning_field(ctx->ctx.tuning, name);
+  });
+}
+
+int osh_device_alloc(osh_ctx* ctx, long long bytes, void** out) {
+  return guarded([&] {
+    OSH_CHECK(ctx && out, "null argument");
+    OSH_CHECK(bytes >= 0, "negative size");
+    *out = bytes ? ctx->ctx.pool->get(std::size_t(bytes)) : nullptr;
+  });
+}
+
+int osh_device_free(osh_ctx* ctx, void* p, long long bytes) {
+  return guarded([&] {
+    OSH_CHECK(ctx, "null argument");
+    if (p) ctx->ctx.pool->put(p, std::size_t(bytes));
+  });
+}
+
+int osh_copy(osh_ctx* ctx, void* dst, const void* src, long long bytes) {
+  return guarded([&] {
+    OSH_CHECK(ctx && ((dst && src) || bytes == 0), "null argument");
+    OSH_CHECK(bytes >= 0, "negative size");
+    if (bytes)
+      OSH_HIP(hipMemcpyAsync(dst, src, std::size_t(bytes), hipMemcpyDefault, ctx->ctx.stream));
+    ctx->ctx.sync();
+  });
+}
+
+int osh_device_fill(osh_ctx* ctx, void* dst, long long n, int item_bytes, const void* value) {
+  return guarded([&] {
+    OSH_CHECK(ctx && value && (dst || n == 0), "null argument");
+    OSH_CHECK(n >= 0, "negative count");
+    OSH_CHECK(item_bytes == 1 || item_bytes == 4 || item_bytes == 8, "items of 1, 4 or 8 bytes");
+    if (n == 0) return;
+    if (item_bytes == 1)
+      fill_device<unsigned char>(ctx->ctx, dst, n, value);
+    else if (item_bytes == 4)
+      fill_device<unsigned>(ctx->ctx, dst, n, value);
+    else
+      fill_device<unsigned long long>(ctx->ctx, dst, n, value);
+    ctx->ctx.sync();
+  });
+}
+
+int osh_pointer_is_device(const void* p, int* out) {
+  return guarded([&] {
+    OSH_CHECK(out, "null argument");
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      *out = 0;
+      return;
+    }
+    *out = a.type == hipMemoryTypeDevice ? 1 : 0;
+  });
+}
+
+int osh_mesh_min_quality(osh_mesh* mesh, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(out, "null argument");
+    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
+    auto& c = mesh->owner->ctx;
+    *out = osh::min_real(c, osh::ask_qualities(c, mesh->m));
+  });
+}
+
+int osh_mesh_max_length(osh_mesh* mesh, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(out, "null argument");
+    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
+    auto& c = mesh->owner->ctx;
+    *out = osh::max_real(c, osh::ask_lengths(c, mesh->m));
   });
 }
 
@@ -1170,7 +1278,8 @@ int osh_interpolate_metrics(osh_ctx* ctx, int dim, int n, const double* a, const
     OSH_CHECK(n >= 0, "negative count");
     auto& c = ctx->ctx;
     osh::LO nc = (dim * (dim + 1)) / 2;
-    std::vector<double> ha(a, a + std::size_t(n) * nc), hb(b, b + std::size_t(n) * nc);
+    std::vector<double> ha = host_copy(c, a, std::size_t(n) * nc);
+    std::vector<double> hb = host_copy(c, b, std::size_t(n) * nc);
     osh::Reals r;
     osh::interpolate_metric_arrays(c, dim, n, osh::to_device(c, ha), osh::to_device(c, hb), t, r);
     download(c, r, out);
@@ -1185,7 +1294,8 @@ int osh_intersect_metrics(osh_ctx* ctx, int dim, int n, const double* m1, const 
     OSH_CHECK(n >= 0, "negative count");
     auto& c = ctx->ctx;
     osh::LO nc = (dim * (dim + 1)) / 2;
-    std::vector<double> ha(m1, m1 + std::size_t(n) * nc), hb(m2, m2 + std::size_t(n) * nc);
+    std::vector<double> ha = host_copy(c, m1, std::size_t(n) * nc);
+    std::vector<double> hb = host_copy(c, m2, std::size_t(n) * nc);
     osh::Reals r;
     osh::intersect_metric_arrays(c, dim, n, osh::to_device(c, ha), osh::to_device(c, hb), r);
     download(c, r, out);
@@ -1197,7 +1307,7 @@ int osh_refine_qualities(osh_mesh* mesh, int n, const int* cands, double* out) {
     check_mesh(mesh);
     OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
     auto& c = mesh->owner->ctx;
-    std::vector<int> h(cands, cands + n);
+    std::vector<int> h = host_copy(c, cands, std::size_t(n));
     for (int e : h) OSH_CHECK(e >= 0 && e < mesh->m.nents[1], "edge out of range");
     auto q = osh::refine_qualities(c, mesh->m, osh::to_device(c, h));
     download(c, q, out);

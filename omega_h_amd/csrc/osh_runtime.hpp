@@ -192,9 +192,10 @@ using LOs = DArray<LO>;
 using Bytes = DArray<I8>;
 using Reals = DArray<Real>;
 
+// `h` may be host memory or any device memory (the runtime tells them apart)
 template <class T>
 inline void upload(Ctx& c, DArray<T>& d, T const* h, LO n) {
-  if (n) OSH_HIP(hipMemcpyAsync(d.data(), h, sizeof(T) * size_t(n), hipMemcpyHostToDevice, c.stream));
+  if (n) OSH_HIP(hipMemcpyAsync(d.data(), h, sizeof(T) * size_t(n), hipMemcpyDefault, c.stream));
 }
 
 template <class T>

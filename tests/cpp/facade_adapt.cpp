@@ -27,6 +27,21 @@ static void write_file(std::string const& path, T const* data, std::size_t n) {
   if (!f) throw std::runtime_error("cannot write " + path);
 }
 
+// a host vector as a device array, built the way a caller builds one
+// (HostWrite, then write())
+static Omega_h::Reals to_reals(std::vector<double> const& v) {
+  Omega_h::HostWrite<Omega_h::Real> h(Omega_h::LO(v.size()));
+  for (std::size_t i = 0; i < v.size(); ++i) h[Omega_h::LO(i)] = v[i];
+  return Omega_h::Reals(h.write());
+}
+
+// a device array written through its host mirror
+template <class T>
+static void write_read(std::string const& path, Omega_h::Read<T> const& r) {
+  Omega_h::HostRead<T> h(r);
+  write_file(path, h.data(), std::size_t(h.size()));
+}
+
 int main(int argc, char** argv) {
   if (argc != 6 && argc != 7) {
     std::cerr << "usage: facade_adapt <dim> <n> <ncomps> <metric.f64> <out_prefix> [every]\n";
@@ -46,7 +61,7 @@ int main(int argc, char** argv) {
     f.read(reinterpret_cast<char*>(metric.data()), std::streamsize(metric.size() * 8));
     if (!f) throw std::runtime_error("cannot read the metric file");
     mesh.add_tag(Omega_h::VERT, "metric", ncomps,
-        Omega_h::Reals(Omega_h::Write<Omega_h::Real>(metric)));
+        to_reals(metric));
     Omega_h::AdaptOpts opts(&mesh);
     opts.verbosity = Omega_h::SILENT;
     if (every) {
@@ -79,11 +94,11 @@ int main(int argc, char** argv) {
     counts.push_back(did ? 1 : 0);
     write_file(prefix + ".counts", counts.data(), counts.size());
     auto ev2v = mesh.ask_elem_verts();
-    write_file(prefix + ".verts", ev2v.data(), std::size_t(ev2v.size()));
+    write_read(prefix + ".verts", ev2v);
     auto coords = mesh.coords();
-    write_file(prefix + ".coords", coords.data(), std::size_t(coords.size()));
+    write_read(prefix + ".coords", coords);
     auto q = mesh.get_array<Omega_h::Real>(dim, "quality");
-    write_file(prefix + ".quality", q.data(), std::size_t(q.size()));
+    write_read(prefix + ".quality", q);
     std::printf("adapted: %d elements, min quality %.6f\n", mesh.nelems(),
         mesh.min_quality());
   } catch (Omega_h::exception const& e) {

@@ -32,6 +32,21 @@ static void write_file(std::string const& path, T const* data, std::size_t n) {
   if (!f) throw std::runtime_error("cannot write " + path);
 }
 
+// a host vector as a device array, built the way a caller builds one
+// (HostWrite, then write())
+static Omega_h::Reals to_reals(std::vector<double> const& v) {
+  Omega_h::HostWrite<Omega_h::Real> h(Omega_h::LO(v.size()));
+  for (std::size_t i = 0; i < v.size(); ++i) h[Omega_h::LO(i)] = v[i];
+  return Omega_h::Reals(h.write());
+}
+
+// a device array written through its host mirror
+template <class T>
+static void write_read(std::string const& path, Omega_h::Read<T> const& r) {
+  Omega_h::HostRead<T> h(r);
+  write_file(path, h.data(), std::size_t(h.size()));
+}
+
 int main(int argc, char** argv) {
   if (argc != 7) {
     std::cerr << "usage: facade_approach <dim> <n> <ncomps> <metric.f64> <target.f64> <out>\n";
@@ -47,9 +62,9 @@ int main(int argc, char** argv) {
         dim == 3 ? 1.0 : 0.0, n, n, dim == 3 ? n : 0);
     std::size_t nv = std::size_t(mesh.nverts());
     mesh.add_tag(Omega_h::VERT, "metric", ncomps,
-        Omega_h::Reals(Omega_h::Write<Omega_h::Real>(read_doubles(argv[4], nv * ncomps))));
+        to_reals(read_doubles(argv[4], nv * ncomps)));
     mesh.add_tag(Omega_h::VERT, "target_metric", ncomps,
-        Omega_h::Reals(Omega_h::Write<Omega_h::Real>(read_doubles(argv[5], nv * ncomps))));
+        to_reals(read_doubles(argv[5], nv * ncomps)));
     Omega_h::AdaptOpts opts(&mesh);
     opts.verbosity = Omega_h::SILENT;
     int steps = 0;
@@ -62,11 +77,11 @@ int main(int argc, char** argv) {
     counts.push_back(steps);
     write_file(prefix + ".counts", counts.data(), counts.size());
     auto ev2v = mesh.ask_elem_verts();
-    write_file(prefix + ".verts", ev2v.data(), std::size_t(ev2v.size()));
+    write_read(prefix + ".verts", ev2v);
     auto coords = mesh.coords();
-    write_file(prefix + ".coords", coords.data(), std::size_t(coords.size()));
+    write_read(prefix + ".coords", coords);
     auto metric = mesh.get_array<Omega_h::Real>(Omega_h::VERT, "metric");
-    write_file(prefix + ".metric", metric.data(), std::size_t(metric.size()));
+    write_read(prefix + ".metric", metric);
     std::printf("approached in %d steps: %d elements\n", steps, mesh.nelems());
   } catch (Omega_h::exception const& e) {
     std::cerr << "Omega_h::exception: " << e.what() << "\n";

@@ -32,6 +32,21 @@ static void write_file(std::string const& path, T const* data, std::size_t n) {
   if (!f) throw std::runtime_error("cannot write " + path);
 }
 
+// a host vector as a device array, built the way a caller builds one
+// (HostWrite, then write())
+static Omega_h::Reals to_reals(std::vector<double> const& v) {
+  Omega_h::HostWrite<Omega_h::Real> h(Omega_h::LO(v.size()));
+  for (std::size_t i = 0; i < v.size(); ++i) h[Omega_h::LO(i)] = v[i];
+  return Omega_h::Reals(h.write());
+}
+
+// a device array written through its host mirror
+template <class T>
+static void write_read(std::string const& path, Omega_h::Read<T> const& r) {
+  Omega_h::HostRead<T> h(r);
+  write_file(path, h.data(), std::size_t(h.size()));
+}
+
 int main(int argc, char** argv) {
   if (argc != 5) {
     std::cerr << "usage: facade_conserve <dim> <n> <metric.f64> <out>\n";
@@ -46,23 +61,23 @@ int main(int argc, char** argv) {
     Mesh mesh = build_box(lib.world(), OMEGA_H_SIMPLEX, 1.0, 1.0, dim == 3 ? 1.0 : 0.0, n, n,
         dim == 3 ? n : 0);
     std::size_t nv = std::size_t(mesh.nverts());
-    mesh.add_tag(VERT, "metric", 1, Reals(Write<Real>(read_doubles(argv[3], nv))));
-    auto coords = mesh.coords();
-    auto ev2v = mesh.ask_elem_verts();
-    Write<Real> density(mesh.nelems());
+    mesh.add_tag(VERT, "metric", 1, to_reals(read_doubles(argv[3], nv)));
+    HostRead<Real> coords(mesh.coords());
+    HostRead<LO> ev2v(mesh.ask_elem_verts());
+    HostWrite<Real> density(mesh.nelems());
     for (LO e = 0; e < mesh.nelems(); ++e) {
       Real y = 0;
       for (int k = 0; k <= dim; ++k) y += coords[ev2v[e * (dim + 1) + k] * dim + 1];
       density[e] = (y / (dim + 1) > 0.5) ? 1e-4 : 1.0;
     }
-    mesh.add_tag(dim, "density", 1, Reals(density));
-    Write<Real> velocity(mesh.nverts() * dim);
+    mesh.add_tag(dim, "density", 1, Reals(density.write()));
+    HostWrite<Real> velocity(mesh.nverts() * dim);
     for (LO v = 0; v < mesh.nverts(); ++v) {
       Real x2 = coords[v * dim] - 0.3;
       Real w = (x2 > 0 ? 1.0 : (x2 < 0 ? -1.0 : 0.0)) * std::sqrt(std::abs(x2));
       for (int k = 0; k < dim; ++k) velocity[v * dim + k] = (k == 0) ? w : 0.0;
     }
-    mesh.add_tag(VERT, "velocity", dim, Reals(velocity));
+    mesh.add_tag(VERT, "velocity", dim, Reals(velocity.write()));
     AdaptOpts opts(&mesh);
     opts.verbosity = SILENT;
     opts.xfer_opts.type_map["density"] = OMEGA_H_CONSERVE;
@@ -79,13 +94,13 @@ int main(int argc, char** argv) {
     for (int d = 0; d <= dim; ++d) counts.push_back(mesh.nents(d));
     write_file(prefix + ".counts", counts.data(), counts.size());
     auto verts = mesh.ask_elem_verts();
-    write_file(prefix + ".verts", verts.data(), std::size_t(verts.size()));
+    write_read(prefix + ".verts", verts);
     auto x = mesh.coords();
-    write_file(prefix + ".coords", x.data(), std::size_t(x.size()));
+    write_read(prefix + ".coords", x);
     auto rho = mesh.get_array<Real>(dim, "density");
-    write_file(prefix + ".density", rho.data(), std::size_t(rho.size()));
+    write_read(prefix + ".density", rho);
     auto vel = mesh.get_array<Real>(VERT, "velocity");
-    write_file(prefix + ".velocity", vel.data(), std::size_t(vel.size()));
+    write_read(prefix + ".velocity", vel);
     std::printf("adapted to %d elements\n", mesh.nelems());
   } catch (Omega_h::exception const& e) {
     std::cerr << "Omega_h::exception: " << e.what() << "\n";

@@ -33,6 +33,21 @@ static void write_file(std::string const& path, T const* data, std::size_t n) {
   if (!f) throw std::runtime_error("cannot write " + path);
 }
 
+// a host vector as a device array, built the way a caller builds one
+// (HostWrite, then write())
+static Omega_h::Reals to_reals(std::vector<double> const& v) {
+  Omega_h::HostWrite<Omega_h::Real> h(Omega_h::LO(v.size()));
+  for (std::size_t i = 0; i < v.size(); ++i) h[Omega_h::LO(i)] = v[i];
+  return Omega_h::Reals(h.write());
+}
+
+// a device array written through its host mirror
+template <class T>
+static void write_read(std::string const& path, Omega_h::Read<T> const& r) {
+  Omega_h::HostRead<T> h(r);
+  write_file(path, h.data(), std::size_t(h.size()));
+}
+
 int main(int argc, char** argv) {
   if (argc != 7) {
     std::cerr << "usage: facade_warp <dim> <n> <ncomps> <metric.f64> <warp.f64> <out>\n";
@@ -48,9 +63,9 @@ int main(int argc, char** argv) {
         dim == 3 ? 1.0 : 0.0, n, n, dim == 3 ? n : 0);
     std::size_t nv = std::size_t(mesh.nverts());
     mesh.add_tag(Omega_h::VERT, "metric", ncomps,
-        Omega_h::Reals(Omega_h::Write<Omega_h::Real>(read_doubles(argv[4], nv * ncomps))));
+        to_reals(read_doubles(argv[4], nv * ncomps)));
     mesh.add_tag(Omega_h::VERT, "warp", dim,
-        Omega_h::Reals(Omega_h::Write<Omega_h::Real>(read_doubles(argv[5], nv * dim))));
+        to_reals(read_doubles(argv[5], nv * dim)));
     mesh.set_parting(OMEGA_H_GHOSTED);
     Omega_h::AdaptOpts opts(&mesh);
     opts.verbosity = Omega_h::SILENT;
@@ -64,14 +79,15 @@ int main(int argc, char** argv) {
     counts.push_back(steps);
     write_file(prefix + ".counts", counts.data(), counts.size());
     auto ev2v = mesh.ask_elem_verts();
-    write_file(prefix + ".verts", ev2v.data(), std::size_t(ev2v.size()));
+    write_read(prefix + ".verts", ev2v);
     auto coords = mesh.coords();
-    write_file(prefix + ".coords", coords.data(), std::size_t(coords.size()));
+    write_read(prefix + ".coords", coords);
     auto sizes = mesh.ask_sizes();
-    write_file(prefix + ".sizes", sizes.data(), std::size_t(sizes.size()));
+    write_read(prefix + ".sizes", sizes);
     auto dual = mesh.ask_dual();
-    std::vector<int> d(dual.a2ab.data(), dual.a2ab.data() + dual.a2ab.size());
-    d.insert(d.end(), dual.ab2b.data(), dual.ab2b.data() + dual.ab2b.size());
+    Omega_h::HostRead<Omega_h::LO> a2ab(dual.a2ab), ab2b(dual.ab2b);
+    std::vector<int> d(a2ab.data(), a2ab.data() + a2ab.size());
+    d.insert(d.end(), ab2b.data(), ab2b.data() + ab2b.size());
     write_file(prefix + ".dual", d.data(), d.size());
     if (mesh.comm()->rank() == 0)
       std::printf("warped in %d steps: %d elements\n", steps, mesh.nelems());

@@ -1,8 +1,9 @@
 """The C++ drop-in facade (include/Omega_h_amd.hpp).
 
-CPU: the header compiles on its own.  GPU: a C++ program written the way an
-Omega_h caller writes adapt() (tests/cpp/facade_adapt.cpp, built by
-__graft_entry__.build()) gives bit-for-bit the oracle's adapted mesh.
+CPU: the header compiles on its own.  GPU: C++ programs written the way an
+Omega_h caller writes adapt() (tests/cpp/facade_*.cpp, built by
+__graft_entry__.build()) give bit-for-bit the oracle's adapted mesh, and the
+facade's arrays live on the device (tests/cpp/facade_device.cpp).
 """
 import os
 import subprocess
@@ -170,3 +171,17 @@ def test_facade_conserve_matches_oracle(tmp_path, dim, n):
                        ("velocity", o.vertex_tag("velocity"))):
         got = np.fromfile(prefix + "." + name, dtype=np.float64)
         assert np.array_equal(got.view(np.int64), want.view(np.int64)), name
+
+
+@pytest.mark.gpu
+def test_facade_arrays_live_on_the_device():
+    """Read/Write of the facade are device memory: tags round-trip through
+    device-to-device copies, fills, ranges, single items and shared storage
+    behave as the reference's arrays, device reductions equal host ones, and
+    an array outlives its Library (tests/cpp/facade_device.cpp)."""
+    prog = os.path.join(ROOT, "omega_h_amd", "_build", "facade_device")
+    assert os.path.exists(prog), "build the facade programs with __graft_entry__.build()"
+    print("facade device arrays", flush=True)
+    r = subprocess.run([prog], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("ok"), r.stdout
