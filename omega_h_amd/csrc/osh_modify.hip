@@ -21,20 +21,36 @@ namespace {
 // the lowest index among the entities above it, which is the first entry of
 // its row of the upward adjacency (rows are sorted by index), without
 // building that adjacency
-__global__ void mark_same_up_kernel(LO nh, int deg, LO const* hl2l, I8 const* low_marks,
-    I8* same, LO* counts, LO* first_up) {
-  LO h = thread_index();
-  if (h >= nh) return;
+// the dimensions above the keys in one launch: job w owns the threads
+// [start[w], start[w + 1])
+struct SameUpJobs {
+  LO start[4];
+  int deg[3];
+  LO const* hl2l[3];
+  I8* same[3];
+  LO* counts[3];
+  LO* first_up[3];
+  int n;
+};
+
+__global__ void mark_same_up_kernel(LO nt, I8 const* low_marks, SameUpJobs j) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  int w = 0;
+  while (w + 1 < j.n && t >= j.start[w + 1]) ++w;
+  LO h = t - j.start[w];
+  int deg = j.deg[w];
+  LO const* hl2l = j.hl2l[w];
   I8 r = 1;
-  for (int j = 0; j < deg; ++j) {
-    LO l = hl2l[h * deg + j];
+  for (int k = 0; k < deg; ++k) {
+    LO l = hl2l[h * deg + k];
     if (low_marks[l]) {
       r = 0;
-      atomicMin(&first_up[l], h);
+      atomicMin(&j.first_up[w][l], h);
     }
   }
-  same[h] = r;
-  counts[h] = r;
+  j.same[w][h] = r;
+  j.counts[w][h] = r;
 }
 
 // new numbers of the old entities in one sweep: same entities take their
@@ -43,31 +59,46 @@ __global__ void mark_same_up_kernel(LO nh, int deg, LO const* hl2l, I8 const* lo
 // their connectivity renumbered (modify_conn, :446-466; a null
 // old_lows2new_lows keeps the lower numbers) and, when `verts` is given,
 // their vertex list renumbered
-__global__ void number_same(LO n, I8 const* same, LO const* same_offsets, LO const* offsets,
-    LO* same2old, LO* same2new, LO* old2new, int deg, LO const* old_down, I8 const* old_codes,
-    LO const* old_lows2new_lows, LO* down, I8* codes, int nvpe, LO const* old_verts,
-    LO const* verts_old2new, LO* verts) {
-  LO o = thread_index();
-  if (o >= n) return;
-  if (same && !same[o]) {
-    old2new[o] = -1;
+struct SameNumbering {
+  I8 const* same;
+  LO const* same_offsets;
+  LO const* offsets;
+  LO* same2old;
+  LO* same2new;
+  LO* old2new;
+  int deg;
+  LO const* old_down;
+  I8 const* old_codes;
+  LO const* old_lows2new_lows;
+  LO* down;
+  I8* codes;
+  int nvpe;
+  LO const* old_verts;
+  LO const* verts_old2new;
+  LO* verts;
+};
+
+__device__ __forceinline__ void number_same(LO o, SameNumbering const& a) {
+  if (a.same && !a.same[o]) {
+    a.old2new[o] = -1;
     return;
   }
-  LO s = same ? same_offsets[o] : o;
-  LO w = offsets[o];
-  same2old[s] = o;
-  same2new[s] = w;
-  old2new[o] = w;
-  if (down)
+  LO s = a.same ? a.same_offsets[o] : o;
+  LO w = a.offsets[o];
+  a.same2old[s] = o;
+  a.same2new[s] = w;
+  a.old2new[o] = w;
+  int deg = a.deg;
+  if (a.down)
     for (int j = 0; j < deg; ++j) {
-      LO l = old_down[o * deg + j];
-      down[w * deg + j] = old_lows2new_lows ? old_lows2new_lows[l] : l;
-      if (codes) codes[w * deg + j] = old_codes[o * deg + j];
+      LO l = a.old_down[o * deg + j];
+      a.down[w * deg + j] = a.old_lows2new_lows ? a.old_lows2new_lows[l] : l;
+      if (a.codes) a.codes[w * deg + j] = a.old_codes[o * deg + j];
     }
-  if (verts)
-    for (int k = 0; k < nvpe; ++k) {
-      LO v = old_verts[o * nvpe + k];
-      verts[w * nvpe + k] = verts_old2new ? verts_old2new[v] : v;
+  if (a.verts)
+    for (int k = 0; k < a.nvpe; ++k) {
+      LO v = a.old_verts[o * a.nvpe + k];
+      a.verts[w * a.nvpe + k] = a.verts_old2new ? a.verts_old2new[v] : v;
     }
 }
 
@@ -79,6 +110,10 @@ struct KeyDimInit {
   LO* first_up[4];  // per dimension above the keys (null elsewhere)
   I8* same;         // when the key dimension is rebuilt
   LO* counts;
+  // with a key mask, the keys are marked in the same sweep: no longer the
+  // same, each counting its products
+  I8 const* key_mask;
+  LO const* key_prod_counts;
 };
 
 __global__ void init_key_dim(LO n, KeyDimInit a) {
@@ -88,8 +123,9 @@ __global__ void init_key_dim(LO n, KeyDimInit a) {
   for (int d = 0; d < 4; ++d)
     if (a.first_up[d]) a.first_up[d][i] = 0x7fffffff;
   if (a.same) {
-    a.same[i] = 1;
-    a.counts[i] = 1;
+    bool key = a.key_mask && a.key_mask[i];
+    a.same[i] = key ? 0 : 1;
+    a.counts[i] = key ? a.key_prod_counts[i] : 1;
   }
 }
 
@@ -153,14 +189,6 @@ struct MaskedReps {
   LO* counts[3];
 };
 
-__global__ void mark_keys_masked(LO n, I8 const* key_mask, LO const* prod_counts, I8* same,
-    LO* counts) {
-  LO i = thread_index();
-  if (i >= n || !key_mask[i]) return;
-  same[i] = 0;
-  counts[i] = prod_counts[i];
-}
-
 __global__ void reps_masked(LO nt, LO nkd, I8 const* key_mask, MaskedReps j, int* error) {
   LO t = thread_index();
   if (t >= nt) return;
@@ -203,35 +231,62 @@ __global__ void rep2md_order_kernel(LO nreps, LO const* a2ab, LO const* ab2b, I8
 // share its products without talking to each other.
 constexpr int kProdLanes = 8;
 
-__global__ void prod_numbers(LO nt, LO const* __restrict__ reps,
-    LO const* __restrict__ keys2kds, LO const* __restrict__ offsets, LO const* __restrict__ order,
-    LO const* __restrict__ keys2prods, LO* __restrict__ prods2new, int deg,
-    LO const* __restrict__ pdown, I8 const* __restrict__ pcodes, LO const* __restrict__ found,
-    LO* __restrict__ down, I8* __restrict__ codes, int nvpe, LO const* __restrict__ pverts,
-    LO* __restrict__ verts, LO const* __restrict__ key_src, LO* __restrict__ src, int* error) {
-  LO t = thread_index();
-  if (t >= nt) return;
+struct ProdNumbering {
+  LO const* reps;
+  LO const* keys2kds;
+  LO const* offsets;
+  LO const* order;
+  LO const* keys2prods;
+  LO* prods2new;
+  int deg;
+  LO const* pdown;
+  I8 const* pcodes;
+  LO const* found;
+  LO* down;
+  I8* codes;
+  int nvpe;
+  LO const* pverts;
+  LO* verts;
+  LO const* key_src;
+  LO* src;
+  int* error;
+};
+
+__device__ __forceinline__ void number_prods(LO t, ProdNumbering const& a) {
   LO k = t / kProdLanes;
   int lane = t % kProdLanes;
-  LO first = keys2prods[k];
-  LO base = offsets[reps[k]];
-  if (order) base += order[keys2kds[k]] + 1;
-  for (LO p = first + lane; p < keys2prods[k + 1]; p += kProdLanes) {
+  LO first = a.keys2prods[k];
+  LO base = a.offsets[a.reps[k]];
+  if (a.order) base += a.order[a.keys2kds[k]] + 1;
+  int deg = a.deg;
+  for (LO p = first + lane; p < a.keys2prods[k + 1]; p += kProdLanes) {
     LO w = base + (p - first);
-    prods2new[p] = w;
-    if (down)
+    a.prods2new[p] = w;
+    if (a.down)
       for (int j = 0; j < deg; ++j) {
         // a side matched by reflect_down must have matched exactly once; an
         // error keeps the index in range until the host sees it
-        bool ok = !found || found[p * deg + j] == 1;
-        if (!ok) *error = kReflectDownNoMatch;
-        down[w * deg + j] = ok ? pdown[p * deg + j] : 0;
-        if (codes) codes[w * deg + j] = ok ? pcodes[p * deg + j] : I8(0);
+        bool ok = !a.found || a.found[p * deg + j] == 1;
+        if (!ok) *a.error = kReflectDownNoMatch;
+        a.down[w * deg + j] = ok ? a.pdown[p * deg + j] : 0;
+        if (a.codes) a.codes[w * deg + j] = ok ? a.pcodes[p * deg + j] : I8(0);
       }
-    if (verts)
-      for (int j = 0; j < nvpe; ++j) verts[w * nvpe + j] = pverts[p * nvpe + j];
-    if (src) src[p] = key_src[k];
+    if (a.verts)
+      for (int j = 0; j < a.nvpe; ++j) a.verts[w * a.nvpe + j] = a.pverts[p * a.nvpe + j];
+    if (a.src) a.src[p] = a.key_src[k];
   }
+}
+
+// the new numbers of one dimension in one launch: threads [0, nold) number
+// the old entities (same ones keep their order), the rest (kProdLanes per
+// key) the products; the two halves write disjoint entries
+__global__ void number_entities(LO n, LO nold, SameNumbering same, ProdNumbering prods) {
+  LO t = thread_index();
+  if (t >= n) return;
+  if (t < nold)
+    number_same(t, same);
+  else
+    number_prods(t - nold, prods);
 }
 
 // one launch for a transferred tag: threads [0, nsame) copy same entities,
@@ -613,13 +668,17 @@ void modify_plans(Ctx& c, DMesh& old, int key_dim, int d0, int d1, LOs const& ke
   }
   RepsJob jobs{};
   int njobs = 0;
+  SameUpJobs up{};
   for (int d = d0; d <= d1; ++d) {
     ModPlan& plan = plans[d - d0];
     if (d > key_dim) {
-      int deg = simplex_degree(d, key_dim);
-      LOs const& hl2l = ask_down(c, old, d, key_dim, nullptr);
-      OSH_LAUNCH(mark_same_up_kernel, plan.nold, c, deg, hl2l.data(), key_marks.data(),
-          plan.same.data(), plan.counts.data(), first_up[d].data());
+      up.deg[up.n] = simplex_degree(d, key_dim);
+      up.hl2l[up.n] = ask_down(c, old, d, key_dim, nullptr).data();
+      up.same[up.n] = plan.same.data();
+      up.counts[up.n] = plan.counts.data();
+      up.first_up[up.n] = first_up[d].data();
+      up.start[up.n + 1] = up.start[up.n] + plan.nold;
+      ++up.n;
       jobs.first_up[njobs] = first_up[d].data();
     } else if (d < key_dim) {
       plan.counts = fill_lo(c, plan.nold, 1);
@@ -634,6 +693,7 @@ void modify_plans(Ctx& c, DMesh& old, int key_dim, int d0, int d1, LOs const& ke
     ++njobs;
   }
   OSH_CHECK(njobs <= 3, "modify_plans: too many dimensions");
+  if (up.n) OSH_LAUNCH(mark_same_up_kernel, up.start[up.n], c, key_marks.data(), up);
   if (njobs) OSH_LAUNCH(reps_kernel, njobs * nkeys, c, nkeys, keys2kds.data(), jobs,
       c.error_word());
 }
@@ -658,16 +718,21 @@ void modify_plans_masked(Ctx& c, DMesh& old, int key_dim, int d1, Bytes const& k
   }
   init.same = plans[0].same.data();
   init.counts = plans[0].counts.data();
+  init.key_mask = key_mask.data();
+  init.key_prod_counts = prod_counts[key_dim];
   OSH_LAUNCH(init_key_dim, nkd, c, init);
-  OSH_LAUNCH(mark_keys_masked, nkd, c, key_mask.data(), prod_counts[key_dim],
-      plans[0].same.data(), plans[0].counts.data());
   MaskedReps j{};
   int nj = 0;
+  SameUpJobs up{};
   for (int d = key_dim + 1; d <= d1; ++d) {
     ModPlan& plan = plans[d - key_dim];
-    LOs const& hl2l = ask_down(c, old, d, key_dim, nullptr);
-    OSH_LAUNCH(mark_same_up_kernel, plan.nold, c, simplex_degree(d, key_dim), hl2l.data(),
-        key_mask.data(), plan.same.data(), plan.counts.data(), first_up[d].data());
+    up.deg[up.n] = simplex_degree(d, key_dim);
+    up.hl2l[up.n] = ask_down(c, old, d, key_dim, nullptr).data();
+    up.same[up.n] = plan.same.data();
+    up.counts[up.n] = plan.counts.data();
+    up.first_up[up.n] = first_up[d].data();
+    up.start[up.n + 1] = up.start[up.n] + plan.nold;
+    ++up.n;
     j.first_up[nj] = first_up[d].data();
     j.prod_counts[nj] = prod_counts[d];
     j.rep_of[nj] = rep_of[d].data();
@@ -675,6 +740,7 @@ void modify_plans_masked(Ctx& c, DMesh& old, int key_dim, int d1, Bytes const& k
     ++nj;
   }
   OSH_CHECK(nj <= 3, "modify_plans_masked: too many dimensions");
+  if (up.n) OSH_LAUNCH(mark_same_up_kernel, up.start[up.n], c, key_mask.data(), up);
   if (nj) OSH_LAUNCH(reps_masked, nj * nkd, c, nkd, key_mask.data(), j, c.error_word());
 }
 
@@ -765,26 +831,44 @@ void modify_build(Ctx& c, DMesh& old, DMesh& nw, ModPlan const& plan, LOs const&
       pdown = prod_verts;
     }
   }
-  OSH_LAUNCH(number_same, nold, c, same.exists() ? same.data() : (I8 const*)nullptr,
-      same.exists() ? plan.same_offsets.data() : (LO const*)nullptr, plan.offsets.data(),
-      out.same2old.data(), out.same2new.data(), out.old2new.data(), deg,
-      ent_dim >= 1 ? old.down[ent_dim].data() : (LO const*)nullptr,
-      with_codes ? old.down_codes[ent_dim].data() : (I8 const*)nullptr,
-      ent_dim >= 1 ? old_lows2new_lows.data() : (LO const*)nullptr,
-      ent_dim >= 1 ? down.data() : (LO*)nullptr, with_codes ? codes.data() : (I8*)nullptr, deg,
-      old_verts, verts_old2new ? verts_old2new->data() : (LO const*)nullptr,
-      verts.exists() ? verts.data() : (LO*)nullptr);
   if (prod_src) *prod_src = LOs(c, nprods);
-  OSH_LAUNCH(prod_numbers, nkeys * kProdLanes, c, plan.reps.data(), keys2kds.data(),
-      plan.offsets.data(),
-      order.exists() ? order.data() : (LO const*)nullptr, keys2prods.data(),
-      out.prods2new.data(), deg, ent_dim >= 1 ? pdown.data() : (LO const*)nullptr,
-      with_codes ? pcodes.data() : (I8 const*)nullptr,
-      with_codes ? found.data() : (LO const*)nullptr, ent_dim >= 1 ? down.data() : (LO*)nullptr,
-      with_codes ? codes.data() : (I8*)nullptr, deg,
-      verts.exists() ? prod_verts.data() : (LO const*)nullptr,
-      verts.exists() ? verts.data() : (LO*)nullptr, keys2kds.data(),
-      prod_src ? prod_src->data() : (LO*)nullptr, c.error_word());
+  SameNumbering sn;
+  sn.same = same.exists() ? same.data() : (I8 const*)nullptr;
+  sn.same_offsets = same.exists() ? plan.same_offsets.data() : (LO const*)nullptr;
+  sn.offsets = plan.offsets.data();
+  sn.same2old = out.same2old.data();
+  sn.same2new = out.same2new.data();
+  sn.old2new = out.old2new.data();
+  sn.deg = deg;
+  sn.old_down = ent_dim >= 1 ? old.down[ent_dim].data() : (LO const*)nullptr;
+  sn.old_codes = with_codes ? old.down_codes[ent_dim].data() : (I8 const*)nullptr;
+  sn.old_lows2new_lows = ent_dim >= 1 ? old_lows2new_lows.data() : (LO const*)nullptr;
+  sn.down = ent_dim >= 1 ? down.data() : (LO*)nullptr;
+  sn.codes = with_codes ? codes.data() : (I8*)nullptr;
+  sn.nvpe = deg;
+  sn.old_verts = old_verts;
+  sn.verts_old2new = verts_old2new ? verts_old2new->data() : (LO const*)nullptr;
+  sn.verts = verts.exists() ? verts.data() : (LO*)nullptr;
+  ProdNumbering pn;
+  pn.reps = plan.reps.data();
+  pn.keys2kds = keys2kds.data();
+  pn.offsets = plan.offsets.data();
+  pn.order = order.exists() ? order.data() : (LO const*)nullptr;
+  pn.keys2prods = keys2prods.data();
+  pn.prods2new = out.prods2new.data();
+  pn.deg = deg;
+  pn.pdown = ent_dim >= 1 ? pdown.data() : (LO const*)nullptr;
+  pn.pcodes = with_codes ? pcodes.data() : (I8 const*)nullptr;
+  pn.found = with_codes ? found.data() : (LO const*)nullptr;
+  pn.down = ent_dim >= 1 ? down.data() : (LO*)nullptr;
+  pn.codes = with_codes ? codes.data() : (I8*)nullptr;
+  pn.nvpe = deg;
+  pn.pverts = verts.exists() ? prod_verts.data() : (LO const*)nullptr;
+  pn.verts = verts.exists() ? verts.data() : (LO*)nullptr;
+  pn.key_src = keys2kds.data();
+  pn.src = prod_src ? prod_src->data() : (LO*)nullptr;
+  pn.error = c.error_word();
+  OSH_LAUNCH(number_entities, nold + nkeys * kProdLanes, c, nold, sn, pn);
   nw.nents[ent_dim] = nnew;
   if (ent_dim == 0) return;
   nw.down[ent_dim] = down;

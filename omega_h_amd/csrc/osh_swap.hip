@@ -131,13 +131,13 @@ __device__ double length_between(LO a, LO b, Real const* coords, Real const* met
   return edge_length<S, M>(p0, p1, m0, m1);
 }
 
-// swap3d_qualities (src/Omega_h_swap3d_qualities.cpp:11-75) in five flat
+// swap3d_qualities (src/Omega_h_swap3d_qualities.cpp:11-75) in four flat
 // sweeps instead of one thread per candidate edge:
 //   swap3d_loops:     the loop of each candidate and its counts of unique
 //                     triangles and unique interior edges
-//   swap3d_tri_quals: one thread per (candidate, unique triangle): the min
-//                     quality of the two tets on that triangle
-//   swap3d_edge_lens: one thread per (candidate, unique edge): its length
+//   swap3d_loop_measures: one thread per (candidate, unique triangle): the
+//                     min quality of the two tets on that triangle, and one
+//                     per (candidate, unique edge): its length
 //   swap3d_mesh_quals: one thread per (candidate, triangulation): its value
 //                     for choose() of src/Omega_h_swap3d_choice.hpp
 //   swap3d_choose:    per candidate, the choice over those values in the
@@ -211,13 +211,30 @@ __global__ void swap3d_loops(LO nt, LO const* cands, LO const* e2t_a2ab, LO cons
 // F32 (mixed precision): FP32 estimates, and a candidate with an element
 // whose max-determinant metric is not clear is marked in `unsure`; with
 // `need` given, the FP64 sweep evaluates only the marked candidates
+// the loop measures of the candidates in one launch: threads [0, ntp) take
+// the (candidate, unique triangle) slots, the rest the (candidate, unique
+// edge) slots
+struct LoopMeasures {
+  LO ntp;
+  LO const* ntris;
+  LO const* nedges;
+  LO const* lsize;
+  LO const* leev;
+  LO const* lverts;
+  Real const* coords;
+  Real const* metric;
+  Tables const* tb;
+  I8 const* need;
+  I8* unsure;
+  Real* tq;
+  Real* el;
+};
+
 template <int M, bool F32>
-__global__ void swap3d_tri_quals(LO nt, LO const* ntris, LO const* lsize, LO const* leev,
-    LO const* lverts, Real const* coords, Real const* metric, Tables const* tb,
+__device__ __forceinline__ void swap3d_tri_qual(LO p, LO const* ntris, LO const* lsize,
+    LO const* leev, LO const* lverts, Real const* coords, Real const* metric, Tables const* tb,
     I8 const* need, I8* unsure, Real* tq) {
   // a fixed stride of kMaxUniqueTris slots per candidate
-  LO p = thread_index();
-  if (p >= nt) return;
   LO cand = p / swap::kMaxUniqueTris;
   int ut = p % swap::kMaxUniqueTris;
   if (ut >= ntris[cand]) return;
@@ -249,11 +266,10 @@ __global__ void swap3d_tri_quals(LO nt, LO const* ntris, LO const* lsize, LO con
 }
 
 template <int M, bool F32>
-__global__ void swap3d_edge_lens(LO nt, LO const* nedges, LO const* lsize, LO const* lverts,
-    Real const* coords, Real const* metric, Tables const* tb, I8 const* need, Real* el) {
+__device__ __forceinline__ void swap3d_edge_len(LO q, LO const* nedges, LO const* lsize,
+    LO const* lverts, Real const* coords, Real const* metric, Tables const* tb, I8 const* need,
+    Real* el) {
   // a fixed stride of kMaxUniqueEdges slots per candidate
-  LO q = thread_index();
-  if (q >= nt) return;
   LO cand = q / swap::kMaxUniqueEdges;
   int ue = q % swap::kMaxUniqueEdges;
   if (ue >= nedges[cand]) return;
@@ -265,6 +281,18 @@ __global__ void swap3d_edge_lens(LO nt, LO const* nedges, LO const* lsize, LO co
     el[q] = double(length_between32<3, M>(a, b, coords, metric));
   else
     el[q] = length_between<3, M>(a, b, coords, metric);
+}
+
+template <int M, bool F32>
+__global__ void swap3d_loop_measures(LO nt, LoopMeasures a) {
+  LO t = thread_index();
+  if (t >= nt) return;
+  if (t < a.ntp)
+    swap3d_tri_qual<M, F32>(t, a.ntris, a.lsize, a.leev, a.lverts, a.coords, a.metric, a.tb,
+        a.need, a.unsure, a.tq);
+  else
+    swap3d_edge_len<M, F32>(t - a.ntp, a.nedges, a.lsize, a.lverts, a.coords, a.metric, a.tb,
+        a.need, a.el);
 }
 
 // one thread per (candidate, triangulation) on a fixed stride of
@@ -512,11 +540,17 @@ __global__ void swap3d_mark_dirty(LO ntets, I8 const* tet_same, LO const* t2e, I
 }
 
 // the carried-over choices of the same edges whose tets did not change
-__global__ void swap3d_cache_transfer(LO nsame, LO const* same2old, LO const* same2new,
-    I8 const* dirty, I8 const* valid, I8 const* cfg, I8 const* keep, Real const* q,
-    I8* n_valid, I8* n_cfg, I8* n_keep, Real* n_q) {
+// threads [0, nsame) carry the same edges' choices; the rest mark the new
+// edges (products) as having none
+__global__ void swap3d_cache_transfer(LO n, LO nsame, LO const* same2old, LO const* same2new,
+    LO const* prods2new, I8 const* dirty, I8 const* valid, I8 const* cfg, I8 const* keep,
+    Real const* q, I8* n_valid, I8* n_cfg, I8* n_keep, Real* n_q) {
   LO i = thread_index();
-  if (i >= nsame) return;
+  if (i >= n) return;
+  if (i >= nsame) {
+    n_valid[prods2new[i - nsame]] = 0;
+    return;
+  }
   LO o = same2old[i], w = same2new[i];
   bool ok = valid[o] && !dirty[o];
   n_valid[w] = I8(ok);
@@ -792,14 +826,16 @@ void swap_rebuild(Ctx& c, DMesh& m, LOs const& keys2edges, LOs const* k2p, LO co
     transfer_class(c, m, nw, d, mo, src, 1, nullptr);
     if (d == 1 && cache) {
       SwapCache next;
-      next.valid = fill_i8(c, mo.nnew, 0);
+      next.valid = Bytes(c, mo.nnew);
       next.cfg = Bytes(c, mo.nnew);
       next.keep = Bytes(c, mo.nnew);
       next.q = Reals(c, mo.nnew);
       LO nsame = mo.same2old.size();
-      OSH_LAUNCH(swap3d_cache_transfer, nsame, c, mo.same2old.data(), mo.same2new.data(),
-          dirty->data(), cache->valid.data(), cache->cfg.data(), cache->keep.data(),
-          cache->q.data(), next.valid.data(), next.cfg.data(), next.keep.data(), next.q.data());
+      OSH_CHECK(nsame + mo.prods2new.size() == mo.nnew, "swap cache: same + new edges");
+      OSH_LAUNCH(swap3d_cache_transfer, mo.nnew, c, nsame, mo.same2old.data(),
+          mo.same2new.data(), mo.prods2new.data(), dirty->data(), cache->valid.data(),
+          cache->cfg.data(), cache->keep.data(), cache->q.data(), next.valid.data(),
+          next.cfg.data(), next.keep.data(), next.q.data());
       *cache = next;
     }
     if (d == dim) {
@@ -871,12 +907,10 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
     unsure = fill_i8(c, ncands, 0);
   }
   I8 const* needp = mixed ? need.data() : nullptr;
-#define OSH_SWQ(M, F32)                                                                  \
-  OSH_LAUNCH((swap3d_tri_quals<M, F32>), ntp, c, ntris.data(), lsize.data(), leev.data(),  \
-      lverts.data(), m.coords.data(), m.metric.data(), tb, needp,                         \
-      mixed ? unsure.data() : (I8*)nullptr, tq.data());                                   \
-  OSH_LAUNCH((swap3d_edge_lens<M, F32>), nep, c, nedges.data(), lsize.data(),              \
-      lverts.data(), m.coords.data(), m.metric.data(), tb, needp, el.data())
+  LoopMeasures lm{ntp, ntris.data(), nedges.data(), lsize.data(), leev.data(), lverts.data(),
+      m.coords.data(), m.metric.data(), tb, needp, mixed ? unsure.data() : (I8*)nullptr,
+      tq.data(), el.data()};
+#define OSH_SWQ(M, F32) OSH_LAUNCH((swap3d_loop_measures<M, F32>), ntp + nep, c, lm)
   if (mixed) {
     if (md == 3) {
       OSH_SWQ(3, true);
