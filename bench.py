@@ -394,9 +394,12 @@ def batch_main(args):
     (one HIP stream and one host thread each, boxes dealt out by size), so
     that the passes of different boxes overlap on the device.  The inputs are
     built before the clock starts.  `value` is the fastest way's throughput:
-    elements entering every mesh-changing pass over the wall time.  Every box
-    must end bit for bit as it does adapted alone (four digests), in FP64 and
-    in the mixed form."""
+    the work of adapting every box alone (elements entering each of its
+    mesh-changing passes) over the way's wall time, so that all ways are
+    priced on the same work (a joined pass also counts the elements of parts
+    it leaves alone; that count is kept as `pass_tets`).  Every box must end
+    bit for bit as it does adapted alone (four digests), in FP64 and in the
+    mixed form."""
     import threading
 
     import numpy as np
@@ -461,7 +464,7 @@ def batch_main(args):
         for k in range(s):
             for i, m in zip(groups[k], meshes[k]):
                 by_box[i] = m
-        return {"s": wall, "Mtets_per_s": sum(nin) / wall / 1e6,
+        return {"s": wall, "pass_tets": sum(nin),
                 "tets_out": sum(m.nents(3) for m in by_box.values())}, by_box
 
     ways = [("one_at_a_time", "alone", 1), ("joined", "joined", 1),
@@ -483,11 +486,14 @@ def batch_main(args):
             key = name if precision == 0 else name + "_mixed"
             results[key] = rec
             log(f"batch {key}: {rec}")
+    work = results["one_at_a_time"]["pass_tets"]
+    for rec in results.values():
+        rec["Mtets_per_s"] = work / rec["s"] / 1e6
     best = max((k for k in results if not k.endswith("_mixed")),
                key=lambda k: results[k]["Mtets_per_s"])
     line = {
-        "metric": "Mtets adapted/s, batched boxes (elements entering each mesh-changing pass "
-                  "over the wall time)",
+        "metric": "Mtets adapted/s, batched boxes (elements entering each box's own "
+                  "mesh-changing passes over the wall time)",
         "value": results[best]["Mtets_per_s"], "unit": "Mtets/s", "n_gpus": 1,
         "steps": 1, "warmup": 1, "ms_per_step": results[best]["s"] * 1e3,
         "higher_is_better": True, "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -500,6 +506,7 @@ def batch_main(args):
                    "expected_tets_unclamped": expected_raw, "expected_tets": expected,
                    "streams": nstreams, "tuning": tuning},
         "tets_in": tets_in,
+        "pass_tets_alone": work,
         "best": best,
         "ways": results,
         "speedup_vs_one_at_a_time": results[best]["Mtets_per_s"] /

@@ -183,7 +183,7 @@ __global__ void apply_vetoes(LO ncands, LO const* cands, I8 const* veto, I8* cod
 // max-determinant metric is not clear marks its side in `unsure`.  With
 // `need` given, the FP64 sweep evaluates only the marked sides.
 template <int D, int M, bool F32>
-__global__ void __launch_bounds__(kBlock, 5) coarsen_quality_lanes(LO nt, int lshift,
+__global__ void __launch_bounds__(kBlock, 6) coarsen_quality_lanes(LO nt, int lshift,
     LO const* cands, LO const* ev2v, LO const* cv2v, LO const* v2c_a2ab, LO const* v2c_ab2b,
     I8 const* v2c_codes, Real const* coords, Real const* metric, I8 const* codes,
     I8 const* need, I8* unsure, Real* partial) {
@@ -211,15 +211,14 @@ __global__ void __launch_bounds__(kBlock, 5) coarsen_quality_lanes(LO nt, int ls
       }
       if (will_die) continue;
       cv[ccv_col] = v_onto;
-      double p[D + 1][D];
-      double m[D + 1][nc];
+      // the quality reads the vertices' coordinates and metrics where they
+      // lie, so that they are loaded as they are used rather than all held
+      // at once (fewer live registers, more waves per SIMD)
       double const* pp[D + 1];
       double const* ms[D + 1];
       for (int k = 0; k < D + 1; ++k) {
-        for (int j = 0; j < D; ++j) p[k][j] = coords[cv[k] * D + j];
-        for (int j = 0; j < nc; ++j) m[k][j] = metric[cv[k] * nc + j];
-        pp[k] = p[k];
-        ms[k] = m[k];
+        pp[k] = coords + cv[k] * D;
+        ms[k] = metric + cv[k] * nc;
       }
       if constexpr (F32) {
         float q = f32::element_quality<D, M>(pp, ms, &unclear);

@@ -20,29 +20,44 @@ __global__ void gt_kernel(LO n, Real const* a, Real v, I8* out) {
   if (i < n) out[i] = a[i] > v;
 }
 
-// one thread per candidate edge: minimum quality of the 2 x (elements around
-// the edge) children, each measured with the max-determinant metric of its
-// cell-side vertices and the edge midpoint
+// 2^lshift lanes per candidate edge (consecutive threads of one wave): the
+// lanes share the 2 x (elements around the edge) children, each measured
+// with the max-determinant metric of its cell-side vertices and the edge
+// midpoint, and combine their minima with shuffles.  The minimum does not
+// depend on the order (min2 never takes a NaN, and the sign of a zero
+// decides nothing downstream: the candidates' qualities are only compared).
 template <int D, int M>
-__global__ void __launch_bounds__(kBlock) refine_qualities_kernel(LO ncands, LO const* cands,
-    LO const* ev2v, LO const* cv2v, LO const* e2c_a2ab, LO const* e2c_ab2b, I8 const* e2c_codes,
-    Real const* coords, Real const* metric, Real const* midpt_metrics, Real* out) {
-  LO cand = thread_index();
-  if (cand >= ncands) return;
+__global__ void __launch_bounds__(kBlock) refine_qualities_kernel(LO nt, int lshift,
+    LO ncands, LO const* cands, LO const* ev2v, LO const* cv2v, LO const* e2c_a2ab,
+    LO const* e2c_ab2b, I8 const* e2c_codes, Real const* coords, Real const* metric,
+    Real const* midpt_metrics, Real* out) {
+  LO t = thread_index();
+  int lanes = 1 << lshift;
+  LO cand = t >> lshift;
+  int lane = t & (lanes - 1);
+  // every lane of a group stays for the shuffles; groups past the end (whole
+  // groups, as nt is a multiple of the lane count) do no work
+  bool valid = t < nt && cand < ncands;
   constexpr int nc = (M == 1) ? 1 : (M == 2 ? 3 : 6);
-  LO e = cands[cand];
-  double midp[D];
-  for (int j = 0; j < D; ++j)
-    midp[j] = (coords[ev2v[e * 2] * D + j] + coords[ev2v[e * 2 + 1] * D + j]) / 2.;
-  double mm[nc];
-  for (int k = 0; k < nc; ++k) mm[k] = midpt_metrics[cand * nc + k];
   double minqual = 1.0;
-  for (LO ec = e2c_a2ab[e]; ec < e2c_a2ab[e + 1]; ++ec) {
+  LO e = valid ? cands[cand] : 0;
+  LO row = valid ? e2c_a2ab[e] : 0;
+  LO nchildren = valid ? 2 * (e2c_a2ab[e + 1] - row) : 0;
+  double midp[D];
+  double mm[nc];
+  if (lane < nchildren) {
+    for (int j = 0; j < D; ++j)
+      midp[j] = (coords[ev2v[e * 2] * D + j] + coords[ev2v[e * 2 + 1] * D + j]) / 2.;
+    for (int k = 0; k < nc; ++k) mm[k] = midpt_metrics[cand * nc + k];
+  }
+  for (LO ch = lane; ch < nchildren; ch += lanes) {
+    LO ec = row + ch / 2;
+    int eev = ch % 2;
     LO c = e2c_ab2b[ec];
     I8 code = e2c_codes[ec];
     int cce = code_which_down(code);
     int rot = code_rotation(code);
-    for (int eev = 0; eev < 2; ++eev) {
+    {
       int cev = eev ^ rot;
       int ccv = down_template(D, 1, cce, cev);
       int ccs = opposite_template(D, 0, ccv);
@@ -73,7 +88,8 @@ __global__ void __launch_bounds__(kBlock) refine_qualities_kernel(LO ncands, LO 
       minqual = math::min2(minqual, q);
     }
   }
-  out[cand] = minqual;
+  for (int d = lanes / 2; d > 0; d /= 2) minqual = math::min2(minqual, __shfl_xor(minqual, d, 64));
+  if (valid && lane == 0) out[cand] = minqual;
 }
 
 __global__ void seed_keys(LO ncands, LO const* cands, Real const* quals, Real qmin,
@@ -186,10 +202,13 @@ Reals refine_qualities(Ctx& c, DMesh& m, LOs const& cands) {
   Reals out(c, n);
   LOs const& cv2v = verts_of(c, m, dim);
   Adj const& e2c = ask_up(c, m, 1, dim);
+  // lanes share a candidate's children, as the coarsen sweeps share a side's
+  int lanes = fit_lanes(n, c.tuning.coarsen_lanes);
+  int lshift = lanes_shift(lanes);
 #define OSH_RQ(D, M)                                                                    \
-  OSH_LAUNCH((refine_qualities_kernel<D, M>), n, c, cands.data(), m.down[1].data(),        \
-      cv2v.data(), e2c.a2ab.data(), e2c.ab2b.data(), e2c.codes.data(), m.coords.data(),     \
-      m.metric.data(), mm.data(), out.data())
+  OSH_LAUNCH((refine_qualities_kernel<D, M>), n * lanes, c, lshift, n, cands.data(),       \
+      m.down[1].data(), cv2v.data(), e2c.a2ab.data(), e2c.ab2b.data(), e2c.codes.data(),    \
+      m.coords.data(), m.metric.data(), mm.data(), out.data())
   if (dim == 3 && md == 3)
     OSH_RQ(3, 3);
   else if (dim == 3)
