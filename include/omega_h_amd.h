@@ -209,6 +209,11 @@ int osh_mesh_set_class(osh_mesh* mesh, int ent_dim, const int* ids, const signed
 /* Mesh::ask_lengths / ask_qualities, src/Omega_h_mesh.cpp:374-388 */
 int osh_mesh_get_lengths(osh_mesh* mesh, double* out);
 int osh_mesh_get_qualities(osh_mesh* mesh, double* out);
+/* sort_by_keys (src/Omega_h_sort.hpp:8-22, src/Omega_h_sort.cpp:43-92):
+ * perm[i] is the index of the i-th smallest of n tuples of `width` (1 to 3)
+ * signed 64-bit keys in lexical order; the sort is stable (equal tuples keep
+ * their order).  The mesh builders use it (find_unique, Hilbert order). */
+int osh_sort_by_keys(osh_ctx* ctx, int n, int width, const long long* keys, int* perm);
 /* Mesh::min_quality / max_length (src/Omega_h_mesh.cpp:698-707): reduced on
  * the device */
 int osh_mesh_min_quality(osh_mesh* mesh, double* out);

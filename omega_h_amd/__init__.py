@@ -103,6 +103,7 @@ def load(path=None):
         "osh_ctx_set_tuning": [P, C.c_char_p, I],
         "osh_ctx_get_tuning": [P, C.c_char_p, IP],
         "osh_ctx_set_profile": [P, I],
+        "osh_sort_by_keys": [P, I, I, C.POINTER(C.c_longlong), IP],
         "osh_ctx_profile_clear": [P],
         "osh_ctx_profile_nframes": [P, IP],
         "osh_ctx_profile_frame": [P, I, IP, DP, C.POINTER(C.c_longlong), C.c_char_p, I],
@@ -263,6 +264,16 @@ class Context:
     def apply_tuning(self, settings):
         for name, value in settings.items():
             self.set_tuning(name, value)
+
+    def sort_by_keys(self, keys, width=1):
+        """The stable lexical sort permutation of len(keys) / width tuples
+        of 64-bit keys (sort_by_keys)."""
+        k = np.ascontiguousarray(keys, dtype=np.int64)
+        n = len(k) // width
+        perm = np.empty(n, dtype=np.int32)
+        check(self.lib.osh_sort_by_keys(self.h, n, width, _p(k, C.c_longlong),
+                                        _p(perm, C.c_int)), self.lib)
+        return perm
 
     PROFILE_TREE, PROFILE_MARKERS, PROFILE_SYNC = 1, 2, 4
 

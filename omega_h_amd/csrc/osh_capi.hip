@@ -861,6 +861,19 @@ int osh_mesh_max_length(osh_mesh* mesh, double* out) {
   });
 }
 
+int osh_sort_by_keys(osh_ctx* ctx, int n, int width, const long long* keys, int* perm) {
+  return guarded([&] {
+    OSH_CHECK(ctx && perm && (keys || n == 0), "null argument");
+    OSH_CHECK(n >= 0, "negative count");
+    OSH_CHECK(width >= 1 && width <= 3, "tuples of 1, 2 or 3 keys");
+    OSH_CHECK(std::int64_t(n) * width <= 0x7fffffff, "too many keys");
+    auto& c = ctx->ctx;
+    osh::DArray<osh::I64> k(c, n * width);
+    osh::upload(c, k, reinterpret_cast<osh::I64 const*>(keys), n * width);
+    download(c, osh::sort_by_keys(c, k, width), perm);
+  });
+}
+
 int osh_ctx_set_profile(osh_ctx* ctx, int flags) {
   return guarded([&] {
     OSH_CHECK(ctx, "null argument");

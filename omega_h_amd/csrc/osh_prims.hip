@@ -73,9 +73,7 @@ Ctx::Ctx() : pool(std::make_shared<Pool>()) {
   OSH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   OSH_HIP(hipHostMalloc(&pinned, 8 * kSlots + 64, hipHostMallocDefault));
   std::memset(pinned, 0, 8 * kSlots + 64);
-  // 1024 partial words, then the count of blocks that have stored theirs
-  OSH_HIP(hipMalloc(&partials, 1025 * sizeof(unsigned long long)));
-  OSH_HIP(hipMemset(partials, 0, 1025 * sizeof(unsigned long long)));
+  OSH_HIP(hipMalloc(&partials, 1024 * sizeof(double)));
   OSH_HIP(hipMalloc((void**)&tile_counter, sizeof(unsigned)));
   unsigned zero = 0;
   OSH_HIP(hipMemcpy(tile_counter, &zero, sizeof(zero), hipMemcpyHostToDevice));
@@ -511,69 +509,22 @@ __device__ __forceinline__ T combine(T a, T b) {
   return a + b;
 }
 
-template <class T>
-__device__ __forceinline__ unsigned long long to_word(T v) {
-  unsigned long long w = 0;
-  __builtin_memcpy(&w, &v, sizeof(T));
-  return w;
-}
-
-template <class T>
-__device__ __forceinline__ T from_word(unsigned long long w) {
-  T v;
-  __builtin_memcpy(&v, &w, sizeof(T));
-  return v;
-}
-
+// grid-stride partial reductions into out[blockIdx.x]; a one-block launch
+// over the partials then writes the result into host-visible memory (the
+// kernel boundary orders the two, so no device-wide fences are needed)
 template <class T, int Op>
-__device__ __forceinline__ T block_combine(T acc, T* buf) {
+__global__ void __launch_bounds__(kBlock) reduce_blocks(LO n, T const* in, T* out, T init) {
+  __shared__ T buf[kBlock];
+  T acc = init;
+  for (LO i = LO(blockIdx.x) * kBlock + LO(threadIdx.x); i < n; i += LO(gridDim.x) * kBlock)
+    acc = combine<T, Op>(acc, in[i]);
   buf[threadIdx.x] = acc;
   __syncthreads();
   for (int s = kBlock / 2; s > 0; s >>= 1) {
     if (int(threadIdx.x) < s) buf[threadIdx.x] = combine<T, Op>(buf[threadIdx.x], buf[threadIdx.x + s]);
     __syncthreads();
   }
-  return buf[0];
-}
-
-// One launch: grid-stride partial reductions, each block's stored as a
-// 64-bit word (agent-scope atomic store) before it counts itself in with a
-// device-wide fence and an atomic add; the block that counts last reads the
-// partials back with agent-scope loads, combines them in block order (the
-// same tree every time, so sums are reproducible), writes the result and
-// resets the count for the next reduction on the stream.
-template <class T, int Op>
-__global__ void __launch_bounds__(kBlock) reduce_blocks(LO n, T const* in,
-    unsigned long long* part, unsigned* done, T* out, T init) {
-  __shared__ T buf[kBlock];
-  __shared__ bool last;
-  T acc = init;
-  for (LO i = LO(blockIdx.x) * kBlock + LO(threadIdx.x); i < n; i += LO(gridDim.x) * kBlock)
-    acc = combine<T, Op>(acc, in[i]);
-  T mine = block_combine<T, Op>(acc, buf);
-  if (gridDim.x == 1) {
-    if (threadIdx.x == 0) *out = mine;
-    return;
-  }
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&part[blockIdx.x], to_word(mine), __ATOMIC_RELAXED,
-        __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    last = atomicAdd(done, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  T a = init;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
-    a = combine<T, Op>(a, from_word<T>(__hip_atomic_load(&part[b], __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT)));
-  __syncthreads();
-  T total = block_combine<T, Op>(a, buf);
-  if (threadIdx.x == 0) {
-    *out = total;
-    __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (threadIdx.x == 0) out[blockIdx.x] = buf[0];
 }
 
 // the result lands in `result` (device-visible) once the stream reaches it
@@ -587,9 +538,17 @@ void reduce_to(Ctx& c, DArray<T> const& a, T init, T* result) {
   }
   LO nb = LO(nblocks(n, kBlock * 4));
   if (nb > 1024) nb = 1024;
-  auto* part = static_cast<unsigned long long*>(c.partials);
-  hipLaunchKernelGGL((reduce_blocks<T, Op>), dim3(unsigned(nb)), dim3(kBlock), 0, c.stream, n,
-      a.data(), part, reinterpret_cast<unsigned*>(part + 1024), result, init);
+  if (nb == 1) {
+    hipLaunchKernelGGL((reduce_blocks<T, Op>), dim3(1), dim3(kBlock), 0, c.stream, n, a.data(),
+        result, init);
+  } else {
+    T* part = static_cast<T*>(c.partials);
+    hipLaunchKernelGGL((reduce_blocks<T, Op>), dim3(nb), dim3(kBlock), 0, c.stream, n, a.data(),
+        part, init);
+    OSH_HIP(hipGetLastError());
+    hipLaunchKernelGGL((reduce_blocks<T, Op>), dim3(1), dim3(kBlock), 0, c.stream, nb, part,
+        result, init);
+  }
   OSH_HIP(hipGetLastError());
 }
 
