@@ -329,6 +329,7 @@ def serve_main(args):
     opts = {2: ctx.default_opts(2), 3: ctx.default_opts(3)}
     for o in opts.values():
         o.verbosity = 0  # SILENT, as the workload specifies
+        o.precision = 1 if args.precision == "mixed" else 0
     rng = np.random.default_rng(args.seed)
     out = []
     for rate in args.rates:
@@ -369,9 +370,11 @@ def serve_main(args):
     line = {
         "metric": "p50/p99 serve latency (adapt requests, continuous batching)",
         "value": out[-1]["p99_s"] if out else None, "unit": "s (p99 at the highest rate)",
-        "n_gpus": 1, "higher_is_better": False, "vs_baseline": None, "dtype": "f64",
+        "n_gpus": 1, "higher_is_better": False, "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 screens, f64 decisions",
         "data": "synthetic",
-        "config": {"workload": f"Poisson adapt requests, 30% 2D / 70% 3D boxes, elements "
+        "config": {"precision": args.precision,
+                   "workload": f"Poisson adapt requests, 30% 2D / 70% 3D boxes, elements "
                                f"log-uniform in [1e3, {args.max_elems:g}], constant metrics with "
                                f"eigen-lengths log-uniform in [{args.h_lo}, {args.h_hi}] "
                                f"clamped to {args.clamp_elems:g} expected elements by the "
@@ -588,6 +591,8 @@ def main():
                     help="adapt: the 64^3 level-set adapt (default); serve: Poisson request "
                          "stream; batch: many boxes at once")
     ap.add_argument("--rates", type=float, nargs="+", default=[5.0, 20.0, 50.0])
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="adapt precision of the serving requests")
     ap.add_argument("--duration", type=float, default=20.0)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget", type=float, default=200e9)

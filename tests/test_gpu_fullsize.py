@@ -94,3 +94,19 @@ def test_full_adapt_properties(ctx):
     _, fdim = m.classification(2)
     assert np.all(fdim[deg == 1] < 3)
     assert np.all(fdim[deg == 2] == 3)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_silent_adapt_ends_where_the_oracle_does(ctx, precision):
+    """adapt() with no per-pass hook and no reports (as bench times it): the
+    same passes, and a final mesh whose four digests equal those the oracle
+    recorded after its last pass."""
+    fx = F.load("levelset_3d_64")
+    m = levelset_box(A, ctx)
+    opts = ctx.default_opts(3)
+    opts.verbosity = 0
+    opts.precision = precision
+    did, passes = m.adapt(opts)
+    assert did == fx["adapted"]
+    assert [list(p[:3]) for p in passes] == [r[:3] for r in fx["passes"]]
+    assert ["%016x" % d for d in m.digests()] == fx["passes"][-1][3:7]

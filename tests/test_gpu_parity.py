@@ -137,11 +137,15 @@ def test_indset_identical(ctx):
         assert_same(o.indset(d, q, c), g.indset(d, q, c), f"indset {d}")
 
 
+# verbosity 2 (the default, EACH_REBUILD) reports every pass; 0 is silent
+@pytest.mark.parametrize("verbosity", [2, 0], ids=["each_rebuild", "silent"])
 @pytest.mark.parametrize("dim,n", [(2, 32), (3, 4), (3, 8)])
-def test_adapt_identical(ctx, dim, n):
+def test_adapt_identical(ctx, dim, n, verbosity):
     o, g = boxes(ctx, dim, n)
     ro, po = o.adapt(O.default_opts(dim))
-    rg, pg = g.adapt(ctx.default_opts(dim))
+    go = ctx.default_opts(dim)
+    go.verbosity = verbosity
+    rg, pg = g.adapt(go)
     assert ro == rg
     assert [p[:3] for p in po] == [p[:3] for p in pg]
     compare_meshes(o, g)
@@ -172,7 +176,9 @@ def test_isotropic_3d_adapt_identical(ctx):
     o.set_metric(met, 1)
     g.set_metric(met, 1)
     ro, po = o.adapt(O.default_opts(3))
-    rg, pg = g.adapt(ctx.default_opts(3))
+    go = ctx.default_opts(3)
+    go.verbosity = 0
+    rg, pg = g.adapt(go)
     assert [p[:3] for p in po] == [p[:3] for p in pg]
     compare_meshes(o, g)
 
@@ -184,7 +190,9 @@ def test_anisotropic_2d_adapt_identical(ctx):
     o.set_metric(met, nc)
     g.set_metric(met, nc)
     ro, po = o.adapt(O.default_opts(2))
-    rg, pg = g.adapt(ctx.default_opts(2))
+    go = ctx.default_opts(2)
+    go.verbosity = 0
+    rg, pg = g.adapt(go)
     assert [p[:3] for p in po] == [p[:3] for p in pg]
     compare_meshes(o, g)
 
