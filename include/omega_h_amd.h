@@ -450,8 +450,34 @@ int osh_server_wait(osh_server* server, int ticket, osh_serve_result* out);
  * pass is not interrupted) */
 int osh_server_set_timeout(osh_server* server, double seconds);
 
-/* cavity kernels on host arrays, for tests and diagnostics:
- * refine_qualities src/Omega_h_refine_qualities.cpp:88-108 */
+/* stage kernels on caller arrays (host or device), for callers that drive
+ * the stages themselves and for tests:
+ * get_mident_metrics (src/Omega_h_metric.cpp:57-90): the log-Euclidean
+ * midpoint metric of each listed edge (metric_ncomps values per edge) */
+int osh_mident_metrics(osh_mesh* mesh, int n, const int* edges, double* out);
+/* swap3d_qualities (src/Omega_h_swap3d_qualities.cpp:11-75) with the choice
+ * of src/Omega_h_swap3d_choice.hpp: for each listed interior edge of a 3D
+ * mesh (no edge twice) the chosen triangulation of its loop (-1 for none)
+ * and the quality of that triangulation when it improves on the edge's
+ * elements, else -1 (what swap_edges keeps).  Honours opts->precision: in
+ * the mixed form a candidate the FP32 screen rejects reports -1 for both. */
+int osh_swap3d_qualities(osh_mesh* mesh, const osh_adapt_opts* opts, int n, const int* edges,
+    double* quals, signed char* configs);
+/* invert_adj (src/Omega_h_adj.cpp:231-263) of a high -> low map with deg
+ * entries per high entity: a2ab (nlow + 1 offsets), ab2b and codes (nhigh *
+ * deg uses); rows are sorted by high index and a code carries the down
+ * code's flip and rotation (down_codes may be null) and the use's position */
+int osh_invert_adj(osh_ctx* ctx, int nhigh, int deg, int nlow, const int* hl2l,
+    const signed char* down_codes, int* a2ab, int* ab2b, signed char* codes);
+/* reflect_down (src/Omega_h_adj.cpp:424-441): for simplices of dimension
+ * high_dim given by vertex lists, their sides among the mesh's entities of
+ * dimension high_dim - 1 and the alignment codes; every side must exist */
+int osh_reflect_down(osh_mesh* mesh, int high_dim, int nprods, const int* prod_verts,
+    int* down, signed char* codes);
+/* collect_marked (src/Omega_h_map.cpp:174-185): the indices of
+ * the nonzero marks in order; out has room for n, *count receives how many */
+int osh_collect_marked(osh_ctx* ctx, int n, const signed char* marks, int* out, int* count);
+/* refine_qualities src/Omega_h_refine_qualities.cpp:88-108 */
 int osh_refine_qualities(osh_mesh* mesh, int n, const int* cands, double* out);
 /* coarsen_qualities(mesh, cands2edges, cand_codes),
  * src/Omega_h_coarsen_qualities.cpp:65-88, on device pointers: out[2i + col]

@@ -104,6 +104,11 @@ def load(path=None):
         "osh_ctx_get_tuning": [P, C.c_char_p, IP],
         "osh_ctx_set_profile": [P, I],
         "osh_sort_by_keys": [P, I, I, C.POINTER(C.c_longlong), IP],
+        "osh_invert_adj": [P, I, I, I, IP, BP, IP, IP, BP],
+        "osh_collect_marked": [P, I, BP, IP, IP],
+        "osh_mident_metrics": [P, I, IP, DP],
+        "osh_swap3d_qualities": [P, C.POINTER(AdaptOpts), I, IP, DP, BP],
+        "osh_reflect_down": [P, I, I, IP, IP, BP],
         "osh_ctx_profile_clear": [P],
         "osh_ctx_profile_nframes": [P, IP],
         "osh_ctx_profile_frame": [P, I, IP, DP, C.POINTER(C.c_longlong), C.c_char_p, I],
@@ -274,6 +279,29 @@ class Context:
         check(self.lib.osh_sort_by_keys(self.h, n, width, _p(k, C.c_longlong),
                                         _p(perm, C.c_int)), self.lib)
         return perm
+
+    def invert_adj(self, hl2l, deg, nlow, down_codes=None):
+        """invert_adj of a high -> low map: (a2ab, ab2b, codes)."""
+        h = np.ascontiguousarray(hl2l, dtype=np.int32)
+        nhigh = len(h) // deg
+        a2ab = np.empty(nlow + 1, dtype=np.int32)
+        ab2b = np.empty(max(len(h), 1), dtype=np.int32)
+        codes = np.empty(max(len(h), 1), dtype=np.int8)
+        dc = None if down_codes is None else np.ascontiguousarray(down_codes, dtype=np.int8)
+        check(self.lib.osh_invert_adj(self.h, nhigh, deg, nlow, _p(h, C.c_int),
+                                      None if dc is None else _p(dc, C.c_byte),
+                                      _p(a2ab, C.c_int), _p(ab2b, C.c_int),
+                                      _p(codes, C.c_byte)), self.lib)
+        return a2ab, ab2b[:len(h)], codes[:len(h)]
+
+    def collect_marked(self, marks):
+        """Indices of the nonzero marks, in order."""
+        m = np.ascontiguousarray(marks, dtype=np.int8)
+        out = np.empty(max(len(m), 1), dtype=np.int32)
+        n = C.c_int()
+        check(self.lib.osh_collect_marked(self.h, len(m), _p(m, C.c_byte), _p(out, C.c_int),
+                                          C.byref(n)), self.lib)
+        return out[:n.value]
 
     PROFILE_TREE, PROFILE_MARKERS, PROFILE_SYNC = 1, 2, 4
 
@@ -566,6 +594,34 @@ class Mesh:
         out = np.zeros(self.nents(self.dim), dtype=np.float64)
         check(self.lib.osh_mesh_get_qualities(self.h, _p(out, C.c_double)), self.lib)
         return out
+
+    def mident_metrics(self, edges):
+        """Log-Euclidean midpoint metrics of the listed edges."""
+        e = np.ascontiguousarray(edges, dtype=np.int32)
+        out = np.empty(max(len(e) * self.metric_ncomps, 1), dtype=np.float64)
+        check(self.lib.osh_mident_metrics(self.h, len(e), _p(e, C.c_int),
+                                          _p(out, C.c_double)), self.lib)
+        return out[:len(e) * self.metric_ncomps]
+
+    def swap3d_qualities(self, opts, edges):
+        """(qualities, configs) of the listed interior edges of a 3D mesh."""
+        e = np.ascontiguousarray(edges, dtype=np.int32)
+        q = np.empty(max(len(e), 1), dtype=np.float64)
+        cfg = np.empty(max(len(e), 1), dtype=np.int8)
+        check(self.lib.osh_swap3d_qualities(self.h, C.byref(opts), len(e), _p(e, C.c_int),
+                                            _p(q, C.c_double), _p(cfg, C.c_byte)), self.lib)
+        return q[:len(e)], cfg[:len(e)]
+
+    def reflect_down(self, high_dim, prod_verts):
+        """The sides (among the mesh's entities one dimension down) and codes
+        of simplices given by vertex lists."""
+        pv = np.ascontiguousarray(prod_verts, dtype=np.int32)
+        n = len(pv) // (high_dim + 1)
+        down = np.empty(max(n * (high_dim + 1), 1), dtype=np.int32)
+        codes = np.empty(max(n * (high_dim + 1), 1), dtype=np.int8)
+        check(self.lib.osh_reflect_down(self.h, high_dim, n, _p(pv, C.c_int),
+                                        _p(down, C.c_int), _p(codes, C.c_byte)), self.lib)
+        return down[:n * (high_dim + 1)], codes[:n * (high_dim + 1)]
 
     def refine_qualities(self, cands):
         c = np.ascontiguousarray(cands, dtype=np.int32)

@@ -390,6 +390,11 @@ __global__ void edge_star_fill(LO ne, LO const* e2f_a2ab, LO const* e2f_ab2b,
 
 }  // namespace
 
+Adj invert_adj_rows(Ctx& c, LO nhigh, int deg, LO nlow, LOs const& hl2l,
+    Bytes const* down_codes) {
+  return invert_adj(c, nhigh, deg, nlow, hl2l, down_codes);
+}
+
 LOs const& verts_of(Ctx& c, DMesh& m, int d) {
   if (d == 1) return m.down[1];
   if (!m.verts[d].exists()) {

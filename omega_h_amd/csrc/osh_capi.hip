@@ -1315,6 +1315,107 @@ int osh_intersect_metrics(osh_ctx* ctx, int dim, int n, const double* m1, const 
   });
 }
 
+int osh_mident_metrics(osh_mesh* mesh, int n, const int* edges, double* out) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(mesh->m.metric.exists(), "mesh has no metric");
+    OSH_CHECK(n >= 0 && (n == 0 || (edges && out)), "null argument");
+    auto& c = mesh->owner->ctx;
+    std::vector<int> h = host_copy(c, edges, std::size_t(n));
+    for (int e : h) OSH_CHECK(e >= 0 && e < mesh->m.nents[1], "edge out of range");
+    osh::LOs d = osh::to_device(c, h);
+    osh::Reals r(c, n * mesh->m.metric_ncomps);
+    if (n) osh::mident_metrics(c, mesh->m, n, d.data(), r.data());
+    download(c, r, out);
+  });
+}
+
+int osh_swap3d_qualities(osh_mesh* mesh, const osh_adapt_opts* opts, int n, const int* edges,
+    double* quals, signed char* configs) {
+  return guarded([&] {
+    check_adaptable(mesh);
+    OSH_CHECK(opts && n >= 0 && (n == 0 || (edges && quals && configs)), "null argument");
+    OSH_CHECK(mesh->m.dim == 3, "swap3d qualities need a 3D mesh");
+    osh::AdaptOptions o;
+    to_options(opts, o);
+    auto& c = mesh->owner->ctx;
+    std::vector<int> h = host_copy(c, edges, std::size_t(n));
+    std::vector<signed char> edim(std::size_t(mesh->m.nents[1]));
+    download(c, mesh->m.class_dim[1], reinterpret_cast<osh::I8*>(edim.data()));
+    std::vector<char> seen(edim.size(), 0);
+    for (int e : h) {
+      OSH_CHECK(e >= 0 && e < mesh->m.nents[1], "edge out of range");
+      OSH_CHECK(edim[std::size_t(e)] == 3, "swap candidates are interior edges");
+      OSH_CHECK(!seen[std::size_t(e)], "an edge listed twice");
+      seen[std::size_t(e)] = 1;
+    }
+    osh::LOs cfg;
+    osh::Reals q;
+    osh::swap3d_choices(c, mesh->m, o, osh::to_device(c, h), cfg, q);
+    std::vector<int> hc = osh::to_host(c, cfg);
+    for (int i = 0; i < n; ++i) configs[i] = static_cast<signed char>(hc[std::size_t(i)]);
+    download(c, q, quals);
+  });
+}
+
+int osh_invert_adj(osh_ctx* ctx, int nhigh, int deg, int nlow, const int* hl2l,
+    const signed char* down_codes, int* a2ab, int* ab2b, signed char* codes) {
+  return guarded([&] {
+    OSH_CHECK(ctx && a2ab && (nhigh == 0 || (hl2l && ab2b && codes)), "null argument");
+    OSH_CHECK(nhigh >= 0 && nlow >= 0 && deg >= 1 && deg <= 6, "bad sizes");
+    OSH_CHECK(std::int64_t(nhigh) * deg <= 0x7fffffff, "too many uses");
+    auto& c = ctx->ctx;
+    std::vector<int> h = host_copy(c, hl2l, std::size_t(nhigh) * std::size_t(deg));
+    for (int l : h) OSH_CHECK(l >= 0 && l < nlow, "low entity out of range");
+    osh::Bytes dc;
+    if (down_codes)
+      dc = osh::to_device(c, host_copy(c, reinterpret_cast<osh::I8 const*>(down_codes),
+                                       std::size_t(nhigh) * std::size_t(deg)));
+    osh::Adj a = osh::invert_adj_rows(c, nhigh, deg, nlow, osh::to_device(c, h),
+        down_codes ? &dc : nullptr);
+    download(c, a.a2ab, a2ab);
+    osh::LO nuses = nhigh * deg;
+    if (nuses) {
+      OSH_HIP(hipMemcpyAsync(ab2b, a.ab2b.data(), sizeof(int) * std::size_t(nuses),
+          hipMemcpyDefault, c.stream));
+      OSH_HIP(hipMemcpyAsync(codes, a.codes.data(), std::size_t(nuses), hipMemcpyDefault,
+          c.stream));
+    }
+    c.sync();
+  });
+}
+
+int osh_reflect_down(osh_mesh* mesh, int high_dim, int nprods, const int* prod_verts,
+    int* down, signed char* codes) {
+  return guarded([&] {
+    check_mesh(mesh);
+    OSH_CHECK(high_dim >= 2 && high_dim <= mesh->m.dim, "high_dim between 2 and the mesh's");
+    OSH_CHECK(nprods >= 0 && (nprods == 0 || (prod_verts && down && codes)), "null argument");
+    auto& c = mesh->owner->ctx;
+    std::vector<int> h = host_copy(c, prod_verts, std::size_t(nprods) * (high_dim + 1));
+    for (int v : h) OSH_CHECK(v >= 0 && v < mesh->m.nents[0], "vertex out of range");
+    osh::LOs d, found;
+    osh::Bytes cd;
+    osh::reflect_down(c, mesh->m, high_dim, nprods, osh::to_device(c, h), d, cd, &found);
+    std::vector<int> hf = osh::to_host(c, found);
+    for (int f : hf) OSH_CHECK(f == 1, "a side of a given simplex is not in the mesh");
+    download(c, d, down);
+    download(c, cd, reinterpret_cast<osh::I8*>(codes));
+  });
+}
+
+int osh_collect_marked(osh_ctx* ctx, int n, const signed char* marks, int* out, int* count) {
+  return guarded([&] {
+    OSH_CHECK(ctx && count && n >= 0 && (n == 0 || (marks && out)), "null argument");
+    auto& c = ctx->ctx;
+    osh::Bytes m = osh::to_device(c, host_copy(c, reinterpret_cast<osh::I8 const*>(marks),
+                                               std::size_t(n)));
+    osh::LOs r = osh::collect_marked(c, m);
+    *count = r.size();
+    download(c, r, out);
+  });
+}
+
 int osh_refine_qualities(osh_mesh* mesh, int n, const int* cands, double* out) {
   return guarded([&] {
     check_mesh(mesh);

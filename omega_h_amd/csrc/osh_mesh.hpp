@@ -142,6 +142,11 @@ Reals const& ask_sizes(Ctx& c, DMesh& m);
 // must treat a use whose count is not 1 as an error (and not read its match).
 void reflect_down(Ctx& c, DMesh& m, int high_dim, LO nprods, LOs const& prod_verts,
     LOs& down, Bytes& codes, LOs* found_out = nullptr);
+// invert_adj (src/Omega_h_adj.cpp:231-263) of any high -> low map with deg
+// entries per high entity: rows sorted by high index, codes carrying the
+// down codes' flip and rotation and the use's position
+Adj invert_adj_rows(Ctx& c, LO nhigh, int deg, LO nlow, LOs const& hl2l,
+    Bytes const* down_codes);
 // unique lower entities of element vertex lists (src/Omega_h_adj.cpp:133-153)
 LOs find_unique(Ctx& c, LO nhigh, int high_dim, int low_dim, LOs const& hv2v);
 

@@ -141,6 +141,12 @@ struct SwapCache {
 // the caller whenever anything else changes the mesh, its metric or the
 // options) the 3D pass evaluates only the candidates not carried over
 bool swap_edges(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache = nullptr);
+// the swap choices of given interior edges of a 3D mesh (swap3d_qualities
+// and the choice of a triangulation): per edge the chosen triangulation (-1
+// for none) and the quality of a triangulation that improves on the edge's
+// elements (-1 otherwise)
+void swap3d_choices(Ctx& c, DMesh& m, AdaptOptions const& o, LOs const& edges,
+    LOs& configs_out, Reals& quals_out);
 
 // known-answer hooks used by the tests
 Reals refine_qualities(Ctx& c, DMesh& m, LOs const& cands);

@@ -865,15 +865,19 @@ Bytes swap_candidates(Ctx& c, DMesh& m, AdaptOptions const& o, Bytes* configs, B
   return edges;
 }
 
-bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
-  ask_lengths(c, m);
-  Bytes configs, keep;
-  Reals quals;
-  Bytes cand = swap_candidates(c, m, o, &configs, keep, quals);
-  // every edge -> tet row this pass reads is a candidate's
-  LOs cands, cand_of_edge;
-  Adj e2t = marked_up(c, m, 1, 3, cand, &cands, true, &cand_of_edge);
-  if (cands.size() == 0) return false;
+// swap3d_qualities (src/Omega_h_swap3d_qualities.cpp:11-75) with the choice
+// of each candidate's triangulation (src/Omega_h_swap3d_choice.hpp): for the
+// candidate edges `cands`, whose edge -> tet rows e2t holds, configs[e]
+// receives the chosen triangulation (-1 for none) and, when it improves on
+// the elements around the edge, keep[e] = 1 and quals[e] its quality
+// (entries of other edges are left as they are).  The candidates' loops stay
+// in `loops` for the products.
+struct SwapLoops {
+  LOs lsize, leev, lverts;
+};
+
+void swap3d_evaluate(Ctx& c, DMesh& m, AdaptOptions const& o, LOs const& cands, Adj const& e2t,
+    SwapCache* cache, Bytes& configs, Reals& quals, Bytes& keep, SwapLoops& loops) {
   LO ne = m.nents[1];
   int md = (m.metric_ncomps == 1) ? 1 : 3;
   Tables const* tb = device_tables(c);
@@ -888,7 +892,12 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
     cache->q = Reals(c, ne);
   }
   Range quality_range(c, "swap3d_qualities");
-  LOs lsize(c, ncands), leev(c, ncands * 2), lverts(c, ncands * swap::kMaxLoop);
+  loops.lsize = LOs(c, ncands);
+  loops.leev = LOs(c, ncands * 2);
+  loops.lverts = LOs(c, ncands * swap::kMaxLoop);
+  LOs const& lsize = loops.lsize;
+  LOs const& leev = loops.leev;
+  LOs const& lverts = loops.lverts;
   LOs ntris(c, ncands), nedges(c, ncands);
   OSH_LAUNCH(swap3d_loops, ncands * kLoopLanes, c, cands.data(), e2t.a2ab.data(), e2t.ab2b.data(),
       e2t.codes.data(), m.down[1].data(), tv2v.data(), tb,
@@ -937,7 +946,24 @@ bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
       e2t.ab2b.data(), eq.data(), tb, needp, configs.data(), quals.data(), keep.data(),
       cache ? cache->valid.data() : (I8*)nullptr, cache ? cache->cfg.data() : (I8*)nullptr,
       cache ? cache->keep.data() : (I8*)nullptr, cache ? cache->q.data() : (Real*)nullptr);
-  quality_range.end();
+}
+
+bool swap_edges_3d(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
+  ask_lengths(c, m);
+  Bytes configs, keep;
+  Reals quals;
+  Bytes cand = swap_candidates(c, m, o, &configs, keep, quals);
+  // every edge -> tet row this pass reads is a candidate's
+  LOs cands, cand_of_edge;
+  Adj e2t = marked_up(c, m, 1, 3, cand, &cands, true, &cand_of_edge);
+  if (cands.size() == 0) return false;
+  LO ne = m.nents[1];
+  Tables const* tb = device_tables(c);
+  SwapLoops loops;
+  swap3d_evaluate(c, m, o, cands, e2t, cache, configs, quals, keep, loops);
+  LOs const& lsize = loops.lsize;
+  LOs const& leev = loops.leev;
+  LOs const& lverts = loops.lverts;
   bool any;
   Bytes keys = find_indset(c, m, 1, quals, keep, &any, &e2t);
   if (!any) return false;
@@ -1043,6 +1069,46 @@ bool swap_edges_2d(Ctx& c, DMesh& m, AdaptOptions const& o) {
 bool swap_edges(Ctx& c, DMesh& m, AdaptOptions const& o, SwapCache* cache) {
   if (m.dim == 3) return swap_edges_3d(c, m, o, cache);
   return swap_edges_2d(c, m, o);
+}
+
+namespace {
+
+__global__ void mark_listed(LO n, LO const* list, I8* marks) {
+  LO i = thread_index();
+  if (i < n) marks[list[i]] = 1;
+}
+
+__global__ void gather_choices(LO n, LO const* edges, I8 const* configs, Real const* quals,
+    I8 const* keep, LO* out_cfg, Real* out_q) {
+  LO i = thread_index();
+  if (i >= n) return;
+  LO e = edges[i];
+  out_cfg[i] = configs[e];
+  out_q[i] = keep[e] ? quals[e] : -1.0;
+}
+
+}  // namespace
+
+void swap3d_choices(Ctx& c, DMesh& m, AdaptOptions const& o, LOs const& edges,
+    LOs& configs_out, Reals& quals_out) {
+  OSH_CHECK(m.dim == 3, "swap3d choices need a 3D mesh");
+  LO ne = m.nents[1];
+  LO n = edges.size();
+  Bytes marks = fill_i8(c, ne, 0);
+  OSH_LAUNCH(mark_listed, n, c, edges.data(), marks.data());
+  LOs cands, cand_of_edge;
+  Adj e2t = marked_up(c, m, 1, 3, marks, &cands, true, &cand_of_edge);
+  Bytes configs = fill_i8(c, ne, -1);
+  Bytes keep = fill_i8(c, ne, 0);
+  Reals quals = fill_real(c, ne, -1.0);
+  if (cands.size() > 0) {
+    SwapLoops loops;
+    swap3d_evaluate(c, m, o, cands, e2t, nullptr, configs, quals, keep, loops);
+  }
+  configs_out = LOs(c, n);
+  quals_out = Reals(c, n);
+  OSH_LAUNCH(gather_choices, n, c, edges.data(), configs.data(), quals.data(), keep.data(),
+      configs_out.data(), quals_out.data());
 }
 
 void find_loops(Ctx& c, DMesh& m, LOs const& edges, LOs& eev2v, LOs& loop_verts, LOs& sizes) {

@@ -574,6 +574,10 @@ static void mident_metrics(om_mesh* m, int n, int const* edges, double* out) {
   }
 }
 
+void om_kat_mident_metrics(om_mesh* m, int n, int const* edges, double* out) {
+  mident_metrics(m, n, edges, out);
+}
+
 /* ------------------------------------------------------------ refine */
 static void refine_qualities(om_mesh* m, int ncands, int const* cands,
     double const* midpt_metrics, double* out) {
@@ -1522,6 +1526,26 @@ static double edge_old_quality(om_mesh* m, int e) {
   double r = 1.79769313486231570815e+308;
   for (int j = e2c->a2ab[e]; j < e2c->a2ab[e + 1]; ++j) r = min2(r, q[e2c->ab2b[j]]);
   return r;
+}
+
+/* the per-candidate part of swap_edges_3d (find_loop, then
+ * choose_int_edges over the loop's triangulations,
+ * src/Omega_h_swap3d_qualities.cpp:11-75, src/Omega_h_swap3d_choice.hpp):
+ * the chosen triangulation, and its quality when it improves on the
+ * elements around the edge (-1 otherwise) */
+void om_kat_swap3d_qualities(om_mesh* m, double max_length, int n, int const* edges,
+    double* quals, signed char* configs) {
+  om_ask_lengths(m);
+  for (int i = 0; i < n; ++i) {
+    int e = edges[i];
+    swap_loop loop;
+    find_loop(m, e, &loop);
+    int config = -1;
+    double q = -1.0;
+    if (loop.size <= OM_MAX_EDGE_SWAP) swap_choose(m, &loop, max_length, &config, &q);
+    configs[i] = (signed char)config;
+    quals[i] = (config >= 0 && q > edge_old_quality(m, e)) ? q : -1.0;
+  }
 }
 
 static void swap_rebuild(om_mesh* m, int nkeys, int const* keys2edges,

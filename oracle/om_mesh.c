@@ -392,6 +392,15 @@ void om_kat_reflect_down(int high_dim, int low_dim, int nhigh, int const* hv2v, 
 
 /* invert_map_by_atomics / invert_map_by_sorting (src/Omega_h_map.cpp:238-280):
  * rows of the inverse sorted by the high index */
+void om_kat_invert_adj(int nhigh, int deg, int nlow, int const* hl2l,
+    signed char const* down_codes, int* a2ab, int* ab2b, signed char* codes) {
+  om_graph* g = invert_adj(nhigh, deg, nlow, hl2l, down_codes);
+  memcpy(a2ab, g->a2ab, sizeof(int) * (size_t)(nlow + 1));
+  memcpy(ab2b, g->ab2b, sizeof(int) * (size_t)nhigh * (size_t)deg);
+  memcpy(codes, g->codes, (size_t)nhigh * (size_t)deg);
+  om_graph_free(g);
+}
+
 void om_kat_invert_map(int n, int const* hl2l, int nlow, int* a2ab, int* ab2b) {
   om_graph* g = invert_adj(n, 1, nlow, hl2l, NULL);
   memcpy(a2ab, g->a2ab, sizeof(int) * (size_t)(nlow + 1));

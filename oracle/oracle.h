@@ -153,6 +153,11 @@ void om_kat_form_uses(int nhigh, int high_dim, int low_dim, int const* hv2v, int
 void om_kat_reflect_down(int high_dim, int low_dim, int nhigh, int const* hv2v, int nlow,
     int const* lv2v, int nverts, int* ab2b, signed char* codes);
 void om_kat_invert_map(int n, int const* hl2l, int nlow, int* a2ab, int* ab2b);
+void om_kat_invert_adj(int nhigh, int deg, int nlow, int const* hl2l,
+    signed char const* down_codes, int* a2ab, int* ab2b, signed char* codes);
+void om_kat_mident_metrics(om_mesh* m, int n, int const* edges, double* out);
+void om_kat_swap3d_qualities(om_mesh* m, double max_length, int n, int const* edges,
+    double* quals, signed char* configs);
 int om_kat_align_vert_index(int n, int index, int code);
 void om_kat_mark_down(om_mesh* m, int high, int low, signed char const* marks,
     signed char* out);

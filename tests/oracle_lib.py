@@ -105,6 +105,10 @@ def lib():
         L.om_kat_reflect_down.argtypes = [C.c_int, C.c_int, C.c_int, IPt, C.c_int, IPt,
                                           C.c_int, IPt, BPt]
         L.om_kat_invert_map.argtypes = [C.c_int, IPt, C.c_int, IPt, IPt]
+        L.om_kat_invert_adj.argtypes = [C.c_int, C.c_int, C.c_int, IPt, BPt, IPt, IPt, BPt]
+        L.om_kat_mident_metrics.argtypes = [P, C.c_int, IPt, C.POINTER(C.c_double)]
+        L.om_kat_swap3d_qualities.argtypes = [P, C.c_double, C.c_int, IPt,
+                                              C.POINTER(C.c_double), BPt]
         L.om_kat_align_vert_index.argtypes = [C.c_int, C.c_int, C.c_int]
         L.om_kat_mark_down.argtypes = [P, C.c_int, C.c_int, BPt, BPt]
         L.om_kat_mark_up.argtypes = [P, C.c_int, C.c_int, BPt, BPt]
@@ -329,6 +333,25 @@ class Mesh:
         lib().om_get_star(self.h, d, _ptr(a2ab, C.c_int), _ptr(ab2b, C.c_int))
         return a2ab, ab2b
 
+    def mident_metrics(self, edges):
+        e = _ints(edges)
+        nc = lib().om_metric_ncomps(self.h)
+        out = np.zeros(max(len(e) * nc, 1), dtype=np.float64)
+        eb = e if len(e) else np.zeros(1, dtype=np.int32)
+        lib().om_kat_mident_metrics(self.h, len(e), _ptr(eb, C.c_int), _ptr(out, C.c_double))
+        return out[:len(e) * nc]
+
+    def swap3d_qualities(self, max_length, edges):
+        """(qualities, configs) of the listed interior edges, as the swap pass
+        decides them."""
+        e = _ints(edges)
+        q = np.zeros(max(len(e), 1), dtype=np.float64)
+        cfg = np.zeros(max(len(e), 1), dtype=np.int8)
+        eb = e if len(e) else np.zeros(1, dtype=np.int32)
+        lib().om_kat_swap3d_qualities(self.h, max_length, len(e), _ptr(eb, C.c_int),
+                                      _ptr(q, C.c_double), _ptr(cfg, C.c_byte))
+        return q[:len(e)], cfg[:len(e)]
+
     def refine_qualities(self, cands):
         c = np.ascontiguousarray(cands, dtype=np.int32)
         out = np.zeros(len(c), dtype=np.float64)
@@ -537,6 +560,21 @@ def reflect_down(hv2v, lv2v, nverts, high_dim, low_dim):
                               _ptr(lb, C.c_int), nverts, _ptr(ab2b, C.c_int),
                               _ptr(codes, C.c_byte))
     return ab2b[:nhigh * deg], codes[:nhigh * deg]
+
+
+def invert_adj(hl2l, deg, nlow, down_codes=None):
+    """invert_adj, src/Omega_h_adj.cpp:231-263: (a2ab, ab2b, codes)."""
+    h = _ints(hl2l)
+    nhigh = len(h) // deg
+    a2ab = np.zeros(nlow + 1, dtype=np.int32)
+    ab2b = np.zeros(max(len(h), 1), dtype=np.int32)
+    codes = np.zeros(max(len(h), 1), dtype=np.int8)
+    hb = h if len(h) else np.zeros(1, dtype=np.int32)
+    dc = None if down_codes is None else np.ascontiguousarray(down_codes, dtype=np.int8)
+    lib().om_kat_invert_adj(nhigh, deg, nlow, _ptr(hb, C.c_int),
+                            None if dc is None else _ptr(dc, C.c_byte), _ptr(a2ab, C.c_int),
+                            _ptr(ab2b, C.c_int), _ptr(codes, C.c_byte))
+    return a2ab, ab2b[:len(h)], codes[:len(h)]
 
 
 def invert_map(hl2l, nlow):
