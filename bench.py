@@ -594,7 +594,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="adapt precision of the serving requests")
     ap.add_argument("--duration", type=float, default=20.0)
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--budget", type=float, default=200e9)
     ap.add_argument("--max-elems", type=float, default=2e6)
     ap.add_argument("--clamp-elems", type=float, default=2.5e5)
