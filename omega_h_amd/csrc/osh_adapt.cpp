@@ -237,12 +237,14 @@ struct PartFlags {
 };
 
 void put_back(Ctx& c, DMesh& joined, std::vector<DMesh>& meshes, std::vector<int> const& ids) {
+  Range range(c, "split_mesh");
   std::vector<DMesh> parts;
   split_mesh(c, joined, int(ids.size()), parts);
   for (std::size_t k = 0; k < ids.size(); ++k) meshes[std::size_t(ids[k])] = parts[k];
 }
 
 DMesh joined_of(Ctx& c, std::vector<DMesh>& meshes, std::vector<int> const& ids) {
+  Range range(c, "join_meshes");
   std::vector<DMesh const*> ps;
   for (int i : ids) ps.push_back(&meshes[std::size_t(i)]);
   DMesh joined;
@@ -253,6 +255,7 @@ DMesh joined_of(Ctx& c, std::vector<DMesh>& meshes, std::vector<int> const& ids)
 void batch_lengths(Ctx& c, std::vector<DMesh>& meshes, std::vector<int> active,
     AdaptOptions const& o, std::vector<PassRecord>* log) {
   if (!o.should_refine && !o.should_coarsen) return;
+  Range range(c, "satisfy_lengths");
   while (!active.empty()) {
     int np = int(active.size());
     DMesh joined = joined_of(c, meshes, active);
@@ -292,6 +295,7 @@ void batch_lengths(Ctx& c, std::vector<DMesh>& meshes, std::vector<int> active,
 void batch_quality(Ctx& c, std::vector<DMesh>& meshes, std::vector<int> group,
     AdaptOptions const& o, std::vector<PassRecord>* log) {
   if (!o.should_swap && !o.should_coarsen_slivers) return;
+  Range range(c, "satisfy_quality");
   std::vector<std::vector<int>> groups;
   if (!group.empty()) groups.push_back(group);
   while (!groups.empty()) {
@@ -359,6 +363,8 @@ std::vector<int> adapt_batch(Ctx& c, std::vector<DMesh>& meshes, AdaptOptions co
   OSH_CHECK(o.min_quality_desired <= 1.0, "min_quality_desired > 1");
   OSH_CHECK(o.nsliver_layers >= 0 && o.nsliver_layers < 100, "nsliver_layers out of range");
   int n = int(meshes.size());
+  Range batch_range(c, "adapt_batch");
+  Range pre_range(c, "pre_adapt");
   std::vector<int> did(std::size_t(n), 0);
   std::vector<int> todo;
   for (int i = 0; i < n; ++i) {
@@ -375,6 +381,7 @@ std::vector<int> adapt_batch(Ctx& c, std::vector<DMesh>& meshes, AdaptOptions co
     did[std::size_t(i)] = 1;
     todo.push_back(i);
   }
+  pre_range.end();
   AdaptOptions quiet = o;
   quiet.verbosity = 0;
   batch_lengths(c, meshes, todo, quiet, log);

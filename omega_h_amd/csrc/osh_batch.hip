@@ -42,13 +42,49 @@ void copy_into(Ctx& c, DArray<T> const& src, DArray<T>& dst, LO at, LO n) {
         hipMemcpyDeviceToDevice, c.stream));
 }
 
-template <class T>
-DArray<T> slice(Ctx& c, DArray<T> const& src, LO at, LO n) {
-  DArray<T> out(c, n);
-  if (n > 0)
-    OSH_HIP(hipMemcpyAsync(out.data(), src.data() + at, sizeof(T) * std::size_t(n),
-        hipMemcpyDeviceToDevice, c.stream));
-  return out;
+// One array of a split: `w` items per entity of dimension `d`, `size` bytes
+// each, go to the parts' own arrays (dst: one pointer per part).  With
+// shift >= 0 the items are indices of entities of dimension `shift`, rebased
+// to each part's first entity of that dimension.
+struct SplitJob {
+  void const* src;
+  void* const* dst;
+  LO nitems;
+  int w;
+  int d;
+  int size;
+  int shift;
+};
+
+// every array of a split in one launch: blockIdx.y picks the array, and each
+// item finds its part by a binary search over the parts' first entities
+// (starts[d * nparts + k], nondecreasing; an empty part starts where the next
+// one does, so the last part starting at or before an entity holds it)
+__global__ void split_copy(SplitJob const* jobs, LO const* starts, int nparts) {
+  SplitJob const j = jobs[blockIdx.y];
+  LO const* st = starts + std::size_t(j.d) * std::size_t(nparts);
+  for (std::int64_t i = std::int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < j.nitems;
+       i += std::int64_t(gridDim.x) * blockDim.x) {
+    LO e = LO(i / j.w);
+    int lo = 0, hi = nparts - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (st[mid] <= e)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    std::int64_t at = i - std::int64_t(st[lo]) * j.w;
+    if (j.size == 8) {
+      static_cast<Real*>(j.dst[lo])[at] = static_cast<Real const*>(j.src)[i];
+    } else if (j.size == 4) {
+      LO v = static_cast<LO const*>(j.src)[i];
+      if (j.shift >= 0) v -= starts[std::size_t(j.shift) * std::size_t(nparts) + lo];
+      static_cast<LO*>(j.dst[lo])[at] = v;
+    } else {
+      static_cast<I8*>(j.dst[lo])[at] = static_cast<I8 const*>(j.src)[i];
+    }
+  }
 }
 
 // first and last entity of each part, and its entity count, from the part
@@ -210,37 +246,85 @@ void split_mesh(Ctx& c, DMesh& m, int nparts, std::vector<DMesh>& out) {
   part_ranges(c, m, nparts, first, count);
   int dim = m.dim;
   out.assign(std::size_t(nparts), DMesh());
-  Reals const& len = m.length;
-  Reals const& qual = m.quality;
+  // the parts' arrays are allocated here and filled by one launch
+  std::vector<SplitJob> jobs;
+  std::vector<void*> dsts;
+  auto add = [&](void const* src, LO nitems, int w, int d, int size, int shift,
+                 auto&& dst_of) {
+    jobs.push_back({src, nullptr, nitems, w, d, size, shift});
+    for (int k = 0; k < nparts; ++k) dsts.push_back(dst_of(out[std::size_t(k)], k));
+  };
   for (int k = 0; k < nparts; ++k) {
     DMesh& r = out[std::size_t(k)];
     r.dim = dim;
     r.metric_ncomps = m.metric_ncomps;
     for (int d = 0; d <= dim; ++d) r.nents[d] = count[d][k];
-    for (int d = 1; d <= dim; ++d) {
-      LO n = r.nents[d] * (d + 1);
-      r.down[d] = LOs(c, n);
-      if (n > 0)
-        hipLaunchKernelGGL(copy_offset, dim3(nblocks(n)), dim3(kBlock), 0, c.stream, n,
-            m.down[d].data() + std::size_t(first[d][k]) * (d + 1), -first[d - 1][k],
-            r.down[d].data());
-      OSH_HIP(hipGetLastError());
-      if (d >= 2) r.down_codes[d] = slice(c, m.down_codes[d], first[d][k] * (d + 1), n);
-    }
-    r.coords = slice(c, m.coords, first[0][k] * dim, r.nents[0] * dim);
-    if (m.metric.exists())
-      r.metric = slice(c, m.metric, first[0][k] * m.metric_ncomps, r.nents[0] * m.metric_ncomps);
-    for (int d = 0; d <= dim; ++d) {
-      r.class_id[d] = slice(c, m.class_id[d], first[d][k], r.nents[d]);
-      r.class_dim[d] = slice(c, m.class_dim[d], first[d][k], r.nents[d]);
-    }
-    for (auto const& t : m.vtags) {
-      if (t.name == kPartTag) continue;
-      r.vtags.push_back({t.name, t.ncomps, t.kind,
-          slice(c, t.data, first[0][k] * t.ncomps, r.nents[0] * t.ncomps)});
-    }
-    if (len.exists()) r.length = slice(c, len, first[1][k], r.nents[1]);
-    if (qual.exists()) r.quality = slice(c, qual, first[dim][k], r.nents[dim]);
+  }
+  for (int d = 1; d <= dim; ++d) {
+    add(m.down[d].data(), m.nents[d] * (d + 1), d + 1, d, 4, d - 1, [&](DMesh& r, int) {
+      r.down[d] = LOs(c, r.nents[d] * (d + 1));
+      return static_cast<void*>(r.down[d].data());
+    });
+    if (d >= 2)
+      add(m.down_codes[d].data(), m.nents[d] * (d + 1), d + 1, d, 1, -1, [&](DMesh& r, int) {
+        r.down_codes[d] = Bytes(c, r.nents[d] * (d + 1));
+        return static_cast<void*>(r.down_codes[d].data());
+      });
+  }
+  add(m.coords.data(), m.nents[0] * dim, dim, 0, 8, -1, [&](DMesh& r, int) {
+    r.coords = Reals(c, r.nents[0] * dim);
+    return static_cast<void*>(r.coords.data());
+  });
+  if (m.metric.exists()) {
+    int nc = m.metric_ncomps;
+    add(m.metric.data(), m.nents[0] * nc, nc, 0, 8, -1, [&](DMesh& r, int) {
+      r.metric = Reals(c, r.nents[0] * nc);
+      return static_cast<void*>(r.metric.data());
+    });
+  }
+  for (int d = 0; d <= dim; ++d) {
+    add(m.class_id[d].data(), m.nents[d], 1, d, 4, -1, [&](DMesh& r, int) {
+      r.class_id[d] = LOs(c, r.nents[d]);
+      return static_cast<void*>(r.class_id[d].data());
+    });
+    add(m.class_dim[d].data(), m.nents[d], 1, d, 1, -1, [&](DMesh& r, int) {
+      r.class_dim[d] = Bytes(c, r.nents[d]);
+      return static_cast<void*>(r.class_dim[d].data());
+    });
+  }
+  for (auto const& t : m.vtags) {
+    if (t.name == kPartTag) continue;
+    int nc = t.ncomps;
+    add(t.data.data(), m.nents[0] * nc, nc, 0, 8, -1, [&](DMesh& r, int) {
+      r.vtags.push_back({t.name, t.ncomps, t.kind, Reals(c, r.nents[0] * nc)});
+      return static_cast<void*>(r.vtags.back().data.data());
+    });
+  }
+  if (m.length.exists())
+    add(m.length.data(), m.nents[1], 1, 1, 8, -1, [&](DMesh& r, int) {
+      r.length = Reals(c, r.nents[1]);
+      return static_cast<void*>(r.length.data());
+    });
+  if (m.quality.exists())
+    add(m.quality.data(), m.nents[dim], 1, dim, 8, -1, [&](DMesh& r, int) {
+      r.quality = Reals(c, r.nents[dim]);
+      return static_cast<void*>(r.quality.data());
+    });
+  std::vector<LO> starts(std::size_t(4) * std::size_t(nparts), 0);
+  for (int d = 0; d <= dim; ++d)
+    std::copy(first[d].begin(), first[d].end(), starts.begin() + std::ptrdiff_t(d) * nparts);
+  DArray<void*> ddsts = to_device(c, dsts);
+  for (std::size_t j = 0; j < jobs.size(); ++j)
+    jobs[j].dst = ddsts.data() + j * std::size_t(nparts);
+  DArray<SplitJob> djobs = to_device(c, jobs);
+  LOs dstarts = to_device(c, starts);
+  LO most = 0;
+  for (auto const& j : jobs) most = std::max(most, j.nitems);
+  if (most > 0) {
+    unsigned gx = std::min(nblocks(most), 4096u);
+    hipLaunchKernelGGL(split_copy, dim3(gx, unsigned(jobs.size())), dim3(kBlock), 0, c.stream,
+        djobs.data(), dstarts.data(), nparts);
+    OSH_HIP(hipGetLastError());
   }
   c.sync();
 }
